@@ -1,0 +1,127 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
+The shipped library never loads the oracle.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS: dict[str, ctypes.CDLL] = {}
+
+
+def build(quiet: bool = True) -> None:
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True,
+                   stdout=subprocess.DEVNULL if quiet else None)
+
+
+def _lib(variant: str = "") -> ctypes.CDLL:
+    name = "libovg_glibc.so" if variant == "glibc" else "libovg.so"
+    if name in _LIBS:
+        return _LIBS[name]
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        build()
+    lib = ctypes.CDLL(path)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    lib.ovg_render_stream.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_size_t,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    lib.ovg_render_stream.restype = ctypes.c_int
+    lib.ovg_unpremultiply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    lib.ovg_draw_key.argtypes = [ctypes.c_uint32]
+    lib.ovg_draw_key.restype = ctypes.c_uint32
+    lib.ovg_snap_vertex.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
+                                    ctypes.POINTER(ctypes.c_float), i32p, i32p]
+    lib.ovg_sample_in_triangle.argtypes = [i32p, i32p, ctypes.c_int32, ctypes.c_int32]
+    lib.ovg_sample_in_triangle.restype = ctypes.c_int
+    for fn in ("ovg_raster_triangle_scan", "ovg_raster_triangle_brute"):
+        getattr(lib, fn).argtypes = [i32p, i32p, ctypes.c_uint32, ctypes.c_uint32, i32p]
+    for fn in ("ovg_winding_scan", "ovg_winding_fan"):
+        getattr(lib, fn).argtypes = [i32p, i32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, i32p]
+    lib.ovg_flatten_cubic.argtypes = [ctypes.c_float] * 9 + [ctypes.POINTER(ctypes.c_float), ctypes.c_uint32]
+    lib.ovg_flatten_cubic.restype = ctypes.c_uint32
+    lib.ovg_blend_over.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    _LIBS[name] = lib
+    return lib
+
+
+def _p(a: np.ndarray, t=ctypes.c_uint32):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def render(width: int, height: int, words: np.ndarray, variant: str = "", with_sig: bool = True):
+    """Returns (premultiplied RGBA8 HxWx4, signature HxW uint32 or None, stats dict, status)."""
+    lib = _lib(variant)
+    words = np.ascontiguousarray(words, np.uint32)
+    rgba = np.zeros((height, width, 4), np.uint8)
+    sig = np.zeros((height, width), np.uint32) if with_sig else None
+    stats = np.zeros(3, np.uint64)
+    st = lib.ovg_render_stream(width, height, _p(words), words.size, rgba.ctypes.data,
+                               sig.ctypes.data if sig is not None else None, stats.ctypes.data)
+    return rgba, sig, {"draws": int(stats[0]), "triangles": int(stats[1]), "fragments": int(stats[2])}, st
+
+
+def unpremultiply(rgba: np.ndarray) -> np.ndarray:
+    lib = _lib()
+    src = np.ascontiguousarray(rgba, np.uint8)
+    out = np.empty_like(src)
+    lib.ovg_unpremultiply(src.ctypes.data, out.ctypes.data, src.size // 4)
+    return out
+
+
+def draw_key(n: int) -> int:
+    return int(_lib().ovg_draw_key(n))
+
+
+def snap(x: float, y: float, w: int, h: int, mat6=(1, 0, 0, 1, 0, 0)) -> tuple[int, int]:
+    lib = _lib()
+    m = (ctypes.c_float * 6)(*mat6)
+    fx, fy = ctypes.c_int32(), ctypes.c_int32()
+    lib.ovg_snap_vertex(x, y, w, h, m, ctypes.byref(fx), ctypes.byref(fy))
+    return fx.value, fy.value
+
+
+def raster_triangle(fx, fy, w, h, method: str = "scan") -> np.ndarray:
+    lib = _lib()
+    X = np.ascontiguousarray(fx, np.int32)
+    Y = np.ascontiguousarray(fy, np.int32)
+    cov = np.zeros((h, w), np.int32)
+    fn = lib.ovg_raster_triangle_scan if method == "scan" else lib.ovg_raster_triangle_brute
+    fn(_p(X, ctypes.c_int32), _p(Y, ctypes.c_int32), w, h, _p(cov, ctypes.c_int32))
+    return cov
+
+
+def sample_in_triangle(fx, fy, sx, sy) -> bool:
+    X = np.ascontiguousarray(fx, np.int32)
+    Y = np.ascontiguousarray(fy, np.int32)
+    return bool(_lib().ovg_sample_in_triangle(_p(X, ctypes.c_int32), _p(Y, ctypes.c_int32), sx, sy))
+
+
+def winding(px, py, w, h, method: str = "scan") -> np.ndarray:
+    lib = _lib()
+    X = np.ascontiguousarray(px, np.int32)
+    Y = np.ascontiguousarray(py, np.int32)
+    out = np.zeros((h, w), np.int32)
+    fn = lib.ovg_winding_scan if method == "scan" else lib.ovg_winding_fan
+    fn(_p(X, ctypes.c_int32), _p(Y, ctypes.c_int32), X.size, w, h, _p(out, ctypes.c_int32))
+    return out
+
+
+def flatten_cubic(p, tol: float = 0.25, cap: int = 1 << 16, variant: str = "") -> np.ndarray:
+    lib = _lib(variant)
+    out = np.zeros(2 * cap, np.float32)
+    n = lib.ovg_flatten_cubic(*[float(v) for v in p], tol, _p(out, ctypes.c_float), cap)
+    return out[:2 * min(n, cap)].reshape(-1, 2)
+
+
+def blend_over(d, s) -> np.ndarray:
+    dd = np.array(d, np.uint8)
+    ss = np.array(s, np.uint8)
+    _lib().ovg_blend_over(dd.ctypes.data, ss.ctypes.data)
+    return dd

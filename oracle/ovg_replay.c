@@ -1,0 +1,207 @@
+/*
+ * CPU oracle — scene command stream interpreter (test infrastructure only).
+ * Maps each opcode onto the oracle's restatement of the corresponding vkvg entry point
+ * (src/vkvg_context.c) and src/vkvg_pattern.c for gradients.
+ */
+#include "ovg_internal.h"
+
+static inline float F(const uint32_t *w) {
+    float f;
+    memcpy(&f, w, 4);
+    return f;
+}
+
+/* vkvg_pattern_edit_radial, src/vkvg_pattern.c:95-119 */
+static void edit_radial(ovg_grad *g, float cx0, float cy0, float radius0, float cx1, float cy1, float radius1) {
+    float c0x = cx0, c0y = cy0, c1x = cx1, c1y = cy1;
+    if (radius0 > radius1 - 1.0f)
+        radius0 = radius1 - 1.0f;
+    float ux = c0x - c1x, uy = c0y - c1y;
+    float l = sqrtf(ux * ux + uy * uy);
+    if (l + radius0 + 1.0f >= radius1) {
+        float vx = ux / l, vy = uy / l;
+        float k = radius1 - radius0 - 1.0f;
+        c0x = c1x + vx * k;
+        c0y = c1y + vy * k;
+    }
+    g->cp[0][0] = c0x; g->cp[0][1] = c0y; g->cp[0][2] = radius0; g->cp[0][3] = 0;
+    g->cp[1][0] = c1x; g->cp[1][1] = c1y; g->cp[1][2] = radius1; g->cp[1][3] = 0;
+}
+/* _update_cur_pattern for LINEAR/RADIAL, src/vkvg_context_internal.c:774-826 (no pattern matrix) */
+static void set_gradient(octx *c, const ovg_grad *src) {
+    ovg_grad g = *src;
+    if (g.count < 2) {
+        c->status = 10; /* VKVG_STATUS_PATTERN_INVALID_GRADIENT */
+        return;
+    }
+    m_transform_point(&c->mat, &g.cp[0][0], &g.cp[0][1]);
+    if (g.type == SRC_LINEAR) {
+        m_transform_point(&c->mat, &g.cp[0][2], &g.cp[0][3]);
+    } else {
+        m_transform_point(&c->mat, &g.cp[1][0], &g.cp[1][1]);
+        m_transform_distance(&c->mat, &g.cp[0][2], &g.cp[0][3]);
+        m_transform_distance(&c->mat, &g.cp[1][2], &g.cp[0][3]); /* as written in the reference */
+    }
+    c->grad = g;
+    c->srcType = g.type;
+}
+
+int ovg_render_stream(uint32_t width, uint32_t height, const uint32_t *w, size_t nw, uint8_t *out_rgba,
+                      uint32_t *out_sig, uint64_t *out_stats) {
+    ovg_surface S;
+    memset(&S, 0, sizeof(S));
+    S.width = width;
+    S.height = height;
+    size_t np = (size_t)width * height;
+    S.rgba = (uint8_t *)calloc(np * 4, 1);
+    S.stencil = (uint8_t *)calloc(np, 1);
+    S.sig = (uint32_t *)calloc(np, sizeof(uint32_t));
+    S.acc = (int32_t *)calloc(np, sizeof(int32_t));
+    octx c;
+    octx_init(&c, &S);
+
+    size_t i = 0;
+    while (i < nw && !c.status) {
+        uint32_t op = w[i++];
+        const uint32_t *a = w + i;
+        switch (op) {
+        case OVG_OP_END: i = nw; break;
+        case OVG_OP_MOVE_TO: o_move_to(&c, F(a), F(a + 1)); i += 2; break;
+        case OVG_OP_LINE_TO: o_line_to(&c, F(a), F(a + 1)); i += 2; break;
+        case OVG_OP_CURVE_TO: o_curve_to(&c, F(a), F(a + 1), F(a + 2), F(a + 3), F(a + 4), F(a + 5)); i += 6; break;
+        case OVG_OP_REL_CURVE_TO:
+            o_rel_curve_to(&c, F(a), F(a + 1), F(a + 2), F(a + 3), F(a + 4), F(a + 5));
+            i += 6;
+            break;
+        case OVG_OP_CLOSE_PATH: o_close_path(&c); break;
+        case OVG_OP_ARC: o_arc(&c, F(a), F(a + 1), F(a + 2), F(a + 3), F(a + 4)); i += 5; break;
+        case OVG_OP_ARC_NEG: o_arc_negative(&c, F(a), F(a + 1), F(a + 2), F(a + 3), F(a + 4)); i += 5; break;
+        case OVG_OP_RECTANGLE: o_rectangle(&c, F(a), F(a + 1), F(a + 2), F(a + 3)); i += 4; break;
+        case OVG_OP_ROUNDED_RECT: o_rounded_rectangle(&c, F(a), F(a + 1), F(a + 2), F(a + 3), F(a + 4)); i += 5; break;
+        case OVG_OP_ELLIPSE: o_ellipse(&c, F(a), F(a + 1), F(a + 2), F(a + 3), F(a + 4)); i += 5; break;
+        case OVG_OP_ELLIPTIC_ARC_TO:
+            o_elliptic_arc_to(&c, F(a), F(a + 1), a[2] != 0, a[3] != 0, F(a + 4), F(a + 5), F(a + 6));
+            i += 7;
+            break;
+        case OVG_OP_NEW_PATH: o_new_path(&c); break;
+        case OVG_OP_NEW_SUB_PATH: o_new_sub_path(&c); break;
+        case OVG_OP_REL_LINE_TO: o_rel_line_to(&c, F(a), F(a + 1)); i += 2; break;
+        case OVG_OP_REL_MOVE_TO: o_rel_move_to(&c, F(a), F(a + 1)); i += 2; break;
+        case OVG_OP_QUADRATIC_TO: o_quadratic_to(&c, F(a), F(a + 1), F(a + 2), F(a + 3)); i += 4; break;
+        case OVG_OP_FILL: r_fill(&c); o_clear_path(&c); break;
+        case OVG_OP_FILL_PRESERVE: r_fill(&c); break;
+        case OVG_OP_STROKE: r_stroke(&c); o_clear_path(&c); break;
+        case OVG_OP_STROKE_PRESERVE: r_stroke(&c); break;
+        case OVG_OP_PAINT: r_paint(&c); break;
+        case OVG_OP_CLEAR: r_clear(&c); break;
+        case OVG_OP_SET_SOURCE_RGBA:
+            c.curColor = o_rgbaf(F(a), F(a + 1), F(a + 2), F(a + 3));
+            c.srcType = SRC_SOLID;
+            i += 4;
+            break;
+        case OVG_OP_SET_SOURCE_RGB:
+            c.curColor = o_rgbaf(F(a), F(a + 1), F(a + 2), 1);
+            c.srcType = SRC_SOLID;
+            i += 3;
+            break;
+        case OVG_OP_SET_SOURCE_COLOR: c.curColor = a[0]; c.srcType = SRC_SOLID; i += 1; break;
+        case OVG_OP_SET_LINE_WIDTH: c.lineWidth = F(a); i += 1; break;
+        case OVG_OP_SET_LINE_JOIN: c.lineJoin = (int)a[0]; i += 1; break;
+        case OVG_OP_SET_LINE_CAP: c.lineCap = (int)a[0]; i += 1; break;
+        case OVG_OP_SET_MITER_LIMIT: c.miterLimit = F(a); i += 1; break;
+        case OVG_OP_SET_FILL_RULE: c.fillRule = (int)a[0]; i += 1; break;
+        case OVG_OP_SET_DASH: {
+            uint32_t n = a[0];
+            float off = F(a + 1);
+            float d[64];
+            uint32_t nn = n > 64 ? 64 : n;
+            for (uint32_t k = 0; k < nn; k++)
+                d[k] = F(a + 2 + k);
+            o_set_dash(&c, d, nn, off);
+            i += 2 + n;
+            break;
+        }
+        case OVG_OP_SET_OPACITY: c.opacity = F(a); i += 1; break;
+        case OVG_OP_SET_OPERATOR: c.op = (int)a[0]; i += 1; break;
+        case OVG_OP_LINEAR:
+        case OVG_OP_RADIAL: {
+            ovg_grad g;
+            memset(&g, 0, sizeof(g));
+            uint32_t ncp = (op == OVG_OP_LINEAR) ? 4 : 6;
+            if (op == OVG_OP_LINEAR) {
+                g.type = SRC_LINEAR;
+                g.cp[0][0] = F(a); g.cp[0][1] = F(a + 1); g.cp[0][2] = F(a + 2); g.cp[0][3] = F(a + 3);
+            } else {
+                g.type = SRC_RADIAL;
+                edit_radial(&g, F(a), F(a + 1), F(a + 2), F(a + 3), F(a + 4), F(a + 5));
+            }
+            uint32_t ns = a[ncp];
+            const uint32_t *st = a + ncp + 1;
+            for (uint32_t k = 0; k < ns; k++) {
+                if (g.count >= 16)
+                    break; /* the reference writes past its 16-entry arrays here */
+                float off = F(st + 5 * k), r = F(st + 5 * k + 1), gg = F(st + 5 * k + 2), b = F(st + 5 * k + 3),
+                      al = F(st + 5 * k + 4);
+                g.colors[g.count][0] = al * r;
+                g.colors[g.count][1] = al * gg;
+                g.colors[g.count][2] = al * b;
+                g.colors[g.count][3] = al;
+                g.stops[g.count] = off;
+                g.count++;
+            }
+            set_gradient(&c, &g);
+            i += ncp + 1 + 5 * ns;
+            break;
+        }
+        case OVG_OP_TRANSLATE: {
+            mat6 t = {1, 0, 0, 1, F(a), F(a + 1)};
+            m_multiply(&c.mat, &t, &c.mat);
+            i += 2;
+            break;
+        }
+        case OVG_OP_SCALE: {
+            mat6 t = {F(a), 0, 0, F(a + 1), 0, 0};
+            m_multiply(&c.mat, &t, &c.mat);
+            i += 2;
+            break;
+        }
+        case OVG_OP_ROTATE: {
+            float s = L_SINF(F(a)), co = L_COSF(F(a));
+            mat6 t = {co, s, -s, co, 0, 0};
+            m_multiply(&c.mat, &t, &c.mat);
+            i += 1;
+            break;
+        }
+        case OVG_OP_IDENTITY_MATRIX: {
+            mat6 t = {1, 0, 0, 1, 0, 0};
+            c.mat = t;
+            break;
+        }
+        case OVG_OP_SET_MATRIX: {
+            mat6 t = {F(a), F(a + 1), F(a + 2), F(a + 3), F(a + 4), F(a + 5)};
+            c.mat = t;
+            i += 6;
+            break;
+        }
+        default:
+            c.status = 6; /* VKVG_STATUS_INVALID_STATUS: unknown opcode */
+            break;
+        }
+    }
+    int status = c.status;
+    if (out_rgba)
+        memcpy(out_rgba, S.rgba, np * 4);
+    if (out_sig)
+        memcpy(out_sig, S.sig, np * sizeof(uint32_t));
+    if (out_stats) {
+        out_stats[0] = S.draw_counter;
+        out_stats[1] = S.tri_count;
+        out_stats[2] = S.frag_count;
+    }
+    octx_free(&c);
+    free(S.rgba);
+    free(S.stencil);
+    free(S.sig);
+    free(S.acc);
+    return status;
+}

@@ -1,0 +1,225 @@
+"""Seeded synthetic scenes for the five benchmark configurations.
+
+Random numbers come from the PCG64 bit generator (O'Neill 2014) through its raw 64-bit
+output stream, which is stable across numpy versions; every conversion to floats is done
+here. Scene k uses seed 0x5EED0000 + 16*k + variant.
+
+Frozen choices not fixed by the configuration text are marked (*) in the docstrings.
+Every scene starts with set_source_rgb(1,1,1); paint() onto a transparent surface.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .stream import (CAP_BUTT, CAP_ROUND, CAP_SQUARE, FILL_EVEN_ODD, FILL_NON_ZERO, JOIN_BEVEL, JOIN_MITER,
+                     JOIN_ROUND, Stream)
+
+
+class Rng:
+    """Uniform / log-uniform / normal draws from the PCG64 raw stream."""
+
+    def __init__(self, seed: int) -> None:
+        self._bg = np.random.PCG64(seed)
+
+    def u(self, n: int | None = None) -> np.ndarray | float:
+        k = 1 if n is None else n
+        raw = self._bg.random_raw(k).astype(np.uint64)
+        v = (raw >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)  # [0,1) with 53 bits
+        return float(v[0]) if n is None else v
+
+    def uniform(self, lo, hi, n=None):
+        return lo + (hi - lo) * self.u(n)
+
+    def loguniform(self, lo, hi, n=None):
+        return np.exp(math.log(lo) + (math.log(hi) - math.log(lo)) * self.u(n)) if n is not None else \
+            math.exp(math.log(lo) + (math.log(hi) - math.log(lo)) * self.u())
+
+    def randint(self, lo, hi, n=None):
+        """integer uniform in [lo, hi] inclusive"""
+        v = np.floor(lo + (hi - lo + 1) * self.u(n)).astype(np.int64) if n is not None else \
+            int(math.floor(lo + (hi - lo + 1) * self.u()))
+        return np.minimum(v, hi) if n is not None else min(v, hi)
+
+    def normal(self, n: int) -> np.ndarray:
+        u1 = self.u(n)
+        u2 = self.u(n)
+        return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * math.pi * u2)
+
+
+@dataclass
+class Scene:
+    name: str
+    width: int
+    height: int
+    stream: Stream
+    n_paths: int
+    ctrl_points: int = 0
+    gradients: int = 0
+    gradient_stops: int = 0
+    meta: dict = field(default_factory=dict)
+
+    def words(self) -> np.ndarray:
+        return self.stream.words()
+
+    def algorithmic_bytes(self, fp32_out: bool = False) -> int:
+        """8*(control points) + 32*paths + (24+20*stops)*gradients + 4*W*H (16*W*H for FP32 out)."""
+        px = self.width * self.height
+        return (8 * self.ctrl_points + 32 * self.n_paths + 24 * self.gradients + 20 * self.gradient_stops
+                + (16 if fp32_out else 4) * px)
+
+
+def _white(s: Stream) -> None:
+    s.set_source_rgb(1.0, 1.0, 1.0)
+    s.paint()
+
+
+def config1(variant: int = 0, n_paths: int = 200, size: int = 512) -> Scene:
+    """512x512: closed paths of n~U{4..12} cubics, control points ~U[0,size)^2, non-zero fill,
+    rgb~U[0,1], alpha~U[0.5,1]."""
+    rng = Rng(0x5EED0000 + 16 * 1 + variant)
+    s = Stream()
+    _white(s)
+    cps = 0
+    for _ in range(n_paths):
+        n = rng.randint(4, 12)
+        pts = rng.uniform(0, size, 2 + 6 * n).astype(np.float32)
+        col = rng.u(3)
+        a = rng.uniform(0.5, 1.0)
+        s.set_source_rgba(col[0], col[1], col[2], a)
+        s.curves(pts[:2], pts[2:].reshape(n, 6), close=True)
+        s.fill()
+        cps += 1 + 3 * n
+    return Scene("c1_512_nz_cubics", size, size, s, n_paths, ctrl_points=cps)
+
+
+def _stroke_style(rng: Rng, s: Stream) -> None:
+    s.set_line_width(float(rng.loguniform(0.5, 20.0)))
+    s.set_line_join([JOIN_MITER, JOIN_ROUND, JOIN_BEVEL][rng.randint(0, 2)])
+    s.set_line_cap([CAP_BUTT, CAP_ROUND, CAP_SQUARE][rng.randint(0, 2)])
+    if rng.u() < 0.25:
+        s.set_dash([8.0, 4.0], 0.0)
+    else:
+        s.set_dash([], 0.0)
+
+
+def _c2_content(rng: Rng, s: Stream, W: int, H: int, n_lines: int, n_stars: int) -> tuple[int, int]:
+    """(a) polylines of m~U{2..64} points ~U over the surface, width log-U[0.5,20], join/cap
+    uniform, miter limit 10, dash [8,4] on 25%, alpha~U[0.5,1]; (b) even-odd stars with
+    n~U{5..31}, r log-U[10,200] (*), centre ~U, vertex k at angle 2*pi*k*s/n, s=floor((n-1)/2) (*)."""
+    cps = 0
+    s.set_miter_limit(10.0)
+    s.set_fill_rule(FILL_NON_ZERO)
+    for _ in range(n_lines):
+        m = rng.randint(2, 64)
+        xy = np.stack([rng.uniform(0, W, m), rng.uniform(0, H, m)], 1).astype(np.float32)
+        col = rng.u(3)
+        s.set_source_rgba(col[0], col[1], col[2], rng.uniform(0.5, 1.0))
+        _stroke_style(rng, s)
+        s.polyline(xy)
+        s.stroke()
+        cps += m
+    s.set_dash([], 0.0)
+    s.set_fill_rule(FILL_EVEN_ODD)
+    for _ in range(n_stars):
+        n = rng.randint(5, 31)
+        r = rng.loguniform(10.0, 200.0)
+        cx, cy = rng.uniform(0, W), rng.uniform(0, H)
+        step = (n - 1) // 2
+        k = np.arange(n)
+        ang = 2.0 * math.pi * k * step / n
+        xy = np.stack([cx + r * np.cos(ang), cy + r * np.sin(ang)], 1).astype(np.float32)
+        col = rng.u(3)
+        s.set_source_rgba(col[0], col[1], col[2], rng.uniform(0.5, 1.0))
+        s.polyline(xy, close=True)
+        s.fill()
+        cps += n
+    s.set_fill_rule(FILL_NON_ZERO)
+    return cps, n_lines + n_stars
+
+
+def config2(variant: int = 0, n_lines: int = 5000, n_stars: int = 2000, width: int = 1920,
+            height: int = 1080) -> Scene:
+    rng = Rng(0x5EED0000 + 16 * 2 + variant)
+    s = Stream()
+    _white(s)
+    cps, n = _c2_content(rng, s, width, height, n_lines, n_stars)
+    return Scene("c2_1080p_stroke_eo", width, height, s, n, ctrl_points=cps)
+
+
+def config3(variant: int = 0, n_paths: int = 20000, size: int = 4096, blobs: int = 16,
+            sigma: float = 300.0, scale_lo: float = 8.0, scale_hi: float = 256.0) -> Scene:
+    """4096x4096 dense scene: paths of 1..40 cubics around centres from a 16-blob Gaussian mixture
+    (means ~U, sigma 300, clamped to the surface (*)); control points = centre + N(0, s^2),
+    s log-U[8,256] (*). 60% non-zero fill (closed), 40% open stroke with width U[0.5,8] and
+    join/cap uniform (*). 30% of sources are gradients, half linear / half radial (*), with
+    2..4 stops, offsets sorted U with ends at 0 and 1 (*). alpha~U[0.2,1]."""
+    rng = Rng(0x5EED0000 + 16 * 3 + variant)
+    k = size / 4096.0
+    means = rng.uniform(0, size, 2 * blobs).reshape(blobs, 2)
+    s = Stream()
+    _white(s)
+    s.set_miter_limit(10.0)
+    cps = grads = stops_total = 0
+    for _ in range(n_paths):
+        b = rng.randint(0, blobs - 1)
+        c = means[b] + sigma * k * rng.normal(2)
+        c = np.clip(c, 0, size - 1)
+        sc = rng.loguniform(scale_lo * k, scale_hi * k)
+        n = rng.randint(1, 40)
+        pts = (np.repeat(c[None, :], 1 + 3 * n, 0).reshape(-1) + sc * rng.normal(2 + 6 * n)).astype(np.float32)
+        alpha = rng.uniform(0.2, 1.0)
+        if rng.u() < 0.3:
+            ns = rng.randint(2, 4)
+            offs = np.sort(rng.u(ns))
+            offs[0], offs[-1] = 0.0, 1.0
+            st = np.zeros((ns, 5), np.float32)
+            st[:, 0] = offs
+            st[:, 1:4] = rng.u(3 * ns).reshape(ns, 3)
+            st[:, 4] = alpha
+            if rng.u() < 0.5:
+                a0 = c + sc * rng.normal(2)
+                a1 = c + sc * rng.normal(2)
+                s.set_linear(a0[0], a0[1], a1[0], a1[1], st)
+            else:
+                r1 = sc * rng.uniform(1.0, 3.0)
+                r0 = r1 * rng.uniform(0.0, 0.5)
+                f = c + 0.25 * sc * rng.normal(2)
+                s.set_radial(f[0], f[1], r0, c[0], c[1], r1, st)
+            grads += 1
+            stops_total += ns
+        else:
+            col = rng.u(3)
+            s.set_source_rgba(col[0], col[1], col[2], alpha)
+        if rng.u() < 0.6:
+            s.curves(pts[:2], pts[2:].reshape(n, 6), close=True)
+            s.fill()
+        else:
+            s.set_line_width(float(rng.uniform(0.5, 8.0)))
+            s.set_line_join([JOIN_MITER, JOIN_ROUND, JOIN_BEVEL][rng.randint(0, 2)])
+            s.set_line_cap([CAP_BUTT, CAP_ROUND, CAP_SQUARE][rng.randint(0, 2)])
+            s.curves(pts[:2], pts[2:].reshape(n, 6), close=False)
+            s.stroke()
+        cps += 1 + 3 * n
+    return Scene("c3_4k_dense", size, size, s, n_paths, ctrl_points=cps, gradients=grads,
+                 gradient_stops=stops_total)
+
+
+def config4(variant: int = 0, n_surfaces: int = 256, side_lo: int = 64, side_hi: int = 2048,
+            paths_lo: int = 10, paths_hi: int = 5000) -> list[Scene]:
+    """256 surfaces with side log-U[64,2048] (square (*)), each holding log-U{10..5000} paths
+    drawn as in config 2, polylines:stars = 5:2 (*)."""
+    rng = Rng(0x5EED0000 + 16 * 4 + variant)
+    out = []
+    for i in range(n_surfaces):
+        side = int(round(rng.loguniform(side_lo, side_hi)))
+        npaths = int(round(rng.loguniform(paths_lo, paths_hi)))
+        n_lines = max(1, (npaths * 5) // 7)
+        n_stars = max(0, npaths - n_lines)
+        s = Stream()
+        _white(s)
+        cps, n = _c2_content(rng, s, side, side, n_lines, n_stars)
+        out.append(Scene(f"c4_{i}", side, side, s, n, ctrl_points=cps))
+    return out
