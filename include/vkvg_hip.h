@@ -1,0 +1,146 @@
+/*
+ * vkvg_hip.h — the thin C ABI between the C host recorder (libvkvg's vkvg_* entry points)
+ * and the gfx950 HIP rendering backend.
+ *
+ * The host recorder turns vkvg path/paint calls into a *frame*: flat arrays of path items
+ * (explicit points and unflattened cubics), subpaths, draws, stroke styles, dash arrays,
+ * gradients and render targets. vkvg_hip_render() uploads a frame and runs the kernel
+ * pipeline  flatten -> stroke-expand -> edges -> tile binning -> coarse -> fine
+ * (integer winding + source-over compositing) into the targets' RGBA8 images in HBM.
+ *
+ * In the reference the same work is split between host geometry
+ * (src/vkvg_context_internal.c:924-1655) and Vulkan draw submission
+ * (src/vkvg_context_internal.c:421-603, vkQueueSubmit + fence wait); this header replaces
+ * that Vulkan submission boundary. Everything here is plain C: no torch or HIP types.
+ */
+#ifndef VKVG_HIP_H
+#define VKVG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef vkvg_public
+#define vkvg_public __attribute__((visibility("default")))
+#endif
+
+/* ---- path items: one explicit point, or one cubic to be flattened on the GPU */
+#define VKH_ITEM_CUBIC  1u /* p[0..7] = current point, c1, c2, end; flattened with AGG */
+#define VKH_ITEM_CURVED 2u /* the item's points lie in a curve segment (stroke joins forced) */
+#define VKH_ITEM_NO_END 4u /* cubic whose end point was removed by close_path */
+
+typedef struct {
+    float    p[8];  /* point: p[0], p[1]; cubic: x0 y0 x1 y1 x2 y2 x3 y3 */
+    uint32_t flags; /* VKH_ITEM_* */
+    float    tol;   /* cubic: squared distance tolerance 0.25/max(sx,sy) (src/vkvg_context.c:556-558) */
+} vkh_item;
+
+#define VKH_SP_CLOSED 1u
+
+typedef struct {
+    uint32_t first_item, n_items;
+    uint32_t flags; /* VKH_SP_CLOSED */
+    uint32_t reserved;
+} vkh_subpath;
+
+/* draw kinds: how the per-pixel integer winding w of the draw's edges becomes the number k
+ * of source-over blends at that pixel */
+enum {
+    VKH_DRAW_FILL_NZ = 0, /* k = (w != 0)                       src/vkvg_context_internal.c:1720-1793 */
+    VKH_DRAW_FILL_EO = 1, /* k = w & 1 (fan INVERT + cover)     src/vkvg_context_internal.c:1583-1655 */
+    VKH_DRAW_STROKE = 2,  /* k = w = covering triangles         src/vkvg_context.c:822-948 */
+    VKH_DRAW_PAINT = 3,   /* k = 1 on every pixel               src/vkvg_context.c:990-1003 */
+    VKH_DRAW_CLEAR = 4,   /* colour := 0 on every pixel         src/vkvg_context.c:734-753 */
+};
+
+typedef struct {
+    uint32_t kind;           /* VKH_DRAW_* */
+    uint32_t op;             /* vkvg_operator_t */
+    uint32_t color;          /* premultiplied RGBA8, byte 0 = R (CreateRgbaf) */
+    int32_t  grad;           /* gradient index, -1 for the solid colour */
+    float    opacity;
+    float    mat[6];         /* CTM (xx yx xy yy x0 y0) applied as in shaders/vkvg_main.vert:74-79 */
+    uint32_t first_subpath;
+    uint32_t n_subpaths;
+    int32_t  style;          /* stroke style index (STROKE only) */
+    uint32_t target;         /* render target index */
+    uint32_t key_index;      /* draw number on its target (coverage-signature key) */
+} vkh_draw;
+
+typedef struct {
+    float    hw;          /* line width / 2 */
+    float    lhMax;       /* miter limit * line width */
+    float    arcStep;     /* _get_arc_step(hw) under the draw's CTM */
+    uint32_t join, cap;   /* vkvg_line_join_t, vkvg_line_cap_t */
+    uint32_t dash_first;  /* index into the dash array */
+    uint32_t dash_count;  /* 0: solid stroke */
+    float    dash_offset;
+} vkh_stroke_style;
+
+typedef struct {
+    uint32_t type;        /* 2 linear, 3 radial (vkvg_pattern_type_t) */
+    uint32_t count;       /* colour stops */
+    float    colors[16][4]; /* premultiplied at add time (src/vkvg_pattern.c:149-168) */
+    float    stops[16];
+    float    cp[2][4];    /* control points, transformed by the CTM at set_source time */
+} vkh_gradient;
+
+typedef struct {
+    void     *rgba;       /* device pointer, width*height*4 bytes, premultiplied RGBA8 */
+    uint32_t  width, height;
+    uint32_t *sig;        /* optional device pointer to width*height uint32 coverage signatures */
+} vkh_target;
+
+typedef struct {
+    const vkh_item         *items;    uint32_t n_items;
+    const vkh_subpath      *subpaths; uint32_t n_subpaths;
+    const vkh_draw         *draws;    uint32_t n_draws;
+    const vkh_stroke_style *styles;   uint32_t n_styles;
+    const float            *dashes;   uint32_t n_dashes;
+    const vkh_gradient     *grads;    uint32_t n_grads;
+    const vkh_target       *targets;  uint32_t n_targets;
+} vkh_frame;
+
+/* per-frame counters (sizes of the intermediate buffers the last render used) */
+typedef struct {
+    uint64_t points, stroke_vertices, stroke_triangles, edges;
+    uint64_t path_tiles, segments, commands, tiles;
+    uint32_t tile_size;
+    float    ms_total;      /* host wall time of the last vkvg_hip_render, ms */
+    float    ms_fine;       /* device time of the fine (coverage + composite) kernel, ms */
+} vkh_stats;
+
+typedef struct vkh_engine vkh_engine;
+
+/* Engine: device buffers, stream and events for one HIP device. Returns NULL on failure. */
+vkvg_public vkh_engine *vkvg_hip_engine_create(int device_ordinal);
+vkvg_public void        vkvg_hip_engine_destroy(vkh_engine *eng);
+/* tile size in pixels (8, 16 or 32); 0 restores the default. Returns 0 on success. */
+vkvg_public int         vkvg_hip_set_tile_size(vkh_engine *eng, uint32_t tile);
+/* Optional external stream (a hipStream_t passed as void*); NULL = the engine's own. */
+vkvg_public void        vkvg_hip_set_stream(vkh_engine *eng, void *stream);
+
+/* device images */
+vkvg_public int  vkvg_hip_image_alloc(vkh_engine *eng, uint32_t width, uint32_t height, void **dev_ptr);
+vkvg_public void vkvg_hip_image_free(vkh_engine *eng, void *dev_ptr);
+vkvg_public int  vkvg_hip_image_clear(vkh_engine *eng, void *dev_ptr, size_t bytes);
+vkvg_public int  vkvg_hip_image_download(vkh_engine *eng, const void *dev_ptr, void *host, size_t bytes);
+vkvg_public int  vkvg_hip_image_upload(vkh_engine *eng, void *dev_ptr, const void *host, size_t bytes);
+/* fused BGRA/RGBA swizzle-free readback with vkvg's truncating unpremultiply
+ * (src/vkvg_surface.c:441-452) on the device, then copy to host */
+vkvg_public int  vkvg_hip_image_read_unpremultiplied(vkh_engine *eng, const void *dev_ptr, uint32_t width,
+                                                     uint32_t height, void *host);
+
+/* Render one frame (any number of draws over any number of targets) and wait for it.
+ * Returns 0 on success, a negative value on a HIP error. */
+vkvg_public int  vkvg_hip_render(vkh_engine *eng, const vkh_frame *frame);
+vkvg_public void vkvg_hip_get_stats(vkh_engine *eng, vkh_stats *out);
+vkvg_public const char *vkvg_hip_last_error(vkh_engine *eng);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -962,9 +962,14 @@ void o_stroke_geometry(octx *c) {
             dc.dashOn = true;
             dc.curDash = 0;
             dc.totDashLength = 0;
-            for (uint32_t i = 0; i < c->dashCount; i++)
+            bool neg = false;
+            for (uint32_t i = 0; i < c->dashCount; i++) {
                 dc.totDashLength += c->dashes[i];
-            if (dc.totDashLength == 0) {
+                neg |= !(c->dashes[i] >= 0.f);
+            }
+            /* the reference rejects a zero total; negative entries and totals below 1/1024 px,
+             * on which its dash walk would run (almost) forever, are rejected the same way */
+            if (dc.totDashLength == 0 || neg || !(dc.totDashLength >= 1.0f / 1024.0f)) {
                 c->status = 13; /* VKVG_STATUS_INVALID_DASH */
                 return;
             }

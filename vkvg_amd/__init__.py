@@ -1,0 +1,231 @@
+"""vkvg_amd — MI355X-native backend of the vkvg 2-D vector graphics C API.
+
+The product is the shared library ``libvkvg.so`` (C API of include/vkvg.h, extensions of
+include/vkvg_amd.h, HIP backend of include/vkvg_hip.h). This module is a thin ctypes layer
+used by the tests, the benchmark and the differentiable-rendering harness. There is no
+fallback: without the built library every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvkvg.so")
+_lib = None
+
+
+class VkvgError(RuntimeError):
+    pass
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("points", ctypes.c_uint64), ("stroke_vertices", ctypes.c_uint64),
+                ("stroke_triangles", ctypes.c_uint64), ("edges", ctypes.c_uint64),
+                ("path_tiles", ctypes.c_uint64), ("segments", ctypes.c_uint64),
+                ("commands", ctypes.c_uint64), ("tiles", ctypes.c_uint64), ("tile_size", ctypes.c_uint32),
+                ("ms_total", ctypes.c_float), ("ms_fine", ctypes.c_float)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+EXPORTED = [
+    # include/vkvg.h
+    "vkvg_matrix_init_identity", "vkvg_matrix_init", "vkvg_matrix_init_translate", "vkvg_matrix_init_scale",
+    "vkvg_matrix_init_rotate", "vkvg_matrix_translate", "vkvg_matrix_scale", "vkvg_matrix_rotate",
+    "vkvg_matrix_multiply", "vkvg_matrix_transform_distance", "vkvg_matrix_transform_point", "vkvg_matrix_invert",
+    "vkvg_matrix_get_scale", "vkvg_device_create", "vkvg_device_destroy", "vkvg_device_status",
+    "vkvg_device_reference", "vkvg_device_get_reference_count", "vkvg_device_set_dpy", "vkvg_device_get_dpy",
+    "vkvg_surface_create", "vkvg_surface_create_from_bitmap", "vkvg_surface_status", "vkvg_surface_reference",
+    "vkvg_surface_get_reference_count", "vkvg_surface_destroy", "vkvg_surface_clear", "vkvg_surface_get_vk_image",
+    "vkvg_surface_get_vk_format", "vkvg_surface_get_width", "vkvg_surface_get_height", "vkvg_surface_write_to_png",
+    "vkvg_surface_write_to_memory", "vkvg_surface_resolve", "vkvg_create", "vkvg_destroy", "vkvg_status",
+    "vkvg_status_to_string", "vkvg_reference", "vkvg_get_reference_count", "vkvg_flush", "vkvg_new_path",
+    "vkvg_close_path", "vkvg_new_sub_path", "vkvg_get_current_point", "vkvg_line_to", "vkvg_rel_line_to",
+    "vkvg_move_to", "vkvg_rel_move_to", "vkvg_arc", "vkvg_arc_negative", "vkvg_curve_to", "vkvg_rel_curve_to",
+    "vkvg_quadratic_to", "vkvg_rel_quadratic_to", "vkvg_rectangle", "vkvg_rounded_rectangle", "vkvg_ellipse",
+    "vkvg_elliptic_arc_to", "vkvg_rel_elliptic_arc_to", "vkvg_has_current_point", "vkvg_stroke",
+    "vkvg_stroke_preserve", "vkvg_fill", "vkvg_fill_preserve", "vkvg_paint", "vkvg_clear", "vkvg_set_opacity",
+    "vkvg_get_opacity", "vkvg_set_source_color", "vkvg_set_source_rgba", "vkvg_set_source_rgb", "vkvg_set_source",
+    "vkvg_set_line_width", "vkvg_set_miter_limit", "vkvg_get_miter_limit", "vkvg_set_line_cap",
+    "vkvg_set_line_join", "vkvg_set_operator", "vkvg_set_fill_rule", "vkvg_set_dash", "vkvg_get_dash",
+    "vkvg_get_line_width", "vkvg_get_line_cap", "vkvg_get_line_join", "vkvg_get_operator", "vkvg_get_fill_rule",
+    "vkvg_get_source", "vkvg_get_target", "vkvg_translate", "vkvg_scale", "vkvg_rotate", "vkvg_transform",
+    "vkvg_set_matrix", "vkvg_get_matrix", "vkvg_identity_matrix", "vkvg_pattern_status", "vkvg_pattern_reference",
+    "vkvg_pattern_get_reference_count", "vkvg_pattern_create_linear", "vkvg_pattern_edit_linear",
+    "vkvg_pattern_get_linear_points", "vkvg_pattern_create_radial", "vkvg_pattern_edit_radial",
+    "vkvg_pattern_get_color_stop_count", "vkvg_pattern_get_color_stop_rgba", "vkvg_pattern_destroy",
+    "vkvg_pattern_add_color_stop", "vkvg_pattern_set_extend", "vkvg_pattern_set_filter", "vkvg_pattern_get_extend",
+    "vkvg_pattern_get_filter", "vkvg_pattern_get_type", "vkvg_pattern_set_matrix", "vkvg_pattern_get_matrix",
+    # include/vkvg_amd.h
+    "vkvg_amd_replay", "vkvg_amd_surface_enable_signature", "vkvg_amd_surface_read_signature",
+    "vkvg_amd_surface_read_premultiplied", "vkvg_amd_surface_device_ptr", "vkvg_amd_render_batch",
+    "vkvg_amd_device_get_stats", "vkvg_amd_device_set_tile_size", "vkvg_amd_device_engine",
+    # include/vkvg_hip.h
+    "vkvg_hip_engine_create", "vkvg_hip_engine_destroy", "vkvg_hip_set_tile_size", "vkvg_hip_set_stream",
+    "vkvg_hip_image_alloc", "vkvg_hip_image_free", "vkvg_hip_image_clear", "vkvg_hip_image_download",
+    "vkvg_hip_image_upload", "vkvg_hip_image_read_unpremultiplied", "vkvg_hip_render", "vkvg_hip_get_stats",
+    "vkvg_hip_last_error",
+]
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libvkvg.so; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise VkvgError(f"{LIB_PATH} is missing: build it with `python -m vkvg_amd.build` "
+                        "(the HIP backend has no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u32, f32, sz = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float, ctypes.c_size_t
+    L.vkvg_device_create.argtypes = [vp]
+    L.vkvg_device_create.restype = vp
+    L.vkvg_device_status.argtypes = [vp]
+    L.vkvg_device_status.restype = ctypes.c_int
+    L.vkvg_device_destroy.argtypes = [vp]
+    L.vkvg_surface_create.argtypes = [vp, u32, u32]
+    L.vkvg_surface_create.restype = vp
+    L.vkvg_surface_status.argtypes = [vp]
+    L.vkvg_surface_status.restype = ctypes.c_int
+    L.vkvg_surface_destroy.argtypes = [vp]
+    L.vkvg_surface_write_to_memory.argtypes = [vp, vp]
+    L.vkvg_surface_write_to_memory.restype = ctypes.c_int
+    L.vkvg_surface_write_to_png.argtypes = [vp, ctypes.c_char_p]
+    L.vkvg_surface_write_to_png.restype = ctypes.c_int
+    L.vkvg_surface_clear.argtypes = [vp]
+    L.vkvg_create.argtypes = [vp]
+    L.vkvg_create.restype = vp
+    L.vkvg_destroy.argtypes = [vp]
+    L.vkvg_flush.argtypes = [vp]
+    L.vkvg_status.argtypes = [vp]
+    L.vkvg_status.restype = ctypes.c_int
+    L.vkvg_amd_replay.argtypes = [vp, ctypes.POINTER(u32), sz]
+    L.vkvg_amd_replay.restype = ctypes.c_int
+    L.vkvg_amd_surface_enable_signature.argtypes = [vp]
+    L.vkvg_amd_surface_enable_signature.restype = ctypes.c_int
+    L.vkvg_amd_surface_read_signature.argtypes = [vp, vp]
+    L.vkvg_amd_surface_read_signature.restype = ctypes.c_int
+    L.vkvg_amd_surface_read_premultiplied.argtypes = [vp, vp]
+    L.vkvg_amd_surface_read_premultiplied.restype = ctypes.c_int
+    L.vkvg_amd_surface_device_ptr.argtypes = [vp]
+    L.vkvg_amd_surface_device_ptr.restype = vp
+    L.vkvg_amd_render_batch.argtypes = [vp, u32, ctypes.POINTER(vp), ctypes.POINTER(ctypes.POINTER(u32)),
+                                        ctypes.POINTER(sz)]
+    L.vkvg_amd_render_batch.restype = ctypes.c_int
+    L.vkvg_amd_device_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.vkvg_amd_device_set_tile_size.argtypes = [vp, u32]
+    L.vkvg_amd_device_set_tile_size.restype = ctypes.c_int
+    L.vkvg_amd_device_engine.argtypes = [vp]
+    L.vkvg_amd_device_engine.restype = vp
+    L.vkvg_hip_set_stream.argtypes = [vp, vp]
+    L.vkvg_hip_last_error.argtypes = [vp]
+    L.vkvg_hip_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _u32p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+class Device:
+    def __init__(self) -> None:
+        L = lib()
+        self.h = L.vkvg_device_create(None)
+        st = L.vkvg_device_status(self.h)
+        if st:
+            raise VkvgError(f"vkvg_device_create failed (status {st})")
+
+    def close(self) -> None:
+        if self.h:
+            lib().vkvg_device_destroy(self.h)
+            self.h = None
+
+    def set_tile_size(self, t: int) -> None:
+        if lib().vkvg_amd_device_set_tile_size(self.h, t):
+            raise VkvgError(f"bad tile size {t}")
+
+    def stats(self) -> dict:
+        s = Stats()
+        lib().vkvg_amd_device_get_stats(self.h, ctypes.byref(s))
+        return s.as_dict()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Surface:
+    def __init__(self, dev: Device, width: int, height: int, signature: bool = False) -> None:
+        L = lib()
+        self.dev = dev
+        self.width, self.height = width, height
+        self.h = L.vkvg_surface_create(dev.h, width, height)
+        st = L.vkvg_surface_status(self.h)
+        if st:
+            raise VkvgError(f"vkvg_surface_create failed (status {st})")
+        if signature and L.vkvg_amd_surface_enable_signature(self.h):
+            raise VkvgError("cannot enable coverage signatures")
+
+    def replay(self, words: np.ndarray) -> None:
+        """vkvg_create + replay of the stream + vkvg_destroy (which renders the frame)."""
+        L = lib()
+        w = np.ascontiguousarray(words, np.uint32)
+        ctx = L.vkvg_create(self.h)
+        st = L.vkvg_status(ctx)
+        if st:
+            raise VkvgError(f"vkvg_create failed (status {st})")
+        st = L.vkvg_amd_replay(ctx, _u32p(w), w.size)
+        L.vkvg_destroy(ctx)
+        if st:
+            raise VkvgError(f"replay failed (status {st}): "
+                            f"{L.vkvg_hip_last_error(L.vkvg_amd_device_engine(self.dev.h)).decode()}")
+
+    def premultiplied(self) -> np.ndarray:
+        out = np.empty((self.height, self.width, 4), np.uint8)
+        if lib().vkvg_amd_surface_read_premultiplied(self.h, out.ctypes.data):
+            raise VkvgError("readback failed")
+        return out
+
+    def rgba(self) -> np.ndarray:
+        """vkvg_surface_write_to_memory: RGBA with the reference's truncating unpremultiply."""
+        out = np.empty((self.height, self.width, 4), np.uint8)
+        if lib().vkvg_surface_write_to_memory(self.h, out.ctypes.data):
+            raise VkvgError("readback failed")
+        return out
+
+    def signature(self) -> np.ndarray:
+        out = np.empty((self.height, self.width), np.uint32)
+        if lib().vkvg_amd_surface_read_signature(self.h, out.ctypes.data):
+            raise VkvgError("signature readback failed (not enabled?)")
+        return out
+
+    def write_png(self, path: str) -> None:
+        if lib().vkvg_surface_write_to_png(self.h, path.encode()):
+            raise VkvgError("write_to_png failed")
+
+    def device_ptr(self) -> int:
+        return int(lib().vkvg_amd_surface_device_ptr(self.h))
+
+    def close(self) -> None:
+        if self.h:
+            lib().vkvg_surface_destroy(self.h)
+            self.h = None
+
+
+def render_batch(dev: Device, surfaces: list[Surface], streams: list[np.ndarray]) -> None:
+    L = lib()
+    n = len(surfaces)
+    ws = [np.ascontiguousarray(s, np.uint32) for s in streams]
+    hs = (ctypes.c_void_p * n)(*[s.h for s in surfaces])
+    ps = (ctypes.POINTER(ctypes.c_uint32) * n)(*[_u32p(w) for w in ws])
+    ls = (ctypes.c_size_t * n)(*[w.size for w in ws])
+    st = L.vkvg_amd_render_batch(dev.h, n, hs, ps, ls)
+    if st:
+        raise VkvgError(f"render_batch failed (status {st})")

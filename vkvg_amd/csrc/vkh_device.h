@@ -1,0 +1,288 @@
+// Device-side building blocks shared by the vkvg HIP kernels (gfx950).
+//
+// Every float expression keeps the operand order of the reference so that, compiled with
+// -ffp-contract=off and IEEE division / square root, the kernels produce the same bits as
+// the reference's host code:
+//   vec2 helpers        src/vectors.h:92-165
+//   AGG flattening      src/vkvg_context_internal.c:1313-1461
+//   joins / caps        src/vkvg_context_internal.c:924-1239
+// Transcendental functions are evaluated in double and rounded once to float.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vkvg_hip.h"
+
+#define VKH_PIF   3.14159265358979323846f
+#define VKH_PIF_2 1.57079632679489661923f
+#define VKH_2_PIF 0.63661977236758134308f /* 2/pi where 2*pi was meant (reproduced) */
+#define VKH_FLT_EPSILON 1.19209290e-7f
+
+struct v2 {
+    float x, y;
+};
+
+__device__ __forceinline__ v2 V(float x, float y) { return v2{x, y}; }
+__device__ __forceinline__ v2 vadd(v2 a, v2 b) { return V(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ v2 vsub(v2 a, v2 b) { return V(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ v2 vmul(v2 a, float m) { return V(a.x * m, a.y * m); }
+__device__ __forceinline__ v2 vdiv(v2 a, float m) { return V(a.x / m, a.y / m); }
+__device__ __forceinline__ v2 vperp(v2 a) { return V(a.y, -a.x); }
+__device__ __forceinline__ float vlen(v2 v) { return sqrtf(v.x * v.x + v.y * v.y); }
+__device__ __forceinline__ v2 vnorm(v2 a) {
+    float m = sqrtf(a.x * a.x + a.y * a.y);
+    return V(a.x / m, a.y / m);
+}
+__device__ __forceinline__ float vdot(v2 a, v2 b) { return (a.x * b.x) + (a.y * b.y); }
+__device__ __forceinline__ bool equf(float a, float b) { return fabsf(a - b) <= VKH_FLT_EPSILON; }
+
+__device__ __forceinline__ float d_atan2f(float y, float x) { return (float)atan2((double)y, (double)x); }
+__device__ __forceinline__ float d_cosf(float a) { return (float)cos((double)a); }
+__device__ __forceinline__ float d_sinf(float a) { return (float)sin((double)a); }
+__device__ __forceinline__ float d_acosf(float a) { return (float)acos((double)a); }
+
+// ---------------------------------------------------------------------------------------
+// AGG adaptive subdivision. The reference recurses depth-first (left child first) and emits
+// at the leaves. Here the traversal is stackless: the current node is identified by its path
+// of left/right choices from the root (up to 101 levels), and its control points are
+// recomputed from the root, which repeats exactly the reference's float operations.
+struct Cubic {
+    float x1, y1, x2, y2, x3, y3, x4, y4;
+};
+
+__device__ __forceinline__ void agg_split(const Cubic &c, bool right, Cubic &o) {
+    float x12 = (c.x1 + c.x2) / 2, y12 = (c.y1 + c.y2) / 2;
+    float x23 = (c.x2 + c.x3) / 2, y23 = (c.y2 + c.y3) / 2;
+    float x34 = (c.x3 + c.x4) / 2, y34 = (c.y3 + c.y4) / 2;
+    float x123 = (x12 + x23) / 2, y123 = (y12 + y23) / 2;
+    float x234 = (x23 + x34) / 2, y234 = (y23 + y34) / 2;
+    float x1234 = (x123 + x234) / 2, y1234 = (y123 + y234) / 2;
+    if (!right)
+        o = Cubic{c.x1, c.y1, x12, y12, x123, y123, x1234, y1234};
+    else
+        o = Cubic{x1234, y1234, x234, y234, x34, y34, c.x4, c.y4};
+}
+
+// Leaf test of one node at `level` (>= 1). Returns the number of points emitted (0 means
+// "subdivide further"), writing them to pts. The comparisons against the reference's double
+// constants (1.7, 0.01) promote the float operand exactly as C does.
+__device__ __forceinline__ int agg_leaf(const Cubic &c, float tol, v2 *pts) {
+    float x1 = c.x1, y1 = c.y1, x2 = c.x2, y2 = c.y2, x3 = c.x3, y3 = c.y3, x4 = c.x4, y4 = c.y4;
+    float dx = x4 - x1, dy = y4 - y1;
+    float d2 = fabsf(((x2 - x4) * dy - (y2 - y4) * dx));
+    float d3 = fabsf(((x3 - x4) * dy - (y3 - y4) * dx));
+    float da1, da2;
+    if ((double)d2 > 1.7 && (double)d3 > 1.7) {
+        if ((d2 + d3) * (d2 + d3) <= (dx * dx + dy * dy) * tol) {
+            float a23 = d_atan2f(y3 - y2, x3 - x2);
+            da1 = fabsf(a23 - d_atan2f(y2 - y1, x2 - x1));
+            da2 = fabsf(d_atan2f(y4 - y3, x4 - x3) - a23);
+            if (da1 >= VKH_PIF)
+                da1 = VKH_2_PIF - da1;
+            if (da2 >= VKH_PIF)
+                da2 = VKH_2_PIF - da2;
+            if (da1 + da2 < (float)0.01) {
+                float x12 = (x1 + x2) / 2, y12 = (y1 + y2) / 2, x23 = (x2 + x3) / 2, y23 = (y2 + y3) / 2;
+                float x34 = (x3 + x4) / 2, y34 = (y3 + y4) / 2;
+                float x123 = (x12 + x23) / 2, y123 = (y12 + y23) / 2, x234 = (x23 + x34) / 2, y234 = (y23 + y34) / 2;
+                pts[0] = V((x123 + x234) / 2, (y123 + y234) / 2);
+                return 1;
+            }
+            if ((double)da1 > 0.01) {
+                pts[0] = V(x2, y2);
+                return 1;
+            }
+            if ((double)da2 > 0.01) {
+                pts[0] = V(x3, y3);
+                return 1;
+            }
+        }
+    } else {
+        if ((double)d2 > 1.7) {
+            if (d2 * d2 <= tol * (dx * dx + dy * dy)) {
+                da1 = fabsf(d_atan2f(y3 - y2, x3 - x2) - d_atan2f(y2 - y1, x2 - x1));
+                if (da1 >= VKH_PIF)
+                    da1 = VKH_2_PIF - da1;
+                if ((double)da1 < 0.01) {
+                    pts[0] = V(x2, y2);
+                    pts[1] = V(x3, y3);
+                    return 2;
+                }
+                if ((double)da1 > 0.01) {
+                    pts[0] = V(x2, y2);
+                    return 1;
+                }
+            }
+        } else if ((double)d3 > 1.7) {
+            if (d3 * d3 <= tol * (dx * dx + dy * dy)) {
+                da1 = fabsf(d_atan2f(y4 - y3, x4 - x3) - d_atan2f(y3 - y2, x3 - x2));
+                if (da1 >= VKH_PIF)
+                    da1 = VKH_2_PIF - da1;
+                if ((double)da1 < 0.01) {
+                    pts[0] = V(x2, y2);
+                    pts[1] = V(x3, y3);
+                    return 2;
+                }
+                if ((double)da1 > 0.01) {
+                    pts[0] = V(x3, y3);
+                    return 1;
+                }
+            }
+        } else {
+            float x12 = (x1 + x2) / 2, y12 = (y1 + y2) / 2, x23 = (x2 + x3) / 2, y23 = (y2 + y3) / 2;
+            float x34 = (x3 + x4) / 2, y34 = (y3 + y4) / 2;
+            float x123 = (x12 + x23) / 2, y123 = (y12 + y23) / 2, x234 = (x23 + x34) / 2, y234 = (y23 + y34) / 2;
+            float x1234 = (x123 + x234) / 2, y1234 = (y123 + y234) / 2;
+            float ex = x1234 - (x1 + x4) / 2;
+            float ey = y1234 - (y1 + y4) / 2;
+            if (ex * ex + ey * ey <= tol) {
+                pts[0] = V(x1234, y1234);
+                return 1;
+            }
+        }
+    }
+    return 0;
+}
+
+// Depth-first traversal; calls emit(v2) for every emitted point in the reference's order.
+// Path bits: bit (level-1) of `path` is 1 when the node at `level` is a right child.
+template <typename Emit>
+__device__ void agg_flatten(const Cubic &root, float tol, Emit emit) {
+    uint64_t path_lo = 0, path_hi = 0; // 128 bits of choices, levels 1..101
+    int level = 1;                      // the root (level 0) always subdivides
+    Cubic node;
+    agg_split(root, false, node);
+    while (true) {
+        bool leaf;
+        if (level > 100) {
+            leaf = true; // recursion limit: returns without emitting
+        } else {
+            v2 pts[2];
+            int n = agg_leaf(node, tol, pts);
+            leaf = n > 0;
+            for (int i = 0; i < n; i++)
+                emit(pts[i]);
+        }
+        if (!leaf) {
+            // descend to the left child
+            Cubic ch;
+            agg_split(node, false, ch);
+            node = ch;
+            level++;
+            // bit (level-1) stays 0 (left)
+            continue;
+        }
+        // backtrack: pop right children until a left child is found, then go to its sibling
+        while (level >= 1) {
+            int b = level - 1;
+            bool is_right = b < 64 ? ((path_lo >> b) & 1ull) : ((path_hi >> (b - 64)) & 1ull);
+            if (!is_right)
+                break;
+            if (b < 64)
+                path_lo &= ~(1ull << b);
+            else
+                path_hi &= ~(1ull << (b - 64));
+            level--;
+        }
+        if (level < 1)
+            return;
+        {
+            int b = level - 1;
+            if (b < 64)
+                path_lo |= (1ull << b);
+            else
+                path_hi |= (1ull << (b - 64));
+        }
+        // recompute the node at `level` from the root along the path
+        Cubic cur = root;
+        for (int l = 1; l <= level; l++) {
+            int b = l - 1;
+            bool r = b < 64 ? ((path_lo >> b) & 1ull) : ((path_hi >> (b - 64)) & 1ull);
+            Cubic nx;
+            agg_split(cur, r, nx);
+            cur = nx;
+        }
+        node = cur;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Vertex transform and 1/256 snap (shaders/vkvg_main.vert:74-79 + viewport transform).
+#define VKH_INVALID_FIX ((int32_t)0x80000000)
+
+__device__ __forceinline__ void snap_vertex(float x, float y, const float *m, float sw, float sh, int32_t &fx,
+                                            int32_t &fy) {
+    float px = (m[0] * x + m[2] * y) + m[4];
+    float py = (m[1] * x + m[3] * y) + m[5];
+    float nx = (px * 2.0f) / sw - 1.0f;
+    float ny = (py * 2.0f) / sh - 1.0f;
+    float hw = sw * 0.5f, hh = sh * 0.5f;
+    float X = nx * hw + hw;
+    float Y = ny * hh + hh;
+    if (isnan(X) || isnan(Y)) {
+        fx = fy = VKH_INVALID_FIX;
+        return;
+    }
+    X = fminf(fmaxf(X, -32768.0f), 32768.0f);
+    Y = fminf(fmaxf(Y, -32768.0f), 32768.0f);
+    fx = (int32_t)rintf(X * 256.0f);
+    fy = (int32_t)rintf(Y * 256.0f);
+}
+
+// ---------------------------------------------------------------------------------------
+// Exact winding predicates on 24.8 fixed-point edges. Samples sit at (256c+128, 256r+128)
+// perturbed by (+eps, +eps^2), which reproduces the top-left tie rule: no sample is ever on
+// an edge. Winding uses a ray towards -x; an edge counts +1 when it runs towards +y.
+struct Edge {
+    int32_t x0, y0, x1, y1;
+};
+
+__device__ __forceinline__ int64_t ceil_div_pos(int64_t n, int64_t d) { // d > 0, |n|,d < 2^53, |n/d| < 2^20
+    double q = (double)n / (double)d;
+    return (int64_t)ceil(q);
+}
+
+// First column whose (perturbed) sample at row sy lies right of the edge; the caller
+// guarantees the edge crosses the row (ytop <= sy < ybot).
+__device__ __forceinline__ int64_t crossing_column(int32_t xt, int32_t yt, int32_t xb, int32_t yb, int32_t sy) {
+    int64_t dyb = (int64_t)yb - yt;
+    int64_t N = dyb * ((int64_t)xt - 128) + ((int64_t)xb - xt) * ((int64_t)sy - yt);
+    return ceil_div_pos(N, 256 * dyb);
+}
+
+// Is the perturbed point (px, py) above the edge (towards -y)? Only called for edges that
+// cross the vertical line through px (min x <= px < max x).
+__device__ __forceinline__ bool point_above(const Edge &e, int32_t px, int32_t py) {
+    int32_t lx, ly, rx, ry;
+    if (e.x0 < e.x1) {
+        lx = e.x0; ly = e.y0; rx = e.x1; ry = e.y1;
+    } else {
+        lx = e.x1; ly = e.y1; rx = e.x0; ry = e.y0;
+    }
+    int64_t E0 = ((int64_t)rx - lx) * ((int64_t)py - ly) - ((int64_t)ry - ly) * ((int64_t)px - lx);
+    return E0 < 0 || (E0 == 0 && ry > ly);
+}
+
+// ---------------------------------------------------------------------------------------
+// UNORM8 compositing (premultiplied OVER in 8-bit arithmetic).
+__device__ __forceinline__ uint32_t mul255(uint32_t a, uint32_t b) {
+    uint32_t t = a * b + 128u;
+    return (t + (t >> 8)) >> 8;
+}
+__device__ __forceinline__ uint8_t f2u8(float f) {
+    if (!(f > 0.f))
+        return 0;
+    if (f >= 1.f)
+        return 255;
+    return (uint8_t)rintf(f * 255.0f);
+}
+__device__ __forceinline__ uint32_t draw_key(uint32_t n) {
+    uint32_t x = n + 0x9E3779B9u;
+    x ^= x >> 16;
+    x *= 0x85EBCA6Bu;
+    x ^= x >> 13;
+    x *= 0xC2B2AE35u;
+    x ^= x >> 16;
+    return x | 1u;
+}

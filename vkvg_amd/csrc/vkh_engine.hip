@@ -1,0 +1,577 @@
+// Engine of the vkvg HIP backend: device buffers, the frame pipeline and the C ABI of
+// include/vkvg_hip.h. One engine per device; a frame may hold draws for many targets.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vkh_internal.h"
+
+// ----------------------------------------------------------------------------------------- scan
+// Exclusive prefix sum over n uint32 values into out[0..n] (out[n] = total): block scans of
+// 1024 values (256 threads x 4), block totals scanned recursively, then added back.
+__global__ __launch_bounds__(256) void k_scan_block(const uint32_t *__restrict__ in, uint32_t n,
+                                                    uint32_t *__restrict__ out, uint32_t *__restrict__ sums) {
+    __shared__ uint32_t wsum[4];
+    uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
+    uint32_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        v[k] = (base + k < n) ? in[base + k] : 0u;
+    uint32_t t = v[0] + v[1] + v[2] + v[3];
+    uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t incl = t;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(incl, o);
+        if (lane >= (uint32_t)o)
+            incl += y;
+    }
+    if (lane == 63)
+        wsum[wid] = incl;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (uint32_t w = 0; w < wid; w++)
+        woff += wsum[w];
+    uint32_t run = woff + incl - t;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (base + k <= n)
+            out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 255 && sums)
+        sums[blockIdx.x] = woff + incl;
+}
+__global__ void k_scan_add(uint32_t *__restrict__ out, uint32_t n, const uint32_t *__restrict__ offs) {
+    uint32_t i = blockIdx.x * 1024u + threadIdx.x;
+    uint32_t o = offs[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        uint32_t j = i + 256u * k;
+        if (j <= n)
+            out[j] += o;
+    }
+}
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct HostBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+} // namespace
+
+struct vkh_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint32_t tile = 16;
+    std::string err;
+    vkh_stats stats{};
+
+    DevBuf blob, units, tgts, item_cnt, item_off, pts, pflags, sp_first, sp_count;
+    DevBuf fe, nv, nt, fe_off, nv_off, nt_off, verts, tris, edges, edraw;
+    DevBuf bbox, dtiles, pt_count, pt_off, row_cnt, row_off, ptiles, nseg, seg_off, segs;
+    DevBuf tile_cnt, tile_off, cmds, scan_tmp, readback;
+    HostBuf hblob, htot;
+};
+
+static bool check(vkh_engine *e, hipError_t r, const char *what) {
+    if (r == hipSuccess)
+        return true;
+    e->err = std::string(what) + ": " + hipGetErrorString(r);
+    return false;
+}
+#define CK(x)                                                                                                          \
+    do {                                                                                                               \
+        if (!check(eng, (x), #x))                                                                                      \
+            return -1;                                                                                                 \
+    } while (0)
+
+static bool ensure(vkh_engine *eng, DevBuf &b, size_t bytes) {
+    if (bytes <= b.cap && b.p)
+        return true;
+    if (b.p)
+        (void)hipFree(b.p);
+    b.p = nullptr;
+    size_t cap = bytes + bytes / 2 + 256;
+    if (!check(eng, hipMalloc(&b.p, cap), "hipMalloc")) {
+        b.cap = 0;
+        return false;
+    }
+    b.cap = cap;
+    return true;
+}
+static bool ensure_host(vkh_engine *eng, HostBuf &b, size_t bytes) {
+    if (bytes <= b.cap && b.p)
+        return true;
+    if (b.p)
+        (void)hipHostFree(b.p);
+    b.p = nullptr;
+    size_t cap = bytes + bytes / 2 + 256;
+    if (!check(eng, hipHostMalloc(&b.p, cap, 0), "hipHostMalloc")) {
+        b.cap = 0;
+        return false;
+    }
+    b.cap = cap;
+    return true;
+}
+template <typename T> static T *P(DevBuf &b) { return (T *)b.p; }
+
+// exclusive scan of n values; out must hold n+1 entries (out[n] = total)
+static int scan_u32(vkh_engine *eng, const uint32_t *in, uint32_t n, uint32_t *out) {
+    hipStream_t s = eng->stream;
+    struct Level {
+        const uint32_t *in;
+        uint32_t n;
+        uint32_t *out;
+        uint32_t nb;
+    };
+    // level sizes: block sums of level k are the input of level k+1
+    std::vector<uint32_t> nbs;
+    size_t need = 0;
+    for (uint32_t m = n;;) {
+        uint32_t nb = (m + 1 + 1023) / 1024;
+        nbs.push_back(nb);
+        if (nb <= 1)
+            break;
+        need += 2ull * nb + 1; // sums + their scan
+        m = nb;
+    }
+    if (!ensure(eng, eng->scan_tmp, (need + 1) * sizeof(uint32_t)))
+        return -1;
+    uint32_t *tmp = P<uint32_t>(eng->scan_tmp);
+    std::vector<Level> lv;
+    const uint32_t *cin = in;
+    uint32_t cn = n;
+    uint32_t *cout = out;
+    for (size_t k = 0; k < nbs.size(); k++) {
+        uint32_t nb = nbs[k];
+        uint32_t *sums = nb > 1 ? tmp : nullptr;
+        hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, s, cin, cn, cout, sums);
+        lv.push_back(Level{cin, cn, cout, nb});
+        if (nb <= 1)
+            break;
+        uint32_t *sscan = tmp + nb;
+        tmp += 2ull * nb + 1;
+        cin = sums;
+        cn = nb;
+        cout = sscan;
+    }
+    for (size_t k = lv.size(); k-- > 1;)
+        hipLaunchKernelGGL(k_scan_add, dim3(lv[k - 1].nb), dim3(256), 0, s, lv[k - 1].out, lv[k - 1].n, lv[k].out);
+    return check(eng, hipGetLastError(), "scan") ? 0 : -1;
+}
+
+static int read_totals(vkh_engine *eng, const std::vector<const uint32_t *> &srcs, uint32_t *dst) {
+    if (!ensure_host(eng, eng->htot, 64 * sizeof(uint32_t)))
+        return -1;
+    uint32_t *h = (uint32_t *)eng->htot.p;
+    for (size_t i = 0; i < srcs.size(); i++)
+        CK(hipMemcpyAsync(h + i, srcs[i], 4, hipMemcpyDeviceToHost, eng->stream));
+    CK(hipStreamSynchronize(eng->stream));
+    for (size_t i = 0; i < srcs.size(); i++)
+        dst[i] = h[i];
+    return 0;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+extern "C" {
+
+vkh_engine *vkvg_hip_engine_create(int device_ordinal) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device_ordinal || device_ordinal < 0)
+        return nullptr;
+    if (hipSetDevice(device_ordinal) != hipSuccess)
+        return nullptr;
+    vkh_engine *e = new vkh_engine();
+    e->device = device_ordinal;
+    if (hipStreamCreateWithFlags(&e->own, hipStreamNonBlocking) != hipSuccess) {
+        delete e;
+        return nullptr;
+    }
+    e->stream = e->own;
+    (void)hipEventCreate(&e->ev0);
+    (void)hipEventCreate(&e->ev1);
+    return e;
+}
+
+void vkvg_hip_engine_destroy(vkh_engine *e) {
+    if (!e)
+        return;
+    (void)hipSetDevice(e->device);
+    if (e->stream)
+        (void)hipStreamSynchronize(e->stream);
+    DevBuf *bufs[] = {&e->blob, &e->units, &e->tgts, &e->item_cnt, &e->item_off, &e->pts, &e->pflags,
+                      &e->sp_first, &e->sp_count, &e->fe, &e->nv, &e->nt, &e->fe_off, &e->nv_off, &e->nt_off,
+                      &e->verts, &e->tris, &e->edges, &e->edraw, &e->bbox, &e->dtiles, &e->pt_count, &e->pt_off,
+                      &e->row_cnt, &e->row_off, &e->ptiles, &e->nseg, &e->seg_off, &e->segs, &e->tile_cnt,
+                      &e->tile_off, &e->cmds, &e->scan_tmp, &e->readback};
+    for (DevBuf *b : bufs)
+        if (b->p)
+            (void)hipFree(b->p);
+    if (e->hblob.p)
+        (void)hipHostFree(e->hblob.p);
+    if (e->htot.p)
+        (void)hipHostFree(e->htot.p);
+    if (e->ev0)
+        (void)hipEventDestroy(e->ev0);
+    if (e->ev1)
+        (void)hipEventDestroy(e->ev1);
+    if (e->own)
+        (void)hipStreamDestroy(e->own);
+    delete e;
+}
+
+int vkvg_hip_set_tile_size(vkh_engine *e, uint32_t tile) {
+    if (!e)
+        return -1;
+    if (tile == 0)
+        tile = 16;
+    if (tile != 8 && tile != 16 && tile != 32)
+        return -1;
+    e->tile = tile;
+    return 0;
+}
+
+void vkvg_hip_set_stream(vkh_engine *e, void *stream) {
+    if (e)
+        e->stream = stream ? (hipStream_t)stream : e->own;
+}
+
+int vkvg_hip_image_alloc(vkh_engine *eng, uint32_t w, uint32_t h, void **dev_ptr) {
+    if (!eng || !dev_ptr)
+        return -1;
+    (void)hipSetDevice(eng->device);
+    size_t bytes = (size_t)w * h * 4;
+    CK(hipMalloc(dev_ptr, bytes ? bytes : 4));
+    CK(hipMemsetAsync(*dev_ptr, 0, bytes ? bytes : 4, eng->stream));
+    CK(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+void vkvg_hip_image_free(vkh_engine *eng, void *p) {
+    if (eng && p) {
+        (void)hipSetDevice(eng->device);
+        (void)hipStreamSynchronize(eng->stream);
+        (void)hipFree(p);
+    }
+}
+int vkvg_hip_image_clear(vkh_engine *eng, void *p, size_t bytes) {
+    if (!eng || !p)
+        return -1;
+    CK(hipMemsetAsync(p, 0, bytes, eng->stream));
+    CK(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+int vkvg_hip_image_download(vkh_engine *eng, const void *p, void *host, size_t bytes) {
+    if (!eng || !p || !host)
+        return -1;
+    CK(hipMemcpyAsync(host, p, bytes, hipMemcpyDeviceToHost, eng->stream));
+    CK(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+int vkvg_hip_image_upload(vkh_engine *eng, void *p, const void *host, size_t bytes) {
+    if (!eng || !p || !host)
+        return -1;
+    CK(hipMemcpyAsync(p, host, bytes, hipMemcpyHostToDevice, eng->stream));
+    CK(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+int vkvg_hip_image_read_unpremultiplied(vkh_engine *eng, const void *p, uint32_t w, uint32_t h, void *host) {
+    if (!eng || !p || !host)
+        return -1;
+    size_t npix = (size_t)w * h;
+    if (!ensure(eng, eng->readback, npix * 4))
+        return -1;
+    vkh::launch_unpremultiply(eng->stream, (const uint8_t *)p, P<uint8_t>(eng->readback), npix);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(host, eng->readback.p, npix * 4, hipMemcpyDeviceToHost, eng->stream));
+    CK(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+
+void vkvg_hip_get_stats(vkh_engine *e, vkh_stats *out) {
+    if (e && out)
+        *out = e->stats;
+}
+const char *vkvg_hip_last_error(vkh_engine *e) { return e ? e->err.c_str() : "null engine"; }
+
+int vkvg_hip_render(vkh_engine *eng, const vkh_frame *f) {
+    if (!eng || !f)
+        return -1;
+    auto t_start = std::chrono::steady_clock::now();
+    (void)hipSetDevice(eng->device);
+    hipStream_t s = eng->stream;
+    const uint32_t T = eng->tile;
+    vkh_stats st{};
+    st.tile_size = T;
+
+    // ---- host-side frame tables: work units and targets
+    std::vector<Unit> units;
+    units.reserve(f->n_subpaths + 16);
+    for (uint32_t d = 0; d < f->n_draws; d++) {
+        const vkh_draw &dr = f->draws[d];
+        if (dr.target >= f->n_targets) {
+            eng->err = "draw target out of range";
+            return -2;
+        }
+        if (d && dr.target < f->draws[d - 1].target) {
+            eng->err = "draws must be grouped by target in ascending order";
+            return -2;
+        }
+        if (dr.kind == VKH_DRAW_PAINT || dr.kind == VKH_DRAW_CLEAR)
+            continue;
+        if (dr.kind == VKH_DRAW_STROKE && (dr.style < 0 || (uint32_t)dr.style >= f->n_styles)) {
+            eng->err = "stroke draw without a style";
+            return -2;
+        }
+        if (dr.grad >= 0 && (uint32_t)dr.grad >= f->n_grads) {
+            eng->err = "gradient index out of range";
+            return -2;
+        }
+        if ((uint64_t)dr.first_subpath + dr.n_subpaths > f->n_subpaths) {
+            eng->err = "subpath range out of bounds";
+            return -2;
+        }
+        for (uint32_t k = 0; k < dr.n_subpaths; k++)
+            units.push_back(Unit{d, dr.first_subpath + k});
+    }
+    for (uint32_t i = 0; i < f->n_subpaths; i++) {
+        if ((uint64_t)f->subpaths[i].first_item + f->subpaths[i].n_items > f->n_items || !f->subpaths[i].n_items) {
+            eng->err = "item range of a subpath out of bounds";
+            return -2;
+        }
+    }
+    for (uint32_t i = 0; i < f->n_styles; i++) {
+        if ((uint64_t)f->styles[i].dash_first + f->styles[i].dash_count > f->n_dashes) {
+            eng->err = "dash range out of bounds";
+            return -2;
+        }
+    }
+    std::vector<TargetInfo> tg(f->n_targets);
+    uint32_t tile_total = 0, bin_total = 0;
+    {
+        uint32_t d = 0;
+        for (uint32_t t = 0; t < f->n_targets; t++) {
+            TargetInfo &ti = tg[t];
+            ti.width = f->targets[t].width;
+            ti.height = f->targets[t].height;
+            if (ti.width == 0 || ti.height == 0 || ti.width > 32768 || ti.height > 32768 || !f->targets[t].rgba) {
+                eng->err = "invalid render target";
+                return -2;
+            }
+            ti.tilesW = (ti.width + T - 1) / T;
+            ti.tilesH = (ti.height + T - 1) / T;
+            ti.tile_base = tile_total;
+            tile_total += ti.tilesW * ti.tilesH;
+            ti.binsW = (ti.tilesW + VKH_BIN_TILES - 1) / VKH_BIN_TILES;
+            ti.binsH = (ti.tilesH + VKH_BIN_TILES - 1) / VKH_BIN_TILES;
+            ti.bin_base = bin_total;
+            bin_total += ti.binsW * ti.binsH;
+            ti.draw_begin = d;
+            while (d < f->n_draws && f->draws[d].target == t)
+                d++;
+            ti.draw_end = d;
+            ti.rgba = (uint8_t *)f->targets[t].rgba;
+            ti.sig = f->targets[t].sig;
+        }
+    }
+    const uint32_t n_units = (uint32_t)units.size();
+    const uint32_t n_items = f->n_items, n_sp = f->n_subpaths, n_draws = f->n_draws;
+
+    // ---- upload everything in one copy through a pinned staging blob
+    size_t o_items = 0;
+    size_t o_sp = align256(o_items + sizeof(vkh_item) * n_items);
+    size_t o_draws = align256(o_sp + sizeof(vkh_subpath) * n_sp);
+    size_t o_styles = align256(o_draws + sizeof(vkh_draw) * n_draws);
+    size_t o_dashes = align256(o_styles + sizeof(vkh_stroke_style) * f->n_styles);
+    size_t o_grads = align256(o_dashes + sizeof(float) * f->n_dashes);
+    size_t o_tg = align256(o_grads + sizeof(vkh_gradient) * f->n_grads);
+    size_t o_units = align256(o_tg + sizeof(TargetInfo) * f->n_targets);
+    size_t total = align256(o_units + sizeof(Unit) * n_units) + 256;
+    if (!ensure_host(eng, eng->hblob, total) || !ensure(eng, eng->blob, total))
+        return -1;
+    uint8_t *hb = (uint8_t *)eng->hblob.p;
+    if (n_items)
+        memcpy(hb + o_items, f->items, sizeof(vkh_item) * n_items);
+    if (n_sp)
+        memcpy(hb + o_sp, f->subpaths, sizeof(vkh_subpath) * n_sp);
+    if (n_draws)
+        memcpy(hb + o_draws, f->draws, sizeof(vkh_draw) * n_draws);
+    if (f->n_styles)
+        memcpy(hb + o_styles, f->styles, sizeof(vkh_stroke_style) * f->n_styles);
+    if (f->n_dashes)
+        memcpy(hb + o_dashes, f->dashes, sizeof(float) * f->n_dashes);
+    if (f->n_grads)
+        memcpy(hb + o_grads, f->grads, sizeof(vkh_gradient) * f->n_grads);
+    if (f->n_targets)
+        memcpy(hb + o_tg, tg.data(), sizeof(TargetInfo) * f->n_targets);
+    if (n_units)
+        memcpy(hb + o_units, units.data(), sizeof(Unit) * n_units);
+    CK(hipMemcpyAsync(eng->blob.p, hb, total, hipMemcpyHostToDevice, s));
+    uint8_t *db = (uint8_t *)eng->blob.p;
+    const vkh_item *d_items = (const vkh_item *)(db + o_items);
+    const vkh_subpath *d_sp = (const vkh_subpath *)(db + o_sp);
+    const vkh_draw *d_draws = (const vkh_draw *)(db + o_draws);
+    const vkh_stroke_style *d_styles = (const vkh_stroke_style *)(db + o_styles);
+    const float *d_dashes = (const float *)(db + o_dashes);
+    const vkh_gradient *d_grads = (const vkh_gradient *)(db + o_grads);
+    const TargetInfo *d_tg = (const TargetInfo *)(db + o_tg);
+    const Unit *d_units = (const Unit *)(db + o_units);
+
+    // ---- flatten
+    if (!ensure(eng, eng->item_cnt, 4 * (n_items + 1)) || !ensure(eng, eng->item_off, 4 * (n_items + 1)))
+        return -1;
+    vkh::launch_flatten_count(s, d_items, n_items, P<uint32_t>(eng->item_cnt));
+    if (scan_u32(eng, P<uint32_t>(eng->item_cnt), n_items, P<uint32_t>(eng->item_off)))
+        return -1;
+    uint32_t tot[8];
+    if (read_totals(eng, {P<uint32_t>(eng->item_off) + n_items}, tot))
+        return -1;
+    const uint32_t n_pts = tot[0];
+    st.points = n_pts;
+    if (!ensure(eng, eng->pts, 8ull * n_pts + 8) || !ensure(eng, eng->pflags, n_pts + 8) ||
+        !ensure(eng, eng->sp_first, 4 * (n_sp + 1)) || !ensure(eng, eng->sp_count, 4 * (n_sp + 1)))
+        return -1;
+    vkh::launch_flatten_emit(s, d_items, n_items, P<uint32_t>(eng->item_off), P<v2>(eng->pts),
+                             P<uint8_t>(eng->pflags));
+    vkh::launch_subpath_ranges(s, d_sp, n_sp, P<uint32_t>(eng->item_off), P<uint32_t>(eng->sp_first),
+                               P<uint32_t>(eng->sp_count));
+    CK(hipGetLastError());
+
+    // ---- per-unit geometry counts
+    size_t un4 = 4ull * (n_units + 1);
+    if (!ensure(eng, eng->fe, un4) || !ensure(eng, eng->nv, un4) || !ensure(eng, eng->nt, un4) ||
+        !ensure(eng, eng->fe_off, un4) || !ensure(eng, eng->nv_off, un4) || !ensure(eng, eng->nt_off, un4))
+        return -1;
+    vkh::launch_unit_count(s, d_units, n_units, d_draws, d_sp, P<uint32_t>(eng->sp_first), P<uint32_t>(eng->sp_count),
+                           P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes, P<uint32_t>(eng->fe),
+                           P<uint32_t>(eng->nv), P<uint32_t>(eng->nt));
+    if (scan_u32(eng, P<uint32_t>(eng->fe), n_units, P<uint32_t>(eng->fe_off)) ||
+        scan_u32(eng, P<uint32_t>(eng->nv), n_units, P<uint32_t>(eng->nv_off)) ||
+        scan_u32(eng, P<uint32_t>(eng->nt), n_units, P<uint32_t>(eng->nt_off)))
+        return -1;
+    if (read_totals(eng,
+                    {P<uint32_t>(eng->fe_off) + n_units, P<uint32_t>(eng->nv_off) + n_units,
+                     P<uint32_t>(eng->nt_off) + n_units},
+                    tot))
+        return -1;
+    const uint32_t n_fill_edges = tot[0], n_verts = tot[1], n_tris = tot[2];
+    const uint64_t n_edges64 = (uint64_t)n_fill_edges + 3ull * n_tris;
+    if (n_edges64 > 0x7FFFFFFFull) {
+        eng->err = "frame too large (edges)";
+        return -3;
+    }
+    const uint32_t n_edges = (uint32_t)n_edges64;
+    st.stroke_vertices = n_verts;
+    st.stroke_triangles = n_tris;
+    st.edges = n_edges;
+    if (!ensure(eng, eng->verts, 8ull * n_verts + 8) || !ensure(eng, eng->tris, 12ull * n_tris + 12) ||
+        !ensure(eng, eng->edges, sizeof(Edge) * (size_t)n_edges + 16) ||
+        !ensure(eng, eng->edraw, 4ull * n_edges + 4))
+        return -1;
+    vkh::launch_stroke_emit(s, d_units, n_units, d_draws, d_sp, P<uint32_t>(eng->sp_first),
+                            P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes,
+                            P<uint32_t>(eng->nv_off), P<uint32_t>(eng->nt_off), P<v2>(eng->verts),
+                            P<uint32_t>(eng->tris));
+    vkh::launch_fill_edges(s, d_units, n_units, P<uint32_t>(eng->fe_off), n_fill_edges, d_draws, d_tg,
+                           P<uint32_t>(eng->sp_first), P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<Edge>(eng->edges),
+                           P<uint32_t>(eng->edraw));
+    vkh::launch_tri_edges(s, d_units, n_units, P<uint32_t>(eng->nt_off), P<uint32_t>(eng->nv_off),
+                          P<uint32_t>(eng->nv), n_tris, d_draws, d_tg, P<v2>(eng->verts), P<uint32_t>(eng->tris),
+                          P<Edge>(eng->edges) + n_fill_edges, P<uint32_t>(eng->edraw) + n_fill_edges);
+    CK(hipGetLastError());
+
+    // ---- tiles of every draw
+    size_t dn4 = 4ull * (n_draws + 1);
+    if (!ensure(eng, eng->bbox, sizeof(int4) * (n_draws + 1)) || !ensure(eng, eng->dtiles, sizeof(DrawTiles) * (n_draws + 1)) ||
+        !ensure(eng, eng->pt_count, dn4) || !ensure(eng, eng->pt_off, dn4) || !ensure(eng, eng->row_cnt, dn4) ||
+        !ensure(eng, eng->row_off, dn4))
+        return -1;
+    vkh::launch_draw_bbox_init(s, P<int4>(eng->bbox), n_draws);
+    vkh::launch_draw_bbox(s, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), n_edges, P<int4>(eng->bbox));
+    vkh::launch_draw_tiles(s, d_draws, n_draws, d_tg, P<int4>(eng->bbox), T, P<DrawTiles>(eng->dtiles),
+                           P<uint32_t>(eng->pt_count), P<uint32_t>(eng->row_cnt));
+    if (scan_u32(eng, P<uint32_t>(eng->pt_count), n_draws, P<uint32_t>(eng->pt_off)) ||
+        scan_u32(eng, P<uint32_t>(eng->row_cnt), n_draws, P<uint32_t>(eng->row_off)))
+        return -1;
+    if (read_totals(eng, {P<uint32_t>(eng->pt_off) + n_draws, P<uint32_t>(eng->row_off) + n_draws}, tot))
+        return -1;
+    const uint32_t n_ptiles = tot[0], n_rows = tot[1];
+    st.path_tiles = n_ptiles;
+    vkh::launch_set_pt_base(s, P<DrawTiles>(eng->dtiles), P<uint32_t>(eng->pt_off), n_draws);
+    if (!ensure(eng, eng->ptiles, sizeof(PathTile) * (size_t)n_ptiles + 16) ||
+        !ensure(eng, eng->nseg, 4ull * (n_ptiles + 1)) || !ensure(eng, eng->seg_off, 4ull * (n_ptiles + 1)))
+        return -1;
+    if (n_ptiles)
+        CK(hipMemsetAsync(eng->ptiles.p, 0, sizeof(PathTile) * (size_t)n_ptiles, s));
+    vkh::launch_edge_bin(s, false, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), n_edges, P<DrawTiles>(eng->dtiles),
+                         T, P<PathTile>(eng->ptiles), nullptr);
+    if (n_draws)
+        vkh::launch_backdrop_prefix(s, P<DrawTiles>(eng->dtiles), P<uint32_t>(eng->row_off), n_draws, n_rows,
+                                    P<PathTile>(eng->ptiles));
+    CK(hipGetLastError());
+    // segment lists
+    vkh::launch_nseg_out(s, P<PathTile>(eng->ptiles), P<uint32_t>(eng->nseg), n_ptiles);
+    if (scan_u32(eng, P<uint32_t>(eng->nseg), n_ptiles, P<uint32_t>(eng->seg_off)))
+        return -1;
+    if (read_totals(eng, {P<uint32_t>(eng->seg_off) + n_ptiles}, tot))
+        return -1;
+    const uint32_t n_segs = tot[0];
+    st.segments = n_segs;
+    if (!ensure(eng, eng->segs, 4ull * n_segs + 4))
+        return -1;
+    vkh::launch_seg_offsets(s, P<PathTile>(eng->ptiles), P<uint32_t>(eng->seg_off), n_ptiles);
+    vkh::launch_edge_bin(s, true, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), n_edges, P<DrawTiles>(eng->dtiles), T,
+                         P<PathTile>(eng->ptiles), P<uint32_t>(eng->segs));
+    CK(hipGetLastError());
+
+    // ---- coarse: ordered command list per tile
+    if (!ensure(eng, eng->tgts, sizeof(TargetInfo) * (f->n_targets + 1)) ||
+        !ensure(eng, eng->tile_cnt, 4ull * (tile_total + 1)) || !ensure(eng, eng->tile_off, 4ull * (tile_total + 1)))
+        return -1;
+    CK(hipMemsetAsync(eng->tile_cnt.p, 0, 4ull * (tile_total + 1), s));
+    vkh::launch_coarse(s, false, bin_total, d_tg, f->n_targets, d_draws, P<DrawTiles>(eng->dtiles),
+                       P<PathTile>(eng->ptiles), P<uint32_t>(eng->tile_cnt), nullptr, nullptr);
+    if (scan_u32(eng, P<uint32_t>(eng->tile_cnt), tile_total, P<uint32_t>(eng->tile_off)))
+        return -1;
+    if (read_totals(eng, {P<uint32_t>(eng->tile_off) + tile_total}, tot))
+        return -1;
+    const uint32_t n_cmds = tot[0];
+    st.commands = n_cmds;
+    st.tiles = tile_total;
+    if (!ensure(eng, eng->cmds, sizeof(Cmd) * (size_t)n_cmds + 16))
+        return -1;
+    vkh::launch_coarse(s, true, bin_total, d_tg, f->n_targets, d_draws, P<DrawTiles>(eng->dtiles),
+                       P<PathTile>(eng->ptiles), nullptr, P<uint32_t>(eng->tile_off), P<Cmd>(eng->cmds));
+    CK(hipGetLastError());
+
+    // ---- fine: coverage + compositing
+    CK(hipEventRecord(eng->ev0, s));
+    vkh::launch_fine(s, T, tile_total, d_tg, f->n_targets, d_draws, d_grads, P<PathTile>(eng->ptiles),
+                     P<uint32_t>(eng->segs), P<Edge>(eng->edges), n_edges, P<uint32_t>(eng->tile_off),
+                     P<Cmd>(eng->cmds));
+    CK(hipGetLastError());
+    CK(hipEventRecord(eng->ev1, s));
+    CK(hipStreamSynchronize(s));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, eng->ev0, eng->ev1);
+    st.ms_fine = ms;
+    st.ms_total = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    eng->stats = st;
+    return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,615 @@
+// Geometry stage of the vkvg HIP backend: flattening, stroke expansion, edge generation.
+//
+//   flatten      one thread per path item; cubics run the AGG subdivision twice (count, then
+//                emit) with a prefix sum between the passes for output allocation
+//                (src/vkvg_context.c:541-566, src/vkvg_context_internal.c:1313-1461)
+//   stroke       one thread per (stroke draw, subpath): joins, caps and dashes emitted into
+//                vertex / triangle buffers exactly as the reference's caches fill
+//                (src/vkvg_context.c:822-948, src/vkvg_context_internal.c:924-1282)
+//   edges        fills: closed polygon edges of every subpath with more than 2 points;
+//                strokes: 3 edges per triangle, oriented so that coverage count = winding
+#include "vkh_internal.h"
+
+// ---------------------------------------------------------------------------------------- flatten
+__global__ void k_flatten_count(const vkh_item *__restrict__ items, uint32_t n, uint32_t *__restrict__ counts) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    vkh_item it = items[i];
+    uint32_t c = 1;
+    if (it.flags & VKH_ITEM_CUBIC) {
+        Cubic cb{it.p[0], it.p[1], it.p[2], it.p[3], it.p[4], it.p[5], it.p[6], it.p[7]};
+        uint32_t k = 0;
+        agg_flatten(cb, it.tol, [&](v2) { k++; });
+        c = k + ((it.flags & VKH_ITEM_NO_END) ? 0u : 1u);
+    }
+    counts[i] = c;
+}
+
+__global__ void k_flatten_emit(const vkh_item *__restrict__ items, uint32_t n, const uint32_t *__restrict__ offs,
+                               v2 *__restrict__ pts, uint8_t *__restrict__ pflags) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    vkh_item it = items[i];
+    uint32_t o = offs[i];
+    uint8_t f = (it.flags & VKH_ITEM_CURVED) ? 1 : 0;
+    if (!(it.flags & VKH_ITEM_CUBIC)) {
+        pts[o] = V(it.p[0], it.p[1]);
+        pflags[o] = f;
+        return;
+    }
+    Cubic cb{it.p[0], it.p[1], it.p[2], it.p[3], it.p[4], it.p[5], it.p[6], it.p[7]};
+    uint32_t k = o;
+    agg_flatten(cb, it.tol, [&](v2 p) {
+        pts[k] = p;
+        pflags[k] = f;
+        k++;
+    });
+    if (!(it.flags & VKH_ITEM_NO_END)) {
+        pts[k] = V(it.p[6], it.p[7]);
+        pflags[k] = f;
+    }
+}
+
+__global__ void k_subpath_ranges(const vkh_subpath *__restrict__ sp, uint32_t n, const uint32_t *__restrict__ item_off,
+                                 uint32_t *__restrict__ sp_first, uint32_t *__restrict__ sp_count) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t a = item_off[sp[i].first_item];
+    uint32_t b = item_off[sp[i].first_item + sp[i].n_items];
+    sp_first[i] = a;
+    sp_count[i] = b - a;
+}
+
+// ---------------------------------------------------------------------------------------- stroke
+struct CountSink {
+    uint32_t nv = 0, ni = 0;
+    __device__ void addv(v2) { nv++; }
+    __device__ void addt(uint32_t, uint32_t, uint32_t) { ni += 3; }
+    __device__ void patch(uint32_t, bool) {}
+};
+struct EmitSink {
+    v2 *verts;
+    uint32_t *idx;
+    uint32_t nv = 0, ni = 0;
+    __device__ void addv(v2 p) { verts[nv++] = p; }
+    __device__ void addt(uint32_t a, uint32_t b, uint32_t c) {
+        idx[ni] = a;
+        idx[ni + 1] = b;
+        idx[ni + 2] = c;
+        ni += 3;
+    }
+    // closing patch of src/vkvg_context.c:917-932 on the last 6 indices of this subpath
+    __device__ void patch(uint32_t ii, bool inverse) {
+        if (ni < 6)
+            return;
+        uint32_t *ind = idx + ni - 6;
+        if (inverse) {
+            ind[1] = ii + 1;
+            ind[4] = ii + 1;
+            ind[5] = ii;
+        } else {
+            ind[1] = ii;
+            ind[4] = ii;
+            ind[5] = ii + 1;
+        }
+    }
+};
+
+struct StrokeCtx {
+    const v2 *P;
+    const uint8_t *F;
+    uint32_t n;
+    float hw, lhMax, arcStep;
+    int join, cap;
+    const float *dashes;
+    uint32_t dashCount;
+};
+
+// _build_vb_step, src/vkvg_context_internal.c:924-1163
+template <typename S>
+__device__ bool vb_step(const StrokeCtx &c, S &s, uint32_t iL, uint32_t cp, uint32_t iR, bool isCurve) {
+    v2 p0 = c.P[cp];
+    v2 v0 = vsub(p0, c.P[iL]);
+    v2 v1 = vsub(c.P[iR], p0);
+    float lv0 = vlen(v0), lv1 = vlen(v1);
+    if (lv0 < VKH_FLT_EPSILON || lv1 < VKH_FLT_EPSILON)
+        return false;
+    v2 v0n = vdiv(v0, lv0), v1n = vdiv(v1, lv1);
+    float dot = vdot(v0n, v1n);
+    float det = v0n.x * v1n.y - v0n.y * v1n.x;
+    if (equf(dot, 1.0f))
+        return false;
+    uint32_t idx = s.nv;
+    if (equf(dot, -1.0f)) {
+        v2 vPerp = vmul(vperp(v0n), c.hw);
+        s.addv(vadd(p0, vPerp));
+        s.addv(vsub(p0, vPerp));
+        s.addt(idx, idx + 1, idx + 2);
+        s.addt(idx, idx + 2, idx + 3);
+        return true;
+    }
+    v2 bisec_n = vnorm(vadd(v0n, v1n));
+    float alpha = d_acosf(dot);
+    if (det < 0)
+        alpha = -alpha;
+    float halfAlpha = alpha / 2.f;
+    float cosHalfAlpha = d_cosf(halfAlpha);
+    float lh = c.hw / cosHalfAlpha;
+    v2 bisec_n_perp = vperp(bisec_n);
+    float rlh = lh;
+    if (dot < 0.f)
+        rlh = fminf(rlh, fminf(lv0, lv1));
+    v2 bisec = vmul(bisec_n_perp, rlh);
+    v2 in_pos, out_pos;
+    if (rlh < lh) {
+        v2 vnPerp = (lv0 < lv1) ? vperp(v1n) : vperp(v0n);
+        v2 vHwPerp = vmul(vnPerp, c.hw);
+        float lbc = cosHalfAlpha * rlh;
+        if (det < 0.f) {
+            in_pos = vadd(vadd(vmul(vnPerp, -lbc), vadd(p0, bisec)), vHwPerp);
+            out_pos = vsub(p0, vmul(bisec_n_perp, lh));
+        } else {
+            in_pos = vsub(vadd(vmul(vnPerp, lbc), vsub(p0, bisec)), vHwPerp);
+            out_pos = vadd(p0, vmul(bisec_n_perp, lh));
+        }
+    } else {
+        if (det < 0.0f) {
+            in_pos = vadd(p0, bisec);
+            out_pos = vsub(p0, bisec);
+        } else {
+            in_pos = vsub(p0, bisec);
+            out_pos = vadd(p0, bisec);
+        }
+    }
+    int join = c.join;
+    if (isCurve)
+        join = (dot < 0.8f) ? 1 : 0;
+    if (join == 0) {
+        if (lh > c.lhMax) {
+            float x = (lh - c.lhMax) * cosHalfAlpha;
+            v2 bisecPerp = vmul(bisec_n, x);
+            bisec = vmul(bisec_n_perp, c.lhMax);
+            if (det < 0) {
+                s.addv(in_pos);
+                v2 p = vsub(p0, bisec);
+                s.addv(vsub(p, bisecPerp));
+                s.addv(vadd(p, bisecPerp));
+                s.addt(idx, idx + 2, idx + 1);
+                s.addt(idx + 2, idx + 4, idx);
+                s.addt(idx, idx + 3, idx + 4);
+                return true;
+            } else {
+                v2 p = vadd(p0, bisec);
+                s.addv(vsub(p, bisecPerp));
+                s.addv(in_pos);
+                s.addv(vadd(p, bisecPerp));
+                s.addt(idx, idx + 2, idx + 1);
+                s.addt(idx + 2, idx + 3, idx + 1);
+                s.addt(idx + 1, idx + 3, idx + 4);
+                return false;
+            }
+        } else {
+            if (det < 0) {
+                s.addv(in_pos);
+                s.addv(out_pos);
+            } else {
+                s.addv(out_pos);
+                s.addv(in_pos);
+            }
+            s.addt(idx, idx + 2, idx + 1);
+            s.addt(idx + 1, idx + 2, idx + 3);
+            return false;
+        }
+    }
+    v2 vp = vperp(v0n);
+    if (det < 0) {
+        s.addv((dot < 0 && rlh < lh) ? in_pos : vadd(p0, bisec));
+        s.addv(vsub(p0, vmul(vp, c.hw)));
+    } else {
+        s.addv(vadd(p0, vmul(vp, c.hw)));
+        s.addv((dot < 0 && rlh < lh) ? in_pos : vsub(p0, bisec));
+    }
+    if (join == 2) {
+        if (det < 0) {
+            s.addt(idx, idx + 2, idx + 1);
+            s.addt(idx + 2, idx + 4, idx + 0);
+            s.addt(idx, idx + 3, idx + 4);
+        } else {
+            s.addt(idx, idx + 2, idx + 1);
+            s.addt(idx + 2, idx + 3, idx + 1);
+            s.addt(idx + 1, idx + 3, idx + 4);
+        }
+    } else if (join == 1) {
+        float a = d_acosf(vp.x);
+        if (vp.y < 0)
+            a = -a;
+        if (det < 0) {
+            a += VKH_PIF;
+            float a1 = a + alpha;
+            a -= c.arcStep;
+            while (a > a1) {
+                s.addv(V(d_cosf(a) * c.hw + p0.x, d_sinf(a) * c.hw + p0.y));
+                a -= c.arcStep;
+            }
+        } else {
+            float a1 = a + alpha;
+            a += c.arcStep;
+            while (a < a1) {
+                s.addv(V(d_cosf(a) * c.hw + p0.x, d_sinf(a) * c.hw + p0.y));
+                a += c.arcStep;
+            }
+        }
+        uint32_t p0Idx = s.nv;
+        s.addt(idx, idx + 2, idx + 1);
+        if (det < 0) {
+            for (uint32_t p = idx + 2; p < p0Idx; p++)
+                s.addt(p, p + 1, idx);
+            s.addt(p0Idx, p0Idx + 2, idx);
+            s.addt(idx, p0Idx + 1, p0Idx + 2);
+        } else {
+            for (uint32_t p = idx + 2; p < p0Idx; p++)
+                s.addt(p, p + 1, idx + 1);
+            s.addt(p0Idx, p0Idx + 1, idx + 1);
+            s.addt(idx + 1, p0Idx + 1, p0Idx + 2);
+        }
+    }
+    vp = vmul(vperp(v1n), c.hw);
+    s.addv(det < 0 ? vsub(p0, vp) : vadd(p0, vp));
+    return (det < 0);
+}
+
+// _draw_stoke_cap, src/vkvg_context_internal.c:1165-1239
+template <typename S>
+__device__ void stroke_cap(const StrokeCtx &c, S &s, v2 p0, v2 n, bool isStart) {
+    uint32_t firstIdx = s.nv;
+    v2 vhw = vmul(n, c.hw);
+    if (isStart) {
+        if (c.cap == 2)
+            p0 = vsub(p0, vhw);
+        vhw = vperp(vhw);
+        if (c.cap == 1) {
+            float a = d_acosf(n.x) + VKH_PIF_2;
+            if (n.y < 0)
+                a = VKH_PIF - a;
+            float a1 = a + VKH_PIF;
+            a += c.arcStep;
+            while (a < a1) {
+                s.addv(V(d_cosf(a) * c.hw + p0.x, d_sinf(a) * c.hw + p0.y));
+                a += c.arcStep;
+            }
+            uint32_t p0Idx = s.nv;
+            for (uint32_t p = firstIdx; p < p0Idx; p++)
+                s.addt(p0Idx + 1, p, p + 1);
+            firstIdx = p0Idx;
+        }
+        s.addv(vadd(p0, vhw));
+        s.addv(vsub(p0, vhw));
+        s.addt(firstIdx, firstIdx + 2, firstIdx + 1);
+        s.addt(firstIdx + 1, firstIdx + 2, firstIdx + 3);
+    } else {
+        if (c.cap == 2)
+            p0 = vadd(p0, vhw);
+        vhw = vperp(vhw);
+        s.addv(vadd(p0, vhw));
+        s.addv(vsub(p0, vhw));
+        firstIdx = s.nv;
+        if (c.cap == 1) {
+            float a = d_acosf(n.x) + VKH_PIF_2;
+            if (n.y < 0)
+                a = VKH_PIF - a;
+            float a1 = a - VKH_PIF;
+            a -= c.arcStep;
+            while (a > a1) {
+                s.addv(V(d_cosf(a) * c.hw + p0.x, d_sinf(a) * c.hw + p0.y));
+                a -= c.arcStep;
+            }
+            uint32_t p0Idx = s.nv - 1;
+            for (uint32_t p = firstIdx - 1; p < p0Idx; p++)
+                s.addt(p + 1, p, firstIdx - 2);
+        }
+    }
+}
+
+struct DashState {
+    bool dashOn;
+    uint32_t curDash;
+    float curDashOffset, totDashLength;
+    v2 normal;
+};
+
+// _draw_dashed_segment, src/vkvg_context_internal.c:1240-1264
+template <typename S>
+__device__ void dashed_segment(const StrokeCtx &c, S &s, DashState &dc, uint32_t iL, uint32_t cp, uint32_t iR,
+                               bool isCurve) {
+    v2 p = c.P[cp];
+    v2 pR = c.P[iR];
+    if (!dc.dashOn)
+        vb_step(c, s, iL, cp, iR, isCurve);
+    v2 d = vsub(pR, p);
+    dc.normal = vnorm(d);
+    float segLen = vlen(d);
+    while (dc.curDashOffset < segLen) {
+        v2 p0 = vadd(p, vmul(dc.normal, dc.curDashOffset));
+        stroke_cap(c, s, p0, dc.normal, dc.dashOn);
+        dc.dashOn = !dc.dashOn;
+        dc.curDashOffset += c.dashes[dc.curDash];
+        if (++dc.curDash == c.dashCount)
+            dc.curDash = 0;
+    }
+    dc.curDashOffset -= segLen;
+    dc.curDashOffset = fmodf(dc.curDashOffset, dc.totDashLength);
+}
+
+// one subpath of _stroke_preserve (src/vkvg_context.c:836-945); the join at point i uses the
+// curve flag of the segment holding point i, the closing join never does.
+template <typename S>
+__device__ void stroke_subpath(const StrokeCtx &c, bool closed, float dashOffset, S &s) {
+    uint32_t n = c.n;
+    if (n < 2)
+        return;
+    uint32_t last = n - 1;
+    uint32_t firstIdx = s.nv;
+    uint32_t iL, iR = 0, cp = 0;
+    DashState dc;
+    dc.dashOn = false;
+    dc.curDash = 0;
+    dc.curDashOffset = 0;
+    dc.totDashLength = 0;
+    dc.normal = V(0, 0);
+    bool dashed = c.dashCount > 0;
+    if (dashed) {
+        dc.dashOn = true;
+        for (uint32_t i = 0; i < c.dashCount; i++)
+            dc.totDashLength += c.dashes[i];
+        dc.curDashOffset = fmodf(dashOffset, dc.totDashLength);
+        iL = last;
+    } else if (closed) {
+        iL = last;
+    } else {
+        v2 a = c.P[0], b = c.P[1];
+        v2 d = V(b.x - a.x, b.y - a.y);
+        float md = sqrtf(d.x * d.x + d.y * d.y);
+        d.x /= md;
+        d.y /= md;
+        stroke_cap(c, s, a, d, true);
+        iL = 0;
+        cp = 1;
+    }
+    while (cp < last) {
+        iR = cp + 1;
+        bool curved = c.F[cp] != 0;
+        if (dashed)
+            dashed_segment(c, s, dc, iL, cp, iR, curved);
+        else
+            vb_step(c, s, iL, cp, iR, curved);
+        iL = cp++;
+    }
+    if (dashed) {
+        if (closed) {
+            iR = 0;
+            dashed_segment(c, s, dc, iL, cp, iR, false);
+        }
+        if (!dc.dashOn) {
+            int32_t prevDash = (int32_t)dc.curDash - 1;
+            if (prevDash < 0)
+                dc.curDash = c.dashCount - 1;
+            float dprev = c.dashes[prevDash < 0 ? (int32_t)c.dashCount - 1 : prevDash];
+            float m = fminf(dprev - dc.curDashOffset, c.dashes[dc.curDash]);
+            v2 p = vsub(c.P[iR], vmul(dc.normal, m));
+            stroke_cap(c, s, p, dc.normal, false);
+        }
+    } else if (closed) {
+        iR = 0;
+        bool inverse = vb_step(c, s, iL, cp, iR, false);
+        s.patch(firstIdx, inverse);
+    } else {
+        v2 a = c.P[cp - 1], b = c.P[cp];
+        v2 d = V(b.x - a.x, b.y - a.y);
+        float md = sqrtf(d.x * d.x + d.y * d.y);
+        d.x /= md;
+        d.y /= md;
+        stroke_cap(c, s, b, d, false);
+    }
+}
+
+__device__ __forceinline__ StrokeCtx make_stroke_ctx(const vkh_stroke_style &st, const float *dashes, const v2 *pts,
+                                                     const uint8_t *pf, uint32_t first, uint32_t n) {
+    StrokeCtx c;
+    c.P = pts + first;
+    c.F = pf + first;
+    c.n = n;
+    c.hw = st.hw;
+    c.lhMax = st.lhMax;
+    c.arcStep = st.arcStep;
+    c.join = (int)st.join;
+    c.cap = (int)st.cap;
+    c.dashes = dashes + st.dash_first;
+    c.dashCount = st.dash_count;
+    return c;
+}
+
+// per unit: fill -> number of polygon edges; stroke -> vertex and triangle counts
+__global__ void k_unit_count(const Unit *__restrict__ units, uint32_t n_units, const vkh_draw *__restrict__ draws,
+                             const vkh_subpath *__restrict__ sps, const uint32_t *__restrict__ sp_first,
+                             const uint32_t *__restrict__ sp_count, const v2 *__restrict__ pts,
+                             const uint8_t *__restrict__ pf, const vkh_stroke_style *__restrict__ styles,
+                             const float *__restrict__ dashes, uint32_t *__restrict__ fill_edges,
+                             uint32_t *__restrict__ nverts, uint32_t *__restrict__ ntris) {
+    uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n_units)
+        return;
+    Unit un = units[u];
+    const vkh_draw &d = draws[un.draw];
+    uint32_t n = sp_count[un.subpath];
+    if (d.kind == VKH_DRAW_STROKE) {
+        StrokeCtx c = make_stroke_ctx(styles[d.style], dashes, pts, pf, sp_first[un.subpath], n);
+        CountSink s;
+        stroke_subpath(c, (sps[un.subpath].flags & VKH_SP_CLOSED) != 0, styles[d.style].dash_offset, s);
+        fill_edges[u] = 0;
+        nverts[u] = s.nv;
+        ntris[u] = s.ni / 3;
+    } else {
+        fill_edges[u] = n > 2 ? n : 0;
+        nverts[u] = 0;
+        ntris[u] = 0;
+    }
+}
+
+__global__ void k_stroke_emit(const Unit *__restrict__ units, uint32_t n_units, const vkh_draw *__restrict__ draws,
+                              const vkh_subpath *__restrict__ sps, const uint32_t *__restrict__ sp_first,
+                              const uint32_t *__restrict__ sp_count, const v2 *__restrict__ pts,
+                              const uint8_t *__restrict__ pf, const vkh_stroke_style *__restrict__ styles,
+                              const float *__restrict__ dashes, const uint32_t *__restrict__ vert_off,
+                              const uint32_t *__restrict__ tri_off, v2 *__restrict__ verts,
+                              uint32_t *__restrict__ tris) {
+    uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n_units)
+        return;
+    Unit un = units[u];
+    const vkh_draw &d = draws[un.draw];
+    if (d.kind != VKH_DRAW_STROKE)
+        return;
+    StrokeCtx c = make_stroke_ctx(styles[d.style], dashes, pts, pf, sp_first[un.subpath], sp_count[un.subpath]);
+    EmitSink s;
+    s.verts = verts + vert_off[u];
+    s.idx = tris + 3 * (size_t)tri_off[u];
+    stroke_subpath(c, (sps[un.subpath].flags & VKH_SP_CLOSED) != 0, styles[d.style].dash_offset, s);
+}
+
+// ---------------------------------------------------------------------------------------- edges
+__device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *a, uint32_t n, uint32_t v) {
+    // first index i in [0,n) with a[i] > v (a non-decreasing)
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] <= v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_fill_edges(const Unit *__restrict__ units, uint32_t n_units, const uint32_t *__restrict__ fill_off,
+                             uint32_t total, const vkh_draw *__restrict__ draws, const TargetInfo *__restrict__ tgts,
+                             const uint32_t *__restrict__ sp_first, const uint32_t *__restrict__ sp_count,
+                             const v2 *__restrict__ pts, Edge *__restrict__ edges, uint32_t *__restrict__ edraw) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total)
+        return;
+    uint32_t u = upper_bound_u32(fill_off, n_units, i) - 1;
+    Unit un = units[u];
+    const vkh_draw &d = draws[un.draw];
+    uint32_t j = i - fill_off[u];
+    uint32_t n = sp_count[un.subpath];
+    uint32_t f = sp_first[un.subpath];
+    v2 a = pts[f + j], b = pts[f + (j + 1 == n ? 0 : j + 1)];
+    float sw = (float)tgts[d.target].width, sh = (float)tgts[d.target].height;
+    Edge e;
+    snap_vertex(a.x, a.y, d.mat, sw, sh, e.x0, e.y0);
+    snap_vertex(b.x, b.y, d.mat, sw, sh, e.x1, e.y1);
+    bool bad = e.x0 == VKH_INVALID_FIX || e.x1 == VKH_INVALID_FIX || (e.x0 == e.x1 && e.y0 == e.y1);
+    edges[i] = e;
+    edraw[i] = bad ? VKH_NO_DRAW : un.draw;
+}
+
+// stroke triangles -> 3 edges each, oriented so every triangle winds +1 (ray to -x, +y edges count +1)
+__global__ void k_tri_edges(const Unit *__restrict__ units, uint32_t n_units, const uint32_t *__restrict__ tri_off,
+                            const uint32_t *__restrict__ vert_off, const uint32_t *__restrict__ nverts,
+                            uint32_t total_tris, const vkh_draw *__restrict__ draws,
+                            const TargetInfo *__restrict__ tgts, const v2 *__restrict__ verts,
+                            const uint32_t *__restrict__ tris, Edge *__restrict__ edges, uint32_t *__restrict__ edraw) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total_tris)
+        return;
+    uint32_t u = upper_bound_u32(tri_off, n_units, t) - 1;
+    Unit un = units[u];
+    const vkh_draw &d = draws[un.draw];
+    uint32_t nv = nverts[u];
+    const v2 *V0 = verts + vert_off[u];
+    uint32_t i0 = tris[3 * (size_t)t], i1 = tris[3 * (size_t)t + 1], i2 = tris[3 * (size_t)t + 2];
+    Edge *out = edges + 3 * (size_t)t;
+    uint32_t *od = edraw + 3 * (size_t)t;
+    bool bad = i0 >= nv || i1 >= nv || i2 >= nv;
+    int32_t X[3] = {0, 0, 0}, Y[3] = {0, 0, 0};
+    if (!bad) {
+        float sw = (float)tgts[d.target].width, sh = (float)tgts[d.target].height;
+        snap_vertex(V0[i0].x, V0[i0].y, d.mat, sw, sh, X[0], Y[0]);
+        snap_vertex(V0[i1].x, V0[i1].y, d.mat, sw, sh, X[1], Y[1]);
+        snap_vertex(V0[i2].x, V0[i2].y, d.mat, sw, sh, X[2], Y[2]);
+        bad = X[0] == VKH_INVALID_FIX || X[1] == VKH_INVALID_FIX || X[2] == VKH_INVALID_FIX;
+    }
+    int64_t a2 = 0;
+    if (!bad) {
+        a2 = (int64_t)(X[1] - X[0]) * (Y[2] - Y[0]) - (int64_t)(Y[1] - Y[0]) * (X[2] - X[0]);
+        bad = (a2 == 0);
+    }
+    if (bad) {
+        for (int k = 0; k < 3; k++) {
+            out[k] = Edge{0, 0, 0, 0};
+            od[k] = VKH_NO_DRAW;
+        }
+        return;
+    }
+    if (a2 > 0) { // positive area winds -1: traverse 0 -> 2 -> 1
+        int32_t tx = X[1], ty = Y[1];
+        X[1] = X[2]; Y[1] = Y[2];
+        X[2] = tx; Y[2] = ty;
+    }
+    for (int k = 0; k < 3; k++) {
+        int k1 = k == 2 ? 0 : k + 1;
+        out[k] = Edge{X[k], Y[k], X[k1], Y[k1]};
+        od[k] = un.draw;
+    }
+}
+
+// ---------------------------------------------------------------------------------------- launchers
+namespace vkh {
+void launch_flatten_count(hipStream_t s, const vkh_item *items, uint32_t n, uint32_t *counts) {
+    if (n)
+        hipLaunchKernelGGL(k_flatten_count, dim3((n + 255) / 256), dim3(256), 0, s, items, n, counts);
+}
+void launch_flatten_emit(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *offs, v2 *pts,
+                         uint8_t *pf) {
+    if (n)
+        hipLaunchKernelGGL(k_flatten_emit, dim3((n + 255) / 256), dim3(256), 0, s, items, n, offs, pts, pf);
+}
+void launch_subpath_ranges(hipStream_t s, const vkh_subpath *sp, uint32_t n, const uint32_t *item_off,
+                           uint32_t *first, uint32_t *count) {
+    if (n)
+        hipLaunchKernelGGL(k_subpath_ranges, dim3((n + 255) / 256), dim3(256), 0, s, sp, n, item_off, first, count);
+}
+void launch_unit_count(hipStream_t s, const Unit *units, uint32_t n_units, const vkh_draw *draws,
+                       const vkh_subpath *sps, const uint32_t *spf, const uint32_t *spc, const v2 *pts,
+                       const uint8_t *pf, const vkh_stroke_style *styles, const float *dashes, uint32_t *fe,
+                       uint32_t *nv, uint32_t *nt) {
+    if (n_units)
+        hipLaunchKernelGGL(k_unit_count, dim3((n_units + 63) / 64), dim3(64), 0, s, units, n_units, draws, sps, spf,
+                           spc, pts, pf, styles, dashes, fe, nv, nt);
+}
+void launch_stroke_emit(hipStream_t s, const Unit *units, uint32_t n_units, const vkh_draw *draws,
+                        const vkh_subpath *sps, const uint32_t *spf, const uint32_t *spc, const v2 *pts,
+                        const uint8_t *pf, const vkh_stroke_style *styles, const float *dashes,
+                        const uint32_t *voff, const uint32_t *toff, v2 *verts, uint32_t *tris) {
+    if (n_units)
+        hipLaunchKernelGGL(k_stroke_emit, dim3((n_units + 63) / 64), dim3(64), 0, s, units, n_units, draws, sps, spf,
+                           spc, pts, pf, styles, dashes, voff, toff, verts, tris);
+}
+void launch_fill_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *fill_off, uint32_t total,
+                       const vkh_draw *draws, const TargetInfo *tg, const uint32_t *spf, const uint32_t *spc,
+                       const v2 *pts, Edge *edges, uint32_t *edraw) {
+    if (total)
+        hipLaunchKernelGGL(k_fill_edges, dim3((total + 255) / 256), dim3(256), 0, s, units, n_units, fill_off, total,
+                           draws, tg, spf, spc, pts, edges, edraw);
+}
+void launch_tri_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *tri_off,
+                      const uint32_t *vert_off, const uint32_t *nverts, uint32_t total_tris, const vkh_draw *draws,
+                      const TargetInfo *tg, const v2 *verts, const uint32_t *tris, Edge *edges, uint32_t *edraw) {
+    if (total_tris)
+        hipLaunchKernelGGL(k_tri_edges, dim3((total_tris + 255) / 256), dim3(256), 0, s, units, n_units, tri_off,
+                           vert_off, nverts, total_tris, draws, tg, verts, tris, edges, edraw);
+}
+} // namespace vkh

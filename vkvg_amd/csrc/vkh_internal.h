@@ -1,0 +1,89 @@
+// Internal types shared by the vkvg HIP backend translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vkh_device.h"
+
+#define VKH_NO_DRAW 0xFFFFFFFFu
+#define VKH_BIN_TILES 16 // a coarse bin is 16 x 16 tiles
+
+// one (draw, subpath) pair of the frame; fills and strokes are expanded per unit
+struct Unit {
+    uint32_t draw;
+    uint32_t subpath;
+};
+
+// per render target, on the device
+struct TargetInfo {
+    uint32_t width, height;
+    uint32_t tilesW, tilesH;
+    uint32_t tile_base;  // first global tile index
+    uint32_t binsW, binsH;
+    uint32_t bin_base;   // first global bin index
+    uint32_t draw_begin, draw_end;
+    uint8_t *rgba;
+    uint32_t *sig;
+};
+
+// per draw, computed on the device
+struct DrawTiles {
+    int32_t bx0, by0, bx1, by1; // inclusive tile bbox within the target, empty when bx0 > bx1
+    uint32_t pt_base;           // first path tile
+    uint32_t bw;                // bbox width in tiles
+};
+
+// per path tile (draw x tile in its bbox)
+struct PathTile {
+    int32_t backdrop;  // winding at the tile's first sample (after the row prefix sum)
+    uint32_t nseg;     // edges touching the tile
+    uint32_t seg_off;  // into the segment index array
+    uint32_t cursor;   // scatter cursor
+};
+
+struct Cmd {
+    uint32_t draw;
+    uint32_t pt; // path tile, VKH_NO_DRAW for paint / clear
+};
+
+namespace vkh {
+void launch_flatten_count(hipStream_t s, const vkh_item *items, uint32_t n, uint32_t *counts);
+void launch_flatten_emit(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *offs, v2 *pts, uint8_t *pf);
+void launch_subpath_ranges(hipStream_t s, const vkh_subpath *sp, uint32_t n, const uint32_t *item_off,
+                           uint32_t *first, uint32_t *count);
+void launch_unit_count(hipStream_t s, const Unit *units, uint32_t n_units, const vkh_draw *draws,
+                       const vkh_subpath *sps, const uint32_t *spf, const uint32_t *spc, const v2 *pts,
+                       const uint8_t *pf, const vkh_stroke_style *styles, const float *dashes, uint32_t *fe,
+                       uint32_t *nv, uint32_t *nt);
+void launch_stroke_emit(hipStream_t s, const Unit *units, uint32_t n_units, const vkh_draw *draws,
+                        const vkh_subpath *sps, const uint32_t *spf, const uint32_t *spc, const v2 *pts,
+                        const uint8_t *pf, const vkh_stroke_style *styles, const float *dashes,
+                        const uint32_t *voff, const uint32_t *toff, v2 *verts, uint32_t *tris);
+void launch_fill_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *fill_off, uint32_t total,
+                       const vkh_draw *draws, const TargetInfo *tg, const uint32_t *spf, const uint32_t *spc,
+                       const v2 *pts, Edge *edges, uint32_t *edraw);
+void launch_tri_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *tri_off,
+                      const uint32_t *vert_off, const uint32_t *nverts, uint32_t total_tris, const vkh_draw *draws,
+                      const TargetInfo *tg, const v2 *verts, const uint32_t *tris, Edge *edges, uint32_t *edraw);
+
+// raster stage (vkh_raster.hip)
+void launch_draw_bbox_init(hipStream_t s, int4 *bbox, uint32_t n_draws);
+void launch_draw_bbox(hipStream_t s, const Edge *edges, const uint32_t *edraw, uint32_t n_edges, int4 *bbox);
+void launch_draw_tiles(hipStream_t s, const vkh_draw *draws, uint32_t n_draws, const TargetInfo *tg,
+                       const int4 *bbox, uint32_t tile, DrawTiles *dt, uint32_t *pt_count, uint32_t *row_count);
+void launch_nseg_out(hipStream_t s, const PathTile *pts, uint32_t *nseg, uint32_t n);
+void launch_set_pt_base(hipStream_t s, DrawTiles *dt, const uint32_t *pt_off, uint32_t n_draws);
+void launch_edge_bin(hipStream_t s, bool scatter, const Edge *edges, const uint32_t *edraw, uint32_t n_edges,
+                     const DrawTiles *dt, uint32_t tile, PathTile *pts, uint32_t *segs);
+void launch_seg_offsets(hipStream_t s, PathTile *pts, const uint32_t *seg_off, uint32_t n);
+void launch_backdrop_prefix(hipStream_t s, const DrawTiles *dt, const uint32_t *row_off, uint32_t n_draws,
+                            uint32_t total_rows, PathTile *pts);
+void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, const TargetInfo *tg, uint32_t n_targets,
+                   const vkh_draw *draws, const DrawTiles *dt, const PathTile *pts, uint32_t *tile_count,
+                   const uint32_t *tile_off, Cmd *cmds);
+void launch_fine(hipStream_t s, uint32_t tile, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
+                 const vkh_draw *draws, const vkh_gradient *grads, const PathTile *pts, const uint32_t *segs,
+                 const Edge *edges, uint32_t n_edges, const uint32_t *tile_off, const Cmd *cmds);
+void launch_unpremultiply(hipStream_t s, const uint8_t *src, uint8_t *dst, size_t npix);
+} // namespace vkh

@@ -1,0 +1,643 @@
+// Raster stage of the vkvg HIP backend: tile binning, coarse command lists, fine raster.
+//
+// Coverage semantics (replaces Vulkan raster + stencil + blend of src/vkvg_device_internal.c:182-401):
+// for every draw the winding w of its oriented edges is evaluated exactly, in 24.8 fixed
+// point, at each pixel centre perturbed by (+eps, +eps^2) (= top-left tie rule), and the
+// draw's source is composited k times with k = (w != 0) for non-zero fills, (w & 1) for
+// even-odd fills, w for strokes (one blend per covering triangle) and 1 for paint.
+//
+// Per tile (T x T pixels) and draw, with (X0, Y0) the tile's first pixel:
+//   w(c, r) = W(tile) + V(r) + H(c, r)
+//   W  winding at (X0, Y0): row-wise prefix over tiles of the crossings of sample row Y0
+//   V  signed crossings of the column X0 between rows Y0 and r (edges inside the tile)
+//   H  signed crossings of row r between columns X0 and c (edges inside the tile)
+// so only the edges touching a tile are read when rasterizing it.
+#include "vkh_internal.h"
+
+#include <hip/hip_runtime.h>
+
+__device__ __forceinline__ int32_t floor_div32(int32_t a, int32_t b) { // b > 0
+    int32_t q = a / b;
+    if ((a % b != 0) && (a < 0))
+        q--;
+    return q;
+}
+
+// ----------------------------------------------------------------------------------- draw bbox
+__global__ void k_bbox_init(int4 *bbox, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        bbox[i] = make_int4(INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN);
+}
+
+__global__ void k_draw_bbox(const Edge *__restrict__ edges, const uint32_t *__restrict__ edraw, uint32_t n,
+                            int4 *__restrict__ bbox) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t d = i < n ? edraw[i] : VKH_NO_DRAW;
+    int32_t x0 = INT32_MAX, y0 = INT32_MAX, x1 = INT32_MIN, y1 = INT32_MIN;
+    if (d != VKH_NO_DRAW) {
+        Edge e = edges[i];
+        x0 = min(e.x0, e.x1);
+        x1 = max(e.x0, e.x1);
+        y0 = min(e.y0, e.y1);
+        y1 = max(e.y0, e.y1);
+    }
+    // wave-uniform fast path: one set of atomics per wave when all lanes share a draw
+    uint32_t d0 = __shfl(d, 0);
+    bool uniform = __all(d == d0);
+    if (uniform) {
+        if (d0 == VKH_NO_DRAW)
+            return;
+        for (int o = 32; o > 0; o >>= 1) {
+            x0 = min(x0, __shfl_xor(x0, o));
+            y0 = min(y0, __shfl_xor(y0, o));
+            x1 = max(x1, __shfl_xor(x1, o));
+            y1 = max(y1, __shfl_xor(y1, o));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMin(&bbox[d0].x, x0);
+            atomicMin(&bbox[d0].y, y0);
+            atomicMax(&bbox[d0].z, x1);
+            atomicMax(&bbox[d0].w, y1);
+        }
+        return;
+    }
+    if (d == VKH_NO_DRAW)
+        return;
+    atomicMin(&bbox[d].x, x0);
+    atomicMin(&bbox[d].y, y0);
+    atomicMax(&bbox[d].z, x1);
+    atomicMax(&bbox[d].w, y1);
+}
+
+__global__ void k_draw_tiles(const vkh_draw *__restrict__ draws, uint32_t n, const TargetInfo *__restrict__ tg,
+                             const int4 *__restrict__ bbox, uint32_t T, DrawTiles *__restrict__ dt,
+                             uint32_t *__restrict__ pt_count, uint32_t *__restrict__ row_count) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const vkh_draw &d = draws[i];
+    const TargetInfo &t = tg[d.target];
+    DrawTiles o;
+    o.pt_base = 0;
+    if (d.kind == VKH_DRAW_PAINT || d.kind == VKH_DRAW_CLEAR) {
+        o.bx0 = 0;
+        o.by0 = 0;
+        o.bx1 = (int32_t)t.tilesW - 1;
+        o.by1 = (int32_t)t.tilesH - 1;
+        o.bw = 0;
+        dt[i] = o;
+        pt_count[i] = 0;
+        row_count[i] = 0;
+        return;
+    }
+    int4 b = bbox[i];
+    int32_t span = 256 * (int32_t)T;
+    if (b.x > b.z) {
+        o.bx0 = o.by0 = 0;
+        o.bx1 = o.by1 = -1;
+    } else {
+        o.bx0 = max(floor_div32(b.x, span), 0);
+        o.by0 = max(floor_div32(b.y, span), 0);
+        o.bx1 = min(floor_div32(b.z, span), (int32_t)t.tilesW - 1);
+        o.by1 = min(floor_div32(b.w, span), (int32_t)t.tilesH - 1);
+    }
+    if (o.bx0 > o.bx1 || o.by0 > o.by1) {
+        o.bx0 = o.by0 = 0;
+        o.bx1 = o.by1 = -1;
+        o.bw = 0;
+        pt_count[i] = 0;
+        row_count[i] = 0;
+    } else {
+        o.bw = (uint32_t)(o.bx1 - o.bx0 + 1);
+        pt_count[i] = o.bw * (uint32_t)(o.by1 - o.by0 + 1);
+        row_count[i] = (uint32_t)(o.by1 - o.by0 + 1);
+    }
+    dt[i] = o;
+}
+
+__global__ void k_set_pt_base(DrawTiles *dt, const uint32_t *pt_off, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        dt[i].pt_base = pt_off[i];
+}
+
+// ----------------------------------------------------------------------------------- edge binning
+// Enumerates, for one edge, the tiles of its draw's bbox that the edge may touch (per tile
+// row, the edge's x-extent inside the row's slab, widened by one pixel). Count pass: bumps
+// the tiles' segment counts and adds the edge's crossing of each tile row's first sample row
+// into the backdrop delta of the first tile whose first column lies right of the crossing.
+// Scatter pass: writes the edge index into each touched tile's segment list.
+__global__ void k_edge_bin(bool scatter, const Edge *__restrict__ edges, const uint32_t *__restrict__ edraw,
+                           uint32_t n, const DrawTiles *__restrict__ dts, uint32_t T, PathTile *__restrict__ pts,
+                           uint32_t *__restrict__ segs) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t d = edraw[i];
+    if (d == VKH_NO_DRAW)
+        return;
+    DrawTiles dt = dts[d];
+    if (dt.bx0 > dt.bx1)
+        return;
+    Edge e = edges[i];
+    int32_t span = 256 * (int32_t)T;
+    int32_t ymin = min(e.y0, e.y1), ymax = max(e.y0, e.y1);
+    int32_t ty0 = max(floor_div32(ymin, span), dt.by0);
+    int32_t ty1 = min(floor_div32(ymax, span), dt.by1);
+    bool horiz = e.y0 == e.y1;
+    double invdy = horiz ? 0.0 : 1.0 / ((double)e.y1 - (double)e.y0);
+    int dir = e.y0 < e.y1 ? 1 : -1;
+    int32_t xt = e.y0 < e.y1 ? e.x0 : e.x1, yt = ymin;
+    int32_t xb = e.y0 < e.y1 ? e.x1 : e.x0, yb = ymax;
+    for (int32_t ty = ty0; ty <= ty1; ty++) {
+        int32_t xlo, xhi;
+        if (horiz) {
+            xlo = min(e.x0, e.x1);
+            xhi = max(e.x0, e.x1);
+        } else {
+            double ya = fmax((double)ty * span, (double)ymin);
+            double yb2 = fmin((double)(ty + 1) * span, (double)ymax);
+            double xa = (double)e.x0 + ((double)e.x1 - e.x0) * ((ya - e.y0) * invdy);
+            double xc = (double)e.x0 + ((double)e.x1 - e.x0) * ((yb2 - e.y0) * invdy);
+            xlo = (int32_t)floor(fmin(xa, xc)) - 256;
+            xhi = (int32_t)ceil(fmax(xa, xc)) + 256;
+        }
+        int32_t tx0 = max(floor_div32(xlo, span), dt.bx0);
+        int32_t tx1 = min(floor_div32(xhi, span), dt.bx1);
+        uint32_t rowbase = dt.pt_base + (uint32_t)(ty - dt.by0) * dt.bw;
+        for (int32_t tx = tx0; tx <= tx1; tx++) {
+            PathTile *p = &pts[rowbase + (uint32_t)(tx - dt.bx0)];
+            if (!scatter) {
+                atomicAdd(&p->nseg, 1u);
+            } else {
+                uint32_t pos = atomicAdd(&p->cursor, 1u);
+                if (pos < p->nseg) // the count pass enumerated the same tiles
+                    segs[p->seg_off + pos] = i;
+            }
+        }
+        if (!scatter && !horiz) {
+            int32_t sy = ty * span + 128; // first sample row of the tile row
+            if (yt <= sy && sy < yb) {
+                int64_t c = crossing_column(xt, yt, xb, yb, sy);
+                // first tile tx with tx*T >= c
+                int64_t t0 = c >= 0 ? (c + T - 1) / T : -((-c) / T);
+                if (t0 < dt.bx0)
+                    t0 = dt.bx0;
+                if (t0 <= dt.bx1)
+                    atomicAdd(&pts[rowbase + (uint32_t)(t0 - dt.bx0)].backdrop, dir);
+            }
+        }
+    }
+}
+
+__global__ void k_nseg_out(const PathTile *pts, uint32_t *nseg, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        nseg[i] = pts[i].nseg;
+}
+__global__ void k_seg_offsets(PathTile *pts, const uint32_t *off, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        pts[i].seg_off = off[i];
+}
+
+// one thread per (draw, tile row): prefix sum of the backdrop deltas along the row
+__global__ void k_backdrop_prefix(const DrawTiles *__restrict__ dts, const uint32_t *__restrict__ row_off,
+                                  uint32_t n_draws, uint32_t total_rows, PathTile *__restrict__ pts) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total_rows)
+        return;
+    uint32_t lo = 0, hi = n_draws;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (row_off[mid] <= i)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    uint32_t d = lo - 1;
+    DrawTiles dt = dts[d];
+    uint32_t r = i - row_off[d];
+    PathTile *p = pts + dt.pt_base + r * dt.bw;
+    int32_t acc = 0;
+    for (uint32_t k = 0; k < dt.bw; k++) {
+        acc += p[k].backdrop;
+        p[k].backdrop = acc;
+    }
+}
+
+// ----------------------------------------------------------------------------------- coarse
+__device__ __forceinline__ int coverage_count(uint32_t kind, int32_t w) {
+    switch (kind) {
+    case VKH_DRAW_FILL_NZ:
+        return w != 0 ? 1 : 0;
+    case VKH_DRAW_FILL_EO:
+        return w & 1;
+    case VKH_DRAW_STROKE:
+        return w > 0 ? w : 0;
+    default:
+        return 1;
+    }
+}
+
+__device__ __forceinline__ uint32_t find_target_by(const TargetInfo *tg, uint32_t n, uint32_t v, bool bins) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        uint32_t base = bins ? tg[mid].bin_base : tg[mid].tile_base;
+        if (base <= v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo - 1;
+}
+
+// one workgroup per bin of 16 x 16 tiles, one thread per tile; walks the target's draws in
+// order, so each tile's command list is ordered by draw without any sort.
+__global__ __launch_bounds__(256) void k_coarse(bool write, const TargetInfo *__restrict__ tg, uint32_t n_targets,
+                                                const vkh_draw *__restrict__ draws, const DrawTiles *__restrict__ dts,
+                                                const PathTile *__restrict__ pts, uint32_t *__restrict__ tile_count,
+                                                const uint32_t *__restrict__ tile_off, Cmd *__restrict__ cmds) {
+    __shared__ uint64_t masks[4];
+    uint32_t bin = blockIdx.x;
+    uint32_t t = find_target_by(tg, n_targets, bin, true);
+    const TargetInfo &T = tg[t];
+    uint32_t lb = bin - T.bin_base;
+    int32_t bxi = (int32_t)(lb % T.binsW), byi = (int32_t)(lb / T.binsW);
+    uint32_t tid = threadIdx.x;
+    int32_t tx = bxi * VKH_BIN_TILES + (int32_t)(tid & 15), ty = byi * VKH_BIN_TILES + (int32_t)(tid >> 4);
+    bool valid = tx < (int32_t)T.tilesW && ty < (int32_t)T.tilesH;
+    uint32_t gtile = T.tile_base + (uint32_t)ty * T.tilesW + (uint32_t)tx;
+    int32_t bx0 = bxi * VKH_BIN_TILES, by0 = byi * VKH_BIN_TILES;
+    int32_t bx1 = bx0 + VKH_BIN_TILES - 1, by1 = by0 + VKH_BIN_TILES - 1;
+    uint32_t count = 0;
+    uint32_t out = (write && valid) ? tile_off[gtile] : 0;
+    for (uint32_t c0 = T.draw_begin; c0 < T.draw_end; c0 += 256) {
+        uint32_t d = c0 + tid;
+        bool hit = false;
+        if (d < T.draw_end) {
+            DrawTiles b = dts[d];
+            hit = b.bx0 <= b.bx1 && b.bx0 <= bx1 && b.bx1 >= bx0 && b.by0 <= by1 && b.by1 >= by0;
+        }
+        uint64_t m = __ballot(hit);
+        if ((tid & 63) == 0)
+            masks[tid >> 6] = m;
+        __syncthreads();
+        for (int w = 0; w < 4; w++) {
+            uint64_t mm = masks[w];
+            while (mm) {
+                int j = __ffsll((unsigned long long)mm) - 1;
+                mm &= mm - 1;
+                uint32_t dd = c0 + (uint32_t)(w * 64 + j);
+                if (!valid)
+                    continue;
+                DrawTiles b = dts[dd];
+                if (tx < b.bx0 || tx > b.bx1 || ty < b.by0 || ty > b.by1)
+                    continue;
+                uint32_t kind = draws[dd].kind;
+                uint32_t ptid = VKH_NO_DRAW;
+                bool emit = true;
+                if (kind != VKH_DRAW_PAINT && kind != VKH_DRAW_CLEAR) {
+                    ptid = b.pt_base + (uint32_t)(ty - b.by0) * b.bw + (uint32_t)(tx - b.bx0);
+                    PathTile p = pts[ptid];
+                    emit = p.nseg > 0 || coverage_count(kind, p.backdrop) != 0;
+                }
+                if (emit) {
+                    if (write)
+                        cmds[out + count] = Cmd{dd, ptid};
+                    count++;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!write && valid)
+        tile_count[gtile] = count;
+}
+
+// ----------------------------------------------------------------------------------- fine
+__device__ __forceinline__ float clamp01(float x) { return fminf(fmaxf(x, 0.f), 1.f); }
+__device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
+    float t = clamp01((x - e0) / (e1 - e0));
+    return t * t * (3.f - 2.f * t);
+}
+__device__ __forceinline__ void mix4(float *c, const float *d, float a) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        c[i] = c[i] * (1.f - a) + d[i] * a;
+}
+// shaders/vkvg_main.frag:83-144 with p = gl_FragCoord.xy / size
+__device__ void gradient_color(const vkh_gradient &g, float W, float H, float fx, float fy, float *col) {
+    float px = fx / W, py = fy / H;
+    if (g.type == 2) {
+        float dist = 1;
+        float p0x = g.cp[0][0] / W, p0y = g.cp[0][1] / H;
+        float p1x = g.cp[0][2] / W, p1y = g.cp[0][3] / H;
+        float dx = p1x - p0x, dy = p1y - p0y;
+        float l = sqrtf(dx * dx + dy * dy);
+        float ux = dx / l, uy = dy / l;
+        if (uy == 0) {
+            if (ux < 0)
+                dist = -(px - p0x) / l;
+            else
+                dist = (px - p0x) / l;
+        } else {
+            float m = -ux / uy;
+            float bb = p0y - m * p0x;
+            dist = ((py - m * px - bb) / sqrtf(1 + m * m)) / l;
+            if (uy < 0)
+                dist = -dist;
+        }
+        for (int i = 0; i < 4; i++)
+            col[i] = g.colors[0][i];
+        mix4(col, g.colors[1], smoothstepf(g.stops[0], g.stops[1], dist));
+        for (int i = 1; i < (int)g.count - 1; ++i)
+            mix4(col, g.colors[i + 1], smoothstepf(g.stops[i], g.stops[i + 1], dist));
+    } else {
+        float c0x = g.cp[0][0] / W, c0y = g.cp[0][1] / H;
+        float c1x = g.cp[1][0] / W, c1y = g.cp[1][1] / H;
+        float r0 = g.cp[0][2] / W, r1 = g.cp[1][2] / W;
+        float gradLength = 1.0f;
+        float diffx = c0x - c1x, diffy = c0y - c1y;
+        float rx = px - c0x, ry = py - c0y;
+        float rl = sqrtf(rx * rx + ry * ry);
+        float rdx = rx / rl, rdy = ry / rl;
+        float a = rdx * rdx + rdy * rdy;
+        float b = 2.0f * (rdx * diffx + rdy * diffy);
+        float cc = (diffx * diffx + diffy * diffy) - r1 * r1;
+        float disc = b * b - 4.0f * a * cc;
+        if (disc >= 0.0f) {
+            float t = (-b + sqrtf(fabsf(disc))) / (2.0f * a);
+            float prx = c0x + rdx * t, pry = c0y + rdy * t;
+            float ddx = prx - c0x, ddy = pry - c0y;
+            gradLength = sqrtf(ddx * ddx + ddy * ddy) - r0;
+        }
+        float qx = px - c0x, qy = py - c0y;
+        float grad = (sqrtf(qx * qx + qy * qy) - r0) / gradLength;
+        for (int i = 0; i < 4; i++)
+            col[i] = g.colors[0][i];
+        mix4(col, g.colors[1], smoothstepf(g.stops[0], g.stops[1], grad));
+        for (int i = 2; i < (int)g.count; i++)
+            mix4(col, g.colors[i], smoothstepf(g.stops[i - 1], g.stops[i], grad));
+    }
+}
+
+__device__ __forceinline__ uint32_t blend_px(uint32_t op, uint32_t d, uint32_t s) {
+    if (op == 0) // CLEAR: logic op clear
+        return 0;
+    uint32_t ia = 255u - (s >> 24);
+    uint32_t r = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        uint32_t dc = (d >> (8 * i)) & 0xFF, sc = (s >> (8 * i)) & 0xFF;
+        uint32_t m = mul255(dc, ia);
+        uint32_t v;
+        if (op == 3) // DIFFERENCE: BLEND_OP_SUBTRACT
+            v = sc > m ? sc - m : 0;
+        else
+            v = min(sc + m, 255u);
+        r |= v << (8 * i);
+    }
+    return r;
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg, uint32_t n_targets,
+                                              const vkh_draw *__restrict__ draws,
+                                              const vkh_gradient *__restrict__ grads,
+                                              const PathTile *__restrict__ pts, const uint32_t *__restrict__ segs,
+                                              const Edge *__restrict__ edges, uint32_t n_edges,
+                                              const uint32_t *__restrict__ tile_off, const Cmd *__restrict__ cmds) {
+    constexpr int NPIX = T * T;
+    constexpr int PPT = NPIX > 256 ? NPIX / 256 : 1; // pixels per thread
+    constexpr int NTHR = NPIX > 256 ? 256 : NPIX;
+    __shared__ int32_t delta[T][T];
+    __shared__ int32_t rowb[T];
+    uint32_t gtile = blockIdx.x;
+    uint32_t c0 = tile_off[gtile], c1 = tile_off[gtile + 1];
+    if (c0 == c1)
+        return;
+    uint32_t tid = threadIdx.x;
+    if (tid >= NTHR)
+        return;
+    uint32_t t = find_target_by(tg, n_targets, gtile, false);
+    const TargetInfo &TI = tg[t];
+    uint32_t lt = gtile - TI.tile_base;
+    int32_t tx = (int32_t)(lt % TI.tilesW), ty = (int32_t)(lt / TI.tilesW);
+    int32_t X0 = tx * T, Y0 = ty * T;
+    float Wf = (float)TI.width, Hf = (float)TI.height;
+
+    int lx[PPT], ly[PPT];
+    bool valid[PPT];
+    uint32_t px[PPT], sig[PPT];
+#pragma unroll
+    for (int q = 0; q < PPT; q++) {
+        int p = (int)tid + q * NTHR;
+        lx[q] = p % T;
+        ly[q] = p / T;
+        valid[q] = (X0 + lx[q]) < (int32_t)TI.width && (Y0 + ly[q]) < (int32_t)TI.height;
+        sig[q] = 0;
+    }
+    // the destination need not be read when the first command overwrites every pixel
+    bool need_read = true;
+    {
+        Cmd f = cmds[c0];
+        const vkh_draw &d0 = draws[f.draw];
+        if (f.pt == VKH_NO_DRAW) { // paint / clear: every pixel of the tile
+            if (d0.kind == VKH_DRAW_CLEAR || d0.op == 0)
+                need_read = false;
+            else if (d0.grad < 0 && (d0.op == 1 || d0.op == 2) && d0.opacity == 1.0f && (d0.color >> 24) == 255u)
+                need_read = false;
+        }
+    }
+    uint32_t *img = (uint32_t *)TI.rgba;
+#pragma unroll
+    for (int q = 0; q < PPT; q++) {
+        size_t o = (size_t)(Y0 + ly[q]) * TI.width + (size_t)(X0 + lx[q]);
+        px[q] = (need_read && valid[q]) ? img[o] : 0u;
+    }
+
+    for (uint32_t ci = c0; ci < c1; ci++) {
+        Cmd cm = cmds[ci];
+        const vkh_draw &d = draws[cm.draw];
+        int32_t w[PPT];
+        bool full = cm.pt == VKH_NO_DRAW;
+        if (!full) {
+            PathTile p = pts[cm.pt];
+#pragma unroll
+            for (int q = 0; q < PPT; q++)
+                w[q] = p.backdrop;
+            if (p.nseg) {
+                for (int k = (int)tid; k < NPIX; k += NTHR)
+                    (&delta[0][0])[k] = 0;
+                if ((int)tid < T)
+                    rowb[tid] = 0;
+                __syncthreads();
+                uint32_t items = p.nseg * (uint32_t)T;
+                int32_t xs0 = 256 * X0 + 128;
+                int32_t s0y = 256 * Y0 + 128;
+                for (uint32_t it = tid; it < items; it += NTHR) {
+                    uint32_t r = it % T;
+                    uint32_t ei = segs[p.seg_off + it / T];
+                    if (ei >= n_edges)
+                        continue;
+                    Edge e = edges[ei];
+                    int32_t sy = 256 * (Y0 + (int32_t)r) + 128;
+                    int32_t ytop = min(e.y0, e.y1), ybot = max(e.y0, e.y1);
+                    if (ytop <= sy && sy < ybot) {
+                        bool down = e.y0 < e.y1;
+                        int64_t c = down ? crossing_column(e.x0, e.y0, e.x1, e.y1, sy)
+                                         : crossing_column(e.x1, e.y1, e.x0, e.y0, sy);
+                        int64_t lc = c - X0;
+                        if (lc >= 1 && lc < T)
+                            atomicAdd(&delta[r][lc], down ? 1 : -1);
+                    }
+                    if (r > 0 && min(e.x0, e.x1) <= xs0 && xs0 < max(e.x0, e.x1)) {
+                        if (point_above(e, xs0, s0y) && !point_above(e, xs0, sy))
+                            atomicAdd(&rowb[r], e.x1 > e.x0 ? -1 : 1);
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < PPT; q++) {
+                    int32_t acc = rowb[ly[q]];
+                    for (int c = 1; c <= lx[q]; c++)
+                        acc += delta[ly[q]][c];
+                    w[q] += acc;
+                }
+                __syncthreads();
+            }
+        }
+        uint32_t key = draw_key(d.key_index);
+        uint32_t solid = 0;
+        if (d.grad < 0) {
+            float op = d.opacity;
+            uint32_t c = d.color;
+            solid = (uint32_t)f2u8(((float)(c & 0xFF) / 255.0f) * op) |
+                    ((uint32_t)f2u8(((float)((c >> 8) & 0xFF) / 255.0f) * op) << 8) |
+                    ((uint32_t)f2u8(((float)((c >> 16) & 0xFF) / 255.0f) * op) << 16) |
+                    ((uint32_t)f2u8(((float)((c >> 24) & 0xFF) / 255.0f) * op) << 24);
+        }
+#pragma unroll
+        for (int q = 0; q < PPT; q++) {
+            int k = full ? 1 : coverage_count(d.kind, w[q]);
+            if (k <= 0 || !valid[q])
+                continue;
+            uint32_t s = solid;
+            if (d.grad >= 0) {
+                float col[4];
+                gradient_color(grads[d.grad], Wf, Hf, (float)(X0 + lx[q]) + 0.5f, (float)(Y0 + ly[q]) + 0.5f, col);
+                s = (uint32_t)f2u8(col[0] * d.opacity) | ((uint32_t)f2u8(col[1] * d.opacity) << 8) |
+                    ((uint32_t)f2u8(col[2] * d.opacity) << 16) | ((uint32_t)f2u8(col[3] * d.opacity) << 24);
+            }
+            uint32_t v = px[q];
+            uint32_t op = d.kind == VKH_DRAW_CLEAR ? 0u : d.op;
+            for (int j = 0; j < k; j++)
+                v = blend_px(op, v, s);
+            px[q] = v;
+            sig[q] += (uint32_t)k * key;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < PPT; q++) {
+        if (!valid[q])
+            continue;
+        size_t o = (size_t)(Y0 + ly[q]) * TI.width + (size_t)(X0 + lx[q]);
+        img[o] = px[q];
+        if (TI.sig)
+            TI.sig[o] += sig[q];
+    }
+}
+
+// readback with the reference's truncating unpremultiply (src/vkvg_surface.c:441-452)
+__global__ void k_unpremultiply(const uint8_t *__restrict__ src, uint8_t *__restrict__ dst, size_t npix) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix)
+        return;
+    uchar4 p = ((const uchar4 *)src)[i];
+    uchar4 q;
+    if (p.w == 0) {
+        q = make_uchar4(0, 0, 0, 0);
+    } else {
+        double alpha = (double)p.w / (double)255.f;
+        q.x = (uint8_t)(int32_t)((double)p.x / alpha);
+        q.y = (uint8_t)(int32_t)((double)p.y / alpha);
+        q.z = (uint8_t)(int32_t)((double)p.z / alpha);
+        q.w = p.w;
+    }
+    ((uchar4 *)dst)[i] = q;
+}
+
+namespace vkh {
+void launch_draw_bbox_init(hipStream_t s, int4 *bbox, uint32_t n) {
+    if (n)
+        hipLaunchKernelGGL(k_bbox_init, dim3((n + 255) / 256), dim3(256), 0, s, bbox, n);
+}
+void launch_draw_bbox(hipStream_t s, const Edge *edges, const uint32_t *edraw, uint32_t n, int4 *bbox) {
+    if (n)
+        hipLaunchKernelGGL(k_draw_bbox, dim3((n + 255) / 256), dim3(256), 0, s, edges, edraw, n, bbox);
+}
+void launch_draw_tiles(hipStream_t s, const vkh_draw *draws, uint32_t n, const TargetInfo *tg, const int4 *bbox,
+                       uint32_t tile, DrawTiles *dt, uint32_t *pt_count, uint32_t *row_count) {
+    if (n)
+        hipLaunchKernelGGL(k_draw_tiles, dim3((n + 255) / 256), dim3(256), 0, s, draws, n, tg, bbox, tile, dt,
+                           pt_count, row_count);
+}
+void launch_set_pt_base(hipStream_t s, DrawTiles *dt, const uint32_t *pt_off, uint32_t n) {
+    if (n)
+        hipLaunchKernelGGL(k_set_pt_base, dim3((n + 255) / 256), dim3(256), 0, s, dt, pt_off, n);
+}
+void launch_edge_bin(hipStream_t s, bool scatter, const Edge *edges, const uint32_t *edraw, uint32_t n,
+                     const DrawTiles *dt, uint32_t tile, PathTile *pts, uint32_t *segs) {
+    if (n)
+        hipLaunchKernelGGL(k_edge_bin, dim3((n + 255) / 256), dim3(256), 0, s, scatter, edges, edraw, n, dt, tile, pts,
+                           segs);
+}
+void launch_nseg_out(hipStream_t s, const PathTile *pts, uint32_t *nseg, uint32_t n) {
+    if (n)
+        hipLaunchKernelGGL(k_nseg_out, dim3((n + 255) / 256), dim3(256), 0, s, pts, nseg, n);
+}
+void launch_seg_offsets(hipStream_t s, PathTile *pts, const uint32_t *seg_off, uint32_t n) {
+    if (n)
+        hipLaunchKernelGGL(k_seg_offsets, dim3((n + 255) / 256), dim3(256), 0, s, pts, seg_off, n);
+}
+void launch_backdrop_prefix(hipStream_t s, const DrawTiles *dt, const uint32_t *row_off, uint32_t n_draws,
+                            uint32_t total_rows, PathTile *pts) {
+    if (total_rows)
+        hipLaunchKernelGGL(k_backdrop_prefix, dim3((total_rows + 255) / 256), dim3(256), 0, s, dt, row_off, n_draws,
+                           total_rows, pts);
+}
+void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, const TargetInfo *tg, uint32_t n_targets,
+                   const vkh_draw *draws, const DrawTiles *dt, const PathTile *pts, uint32_t *tile_count,
+                   const uint32_t *tile_off, Cmd *cmds) {
+    if (n_bins)
+        hipLaunchKernelGGL(k_coarse, dim3(n_bins), dim3(256), 0, s, write, tg, n_targets, draws, dt, pts, tile_count,
+                           tile_off, cmds);
+}
+void launch_fine(hipStream_t s, uint32_t tile, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
+                 const vkh_draw *draws, const vkh_gradient *grads, const PathTile *pts, const uint32_t *segs,
+                 const Edge *edges, uint32_t n_edges, const uint32_t *tile_off, const Cmd *cmds) {
+    if (!n_tiles)
+        return;
+    switch (tile) {
+    case 8:
+        hipLaunchKernelGGL(k_fine<8>, dim3(n_tiles), dim3(64), 0, s, tg, n_targets, draws, grads, pts, segs, edges,
+                           n_edges, tile_off, cmds);
+        break;
+    case 32:
+        hipLaunchKernelGGL(k_fine<32>, dim3(n_tiles), dim3(256), 0, s, tg, n_targets, draws, grads, pts, segs, edges,
+                           n_edges, tile_off, cmds);
+        break;
+    default:
+        hipLaunchKernelGGL(k_fine<16>, dim3(n_tiles), dim3(256), 0, s, tg, n_targets, draws, grads, pts, segs, edges,
+                           n_edges, tile_off, cmds);
+        break;
+    }
+}
+void launch_unpremultiply(hipStream_t s, const uint8_t *src, uint8_t *dst, size_t npix) {
+    if (npix)
+        hipLaunchKernelGGL(k_unpremultiply, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, src, dst, npix);
+}
+} // namespace vkh
