@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the vkvg MI355X backend.
+
+Headline: the dense 4096x4096 scene (20,000 paths of 1-40 cubics, 60% non-zero fills / 40%
+strokes, 30% gradient sources). One step = one frame through the whole kernel pipeline
+(flatten -> stroke -> edges -> binning -> coarse -> fine composite) with the frame's inputs
+already resident in HBM. value = paths per second (Mpaths/s); Gpix/s composited is reported
+beside it. Also measured: the batched form (256 uneven surfaces in one frame vs 256 plain
+per-surface frames) and, when built, the differentiable renderer's fwd+bwd vs forward time.
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+METRIC = "Mpaths/s + Gpix/s composited (4K scene); batched & fwd+bwd ms vs plain"
+
+
+def _sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def _read_traffic(name: str):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def bench_c3(args, dev, vkvg_amd, scenes):
+    sc = scenes.config3()
+    surf = vkvg_amd.Surface(dev, sc.width, sc.height)
+    surf.upload(sc.words())  # frame resident in HBM
+    for _ in range(args.warmup):
+        vkvg_amd.execute(dev)
+    _sync()
+    fine_ms, dev_ms, stage = [], [], {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        vkvg_amd.execute(dev)
+        st = dev.stats()
+        fine_ms.append(st["ms_fine"])
+        dev_ms.append(st["ms_device"])
+        for k, v in st["ms_stage"].items():
+            stage.setdefault(k, []).append(v)
+    _sync()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    st = dev.stats()
+    surf.close()
+    stage_avg = {k: float(np.mean(v)) for k, v in stage.items()}
+    return sc, ms_step, float(np.mean(fine_ms)), float(np.mean(dev_ms)), stage_avg, st
+
+
+def bench_c4(args, dev, vkvg_amd, scenes):
+    subs = scenes.config4()
+    surfs = [vkvg_amd.Surface(dev, s.width, s.height) for s in subs]
+    words = [s.words() for s in subs]
+    # plain: one frame per surface (each execute timed on resident inputs)
+    plain = 0.0
+    for s, w in zip(surfs, words):
+        s.upload(w)
+        vkvg_amd.execute(dev)  # warm
+        _sync()
+        t = time.perf_counter()
+        vkvg_amd.execute(dev)
+        plain += time.perf_counter() - t
+    # batched: one frame over all 256 surfaces
+    vkvg_amd.upload_batch(dev, surfs, words)
+    for _ in range(max(1, args.warmup)):
+        vkvg_amd.execute(dev)
+    _sync()
+    reps = max(3, args.steps // 4)
+    t = time.perf_counter()
+    for _ in range(reps):
+        vkvg_amd.execute(dev)
+    _sync()
+    batched = (time.perf_counter() - t) / reps
+    for s in surfs:
+        s.close()
+    return {"plain_ms": plain * 1e3, "batched_ms": batched * 1e3, "ratio": plain / batched,
+            "surfaces": len(subs), "paths": int(sum(s.n_paths for s in subs)),
+            "pixels": int(sum(s.width * s.height for s in subs))}
+
+
+def bench_fwd_bwd(args):
+    try:
+        from vkvg_amd import diff
+    except Exception:
+        return None
+    if not hasattr(diff, "bench"):
+        return None
+    return diff.bench(steps=max(5, args.steps), warmup=max(1, args.warmup))
+
+
+def cpu_baseline(n_paths: int = 5000):
+    """The CPU restatement of vkvg (oracle, single thread) on the first n_paths paths of the
+    same 4096x4096 scene; lavapipe is not available on these machines."""
+    from oracle import ovg
+    from vkvg_amd import scenes
+    sub = scenes.config3(n_paths=n_paths)
+    words = sub.words()
+    t = time.perf_counter()
+    ovg.render(sub.width, sub.height, words, with_sig=False)
+    dt = time.perf_counter() - t
+    return {"value": n_paths / dt / 1e6, "unit": "Mpaths/s", "cores": 1, "kind": "port",
+            "sample": f"first {n_paths} of the 20000 paths of the 4096x4096 dense scene "
+                      f"(+ white paint), CPU restatement of vkvg, 1 thread, {dt:.1f} s",
+            "seconds": dt, "nproc": os.cpu_count()}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-paths", type=int, default=5000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this backend renders on one GPU; multi-GPU sharding is design-only "
+                                   "(see DESIGN.md), run with --gpus 1"}))
+        return 2
+    import vkvg_amd
+    from vkvg_amd import scenes
+
+    dev = vkvg_amd.Device()
+    dev.set_tile_size(args.tile)
+    sc, ms_step, fine_ms, dev_ms, stage, st = bench_c3(args, dev, vkvg_amd, scenes)
+    paths_per_s = sc.n_paths / (ms_step / 1e3)
+    gpix = sc.width * sc.height / (ms_step / 1e3) / 1e9
+    # dominant kernel: the fine coverage + composite kernel; its algorithmic bytes per launch
+    # are the frame's image write plus per-path style and gradient records (geometry reads are
+    # intermediates and do not count)
+    fine_bytes = 4 * sc.width * sc.height + 32 * sc.n_paths + 24 * sc.gradients + 20 * sc.gradient_stops
+    achieved = fine_bytes / (fine_ms / 1e3) / 1e9
+    traffic = _read_traffic("fine_c3_pmc.json")
+    out = {
+        "metric": METRIC,
+        "value": paths_per_s / 1e6,
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "dense 4096x4096 RGBA8 scene: 20000 paths of 1-40 cubics, 16-blob Gaussian "
+                               "mixture, 60% non-zero fill / 40% stroke 0.5-8 px, 30% linear/radial gradients",
+                   "surface": "4096x4096", "paths": sc.n_paths, "control_points": sc.ctrl_points,
+                   "gradients": sc.gradients, "tile": st["tile_size"], "inputs": "resident in HBM"},
+        "gpix_per_s": gpix,
+        "frame_algorithmic_bytes": sc.algorithmic_bytes(),
+        "frame_hbm_gbs": sc.algorithmic_bytes() / (ms_step / 1e3) / 1e9,
+        "device_ms": dev_ms,
+        "stages_ms": stage,
+        "counts": {k: st[k] for k in ("points", "stroke_triangles", "edges", "path_tiles", "segments", "commands")},
+        "roofline": {"kernel": "k_fine<16>", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": fine_bytes, "avg_launch_ms": fine_ms},
+    }
+    if not args.no_batch:
+        out["batched"] = bench_c4(args, dev, vkvg_amd, scenes)
+    fb = bench_fwd_bwd(args)
+    if fb is not None:
+        out["fwd_bwd"] = fb
+    dev.close()
+    if not args.no_cpu:
+        cb = cpu_baseline(args.cpu_paths)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = out["value"] / cb["value"]
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -38,6 +38,15 @@ vkvg_public void *vkvg_amd_surface_device_ptr(VkvgSurface surf);
  * destroying the context, surface by surface. */
 vkvg_public vkvg_status_t vkvg_amd_render_batch(VkvgDevice dev, uint32_t n, const VkvgSurface *surfaces,
                                                 const uint32_t *const *streams, const size_t *n_words);
+/* Same as vkvg_amd_render_batch without running the pipeline: the merged frame is left
+ * resident on the device for vkvg_amd_execute. */
+vkvg_public vkvg_status_t vkvg_amd_upload_batch(VkvgDevice dev, uint32_t n, const VkvgSurface *surfaces,
+                                                const uint32_t *const *streams, const size_t *n_words);
+/* Uploads the context's pending frame to the device without rendering it (the frame is then
+ * owned by the device: vkvg_amd_execute renders it, as many times as called). */
+vkvg_public vkvg_status_t vkvg_amd_upload(VkvgContext ctx);
+/* Runs the pipeline on the device's resident frame; returns 0 on success. */
+vkvg_public int vkvg_amd_execute(VkvgDevice dev);
 
 /* Backend controls */
 vkvg_public void       vkvg_amd_device_get_stats(VkvgDevice dev, vkh_stats *out);

@@ -109,8 +109,10 @@ typedef struct {
     uint64_t points, stroke_vertices, stroke_triangles, edges;
     uint64_t path_tiles, segments, commands, tiles;
     uint32_t tile_size;
-    float    ms_total;      /* host wall time of the last vkvg_hip_render, ms */
+    float    ms_total;      /* host wall time of the last vkvg_hip_execute, ms */
     float    ms_fine;       /* device time of the fine (coverage + composite) kernel, ms */
+    float    ms_device;     /* device time from the first to the last kernel of the frame, ms */
+    float    ms_stage[6];   /* flatten, geometry (stroke + edges), tile setup, binning, coarse, fine */
 } vkh_stats;
 
 typedef struct vkh_engine vkh_engine;
@@ -134,9 +136,14 @@ vkvg_public int  vkvg_hip_image_upload(vkh_engine *eng, void *dev_ptr, const voi
 vkvg_public int  vkvg_hip_image_read_unpremultiplied(vkh_engine *eng, const void *dev_ptr, uint32_t width,
                                                      uint32_t height, void *host);
 
-/* Render one frame (any number of draws over any number of targets) and wait for it.
- * Returns 0 on success, a negative value on a HIP error. */
+/* Render one frame (any number of draws over any number of targets) and wait for it:
+ * vkvg_hip_upload followed by vkvg_hip_execute. Returns 0 on success, < 0 on error. */
 vkvg_public int  vkvg_hip_render(vkh_engine *eng, const vkh_frame *frame);
+/* Copies a frame into device memory (one transfer); it stays resident until the next upload. */
+vkvg_public int  vkvg_hip_upload(vkh_engine *eng, const vkh_frame *frame);
+/* Runs the kernel pipeline on the resident frame (re-executable; the frame's targets are
+ * composited again each time). */
+vkvg_public int  vkvg_hip_execute(vkh_engine *eng);
 vkvg_public void vkvg_hip_get_stats(vkh_engine *eng, vkh_stats *out);
 vkvg_public const char *vkvg_hip_last_error(vkh_engine *eng);
 

@@ -28,3 +28,14 @@ def test_c2_strokes_and_eo_stars_small(device):
 
 def test_c3_dense_small(device):
     _check(device, scenes.config3(n_paths=400, size=768))
+
+
+def test_c3_full_resolution_first_1000_paths(device):
+    # the real 4096x4096 dense scene, truncated to its first 1000 paths so the oracle stays fast
+    _check(device, scenes.config3(n_paths=1000))
+
+
+@pytest.mark.parametrize("tile", [8, 32])
+def test_tile_sizes(device, tile):
+    _check(device, scenes.config1(n_paths=120), tile=tile)
+    _check(device, scenes.config2(n_lines=150, n_stars=60, width=500, height=300, variant=1), tile=tile)

@@ -26,10 +26,15 @@ class Stats(ctypes.Structure):
                 ("stroke_triangles", ctypes.c_uint64), ("edges", ctypes.c_uint64),
                 ("path_tiles", ctypes.c_uint64), ("segments", ctypes.c_uint64),
                 ("commands", ctypes.c_uint64), ("tiles", ctypes.c_uint64), ("tile_size", ctypes.c_uint32),
-                ("ms_total", ctypes.c_float), ("ms_fine", ctypes.c_float)]
+                ("ms_total", ctypes.c_float), ("ms_fine", ctypes.c_float), ("ms_device", ctypes.c_float),
+                ("ms_stage", ctypes.c_float * 6)]
+
+    STAGES = ("flatten", "geometry", "tile_setup", "binning", "coarse", "fine")
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "ms_stage"}
+        d["ms_stage"] = {n: float(v) for n, v in zip(self.STAGES, self.ms_stage)}
+        return d
 
 
 EXPORTED = [
@@ -64,10 +69,12 @@ EXPORTED = [
     "vkvg_amd_replay", "vkvg_amd_surface_enable_signature", "vkvg_amd_surface_read_signature",
     "vkvg_amd_surface_read_premultiplied", "vkvg_amd_surface_device_ptr", "vkvg_amd_render_batch",
     "vkvg_amd_device_get_stats", "vkvg_amd_device_set_tile_size", "vkvg_amd_device_engine",
+    "vkvg_amd_upload", "vkvg_amd_execute", "vkvg_amd_upload_batch",
     # include/vkvg_hip.h
     "vkvg_hip_engine_create", "vkvg_hip_engine_destroy", "vkvg_hip_set_tile_size", "vkvg_hip_set_stream",
     "vkvg_hip_image_alloc", "vkvg_hip_image_free", "vkvg_hip_image_clear", "vkvg_hip_image_download",
-    "vkvg_hip_image_upload", "vkvg_hip_image_read_unpremultiplied", "vkvg_hip_render", "vkvg_hip_get_stats",
+    "vkvg_hip_image_upload", "vkvg_hip_image_read_unpremultiplied", "vkvg_hip_render", "vkvg_hip_upload",
+    "vkvg_hip_execute", "vkvg_hip_get_stats",
     "vkvg_hip_last_error",
 ]
 
@@ -121,6 +128,12 @@ def lib() -> ctypes.CDLL:
     L.vkvg_amd_device_set_tile_size.restype = ctypes.c_int
     L.vkvg_amd_device_engine.argtypes = [vp]
     L.vkvg_amd_device_engine.restype = vp
+    L.vkvg_amd_upload.argtypes = [vp]
+    L.vkvg_amd_upload.restype = ctypes.c_int
+    L.vkvg_amd_execute.argtypes = [vp]
+    L.vkvg_amd_execute.restype = ctypes.c_int
+    L.vkvg_amd_upload_batch.argtypes = L.vkvg_amd_render_batch.argtypes
+    L.vkvg_amd_upload_batch.restype = ctypes.c_int
     L.vkvg_hip_set_stream.argtypes = [vp, vp]
     L.vkvg_hip_last_error.argtypes = [vp]
     L.vkvg_hip_last_error.restype = ctypes.c_char_p
@@ -187,6 +200,21 @@ class Surface:
             raise VkvgError(f"replay failed (status {st}): "
                             f"{L.vkvg_hip_last_error(L.vkvg_amd_device_engine(self.dev.h)).decode()}")
 
+    def upload(self, words: np.ndarray) -> None:
+        """Replays the stream and leaves its frame resident on the device (not rendered)."""
+        L = lib()
+        w = np.ascontiguousarray(words, np.uint32)
+        ctx = L.vkvg_create(self.h)
+        st = L.vkvg_status(ctx)
+        if st:
+            raise VkvgError(f"vkvg_create failed (status {st})")
+        st = L.vkvg_amd_replay(ctx, _u32p(w), w.size)
+        if not st:
+            st = L.vkvg_amd_upload(ctx)
+        L.vkvg_destroy(ctx)
+        if st:
+            raise VkvgError(f"upload failed (status {st})")
+
     def premultiplied(self) -> np.ndarray:
         out = np.empty((self.height, self.width, 4), np.uint8)
         if lib().vkvg_amd_surface_read_premultiplied(self.h, out.ctypes.data):
@@ -217,6 +245,25 @@ class Surface:
         if self.h:
             lib().vkvg_surface_destroy(self.h)
             self.h = None
+
+
+def execute(dev: Device) -> None:
+    """Runs the pipeline on the device's resident frame."""
+    L = lib()
+    if L.vkvg_amd_execute(dev.h):
+        raise VkvgError(L.vkvg_hip_last_error(L.vkvg_amd_device_engine(dev.h)).decode())
+
+
+def upload_batch(dev: Device, surfaces: list, streams: list) -> None:
+    L = lib()
+    n = len(surfaces)
+    ws = [np.ascontiguousarray(s, np.uint32) for s in streams]
+    hs = (ctypes.c_void_p * n)(*[s.h for s in surfaces])
+    ps = (ctypes.POINTER(ctypes.c_uint32) * n)(*[_u32p(w) for w in ws])
+    ls = (ctypes.c_size_t * n)(*[w.size for w in ws])
+    st = L.vkvg_amd_upload_batch(dev.h, n, hs, ps, ls)
+    if st:
+        raise VkvgError(f"upload_batch failed (status {st})")
 
 
 def render_batch(dev: Device, surfaces: list[Surface], streams: list[np.ndarray]) -> None:

@@ -85,6 +85,14 @@ struct vkh_engine {
     DevBuf bbox, dtiles, pt_count, pt_off, row_cnt, row_off, ptiles, nseg, seg_off, segs;
     DevBuf tile_cnt, tile_off, cmds, scan_tmp, readback;
     HostBuf hblob, htot;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    hipEvent_t ev_stage[6] = {};
+
+    // the uploaded (resident) frame
+    bool resident = false;
+    size_t o_items, o_sp, o_draws, o_styles, o_dashes, o_grads, o_tg, o_units;
+    uint32_t n_items, n_sp, n_draws, n_styles, n_dashes, n_grads, n_targets, n_units;
+    uint32_t tile_total, bin_total, frame_tile;
 };
 
 static bool check(vkh_engine *e, hipError_t r, const char *what) {
@@ -205,6 +213,10 @@ vkh_engine *vkvg_hip_engine_create(int device_ordinal) {
     e->stream = e->own;
     (void)hipEventCreate(&e->ev0);
     (void)hipEventCreate(&e->ev1);
+    (void)hipEventCreate(&e->ev_begin);
+    (void)hipEventCreate(&e->ev_end);
+    for (hipEvent_t &ev : e->ev_stage)
+        (void)hipEventCreate(&ev);
     return e;
 }
 
@@ -230,6 +242,13 @@ void vkvg_hip_engine_destroy(vkh_engine *e) {
         (void)hipEventDestroy(e->ev0);
     if (e->ev1)
         (void)hipEventDestroy(e->ev1);
+    if (e->ev_begin)
+        (void)hipEventDestroy(e->ev_begin);
+    if (e->ev_end)
+        (void)hipEventDestroy(e->ev_end);
+    for (hipEvent_t ev : e->ev_stage)
+        if (ev)
+            (void)hipEventDestroy(ev);
     if (e->own)
         (void)hipStreamDestroy(e->own);
     delete e;
@@ -308,15 +327,13 @@ void vkvg_hip_get_stats(vkh_engine *e, vkh_stats *out) {
 }
 const char *vkvg_hip_last_error(vkh_engine *e) { return e ? e->err.c_str() : "null engine"; }
 
-int vkvg_hip_render(vkh_engine *eng, const vkh_frame *f) {
+int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
     if (!eng || !f)
         return -1;
-    auto t_start = std::chrono::steady_clock::now();
     (void)hipSetDevice(eng->device);
     hipStream_t s = eng->stream;
     const uint32_t T = eng->tile;
-    vkh_stats st{};
-    st.tile_size = T;
+    eng->resident = false;
 
     // ---- host-side frame tables: work units and targets
     std::vector<Unit> units;
@@ -421,15 +438,54 @@ int vkvg_hip_render(vkh_engine *eng, const vkh_frame *f) {
     if (n_units)
         memcpy(hb + o_units, units.data(), sizeof(Unit) * n_units);
     CK(hipMemcpyAsync(eng->blob.p, hb, total, hipMemcpyHostToDevice, s));
+    eng->o_items = o_items;
+    eng->o_sp = o_sp;
+    eng->o_draws = o_draws;
+    eng->o_styles = o_styles;
+    eng->o_dashes = o_dashes;
+    eng->o_grads = o_grads;
+    eng->o_tg = o_tg;
+    eng->o_units = o_units;
+    eng->n_items = n_items;
+    eng->n_sp = n_sp;
+    eng->n_draws = n_draws;
+    eng->n_styles = f->n_styles;
+    eng->n_dashes = f->n_dashes;
+    eng->n_grads = f->n_grads;
+    eng->n_targets = f->n_targets;
+    eng->n_units = n_units;
+    eng->tile_total = tile_total;
+    eng->bin_total = bin_total;
+    eng->frame_tile = T;
+    eng->resident = true;
+    return 0;
+}
+
+int vkvg_hip_execute(vkh_engine *eng) {
+    if (!eng || !eng->resident) {
+        if (eng)
+            eng->err = "no resident frame";
+        return -1;
+    }
+    auto t_start = std::chrono::steady_clock::now();
+    (void)hipSetDevice(eng->device);
+    hipStream_t s = eng->stream;
+    const uint32_t T = eng->frame_tile;
+    vkh_stats st{};
+    st.tile_size = T;
+    const uint32_t n_items = eng->n_items, n_sp = eng->n_sp, n_draws = eng->n_draws, n_units = eng->n_units;
+    const uint32_t tile_total = eng->tile_total, bin_total = eng->bin_total;
+    const uint32_t n_targets = eng->n_targets;
+    CK(hipEventRecord(eng->ev_begin, s));
     uint8_t *db = (uint8_t *)eng->blob.p;
-    const vkh_item *d_items = (const vkh_item *)(db + o_items);
-    const vkh_subpath *d_sp = (const vkh_subpath *)(db + o_sp);
-    const vkh_draw *d_draws = (const vkh_draw *)(db + o_draws);
-    const vkh_stroke_style *d_styles = (const vkh_stroke_style *)(db + o_styles);
-    const float *d_dashes = (const float *)(db + o_dashes);
-    const vkh_gradient *d_grads = (const vkh_gradient *)(db + o_grads);
-    const TargetInfo *d_tg = (const TargetInfo *)(db + o_tg);
-    const Unit *d_units = (const Unit *)(db + o_units);
+    const vkh_item *d_items = (const vkh_item *)(db + eng->o_items);
+    const vkh_subpath *d_sp = (const vkh_subpath *)(db + eng->o_sp);
+    const vkh_draw *d_draws = (const vkh_draw *)(db + eng->o_draws);
+    const vkh_stroke_style *d_styles = (const vkh_stroke_style *)(db + eng->o_styles);
+    const float *d_dashes = (const float *)(db + eng->o_dashes);
+    const vkh_gradient *d_grads = (const vkh_gradient *)(db + eng->o_grads);
+    const TargetInfo *d_tg = (const TargetInfo *)(db + eng->o_tg);
+    const Unit *d_units = (const Unit *)(db + eng->o_units);
 
     // ---- flatten
     if (!ensure(eng, eng->item_cnt, 4 * (n_items + 1)) || !ensure(eng, eng->item_off, 4 * (n_items + 1)))
@@ -451,6 +507,7 @@ int vkvg_hip_render(vkh_engine *eng, const vkh_frame *f) {
                                P<uint32_t>(eng->sp_count));
     CK(hipGetLastError());
 
+    CK(hipEventRecord(eng->ev_stage[0], s)); // flatten done
     // ---- per-unit geometry counts
     size_t un4 = 4ull * (n_units + 1);
     if (!ensure(eng, eng->fe, un4) || !ensure(eng, eng->nv, un4) || !ensure(eng, eng->nt, un4) ||
@@ -494,6 +551,7 @@ int vkvg_hip_render(vkh_engine *eng, const vkh_frame *f) {
                           P<Edge>(eng->edges) + n_fill_edges, P<uint32_t>(eng->edraw) + n_fill_edges);
     CK(hipGetLastError());
 
+    CK(hipEventRecord(eng->ev_stage[1], s)); // geometry done
     // ---- tiles of every draw
     size_t dn4 = 4ull * (n_draws + 1);
     if (!ensure(eng, eng->bbox, sizeof(int4) * (n_draws + 1)) || !ensure(eng, eng->dtiles, sizeof(DrawTiles) * (n_draws + 1)) ||
@@ -515,6 +573,7 @@ int vkvg_hip_render(vkh_engine *eng, const vkh_frame *f) {
     if (!ensure(eng, eng->ptiles, sizeof(PathTile) * (size_t)n_ptiles + 16) ||
         !ensure(eng, eng->nseg, 4ull * (n_ptiles + 1)) || !ensure(eng, eng->seg_off, 4ull * (n_ptiles + 1)))
         return -1;
+    CK(hipEventRecord(eng->ev_stage[2], s)); // tile setup done
     if (n_ptiles)
         CK(hipMemsetAsync(eng->ptiles.p, 0, sizeof(PathTile) * (size_t)n_ptiles, s));
     vkh::launch_edge_bin(s, false, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), n_edges, P<DrawTiles>(eng->dtiles),
@@ -538,12 +597,13 @@ int vkvg_hip_render(vkh_engine *eng, const vkh_frame *f) {
                          P<PathTile>(eng->ptiles), P<uint32_t>(eng->segs));
     CK(hipGetLastError());
 
+    CK(hipEventRecord(eng->ev_stage[3], s)); // binning done
     // ---- coarse: ordered command list per tile
-    if (!ensure(eng, eng->tgts, sizeof(TargetInfo) * (f->n_targets + 1)) ||
+    if (!ensure(eng, eng->tgts, sizeof(TargetInfo) * (n_targets + 1)) ||
         !ensure(eng, eng->tile_cnt, 4ull * (tile_total + 1)) || !ensure(eng, eng->tile_off, 4ull * (tile_total + 1)))
         return -1;
     CK(hipMemsetAsync(eng->tile_cnt.p, 0, 4ull * (tile_total + 1), s));
-    vkh::launch_coarse(s, false, bin_total, d_tg, f->n_targets, d_draws, P<DrawTiles>(eng->dtiles),
+    vkh::launch_coarse(s, false, bin_total, d_tg, n_targets, d_draws, P<DrawTiles>(eng->dtiles),
                        P<PathTile>(eng->ptiles), P<uint32_t>(eng->tile_cnt), nullptr, nullptr);
     if (scan_u32(eng, P<uint32_t>(eng->tile_cnt), tile_total, P<uint32_t>(eng->tile_off)))
         return -1;
@@ -554,24 +614,43 @@ int vkvg_hip_render(vkh_engine *eng, const vkh_frame *f) {
     st.tiles = tile_total;
     if (!ensure(eng, eng->cmds, sizeof(Cmd) * (size_t)n_cmds + 16))
         return -1;
-    vkh::launch_coarse(s, true, bin_total, d_tg, f->n_targets, d_draws, P<DrawTiles>(eng->dtiles),
+    vkh::launch_coarse(s, true, bin_total, d_tg, n_targets, d_draws, P<DrawTiles>(eng->dtiles),
                        P<PathTile>(eng->ptiles), nullptr, P<uint32_t>(eng->tile_off), P<Cmd>(eng->cmds));
     CK(hipGetLastError());
 
     // ---- fine: coverage + compositing
+    CK(hipEventRecord(eng->ev_stage[4], s)); // coarse done
     CK(hipEventRecord(eng->ev0, s));
-    vkh::launch_fine(s, T, tile_total, d_tg, f->n_targets, d_draws, d_grads, P<PathTile>(eng->ptiles),
+    vkh::launch_fine(s, T, tile_total, d_tg, n_targets, d_draws, d_grads, P<PathTile>(eng->ptiles),
                      P<uint32_t>(eng->segs), P<Edge>(eng->edges), n_edges, P<uint32_t>(eng->tile_off),
                      P<Cmd>(eng->cmds));
     CK(hipGetLastError());
     CK(hipEventRecord(eng->ev1, s));
+    CK(hipEventRecord(eng->ev_end, s));
     CK(hipStreamSynchronize(s));
     float ms = 0;
     (void)hipEventElapsedTime(&ms, eng->ev0, eng->ev1);
     st.ms_fine = ms;
+    ms = 0;
+    (void)hipEventElapsedTime(&ms, eng->ev_begin, eng->ev_end);
+    st.ms_device = ms;
+    hipEvent_t marks[7] = {eng->ev_begin, eng->ev_stage[0], eng->ev_stage[1], eng->ev_stage[2],
+                           eng->ev_stage[3], eng->ev_stage[4], eng->ev_end};
+    for (int k = 0; k < 6; k++) {
+        ms = 0;
+        (void)hipEventElapsedTime(&ms, marks[k], marks[k + 1]);
+        st.ms_stage[k] = ms;
+    }
     st.ms_total = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     eng->stats = st;
     return 0;
+}
+
+int vkvg_hip_render(vkh_engine *eng, const vkh_frame *f) {
+    int r = vkvg_hip_upload(eng, f);
+    if (r)
+        return r;
+    return vkvg_hip_execute(eng);
 }
 
 } // extern "C"

@@ -704,39 +704,59 @@ static void frame_reset_keep_path(VkvgContext c) {
         f->grads.n = 0;
 }
 
-static void submit_frame(VkvgContext c) {
+static void fill_frame(VkvgContext c, vkh_frame *fr, vkh_target *t) {
+    frame_t *f = &c->f;
+    t->rgba = c->surf->img;
+    t->width = c->surf->width;
+    t->height = c->surf->height;
+    t->sig = c->surf->sig;
+    for (uint32_t i = 0; i < f->draws.n; i++)
+        f->draws.p[i].target = 0;
+    fr->items = f->items.p;
+    fr->n_items = f->items.n;
+    fr->subpaths = f->sps.p;
+    fr->n_subpaths = f->sps.n;
+    fr->draws = f->draws.p;
+    fr->n_draws = f->draws.n;
+    fr->styles = f->styles.p;
+    fr->n_styles = f->styles.n;
+    fr->dashes = f->dashes.p;
+    fr->n_dashes = f->dashes.n;
+    fr->grads = f->grads.p;
+    fr->n_grads = f->grads.n;
+    fr->targets = t;
+    fr->n_targets = 1;
+}
+
+/* upload_only: copy the frame to the device without running the pipeline */
+static void submit_frame_ex(VkvgContext c, bool upload_only) {
     frame_t *f = &c->f;
     if (f->draws.n == 0 || c->deferred)
         return;
     vkh_target t;
-    t.rgba = c->surf->img;
-    t.width = c->surf->width;
-    t.height = c->surf->height;
-    t.sig = c->surf->sig;
-    for (uint32_t i = 0; i < f->draws.n; i++)
-        f->draws.p[i].target = 0;
     vkh_frame fr;
-    fr.items = f->items.p;
-    fr.n_items = f->items.n;
-    fr.subpaths = f->sps.p;
-    fr.n_subpaths = f->sps.n;
-    fr.draws = f->draws.p;
-    fr.n_draws = f->draws.n;
-    fr.styles = f->styles.p;
-    fr.n_styles = f->styles.n;
-    fr.dashes = f->dashes.p;
-    fr.n_dashes = f->dashes.n;
-    fr.grads = f->grads.p;
-    fr.n_grads = f->grads.n;
-    fr.targets = &t;
-    fr.n_targets = 1;
-    int r = vkvg_hip_render(c->surf->dev->eng, &fr);
+    fill_frame(c, &fr, &t);
+    vkh_engine *eng = c->surf->dev->eng;
+    int r = upload_only ? vkvg_hip_upload(eng, &fr) : vkvg_hip_render(eng, &fr);
     if (r) {
-        fprintf(stderr, "vkvg: render failed: %s\n", vkvg_hip_last_error(c->surf->dev->eng));
+        fprintf(stderr, "vkvg: render failed: %s\n", vkvg_hip_last_error(eng));
         c->status = VKVG_STATUS_DEVICE_ERROR;
         return;
     }
     frame_reset_keep_path(c);
+}
+static void submit_frame(VkvgContext c) { submit_frame_ex(c, false); }
+
+vkvg_status_t vkvg_amd_upload(VkvgContext ctx) {
+    if (vkvg_status(ctx))
+        return vkvg_status(ctx);
+    submit_frame_ex(ctx, true);
+    return ctx->status;
+}
+int vkvg_amd_execute(VkvgDevice dev) {
+    if (vkvg_device_status(dev))
+        return -1;
+    return vkvg_hip_execute(dev->eng);
 }
 
 void vkvg_flush(VkvgContext ctx) {
@@ -1657,7 +1677,7 @@ vkvg_status_t vkvg_amd_replay(VkvgContext ctx, const uint32_t *words, size_t n) 
 }
 
 /* ---------------------------------------------------------------------------- batch */
-vkvg_status_t vkvg_amd_render_batch(VkvgDevice dev, uint32_t n, const VkvgSurface *surfs,
+vkvg_status_t vkvg_amd_upload_batch(VkvgDevice dev, uint32_t n, const VkvgSurface *surfs,
                                     const uint32_t *const *streams, const size_t *n_words) {
     if (vkvg_device_status(dev))
         return VKVG_STATUS_DEVICE_ERROR;
@@ -1667,6 +1687,7 @@ vkvg_status_t vkvg_amd_render_batch(VkvgDevice dev, uint32_t n, const VkvgSurfac
     vkh_target *tg = (vkh_target *)calloc(n, sizeof(vkh_target));
     frame_t all;
     memset(&all, 0, sizeof(all));
+    uint32_t n_draws_total = 0;
     vkvg_status_t st = VKVG_STATUS_SUCCESS;
     if (!ctxs || !tg) {
         st = VKVG_STATUS_NO_MEMORY;
@@ -1722,6 +1743,7 @@ vkvg_status_t vkvg_amd_render_batch(VkvgDevice dev, uint32_t n, const VkvgSurfac
         tg[i].height = surfs[i]->height;
         tg[i].sig = surfs[i]->sig;
     }
+    n_draws_total = all.draws.n;
     if (!st && all.draws.n) {
         vkh_frame fr;
         fr.items = all.items.p;
@@ -1738,8 +1760,8 @@ vkvg_status_t vkvg_amd_render_batch(VkvgDevice dev, uint32_t n, const VkvgSurfac
         fr.n_grads = all.grads.n;
         fr.targets = tg;
         fr.n_targets = n;
-        if (vkvg_hip_render(dev->eng, &fr)) {
-            fprintf(stderr, "vkvg: batch render failed: %s\n", vkvg_hip_last_error(dev->eng));
+        if (vkvg_hip_upload(dev->eng, &fr)) {
+            fprintf(stderr, "vkvg: batch upload failed: %s\n", vkvg_hip_last_error(dev->eng));
             st = VKVG_STATUS_DEVICE_ERROR;
         }
     }
@@ -1753,5 +1775,21 @@ done:
     frame_free(&all);
     free(ctxs);
     free(tg);
+    if (!st && n_draws_total == 0)
+        st = VKVG_STATUS_INVALID_STATUS; /* nothing to execute */
     return st;
+}
+
+vkvg_status_t vkvg_amd_render_batch(VkvgDevice dev, uint32_t n, const VkvgSurface *surfs,
+                                    const uint32_t *const *streams, const size_t *n_words) {
+    vkvg_status_t st = vkvg_amd_upload_batch(dev, n, surfs, streams, n_words);
+    if (st == VKVG_STATUS_INVALID_STATUS)
+        return VKVG_STATUS_SUCCESS; /* empty batch */
+    if (st)
+        return st;
+    if (vkvg_hip_execute(dev->eng)) {
+        fprintf(stderr, "vkvg: batch render failed: %s\n", vkvg_hip_last_error(dev->eng));
+        return VKVG_STATUS_DEVICE_ERROR;
+    }
+    return VKVG_STATUS_SUCCESS;
 }
