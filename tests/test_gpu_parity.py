@@ -39,3 +39,19 @@ def test_c3_full_resolution_first_1000_paths(device):
 def test_tile_sizes(device, tile):
     _check(device, scenes.config1(n_paths=120), tile=tile)
     _check(device, scenes.config2(n_lines=150, n_stars=60, width=500, height=300, variant=1), tile=tile)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_feature_mix(device, variant):
+    # joins/caps on closed strokes, dash offsets, arcs, ellipses, EO, quadratics, relative
+    # curves, transforms, gradients, opacity, DIFFERENCE and CLEAR operators, paint with a path
+    _check(device, scenes.feature_mix(variant))
+
+
+def test_clear(device):
+    _check(device, scenes.clear_scene())
+
+
+@pytest.mark.parametrize("tile", [8, 32])
+def test_feature_mix_tiles(device, tile):
+    _check(device, scenes.feature_mix(3, size=300, n=48), tile=tile)

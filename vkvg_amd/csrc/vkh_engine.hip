@@ -84,6 +84,7 @@ struct vkh_engine {
     DevBuf fe, nv, nt, fe_off, nv_off, nt_off, verts, tris, edges, edraw;
     DevBuf bbox, dtiles, pt_count, pt_off, row_cnt, row_off, ptiles, nseg, seg_off, segs;
     DevBuf tile_cnt, tile_off, cmds, scan_tmp, readback;
+    DevBuf step_nv, step_nt, sv_off, st_off, u_toff, u_vbeg, u_vend;
     HostBuf hblob, htot;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     hipEvent_t ev_stage[6] = {};
@@ -230,7 +231,8 @@ void vkvg_hip_engine_destroy(vkh_engine *e) {
                       &e->sp_first, &e->sp_count, &e->fe, &e->nv, &e->nt, &e->fe_off, &e->nv_off, &e->nt_off,
                       &e->verts, &e->tris, &e->edges, &e->edraw, &e->bbox, &e->dtiles, &e->pt_count, &e->pt_off,
                       &e->row_cnt, &e->row_off, &e->ptiles, &e->nseg, &e->seg_off, &e->segs, &e->tile_cnt,
-                      &e->tile_off, &e->cmds, &e->scan_tmp, &e->readback};
+                      &e->tile_off, &e->cmds, &e->scan_tmp, &e->readback, &e->step_nv, &e->step_nt,
+                      &e->sv_off, &e->st_off, &e->u_toff, &e->u_vbeg, &e->u_vend};
     for (DevBuf *b : bufs)
         if (b->p)
             (void)hipFree(b->p);
@@ -508,24 +510,34 @@ int vkvg_hip_execute(vkh_engine *eng) {
     CK(hipGetLastError());
 
     CK(hipEventRecord(eng->ev_stage[0], s)); // flatten done
-    // ---- per-unit geometry counts
+    // ---- per-unit geometry: fill edge counts and stroke work steps
     size_t un4 = 4ull * (n_units + 1);
-    if (!ensure(eng, eng->fe, un4) || !ensure(eng, eng->nv, un4) || !ensure(eng, eng->nt, un4) ||
-        !ensure(eng, eng->fe_off, un4) || !ensure(eng, eng->nv_off, un4) || !ensure(eng, eng->nt_off, un4))
+    if (!ensure(eng, eng->fe, un4) || !ensure(eng, eng->nv, un4) || !ensure(eng, eng->fe_off, un4) ||
+        !ensure(eng, eng->nv_off, un4) || !ensure(eng, eng->u_toff, un4) || !ensure(eng, eng->u_vbeg, un4) ||
+        !ensure(eng, eng->u_vend, un4))
         return -1;
-    vkh::launch_unit_count(s, d_units, n_units, d_draws, d_sp, P<uint32_t>(eng->sp_first), P<uint32_t>(eng->sp_count),
-                           P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes, P<uint32_t>(eng->fe),
-                           P<uint32_t>(eng->nv), P<uint32_t>(eng->nt));
+    uint32_t *d_steps = P<uint32_t>(eng->nv), *d_step_off = P<uint32_t>(eng->nv_off);
+    vkh::launch_unit_count(s, d_units, n_units, d_draws, P<uint32_t>(eng->sp_count), d_styles, P<uint32_t>(eng->fe),
+                           d_steps);
     if (scan_u32(eng, P<uint32_t>(eng->fe), n_units, P<uint32_t>(eng->fe_off)) ||
-        scan_u32(eng, P<uint32_t>(eng->nv), n_units, P<uint32_t>(eng->nv_off)) ||
-        scan_u32(eng, P<uint32_t>(eng->nt), n_units, P<uint32_t>(eng->nt_off)))
+        scan_u32(eng, d_steps, n_units, d_step_off))
         return -1;
-    if (read_totals(eng,
-                    {P<uint32_t>(eng->fe_off) + n_units, P<uint32_t>(eng->nv_off) + n_units,
-                     P<uint32_t>(eng->nt_off) + n_units},
-                    tot))
+    if (read_totals(eng, {P<uint32_t>(eng->fe_off) + n_units, d_step_off + n_units}, tot))
         return -1;
-    const uint32_t n_fill_edges = tot[0], n_verts = tot[1], n_tris = tot[2];
+    const uint32_t n_fill_edges = tot[0], n_steps = tot[1];
+    size_t sn4 = 4ull * (n_steps + 1);
+    if (!ensure(eng, eng->step_nv, sn4) || !ensure(eng, eng->step_nt, sn4) || !ensure(eng, eng->sv_off, sn4) ||
+        !ensure(eng, eng->st_off, sn4))
+        return -1;
+    vkh::launch_stroke_steps(s, false, d_units, n_units, d_step_off, n_steps, d_draws, d_sp, P<uint32_t>(eng->sp_first),
+                             P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes,
+                             P<uint32_t>(eng->step_nv), P<uint32_t>(eng->step_nt), nullptr, nullptr, nullptr, nullptr);
+    if (scan_u32(eng, P<uint32_t>(eng->step_nv), n_steps, P<uint32_t>(eng->sv_off)) ||
+        scan_u32(eng, P<uint32_t>(eng->step_nt), n_steps, P<uint32_t>(eng->st_off)))
+        return -1;
+    if (read_totals(eng, {P<uint32_t>(eng->sv_off) + n_steps, P<uint32_t>(eng->st_off) + n_steps}, tot))
+        return -1;
+    const uint32_t n_verts = tot[0], n_tris = tot[1];
     const uint64_t n_edges64 = (uint64_t)n_fill_edges + 3ull * n_tris;
     if (n_edges64 > 0x7FFFFFFFull) {
         eng->err = "frame too large (edges)";
@@ -539,15 +551,20 @@ int vkvg_hip_execute(vkh_engine *eng) {
         !ensure(eng, eng->edges, sizeof(Edge) * (size_t)n_edges + 16) ||
         !ensure(eng, eng->edraw, 4ull * n_edges + 4))
         return -1;
-    vkh::launch_stroke_emit(s, d_units, n_units, d_draws, d_sp, P<uint32_t>(eng->sp_first),
-                            P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes,
-                            P<uint32_t>(eng->nv_off), P<uint32_t>(eng->nt_off), P<v2>(eng->verts),
-                            P<uint32_t>(eng->tris));
+    vkh::launch_stroke_steps(s, true, d_units, n_units, d_step_off, n_steps, d_draws, d_sp, P<uint32_t>(eng->sp_first),
+                             P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes,
+                             nullptr, nullptr, P<uint32_t>(eng->sv_off), P<uint32_t>(eng->st_off), P<v2>(eng->verts),
+                             P<uint32_t>(eng->tris));
+    vkh::launch_stroke_patch(s, d_units, n_units, d_step_off, d_draws, d_sp, P<uint32_t>(eng->sp_first),
+                             P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes,
+                             P<uint32_t>(eng->sv_off), P<uint32_t>(eng->st_off), P<uint32_t>(eng->tris));
+    vkh::launch_unit_ranges(s, n_units, d_step_off, P<uint32_t>(eng->sv_off), P<uint32_t>(eng->st_off),
+                            P<uint32_t>(eng->u_toff), P<uint32_t>(eng->u_vbeg), P<uint32_t>(eng->u_vend));
     vkh::launch_fill_edges(s, d_units, n_units, P<uint32_t>(eng->fe_off), n_fill_edges, d_draws, d_tg,
                            P<uint32_t>(eng->sp_first), P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<Edge>(eng->edges),
                            P<uint32_t>(eng->edraw));
-    vkh::launch_tri_edges(s, d_units, n_units, P<uint32_t>(eng->nt_off), P<uint32_t>(eng->nv_off),
-                          P<uint32_t>(eng->nv), n_tris, d_draws, d_tg, P<v2>(eng->verts), P<uint32_t>(eng->tris),
+    vkh::launch_tri_edges(s, d_units, n_units, P<uint32_t>(eng->u_toff), P<uint32_t>(eng->u_vbeg),
+                          P<uint32_t>(eng->u_vend), n_tris, d_draws, d_tg, P<v2>(eng->verts), P<uint32_t>(eng->tris),
                           P<Edge>(eng->edges) + n_fill_edges, P<uint32_t>(eng->edraw) + n_fill_edges);
     CK(hipGetLastError());
 

@@ -431,13 +431,37 @@ __device__ __forceinline__ StrokeCtx make_stroke_ctx(const vkh_stroke_style &st,
     return c;
 }
 
-// per unit: fill -> number of polygon edges; stroke -> vertex and triangle counts
+// One step of a non-dashed sub-path: step 0 and step n-1 are the caps of an open path, the
+// others a join at point i; a closed path has a join at every point, the last one closing
+// (src/vkvg_context.c:869-935). A step's output depends only on its three points, so steps
+// run in parallel; the vertex indices a step emits are global (base = the step's vertex
+// offset), so the references a rect makes to the next step's first two vertices resolve to
+// the same vertices as in the reference's sequential emission.
+template <typename S>
+__device__ bool stroke_step(const StrokeCtx &c, bool closed, uint32_t i, S &s) {
+    const uint32_t last = c.n - 1;
+    if (!closed) {
+        if (i == 0 || i == last) {
+            v2 a = i == 0 ? c.P[0] : c.P[last - 1], b = i == 0 ? c.P[1] : c.P[last];
+            v2 d = V(b.x - a.x, b.y - a.y);
+            float md = sqrtf(d.x * d.x + d.y * d.y);
+            d.x /= md;
+            d.y /= md;
+            stroke_cap(c, s, i == 0 ? a : b, d, i == 0);
+            return false;
+        }
+        return vb_step(c, s, i - 1, i, i + 1, c.F[i] != 0);
+    }
+    if (i == last)
+        return vb_step(c, s, last - 1, last, 0, false);
+    return vb_step(c, s, i == 0 ? last : i - 1, i, i + 1, c.F[i] != 0);
+}
+
+// per unit: fill -> number of polygon edges; stroke -> number of work steps (one per point for
+// a solid stroke, one for a whole dashed sub-path)
 __global__ void k_unit_count(const Unit *__restrict__ units, uint32_t n_units, const vkh_draw *__restrict__ draws,
-                             const vkh_subpath *__restrict__ sps, const uint32_t *__restrict__ sp_first,
-                             const uint32_t *__restrict__ sp_count, const v2 *__restrict__ pts,
-                             const uint8_t *__restrict__ pf, const vkh_stroke_style *__restrict__ styles,
-                             const float *__restrict__ dashes, uint32_t *__restrict__ fill_edges,
-                             uint32_t *__restrict__ nverts, uint32_t *__restrict__ ntris) {
+                             const uint32_t *__restrict__ sp_count, const vkh_stroke_style *__restrict__ styles,
+                             uint32_t *__restrict__ fill_edges, uint32_t *__restrict__ steps) {
     uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= n_units)
         return;
@@ -445,38 +469,122 @@ __global__ void k_unit_count(const Unit *__restrict__ units, uint32_t n_units, c
     const vkh_draw &d = draws[un.draw];
     uint32_t n = sp_count[un.subpath];
     if (d.kind == VKH_DRAW_STROKE) {
-        StrokeCtx c = make_stroke_ctx(styles[d.style], dashes, pts, pf, sp_first[un.subpath], n);
-        CountSink s;
-        stroke_subpath(c, (sps[un.subpath].flags & VKH_SP_CLOSED) != 0, styles[d.style].dash_offset, s);
         fill_edges[u] = 0;
-        nverts[u] = s.nv;
-        ntris[u] = s.ni / 3;
+        steps[u] = n < 2 ? 0u : (styles[d.style].dash_count ? 1u : n);
     } else {
         fill_edges[u] = n > 2 ? n : 0;
-        nverts[u] = 0;
-        ntris[u] = 0;
+        steps[u] = 0;
     }
 }
 
-__global__ void k_stroke_emit(const Unit *__restrict__ units, uint32_t n_units, const vkh_draw *__restrict__ draws,
-                              const vkh_subpath *__restrict__ sps, const uint32_t *__restrict__ sp_first,
-                              const uint32_t *__restrict__ sp_count, const v2 *__restrict__ pts,
-                              const uint8_t *__restrict__ pf, const vkh_stroke_style *__restrict__ styles,
-                              const float *__restrict__ dashes, const uint32_t *__restrict__ vert_off,
-                              const uint32_t *__restrict__ tri_off, v2 *__restrict__ verts,
-                              uint32_t *__restrict__ tris) {
+__device__ __forceinline__ uint32_t unit_of(const uint32_t *off, uint32_t n, uint32_t v) {
+    uint32_t lo = 0, hi = n; // first index with off[i] > v
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo - 1;
+}
+
+template <bool EMIT>
+__global__ void k_stroke_steps(const Unit *__restrict__ units, uint32_t n_units, const uint32_t *__restrict__ step_off,
+                               uint32_t n_steps, const vkh_draw *__restrict__ draws,
+                               const vkh_subpath *__restrict__ sps, const uint32_t *__restrict__ sp_first,
+                               const uint32_t *__restrict__ sp_count, const v2 *__restrict__ pts,
+                               const uint8_t *__restrict__ pf, const vkh_stroke_style *__restrict__ styles,
+                               const float *__restrict__ dashes, uint32_t *__restrict__ step_nv,
+                               uint32_t *__restrict__ step_nt, const uint32_t *__restrict__ sv_off,
+                               const uint32_t *__restrict__ st_off, v2 *__restrict__ verts,
+                               uint32_t *__restrict__ tris) {
+    uint32_t sidx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sidx >= n_steps)
+        return;
+    uint32_t u = unit_of(step_off, n_units, sidx);
+    Unit un = units[u];
+    const vkh_draw &d = draws[un.draw];
+    const vkh_stroke_style &st = styles[d.style];
+    StrokeCtx c = make_stroke_ctx(st, dashes, pts, pf, sp_first[un.subpath], sp_count[un.subpath]);
+    bool closed = (sps[un.subpath].flags & VKH_SP_CLOSED) != 0;
+    uint32_t i = sidx - step_off[u];
+    if (!EMIT) {
+        CountSink k;
+        if (st.dash_count)
+            stroke_subpath(c, closed, st.dash_offset, k);
+        else
+            stroke_step(c, closed, i, k);
+        step_nv[sidx] = k.nv;
+        step_nt[sidx] = k.ni / 3;
+    } else {
+        EmitSink k;
+        k.verts = verts;
+        k.nv = sv_off[sidx];
+        k.idx = tris + 3 * (size_t)st_off[sidx];
+        k.ni = 0;
+        if (st.dash_count)
+            stroke_subpath(c, closed, st.dash_offset, k);
+        else
+            stroke_step(c, closed, i, k);
+    }
+}
+
+// closing patch of src/vkvg_context.c:917-932 for closed solid strokes, applied on the last two
+// triangles of the sub-path once every step has written its triangles
+__global__ void k_stroke_patch(const Unit *__restrict__ units, uint32_t n_units, const uint32_t *__restrict__ step_off,
+                               const vkh_draw *__restrict__ draws, const vkh_subpath *__restrict__ sps,
+                               const uint32_t *__restrict__ sp_first, const uint32_t *__restrict__ sp_count,
+                               const v2 *__restrict__ pts, const uint8_t *__restrict__ pf,
+                               const vkh_stroke_style *__restrict__ styles, const float *__restrict__ dashes,
+                               const uint32_t *__restrict__ sv_off, const uint32_t *__restrict__ st_off,
+                               uint32_t *__restrict__ tris) {
     uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= n_units)
         return;
     Unit un = units[u];
     const vkh_draw &d = draws[un.draw];
-    if (d.kind != VKH_DRAW_STROKE)
+    if (d.kind != VKH_DRAW_STROKE || !(sps[un.subpath].flags & VKH_SP_CLOSED))
         return;
-    StrokeCtx c = make_stroke_ctx(styles[d.style], dashes, pts, pf, sp_first[un.subpath], sp_count[un.subpath]);
-    EmitSink s;
-    s.verts = verts + vert_off[u];
-    s.idx = tris + 3 * (size_t)tri_off[u];
-    stroke_subpath(c, (sps[un.subpath].flags & VKH_SP_CLOSED) != 0, styles[d.style].dash_offset, s);
+    const vkh_stroke_style &st = styles[d.style];
+    if (st.dash_count)
+        return;
+    uint32_t s0 = step_off[u], s1 = step_off[u + 1];
+    if (s1 <= s0)
+        return;
+    uint32_t t0 = st_off[s0], t1 = st_off[s1];
+    if (t1 - t0 < 2)
+        return;
+    StrokeCtx c = make_stroke_ctx(st, dashes, pts, pf, sp_first[un.subpath], sp_count[un.subpath]);
+    CountSink k;
+    bool inverse = stroke_step(c, true, s1 - s0 - 1, k);
+    uint32_t ii = sv_off[s0];
+    uint32_t *ind = tris + 3 * (size_t)t1 - 6;
+    if (inverse) {
+        ind[1] = ii + 1;
+        ind[4] = ii + 1;
+        ind[5] = ii;
+    } else {
+        ind[1] = ii;
+        ind[4] = ii;
+        ind[5] = ii + 1;
+    }
+}
+
+// per unit: first triangle and vertex range of its stroke steps
+__global__ void k_unit_ranges(uint32_t n_units, const uint32_t *__restrict__ step_off,
+                              const uint32_t *__restrict__ sv_off, const uint32_t *__restrict__ st_off,
+                              uint32_t *__restrict__ u_toff, uint32_t *__restrict__ u_vbeg,
+                              uint32_t *__restrict__ u_vend) {
+    uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u > n_units)
+        return;
+    uint32_t s0 = step_off[u];
+    u_toff[u] = st_off[s0];
+    if (u < n_units) {
+        u_vbeg[u] = sv_off[s0];
+        u_vend[u] = sv_off[step_off[u + 1]];
+    }
 }
 
 // ---------------------------------------------------------------------------------------- edges
@@ -516,30 +624,31 @@ __global__ void k_fill_edges(const Unit *__restrict__ units, uint32_t n_units, c
     edraw[i] = bad ? VKH_NO_DRAW : un.draw;
 }
 
-// stroke triangles -> 3 edges each, oriented so every triangle winds +1 (ray to -x, +y edges count +1)
-__global__ void k_tri_edges(const Unit *__restrict__ units, uint32_t n_units, const uint32_t *__restrict__ tri_off,
-                            const uint32_t *__restrict__ vert_off, const uint32_t *__restrict__ nverts,
+// stroke triangles -> 3 edges each, oriented so every triangle winds +1 (ray to -x, +y edges
+// count +1); indices outside the sub-path's own vertices (never drawn by a valid stream) or a
+// degenerate triangle produce no edge
+__global__ void k_tri_edges(const Unit *__restrict__ units, uint32_t n_units, const uint32_t *__restrict__ u_toff,
+                            const uint32_t *__restrict__ u_vbeg, const uint32_t *__restrict__ u_vend,
                             uint32_t total_tris, const vkh_draw *__restrict__ draws,
                             const TargetInfo *__restrict__ tgts, const v2 *__restrict__ verts,
                             const uint32_t *__restrict__ tris, Edge *__restrict__ edges, uint32_t *__restrict__ edraw) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total_tris)
         return;
-    uint32_t u = upper_bound_u32(tri_off, n_units, t) - 1;
+    uint32_t u = upper_bound_u32(u_toff, n_units, t) - 1;
     Unit un = units[u];
     const vkh_draw &d = draws[un.draw];
-    uint32_t nv = nverts[u];
-    const v2 *V0 = verts + vert_off[u];
+    uint32_t vb = u_vbeg[u], ve = u_vend[u];
     uint32_t i0 = tris[3 * (size_t)t], i1 = tris[3 * (size_t)t + 1], i2 = tris[3 * (size_t)t + 2];
     Edge *out = edges + 3 * (size_t)t;
     uint32_t *od = edraw + 3 * (size_t)t;
-    bool bad = i0 >= nv || i1 >= nv || i2 >= nv;
+    bool bad = i0 < vb || i1 < vb || i2 < vb || i0 >= ve || i1 >= ve || i2 >= ve;
     int32_t X[3] = {0, 0, 0}, Y[3] = {0, 0, 0};
     if (!bad) {
         float sw = (float)tgts[d.target].width, sh = (float)tgts[d.target].height;
-        snap_vertex(V0[i0].x, V0[i0].y, d.mat, sw, sh, X[0], Y[0]);
-        snap_vertex(V0[i1].x, V0[i1].y, d.mat, sw, sh, X[1], Y[1]);
-        snap_vertex(V0[i2].x, V0[i2].y, d.mat, sw, sh, X[2], Y[2]);
+        snap_vertex(verts[i0].x, verts[i0].y, d.mat, sw, sh, X[0], Y[0]);
+        snap_vertex(verts[i1].x, verts[i1].y, d.mat, sw, sh, X[1], Y[1]);
+        snap_vertex(verts[i2].x, verts[i2].y, d.mat, sw, sh, X[2], Y[2]);
         bad = X[0] == VKH_INVALID_FIX || X[1] == VKH_INVALID_FIX || X[2] == VKH_INVALID_FIX;
     }
     int64_t a2 = 0;
@@ -556,8 +665,10 @@ __global__ void k_tri_edges(const Unit *__restrict__ units, uint32_t n_units, co
     }
     if (a2 > 0) { // positive area winds -1: traverse 0 -> 2 -> 1
         int32_t tx = X[1], ty = Y[1];
-        X[1] = X[2]; Y[1] = Y[2];
-        X[2] = tx; Y[2] = ty;
+        X[1] = X[2];
+        Y[1] = Y[2];
+        X[2] = tx;
+        Y[2] = ty;
     }
     for (int k = 0; k < 3; k++) {
         int k1 = k == 2 ? 0 : k + 1;
@@ -583,20 +694,39 @@ void launch_subpath_ranges(hipStream_t s, const vkh_subpath *sp, uint32_t n, con
         hipLaunchKernelGGL(k_subpath_ranges, dim3((n + 255) / 256), dim3(256), 0, s, sp, n, item_off, first, count);
 }
 void launch_unit_count(hipStream_t s, const Unit *units, uint32_t n_units, const vkh_draw *draws,
-                       const vkh_subpath *sps, const uint32_t *spf, const uint32_t *spc, const v2 *pts,
-                       const uint8_t *pf, const vkh_stroke_style *styles, const float *dashes, uint32_t *fe,
-                       uint32_t *nv, uint32_t *nt) {
+                       const uint32_t *spc, const vkh_stroke_style *styles, uint32_t *fe, uint32_t *steps) {
     if (n_units)
-        hipLaunchKernelGGL(k_unit_count, dim3((n_units + 63) / 64), dim3(64), 0, s, units, n_units, draws, sps, spf,
-                           spc, pts, pf, styles, dashes, fe, nv, nt);
+        hipLaunchKernelGGL(k_unit_count, dim3((n_units + 255) / 256), dim3(256), 0, s, units, n_units, draws, spc,
+                           styles, fe, steps);
 }
-void launch_stroke_emit(hipStream_t s, const Unit *units, uint32_t n_units, const vkh_draw *draws,
-                        const vkh_subpath *sps, const uint32_t *spf, const uint32_t *spc, const v2 *pts,
-                        const uint8_t *pf, const vkh_stroke_style *styles, const float *dashes,
-                        const uint32_t *voff, const uint32_t *toff, v2 *verts, uint32_t *tris) {
+void launch_stroke_steps(hipStream_t s, bool emit, const Unit *units, uint32_t n_units, const uint32_t *step_off,
+                         uint32_t n_steps, const vkh_draw *draws, const vkh_subpath *sps, const uint32_t *spf,
+                         const uint32_t *spc, const v2 *pts, const uint8_t *pf, const vkh_stroke_style *styles,
+                         const float *dashes, uint32_t *step_nv, uint32_t *step_nt, const uint32_t *sv_off,
+                         const uint32_t *st_off, v2 *verts, uint32_t *tris) {
+    if (!n_steps)
+        return;
+    if (emit)
+        hipLaunchKernelGGL(k_stroke_steps<true>, dim3((n_steps + 255) / 256), dim3(256), 0, s, units, n_units, step_off,
+                           n_steps, draws, sps, spf, spc, pts, pf, styles, dashes, step_nv, step_nt, sv_off, st_off,
+                           verts, tris);
+    else
+        hipLaunchKernelGGL(k_stroke_steps<false>, dim3((n_steps + 255) / 256), dim3(256), 0, s, units, n_units,
+                           step_off, n_steps, draws, sps, spf, spc, pts, pf, styles, dashes, step_nv, step_nt, sv_off,
+                           st_off, verts, tris);
+}
+void launch_stroke_patch(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *step_off,
+                         const vkh_draw *draws, const vkh_subpath *sps, const uint32_t *spf, const uint32_t *spc,
+                         const v2 *pts, const uint8_t *pf, const vkh_stroke_style *styles, const float *dashes,
+                         const uint32_t *sv_off, const uint32_t *st_off, uint32_t *tris) {
     if (n_units)
-        hipLaunchKernelGGL(k_stroke_emit, dim3((n_units + 63) / 64), dim3(64), 0, s, units, n_units, draws, sps, spf,
-                           spc, pts, pf, styles, dashes, voff, toff, verts, tris);
+        hipLaunchKernelGGL(k_stroke_patch, dim3((n_units + 255) / 256), dim3(256), 0, s, units, n_units, step_off,
+                           draws, sps, spf, spc, pts, pf, styles, dashes, sv_off, st_off, tris);
+}
+void launch_unit_ranges(hipStream_t s, uint32_t n_units, const uint32_t *step_off, const uint32_t *sv_off,
+                        const uint32_t *st_off, uint32_t *u_toff, uint32_t *u_vbeg, uint32_t *u_vend) {
+    hipLaunchKernelGGL(k_unit_ranges, dim3((n_units + 1 + 255) / 256), dim3(256), 0, s, n_units, step_off, sv_off,
+                       st_off, u_toff, u_vbeg, u_vend);
 }
 void launch_fill_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *fill_off, uint32_t total,
                        const vkh_draw *draws, const TargetInfo *tg, const uint32_t *spf, const uint32_t *spc,
@@ -605,11 +735,11 @@ void launch_fill_edges(hipStream_t s, const Unit *units, uint32_t n_units, const
         hipLaunchKernelGGL(k_fill_edges, dim3((total + 255) / 256), dim3(256), 0, s, units, n_units, fill_off, total,
                            draws, tg, spf, spc, pts, edges, edraw);
 }
-void launch_tri_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *tri_off,
-                      const uint32_t *vert_off, const uint32_t *nverts, uint32_t total_tris, const vkh_draw *draws,
+void launch_tri_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *u_toff,
+                      const uint32_t *u_vbeg, const uint32_t *u_vend, uint32_t total_tris, const vkh_draw *draws,
                       const TargetInfo *tg, const v2 *verts, const uint32_t *tris, Edge *edges, uint32_t *edraw) {
     if (total_tris)
-        hipLaunchKernelGGL(k_tri_edges, dim3((total_tris + 255) / 256), dim3(256), 0, s, units, n_units, tri_off,
-                           vert_off, nverts, total_tris, draws, tg, verts, tris, edges, edraw);
+        hipLaunchKernelGGL(k_tri_edges, dim3((total_tris + 255) / 256), dim3(256), 0, s, units, n_units, u_toff,
+                           u_vbeg, u_vend, total_tris, draws, tg, verts, tris, edges, edraw);
 }
 } // namespace vkh

@@ -53,18 +53,23 @@ void launch_flatten_emit(hipStream_t s, const vkh_item *items, uint32_t n, const
 void launch_subpath_ranges(hipStream_t s, const vkh_subpath *sp, uint32_t n, const uint32_t *item_off,
                            uint32_t *first, uint32_t *count);
 void launch_unit_count(hipStream_t s, const Unit *units, uint32_t n_units, const vkh_draw *draws,
-                       const vkh_subpath *sps, const uint32_t *spf, const uint32_t *spc, const v2 *pts,
-                       const uint8_t *pf, const vkh_stroke_style *styles, const float *dashes, uint32_t *fe,
-                       uint32_t *nv, uint32_t *nt);
-void launch_stroke_emit(hipStream_t s, const Unit *units, uint32_t n_units, const vkh_draw *draws,
-                        const vkh_subpath *sps, const uint32_t *spf, const uint32_t *spc, const v2 *pts,
-                        const uint8_t *pf, const vkh_stroke_style *styles, const float *dashes,
-                        const uint32_t *voff, const uint32_t *toff, v2 *verts, uint32_t *tris);
+                       const uint32_t *spc, const vkh_stroke_style *styles, uint32_t *fe, uint32_t *steps);
+void launch_stroke_steps(hipStream_t s, bool emit, const Unit *units, uint32_t n_units, const uint32_t *step_off,
+                         uint32_t n_steps, const vkh_draw *draws, const vkh_subpath *sps, const uint32_t *spf,
+                         const uint32_t *spc, const v2 *pts, const uint8_t *pf, const vkh_stroke_style *styles,
+                         const float *dashes, uint32_t *step_nv, uint32_t *step_nt, const uint32_t *sv_off,
+                         const uint32_t *st_off, v2 *verts, uint32_t *tris);
+void launch_stroke_patch(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *step_off,
+                         const vkh_draw *draws, const vkh_subpath *sps, const uint32_t *spf, const uint32_t *spc,
+                         const v2 *pts, const uint8_t *pf, const vkh_stroke_style *styles, const float *dashes,
+                         const uint32_t *sv_off, const uint32_t *st_off, uint32_t *tris);
+void launch_unit_ranges(hipStream_t s, uint32_t n_units, const uint32_t *step_off, const uint32_t *sv_off,
+                        const uint32_t *st_off, uint32_t *u_toff, uint32_t *u_vbeg, uint32_t *u_vend);
 void launch_fill_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *fill_off, uint32_t total,
                        const vkh_draw *draws, const TargetInfo *tg, const uint32_t *spf, const uint32_t *spc,
                        const v2 *pts, Edge *edges, uint32_t *edraw);
-void launch_tri_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *tri_off,
-                      const uint32_t *vert_off, const uint32_t *nverts, uint32_t total_tris, const vkh_draw *draws,
+void launch_tri_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *u_toff,
+                      const uint32_t *u_vbeg, const uint32_t *u_vend, uint32_t total_tris, const vkh_draw *draws,
                       const TargetInfo *tg, const v2 *verts, const uint32_t *tris, Edge *edges, uint32_t *edraw);
 
 // raster stage (vkh_raster.hip)

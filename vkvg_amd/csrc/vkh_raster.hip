@@ -403,6 +403,12 @@ __device__ __forceinline__ uint32_t blend_px(uint32_t op, uint32_t d, uint32_t s
     return r;
 }
 
+// One workgroup per T x T tile. Commands are processed in batches of B: the batch's command,
+// draw and path-tile records are staged in LDS with one parallel load; the (edge, tile row)
+// crossings of all the batch's commands are accumulated in parallel into per-command LDS row
+// deltas; each row is prefix-summed once; then every pixel applies the batch's commands in
+// order. The dependent global loads (command -> path tile -> segment -> edge) are thus paid once
+// per batch instead of once per command.
 template <int T>
 __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg, uint32_t n_targets,
                                               const vkh_draw *__restrict__ draws,
@@ -411,30 +417,35 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
                                               const Edge *__restrict__ edges, uint32_t n_edges,
                                               const uint32_t *__restrict__ tile_off, const Cmd *__restrict__ cmds) {
     constexpr int NPIX = T * T;
-    constexpr int PPT = NPIX > 256 ? NPIX / 256 : 1; // pixels per thread
-    constexpr int NTHR = NPIX > 256 ? 256 : NPIX;
-    __shared__ int32_t delta[T][T];
-    __shared__ int32_t rowb[T];
+    constexpr int NT = NPIX >= 256 ? 256 : NPIX;  // threads
+    constexpr int PPT = NPIX / NT;                // pixels per thread
+    constexpr int B = 8192 / NPIX;                // commands per batch (32 KB of row deltas)
+    __shared__ int32_t sd[B][T][T];               // row deltas, then per-row inclusive prefix
+    __shared__ int32_t srow[B][T];                // V term per row
+    __shared__ uint32_t c_item[B + 1];            // prefix of (edge, row) items
+    __shared__ uint32_t c_nseg[B], c_segoff[B], c_kind[B], c_op[B], c_color[B], c_key[B];
+    __shared__ int32_t c_back[B], c_grad[B];
+    __shared__ float c_opa[B];
+
     uint32_t gtile = blockIdx.x;
     uint32_t c0 = tile_off[gtile], c1 = tile_off[gtile + 1];
     if (c0 == c1)
         return;
-    uint32_t tid = threadIdx.x;
-    if (tid >= NTHR)
-        return;
+    const uint32_t tid = threadIdx.x;
     uint32_t t = find_target_by(tg, n_targets, gtile, false);
     const TargetInfo &TI = tg[t];
     uint32_t lt = gtile - TI.tile_base;
     int32_t tx = (int32_t)(lt % TI.tilesW), ty = (int32_t)(lt / TI.tilesW);
     int32_t X0 = tx * T, Y0 = ty * T;
     float Wf = (float)TI.width, Hf = (float)TI.height;
+    const int32_t xs0 = 256 * X0 + 128, s0y = 256 * Y0 + 128;
 
     int lx[PPT], ly[PPT];
     bool valid[PPT];
     uint32_t px[PPT], sig[PPT];
 #pragma unroll
     for (int q = 0; q < PPT; q++) {
-        int p = (int)tid + q * NTHR;
+        int p = (int)tid + q * NT;
         lx[q] = p % T;
         ly[q] = p / T;
         valid[q] = (X0 + lx[q]) < (int32_t)TI.width && (Y0 + ly[q]) < (int32_t)TI.height;
@@ -445,7 +456,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
     {
         Cmd f = cmds[c0];
         const vkh_draw &d0 = draws[f.draw];
-        if (f.pt == VKH_NO_DRAW) { // paint / clear: every pixel of the tile
+        if (f.pt == VKH_NO_DRAW) {
             if (d0.kind == VKH_DRAW_CLEAR || d0.op == 0)
                 need_read = false;
             else if (d0.grad < 0 && (d0.op == 1 || d0.op == 2) && d0.opacity == 1.0f && (d0.color >> 24) == 255u)
@@ -459,86 +470,133 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
         px[q] = (need_read && valid[q]) ? img[o] : 0u;
     }
 
-    for (uint32_t ci = c0; ci < c1; ci++) {
-        Cmd cm = cmds[ci];
-        const vkh_draw &d = draws[cm.draw];
-        int32_t w[PPT];
-        bool full = cm.pt == VKH_NO_DRAW;
-        if (!full) {
-            PathTile p = pts[cm.pt];
-#pragma unroll
-            for (int q = 0; q < PPT; q++)
-                w[q] = p.backdrop;
-            if (p.nseg) {
-                for (int k = (int)tid; k < NPIX; k += NTHR)
-                    (&delta[0][0])[k] = 0;
-                if ((int)tid < T)
-                    rowb[tid] = 0;
-                __syncthreads();
-                uint32_t items = p.nseg * (uint32_t)T;
-                int32_t xs0 = 256 * X0 + 128;
-                int32_t s0y = 256 * Y0 + 128;
-                for (uint32_t it = tid; it < items; it += NTHR) {
-                    uint32_t r = it % T;
-                    uint32_t ei = segs[p.seg_off + it / T];
-                    if (ei >= n_edges)
-                        continue;
-                    Edge e = edges[ei];
-                    int32_t sy = 256 * (Y0 + (int32_t)r) + 128;
-                    int32_t ytop = min(e.y0, e.y1), ybot = max(e.y0, e.y1);
-                    if (ytop <= sy && sy < ybot) {
-                        bool down = e.y0 < e.y1;
-                        int64_t c = down ? crossing_column(e.x0, e.y0, e.x1, e.y1, sy)
-                                         : crossing_column(e.x1, e.y1, e.x0, e.y0, sy);
-                        int64_t lc = c - X0;
-                        if (lc >= 1 && lc < T)
-                            atomicAdd(&delta[r][lc], down ? 1 : -1);
-                    }
-                    if (r > 0 && min(e.x0, e.x1) <= xs0 && xs0 < max(e.x0, e.x1)) {
-                        if (point_above(e, xs0, s0y) && !point_above(e, xs0, sy))
-                            atomicAdd(&rowb[r], e.x1 > e.x0 ? -1 : 1);
-                    }
-                }
-                __syncthreads();
-#pragma unroll
-                for (int q = 0; q < PPT; q++) {
-                    int32_t acc = rowb[ly[q]];
-                    for (int c = 1; c <= lx[q]; c++)
-                        acc += delta[ly[q]][c];
-                    w[q] += acc;
-                }
-                __syncthreads();
+    for (uint32_t base = c0; base < c1; base += B) {
+        const uint32_t nb = min((uint32_t)B, c1 - base);
+        // ---- stage the batch's records
+        for (uint32_t j = tid; j < nb; j += NT) {
+            Cmd cm = cmds[base + j];
+            const vkh_draw &d = draws[cm.draw];
+            uint32_t kind = d.kind;
+            c_kind[j] = (cm.pt == VKH_NO_DRAW) ? (kind == VKH_DRAW_CLEAR ? VKH_DRAW_CLEAR : VKH_DRAW_PAINT) : kind;
+            c_op[j] = kind == VKH_DRAW_CLEAR ? 0u : d.op;
+            c_color[j] = d.color;
+            c_grad[j] = d.grad;
+            c_opa[j] = d.opacity;
+            c_key[j] = draw_key(d.key_index);
+            if (cm.pt == VKH_NO_DRAW) {
+                c_back[j] = 0;
+                c_nseg[j] = 0;
+                c_segoff[j] = 0;
+            } else {
+                PathTile p = pts[cm.pt];
+                c_back[j] = p.backdrop;
+                c_nseg[j] = p.nseg;
+                c_segoff[j] = p.seg_off;
             }
         }
-        uint32_t key = draw_key(d.key_index);
-        uint32_t solid = 0;
-        if (d.grad < 0) {
-            float op = d.opacity;
-            uint32_t c = d.color;
-            solid = (uint32_t)f2u8(((float)(c & 0xFF) / 255.0f) * op) |
-                    ((uint32_t)f2u8(((float)((c >> 8) & 0xFF) / 255.0f) * op) << 8) |
-                    ((uint32_t)f2u8(((float)((c >> 16) & 0xFF) / 255.0f) * op) << 16) |
-                    ((uint32_t)f2u8(((float)((c >> 24) & 0xFF) / 255.0f) * op) << 24);
+        for (uint32_t k = tid; k < nb * (uint32_t)NPIX; k += NT)
+            (&sd[0][0][0])[k] = 0;
+        for (uint32_t k = tid; k < nb * (uint32_t)T; k += NT)
+            (&srow[0][0])[k] = 0;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t acc = 0;
+            for (uint32_t j = 0; j < nb; j++) {
+                c_item[j] = acc;
+                acc += c_nseg[j] * (uint32_t)T;
+            }
+            c_item[nb] = acc;
         }
-#pragma unroll
-        for (int q = 0; q < PPT; q++) {
-            int k = full ? 1 : coverage_count(d.kind, w[q]);
-            if (k <= 0 || !valid[q])
+        __syncthreads();
+        // ---- (edge, row) crossings of all the batch's commands
+        const uint32_t total = c_item[nb];
+        for (uint32_t it = tid; it < total; it += NT) {
+            uint32_t lo = 0, hi = nb; // last j with c_item[j] <= it
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (c_item[mid] <= it)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const uint32_t j = lo;
+            const uint32_t local = it - c_item[j];
+            const uint32_t r = local % T;
+            const uint32_t ei = segs[c_segoff[j] + local / T];
+            if (ei >= n_edges)
                 continue;
-            uint32_t s = solid;
-            if (d.grad >= 0) {
-                float col[4];
-                gradient_color(grads[d.grad], Wf, Hf, (float)(X0 + lx[q]) + 0.5f, (float)(Y0 + ly[q]) + 0.5f, col);
-                s = (uint32_t)f2u8(col[0] * d.opacity) | ((uint32_t)f2u8(col[1] * d.opacity) << 8) |
-                    ((uint32_t)f2u8(col[2] * d.opacity) << 16) | ((uint32_t)f2u8(col[3] * d.opacity) << 24);
+            const Edge e = edges[ei];
+            const int32_t sy = 256 * (Y0 + (int32_t)r) + 128;
+            const int32_t ytop = min(e.y0, e.y1), ybot = max(e.y0, e.y1);
+            if (ytop <= sy && sy < ybot) {
+                const bool down = e.y0 < e.y1;
+                const int64_t c = down ? crossing_column(e.x0, e.y0, e.x1, e.y1, sy)
+                                       : crossing_column(e.x1, e.y1, e.x0, e.y0, sy);
+                const int64_t lc = c - X0;
+                if (lc >= 1 && lc < T)
+                    atomicAdd(&sd[j][r][lc], down ? 1 : -1);
             }
-            uint32_t v = px[q];
-            uint32_t op = d.kind == VKH_DRAW_CLEAR ? 0u : d.op;
-            for (int j = 0; j < k; j++)
-                v = blend_px(op, v, s);
-            px[q] = v;
-            sig[q] += (uint32_t)k * key;
+            if (r > 0 && min(e.x0, e.x1) <= xs0 && xs0 < max(e.x0, e.x1)) {
+                if (point_above(e, xs0, s0y) && !point_above(e, xs0, sy))
+                    atomicAdd(&srow[j][r], e.x1 > e.x0 ? -1 : 1);
+            }
         }
+        __syncthreads();
+        // ---- per (command, row): inclusive prefix with the row's V term
+        for (uint32_t q = tid; q < nb * (uint32_t)T; q += NT) {
+            const uint32_t j = q / T, r = q % T;
+            if (!c_nseg[j])
+                continue;
+            int32_t acc = srow[j][r];
+#pragma unroll
+            for (int c = 0; c < T; c++) {
+                acc += sd[j][r][c];
+                sd[j][r][c] = acc;
+            }
+        }
+        __syncthreads();
+        // ---- ordered compositing
+        for (uint32_t j = 0; j < nb; j++) {
+            const uint32_t kind = c_kind[j];
+            const bool full = kind == VKH_DRAW_PAINT || kind == VKH_DRAW_CLEAR;
+            const bool hasseg = c_nseg[j] != 0;
+            const int32_t bk = c_back[j];
+            const int32_t g = c_grad[j];
+            const float op_ = c_opa[j];
+            const uint32_t op = c_op[j];
+            const uint32_t key = c_key[j];
+            uint32_t solid = 0;
+            if (g < 0) {
+                const uint32_t c = c_color[j];
+                solid = (uint32_t)f2u8(((float)(c & 0xFF) / 255.0f) * op_) |
+                        ((uint32_t)f2u8(((float)((c >> 8) & 0xFF) / 255.0f) * op_) << 8) |
+                        ((uint32_t)f2u8(((float)((c >> 16) & 0xFF) / 255.0f) * op_) << 16) |
+                        ((uint32_t)f2u8(((float)((c >> 24) & 0xFF) / 255.0f) * op_) << 24);
+            }
+#pragma unroll
+            for (int q = 0; q < PPT; q++) {
+                int k = 1;
+                if (!full) {
+                    const int32_t w = bk + (hasseg ? sd[j][ly[q]][lx[q]] : 0);
+                    k = coverage_count(kind, w);
+                }
+                if (k <= 0 || !valid[q])
+                    continue;
+                uint32_t s = solid;
+                if (g >= 0) {
+                    float col[4];
+                    gradient_color(grads[g], Wf, Hf, (float)(X0 + lx[q]) + 0.5f, (float)(Y0 + ly[q]) + 0.5f, col);
+                    s = (uint32_t)f2u8(col[0] * op_) | ((uint32_t)f2u8(col[1] * op_) << 8) |
+                        ((uint32_t)f2u8(col[2] * op_) << 16) | ((uint32_t)f2u8(col[3] * op_) << 24);
+                }
+                uint32_t v = px[q];
+                for (int jj = 0; jj < k; jj++)
+                    v = blend_px(op, v, s);
+                px[q] = v;
+                sig[q] += (uint32_t)k * key;
+            }
+        }
+        __syncthreads();
     }
 #pragma unroll
     for (int q = 0; q < PPT; q++) {

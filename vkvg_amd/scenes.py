@@ -223,3 +223,116 @@ def config4(variant: int = 0, n_surfaces: int = 256, side_lo: int = 64, side_hi:
         cps, n = _c2_content(rng, s, side, side, n_lines, n_stars)
         out.append(Scene(f"c4_{i}", side, side, s, n, ctrl_points=cps))
     return out
+
+
+def feature_mix(variant: int = 0, size: int = 384, n: int = 60) -> Scene:
+    """Exercises the API surface beyond the benchmark configs: closed and open strokes with every
+    join / cap, dash offsets, arcs, ellipses, rounded rectangles, quadratic and relative curves,
+    transforms, opacity, the CLEAR and DIFFERENCE operators, fill_preserve + stroke, paint with a
+    pending path, clear."""
+    rng = Rng(0x5EED0000 + 16 * 9 + variant)
+    s = Stream()
+    _white(s)
+    cps = 0
+    for i in range(n):
+        kind = i % 12
+        col = rng.u(3)
+        s.set_source_rgba(col[0], col[1], col[2], rng.uniform(0.3, 1.0))
+        s.set_line_width(float(rng.loguniform(0.7, 14.0)))
+        s.set_line_join(int(rng.randint(0, 2)))
+        s.set_line_cap(int(rng.randint(0, 2)))
+        s.set_miter_limit(float(rng.uniform(1.5, 12.0)))
+        cx, cy = rng.uniform(0, size), rng.uniform(0, size)
+        r = rng.loguniform(5, size / 3)
+        if kind == 0:  # closed polygon stroke (closing join + index patch)
+            m = rng.randint(3, 9)
+            ang = np.sort(rng.uniform(0, 2 * math.pi, m))
+            s.polyline(np.stack([cx + r * np.cos(ang), cy + r * np.sin(ang)], 1), close=True)
+            s.stroke()
+            cps += m
+        elif kind == 1:  # dashed closed / open with offset
+            s.set_dash([float(rng.uniform(2, 12)), float(rng.uniform(1, 8))], float(rng.uniform(0, 10)))
+            m = rng.randint(2, 7)
+            s.polyline(np.stack([rng.uniform(0, size, m), rng.uniform(0, size, m)], 1), close=bool(i % 2))
+            s.stroke()
+            s.set_dash([], 0.0)
+            cps += m
+        elif kind == 2:  # arc fill + arc_negative stroke
+            s.arc(cx, cy, r, float(rng.uniform(0, 6)), float(rng.uniform(0, 6)))
+            s.fill()
+            s.arc_negative(cx, cy, r * 0.7, float(rng.uniform(0, 6)), float(rng.uniform(0, 6)))
+            s.stroke()
+        elif kind == 3:  # ellipse fill_preserve + stroke
+            s.ellipse(r, r * rng.uniform(0.3, 1.0), cx, cy, float(rng.uniform(0, 3)))
+            s.fill_preserve()
+            s.set_source_rgba(0, 0, 0, 0.8)
+            s.stroke()
+        elif kind == 4:  # rounded rectangle, even-odd
+            s.set_fill_rule(FILL_EVEN_ODD)
+            s.rounded_rectangle(cx - r, cy - r / 2, 2 * r, r, r * 0.3)
+            s.rectangle(cx - r / 2, cy - r / 4, r, r / 2)
+            s.fill()
+            s.set_fill_rule(FILL_NON_ZERO)
+        elif kind == 5:  # quadratic + relative curves, open stroke
+            s.move_to(cx, cy)
+            s.quadratic_to(cx + r, cy - r, cx + 2 * r, cy)
+            s.rel_curve_to(r, r, -r, r, 0, 2 * r)
+            s.rel_line_to(-r, 0)
+            s.stroke()
+            cps += 8
+        elif kind == 6:  # transformed gradient fill
+            s.translate(cx, cy)
+            s.rotate(float(rng.uniform(0, 3)))
+            s.scale(float(rng.uniform(0.5, 2)), float(rng.uniform(0.5, 2)))
+            s.set_linear(-r, 0, r, 0, [[0, 1, 0, 0, 0.9], [0.5, 0, 1, 0, 0.6], [1, 0, 0, 1, 0.9]])
+            s.rectangle(-r / 2, -r / 2, r, r)
+            s.fill()
+            s.identity_matrix()
+        elif kind == 7:  # radial gradient stroke with opacity
+            s.set_opacity(float(rng.uniform(0.3, 0.9)))
+            s.set_radial(cx, cy, 0, cx, cy, r, [[0, 1, 1, 0, 1], [1, 0.2, 0, 0.6, 0.5]])
+            s.arc(cx, cy, r * 0.8, 0, 6.2831853)
+            s.stroke()
+            s.set_opacity(1.0)
+        elif kind == 8:  # DIFFERENCE then CLEAR operator on small shapes
+            s.set_operator(3)
+            s.rectangle(cx, cy, r / 2, r / 3)
+            s.fill()
+            s.set_operator(0)
+            s.rectangle(cx + 2, cy + 2, r / 5, r / 5)
+            s.fill()
+            s.set_operator(2)
+        elif kind == 9:  # self-intersecting cubic loop, non-zero
+            pts = rng.uniform(0, size, 2 + 6 * 3).astype(np.float32)
+            s.curves(pts[:2], pts[2:].reshape(3, 6), close=True)
+            s.fill()
+            cps += 10
+        elif kind == 10:  # elliptic arcs with curve-forced joins, stroked closed
+            s.move_to(cx, cy)
+            s.elliptic_arc_to(cx + r, cy + r / 2, bool(i % 2), bool(i % 3), r, r / 2, float(rng.uniform(0, 1)))
+            s.line_to(cx, cy + r)
+            s.close_path()
+            s.stroke()
+        else:  # paint with a pending path = fill; new sub paths
+            s.move_to(cx, cy)
+            s.line_to(cx + r, cy)
+            s.line_to(cx + r / 2, cy + r)
+            s.new_sub_path()
+            s.move_to(cx + r / 3, cy + r / 4)
+            s.line_to(cx + r, cy + r)
+            s.line_to(cx, cy + r / 2)
+            s.paint()
+    return Scene("feature_mix", size, size, s, n, ctrl_points=cps)
+
+
+def clear_scene(size: int = 96) -> Scene:
+    s = Stream()
+    _white(s)
+    s.set_source_rgba(1, 0, 0, 1)
+    s.rectangle(10, 10, 30, 30)
+    s.fill()
+    s.clear()
+    s.set_source_rgba(0, 0, 1, 0.5)
+    s.rectangle(20, 20, 40, 40)
+    s.fill()
+    return Scene("clear", size, size, s, 2)
