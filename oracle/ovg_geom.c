@@ -981,8 +981,10 @@ void o_stroke_geometry(octx *c) {
             stroke_cap(c, &s, c->points[s.cp], vline_norm(c->points[s.cp], c->points[s.cp + 1]), true);
             s.iL = s.cp++;
         }
+        bool walked = false;
         if (curved_path) {
             while (s.cp < lastIdx) {
+                walked = true;
                 bool curved = (c->pathes[ptrPath + ptrSegment] & P_CURVES) != 0;
                 if (lastSegIdx == lastIdx)
                     lastSegIdx--;
@@ -1036,7 +1038,18 @@ void o_stroke_geometry(octx *c) {
             stroke_cap(c, &s, c->points[s.cp], vline_norm(c->points[s.cp - 1], c->points[s.cp]), false);
         }
         s.cp = firstIdx + count;
-        if (ptrSegment > 0)
+        if (curved_path && !walked) {
+            /* Deviation: an open, solid, curved sub-path of two points never enters the segment
+             * walk above, so the reference (src/vkvg_context.c:935-938) advances by one record
+             * and strokes the curve-segment record as a further sub-path made of the points that
+             * follow in its point buffer: leftovers of earlier paths or uninitialised memory.
+             * Here the sub-path's segment records are skipped the way the fill walks them
+             * (src/vkvg_context_internal.c:1645-1652). */
+            uint32_t q = ptrPath + 1, tot = 0;
+            while (tot < count && q < c->pathPtr)
+                tot += c->pathes[q++] & P_MASK;
+            ptrPath = q;
+        } else if (ptrSegment > 0)
             ptrPath += ptrSegment;
         else
             ptrPath++;

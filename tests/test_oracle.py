@@ -322,3 +322,30 @@ def test_oracle_renders_seeded_scenes_deterministically():
     a = ovg.render(128, 128, sc.words())
     b = ovg.render(128, 128, sc.words())
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def _short_arc_scene(with_previous_path: bool) -> np.ndarray:
+    s = Stream()
+    s.set_line_width(7.86)
+    s.set_line_cap(1)
+    if with_previous_path:
+        s.set_source_rgba(0, 0, 1, 1)
+        s.arc(199.2, 101.6, 16.4, 5.839, 2.350)  # leaves ~18 flattened points in the point buffer
+        s.fill()
+    s.set_source_rgba(1, 0, 0, 1)
+    s.arc_negative(199.2, 101.6, 11.49, 2.733, 2.590)  # shorter than one arc step: two points
+    s.stroke()
+    return s.words()
+
+
+def test_two_point_curved_stroke_ignores_earlier_paths():
+    # An open two-point curved sub-path is stroked once (the reference would also stroke its
+    # curve-segment record as a sub-path of stale points); the stroke must not depend on what
+    # earlier paths left in the point buffer.
+    _, sig_a, st_a, _ = ovg.render(256, 160, _short_arc_scene(True))
+    _, sig_b, st_b, _ = ovg.render(256, 160, _short_arc_scene(False))
+    k0, k1 = ovg.draw_key(0), ovg.draw_key(1)
+    assert st_a["triangles"] == st_b["triangles"] == 8  # two round caps; the fill adds none
+    mask_b = sig_b == k0  # the stroke is draw 0 alone, draw 1 after the fill
+    mask_a = (sig_a == k1) | (sig_a == ((k0 + k1) & 0xFFFFFFFF))
+    assert mask_b.sum() > 0 and np.array_equal(mask_a, mask_b)

@@ -404,11 +404,11 @@ __device__ __forceinline__ uint32_t blend_px(uint32_t op, uint32_t d, uint32_t s
 }
 
 // One workgroup per T x T tile. Commands are processed in batches of B: the batch's command,
-// draw and path-tile records are staged in LDS with one parallel load; the (edge, tile row)
-// crossings of all the batch's commands are accumulated in parallel into per-command LDS row
-// deltas; each row is prefix-summed once; then every pixel applies the batch's commands in
-// order. The dependent global loads (command -> path tile -> segment -> edge) are thus paid once
-// per batch instead of once per command.
+// draw and path-tile records are staged in LDS with one parallel load; every segment of the batch
+// (one thread each) adds its edge's crossings of the tile's sample rows into per-command LDS row
+// deltas, walking only the rows the edge spans; each row is prefix-summed once; then every pixel
+// applies the batch's commands in order. The dependent global loads (command -> path tile ->
+// segment -> edge) are thus paid once per batch and once per segment.
 template <int T>
 __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg, uint32_t n_targets,
                                               const vkh_draw *__restrict__ draws,
@@ -421,8 +421,8 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
     constexpr int PPT = NPIX / NT;                // pixels per thread
     constexpr int B = 8192 / NPIX;                // commands per batch (32 KB of row deltas)
     __shared__ int32_t sd[B][T][T];               // row deltas, then per-row inclusive prefix
-    __shared__ int32_t srow[B][T];                // V term per row
-    __shared__ uint32_t c_item[B + 1];            // prefix of (edge, row) items
+    __shared__ int32_t srow[B][T];                // V term: +-1 at the first row below a column crossing
+    __shared__ uint32_t c_item[B + 1];            // prefix of the commands' segment counts
     __shared__ uint32_t c_nseg[B], c_segoff[B], c_kind[B], c_op[B], c_color[B], c_key[B];
     __shared__ int32_t c_back[B], c_grad[B];
     __shared__ float c_opa[B];
@@ -503,12 +503,13 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
             uint32_t acc = 0;
             for (uint32_t j = 0; j < nb; j++) {
                 c_item[j] = acc;
-                acc += c_nseg[j] * (uint32_t)T;
+                acc += c_nseg[j];
             }
             c_item[nb] = acc;
         }
         __syncthreads();
-        // ---- (edge, row) crossings of all the batch's commands
+        // ---- crossings: one thread per segment of the batch, walking only the sample rows its
+        // edge spans inside the tile
         const uint32_t total = c_item[nb];
         for (uint32_t it = tid; it < total; it += NT) {
             uint32_t lo = 0, hi = nb; // last j with c_item[j] <= it
@@ -520,25 +521,48 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
                     hi = mid;
             }
             const uint32_t j = lo;
-            const uint32_t local = it - c_item[j];
-            const uint32_t r = local % T;
-            const uint32_t ei = segs[c_segoff[j] + local / T];
+            const uint32_t ei = segs[c_segoff[j] + (it - c_item[j])];
             if (ei >= n_edges)
                 continue;
             const Edge e = edges[ei];
-            const int32_t sy = 256 * (Y0 + (int32_t)r) + 128;
-            const int32_t ytop = min(e.y0, e.y1), ybot = max(e.y0, e.y1);
-            if (ytop <= sy && sy < ybot) {
-                const bool down = e.y0 < e.y1;
-                const int64_t c = down ? crossing_column(e.x0, e.y0, e.x1, e.y1, sy)
-                                       : crossing_column(e.x1, e.y1, e.x0, e.y0, sy);
-                const int64_t lc = c - X0;
-                if (lc >= 1 && lc < T)
-                    atomicAdd(&sd[j][r][lc], down ? 1 : -1);
+            // V term: the edge separates (xs0, s0y) from (xs0, sy(r)) for every row r >= r*
+            if (min(e.x0, e.x1) <= xs0 && xs0 < max(e.x0, e.x1) && point_above(e, xs0, s0y)) {
+                int32_t a = 1, b = T; // first r in [1, T) with !point_above(sy(r)), T if none
+                while (a < b) {
+                    int32_t m = (a + b) >> 1;
+                    if (point_above(e, xs0, s0y + 256 * m))
+                        a = m + 1;
+                    else
+                        b = m;
+                }
+                if (a < T)
+                    atomicAdd(&srow[j][a], e.x1 > e.x0 ? -1 : 1);
             }
-            if (r > 0 && min(e.x0, e.x1) <= xs0 && xs0 < max(e.x0, e.x1)) {
-                if (point_above(e, xs0, s0y) && !point_above(e, xs0, sy))
-                    atomicAdd(&srow[j][r], e.x1 > e.x0 ? -1 : 1);
+            // H term: rows with ytop <= sy(r) < ybot
+            const bool down = e.y0 < e.y1;
+            const int32_t xt = down ? e.x0 : e.x1, yt = down ? e.y0 : e.y1;
+            const int32_t xb = down ? e.x1 : e.x0, yb = down ? e.y1 : e.y0;
+            const int32_t r_lo = max(((yt - 128 + 255) >> 8) - Y0, 0);
+            const int32_t r_hi = min(((yb - 128 + 255) >> 8) - 1 - Y0, T - 1);
+            if (r_lo > r_hi)
+                continue;
+            const int32_t sgn = down ? 1 : -1;
+            int64_t q, rem, dq, dr;
+            crossing_dda(xt, yt, xb, yb, s0y + 256 * r_lo, q, rem, dq, dr);
+            const int64_t D = 256 * ((int64_t)yb - yt);
+            for (int32_t r = r_lo;; r++) {
+                const int64_t lc = q - X0;
+                if (lc >= 1 && lc < T)
+                    atomicAdd(&sd[j][r][lc], sgn);
+                if (r == r_hi)
+                    break;
+                q += dq;
+                if (dr > rem) {
+                    q += 1;
+                    rem += D - dr;
+                } else {
+                    rem -= dr;
+                }
             }
         }
         __syncthreads();
@@ -547,7 +571,9 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
             const uint32_t j = q / T, r = q % T;
             if (!c_nseg[j])
                 continue;
-            int32_t acc = srow[j][r];
+            int32_t acc = 0;
+            for (uint32_t rr = 1; rr <= r; rr++)
+                acc += srow[j][rr];
 #pragma unroll
             for (int c = 0; c < T; c++) {
                 acc += sd[j][r][c];
