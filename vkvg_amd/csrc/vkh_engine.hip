@@ -82,7 +82,7 @@ struct vkh_engine {
 
     DevBuf blob, units, tgts, item_cnt, item_off, pts, pflags, sp_first, sp_count;
     DevBuf fe, nv, nt, fe_off, nv_off, nt_off, verts, tris, edges, edraw;
-    DevBuf bbox, dtiles, pt_count, pt_off, row_cnt, row_off, ptiles, nseg, seg_off, segs;
+    DevBuf bbox, erange, dtiles, pt_count, pt_off, ptiles, seg_alloc, segs;
     DevBuf tile_cnt, tile_off, cmds, scan_tmp, readback;
     DevBuf step_nv, step_nt, sv_off, st_off, u_toff, u_vbeg, u_vend;
     HostBuf hblob, htot;
@@ -230,7 +230,7 @@ void vkvg_hip_engine_destroy(vkh_engine *e) {
     DevBuf *bufs[] = {&e->blob, &e->units, &e->tgts, &e->item_cnt, &e->item_off, &e->pts, &e->pflags,
                       &e->sp_first, &e->sp_count, &e->fe, &e->nv, &e->nt, &e->fe_off, &e->nv_off, &e->nt_off,
                       &e->verts, &e->tris, &e->edges, &e->edraw, &e->bbox, &e->dtiles, &e->pt_count, &e->pt_off,
-                      &e->row_cnt, &e->row_off, &e->ptiles, &e->nseg, &e->seg_off, &e->segs, &e->tile_cnt,
+                      &e->erange, &e->ptiles, &e->seg_alloc, &e->segs, &e->tile_cnt,
                       &e->tile_off, &e->cmds, &e->scan_tmp, &e->readback, &e->step_nv, &e->step_nt,
                       &e->sv_off, &e->st_off, &e->u_toff, &e->u_vbeg, &e->u_vend};
     for (DevBuf *b : bufs)
@@ -572,47 +572,47 @@ int vkvg_hip_execute(vkh_engine *eng) {
     // ---- tiles of every draw
     size_t dn4 = 4ull * (n_draws + 1);
     if (!ensure(eng, eng->bbox, sizeof(int4) * (n_draws + 1)) || !ensure(eng, eng->dtiles, sizeof(DrawTiles) * (n_draws + 1)) ||
-        !ensure(eng, eng->pt_count, dn4) || !ensure(eng, eng->pt_off, dn4) || !ensure(eng, eng->row_cnt, dn4) ||
-        !ensure(eng, eng->row_off, dn4))
+        !ensure(eng, eng->erange, sizeof(uint2) * (n_draws + 1)) || !ensure(eng, eng->pt_count, dn4) ||
+        !ensure(eng, eng->pt_off, dn4) || !ensure(eng, eng->seg_alloc, 256))
         return -1;
-    vkh::launch_draw_bbox_init(s, P<int4>(eng->bbox), n_draws);
-    vkh::launch_draw_bbox(s, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), n_edges, P<int4>(eng->bbox));
+    vkh::launch_draw_bbox_init(s, P<int4>(eng->bbox), P<uint2>(eng->erange), n_draws);
+    vkh::launch_draw_bbox(s, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), n_edges, P<int4>(eng->bbox),
+                          P<uint2>(eng->erange));
     vkh::launch_draw_tiles(s, d_draws, n_draws, d_tg, P<int4>(eng->bbox), T, P<DrawTiles>(eng->dtiles),
-                           P<uint32_t>(eng->pt_count), P<uint32_t>(eng->row_cnt));
-    if (scan_u32(eng, P<uint32_t>(eng->pt_count), n_draws, P<uint32_t>(eng->pt_off)) ||
-        scan_u32(eng, P<uint32_t>(eng->row_cnt), n_draws, P<uint32_t>(eng->row_off)))
+                           P<uint32_t>(eng->pt_count));
+    if (scan_u32(eng, P<uint32_t>(eng->pt_count), n_draws, P<uint32_t>(eng->pt_off)))
         return -1;
-    if (read_totals(eng, {P<uint32_t>(eng->pt_off) + n_draws, P<uint32_t>(eng->row_off) + n_draws}, tot))
+    if (read_totals(eng, {P<uint32_t>(eng->pt_off) + n_draws}, tot))
         return -1;
-    const uint32_t n_ptiles = tot[0], n_rows = tot[1];
+    const uint32_t n_ptiles = tot[0];
     st.path_tiles = n_ptiles;
     vkh::launch_set_pt_base(s, P<DrawTiles>(eng->dtiles), P<uint32_t>(eng->pt_off), n_draws);
-    if (!ensure(eng, eng->ptiles, sizeof(PathTile) * (size_t)n_ptiles + 16) ||
-        !ensure(eng, eng->nseg, 4ull * (n_ptiles + 1)) || !ensure(eng, eng->seg_off, 4ull * (n_ptiles + 1)))
+    if (!ensure(eng, eng->ptiles, sizeof(PathTile) * (size_t)n_ptiles + 16) || !ensure(eng, eng->segs, 4096))
         return -1;
     CK(hipEventRecord(eng->ev_stage[2], s)); // tile setup done
-    if (n_ptiles)
-        CK(hipMemsetAsync(eng->ptiles.p, 0, sizeof(PathTile) * (size_t)n_ptiles, s));
-    vkh::launch_edge_bin(s, false, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), n_edges, P<DrawTiles>(eng->dtiles),
-                         T, P<PathTile>(eng->ptiles), nullptr);
-    if (n_draws)
-        vkh::launch_backdrop_prefix(s, P<DrawTiles>(eng->dtiles), P<uint32_t>(eng->row_off), n_draws, n_rows,
-                                    P<PathTile>(eng->ptiles));
-    CK(hipGetLastError());
-    // segment lists
-    vkh::launch_nseg_out(s, P<PathTile>(eng->ptiles), P<uint32_t>(eng->nseg), n_ptiles);
-    if (scan_u32(eng, P<uint32_t>(eng->nseg), n_ptiles, P<uint32_t>(eng->seg_off)))
-        return -1;
-    if (read_totals(eng, {P<uint32_t>(eng->seg_off) + n_ptiles}, tot))
-        return -1;
-    const uint32_t n_segs = tot[0];
+    // ---- binning: per-draw segment lists and backdrops; bins again once if the segment buffer
+    // was too small (the allocator then holds the exact total)
+    uint32_t n_segs = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        const uint32_t cap = (uint32_t)std::min<size_t>(eng->segs.cap / 4, 0xFFFFFFFFull);
+        vkh::launch_zero_u32(s, P<uint32_t>(eng->seg_alloc), 1);
+        vkh::launch_bin_draws(s, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), P<uint2>(eng->erange),
+                              P<DrawTiles>(eng->dtiles), n_draws, T, P<PathTile>(eng->ptiles), P<uint32_t>(eng->segs),
+                              cap, P<uint32_t>(eng->seg_alloc));
+        CK(hipGetLastError());
+        if (read_totals(eng, {P<uint32_t>(eng->seg_alloc)}, tot))
+            return -1;
+        n_segs = tot[0];
+        if (n_segs <= cap)
+            break;
+        if (pass == 1) {
+            eng->err = "segment buffer overflow after resize";
+            return -1;
+        }
+        if (!ensure(eng, eng->segs, 4ull * n_segs + 4))
+            return -1;
+    }
     st.segments = n_segs;
-    if (!ensure(eng, eng->segs, 4ull * n_segs + 4))
-        return -1;
-    vkh::launch_seg_offsets(s, P<PathTile>(eng->ptiles), P<uint32_t>(eng->seg_off), n_ptiles);
-    vkh::launch_edge_bin(s, true, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), n_edges, P<DrawTiles>(eng->dtiles), T,
-                         P<PathTile>(eng->ptiles), P<uint32_t>(eng->segs));
-    CK(hipGetLastError());
 
     CK(hipEventRecord(eng->ev_stage[3], s)); // binning done
     // ---- coarse: ordered command list per tile

@@ -624,56 +624,115 @@ __global__ void k_fill_edges(const Unit *__restrict__ units, uint32_t n_units, c
     edraw[i] = bad ? VKH_NO_DRAW : un.draw;
 }
 
-// stroke triangles -> 3 edges each, oriented so every triangle winds +1 (ray to -x, +y edges
-// count +1); indices outside the sub-path's own vertices (never drawn by a valid stream) or a
-// degenerate triangle produce no edge
-__global__ void k_tri_edges(const Unit *__restrict__ units, uint32_t n_units, const uint32_t *__restrict__ u_toff,
-                            const uint32_t *__restrict__ u_vbeg, const uint32_t *__restrict__ u_vend,
-                            uint32_t total_tris, const vkh_draw *__restrict__ draws,
-                            const TargetInfo *__restrict__ tgts, const v2 *__restrict__ verts,
-                            const uint32_t *__restrict__ tris, Edge *__restrict__ edges, uint32_t *__restrict__ edraw) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total_tris)
+__device__ __forceinline__ bool tri_has_edge(const int32_t (&N)[3][4], int32_t x0, int32_t y0, int32_t x1,
+                                              int32_t y1) {
+    bool h = false;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        h |= N[k][0] == x0 && N[k][1] == y0 && N[k][2] == x1 && N[k][3] == y1;
+    return h;
+}
+
+// Stroke triangles -> 3 edges each, oriented so every triangle winds +1 (ray to -x, +y edges
+// count +1; area > 0 is traversed 0 -> 2 -> 1); indices outside the sub-path's own vertices
+// (never drawn by a valid stream) or a degenerate triangle produce no edge. A stroke's coverage count is the sum of its triangles'
+// windings, and an edge a->b and its reverse b->a contribute exact negatives to every winding
+// term, so such pairs between neighbouring triangles of the same draw (the diagonal of each
+// segment quad, the spokes of round joins and caps) are dropped. A pair (t, t+1) is dropped only
+// when t-1 does not hold b->a and t+2 does not hold a->b, and only when t-1 .. t+2 lie in the
+// same wavefront, so the two triangles always take the same decision.
+__global__ __launch_bounds__(256) void k_tri_edges(const Unit *__restrict__ units, uint32_t n_units,
+                                                   const uint32_t *__restrict__ u_toff,
+                                                   const uint32_t *__restrict__ u_vbeg,
+                                                   const uint32_t *__restrict__ u_vend, uint32_t total_tris,
+                                                   const vkh_draw *__restrict__ draws,
+                                                   const TargetInfo *__restrict__ tgts, const v2 *__restrict__ verts,
+                                                   const uint32_t *__restrict__ tris, Edge *__restrict__ edges,
+                                                   uint32_t *__restrict__ edraw) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < total_tris;
+    uint32_t draw = VKH_NO_DRAW;
+    int32_t X[3] = {0, 0, 0}, Y[3] = {0, 0, 0};
+    if (live) {
+        uint32_t u = upper_bound_u32(u_toff, n_units, t) - 1;
+        Unit un = units[u];
+        const vkh_draw &d = draws[un.draw];
+        uint32_t vb = u_vbeg[u], ve = u_vend[u];
+        uint32_t i0 = tris[3 * (size_t)t], i1 = tris[3 * (size_t)t + 1], i2 = tris[3 * (size_t)t + 2];
+        bool bad = i0 < vb || i1 < vb || i2 < vb || i0 >= ve || i1 >= ve || i2 >= ve;
+        if (!bad) {
+            float sw = (float)tgts[d.target].width, sh = (float)tgts[d.target].height;
+            snap_vertex(verts[i0].x, verts[i0].y, d.mat, sw, sh, X[0], Y[0]);
+            snap_vertex(verts[i1].x, verts[i1].y, d.mat, sw, sh, X[1], Y[1]);
+            snap_vertex(verts[i2].x, verts[i2].y, d.mat, sw, sh, X[2], Y[2]);
+            bad = X[0] == VKH_INVALID_FIX || X[1] == VKH_INVALID_FIX || X[2] == VKH_INVALID_FIX;
+        }
+        if (!bad) {
+            int64_t a2 = (int64_t)(X[1] - X[0]) * (Y[2] - Y[0]) - (int64_t)(Y[1] - Y[0]) * (X[2] - X[0]);
+            bad = (a2 == 0);
+            if (a2 > 0) { // positive area winds -1: traverse 0 -> 2 -> 1
+                int32_t tx = X[1], ty = Y[1];
+                X[1] = X[2];
+                Y[1] = Y[2];
+                X[2] = tx;
+                Y[2] = ty;
+            }
+        }
+        if (!bad)
+            draw = un.draw;
+    }
+    int32_t E[3][4];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int k1 = k == 2 ? 0 : k + 1;
+        E[k][0] = X[k];
+        E[k][1] = Y[k];
+        E[k][2] = X[k1];
+        E[k][3] = Y[k1];
+    }
+    // neighbours t-2 .. t+2 through lane shuffles (all 64 lanes take part)
+    const int lane = (int)(threadIdx.x & 63);
+    bool rev_m1[3], rev_p1[3], same_m2[3], same_p2[3];
+#pragma unroll
+    for (int o = -2; o <= 2; o++) {
+        if (o == 0)
+            continue;
+        const int src = min(max(lane + o, 0), 63);
+        const uint32_t nd = __shfl(draw, src);
+        int32_t N[3][4];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                N[k][c] = __shfl(E[k][c], src);
+        const bool ok = draw != VKH_NO_DRAW && nd == draw && lane + o >= 0 && lane + o <= 63;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const bool rev = ok && tri_has_edge(N, E[k][2], E[k][3], E[k][0], E[k][1]);
+            const bool same = ok && tri_has_edge(N, E[k][0], E[k][1], E[k][2], E[k][3]);
+            if (o == -1)
+                rev_m1[k] = rev;
+            if (o == 1)
+                rev_p1[k] = rev;
+            if (o == -2)
+                same_m2[k] = same;
+            if (o == 2)
+                same_p2[k] = same;
+        }
+    }
+    if (!live)
         return;
-    uint32_t u = upper_bound_u32(u_toff, n_units, t) - 1;
-    Unit un = units[u];
-    const vkh_draw &d = draws[un.draw];
-    uint32_t vb = u_vbeg[u], ve = u_vend[u];
-    uint32_t i0 = tris[3 * (size_t)t], i1 = tris[3 * (size_t)t + 1], i2 = tris[3 * (size_t)t + 2];
     Edge *out = edges + 3 * (size_t)t;
     uint32_t *od = edraw + 3 * (size_t)t;
-    bool bad = i0 < vb || i1 < vb || i2 < vb || i0 >= ve || i1 >= ve || i2 >= ve;
-    int32_t X[3] = {0, 0, 0}, Y[3] = {0, 0, 0};
-    if (!bad) {
-        float sw = (float)tgts[d.target].width, sh = (float)tgts[d.target].height;
-        snap_vertex(verts[i0].x, verts[i0].y, d.mat, sw, sh, X[0], Y[0]);
-        snap_vertex(verts[i1].x, verts[i1].y, d.mat, sw, sh, X[1], Y[1]);
-        snap_vertex(verts[i2].x, verts[i2].y, d.mat, sw, sh, X[2], Y[2]);
-        bad = X[0] == VKH_INVALID_FIX || X[1] == VKH_INVALID_FIX || X[2] == VKH_INVALID_FIX;
-    }
-    int64_t a2 = 0;
-    if (!bad) {
-        a2 = (int64_t)(X[1] - X[0]) * (Y[2] - Y[0]) - (int64_t)(Y[1] - Y[0]) * (X[2] - X[0]);
-        bad = (a2 == 0);
-    }
-    if (bad) {
-        for (int k = 0; k < 3; k++) {
-            out[k] = Edge{0, 0, 0, 0};
-            od[k] = VKH_NO_DRAW;
-        }
-        return;
-    }
-    if (a2 > 0) { // positive area winds -1: traverse 0 -> 2 -> 1
-        int32_t tx = X[1], ty = Y[1];
-        X[1] = X[2];
-        Y[1] = Y[2];
-        X[2] = tx;
-        Y[2] = ty;
-    }
+    const bool fwd_visible = lane >= 1 && lane <= 61; // t-1 .. t+2 in this wavefront
+    const bool bwd_visible = lane >= 2 && lane <= 62; // t-2 .. t+1 in this wavefront
+#pragma unroll
     for (int k = 0; k < 3; k++) {
-        int k1 = k == 2 ? 0 : k + 1;
-        out[k] = Edge{X[k], Y[k], X[k1], Y[k1]};
-        od[k] = un.draw;
+        const bool fwd = fwd_visible && rev_p1[k] && !rev_m1[k] && !same_p2[k];
+        const bool bwd = bwd_visible && rev_m1[k] && !same_m2[k] && !rev_p1[k];
+        const bool keep = draw != VKH_NO_DRAW && !fwd && !bwd;
+        out[k] = keep ? Edge{E[k][0], E[k][1], E[k][2], E[k][3]} : Edge{0, 0, 0, 0};
+        od[k] = keep ? draw : VKH_NO_DRAW;
     }
 }
 

@@ -39,7 +39,7 @@ struct PathTile {
     int32_t backdrop;  // winding at the tile's first sample (after the row prefix sum)
     uint32_t nseg;     // edges touching the tile
     uint32_t seg_off;  // into the segment index array
-    uint32_t cursor;   // scatter cursor
+    uint32_t reserved; // keeps the record at 16 bytes
 };
 
 struct Cmd {
@@ -73,17 +73,16 @@ void launch_tri_edges(hipStream_t s, const Unit *units, uint32_t n_units, const 
                       const TargetInfo *tg, const v2 *verts, const uint32_t *tris, Edge *edges, uint32_t *edraw);
 
 // raster stage (vkh_raster.hip)
-void launch_draw_bbox_init(hipStream_t s, int4 *bbox, uint32_t n_draws);
-void launch_draw_bbox(hipStream_t s, const Edge *edges, const uint32_t *edraw, uint32_t n_edges, int4 *bbox);
+void launch_draw_bbox_init(hipStream_t s, int4 *bbox, uint2 *erange, uint32_t n_draws);
+void launch_draw_bbox(hipStream_t s, const Edge *edges, const uint32_t *edraw, uint32_t n_edges, int4 *bbox,
+                      uint2 *erange);
 void launch_draw_tiles(hipStream_t s, const vkh_draw *draws, uint32_t n_draws, const TargetInfo *tg,
-                       const int4 *bbox, uint32_t tile, DrawTiles *dt, uint32_t *pt_count, uint32_t *row_count);
-void launch_nseg_out(hipStream_t s, const PathTile *pts, uint32_t *nseg, uint32_t n);
+                       const int4 *bbox, uint32_t tile, DrawTiles *dt, uint32_t *pt_count);
 void launch_set_pt_base(hipStream_t s, DrawTiles *dt, const uint32_t *pt_off, uint32_t n_draws);
-void launch_edge_bin(hipStream_t s, bool scatter, const Edge *edges, const uint32_t *edraw, uint32_t n_edges,
-                     const DrawTiles *dt, uint32_t tile, PathTile *pts, uint32_t *segs);
-void launch_seg_offsets(hipStream_t s, PathTile *pts, const uint32_t *seg_off, uint32_t n);
-void launch_backdrop_prefix(hipStream_t s, const DrawTiles *dt, const uint32_t *row_off, uint32_t n_draws,
-                            uint32_t total_rows, PathTile *pts);
+void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
+void launch_bin_draws(hipStream_t s, const Edge *edges, const uint32_t *edraw, const uint2 *erange,
+                      const DrawTiles *dt, uint32_t n_draws, uint32_t tile, PathTile *pts, uint32_t *segs,
+                      uint32_t seg_cap, uint32_t *seg_alloc);
 void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, const TargetInfo *tg, uint32_t n_targets,
                    const vkh_draw *draws, const DrawTiles *dt, const PathTile *pts, uint32_t *tile_count,
                    const uint32_t *tile_off, Cmd *cmds);

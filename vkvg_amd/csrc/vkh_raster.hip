@@ -24,14 +24,17 @@ __device__ __forceinline__ int32_t floor_div32(int32_t a, int32_t b) { // b > 0
 }
 
 // ----------------------------------------------------------------------------------- draw bbox
-__global__ void k_bbox_init(int4 *bbox, uint32_t n) {
+__global__ void k_bbox_init(int4 *bbox, uint2 *erange, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
+    if (i < n) {
         bbox[i] = make_int4(INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN);
+        erange[i] = make_uint2(0xFFFFFFFFu, 0u);
+    }
 }
 
+// per draw: bbox of its edges and the range [first, last + 1) of its edge indices
 __global__ void k_draw_bbox(const Edge *__restrict__ edges, const uint32_t *__restrict__ edraw, uint32_t n,
-                            int4 *__restrict__ bbox) {
+                            int4 *__restrict__ bbox, uint2 *__restrict__ erange) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t d = i < n ? edraw[i] : VKH_NO_DRAW;
     int32_t x0 = INT32_MAX, y0 = INT32_MAX, x1 = INT32_MIN, y1 = INT32_MIN;
@@ -42,37 +45,39 @@ __global__ void k_draw_bbox(const Edge *__restrict__ edges, const uint32_t *__re
         y0 = min(e.y0, e.y1);
         y1 = max(e.y0, e.y1);
     }
-    // wave-uniform fast path: one set of atomics per wave when all lanes share a draw
-    uint32_t d0 = __shfl(d, 0);
-    bool uniform = __all(d == d0);
-    if (uniform) {
-        if (d0 == VKH_NO_DRAW)
-            return;
+    // one set of atomics per distinct draw of the wavefront (edges of a draw are contiguous, so
+    // a wavefront usually holds one or two draws; dropped edges carry no draw)
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t pending = __ballot(d != VKH_NO_DRAW);
+    while (pending) {
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1;
+        const uint32_t dl = __shfl(d, (int)leader);
+        const bool mine = d == dl;
+        int32_t a = mine ? x0 : INT32_MAX, b = mine ? y0 : INT32_MAX;
+        int32_t c = mine ? x1 : INT32_MIN, e = mine ? y1 : INT32_MIN;
         for (int o = 32; o > 0; o >>= 1) {
-            x0 = min(x0, __shfl_xor(x0, o));
-            y0 = min(y0, __shfl_xor(y0, o));
-            x1 = max(x1, __shfl_xor(x1, o));
-            y1 = max(y1, __shfl_xor(y1, o));
+            a = min(a, __shfl_xor(a, o));
+            b = min(b, __shfl_xor(b, o));
+            c = max(c, __shfl_xor(c, o));
+            e = max(e, __shfl_xor(e, o));
         }
-        if ((threadIdx.x & 63) == 0) {
-            atomicMin(&bbox[d0].x, x0);
-            atomicMin(&bbox[d0].y, y0);
-            atomicMax(&bbox[d0].z, x1);
-            atomicMax(&bbox[d0].w, y1);
+        const uint64_t mm = __ballot(mine);
+        if (lane == leader) {
+            atomicMin(&bbox[dl].x, a);
+            atomicMin(&bbox[dl].y, b);
+            atomicMax(&bbox[dl].z, c);
+            atomicMax(&bbox[dl].w, e);
+            const uint32_t wbase = i - lane;
+            atomicMin(&erange[dl].x, wbase + leader);
+            atomicMax(&erange[dl].y, wbase + 64u - (uint32_t)__clzll((long long)mm));
         }
-        return;
+        pending &= ~mm;
     }
-    if (d == VKH_NO_DRAW)
-        return;
-    atomicMin(&bbox[d].x, x0);
-    atomicMin(&bbox[d].y, y0);
-    atomicMax(&bbox[d].z, x1);
-    atomicMax(&bbox[d].w, y1);
 }
 
 __global__ void k_draw_tiles(const vkh_draw *__restrict__ draws, uint32_t n, const TargetInfo *__restrict__ tg,
                              const int4 *__restrict__ bbox, uint32_t T, DrawTiles *__restrict__ dt,
-                             uint32_t *__restrict__ pt_count, uint32_t *__restrict__ row_count) {
+                             uint32_t *__restrict__ pt_count) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
@@ -88,7 +93,6 @@ __global__ void k_draw_tiles(const vkh_draw *__restrict__ draws, uint32_t n, con
         o.bw = 0;
         dt[i] = o;
         pt_count[i] = 0;
-        row_count[i] = 0;
         return;
     }
     int4 b = bbox[i];
@@ -107,11 +111,9 @@ __global__ void k_draw_tiles(const vkh_draw *__restrict__ draws, uint32_t n, con
         o.bx1 = o.by1 = -1;
         o.bw = 0;
         pt_count[i] = 0;
-        row_count[i] = 0;
     } else {
         o.bw = (uint32_t)(o.bx1 - o.bx0 + 1);
         pt_count[i] = o.bw * (uint32_t)(o.by1 - o.by0 + 1);
-        row_count[i] = (uint32_t)(o.by1 - o.by0 + 1);
     }
     dt[i] = o;
 }
@@ -123,33 +125,31 @@ __global__ void k_set_pt_base(DrawTiles *dt, const uint32_t *pt_off, uint32_t n)
 }
 
 // ----------------------------------------------------------------------------------- edge binning
-// Enumerates, for one edge, the tiles of its draw's bbox that the edge may touch (per tile
-// row, the edge's x-extent inside the row's slab, widened by one pixel). Count pass: bumps
-// the tiles' segment counts and adds the edge's crossing of each tile row's first sample row
-// into the backdrop delta of the first tile whose first column lies right of the crossing.
-// Scatter pass: writes the edge index into each touched tile's segment list.
-__global__ void k_edge_bin(bool scatter, const Edge *__restrict__ edges, const uint32_t *__restrict__ edraw,
-                           uint32_t n, const DrawTiles *__restrict__ dts, uint32_t T, PathTile *__restrict__ pts,
-                           uint32_t *__restrict__ segs) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    uint32_t d = edraw[i];
-    if (d == VKH_NO_DRAW)
-        return;
-    DrawTiles dt = dts[d];
-    if (dt.bx0 > dt.bx1)
-        return;
-    Edge e = edges[i];
-    int32_t span = 256 * (int32_t)T;
-    int32_t ymin = min(e.y0, e.y1), ymax = max(e.y0, e.y1);
-    int32_t ty0 = max(floor_div32(ymin, span), dt.by0);
-    int32_t ty1 = min(floor_div32(ymax, span), dt.by1);
-    bool horiz = e.y0 == e.y1;
-    double invdy = horiz ? 0.0 : 1.0 / ((double)e.y1 - (double)e.y0);
-    int dir = e.y0 < e.y1 ? 1 : -1;
-    int32_t xt = e.y0 < e.y1 ? e.x0 : e.x1, yt = ymin;
-    int32_t xb = e.y0 < e.y1 ? e.x1 : e.x0, yb = ymax;
+// One workgroup per draw bins the draw's edges into its bbox tile grid, in bands of whole tile
+// rows that fit the LDS tables (VKH_BIN_LDS_TILES tiles). An edge is enumerated into the tiles
+// it may touch: per tile row, the edge's x-extent inside the row's slab widened by one pixel.
+// Count pass (LDS atomics): segments per tile, and the edge's crossing of each tile row's first
+// sample row added to the backdrop delta of the first tile whose first column lies right of the
+// crossing. The counts are scanned in LDS, the band's segment range is taken from a global
+// allocator with one atomic, the backdrop deltas are prefix-summed along each row and the path
+// tiles are written; the scatter pass then writes the edge indices through LDS cursors. A band
+// whose range does not fit the segment buffer writes no segments; the host sees the allocator
+// beyond the capacity, grows the buffer and bins again.
+#define VKH_BIN_LDS_TILES 4096
+
+template <bool SCATTER>
+__device__ __forceinline__ void bin_edge(const Edge &e, uint32_t ei, const DrawTiles &dt, int32_t T, int32_t ty_lo,
+                                         int32_t ty_hi, uint32_t *s_cnt, int32_t *s_back, uint32_t *segs,
+                                         uint32_t base) {
+    const int32_t span = 256 * T;
+    const int32_t ymin = min(e.y0, e.y1), ymax = max(e.y0, e.y1);
+    const int32_t ty0 = max(floor_div32(ymin, span), ty_lo);
+    const int32_t ty1 = min(floor_div32(ymax, span), ty_hi);
+    const bool horiz = e.y0 == e.y1;
+    const double invdy = horiz ? 0.0 : 1.0 / ((double)e.y1 - (double)e.y0);
+    const int dir = e.y0 < e.y1 ? 1 : -1;
+    const int32_t xt = e.y0 < e.y1 ? e.x0 : e.x1, yt = ymin;
+    const int32_t xb = e.y0 < e.y1 ? e.x1 : e.x0, yb = ymax;
     for (int32_t ty = ty0; ty <= ty1; ty++) {
         int32_t xlo, xhi;
         if (horiz) {
@@ -163,68 +163,136 @@ __global__ void k_edge_bin(bool scatter, const Edge *__restrict__ edges, const u
             xlo = (int32_t)floor(fmin(xa, xc)) - 256;
             xhi = (int32_t)ceil(fmax(xa, xc)) + 256;
         }
-        int32_t tx0 = max(floor_div32(xlo, span), dt.bx0);
-        int32_t tx1 = min(floor_div32(xhi, span), dt.bx1);
-        uint32_t rowbase = dt.pt_base + (uint32_t)(ty - dt.by0) * dt.bw;
+        const int32_t tx0 = max(floor_div32(xlo, span), dt.bx0);
+        const int32_t tx1 = min(floor_div32(xhi, span), dt.bx1);
+        const uint32_t rowbase = (uint32_t)(ty - ty_lo) * dt.bw;
         for (int32_t tx = tx0; tx <= tx1; tx++) {
-            PathTile *p = &pts[rowbase + (uint32_t)(tx - dt.bx0)];
-            if (!scatter) {
-                atomicAdd(&p->nseg, 1u);
-            } else {
-                uint32_t pos = atomicAdd(&p->cursor, 1u);
-                if (pos < p->nseg) // the count pass enumerated the same tiles
-                    segs[p->seg_off + pos] = i;
-            }
+            const uint32_t k = rowbase + (uint32_t)(tx - dt.bx0);
+            if (!SCATTER)
+                atomicAdd(&s_cnt[k], 1u);
+            else
+                segs[base + atomicAdd(&s_cnt[k], 1u)] = ei;
         }
-        if (!scatter && !horiz) {
-            int32_t sy = ty * span + 128; // first sample row of the tile row
+        if (!SCATTER && !horiz) {
+            const int32_t sy = ty * span + 128; // first sample row of the tile row
             if (yt <= sy && sy < yb) {
-                int64_t c = crossing_column(xt, yt, xb, yb, sy);
+                const int64_t c = crossing_column(xt, yt, xb, yb, sy);
                 // first tile tx with tx*T >= c
                 int64_t t0 = c >= 0 ? (c + T - 1) / T : -((-c) / T);
                 if (t0 < dt.bx0)
                     t0 = dt.bx0;
                 if (t0 <= dt.bx1)
-                    atomicAdd(&pts[rowbase + (uint32_t)(t0 - dt.bx0)].backdrop, dir);
+                    atomicAdd(&s_back[rowbase + (uint32_t)(t0 - dt.bx0)], dir);
             }
         }
     }
 }
 
-__global__ void k_nseg_out(const PathTile *pts, uint32_t *nseg, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        nseg[i] = pts[i].nseg;
-}
-__global__ void k_seg_offsets(PathTile *pts, const uint32_t *off, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        pts[i].seg_off = off[i];
+// exclusive scan in place of a[0 .. n), n <= 16 * 256, by a 256-thread workgroup; returns the total
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t *a, uint32_t n, uint32_t *s_wave) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t beg = min(tid * 16u, n), end = min(beg + 16u, n);
+    uint32_t sum = 0;
+    for (uint32_t k = beg; k < end; k++)
+        sum += a[k];
+    uint32_t x = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if ((int)lane >= o)
+            x += y;
+    }
+    if (lane == 63)
+        s_wave[w] = x;
+    __syncthreads();
+    uint32_t run = x - sum, total = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+        run += k < w ? s_wave[k] : 0u;
+        total += s_wave[k];
+    }
+    for (uint32_t k = beg; k < end; k++) {
+        const uint32_t v = a[k];
+        a[k] = run;
+        run += v;
+    }
+    __syncthreads();
+    return total;
 }
 
-// one thread per (draw, tile row): prefix sum of the backdrop deltas along the row
-__global__ void k_backdrop_prefix(const DrawTiles *__restrict__ dts, const uint32_t *__restrict__ row_off,
-                                  uint32_t n_draws, uint32_t total_rows, PathTile *__restrict__ pts) {
+__global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges, const uint32_t *__restrict__ edraw,
+                                                  const uint2 *__restrict__ erange,
+                                                  const DrawTiles *__restrict__ dts, uint32_t T,
+                                                  PathTile *__restrict__ pts, uint32_t *__restrict__ segs,
+                                                  uint32_t seg_cap, uint32_t *__restrict__ seg_alloc) {
+    __shared__ uint32_t s_cnt[VKH_BIN_LDS_TILES];
+    __shared__ int32_t s_back[VKH_BIN_LDS_TILES];
+    __shared__ uint32_t s_wave[4];
+    __shared__ uint32_t s_base;
+    const uint32_t d = blockIdx.x, tid = threadIdx.x;
+    const DrawTiles dt = dts[d];
+    if (dt.bw == 0 || dt.bx0 > dt.bx1 || dt.by0 > dt.by1)
+        return; // paint / clear, or no edges
+    const uint2 er = erange[d];
+    const uint32_t bw = dt.bw, bh = (uint32_t)(dt.by1 - dt.by0 + 1);
+    const uint32_t band_rows = max(1u, VKH_BIN_LDS_TILES / bw); // bw <= 4096 (width <= 32768, T >= 8)
+    for (uint32_t r0 = 0; r0 < bh; r0 += band_rows) {
+        const uint32_t nr = min(band_rows, bh - r0);
+        const uint32_t nt = nr * bw;
+        const int32_t ty_lo = dt.by0 + (int32_t)r0, ty_hi = ty_lo + (int32_t)nr - 1;
+        for (uint32_t k = tid; k < nt; k += 256) {
+            s_cnt[k] = 0;
+            s_back[k] = 0;
+        }
+        __syncthreads();
+        for (uint32_t i = er.x + tid; i < er.y; i += 256) {
+            if (edraw[i] != d)
+                continue;
+            const Edge e = edges[i];
+            bin_edge<false>(e, i, dt, (int32_t)T, ty_lo, ty_hi, s_cnt, s_back, nullptr, 0);
+        }
+        __syncthreads();
+        const uint32_t total = block_exclusive_scan(s_cnt, nt, s_wave);
+        if (tid == 0)
+            s_base = atomicAdd(seg_alloc, total);
+        for (uint32_t r = tid; r < nr; r += 256) {
+            int32_t acc = 0;
+            for (uint32_t k = r * bw; k < (r + 1) * bw; k++) {
+                acc += s_back[k];
+                s_back[k] = acc;
+            }
+        }
+        __syncthreads();
+        const uint32_t base = s_base;
+        const bool fits = (uint64_t)base + total <= (uint64_t)seg_cap;
+        PathTile *out = pts + dt.pt_base + r0 * bw;
+        for (uint32_t k = tid; k < nt; k += 256) {
+            const uint32_t off = s_cnt[k];
+            const uint32_t next = k + 1 < nt ? s_cnt[k + 1] : total;
+            PathTile p;
+            p.backdrop = s_back[k];
+            p.nseg = next - off;
+            p.seg_off = base + off;
+            p.reserved = 0;
+            out[k] = p;
+        }
+        __syncthreads();
+        if (fits) {
+            for (uint32_t i = er.x + tid; i < er.y; i += 256) {
+                if (edraw[i] != d)
+                    continue;
+                const Edge e = edges[i];
+                bin_edge<true>(e, i, dt, (int32_t)T, ty_lo, ty_hi, s_cnt, s_back, segs, base);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total_rows)
-        return;
-    uint32_t lo = 0, hi = n_draws;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (row_off[mid] <= i)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    uint32_t d = lo - 1;
-    DrawTiles dt = dts[d];
-    uint32_t r = i - row_off[d];
-    PathTile *p = pts + dt.pt_base + r * dt.bw;
-    int32_t acc = 0;
-    for (uint32_t k = 0; k < dt.bw; k++) {
-        acc += p[k].backdrop;
-        p[k].backdrop = acc;
-    }
+    if (i < n)
+        p[i] = 0;
 }
 
 // ----------------------------------------------------------------------------------- coarse
@@ -655,43 +723,35 @@ __global__ void k_unpremultiply(const uint8_t *__restrict__ src, uint8_t *__rest
 }
 
 namespace vkh {
-void launch_draw_bbox_init(hipStream_t s, int4 *bbox, uint32_t n) {
+void launch_draw_bbox_init(hipStream_t s, int4 *bbox, uint2 *erange, uint32_t n) {
     if (n)
-        hipLaunchKernelGGL(k_bbox_init, dim3((n + 255) / 256), dim3(256), 0, s, bbox, n);
+        hipLaunchKernelGGL(k_bbox_init, dim3((n + 255) / 256), dim3(256), 0, s, bbox, erange, n);
 }
-void launch_draw_bbox(hipStream_t s, const Edge *edges, const uint32_t *edraw, uint32_t n, int4 *bbox) {
+void launch_draw_bbox(hipStream_t s, const Edge *edges, const uint32_t *edraw, uint32_t n, int4 *bbox,
+                      uint2 *erange) {
     if (n)
-        hipLaunchKernelGGL(k_draw_bbox, dim3((n + 255) / 256), dim3(256), 0, s, edges, edraw, n, bbox);
+        hipLaunchKernelGGL(k_draw_bbox, dim3((n + 255) / 256), dim3(256), 0, s, edges, edraw, n, bbox, erange);
 }
 void launch_draw_tiles(hipStream_t s, const vkh_draw *draws, uint32_t n, const TargetInfo *tg, const int4 *bbox,
-                       uint32_t tile, DrawTiles *dt, uint32_t *pt_count, uint32_t *row_count) {
+                       uint32_t tile, DrawTiles *dt, uint32_t *pt_count) {
     if (n)
         hipLaunchKernelGGL(k_draw_tiles, dim3((n + 255) / 256), dim3(256), 0, s, draws, n, tg, bbox, tile, dt,
-                           pt_count, row_count);
+                           pt_count);
 }
 void launch_set_pt_base(hipStream_t s, DrawTiles *dt, const uint32_t *pt_off, uint32_t n) {
     if (n)
         hipLaunchKernelGGL(k_set_pt_base, dim3((n + 255) / 256), dim3(256), 0, s, dt, pt_off, n);
 }
-void launch_edge_bin(hipStream_t s, bool scatter, const Edge *edges, const uint32_t *edraw, uint32_t n,
-                     const DrawTiles *dt, uint32_t tile, PathTile *pts, uint32_t *segs) {
+void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n) {
     if (n)
-        hipLaunchKernelGGL(k_edge_bin, dim3((n + 255) / 256), dim3(256), 0, s, scatter, edges, edraw, n, dt, tile, pts,
-                           segs);
+        hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
 }
-void launch_nseg_out(hipStream_t s, const PathTile *pts, uint32_t *nseg, uint32_t n) {
-    if (n)
-        hipLaunchKernelGGL(k_nseg_out, dim3((n + 255) / 256), dim3(256), 0, s, pts, nseg, n);
-}
-void launch_seg_offsets(hipStream_t s, PathTile *pts, const uint32_t *seg_off, uint32_t n) {
-    if (n)
-        hipLaunchKernelGGL(k_seg_offsets, dim3((n + 255) / 256), dim3(256), 0, s, pts, seg_off, n);
-}
-void launch_backdrop_prefix(hipStream_t s, const DrawTiles *dt, const uint32_t *row_off, uint32_t n_draws,
-                            uint32_t total_rows, PathTile *pts) {
-    if (total_rows)
-        hipLaunchKernelGGL(k_backdrop_prefix, dim3((total_rows + 255) / 256), dim3(256), 0, s, dt, row_off, n_draws,
-                           total_rows, pts);
+void launch_bin_draws(hipStream_t s, const Edge *edges, const uint32_t *edraw, const uint2 *erange,
+                      const DrawTiles *dt, uint32_t n_draws, uint32_t tile, PathTile *pts, uint32_t *segs,
+                      uint32_t seg_cap, uint32_t *seg_alloc) {
+    if (n_draws)
+        hipLaunchKernelGGL(k_bin_draw, dim3(n_draws), dim3(256), 0, s, edges, edraw, erange, dt, tile, pts, segs,
+                           seg_cap, seg_alloc);
 }
 void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, const TargetInfo *tg, uint32_t n_targets,
                    const vkh_draw *draws, const DrawTiles *dt, const PathTile *pts, uint32_t *tile_count,
