@@ -82,7 +82,7 @@ struct vkh_engine {
 
     DevBuf blob, units, tgts, item_cnt, item_off, pts, pflags, sp_first, sp_count;
     DevBuf fe, nv, nt, fe_off, nv_off, nt_off, verts, tris, edges, edraw;
-    DevBuf bbox, erange, dtiles, pt_count, pt_off, ptiles, seg_alloc, segs;
+    DevBuf bbox, erange, dtiles, pt_count, pt_off, ptiles, seg_alloc, segs, dprep;
     DevBuf tile_cnt, tile_off, cmds, scan_tmp, readback;
     DevBuf step_nv, step_nt, sv_off, st_off, u_toff, u_vbeg, u_vend;
     HostBuf hblob, htot;
@@ -230,7 +230,7 @@ void vkvg_hip_engine_destroy(vkh_engine *e) {
     DevBuf *bufs[] = {&e->blob, &e->units, &e->tgts, &e->item_cnt, &e->item_off, &e->pts, &e->pflags,
                       &e->sp_first, &e->sp_count, &e->fe, &e->nv, &e->nt, &e->fe_off, &e->nv_off, &e->nt_off,
                       &e->verts, &e->tris, &e->edges, &e->edraw, &e->bbox, &e->dtiles, &e->pt_count, &e->pt_off,
-                      &e->erange, &e->ptiles, &e->seg_alloc, &e->segs, &e->tile_cnt,
+                      &e->erange, &e->ptiles, &e->seg_alloc, &e->segs, &e->dprep, &e->tile_cnt,
                       &e->tile_off, &e->cmds, &e->scan_tmp, &e->readback, &e->step_nv, &e->step_nt,
                       &e->sv_off, &e->st_off, &e->u_toff, &e->u_vbeg, &e->u_vend};
     for (DevBuf *b : bufs)
@@ -580,6 +580,9 @@ int vkvg_hip_execute(vkh_engine *eng) {
                           P<uint2>(eng->erange));
     vkh::launch_draw_tiles(s, d_draws, n_draws, d_tg, P<int4>(eng->bbox), T, P<DrawTiles>(eng->dtiles),
                            P<uint32_t>(eng->pt_count));
+    if (!ensure(eng, eng->dprep, sizeof(DrawPrep) * (n_draws + 1)))
+        return -1;
+    vkh::launch_draw_prep(s, d_draws, n_draws, d_tg, d_grads, P<DrawPrep>(eng->dprep));
     if (scan_u32(eng, P<uint32_t>(eng->pt_count), n_draws, P<uint32_t>(eng->pt_off)))
         return -1;
     if (read_totals(eng, {P<uint32_t>(eng->pt_off) + n_draws}, tot))
@@ -638,7 +641,7 @@ int vkvg_hip_execute(vkh_engine *eng) {
     // ---- fine: coverage + compositing
     CK(hipEventRecord(eng->ev_stage[4], s)); // coarse done
     CK(hipEventRecord(eng->ev0, s));
-    vkh::launch_fine(s, T, tile_total, d_tg, n_targets, d_draws, d_grads, P<PathTile>(eng->ptiles),
+    vkh::launch_fine(s, T, tile_total, d_tg, n_targets, P<DrawPrep>(eng->dprep), d_grads, P<PathTile>(eng->ptiles),
                      P<uint32_t>(eng->segs), P<Edge>(eng->edges), n_edges, P<uint32_t>(eng->tile_off),
                      P<Cmd>(eng->cmds));
     CK(hipGetLastError());

@@ -42,6 +42,26 @@ struct PathTile {
     uint32_t reserved; // keeps the record at 16 bytes
 };
 
+// per draw, computed once per frame: what the compositing loop needs, with the per-draw parts of
+// the source evaluated once (same float operations as per pixel, so the values are identical)
+enum : uint32_t {
+    VKH_DP_GRAD = 1u << 9,     // meta: gradient source
+    VKH_DP_HASSEG = 1u << 8,   // meta (set per command in the fine kernel): the path tile has segments
+    VKH_GF_UY0 = 1u,           // linear: direction has no y component
+    VKH_GF_UXNEG = 2u,         // linear: ... and points to -x
+    VKH_GF_UYNEG = 4u,         // linear: y component < 0
+};
+struct DrawPrep {
+    uint32_t meta;   // kind | op << 4 | VKH_DP_GRAD
+    uint32_t solid;  // premultiplied RGBA8 of a solid source, opacity applied
+    uint32_t key;    // coverage-signature key of the draw
+    int32_t grad;    // gradient index or -1
+    float opacity;
+    uint32_t gflags; // VKH_GF_*
+    float g[6];      // gradient invariants: linear p0x, l, m, bb, sqrt(1+m*m); radial c0x, c0y, r0, dx, dy, cc
+    uint32_t pad[4];
+};
+
 struct Cmd {
     uint32_t draw;
     uint32_t pt; // path tile, VKH_NO_DRAW for paint / clear
@@ -83,11 +103,13 @@ void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
 void launch_bin_draws(hipStream_t s, const Edge *edges, const uint32_t *edraw, const uint2 *erange,
                       const DrawTiles *dt, uint32_t n_draws, uint32_t tile, PathTile *pts, uint32_t *segs,
                       uint32_t seg_cap, uint32_t *seg_alloc);
+void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n_draws, const TargetInfo *tg,
+                      const vkh_gradient *grads, DrawPrep *out);
 void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, const TargetInfo *tg, uint32_t n_targets,
                    const vkh_draw *draws, const DrawTiles *dt, const PathTile *pts, uint32_t *tile_count,
                    const uint32_t *tile_off, Cmd *cmds);
 void launch_fine(hipStream_t s, uint32_t tile, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
-                 const vkh_draw *draws, const vkh_gradient *grads, const PathTile *pts, const uint32_t *segs,
+                 const DrawPrep *dprep, const vkh_gradient *grads, const PathTile *pts, const uint32_t *segs,
                  const Edge *edges, uint32_t n_edges, const uint32_t *tile_off, const Cmd *cmds);
 void launch_unpremultiply(hipStream_t s, const uint8_t *src, uint8_t *dst, size_t npix);
 } // namespace vkh

@@ -396,26 +396,82 @@ __device__ __forceinline__ void mix4(float *c, const float *d, float a) {
     for (int i = 0; i < 4; i++)
         c[i] = c[i] * (1.f - a) + d[i] * a;
 }
-// shaders/vkvg_main.frag:83-144 with p = gl_FragCoord.xy / size
-__device__ void gradient_color(const vkh_gradient &g, float W, float H, float fx, float fy, float *col) {
-    float px = fx / W, py = fy / H;
-    if (g.type == 2) {
-        float dist = 1;
-        float p0x = g.cp[0][0] / W, p0y = g.cp[0][1] / H;
-        float p1x = g.cp[0][2] / W, p1y = g.cp[0][3] / H;
-        float dx = p1x - p0x, dy = p1y - p0y;
-        float l = sqrtf(dx * dx + dy * dy);
-        float ux = dx / l, uy = dy / l;
-        if (uy == 0) {
-            if (ux < 0)
-                dist = -(px - p0x) / l;
-            else
-                dist = (px - p0x) / l;
+
+// Per draw: premultiplied solid colour, signature key, normalised kind/op, and the gradient's
+// pixel-independent terms of shaders/vkvg_main.frag:83-144 (p = gl_FragCoord.xy / size), each
+// computed with the same float operations the per-pixel evaluation used to repeat.
+__global__ void k_draw_prep(const vkh_draw *__restrict__ draws, uint32_t n, const TargetInfo *__restrict__ tg,
+                            const vkh_gradient *__restrict__ grads, DrawPrep *__restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const vkh_draw &d = draws[i];
+    DrawPrep p;
+    const uint32_t kind = d.kind;
+    const uint32_t op = kind == VKH_DRAW_CLEAR ? 0u : d.op;
+    p.meta = kind | (op << 4) | (d.grad >= 0 ? VKH_DP_GRAD : 0u);
+    p.key = draw_key(d.key_index);
+    p.grad = d.grad;
+    p.opacity = d.opacity;
+    const uint32_t c = d.color;
+    const float o = d.opacity;
+    p.solid = (uint32_t)f2u8(((float)(c & 0xFF) / 255.0f) * o) |
+              ((uint32_t)f2u8(((float)((c >> 8) & 0xFF) / 255.0f) * o) << 8) |
+              ((uint32_t)f2u8(((float)((c >> 16) & 0xFF) / 255.0f) * o) << 16) |
+              ((uint32_t)f2u8(((float)((c >> 24) & 0xFF) / 255.0f) * o) << 24);
+    p.gflags = 0;
+    for (int k = 0; k < 6; k++)
+        p.g[k] = 0.f;
+    for (int k = 0; k < 4; k++)
+        p.pad[k] = 0;
+    if (d.grad >= 0) {
+        const vkh_gradient &g = grads[d.grad];
+        const float W = (float)tg[d.target].width, H = (float)tg[d.target].height;
+        if (g.type == 2) {
+            float p0x = g.cp[0][0] / W, p0y = g.cp[0][1] / H;
+            float p1x = g.cp[0][2] / W, p1y = g.cp[0][3] / H;
+            float dx = p1x - p0x, dy = p1y - p0y;
+            float l = sqrtf(dx * dx + dy * dy);
+            float ux = dx / l, uy = dy / l;
+            p.g[0] = p0x;
+            p.g[1] = l;
+            if (uy == 0) {
+                p.gflags = VKH_GF_UY0 | (ux < 0 ? VKH_GF_UXNEG : 0u);
+            } else {
+                float m = -ux / uy;
+                p.g[2] = m;
+                p.g[3] = p0y - m * p0x;
+                p.g[4] = sqrtf(1 + m * m);
+                p.gflags = uy < 0 ? VKH_GF_UYNEG : 0u;
+            }
         } else {
-            float m = -ux / uy;
-            float bb = p0y - m * p0x;
-            dist = ((py - m * px - bb) / sqrtf(1 + m * m)) / l;
-            if (uy < 0)
+            float c0x = g.cp[0][0] / W, c0y = g.cp[0][1] / H;
+            float c1x = g.cp[1][0] / W, c1y = g.cp[1][1] / H;
+            float r0 = g.cp[0][2] / W, r1 = g.cp[1][2] / W;
+            float diffx = c0x - c1x, diffy = c0y - c1y;
+            p.g[0] = c0x;
+            p.g[1] = c0y;
+            p.g[2] = r0;
+            p.g[3] = diffx;
+            p.g[4] = diffy;
+            p.g[5] = (diffx * diffx + diffy * diffy) - r1 * r1;
+        }
+    }
+    out[i] = p;
+}
+
+// shaders/vkvg_main.frag:83-144 at p = (px, py), with the draw's invariant terms from DrawPrep
+__device__ void gradient_color(const vkh_gradient &g, const DrawPrep &P, float px, float py, float *col) {
+    if (g.type == 2) {
+        float dist;
+        if (P.gflags & VKH_GF_UY0) {
+            if (P.gflags & VKH_GF_UXNEG)
+                dist = -(px - P.g[0]) / P.g[1];
+            else
+                dist = (px - P.g[0]) / P.g[1];
+        } else {
+            dist = ((py - P.g[2] * px - P.g[3]) / P.g[4]) / P.g[1];
+            if (P.gflags & VKH_GF_UYNEG)
                 dist = -dist;
         }
         for (int i = 0; i < 4; i++)
@@ -424,17 +480,13 @@ __device__ void gradient_color(const vkh_gradient &g, float W, float H, float fx
         for (int i = 1; i < (int)g.count - 1; ++i)
             mix4(col, g.colors[i + 1], smoothstepf(g.stops[i], g.stops[i + 1], dist));
     } else {
-        float c0x = g.cp[0][0] / W, c0y = g.cp[0][1] / H;
-        float c1x = g.cp[1][0] / W, c1y = g.cp[1][1] / H;
-        float r0 = g.cp[0][2] / W, r1 = g.cp[1][2] / W;
+        const float c0x = P.g[0], c0y = P.g[1], r0 = P.g[2], diffx = P.g[3], diffy = P.g[4], cc = P.g[5];
         float gradLength = 1.0f;
-        float diffx = c0x - c1x, diffy = c0y - c1y;
         float rx = px - c0x, ry = py - c0y;
         float rl = sqrtf(rx * rx + ry * ry);
         float rdx = rx / rl, rdy = ry / rl;
         float a = rdx * rdx + rdy * rdy;
         float b = 2.0f * (rdx * diffx + rdy * diffy);
-        float cc = (diffx * diffx + diffy * diffy) - r1 * r1;
         float disc = b * b - 4.0f * a * cc;
         if (disc >= 0.0f) {
             float t = (-b + sqrtf(fabsf(disc))) / (2.0f * a);
@@ -471,15 +523,23 @@ __device__ __forceinline__ uint32_t blend_px(uint32_t op, uint32_t d, uint32_t s
     return r;
 }
 
-// One workgroup per T x T tile. Commands are processed in batches of B: the batch's command,
-// draw and path-tile records are staged in LDS with one parallel load; every segment of the batch
-// (one thread each) adds its edge's crossings of the tile's sample rows into per-command LDS row
-// deltas, walking only the rows the edge spans; each row is prefix-summed once; then every pixel
-// applies the batch's commands in order. The dependent global loads (command -> path tile ->
-// segment -> edge) are thus paid once per batch and once per segment.
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+// row-delta words of LDS per fine workgroup (B = words / pixels commands per batch)
+#ifndef VKH_FINE_LDS_WORDS
+#define VKH_FINE_LDS_WORDS 4096
+#endif
+
+// One workgroup per T x T tile. Commands are processed in batches of B: the batch's command
+// records (draw record + path tile) are staged in LDS with one parallel load; every segment of the
+// batch (one thread each) adds its edge's crossings of the tile's sample rows into per-command
+// LDS row deltas, walking only the rows the edge spans; each row is prefix-summed once; then every
+// pixel applies the batch's commands in order, reading each command's record as wave-uniform
+// values. The dependent global loads (command -> draw / path tile -> segment -> edge) are thus
+// paid once per batch and once per segment.
 template <int T>
 __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg, uint32_t n_targets,
-                                              const vkh_draw *__restrict__ draws,
+                                              const DrawPrep *__restrict__ dprep,
                                               const vkh_gradient *__restrict__ grads,
                                               const PathTile *__restrict__ pts, const uint32_t *__restrict__ segs,
                                               const Edge *__restrict__ edges, uint32_t n_edges,
@@ -487,13 +547,12 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
     constexpr int NPIX = T * T;
     constexpr int NT = NPIX >= 256 ? 256 : NPIX;  // threads
     constexpr int PPT = NPIX / NT;                // pixels per thread
-    constexpr int B = 8192 / NPIX;                // commands per batch (32 KB of row deltas)
+    constexpr int B = VKH_FINE_LDS_WORDS / NPIX;  // commands per batch
     __shared__ int32_t sd[B][T][T];               // row deltas, then per-row inclusive prefix
     __shared__ int32_t srow[B][T];                // V term: +-1 at the first row below a column crossing
     __shared__ uint32_t c_item[B + 1];            // prefix of the commands' segment counts
-    __shared__ uint32_t c_nseg[B], c_segoff[B], c_kind[B], c_op[B], c_color[B], c_key[B];
-    __shared__ int32_t c_back[B], c_grad[B];
-    __shared__ float c_opa[B];
+    __shared__ uint32_t c_nseg[B], c_segoff[B], c_meta[B], c_solid[B], c_key[B], c_draw[B];
+    __shared__ int32_t c_back[B];
 
     uint32_t gtile = blockIdx.x;
     uint32_t c0 = tile_off[gtile], c1 = tile_off[gtile + 1];
@@ -505,12 +564,13 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
     uint32_t lt = gtile - TI.tile_base;
     int32_t tx = (int32_t)(lt % TI.tilesW), ty = (int32_t)(lt / TI.tilesW);
     int32_t X0 = tx * T, Y0 = ty * T;
-    float Wf = (float)TI.width, Hf = (float)TI.height;
+    const float Wf = (float)TI.width, Hf = (float)TI.height;
     const int32_t xs0 = 256 * X0 + 128, s0y = 256 * Y0 + 128;
 
     int lx[PPT], ly[PPT];
     bool valid[PPT];
     uint32_t px[PPT], sig[PPT];
+    float gx[PPT], gy[PPT]; // gradient coordinates gl_FragCoord.xy / size
 #pragma unroll
     for (int q = 0; q < PPT; q++) {
         int p = (int)tid + q * NT;
@@ -518,16 +578,19 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
         ly[q] = p / T;
         valid[q] = (X0 + lx[q]) < (int32_t)TI.width && (Y0 + ly[q]) < (int32_t)TI.height;
         sig[q] = 0;
+        gx[q] = ((float)(X0 + lx[q]) + 0.5f) / Wf;
+        gy[q] = ((float)(Y0 + ly[q]) + 0.5f) / Hf;
     }
     // the destination need not be read when the first command overwrites every pixel
     bool need_read = true;
     {
-        Cmd f = cmds[c0];
-        const vkh_draw &d0 = draws[f.draw];
+        const Cmd f = cmds[c0];
         if (f.pt == VKH_NO_DRAW) {
-            if (d0.kind == VKH_DRAW_CLEAR || d0.op == 0)
+            const DrawPrep &d0 = dprep[f.draw];
+            const uint32_t kind0 = d0.meta & 15, op0 = (d0.meta >> 4) & 15;
+            if (kind0 == VKH_DRAW_CLEAR || op0 == 0)
                 need_read = false;
-            else if (d0.grad < 0 && (d0.op == 1 || d0.op == 2) && d0.opacity == 1.0f && (d0.color >> 24) == 255u)
+            else if (!(d0.meta & VKH_DP_GRAD) && (op0 == 1 || op0 == 2) && (d0.solid >> 24) == 255u)
                 need_read = false;
         }
     }
@@ -542,25 +605,27 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
         const uint32_t nb = min((uint32_t)B, c1 - base);
         // ---- stage the batch's records
         for (uint32_t j = tid; j < nb; j += NT) {
-            Cmd cm = cmds[base + j];
-            const vkh_draw &d = draws[cm.draw];
-            uint32_t kind = d.kind;
-            c_kind[j] = (cm.pt == VKH_NO_DRAW) ? (kind == VKH_DRAW_CLEAR ? VKH_DRAW_CLEAR : VKH_DRAW_PAINT) : kind;
-            c_op[j] = kind == VKH_DRAW_CLEAR ? 0u : d.op;
-            c_color[j] = d.color;
-            c_grad[j] = d.grad;
-            c_opa[j] = d.opacity;
-            c_key[j] = draw_key(d.key_index);
+            const Cmd cm = cmds[base + j];
+            const DrawPrep &d = dprep[cm.draw];
+            uint32_t meta = d.meta;
+            const uint32_t kind = meta & 15;
             if (cm.pt == VKH_NO_DRAW) {
+                meta = (meta & ~15u) | (kind == VKH_DRAW_CLEAR ? VKH_DRAW_CLEAR : VKH_DRAW_PAINT);
                 c_back[j] = 0;
                 c_nseg[j] = 0;
                 c_segoff[j] = 0;
             } else {
-                PathTile p = pts[cm.pt];
+                const PathTile p = pts[cm.pt];
                 c_back[j] = p.backdrop;
                 c_nseg[j] = p.nseg;
                 c_segoff[j] = p.seg_off;
+                if (p.nseg)
+                    meta |= VKH_DP_HASSEG;
             }
+            c_meta[j] = meta;
+            c_solid[j] = d.solid;
+            c_key[j] = d.key;
+            c_draw[j] = cm.draw;
         }
         for (uint32_t k = tid; k < nb * (uint32_t)NPIX; k += NT)
             (&sd[0][0][0])[k] = 0;
@@ -651,22 +716,14 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
         __syncthreads();
         // ---- ordered compositing
         for (uint32_t j = 0; j < nb; j++) {
-            const uint32_t kind = c_kind[j];
+            const uint32_t meta = uniform_u32(c_meta[j]);
+            const uint32_t kind = meta & 15, op = (meta >> 4) & 15;
             const bool full = kind == VKH_DRAW_PAINT || kind == VKH_DRAW_CLEAR;
-            const bool hasseg = c_nseg[j] != 0;
-            const int32_t bk = c_back[j];
-            const int32_t g = c_grad[j];
-            const float op_ = c_opa[j];
-            const uint32_t op = c_op[j];
-            const uint32_t key = c_key[j];
-            uint32_t solid = 0;
-            if (g < 0) {
-                const uint32_t c = c_color[j];
-                solid = (uint32_t)f2u8(((float)(c & 0xFF) / 255.0f) * op_) |
-                        ((uint32_t)f2u8(((float)((c >> 8) & 0xFF) / 255.0f) * op_) << 8) |
-                        ((uint32_t)f2u8(((float)((c >> 16) & 0xFF) / 255.0f) * op_) << 16) |
-                        ((uint32_t)f2u8(((float)((c >> 24) & 0xFF) / 255.0f) * op_) << 24);
-            }
+            const bool hasseg = (meta & VKH_DP_HASSEG) != 0;
+            const int32_t bk = (int32_t)uniform_u32((uint32_t)c_back[j]);
+            const uint32_t key = uniform_u32(c_key[j]);
+            const uint32_t solid = uniform_u32(c_solid[j]);
+            const uint32_t dd = uniform_u32(c_draw[j]);
 #pragma unroll
             for (int q = 0; q < PPT; q++) {
                 int k = 1;
@@ -677,9 +734,11 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
                 if (k <= 0 || !valid[q])
                     continue;
                 uint32_t s = solid;
-                if (g >= 0) {
+                if (meta & VKH_DP_GRAD) {
+                    const DrawPrep &P = dprep[dd];
+                    const float op_ = P.opacity;
                     float col[4];
-                    gradient_color(grads[g], Wf, Hf, (float)(X0 + lx[q]) + 0.5f, (float)(Y0 + ly[q]) + 0.5f, col);
+                    gradient_color(grads[P.grad], P, gx[q], gy[q], col);
                     s = (uint32_t)f2u8(col[0] * op_) | ((uint32_t)f2u8(col[1] * op_) << 8) |
                         ((uint32_t)f2u8(col[2] * op_) << 16) | ((uint32_t)f2u8(col[3] * op_) << 24);
                 }
@@ -760,22 +819,27 @@ void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, const TargetInfo 
         hipLaunchKernelGGL(k_coarse, dim3(n_bins), dim3(256), 0, s, write, tg, n_targets, draws, dt, pts, tile_count,
                            tile_off, cmds);
 }
+void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n, const TargetInfo *tg,
+                      const vkh_gradient *grads, DrawPrep *out) {
+    if (n)
+        hipLaunchKernelGGL(k_draw_prep, dim3((n + 255) / 256), dim3(256), 0, s, draws, n, tg, grads, out);
+}
 void launch_fine(hipStream_t s, uint32_t tile, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
-                 const vkh_draw *draws, const vkh_gradient *grads, const PathTile *pts, const uint32_t *segs,
+                 const DrawPrep *dprep, const vkh_gradient *grads, const PathTile *pts, const uint32_t *segs,
                  const Edge *edges, uint32_t n_edges, const uint32_t *tile_off, const Cmd *cmds) {
     if (!n_tiles)
         return;
     switch (tile) {
     case 8:
-        hipLaunchKernelGGL(k_fine<8>, dim3(n_tiles), dim3(64), 0, s, tg, n_targets, draws, grads, pts, segs, edges,
+        hipLaunchKernelGGL(k_fine<8>, dim3(n_tiles), dim3(64), 0, s, tg, n_targets, dprep, grads, pts, segs, edges,
                            n_edges, tile_off, cmds);
         break;
     case 32:
-        hipLaunchKernelGGL(k_fine<32>, dim3(n_tiles), dim3(256), 0, s, tg, n_targets, draws, grads, pts, segs, edges,
+        hipLaunchKernelGGL(k_fine<32>, dim3(n_tiles), dim3(256), 0, s, tg, n_targets, dprep, grads, pts, segs, edges,
                            n_edges, tile_off, cmds);
         break;
     default:
-        hipLaunchKernelGGL(k_fine<16>, dim3(n_tiles), dim3(256), 0, s, tg, n_targets, draws, grads, pts, segs, edges,
+        hipLaunchKernelGGL(k_fine<16>, dim3(n_tiles), dim3(256), 0, s, tg, n_targets, dprep, grads, pts, segs, edges,
                            n_edges, tile_off, cmds);
         break;
     }
