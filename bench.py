@@ -130,6 +130,20 @@ def cpu_baseline(n_paths: int = 5000):
             "seconds": dt, "nproc": os.cpu_count()}
 
 
+def copy_probe(mib: int) -> int:
+    """A plain elementwise copy of known size (reads and writes mib MiB once per launch) used to
+    calibrate the FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,7 +153,11 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-paths", type=int, default=5000)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy kernel of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print(json.dumps({"error": "this backend renders on one GPU; multi-GPU sharding is design-only "
                                    "(see DESIGN.md), run with --gpus 1"}))

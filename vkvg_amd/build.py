@@ -28,7 +28,7 @@ HIP_FLAGS = [
 C_FLAGS = ["-O2", "-fPIC", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-Wall",
            "-Wno-unused-function", f"-I{INC}"]
 
-HIP_SOURCES = ["vkh_geom.hip", "vkh_raster.hip", "vkh_engine.hip"]
+HIP_SOURCES = ["vkh_geom.hip", "vkh_raster.hip", "vkh_engine.hip", "vkh_diff.hip"]
 C_SOURCES = ["vkvg_api.c"]
 
 

@@ -1,0 +1,995 @@
+// Differentiable renderer of the vkvg HIP backend: analytic area coverage into FP32 RGBA and its
+// backward pass (C ABI of include/vkvg_diff.h). Semantics are stated in that header.
+//
+// Pipeline per forward call:
+//   k_dflatten      one thread per polyline point: Bernstein sample of its cubic
+//   k_dnormals      one thread per point: unit left normal of the segment leaving it (strokes)
+//   k_dedges        one thread per point: the fill edge, or the stroke's rectangle + 2 bevel
+//                   triangles (10 edges), each endpoint tagged (point, normal, sign) for backward
+//   k_dbbox         one workgroup per path: bbox of its edges (deterministic reduction)
+//   draw tiles, scan, k_dbin (per path, LDS counts / scan / cursors; every tile from the edge's
+//   left end to the path's right bbox end, so a tile's list holds all edges left of it), coarse
+//   k_dfine         one workgroup per 16x16 tile: per batch of 8 commands, one thread per
+//                   (command, pixel row) accumulates the row's area deltas in a fixed order and
+//                   prefix-sums them; then one thread per pixel composites in draw order. With
+//                   checkpoints on, the pixel state before every batch is kept.
+// Backward:
+//   k_dback         per tile, batches in reverse: recomputes the batch's coverage, replays the
+//                   batch's compositing from its checkpoint, walks it backwards to get dL/dA per
+//                   (command, pixel) and the colour gradients, then turns dL/dA into edge-endpoint
+//                   gradients with the boundary integral over every edge piece inside a pixel
+//   k_dg_edges, k_dg_normals, k_dg_ctrl, k_dg_paths: chain rule to points, normals, half-widths,
+//                   control points, colours and widths
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vkh_internal.h"
+#include "vkvg_diff.h"
+
+#define VKD_NSEG 16 // polyline segments per cubic
+#define VKD_T 16    // tile size in pixels
+#define VKD_B 8     // commands per batch
+
+namespace {
+
+struct DSrc { // endpoint tags of an edge: point index, normal index | (sign + 1) << 30
+    uint32_t ia, ka, ib, kb;
+};
+
+__device__ __forceinline__ float2 f2(float x, float y) { return make_float2(x, y); }
+
+// --------------------------------------------------------------------------------- geometry
+__global__ void k_dflatten(uint32_t n_pts, const uint32_t *__restrict__ pt_path,
+                           const uint32_t *__restrict__ path_first_pt, const uint32_t *__restrict__ path_first_ctrl,
+                           const uint32_t *__restrict__ path_ncubic, const float2 *__restrict__ ctrl,
+                           float2 *__restrict__ pts) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pts)
+        return;
+    uint32_t path = pt_path[p];
+    uint32_t j = p - path_first_pt[path];
+    uint32_t q = j / VKD_NSEG, i = j % VKD_NSEG;
+    uint32_t n3 = 3 * path_ncubic[path];
+    const float2 *c = ctrl + path_first_ctrl[path];
+    float2 c0 = c[3 * q], c1 = c[3 * q + 1], c2 = c[3 * q + 2], c3 = c[(3 * q + 3) % n3];
+    float t = (float)i / (float)VKD_NSEG, u = 1.0f - t;
+    float b0 = u * u * u, b1 = 3.0f * u * u * t, b2 = 3.0f * u * t * t, b3 = t * t * t;
+    pts[p] = f2(b0 * c0.x + b1 * c1.x + b2 * c2.x + b3 * c3.x, b0 * c0.y + b1 * c1.y + b2 * c2.y + b3 * c3.y);
+}
+
+__device__ __forceinline__ uint32_t next_pt(uint32_t p, uint32_t first, uint32_t end) {
+    return p + 1 == end ? first : p + 1;
+}
+__device__ __forceinline__ uint32_t prev_pt(uint32_t p, uint32_t first, uint32_t end) {
+    return p == first ? end - 1 : p - 1;
+}
+
+__global__ void k_dnormals(uint32_t n_pts, const uint32_t *__restrict__ pt_path,
+                           const uint32_t *__restrict__ path_first_pt, const float2 *__restrict__ pts,
+                           float2 *__restrict__ nrm) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pts)
+        return;
+    uint32_t path = pt_path[p];
+    uint32_t nx = next_pt(p, path_first_pt[path], path_first_pt[path + 1]);
+    float dx = pts[nx].x - pts[p].x, dy = pts[nx].y - pts[p].y;
+    float L = sqrtf(dx * dx + dy * dy);
+    nrm[p] = L > 0.f ? f2(-dy / L, dx / L) : f2(0.f, 0.f);
+}
+
+__device__ __forceinline__ uint32_t tag(uint32_t k, int s) { return k | ((uint32_t)(s + 1) << 30); }
+__device__ __forceinline__ float2 vpos(const float2 *pts, const float2 *nrm, float h, uint32_t i, uint32_t k, int s) {
+    if (!s)
+        return pts[i];
+    float sh = (float)s * h;
+    return f2(pts[i].x + sh * nrm[k].x, pts[i].y + sh * nrm[k].y);
+}
+
+__global__ void k_dedges(uint32_t n_pts, const uint32_t *__restrict__ pt_path,
+                         const uint32_t *__restrict__ path_first_pt, const uint32_t *__restrict__ path_first_edge,
+                         const uint32_t *__restrict__ path_stroke, const float *__restrict__ width,
+                         const float2 *__restrict__ pts, const float2 *__restrict__ nrm, float4 *__restrict__ edges,
+                         DSrc *__restrict__ src) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pts)
+        return;
+    uint32_t path = pt_path[p];
+    uint32_t first = path_first_pt[path], end = path_first_pt[path + 1];
+    uint32_t j = p - first;
+    uint32_t nx = next_pt(p, first, end);
+    if (!path_stroke[path]) {
+        uint32_t e = path_first_edge[path] + j;
+        edges[e] = make_float4(pts[p].x, pts[p].y, pts[nx].x, pts[nx].y);
+        src[e] = DSrc{p, tag(0, 0), nx, tag(0, 0)};
+        return;
+    }
+    const float h = 0.5f * width[path];
+    uint32_t e = path_first_edge[path] + 10 * j;
+    uint32_t vi[4] = {p, nx, nx, p}, vk[4] = {p, p, p, p};
+    int vs[4] = {1, 1, -1, -1};
+    for (int a = 0; a < 4; a++) {
+        int b = (a + 1) & 3;
+        float2 A = vpos(pts, nrm, h, vi[a], vk[a], vs[a]), Bv = vpos(pts, nrm, h, vi[b], vk[b], vs[b]);
+        edges[e] = make_float4(A.x, A.y, Bv.x, Bv.y);
+        src[e] = DSrc{vi[a], tag(vk[a], vs[a]), vi[b], tag(vk[b], vs[b])};
+        e++;
+    }
+    uint32_t pp = prev_pt(p, first, end);
+    float2 n0 = nrm[pp], n1 = nrm[p];
+    bool swap = n0.x * n1.y - n0.y * n1.x > 0.f;
+    for (int s = 1; s >= -1; s -= 2) {
+        uint32_t ti[3] = {p, p, p}, tk[3] = {0, pp, p};
+        int ts[3] = {0, s, s};
+        if (swap) {
+            tk[1] = p;
+            tk[2] = pp;
+        }
+        for (int a = 0; a < 3; a++) {
+            int b = a == 2 ? 0 : a + 1;
+            float2 A = vpos(pts, nrm, h, ti[a], tk[a], ts[a]), Bv = vpos(pts, nrm, h, ti[b], tk[b], ts[b]);
+            edges[e] = make_float4(A.x, A.y, Bv.x, Bv.y);
+            src[e] = DSrc{ti[a], tag(tk[a], ts[a]), ti[b], tag(tk[b], ts[b])};
+            e++;
+        }
+    }
+}
+
+// bbox of a path's edges in 24.8 fixed point (floor / ceil), one workgroup per path
+__global__ __launch_bounds__(256) void k_dbbox(const uint32_t *__restrict__ path_first_edge,
+                                               const float4 *__restrict__ edges, int4 *__restrict__ bbox) {
+    __shared__ int32_t red[4][4];
+    uint32_t d = blockIdx.x, tid = threadIdx.x;
+    float x0 = 3.0e38f, y0 = 3.0e38f, x1 = -3.0e38f, y1 = -3.0e38f;
+    for (uint32_t e = path_first_edge[d] + tid; e < path_first_edge[d + 1]; e += 256) {
+        float4 v = edges[e];
+        if (v.y == v.w || isnan(v.x) || isnan(v.y) || isnan(v.z) || isnan(v.w))
+            continue; // horizontal edges contribute nothing
+        x0 = fminf(x0, fminf(v.x, v.z));
+        x1 = fmaxf(x1, fmaxf(v.x, v.z));
+        y0 = fminf(y0, fminf(v.y, v.w));
+        y1 = fmaxf(y1, fmaxf(v.y, v.w));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        x0 = fminf(x0, __shfl_xor(x0, o));
+        y0 = fminf(y0, __shfl_xor(y0, o));
+        x1 = fmaxf(x1, __shfl_xor(x1, o));
+        y1 = fmaxf(y1, __shfl_xor(y1, o));
+    }
+    auto fx = [](float v, bool up) -> int32_t {
+        float s = v * 256.0f;
+        s = up ? ceilf(s) : floorf(s);
+        return (int32_t)fminf(fmaxf(s, -8.0e6f), 8.0e6f);
+    };
+    if ((tid & 63) == 0) {
+        int w = tid >> 6;
+        red[w][0] = x0 > x1 ? INT32_MAX : fx(x0, false);
+        red[w][1] = x0 > x1 ? INT32_MAX : fx(y0, false);
+        red[w][2] = x0 > x1 ? INT32_MIN : fx(x1, true);
+        red[w][3] = x0 > x1 ? INT32_MIN : fx(y1, true);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int4 b = make_int4(INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN);
+        for (int w = 0; w < 4; w++) {
+            b.x = min(b.x, red[w][0]);
+            b.y = min(b.y, red[w][1]);
+            b.z = max(b.z, red[w][2]);
+            b.w = max(b.w, red[w][3]);
+        }
+        bbox[d] = b;
+    }
+}
+
+// ---------------------------------------------------------------------------------- binning
+// One workgroup per path, in bands of whole tile rows fitting the LDS table: an edge goes into
+// every tile of the rows it spans from its leftmost tile to the bbox's right end.
+#define VKD_BIN_LDS 4096
+__device__ __forceinline__ int32_t tile_of(float v) { // clamped before the conversion
+    return (int32_t)fminf(fmaxf(floorf(v / VKD_T), -1.0e6f), 1.0e6f);
+}
+// tile rows and first tile column of an edge; empty (r0 > r1) for edges with a NaN coordinate
+__device__ __forceinline__ void dbin_range(const float4 &v, const DrawTiles &dt, int32_t ty_lo, int32_t ty_hi,
+                                           int32_t &r0, int32_t &r1, int32_t &c0) {
+    if (isnan(v.x) || isnan(v.y) || isnan(v.z) || isnan(v.w)) {
+        r0 = 1;
+        r1 = 0;
+        c0 = 0;
+        return;
+    }
+    float ymin = fminf(v.y, v.w), ymax = fmaxf(v.y, v.w), xmin = fminf(v.x, v.z);
+    r0 = max(tile_of(ymin), ty_lo);
+    r1 = min(tile_of(ymax), ty_hi);
+    c0 = max(tile_of(xmin), dt.bx0);
+}
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t *a, uint32_t n, uint32_t *s_wave) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t beg = min(tid * 16u, n), end = min(beg + 16u, n);
+    uint32_t sum = 0;
+    for (uint32_t k = beg; k < end; k++)
+        sum += a[k];
+    uint32_t x = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if ((int)lane >= o)
+            x += y;
+    }
+    if (lane == 63)
+        s_wave[w] = x;
+    __syncthreads();
+    uint32_t run = x - sum, total = 0;
+    for (uint32_t k = 0; k < 4; k++) {
+        run += k < w ? s_wave[k] : 0u;
+        total += s_wave[k];
+    }
+    for (uint32_t k = beg; k < end; k++) {
+        uint32_t v = a[k];
+        a[k] = run;
+        run += v;
+    }
+    __syncthreads();
+    return total;
+}
+
+__global__ __launch_bounds__(256) void k_dbin(const uint32_t *__restrict__ path_first_edge,
+                                              const float4 *__restrict__ edges, const DrawTiles *__restrict__ dts,
+                                              PathTile *__restrict__ ptl, uint32_t *__restrict__ segs, uint32_t seg_cap,
+                                              uint32_t *__restrict__ seg_alloc) {
+    __shared__ uint32_t s_cnt[VKD_BIN_LDS];
+    __shared__ uint32_t s_wave[4];
+    __shared__ uint32_t s_base;
+    const uint32_t d = blockIdx.x, tid = threadIdx.x;
+    const DrawTiles dt = dts[d];
+    if (dt.bw == 0 || dt.bx0 > dt.bx1 || dt.by0 > dt.by1)
+        return;
+    const uint32_t e0 = path_first_edge[d], e1 = path_first_edge[d + 1];
+    const uint32_t bw = dt.bw, bh = (uint32_t)(dt.by1 - dt.by0 + 1);
+    const uint32_t band_rows = max(1u, VKD_BIN_LDS / bw);
+    for (uint32_t rb = 0; rb < bh; rb += band_rows) {
+        const uint32_t nr = min(band_rows, bh - rb), nt = nr * bw;
+        const int32_t ty_lo = dt.by0 + (int32_t)rb, ty_hi = ty_lo + (int32_t)nr - 1;
+        for (uint32_t k = tid; k < nt; k += 256)
+            s_cnt[k] = 0;
+        __syncthreads();
+        for (uint32_t e = e0 + tid; e < e1; e += 256) {
+            float4 v = edges[e];
+            if (v.y == v.w)
+                continue;
+            int32_t r0, r1, c0;
+            dbin_range(v, dt, ty_lo, ty_hi, r0, r1, c0);
+            for (int32_t r = r0; r <= r1; r++)
+                for (int32_t c = c0; c <= dt.bx1; c++)
+                    atomicAdd(&s_cnt[(uint32_t)(r - ty_lo) * bw + (uint32_t)(c - dt.bx0)], 1u);
+        }
+        __syncthreads();
+        const uint32_t total = block_excl_scan(s_cnt, nt, s_wave);
+        if (tid == 0)
+            s_base = atomicAdd(seg_alloc, total);
+        __syncthreads();
+        const uint32_t base = s_base;
+        const bool fits = (uint64_t)base + total <= (uint64_t)seg_cap;
+        PathTile *out = ptl + dt.pt_base + rb * bw;
+        for (uint32_t k = tid; k < nt; k += 256) {
+            uint32_t off = s_cnt[k], next = k + 1 < nt ? s_cnt[k + 1] : total;
+            PathTile pt;
+            pt.backdrop = 0;
+            pt.nseg = next - off;
+            pt.seg_off = base + off;
+            pt.reserved = 0;
+            out[k] = pt;
+        }
+        __syncthreads();
+        if (fits) {
+            for (uint32_t e = e0 + tid; e < e1; e += 256) {
+                float4 v = edges[e];
+                if (v.y == v.w)
+                    continue;
+                int32_t r0, r1, c0;
+                dbin_range(v, dt, ty_lo, ty_hi, r0, r1, c0);
+                for (int32_t r = r0; r <= r1; r++)
+                    for (int32_t c = c0; c <= dt.bx1; c++)
+                        segs[base + atomicAdd(&s_cnt[(uint32_t)(r - ty_lo) * bw + (uint32_t)(c - dt.bx0)], 1u)] = e;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------------------- edge pieces
+// Calls f(lc, dy, fm, s0, s1) for the pieces of edge v inside pixel row [yr, yr + 1): lc = -1 for
+// the part left of the tile column X0 (merged), 0..T-1 for tile cells; dy = (s1 - s0)(by - ay)
+// signed; fm the piece's mean x inside its cell; s0 < s1 its parameter range on the edge.
+template <typename F>
+__device__ __forceinline__ void row_pieces(const float4 &v, float yr, float X0, F f) {
+    const float ax = v.x, ay = v.y, bx = v.z, by = v.w;
+    if (ay == by)
+        return;
+    const float ylo = fmaxf(fminf(ay, by), yr), yhi = fminf(fmaxf(ay, by), yr + 1.0f);
+    if (!(ylo < yhi))
+        return;
+    const float dyt = by - ay, dxt = bx - ax;
+    const float sa = (ylo - ay) / dyt, sb = (yhi - ay) / dyt;
+    const float s0 = fminf(sa, sb), s1 = fmaxf(sa, sb);
+    const float x0 = ax + s0 * dxt, x1 = ax + s1 * dxt;
+    const float xl = fminf(x0, x1), xr = fmaxf(x0, x1);
+    const bool vert = bx == ax;
+    auto emit = [&](int lc, float xa, float xb) {
+        float p0 = s0, p1 = s1;
+        if (!vert) {
+            float u = (xa - ax) / dxt, w = (xb - ax) / dxt;
+            p0 = fminf(u, w);
+            p1 = fmaxf(u, w);
+        }
+        float dy = (p1 - p0) * dyt;
+        float fm = lc >= 0 ? 0.5f * (xa + xb) - (X0 + (float)lc) : 0.f;
+        f(lc, dy, fm, p0, p1);
+    };
+    if (xl < X0)
+        emit(-1, xl, fminf(xr, X0));
+    const int32_t cl = (int32_t)floorf(xl) - (int32_t)X0;
+    const int32_t cr = (xr > xl ? (int32_t)floorf(xr) : (int32_t)floorf(xl)) - (int32_t)X0;
+    for (int32_t c = max(cl, 0); c <= min(cr, VKD_T - 1); c++) {
+        const float cx = X0 + (float)c;
+        const float xa = fmaxf(xl, cx), xb = fminf(xr, cx + 1.0f);
+        if (xb < xa || (xb == xa && xr > xl))
+            continue;
+        emit(c, xa, xb);
+    }
+}
+
+// ---------------------------------------------------------------------------------- forward
+__device__ __forceinline__ float4 premul(float4 c) { return make_float4(c.x * c.w, c.y * c.w, c.z * c.w, c.w); }
+
+// coverage of batch commands: thread (j, r) owns row r of command j (deterministic order)
+__device__ __forceinline__ void batch_coverage(float (*acc)[VKD_T][VKD_T + 1], uint32_t nb, const uint32_t *c_nseg,
+                                               const uint32_t *c_segoff, const uint32_t *__restrict__ segs,
+                                               const float4 *__restrict__ edges, float X0, float Y0) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t j = tid / VKD_T, r = tid % VKD_T;
+    if (j >= nb)
+        return;
+    float *a = acc[j][r];
+    for (int c = 0; c <= VKD_T; c++)
+        a[c] = 0.f;
+    const float yr = Y0 + (float)r;
+    const uint32_t n = c_nseg[j], off = c_segoff[j];
+    for (uint32_t s = 0; s < n; s++) {
+        const float4 v = edges[segs[off + s]];
+        row_pieces(v, yr, X0, [&](int lc, float dy, float fm, float, float) {
+            if (lc < 0) {
+                a[0] += dy;
+            } else {
+                a[lc] += dy * (1.0f - fm);
+                a[lc + 1] += dy * fm;
+            }
+        });
+    }
+    float run = 0.f;
+    for (int c = 0; c < VKD_T; c++) {
+        run += a[c];
+        a[c] = run;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dfine(uint32_t W, uint32_t H, uint32_t tilesW,
+                                               const uint32_t *__restrict__ tile_off, const Cmd *__restrict__ cmds,
+                                               const PathTile *__restrict__ ptl, const uint32_t *__restrict__ segs,
+                                               const float4 *__restrict__ edges, const float4 *__restrict__ rgba,
+                                               float4 *__restrict__ image, float4 *__restrict__ ckpt,
+                                               const uint32_t *__restrict__ batch_off) {
+    __shared__ float acc[VKD_B][VKD_T][VKD_T + 1];
+    __shared__ uint32_t c_nseg[VKD_B], c_segoff[VKD_B], c_draw[VKD_B];
+    const uint32_t t = blockIdx.x, tid = threadIdx.x;
+    const uint32_t c0 = tile_off[t], c1 = tile_off[t + 1];
+    const int32_t X0 = (int32_t)(t % tilesW) * VKD_T, Y0 = (int32_t)(t / tilesW) * VKD_T;
+    const int32_t px = X0 + (int32_t)(tid % VKD_T), py = Y0 + (int32_t)(tid / VKD_T);
+    const bool valid = px < (int32_t)W && py < (int32_t)H;
+    float4 st = make_float4(1.f, 1.f, 1.f, 1.f);
+    uint32_t b = 0;
+    for (uint32_t base = c0; base < c1; base += VKD_B, b++) {
+        const uint32_t nb = min((uint32_t)VKD_B, c1 - base);
+        if (ckpt)
+            ckpt[(size_t)(batch_off[t] + b) * 256 + tid] = st;
+        if (tid < nb) {
+            Cmd cm = cmds[base + tid];
+            PathTile p = ptl[cm.pt];
+            c_draw[tid] = cm.draw;
+            c_nseg[tid] = p.nseg;
+            c_segoff[tid] = p.seg_off;
+        }
+        __syncthreads();
+        batch_coverage(acc, nb, c_nseg, c_segoff, segs, edges, (float)X0, (float)Y0);
+        __syncthreads();
+        for (uint32_t j = 0; j < nb; j++) {
+            const float A = acc[j][tid / VKD_T][tid % VKD_T];
+            const float k = fminf(1.0f, fabsf(A));
+            const float4 c = premul(rgba[c_draw[j]]);
+            const float ia = 1.0f - c.w * k;
+            st = make_float4(c.x * k + st.x * ia, c.y * k + st.y * ia, c.z * k + st.z * ia, c.w * k + st.w * ia);
+        }
+        __syncthreads();
+    }
+    if (valid)
+        image[(size_t)py * W + px] = st;
+}
+
+// --------------------------------------------------------------------------------- backward
+// sums each of v[0..31] over the 64 lanes of the wavefront with 31 + 1 shuffles; lane l returns
+// the total of value index (l >> 1) & 31, written to *idx
+__device__ __forceinline__ float wave_multi_sum32(float (&v)[32], uint32_t lane, uint32_t *idx) {
+    uint32_t base = 0;
+#pragma unroll
+    for (int step = 0; step < 5; step++) {
+        const int o = 32 >> step;  // 32, 16, 8, 4, 2
+        const int half = 16 >> step; // values kept after this step
+        const bool hi = (lane & (uint32_t)o) != 0;
+#pragma unroll
+        for (int i = 0; i < half; i++) {
+            float send = hi ? v[i] : v[i + half];
+            float keep = hi ? v[i + half] : v[i];
+            v[i] = keep + __shfl_xor(send, o);
+        }
+        if (hi)
+            base += (uint32_t)half;
+    }
+    *idx = base;
+    return v[0] + __shfl_xor(v[0], 1);
+}
+
+__global__ __launch_bounds__(256) void k_dback(uint32_t W, uint32_t H, uint32_t tilesW,
+                                               const uint32_t *__restrict__ tile_off, const Cmd *__restrict__ cmds,
+                                               const PathTile *__restrict__ ptl, const uint32_t *__restrict__ segs,
+                                               const float4 *__restrict__ edges, const float4 *__restrict__ rgba,
+                                               const float4 *__restrict__ grad_img, const float4 *__restrict__ ckpt,
+                                               const uint32_t *__restrict__ batch_off, float4 *__restrict__ g_edge,
+                                               float *__restrict__ g_col) {
+    __shared__ float acc[VKD_B][VKD_T][VKD_T + 1];
+    __shared__ float G[VKD_B][VKD_T][VKD_T];
+    __shared__ float s_red[4][32];
+    __shared__ uint32_t c_nseg[VKD_B], c_segoff[VKD_B], c_draw[VKD_B];
+    const uint32_t t = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const uint32_t c0 = tile_off[t], c1 = tile_off[t + 1];
+    if (c0 == c1)
+        return;
+    const int32_t X0 = (int32_t)(t % tilesW) * VKD_T, Y0 = (int32_t)(t / tilesW) * VKD_T;
+    const int32_t px = X0 + (int32_t)(tid % VKD_T), py = Y0 + (int32_t)(tid / VKD_T);
+    const bool valid = px < (int32_t)W && py < (int32_t)H;
+    float4 g = valid ? grad_img[(size_t)py * W + px] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t nbatch = (c1 - c0 + VKD_B - 1) / VKD_B;
+    for (uint32_t bb = nbatch; bb-- > 0;) {
+        const uint32_t base = c0 + bb * VKD_B;
+        const uint32_t nb = min((uint32_t)VKD_B, c1 - base);
+        if (tid < nb) {
+            Cmd cm = cmds[base + tid];
+            PathTile p = ptl[cm.pt];
+            c_draw[tid] = cm.draw;
+            c_nseg[tid] = p.nseg;
+            c_segoff[tid] = p.seg_off;
+        }
+        __syncthreads();
+        batch_coverage(acc, nb, c_nseg, c_segoff, segs, edges, (float)X0, (float)Y0);
+        __syncthreads();
+        // replay the batch's compositing from its checkpoint, then walk it backwards
+        float4 st[VKD_B + 1];
+        st[0] = ckpt[(size_t)(batch_off[t] + bb) * 256 + tid];
+        float Av[VKD_B];
+#pragma unroll
+        for (uint32_t j = 0; j < VKD_B; j++) {
+            Av[j] = j < nb ? acc[j][tid / VKD_T][tid % VKD_T] : 0.f;
+            if (j < nb) {
+                const float k = fminf(1.0f, fabsf(Av[j]));
+                const float4 c = premul(rgba[c_draw[j]]);
+                const float ia = 1.0f - c.w * k;
+                st[j + 1] = make_float4(c.x * k + st[j].x * ia, c.y * k + st[j].y * ia, c.z * k + st[j].z * ia,
+                                        c.w * k + st[j].w * ia);
+            } else {
+                st[j + 1] = st[j];
+            }
+        }
+        float gc[32];
+#pragma unroll
+        for (int q = 0; q < 32; q++)
+            gc[q] = 0.f;
+#pragma unroll
+        for (int jj = VKD_B - 1; jj >= 0; jj--) {
+            const uint32_t j = (uint32_t)jj;
+            if (j >= nb)
+                continue;
+            const float A = Av[j];
+            const float k = fminf(1.0f, fabsf(A));
+            const float fp = fabsf(A) < 1.0f ? (A > 0.f ? 1.f : (A < 0.f ? -1.f : 0.f)) : 0.f;
+            const float4 c = premul(rgba[c_draw[j]]);
+            const float4 s = st[j];
+            float dk = 0.f, gs = 0.f;
+            if (valid) {
+                dk = g.x * (c.x - c.w * s.x) + g.y * (c.y - c.w * s.y) + g.z * (c.z - c.w * s.z) + g.w * (c.w - c.w * s.w);
+                gs = g.x * s.x + g.y * s.y + g.z * s.z + g.w * s.w;
+                gc[4 * j + 0] = g.x * k;
+                gc[4 * j + 1] = g.y * k;
+                gc[4 * j + 2] = g.z * k;
+                gc[4 * j + 3] = g.w * k - gs * k;
+            }
+            G[j][tid / VKD_T][tid % VKD_T] = valid ? dk * fp : 0.f;
+            const float ia = 1.0f - c.w * k;
+            g = make_float4(g.x * ia, g.y * ia, g.z * ia, g.w * ia);
+        }
+        // colour gradients: reduce the 8 commands x 4 channels over the tile
+        uint32_t idx;
+        const float tot = wave_multi_sum32(gc, lane, &idx);
+        if ((lane & 1) == 0)
+            s_red[tid >> 6][idx] = tot;
+        __syncthreads();
+        if (tid < 4 * nb) {
+            const float v = s_red[0][tid] + s_red[1][tid] + s_red[2][tid] + s_red[3][tid];
+            if (v != 0.f)
+                atomicAdd(&g_col[4 * c_draw[tid >> 2] + (tid & 3)], v);
+        }
+        // edge-endpoint gradients: thread (j, r) integrates dL/dA along the pieces in its row
+        {
+            const uint32_t j = tid / VKD_T, r = tid % VKD_T;
+            if (j < nb) {
+                const float yr = (float)(Y0 + (int32_t)r);
+                const uint32_t n = c_nseg[j], off = c_segoff[j];
+                for (uint32_t s = 0; s < n; s++) {
+                    const uint32_t ei = segs[off + s];
+                    const float4 v = edges[ei];
+                    float ga = 0.f, gb = 0.f; // scalar factors of (-dy, dx)
+                    row_pieces(v, yr, (float)X0, [&](int lc, float, float, float p0, float p1) {
+                        if (lc < 0)
+                            return;
+                        const float Gv = G[j][r][lc];
+                        const float I1 = 0.5f * (p1 * p1 - p0 * p0);
+                        const float I0 = (p1 - p0) - I1;
+                        ga += Gv * I0;
+                        gb += Gv * I1;
+                    });
+                    if (ga != 0.f || gb != 0.f) {
+                        const float dx = v.z - v.x, dy = v.w - v.y;
+                        float *ge = (float *)(g_edge + ei);
+                        atomicAdd(ge + 0, -dy * ga);
+                        atomicAdd(ge + 1, dx * ga);
+                        atomicAdd(ge + 2, -dy * gb);
+                        atomicAdd(ge + 3, dx * gb);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------------------- chain rule
+__global__ void k_dg_edges(uint32_t n_edges, const DSrc *__restrict__ src, const float4 *__restrict__ g_edge,
+                           const uint32_t *__restrict__ pt_path, const float *__restrict__ width,
+                           const float2 *__restrict__ nrm, float2 *__restrict__ g_pt, float2 *__restrict__ g_n,
+                           float *__restrict__ g_h) {
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_edges)
+        return;
+    const float4 ge = g_edge[e];
+    if (ge.x == 0.f && ge.y == 0.f && ge.z == 0.f && ge.w == 0.f)
+        return;
+    const DSrc s = src[e];
+    const uint32_t ii[2] = {s.ia, s.ib}, kk[2] = {s.ka, s.kb};
+    const float gx[2] = {ge.x, ge.z}, gy[2] = {ge.y, ge.w};
+    for (int a = 0; a < 2; a++) {
+        atomicAdd(&g_pt[ii[a]].x, gx[a]);
+        atomicAdd(&g_pt[ii[a]].y, gy[a]);
+        const int sg = (int)(kk[a] >> 30) - 1;
+        if (sg) {
+            const uint32_t k = kk[a] & 0x3FFFFFFFu;
+            const uint32_t path = pt_path[ii[a]];
+            const float h = 0.5f * width[path];
+            atomicAdd(&g_n[k].x, (float)sg * h * gx[a]);
+            atomicAdd(&g_n[k].y, (float)sg * h * gy[a]);
+            atomicAdd(&g_h[path], (float)sg * (nrm[k].x * gx[a] + nrm[k].y * gy[a]));
+        }
+    }
+}
+
+// normal n = (-d.y, d.x)/|d| of the segment p -> next: g_d = ((g.y, -g.x) - d (n.g)/|d|)/|d|
+__global__ void k_dg_normals(uint32_t n_pts, const uint32_t *__restrict__ pt_path,
+                             const uint32_t *__restrict__ path_first_pt, const uint32_t *__restrict__ path_stroke,
+                             const float2 *__restrict__ pts, const float2 *__restrict__ nrm,
+                             const float2 *__restrict__ g_n, float2 *__restrict__ g_pt) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pts)
+        return;
+    const uint32_t path = pt_path[p];
+    if (!path_stroke[path])
+        return;
+    const float2 gn = g_n[p];
+    if (gn.x == 0.f && gn.y == 0.f)
+        return;
+    const uint32_t nx = next_pt(p, path_first_pt[path], path_first_pt[path + 1]);
+    const float dx = pts[nx].x - pts[p].x, dy = pts[nx].y - pts[p].y;
+    const float L = sqrtf(dx * dx + dy * dy);
+    if (!(L > 0.f))
+        return;
+    const float ng = nrm[p].x * gn.x + nrm[p].y * gn.y;
+    const float gdx = (gn.y - dx * ng / L) / L, gdy = (-gn.x - dy * ng / L) / L;
+    atomicAdd(&g_pt[p].x, -gdx);
+    atomicAdd(&g_pt[p].y, -gdy);
+    atomicAdd(&g_pt[nx].x, gdx);
+    atomicAdd(&g_pt[nx].y, gdy);
+}
+
+// one thread per control point: gathers the Bernstein-weighted point gradients of its cubic(s)
+__global__ void k_dg_ctrl(uint32_t n_ctrl, const uint32_t *__restrict__ ctrl_path,
+                          const uint32_t *__restrict__ path_first_ctrl, const uint32_t *__restrict__ path_first_pt,
+                          const uint32_t *__restrict__ path_ncubic, const float2 *__restrict__ g_pt,
+                          float2 *__restrict__ g_ctrl) {
+    uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= n_ctrl)
+        return;
+    const uint32_t path = ctrl_path[ci];
+    const uint32_t local = ci - path_first_ctrl[path], n = path_ncubic[path];
+    const uint32_t q = local / 3, m = local % 3;
+    const float2 *gp = g_pt + path_first_pt[path];
+    float gx = 0.f, gy = 0.f;
+    for (int i = 0; i < VKD_NSEG; i++) {
+        const float t = (float)i / (float)VKD_NSEG, u = 1.0f - t;
+        const float B[4] = {u * u * u, 3.0f * u * u * t, 3.0f * u * t * t, t * t * t};
+        const float2 a = gp[q * VKD_NSEG + i];
+        gx += B[m] * a.x;
+        gy += B[m] * a.y;
+        if (m == 0) { // also the end point of the previous cubic
+            const uint32_t qp = (q + n - 1) % n;
+            const float2 bq = gp[qp * VKD_NSEG + i];
+            gx += B[3] * bq.x;
+            gy += B[3] * bq.y;
+        }
+    }
+    g_ctrl[ci] = f2(gx, gy);
+}
+
+// per path: colour chain c = (r a, g a, b a, a) and width = 2 h
+__global__ void k_dg_paths(uint32_t n_paths, const float4 *__restrict__ rgba, const float *__restrict__ g_col,
+                           const float *__restrict__ g_h, const uint32_t *__restrict__ path_stroke,
+                           float4 *__restrict__ g_rgba, float *__restrict__ g_width) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_paths)
+        return;
+    const float4 c = rgba[p];
+    const float g0 = g_col[4 * p], g1 = g_col[4 * p + 1], g2 = g_col[4 * p + 2], g3 = g_col[4 * p + 3];
+    g_rgba[p] = make_float4(c.w * g0, c.w * g1, c.w * g2, c.x * g0 + c.y * g1 + c.z * g2 + g3);
+    g_width[p] = path_stroke[p] ? 0.5f * g_h[p] : 0.f;
+}
+
+__global__ void k_dbatches(uint32_t n_tiles, const uint32_t *__restrict__ tile_off, uint32_t *__restrict__ nb) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n_tiles)
+        nb[t] = (tile_off[t + 1] - tile_off[t] + VKD_B - 1) / VKD_B;
+}
+
+__global__ void k_dzero(uint32_t *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0;
+}
+
+struct Buf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+} // namespace
+
+struct vkvg_diff_scene_t {
+    uint32_t W = 0, H = 0, n_paths = 0, n_pts = 0, n_ctrl = 0, n_edges = 0, tilesW = 0, tilesH = 0, n_tiles = 0;
+    uint32_t bin_total = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};
+    std::string err;
+    float ms_fwd = 0, ms_bwd = 0;
+    uint32_t n_segs = 0, n_cmds = 0, n_batches = 0;
+    bool have_ckpt = false;
+    // static topology
+    Buf pt_path, ctrl_path, first_pt, first_ctrl, first_edge, ncubic, stroke, draws, tg;
+    // per call
+    Buf pts, nrm, edges, src, bbox, dtiles, pt_count, pt_off, ptiles, segs, seg_alloc, tile_cnt, tile_off, cmds, nbat,
+        batch_off, ckpt, scan_tmp, host_tot;
+    Buf g_edge, g_pt, g_n, g_h, g_col;
+};
+
+namespace {
+
+bool ck(vkvg_diff_scene_t *s, hipError_t e, const char *what) {
+    if (e == hipSuccess)
+        return true;
+    s->err = std::string(what) + ": " + hipGetErrorString(e);
+    return false;
+}
+bool ensure(vkvg_diff_scene_t *s, Buf &b, size_t bytes) {
+    if (b.p && bytes <= b.cap)
+        return true;
+    if (b.p)
+        (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    size_t cap = bytes + bytes / 4 + 256;
+    if (!ck(s, hipMalloc(&b.p, cap), "hipMalloc"))
+        return false;
+    b.cap = cap;
+    return true;
+}
+template <typename T> T *P(Buf &b) { return (T *)b.p; }
+bool upload(vkvg_diff_scene_t *s, Buf &b, const void *h, size_t bytes) {
+    return ensure(s, b, bytes) && ck(s, hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice), "upload");
+}
+bool read_u32(vkvg_diff_scene_t *s, const uint32_t *dptr, uint32_t *out) {
+    if (!ck(s, hipMemcpyAsync(out, dptr, 4, hipMemcpyDeviceToHost, s->stream), "readback"))
+        return false;
+    return ck(s, hipStreamSynchronize(s->stream), "sync");
+}
+bool scan(vkvg_diff_scene_t *s, const uint32_t *in, uint32_t n, uint32_t *out) {
+    if (!ensure(s, s->scan_tmp, vkh::scan_temp_words(n) * 4))
+        return false;
+    vkh::scan_u32(s->stream, in, n, out, P<uint32_t>(s->scan_tmp));
+    return ck(s, hipGetLastError(), "scan");
+}
+void zero(vkvg_diff_scene_t *s, Buf &b, size_t words) {
+    if (words)
+        hipLaunchKernelGGL(k_dzero, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s->stream, P<uint32_t>(b),
+                           words);
+}
+unsigned blocks(size_t n) { return (unsigned)((n + 255) / 256); }
+
+} // namespace
+
+extern "C" {
+
+VkvgDiffScene vkvg_diff_create(uint32_t width, uint32_t height, uint32_t n_paths, const uint32_t *ncubic,
+                               const uint32_t *is_stroke) {
+    if (!width || !height || width > 16384 || height > 16384 || !n_paths || !ncubic || !is_stroke)
+        return nullptr;
+    auto *s = new vkvg_diff_scene_t();
+    s->W = width;
+    s->H = height;
+    s->n_paths = n_paths;
+    s->tilesW = (width + VKD_T - 1) / VKD_T;
+    s->tilesH = (height + VKD_T - 1) / VKD_T;
+    s->n_tiles = s->tilesW * s->tilesH;
+    std::vector<uint32_t> first_pt(n_paths + 1), first_ctrl(n_paths + 1), first_edge(n_paths + 1), pt_path, ctrl_path;
+    std::vector<uint32_t> stroke(n_paths), nc(n_paths);
+    uint64_t np = 0, nctl = 0, ne = 0;
+    for (uint32_t i = 0; i < n_paths; i++) {
+        if (ncubic[i] == 0 || ncubic[i] > (1u << 20)) {
+            delete s;
+            return nullptr;
+        }
+        first_pt[i] = (uint32_t)np;
+        first_ctrl[i] = (uint32_t)nctl;
+        first_edge[i] = (uint32_t)ne;
+        nc[i] = ncubic[i];
+        stroke[i] = is_stroke[i] ? 1u : 0u;
+        uint64_t m = (uint64_t)ncubic[i] * VKD_NSEG;
+        np += m;
+        nctl += 3ull * ncubic[i];
+        ne += stroke[i] ? 10 * m : m;
+        if (np > (1u << 28) || ne > (1u << 30)) {
+            delete s;
+            return nullptr;
+        }
+        pt_path.insert(pt_path.end(), (size_t)m, i);
+        ctrl_path.insert(ctrl_path.end(), 3 * (size_t)ncubic[i], i);
+    }
+    first_pt[n_paths] = (uint32_t)np;
+    first_ctrl[n_paths] = (uint32_t)nctl;
+    first_edge[n_paths] = (uint32_t)ne;
+    s->n_pts = (uint32_t)np;
+    s->n_ctrl = (uint32_t)nctl;
+    s->n_edges = (uint32_t)ne;
+    // draw records for the shared tile / coarse kernels: one target, kind per path
+    std::vector<vkh_draw> dr(n_paths);
+    memset(dr.data(), 0, sizeof(vkh_draw) * n_paths);
+    for (uint32_t i = 0; i < n_paths; i++) {
+        dr[i].kind = stroke[i] ? VKH_DRAW_STROKE : VKH_DRAW_FILL_NZ;
+        dr[i].target = 0;
+        dr[i].grad = -1;
+    }
+    TargetInfo ti;
+    memset(&ti, 0, sizeof(ti));
+    ti.width = width;
+    ti.height = height;
+    ti.tilesW = s->tilesW;
+    ti.tilesH = s->tilesH;
+    ti.tile_base = 0;
+    ti.binsW = (s->tilesW + VKH_BIN_TILES - 1) / VKH_BIN_TILES;
+    ti.binsH = (s->tilesH + VKH_BIN_TILES - 1) / VKH_BIN_TILES;
+    ti.bin_base = 0;
+    ti.draw_begin = 0;
+    ti.draw_end = n_paths;
+    s->bin_total = ti.binsW * ti.binsH;
+    bool ok = ck(s, hipStreamCreate(&s->stream), "stream");
+    for (int k = 0; ok && k < 4; k++)
+        ok = ck(s, hipEventCreate(&s->ev[k]), "event");
+    ok = ok && upload(s, s->pt_path, pt_path.data(), 4ull * np) && upload(s, s->ctrl_path, ctrl_path.data(), 4ull * nctl) &&
+         upload(s, s->first_pt, first_pt.data(), 4ull * (n_paths + 1)) &&
+         upload(s, s->first_ctrl, first_ctrl.data(), 4ull * (n_paths + 1)) &&
+         upload(s, s->first_edge, first_edge.data(), 4ull * (n_paths + 1)) &&
+         upload(s, s->ncubic, nc.data(), 4ull * n_paths) && upload(s, s->stroke, stroke.data(), 4ull * n_paths) &&
+         upload(s, s->draws, dr.data(), sizeof(vkh_draw) * n_paths) && upload(s, s->tg, &ti, sizeof(ti)) &&
+         ensure(s, s->pts, 8ull * np) && ensure(s, s->nrm, 8ull * np) && ensure(s, s->edges, 16ull * ne) &&
+         ensure(s, s->src, sizeof(DSrc) * ne) && ensure(s, s->bbox, 16ull * n_paths) &&
+         ensure(s, s->dtiles, sizeof(DrawTiles) * n_paths) && ensure(s, s->pt_count, 4ull * (n_paths + 1)) &&
+         ensure(s, s->pt_off, 4ull * (n_paths + 1)) && ensure(s, s->seg_alloc, 256) && ensure(s, s->segs, 4096) &&
+         ensure(s, s->tile_cnt, 4ull * (s->n_tiles + 1)) && ensure(s, s->tile_off, 4ull * (s->n_tiles + 1)) &&
+         ensure(s, s->nbat, 4ull * (s->n_tiles + 1)) && ensure(s, s->batch_off, 4ull * (s->n_tiles + 1)) &&
+         ensure(s, s->g_edge, 16ull * ne) && ensure(s, s->g_pt, 8ull * np) && ensure(s, s->g_n, 8ull * np) &&
+         ensure(s, s->g_h, 4ull * n_paths) && ensure(s, s->g_col, 16ull * n_paths);
+    if (!ok) {
+        fprintf(stderr, "vkvg_diff_create: %s\n", s->err.c_str());
+        vkvg_diff_destroy(s);
+        return nullptr;
+    }
+    return s;
+}
+
+void vkvg_diff_destroy(VkvgDiffScene s) {
+    if (!s)
+        return;
+    if (s->stream)
+        (void)hipStreamSynchronize(s->stream);
+    Buf *bufs[] = {&s->pt_path, &s->ctrl_path, &s->first_pt, &s->first_ctrl, &s->first_edge, &s->ncubic, &s->stroke,
+                   &s->draws, &s->tg, &s->pts, &s->nrm, &s->edges, &s->src, &s->bbox, &s->dtiles, &s->pt_count,
+                   &s->pt_off, &s->ptiles, &s->segs, &s->seg_alloc, &s->tile_cnt, &s->tile_off, &s->cmds, &s->nbat,
+                   &s->batch_off, &s->ckpt, &s->scan_tmp, &s->host_tot, &s->g_edge, &s->g_pt, &s->g_n, &s->g_h,
+                   &s->g_col};
+    for (Buf *b : bufs)
+        if (b->p)
+            (void)hipFree(b->p);
+    for (int k = 0; k < 4; k++)
+        if (s->ev[k])
+            (void)hipEventDestroy(s->ev[k]);
+    if (s->stream)
+        (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int vkvg_diff_forward(VkvgDiffScene s, const float *ctrl, const float *rgba, const float *width, float *image,
+                      int keep) {
+    if (!s)
+        return -1;
+    if (!ctrl || !rgba || !width || !image) {
+        s->err = "null pointer";
+        return -1;
+    }
+    hipStream_t st = s->stream;
+    s->have_ckpt = false;
+    if (!ck(s, hipEventRecord(s->ev[0], st), "event"))
+        return -1;
+    const uint32_t np = s->n_pts, nd = s->n_paths;
+    hipLaunchKernelGGL(k_dflatten, dim3(blocks(np)), dim3(256), 0, st, np, P<uint32_t>(s->pt_path),
+                       P<uint32_t>(s->first_pt), P<uint32_t>(s->first_ctrl), P<uint32_t>(s->ncubic),
+                       (const float2 *)ctrl, P<float2>(s->pts));
+    hipLaunchKernelGGL(k_dnormals, dim3(blocks(np)), dim3(256), 0, st, np, P<uint32_t>(s->pt_path),
+                       P<uint32_t>(s->first_pt), P<float2>(s->pts), P<float2>(s->nrm));
+    hipLaunchKernelGGL(k_dedges, dim3(blocks(np)), dim3(256), 0, st, np, P<uint32_t>(s->pt_path),
+                       P<uint32_t>(s->first_pt), P<uint32_t>(s->first_edge), P<uint32_t>(s->stroke), width,
+                       P<float2>(s->pts), P<float2>(s->nrm), P<float4>(s->edges), P<DSrc>(s->src));
+    hipLaunchKernelGGL(k_dbbox, dim3(nd), dim3(256), 0, st, P<uint32_t>(s->first_edge), P<float4>(s->edges),
+                       P<int4>(s->bbox));
+    vkh::launch_draw_tiles(st, P<vkh_draw>(s->draws), nd, P<TargetInfo>(s->tg), P<int4>(s->bbox), VKD_T,
+                           P<DrawTiles>(s->dtiles), P<uint32_t>(s->pt_count));
+    if (!scan(s, P<uint32_t>(s->pt_count), nd, P<uint32_t>(s->pt_off)))
+        return -1;
+    uint32_t n_ptiles = 0;
+    if (!read_u32(s, P<uint32_t>(s->pt_off) + nd, &n_ptiles))
+        return -1;
+    vkh::launch_set_pt_base(st, P<DrawTiles>(s->dtiles), P<uint32_t>(s->pt_off), nd);
+    if (!ensure(s, s->ptiles, sizeof(PathTile) * ((size_t)n_ptiles + 1)))
+        return -1;
+    for (int pass = 0; pass < 2; pass++) {
+        const uint32_t cap = (uint32_t)std::min<size_t>(s->segs.cap / 4, 0xFFFFFFFFull);
+        zero(s, s->seg_alloc, 1);
+        hipLaunchKernelGGL(k_dbin, dim3(nd), dim3(256), 0, st, P<uint32_t>(s->first_edge), P<float4>(s->edges),
+                           P<DrawTiles>(s->dtiles), P<PathTile>(s->ptiles), P<uint32_t>(s->segs), cap,
+                           P<uint32_t>(s->seg_alloc));
+        if (!read_u32(s, P<uint32_t>(s->seg_alloc), &s->n_segs))
+            return -1;
+        if (s->n_segs <= cap)
+            break;
+        if (pass == 1 || !ensure(s, s->segs, 4ull * s->n_segs + 4)) {
+            if (s->err.empty())
+                s->err = "segment buffer overflow";
+            return -1;
+        }
+    }
+    zero(s, s->tile_cnt, s->n_tiles + 1);
+    vkh::launch_coarse(st, false, s->bin_total, P<TargetInfo>(s->tg), 1, P<vkh_draw>(s->draws),
+                       P<DrawTiles>(s->dtiles), P<PathTile>(s->ptiles), P<uint32_t>(s->tile_cnt), nullptr, nullptr);
+    if (!scan(s, P<uint32_t>(s->tile_cnt), s->n_tiles, P<uint32_t>(s->tile_off)))
+        return -1;
+    if (!read_u32(s, P<uint32_t>(s->tile_off) + s->n_tiles, &s->n_cmds))
+        return -1;
+    if (!ensure(s, s->cmds, sizeof(Cmd) * ((size_t)s->n_cmds + 1)))
+        return -1;
+    vkh::launch_coarse(st, true, s->bin_total, P<TargetInfo>(s->tg), 1, P<vkh_draw>(s->draws),
+                       P<DrawTiles>(s->dtiles), P<PathTile>(s->ptiles), nullptr, P<uint32_t>(s->tile_off),
+                       P<Cmd>(s->cmds));
+    float4 *ck_ptr = nullptr;
+    if (keep) {
+        hipLaunchKernelGGL(k_dbatches, dim3(blocks(s->n_tiles)), dim3(256), 0, st, s->n_tiles,
+                           P<uint32_t>(s->tile_off), P<uint32_t>(s->nbat));
+        if (!scan(s, P<uint32_t>(s->nbat), s->n_tiles, P<uint32_t>(s->batch_off)))
+            return -1;
+        if (!read_u32(s, P<uint32_t>(s->batch_off) + s->n_tiles, &s->n_batches))
+            return -1;
+        if (!ensure(s, s->ckpt, 16ull * 256 * ((size_t)s->n_batches + 1)))
+            return -1;
+        ck_ptr = P<float4>(s->ckpt);
+    }
+    hipLaunchKernelGGL(k_dfine, dim3(s->n_tiles), dim3(256), 0, st, s->W, s->H, s->tilesW, P<uint32_t>(s->tile_off),
+                       P<Cmd>(s->cmds), P<PathTile>(s->ptiles), P<uint32_t>(s->segs), P<float4>(s->edges),
+                       (const float4 *)rgba, (float4 *)image, ck_ptr, P<uint32_t>(s->batch_off));
+    if (!ck(s, hipGetLastError(), "forward launch") || !ck(s, hipEventRecord(s->ev[1], st), "event") ||
+        !ck(s, hipStreamSynchronize(st), "forward"))
+        return -1;
+    (void)hipEventElapsedTime(&s->ms_fwd, s->ev[0], s->ev[1]);
+    s->have_ckpt = keep != 0;
+    return 0;
+}
+
+int vkvg_diff_backward(VkvgDiffScene s, const float *ctrl, const float *rgba, const float *width,
+                       const float *grad_image, float *grad_ctrl, float *grad_rgba, float *grad_width) {
+    if (!s)
+        return -1;
+    if (!s->have_ckpt) {
+        s->err = "backward needs a preceding forward with keep != 0";
+        return -1;
+    }
+    if (!ctrl || !rgba || !width || !grad_image || !grad_ctrl || !grad_rgba || !grad_width) {
+        s->err = "null pointer";
+        return -1;
+    }
+    (void)ctrl;
+    hipStream_t st = s->stream;
+    if (!ck(s, hipEventRecord(s->ev[2], st), "event"))
+        return -1;
+    zero(s, s->g_edge, 4ull * s->n_edges);
+    zero(s, s->g_pt, 2ull * s->n_pts);
+    zero(s, s->g_n, 2ull * s->n_pts);
+    zero(s, s->g_h, s->n_paths);
+    zero(s, s->g_col, 4ull * s->n_paths);
+    hipLaunchKernelGGL(k_dback, dim3(s->n_tiles), dim3(256), 0, st, s->W, s->H, s->tilesW, P<uint32_t>(s->tile_off),
+                       P<Cmd>(s->cmds), P<PathTile>(s->ptiles), P<uint32_t>(s->segs), P<float4>(s->edges),
+                       (const float4 *)rgba, (const float4 *)grad_image, P<float4>(s->ckpt),
+                       P<uint32_t>(s->batch_off), P<float4>(s->g_edge), P<float>(s->g_col));
+    hipLaunchKernelGGL(k_dg_edges, dim3(blocks(s->n_edges)), dim3(256), 0, st, s->n_edges, P<DSrc>(s->src),
+                       P<float4>(s->g_edge), P<uint32_t>(s->pt_path), width, P<float2>(s->nrm), P<float2>(s->g_pt),
+                       P<float2>(s->g_n), P<float>(s->g_h));
+    hipLaunchKernelGGL(k_dg_normals, dim3(blocks(s->n_pts)), dim3(256), 0, st, s->n_pts, P<uint32_t>(s->pt_path),
+                       P<uint32_t>(s->first_pt), P<uint32_t>(s->stroke), P<float2>(s->pts), P<float2>(s->nrm),
+                       P<float2>(s->g_n), P<float2>(s->g_pt));
+    hipLaunchKernelGGL(k_dg_ctrl, dim3(blocks(s->n_ctrl)), dim3(256), 0, st, s->n_ctrl, P<uint32_t>(s->ctrl_path),
+                       P<uint32_t>(s->first_ctrl), P<uint32_t>(s->first_pt), P<uint32_t>(s->ncubic),
+                       P<float2>(s->g_pt), (float2 *)grad_ctrl);
+    hipLaunchKernelGGL(k_dg_paths, dim3(blocks(s->n_paths)), dim3(256), 0, st, s->n_paths, (const float4 *)rgba,
+                       P<float>(s->g_col), P<float>(s->g_h), P<uint32_t>(s->stroke), (float4 *)grad_rgba, grad_width);
+    if (!ck(s, hipGetLastError(), "backward launch") || !ck(s, hipEventRecord(s->ev[3], st), "event") ||
+        !ck(s, hipStreamSynchronize(st), "backward"))
+        return -1;
+    (void)hipEventElapsedTime(&s->ms_bwd, s->ev[2], s->ev[3]);
+    return 0;
+}
+
+float vkvg_diff_last_ms(VkvgDiffScene s, int backward) { return s ? (backward ? s->ms_bwd : s->ms_fwd) : 0.f; }
+
+void vkvg_diff_stats(VkvgDiffScene s, uint32_t out[4]) {
+    if (!s || !out)
+        return;
+    out[0] = s->n_edges;
+    out[1] = s->n_segs;
+    out[2] = s->n_cmds;
+    out[3] = s->n_batches;
+}
+
+const char *vkvg_diff_last_error(VkvgDiffScene s) { return s ? s->err.c_str() : "null scene"; }
+
+} // extern "C"

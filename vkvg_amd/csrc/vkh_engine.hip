@@ -57,6 +57,50 @@ __global__ void k_scan_add(uint32_t *__restrict__ out, uint32_t n, const uint32_
     }
 }
 
+namespace vkh {
+// temp words needed by scan_u32 for n values
+size_t scan_temp_words(uint32_t n) {
+    size_t need = 0;
+    for (uint32_t m = n;;) {
+        uint32_t nb = (m + 1 + 1023) / 1024;
+        if (nb <= 1)
+            break;
+        need += 2ull * nb + 1; // sums + their scan
+        m = nb;
+    }
+    return need + 1;
+}
+// exclusive scan of n values; out must hold n+1 entries (out[n] = total); tmp holds
+// scan_temp_words(n) words
+void scan_u32(hipStream_t s, const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *tmp) {
+    struct Level {
+        uint32_t n;
+        uint32_t *out;
+        uint32_t nb;
+    };
+    Level lv[8];
+    int nl = 0;
+    const uint32_t *cin = in;
+    uint32_t cn = n;
+    uint32_t *cout = out;
+    while (true) {
+        uint32_t nb = (cn + 1 + 1023) / 1024;
+        uint32_t *sums = nb > 1 ? tmp : nullptr;
+        hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, s, cin, cn, cout, sums);
+        lv[nl++] = Level{cn, cout, nb};
+        if (nb <= 1)
+            break;
+        uint32_t *sscan = tmp + nb;
+        tmp += 2ull * nb + 1;
+        cin = sums;
+        cn = nb;
+        cout = sscan;
+    }
+    for (int k = nl; k-- > 1;)
+        hipLaunchKernelGGL(k_scan_add, dim3(lv[k - 1].nb), dim3(256), 0, s, lv[k - 1].out, lv[k - 1].n, lv[k].out);
+}
+} // namespace vkh
+
 namespace {
 
 struct DevBuf {
@@ -138,48 +182,10 @@ static bool ensure_host(vkh_engine *eng, HostBuf &b, size_t bytes) {
 }
 template <typename T> static T *P(DevBuf &b) { return (T *)b.p; }
 
-// exclusive scan of n values; out must hold n+1 entries (out[n] = total)
 static int scan_u32(vkh_engine *eng, const uint32_t *in, uint32_t n, uint32_t *out) {
-    hipStream_t s = eng->stream;
-    struct Level {
-        const uint32_t *in;
-        uint32_t n;
-        uint32_t *out;
-        uint32_t nb;
-    };
-    // level sizes: block sums of level k are the input of level k+1
-    std::vector<uint32_t> nbs;
-    size_t need = 0;
-    for (uint32_t m = n;;) {
-        uint32_t nb = (m + 1 + 1023) / 1024;
-        nbs.push_back(nb);
-        if (nb <= 1)
-            break;
-        need += 2ull * nb + 1; // sums + their scan
-        m = nb;
-    }
-    if (!ensure(eng, eng->scan_tmp, (need + 1) * sizeof(uint32_t)))
+    if (!ensure(eng, eng->scan_tmp, vkh::scan_temp_words(n) * sizeof(uint32_t)))
         return -1;
-    uint32_t *tmp = P<uint32_t>(eng->scan_tmp);
-    std::vector<Level> lv;
-    const uint32_t *cin = in;
-    uint32_t cn = n;
-    uint32_t *cout = out;
-    for (size_t k = 0; k < nbs.size(); k++) {
-        uint32_t nb = nbs[k];
-        uint32_t *sums = nb > 1 ? tmp : nullptr;
-        hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, s, cin, cn, cout, sums);
-        lv.push_back(Level{cin, cn, cout, nb});
-        if (nb <= 1)
-            break;
-        uint32_t *sscan = tmp + nb;
-        tmp += 2ull * nb + 1;
-        cin = sums;
-        cn = nb;
-        cout = sscan;
-    }
-    for (size_t k = lv.size(); k-- > 1;)
-        hipLaunchKernelGGL(k_scan_add, dim3(lv[k - 1].nb), dim3(256), 0, s, lv[k - 1].out, lv[k - 1].n, lv[k].out);
+    vkh::scan_u32(eng->stream, in, n, out, P<uint32_t>(eng->scan_tmp));
     return check(eng, hipGetLastError(), "scan") ? 0 : -1;
 }
 

@@ -68,6 +68,8 @@ struct Cmd {
 };
 
 namespace vkh {
+size_t scan_temp_words(uint32_t n);
+void scan_u32(hipStream_t s, const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *tmp);
 void launch_flatten_count(hipStream_t s, const vkh_item *items, uint32_t n, uint32_t *counts);
 void launch_flatten_emit(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *offs, v2 *pts, uint8_t *pf);
 void launch_subpath_ranges(hipStream_t s, const vkh_subpath *sp, uint32_t n, const uint32_t *item_off,

@@ -336,3 +336,45 @@ def clear_scene(size: int = 96) -> Scene:
     s.rectangle(20, 20, 40, 40)
     s.fill()
     return Scene("clear", size, size, s, 2)
+
+
+@dataclass
+class DiffScene:
+    """Differentiable-render scene: topology plus FP32 parameters and the target's control points."""
+    name: str
+    width: int
+    height: int
+    ncubic: np.ndarray      # cubics per closed path
+    stroke: np.ndarray      # bool per path
+    ctrl: np.ndarray        # (n_ctrl, 2) float32, 3 per cubic
+    rgba: np.ndarray        # (n_paths, 4) float32, straight colour and alpha
+    lwidth: np.ndarray      # (n_paths,) float32 stroke widths (unused for fills)
+    target_ctrl: np.ndarray  # control points of the target scene
+
+    @property
+    def n_paths(self) -> int:
+        return len(self.ncubic)
+
+
+def config5(variant: int = 0, n_paths: int = 1024, size: int = 512, smin: float = 8.0,
+            smax: float = 64.0) -> DiffScene:
+    """512x512 FP32: closed paths of n~U{3..8} cubics, centre ~U over the surface, control points
+    = centre + N(0, s^2) with s log-U[8,64] (*); half of the paths (p = 0.5 (*)) stroked with
+    width U[1,6], the others non-zero filled; rgb~U[0,1], alpha~U[0.5,1] (*). The target moves
+    every control point by N(0, 4^2) px."""
+    rng = Rng(0x5EED0000 + 16 * 5 + variant)
+    ncubic, stroke, ctrl, rgba, lw = [], [], [], [], []
+    for _ in range(n_paths):
+        n = rng.randint(3, 8)
+        c = rng.uniform(0, size, 2)
+        s = rng.loguniform(smin, smax)
+        ctrl.append(c[None, :] + s * rng.normal(6 * n).reshape(3 * n, 2))
+        st = rng.u() < 0.5
+        stroke.append(st)
+        lw.append(rng.uniform(1.0, 6.0) if st else 0.0)
+        rgba.append(np.concatenate([rng.u(3), [rng.uniform(0.5, 1.0)]]))
+        ncubic.append(n)
+    ctrl = np.concatenate(ctrl).astype(np.float32)
+    target = (ctrl + 4.0 * rng.normal(ctrl.size).reshape(ctrl.shape)).astype(np.float32)
+    return DiffScene("c5_512_diff", size, size, np.array(ncubic, np.int64), np.array(stroke, bool), ctrl,
+                     np.array(rgba, np.float32), np.array(lw, np.float32), target)
