@@ -48,36 +48,52 @@ def _read_traffic(name: str):
         return None
 
 
+def _timed(dev, vkvg_amd, steps):
+    """Executes the resident frame `steps` times; returns wall ms per frame and per-run stats."""
+    _sync()
+    runs = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        vkvg_amd.execute(dev)
+        runs.append(dev.stats())
+    _sync()
+    return (time.perf_counter() - t0) * 1e3 / steps, runs
+
+
 def bench_c3(args, dev, vkvg_amd, scenes):
     sc = scenes.config3()
     surf = vkvg_amd.Surface(dev, sc.width, sc.height)
     surf.upload(sc.words())  # frame resident in HBM
+    tune = dev.autotune(reps=2) if not args.no_tune else None
+    # eager executions (per-stage events, fine-kernel time for the roofline)
     for _ in range(args.warmup):
         vkvg_amd.execute(dev)
-    _sync()
-    fine_ms, dev_ms, stage = [], [], {}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        vkvg_amd.execute(dev)
-        st = dev.stats()
-        fine_ms.append(st["ms_fine"])
-        dev_ms.append(st["ms_device"])
-        for k, v in st["ms_stage"].items():
-            stage.setdefault(k, []).append(v)
-    _sync()
-    t1 = time.perf_counter()
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    st = dev.stats()
+    ms_eager, runs = _timed(dev, vkvg_amd, args.steps)
+    fine_ms = float(np.mean([r["ms_fine"] for r in runs]))
+    stage = {k: float(np.mean([r["ms_stage"][k] for r in runs])) for k in runs[0]["ms_stage"]}
+    st = runs[-1]
+    # headline: the same frame replayed as one captured graph per step
+    ms_step, dev_ms = ms_eager, float(np.mean([r["ms_device"] for r in runs]))
+    if not args.no_graph:
+        dev.set_graph(True)
+        vkvg_amd.execute(dev)  # sizing run + capture
+        for _ in range(args.warmup):
+            vkvg_amd.execute(dev)
+        ms_step, g_runs = _timed(dev, vkvg_amd, args.steps)
+        assert all(r["graph"] == 1 for r in g_runs)
+        dev_ms = float(np.mean([r["ms_device"] for r in g_runs]))
+        dev.set_graph(False)
     surf.close()
-    stage_avg = {k: float(np.mean(v)) for k, v in stage.items()}
-    return sc, ms_step, float(np.mean(fine_ms)), float(np.mean(dev_ms)), stage_avg, st
+    return sc, ms_step, ms_eager, fine_ms, dev_ms, stage, st, tune
 
 
 def bench_c4(args, dev, vkvg_amd, scenes):
     subs = scenes.config4()
     surfs = [vkvg_amd.Surface(dev, s.width, s.height) for s in subs]
     words = [s.words() for s in subs]
-    # plain: one frame per surface (each execute timed on resident inputs)
+    # plain: one frame per surface (each execute timed on resident inputs), default configuration
+    dev.set_tile_size(16)
+    dev.set_fine_words(4096)
     plain = 0.0
     for s, w in zip(surfs, words):
         s.upload(w)
@@ -86,22 +102,27 @@ def bench_c4(args, dev, vkvg_amd, scenes):
         t = time.perf_counter()
         vkvg_amd.execute(dev)
         plain += time.perf_counter() - t
-    # batched: one frame over all 256 surfaces
+    # batched: one frame over all 256 surfaces, autotuned, replayed as one graph launch
     vkvg_amd.upload_batch(dev, surfs, words)
+    tune = dev.autotune(reps=1) if not args.no_tune else None
+    reps = max(3, args.steps // 4)
     for _ in range(max(1, args.warmup)):
         vkvg_amd.execute(dev)
-    _sync()
-    reps = max(3, args.steps // 4)
-    t = time.perf_counter()
-    for _ in range(reps):
+    eager, _ = _timed(dev, vkvg_amd, reps)
+    batched = eager
+    if not args.no_graph:
+        dev.set_graph(True)
         vkvg_amd.execute(dev)
-    _sync()
-    batched = (time.perf_counter() - t) / reps
+        batched, _ = _timed(dev, vkvg_amd, reps)
+        dev.set_graph(False)
     for s in surfs:
         s.close()
-    return {"plain_ms": plain * 1e3, "batched_ms": batched * 1e3, "ratio": plain / batched,
-            "surfaces": len(subs), "paths": int(sum(s.n_paths for s in subs)),
-            "pixels": int(sum(s.width * s.height for s in subs))}
+    dev.set_tile_size(16)
+    dev.set_fine_words(4096)
+    return {"plain_ms": plain * 1e3, "batched_ms": batched, "batched_eager_ms": eager,
+            "ratio": plain * 1e3 / batched, "surfaces": len(subs), "paths": int(sum(s.n_paths for s in subs)),
+            "pixels": int(sum(s.width * s.height for s in subs)),
+            "tuned": tune["best"] if tune else None}
 
 
 def bench_fwd_bwd(args):
@@ -153,6 +174,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-paths", type=int, default=5000)
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (tile 16, 4096 words)")
+    ap.add_argument("--no-graph", action="store_true", help="time eager executions instead of graph replays")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -167,7 +190,7 @@ def main() -> int:
 
     dev = vkvg_amd.Device()
     dev.set_tile_size(args.tile)
-    sc, ms_step, fine_ms, dev_ms, stage, st = bench_c3(args, dev, vkvg_amd, scenes)
+    sc, ms_step, ms_eager, fine_ms, dev_ms, stage, st, tune = bench_c3(args, dev, vkvg_amd, scenes)
     paths_per_s = sc.n_paths / (ms_step / 1e3)
     gpix = sc.width * sc.height / (ms_step / 1e3) / 1e9
     # dominant kernel: the fine coverage + composite kernel; its algorithmic bytes per launch
@@ -192,14 +215,18 @@ def main() -> int:
         "config": {"workload": "dense 4096x4096 RGBA8 scene: 20000 paths of 1-40 cubics, 16-blob Gaussian "
                                "mixture, 60% non-zero fill / 40% stroke 0.5-8 px, 30% linear/radial gradients",
                    "surface": "4096x4096", "paths": sc.n_paths, "control_points": sc.ctrl_points,
-                   "gradients": sc.gradients, "tile": st["tile_size"], "inputs": "resident in HBM"},
+                   "gradients": sc.gradients, "tile": st["tile_size"], "fine_words": st["fine_words"],
+                   "inputs": "resident in HBM"},
         "gpix_per_s": gpix,
         "frame_algorithmic_bytes": sc.algorithmic_bytes(),
         "frame_hbm_gbs": sc.algorithmic_bytes() / (ms_step / 1e3) / 1e9,
         "device_ms": dev_ms,
+        "eager_ms_per_step": ms_eager,
+        "graph_replay": not args.no_graph,
+        "tuned": tune["best"] if tune else {"tile": st["tile_size"], "fine_words": st["fine_words"]},
         "stages_ms": stage,
         "counts": {k: st[k] for k in ("points", "stroke_triangles", "edges", "path_tiles", "segments", "commands")},
-        "roofline": {"kernel": "k_fine<16>", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"kernel": f"k_fine<{st['tile_size']},{st['fine_words']}>", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": fine_bytes, "avg_launch_ms": fine_ms},
     }

@@ -25,6 +25,10 @@
 
 #include <stdint.h>
 
+#ifndef vkvg_public
+#define vkvg_public __attribute__((visibility("default")))
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -33,25 +37,25 @@ typedef struct vkvg_diff_scene_t *VkvgDiffScene;
 
 /* Topology: ncubic[i] >= 1 cubics per closed path, is_stroke[i] != 0 for a stroked path (host
  * arrays). Returns NULL on invalid input or allocation failure. */
-VkvgDiffScene vkvg_diff_create(uint32_t width, uint32_t height, uint32_t n_paths, const uint32_t *ncubic,
+vkvg_public VkvgDiffScene vkvg_diff_create(uint32_t width, uint32_t height, uint32_t n_paths, const uint32_t *ncubic,
                                const uint32_t *is_stroke);
-void vkvg_diff_destroy(VkvgDiffScene s);
+vkvg_public void vkvg_diff_destroy(VkvgDiffScene s);
 
 /* Renders the image. With keep != 0 the compositing checkpoints the backward pass needs are kept
  * (coverage itself is recomputed by the backward pass). */
-int vkvg_diff_forward(VkvgDiffScene s, const float *ctrl, const float *rgba, const float *width, float *image,
+vkvg_public int vkvg_diff_forward(VkvgDiffScene s, const float *ctrl, const float *rgba, const float *width, float *image,
                       int keep);
 
 /* Gradients of the loss given grad_image = dL/dimage. Must follow a vkvg_diff_forward with
  * keep != 0 on the same parameters. Outputs are overwritten. */
-int vkvg_diff_backward(VkvgDiffScene s, const float *ctrl, const float *rgba, const float *width,
+vkvg_public int vkvg_diff_backward(VkvgDiffScene s, const float *ctrl, const float *rgba, const float *width,
                        const float *grad_image, float *grad_ctrl, float *grad_rgba, float *grad_width);
 
 /* Device time of the last forward / backward call in milliseconds, and frame statistics:
  * out[0] edges, out[1] segments, out[2] commands, out[3] batches. */
-float vkvg_diff_last_ms(VkvgDiffScene s, int backward);
-void vkvg_diff_stats(VkvgDiffScene s, uint32_t out[4]);
-const char *vkvg_diff_last_error(VkvgDiffScene s);
+vkvg_public float vkvg_diff_last_ms(VkvgDiffScene s, int backward);
+vkvg_public void vkvg_diff_stats(VkvgDiffScene s, uint32_t out[4]);
+vkvg_public const char *vkvg_diff_last_error(VkvgDiffScene s);
 
 #ifdef __cplusplus
 }

@@ -113,7 +113,19 @@ typedef struct {
     float    ms_fine;       /* device time of the fine (coverage + composite) kernel, ms */
     float    ms_device;     /* device time from the first to the last kernel of the frame, ms */
     float    ms_stage[6];   /* flatten, geometry (stroke + edges), tile setup, binning, coarse, fine */
+    uint32_t graph;         /* 1 when the frame ran as a captured graph (stage times are then 0) */
+    uint32_t fine_words;    /* fine batch capacity in LDS words (commands per batch = words / tile^2) */
 } vkh_stats;
+
+/* one autotuner measurement */
+typedef struct {
+    uint32_t tile, fine_words;
+    float    ms;            /* best device time of the measured executions */
+} vkh_tune_entry;
+typedef struct {
+    uint32_t       n, best;
+    vkh_tune_entry e[16];
+} vkh_tune;
 
 typedef struct vkh_engine vkh_engine;
 
@@ -124,6 +136,17 @@ vkvg_public void        vkvg_hip_engine_destroy(vkh_engine *eng);
 vkvg_public int         vkvg_hip_set_tile_size(vkh_engine *eng, uint32_t tile);
 /* Optional external stream (a hipStream_t passed as void*); NULL = the engine's own. */
 vkvg_public void        vkvg_hip_set_stream(vkh_engine *eng, void *stream);
+/* Fine batch capacity in LDS words (2048, 4096 or 8192; 0 = 4096): commands staged per batch =
+ * words / tile^2. Applies to the next execution. */
+vkvg_public int         vkvg_hip_set_fine_words(vkh_engine *eng, uint32_t words);
+/* Re-tiles the resident frame (8, 16 or 32 px) without uploading it again. */
+vkvg_public int         vkvg_hip_retile(vkh_engine *eng, uint32_t tile);
+/* Graph replay: with on != 0, vkvg_hip_execute captures the resident frame's launch sequence
+ * after its first (sizing) execution and replays it as one hipGraph launch afterwards. */
+vkvg_public void        vkvg_hip_set_graph(vkh_engine *eng, int on);
+/* Times every tile size x fine capacity on the resident frame (reps executions each, best time
+ * kept), applies the fastest pair and reports all of them. */
+vkvg_public int         vkvg_hip_autotune(vkh_engine *eng, uint32_t reps, vkh_tune *out);
 
 /* device images */
 vkvg_public int  vkvg_hip_image_alloc(vkh_engine *eng, uint32_t width, uint32_t height, void **dev_ptr);

@@ -10,7 +10,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     names = set()
-    for h in ("vkvg.h", "vkvg_amd.h", "vkvg_hip.h"):
+    for h in ("vkvg.h", "vkvg_amd.h", "vkvg_hip.h", "vkvg_diff.h"):
         src = open(os.path.join(ROOT, "include", h)).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
         for m in re.finditer(r"vkvg_public\s+[^;]*?\b(vkvg_\w+)\s*\(", src):

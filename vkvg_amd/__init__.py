@@ -27,7 +27,7 @@ class Stats(ctypes.Structure):
                 ("path_tiles", ctypes.c_uint64), ("segments", ctypes.c_uint64),
                 ("commands", ctypes.c_uint64), ("tiles", ctypes.c_uint64), ("tile_size", ctypes.c_uint32),
                 ("ms_total", ctypes.c_float), ("ms_fine", ctypes.c_float), ("ms_device", ctypes.c_float),
-                ("ms_stage", ctypes.c_float * 6)]
+                ("ms_stage", ctypes.c_float * 6), ("graph", ctypes.c_uint32), ("fine_words", ctypes.c_uint32)]
 
     STAGES = ("flatten", "geometry", "tile_setup", "binning", "coarse", "fine")
 
@@ -76,6 +76,9 @@ EXPORTED = [
     "vkvg_hip_image_upload", "vkvg_hip_image_read_unpremultiplied", "vkvg_hip_render", "vkvg_hip_upload",
     "vkvg_hip_execute", "vkvg_hip_get_stats",
     "vkvg_hip_last_error",
+    # engine tuning (include/vkvg_hip.h) and the differentiable renderer (include/vkvg_diff.h)
+    "vkvg_hip_autotune", "vkvg_hip_retile", "vkvg_hip_set_fine_words", "vkvg_hip_set_graph",
+    "vkvg_diff_create", "vkvg_diff_destroy", "vkvg_diff_forward", "vkvg_diff_backward", "vkvg_diff_last_ms", "vkvg_diff_stats", "vkvg_diff_last_error",
 ]
 
 
@@ -166,6 +169,46 @@ class Device:
         s = Stats()
         lib().vkvg_amd_device_get_stats(self.h, ctypes.byref(s))
         return s.as_dict()
+
+    def _engine(self):
+        L = lib()
+        L.vkvg_amd_device_engine.restype = ctypes.c_void_p
+        L.vkvg_amd_device_engine.argtypes = [ctypes.c_void_p]
+        e = L.vkvg_amd_device_engine(self.h)
+        if not e:
+            raise VkvgError("device has no engine")
+        return e
+
+    def set_graph(self, on: bool) -> None:
+        """Replay the resident frame as one captured hipGraph after its first execution."""
+        L = lib()
+        L.vkvg_hip_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.vkvg_hip_set_graph(self._engine(), 1 if on else 0)
+
+    def set_fine_words(self, words: int) -> None:
+        L = lib()
+        L.vkvg_hip_set_fine_words.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+        if L.vkvg_hip_set_fine_words(self._engine(), words):
+            raise VkvgError(f"bad fine capacity {words}")
+
+    def autotune(self, reps: int = 3) -> dict:
+        """Times tile 8/16/32 x fine capacity 2048/4096/8192 words on the resident frame and keeps
+        the fastest; returns the table and the choice."""
+        class Entry(ctypes.Structure):
+            _fields_ = [("tile", ctypes.c_uint32), ("fine_words", ctypes.c_uint32), ("ms", ctypes.c_float)]
+
+        class Tune(ctypes.Structure):
+            _fields_ = [("n", ctypes.c_uint32), ("best", ctypes.c_uint32), ("e", Entry * 16)]
+        L = lib()
+        L.vkvg_hip_autotune.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+        L.vkvg_hip_autotune.restype = ctypes.c_int
+        t = Tune()
+        if L.vkvg_hip_autotune(self._engine(), reps, ctypes.byref(t)):
+            L.vkvg_hip_last_error.restype = ctypes.c_char_p
+            L.vkvg_hip_last_error.argtypes = [ctypes.c_void_p]
+            raise VkvgError(f"autotune failed: {L.vkvg_hip_last_error(self._engine()).decode()}")
+        table = [{"tile": t.e[k].tile, "fine_words": t.e[k].fine_words, "ms": float(t.e[k].ms)} for k in range(t.n)]
+        return {"best": table[t.best], "table": table}
 
     def __enter__(self):
         return self

@@ -133,6 +133,16 @@ struct vkh_engine {
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     hipEvent_t ev_stage[6] = {};
 
+    // graph replay of the resident frame (sizes cached by the last sizing run)
+    bool graph_on = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    uint64_t gkey = 0, gfail_key = 0;
+    uint64_t frame_gen = 0, alloc_gen = 0;
+    uint32_t sz[8] = {};  // points, fill edges, stroke steps, stroke vertices, triangles, path tiles, segments, commands
+    uint32_t fine_words = 4096;
+    std::vector<TargetInfo> h_tg;
+
     // the uploaded (resident) frame
     bool resident = false;
     size_t o_items, o_sp, o_draws, o_styles, o_dashes, o_grads, o_tg, o_units;
@@ -155,6 +165,7 @@ static bool check(vkh_engine *e, hipError_t r, const char *what) {
 static bool ensure(vkh_engine *eng, DevBuf &b, size_t bytes) {
     if (bytes <= b.cap && b.p)
         return true;
+    eng->alloc_gen++; // invalidates a captured graph that points at the old buffer
     if (b.p)
         (void)hipFree(b.p);
     b.p = nullptr;
@@ -182,8 +193,18 @@ static bool ensure_host(vkh_engine *eng, HostBuf &b, size_t bytes) {
 }
 template <typename T> static T *P(DevBuf &b) { return (T *)b.p; }
 
-static int scan_u32(vkh_engine *eng, const uint32_t *in, uint32_t n, uint32_t *out) {
-    if (!ensure(eng, eng->scan_tmp, vkh::scan_temp_words(n) * sizeof(uint32_t)))
+// in capture mode a buffer must already be large enough: nothing is allocated
+static bool need(vkh_engine *eng, bool cap, DevBuf &b, size_t bytes) {
+    if (!cap)
+        return ensure(eng, b, bytes);
+    if (b.p && bytes <= b.cap)
+        return true;
+    eng->err = "graph capture would need an allocation";
+    return false;
+}
+
+static int scan_u32(vkh_engine *eng, const uint32_t *in, uint32_t n, uint32_t *out, bool cap = false) {
+    if (!need(eng, cap, eng->scan_tmp, vkh::scan_temp_words(n) * sizeof(uint32_t)))
         return -1;
     vkh::scan_u32(eng->stream, in, n, out, P<uint32_t>(eng->scan_tmp));
     return check(eng, hipGetLastError(), "scan") ? 0 : -1;
@@ -233,6 +254,10 @@ void vkvg_hip_engine_destroy(vkh_engine *e) {
     (void)hipSetDevice(e->device);
     if (e->stream)
         (void)hipStreamSynchronize(e->stream);
+    if (e->gexec)
+        (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph)
+        (void)hipGraphDestroy(e->graph);
     DevBuf *bufs[] = {&e->blob, &e->units, &e->tgts, &e->item_cnt, &e->item_off, &e->pts, &e->pflags,
                       &e->sp_first, &e->sp_count, &e->fe, &e->nv, &e->nt, &e->fe_off, &e->nv_off, &e->nt_off,
                       &e->verts, &e->tris, &e->edges, &e->edraw, &e->bbox, &e->dtiles, &e->pt_count, &e->pt_off,
@@ -276,6 +301,22 @@ int vkvg_hip_set_tile_size(vkh_engine *e, uint32_t tile) {
 void vkvg_hip_set_stream(vkh_engine *e, void *stream) {
     if (e)
         e->stream = stream ? (hipStream_t)stream : e->own;
+}
+
+int vkvg_hip_set_fine_words(vkh_engine *e, uint32_t words) {
+    if (!e)
+        return -1;
+    if (words == 0)
+        words = 4096;
+    if (words != 2048 && words != 4096 && words != 8192)
+        return -1;
+    e->fine_words = words;
+    return 0;
+}
+
+void vkvg_hip_set_graph(vkh_engine *e, int on) {
+    if (e)
+        e->graph_on = on != 0;
 }
 
 int vkvg_hip_image_alloc(vkh_engine *eng, uint32_t w, uint32_t h, void **dev_ptr) {
@@ -342,6 +383,7 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
     hipStream_t s = eng->stream;
     const uint32_t T = eng->tile;
     eng->resident = false;
+    CK(hipStreamSynchronize(s)); // the staging blob may still feed an earlier copy
 
     // ---- host-side frame tables: work units and targets
     std::vector<Unit> units;
@@ -465,26 +507,42 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
     eng->tile_total = tile_total;
     eng->bin_total = bin_total;
     eng->frame_tile = T;
+    eng->h_tg = tg;
+    eng->frame_gen++;
+    eng->gfail_key = 0;
     eng->resident = true;
     return 0;
 }
 
-int vkvg_hip_execute(vkh_engine *eng) {
-    if (!eng || !eng->resident) {
-        if (eng)
-            eng->err = "no resident frame";
-        return -1;
+// totals of the pipeline: read back (sizing run, cached in eng->sz[slot..]) or taken from the
+// cache (capture run, which must not synchronise)
+static int totals(vkh_engine *eng, bool cap, const std::vector<const uint32_t *> &srcs, uint32_t *dst,
+                  uint32_t slot) {
+    if (cap) {
+        for (size_t i = 0; i < srcs.size(); i++)
+            dst[i] = eng->sz[slot + i];
+        return 0;
     }
+    if (read_totals(eng, srcs, dst))
+        return -1;
+    for (size_t i = 0; i < srcs.size(); i++)
+        eng->sz[slot + i] = dst[i];
+    return 0;
+}
+
+static int run_pipeline(vkh_engine *eng, bool cap) {
     auto t_start = std::chrono::steady_clock::now();
     (void)hipSetDevice(eng->device);
     hipStream_t s = eng->stream;
     const uint32_t T = eng->frame_tile;
     vkh_stats st{};
     st.tile_size = T;
+    st.fine_words = eng->fine_words;
     const uint32_t n_items = eng->n_items, n_sp = eng->n_sp, n_draws = eng->n_draws, n_units = eng->n_units;
     const uint32_t tile_total = eng->tile_total, bin_total = eng->bin_total;
     const uint32_t n_targets = eng->n_targets;
-    CK(hipEventRecord(eng->ev_begin, s));
+    if (!cap)
+        CK(hipEventRecord(eng->ev_begin, s));
     uint8_t *db = (uint8_t *)eng->blob.p;
     const vkh_item *d_items = (const vkh_item *)(db + eng->o_items);
     const vkh_subpath *d_sp = (const vkh_subpath *)(db + eng->o_sp);
@@ -496,18 +554,18 @@ int vkvg_hip_execute(vkh_engine *eng) {
     const Unit *d_units = (const Unit *)(db + eng->o_units);
 
     // ---- flatten
-    if (!ensure(eng, eng->item_cnt, 4 * (n_items + 1)) || !ensure(eng, eng->item_off, 4 * (n_items + 1)))
+    if (!need(eng, cap, eng->item_cnt, 4 * (n_items + 1)) || !need(eng, cap, eng->item_off, 4 * (n_items + 1)))
         return -1;
     vkh::launch_flatten_count(s, d_items, n_items, P<uint32_t>(eng->item_cnt));
-    if (scan_u32(eng, P<uint32_t>(eng->item_cnt), n_items, P<uint32_t>(eng->item_off)))
+    if (scan_u32(eng, P<uint32_t>(eng->item_cnt), n_items, P<uint32_t>(eng->item_off), cap))
         return -1;
     uint32_t tot[8];
-    if (read_totals(eng, {P<uint32_t>(eng->item_off) + n_items}, tot))
+    if (totals(eng, cap, {P<uint32_t>(eng->item_off) + n_items}, tot, 0))
         return -1;
     const uint32_t n_pts = tot[0];
     st.points = n_pts;
-    if (!ensure(eng, eng->pts, 8ull * n_pts + 8) || !ensure(eng, eng->pflags, n_pts + 8) ||
-        !ensure(eng, eng->sp_first, 4 * (n_sp + 1)) || !ensure(eng, eng->sp_count, 4 * (n_sp + 1)))
+    if (!need(eng, cap, eng->pts, 8ull * n_pts + 8) || !need(eng, cap, eng->pflags, n_pts + 8) ||
+        !need(eng, cap, eng->sp_first, 4 * (n_sp + 1)) || !need(eng, cap, eng->sp_count, 4 * (n_sp + 1)))
         return -1;
     vkh::launch_flatten_emit(s, d_items, n_items, P<uint32_t>(eng->item_off), P<v2>(eng->pts),
                              P<uint8_t>(eng->pflags));
@@ -515,33 +573,34 @@ int vkvg_hip_execute(vkh_engine *eng) {
                                P<uint32_t>(eng->sp_count));
     CK(hipGetLastError());
 
-    CK(hipEventRecord(eng->ev_stage[0], s)); // flatten done
+    if (!cap)
+        CK(hipEventRecord(eng->ev_stage[0], s)); // flatten done
     // ---- per-unit geometry: fill edge counts and stroke work steps
     size_t un4 = 4ull * (n_units + 1);
-    if (!ensure(eng, eng->fe, un4) || !ensure(eng, eng->nv, un4) || !ensure(eng, eng->fe_off, un4) ||
-        !ensure(eng, eng->nv_off, un4) || !ensure(eng, eng->u_toff, un4) || !ensure(eng, eng->u_vbeg, un4) ||
-        !ensure(eng, eng->u_vend, un4))
+    if (!need(eng, cap, eng->fe, un4) || !need(eng, cap, eng->nv, un4) || !need(eng, cap, eng->fe_off, un4) ||
+        !need(eng, cap, eng->nv_off, un4) || !need(eng, cap, eng->u_toff, un4) || !need(eng, cap, eng->u_vbeg, un4) ||
+        !need(eng, cap, eng->u_vend, un4))
         return -1;
     uint32_t *d_steps = P<uint32_t>(eng->nv), *d_step_off = P<uint32_t>(eng->nv_off);
     vkh::launch_unit_count(s, d_units, n_units, d_draws, P<uint32_t>(eng->sp_count), d_styles, P<uint32_t>(eng->fe),
                            d_steps);
-    if (scan_u32(eng, P<uint32_t>(eng->fe), n_units, P<uint32_t>(eng->fe_off)) ||
-        scan_u32(eng, d_steps, n_units, d_step_off))
+    if (scan_u32(eng, P<uint32_t>(eng->fe), n_units, P<uint32_t>(eng->fe_off), cap) ||
+        scan_u32(eng, d_steps, n_units, d_step_off, cap))
         return -1;
-    if (read_totals(eng, {P<uint32_t>(eng->fe_off) + n_units, d_step_off + n_units}, tot))
+    if (totals(eng, cap, {P<uint32_t>(eng->fe_off) + n_units, d_step_off + n_units}, tot, 1))
         return -1;
     const uint32_t n_fill_edges = tot[0], n_steps = tot[1];
     size_t sn4 = 4ull * (n_steps + 1);
-    if (!ensure(eng, eng->step_nv, sn4) || !ensure(eng, eng->step_nt, sn4) || !ensure(eng, eng->sv_off, sn4) ||
-        !ensure(eng, eng->st_off, sn4))
+    if (!need(eng, cap, eng->step_nv, sn4) || !need(eng, cap, eng->step_nt, sn4) || !need(eng, cap, eng->sv_off, sn4) ||
+        !need(eng, cap, eng->st_off, sn4))
         return -1;
     vkh::launch_stroke_steps(s, false, d_units, n_units, d_step_off, n_steps, d_draws, d_sp, P<uint32_t>(eng->sp_first),
                              P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes,
                              P<uint32_t>(eng->step_nv), P<uint32_t>(eng->step_nt), nullptr, nullptr, nullptr, nullptr);
-    if (scan_u32(eng, P<uint32_t>(eng->step_nv), n_steps, P<uint32_t>(eng->sv_off)) ||
-        scan_u32(eng, P<uint32_t>(eng->step_nt), n_steps, P<uint32_t>(eng->st_off)))
+    if (scan_u32(eng, P<uint32_t>(eng->step_nv), n_steps, P<uint32_t>(eng->sv_off), cap) ||
+        scan_u32(eng, P<uint32_t>(eng->step_nt), n_steps, P<uint32_t>(eng->st_off), cap))
         return -1;
-    if (read_totals(eng, {P<uint32_t>(eng->sv_off) + n_steps, P<uint32_t>(eng->st_off) + n_steps}, tot))
+    if (totals(eng, cap, {P<uint32_t>(eng->sv_off) + n_steps, P<uint32_t>(eng->st_off) + n_steps}, tot, 3))
         return -1;
     const uint32_t n_verts = tot[0], n_tris = tot[1];
     const uint64_t n_edges64 = (uint64_t)n_fill_edges + 3ull * n_tris;
@@ -553,9 +612,9 @@ int vkvg_hip_execute(vkh_engine *eng) {
     st.stroke_vertices = n_verts;
     st.stroke_triangles = n_tris;
     st.edges = n_edges;
-    if (!ensure(eng, eng->verts, 8ull * n_verts + 8) || !ensure(eng, eng->tris, 12ull * n_tris + 12) ||
-        !ensure(eng, eng->edges, sizeof(Edge) * (size_t)n_edges + 16) ||
-        !ensure(eng, eng->edraw, 4ull * n_edges + 4))
+    if (!need(eng, cap, eng->verts, 8ull * n_verts + 8) || !need(eng, cap, eng->tris, 12ull * n_tris + 12) ||
+        !need(eng, cap, eng->edges, sizeof(Edge) * (size_t)n_edges + 16) ||
+        !need(eng, cap, eng->edraw, 4ull * n_edges + 4))
         return -1;
     vkh::launch_stroke_steps(s, true, d_units, n_units, d_step_off, n_steps, d_draws, d_sp, P<uint32_t>(eng->sp_first),
                              P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes,
@@ -574,83 +633,90 @@ int vkvg_hip_execute(vkh_engine *eng) {
                           P<Edge>(eng->edges) + n_fill_edges, P<uint32_t>(eng->edraw) + n_fill_edges);
     CK(hipGetLastError());
 
-    CK(hipEventRecord(eng->ev_stage[1], s)); // geometry done
+    if (!cap)
+        CK(hipEventRecord(eng->ev_stage[1], s)); // geometry done
     // ---- tiles of every draw
     size_t dn4 = 4ull * (n_draws + 1);
-    if (!ensure(eng, eng->bbox, sizeof(int4) * (n_draws + 1)) || !ensure(eng, eng->dtiles, sizeof(DrawTiles) * (n_draws + 1)) ||
-        !ensure(eng, eng->erange, sizeof(uint2) * (n_draws + 1)) || !ensure(eng, eng->pt_count, dn4) ||
-        !ensure(eng, eng->pt_off, dn4) || !ensure(eng, eng->seg_alloc, 256))
+    if (!need(eng, cap, eng->bbox, sizeof(int4) * (n_draws + 1)) || !need(eng, cap, eng->dtiles, sizeof(DrawTiles) * (n_draws + 1)) ||
+        !need(eng, cap, eng->erange, sizeof(uint2) * (n_draws + 1)) || !need(eng, cap, eng->pt_count, dn4) ||
+        !need(eng, cap, eng->pt_off, dn4) || !need(eng, cap, eng->seg_alloc, 256))
         return -1;
     vkh::launch_draw_bbox_init(s, P<int4>(eng->bbox), P<uint2>(eng->erange), n_draws);
     vkh::launch_draw_bbox(s, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), n_edges, P<int4>(eng->bbox),
                           P<uint2>(eng->erange));
     vkh::launch_draw_tiles(s, d_draws, n_draws, d_tg, P<int4>(eng->bbox), T, P<DrawTiles>(eng->dtiles),
                            P<uint32_t>(eng->pt_count));
-    if (!ensure(eng, eng->dprep, sizeof(DrawPrep) * (n_draws + 1)))
+    if (!need(eng, cap, eng->dprep, sizeof(DrawPrep) * (n_draws + 1)))
         return -1;
     vkh::launch_draw_prep(s, d_draws, n_draws, d_tg, d_grads, P<DrawPrep>(eng->dprep));
-    if (scan_u32(eng, P<uint32_t>(eng->pt_count), n_draws, P<uint32_t>(eng->pt_off)))
+    if (scan_u32(eng, P<uint32_t>(eng->pt_count), n_draws, P<uint32_t>(eng->pt_off), cap))
         return -1;
-    if (read_totals(eng, {P<uint32_t>(eng->pt_off) + n_draws}, tot))
+    if (totals(eng, cap, {P<uint32_t>(eng->pt_off) + n_draws}, tot, 5))
         return -1;
     const uint32_t n_ptiles = tot[0];
     st.path_tiles = n_ptiles;
     vkh::launch_set_pt_base(s, P<DrawTiles>(eng->dtiles), P<uint32_t>(eng->pt_off), n_draws);
-    if (!ensure(eng, eng->ptiles, sizeof(PathTile) * (size_t)n_ptiles + 16) || !ensure(eng, eng->segs, 4096))
+    if (!need(eng, cap, eng->ptiles, sizeof(PathTile) * (size_t)n_ptiles + 16) || !need(eng, cap, eng->segs, 4096))
         return -1;
-    CK(hipEventRecord(eng->ev_stage[2], s)); // tile setup done
+    if (!cap)
+        CK(hipEventRecord(eng->ev_stage[2], s)); // tile setup done
     // ---- binning: per-draw segment lists and backdrops; bins again once if the segment buffer
     // was too small (the allocator then holds the exact total)
     uint32_t n_segs = 0;
     for (int pass = 0; pass < 2; pass++) {
-        const uint32_t cap = (uint32_t)std::min<size_t>(eng->segs.cap / 4, 0xFFFFFFFFull);
+        const uint32_t seg_cap = (uint32_t)std::min<size_t>(eng->segs.cap / 4, 0xFFFFFFFFull);
         vkh::launch_zero_u32(s, P<uint32_t>(eng->seg_alloc), 1);
         vkh::launch_bin_draws(s, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), P<uint2>(eng->erange),
                               P<DrawTiles>(eng->dtiles), n_draws, T, P<PathTile>(eng->ptiles), P<uint32_t>(eng->segs),
-                              cap, P<uint32_t>(eng->seg_alloc));
+                              seg_cap, P<uint32_t>(eng->seg_alloc));
         CK(hipGetLastError());
-        if (read_totals(eng, {P<uint32_t>(eng->seg_alloc)}, tot))
+        if (totals(eng, cap, {P<uint32_t>(eng->seg_alloc)}, tot, 6))
             return -1;
         n_segs = tot[0];
-        if (n_segs <= cap)
+        if (n_segs <= seg_cap)
             break;
         if (pass == 1) {
             eng->err = "segment buffer overflow after resize";
             return -1;
         }
-        if (!ensure(eng, eng->segs, 4ull * n_segs + 4))
+        if (!need(eng, cap, eng->segs, 4ull * n_segs + 4))
             return -1;
     }
     st.segments = n_segs;
 
-    CK(hipEventRecord(eng->ev_stage[3], s)); // binning done
+    if (!cap)
+        CK(hipEventRecord(eng->ev_stage[3], s)); // binning done
     // ---- coarse: ordered command list per tile
-    if (!ensure(eng, eng->tgts, sizeof(TargetInfo) * (n_targets + 1)) ||
-        !ensure(eng, eng->tile_cnt, 4ull * (tile_total + 1)) || !ensure(eng, eng->tile_off, 4ull * (tile_total + 1)))
+    if (!need(eng, cap, eng->tgts, sizeof(TargetInfo) * (n_targets + 1)) ||
+        !need(eng, cap, eng->tile_cnt, 4ull * (tile_total + 1)) || !need(eng, cap, eng->tile_off, 4ull * (tile_total + 1)))
         return -1;
-    CK(hipMemsetAsync(eng->tile_cnt.p, 0, 4ull * (tile_total + 1), s));
+    vkh::launch_zero_u32(s, P<uint32_t>(eng->tile_cnt), tile_total + 1);
     vkh::launch_coarse(s, false, bin_total, d_tg, n_targets, d_draws, P<DrawTiles>(eng->dtiles),
                        P<PathTile>(eng->ptiles), P<uint32_t>(eng->tile_cnt), nullptr, nullptr);
-    if (scan_u32(eng, P<uint32_t>(eng->tile_cnt), tile_total, P<uint32_t>(eng->tile_off)))
+    if (scan_u32(eng, P<uint32_t>(eng->tile_cnt), tile_total, P<uint32_t>(eng->tile_off), cap))
         return -1;
-    if (read_totals(eng, {P<uint32_t>(eng->tile_off) + tile_total}, tot))
+    if (totals(eng, cap, {P<uint32_t>(eng->tile_off) + tile_total}, tot, 7))
         return -1;
     const uint32_t n_cmds = tot[0];
     st.commands = n_cmds;
     st.tiles = tile_total;
-    if (!ensure(eng, eng->cmds, sizeof(Cmd) * (size_t)n_cmds + 16))
+    if (!need(eng, cap, eng->cmds, sizeof(Cmd) * (size_t)n_cmds + 16))
         return -1;
     vkh::launch_coarse(s, true, bin_total, d_tg, n_targets, d_draws, P<DrawTiles>(eng->dtiles),
                        P<PathTile>(eng->ptiles), nullptr, P<uint32_t>(eng->tile_off), P<Cmd>(eng->cmds));
     CK(hipGetLastError());
 
     // ---- fine: coverage + compositing
-    CK(hipEventRecord(eng->ev_stage[4], s)); // coarse done
-    CK(hipEventRecord(eng->ev0, s));
-    vkh::launch_fine(s, T, tile_total, d_tg, n_targets, P<DrawPrep>(eng->dprep), d_grads, P<PathTile>(eng->ptiles),
+    if (!cap)
+        CK(hipEventRecord(eng->ev_stage[4], s)); // coarse done
+    if (!cap)
+        CK(hipEventRecord(eng->ev0, s));
+    vkh::launch_fine(s, T, eng->fine_words, tile_total, d_tg, n_targets, P<DrawPrep>(eng->dprep), d_grads, P<PathTile>(eng->ptiles),
                      P<uint32_t>(eng->segs), P<Edge>(eng->edges), n_edges, P<uint32_t>(eng->tile_off),
                      P<Cmd>(eng->cmds));
     CK(hipGetLastError());
+    if (cap)
+        return 0;
     CK(hipEventRecord(eng->ev1, s));
     CK(hipEventRecord(eng->ev_end, s));
     CK(hipStreamSynchronize(s));
@@ -669,6 +735,164 @@ int vkvg_hip_execute(vkh_engine *eng) {
     }
     st.ms_total = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     eng->stats = st;
+    return 0;
+}
+
+static uint64_t graph_key(const vkh_engine *eng) {
+    return (eng->frame_gen * 1000003ull + eng->alloc_gen) * 64ull + eng->frame_tile * 2ull + (eng->fine_words >> 11);
+}
+
+static void drop_graph(vkh_engine *eng) {
+    if (eng->gexec)
+        (void)hipGraphExecDestroy(eng->gexec);
+    if (eng->graph)
+        (void)hipGraphDestroy(eng->graph);
+    eng->gexec = nullptr;
+    eng->graph = nullptr;
+    eng->gkey = 0;
+}
+
+// Captures the resident frame's launch sequence (sizes from the sizing run just done) into a
+// graph; on any failure the frame keeps running eagerly.
+static void capture_graph(vkh_engine *eng) {
+    drop_graph(eng);
+    hipStream_t s = eng->stream;
+    const uint64_t key = graph_key(eng);
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        eng->gfail_key = key;
+        (void)hipGetLastError();
+        return;
+    }
+    std::string keep_err = eng->err;
+    int rc = run_pipeline(eng, true);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc == 0 && e == hipSuccess && g && hipGraphInstantiate(&eng->gexec, g, nullptr, nullptr, 0) == hipSuccess) {
+        eng->graph = g;
+        eng->gkey = graph_key(eng);
+    } else {
+        if (g)
+            (void)hipGraphDestroy(g);
+        eng->gexec = nullptr;
+        eng->gfail_key = key;
+        (void)hipGetLastError();
+    }
+    eng->err = keep_err;
+}
+
+int vkvg_hip_execute(vkh_engine *eng) {
+    if (!eng || !eng->resident) {
+        if (eng)
+            eng->err = "no resident frame";
+        return -1;
+    }
+    (void)hipSetDevice(eng->device);
+    hipStream_t s = eng->stream;
+    if (eng->graph_on && eng->gexec && eng->gkey == graph_key(eng)) {
+        auto t0 = std::chrono::steady_clock::now();
+        CK(hipEventRecord(eng->ev_begin, s));
+        CK(hipGraphLaunch(eng->gexec, s));
+        CK(hipEventRecord(eng->ev_end, s));
+        CK(hipStreamSynchronize(s));
+        vkh_stats st = eng->stats;
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, eng->ev_begin, eng->ev_end);
+        st.ms_device = ms;
+        st.ms_fine = 0;
+        for (float &x : st.ms_stage)
+            x = 0;
+        st.ms_total = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        st.graph = 1;
+        eng->stats = st;
+        return 0;
+    }
+    int r = run_pipeline(eng, false);
+    if (r == 0 && eng->graph_on && eng->gfail_key != graph_key(eng))
+        capture_graph(eng);
+    return r;
+}
+
+
+
+int vkvg_hip_retile(vkh_engine *eng, uint32_t T) {
+    if (!eng || !eng->resident) {
+        if (eng)
+            eng->err = "no resident frame";
+        return -1;
+    }
+    if (T != 8 && T != 16 && T != 32)
+        return -1;
+    if (T == eng->frame_tile)
+        return 0;
+    (void)hipSetDevice(eng->device);
+    hipStream_t s = eng->stream;
+    CK(hipStreamSynchronize(s));
+    uint32_t tile_total = 0, bin_total = 0;
+    for (TargetInfo &ti : eng->h_tg) {
+        ti.tilesW = (ti.width + T - 1) / T;
+        ti.tilesH = (ti.height + T - 1) / T;
+        ti.tile_base = tile_total;
+        tile_total += ti.tilesW * ti.tilesH;
+        ti.binsW = (ti.tilesW + VKH_BIN_TILES - 1) / VKH_BIN_TILES;
+        ti.binsH = (ti.tilesH + VKH_BIN_TILES - 1) / VKH_BIN_TILES;
+        ti.bin_base = bin_total;
+        bin_total += ti.binsW * ti.binsH;
+    }
+    uint8_t *hb = (uint8_t *)eng->hblob.p;
+    const size_t bytes = sizeof(TargetInfo) * eng->h_tg.size();
+    memcpy(hb + eng->o_tg, eng->h_tg.data(), bytes);
+    CK(hipMemcpyAsync((uint8_t *)eng->blob.p + eng->o_tg, hb + eng->o_tg, bytes, hipMemcpyHostToDevice, s));
+    CK(hipStreamSynchronize(s));
+    eng->tile_total = tile_total;
+    eng->bin_total = bin_total;
+    eng->frame_tile = T;
+    eng->frame_gen++;
+    eng->gfail_key = 0;
+    return 0;
+}
+
+int vkvg_hip_autotune(vkh_engine *eng, uint32_t reps, vkh_tune *out) {
+    if (!eng || !eng->resident) {
+        if (eng)
+            eng->err = "no resident frame";
+        return -1;
+    }
+    if (reps == 0)
+        reps = 3;
+    const bool graph = eng->graph_on;
+    eng->graph_on = false;
+    static const uint32_t tiles[3] = {8, 16, 32}, words[3] = {2048, 4096, 8192};
+    vkh_tune res;
+    memset(&res, 0, sizeof(res));
+    float best = 1e30f;
+    for (uint32_t T : tiles) {
+        if (vkvg_hip_retile(eng, T))
+            return -1;
+        for (uint32_t w : words) {
+            eng->fine_words = w;
+            if (vkvg_hip_execute(eng)) // sizing run, also a warm-up
+                return -1;
+            float m = 1e30f;
+            for (uint32_t r = 0; r < reps; r++) {
+                if (vkvg_hip_execute(eng))
+                    return -1;
+                m = std::min(m, eng->stats.ms_device);
+            }
+            res.e[res.n] = vkh_tune_entry{T, w, m};
+            if (m < best) {
+                best = m;
+                res.best = res.n;
+            }
+            res.n++;
+        }
+    }
+    eng->fine_words = res.e[res.best].fine_words;
+    if (vkvg_hip_retile(eng, res.e[res.best].tile))
+        return -1;
+    eng->tile = res.e[res.best].tile;
+    eng->graph_on = graph;
+    if (out)
+        *out = res;
     return 0;
 }
 

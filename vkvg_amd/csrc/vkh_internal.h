@@ -110,8 +110,9 @@ void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n_draws, co
 void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, const TargetInfo *tg, uint32_t n_targets,
                    const vkh_draw *draws, const DrawTiles *dt, const PathTile *pts, uint32_t *tile_count,
                    const uint32_t *tile_off, Cmd *cmds);
-void launch_fine(hipStream_t s, uint32_t tile, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
-                 const DrawPrep *dprep, const vkh_gradient *grads, const PathTile *pts, const uint32_t *segs,
-                 const Edge *edges, uint32_t n_edges, const uint32_t *tile_off, const Cmd *cmds);
+void launch_fine(hipStream_t s, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,
+                 uint32_t n_targets, const DrawPrep *dprep, const vkh_gradient *grads, const PathTile *pts,
+                 const uint32_t *segs, const Edge *edges, uint32_t n_edges, const uint32_t *tile_off,
+                 const Cmd *cmds);
 void launch_unpremultiply(hipStream_t s, const uint8_t *src, uint8_t *dst, size_t npix);
 } // namespace vkh

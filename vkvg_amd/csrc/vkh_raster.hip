@@ -525,10 +525,6 @@ __device__ __forceinline__ uint32_t blend_px(uint32_t op, uint32_t d, uint32_t s
 
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
-// row-delta words of LDS per fine workgroup (B = words / pixels commands per batch)
-#ifndef VKH_FINE_LDS_WORDS
-#define VKH_FINE_LDS_WORDS 4096
-#endif
 
 // One workgroup per T x T tile. Commands are processed in batches of B: the batch's command
 // records (draw record + path tile) are staged in LDS with one parallel load; every segment of the
@@ -537,7 +533,8 @@ __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) { return (uint32_t)_
 // pixel applies the batch's commands in order, reading each command's record as wave-uniform
 // values. The dependent global loads (command -> draw / path tile -> segment -> edge) are thus
 // paid once per batch and once per segment.
-template <int T>
+// WORDS: row-delta words of LDS per workgroup (B = WORDS / pixels commands per batch)
+template <int T, int WORDS>
 __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg, uint32_t n_targets,
                                               const DrawPrep *__restrict__ dprep,
                                               const vkh_gradient *__restrict__ grads,
@@ -547,7 +544,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
     constexpr int NPIX = T * T;
     constexpr int NT = NPIX >= 256 ? 256 : NPIX;  // threads
     constexpr int PPT = NPIX / NT;                // pixels per thread
-    constexpr int B = VKH_FINE_LDS_WORDS / NPIX;  // commands per batch
+    constexpr int B = WORDS / NPIX;               // commands per batch
     __shared__ int32_t sd[B][T][T];               // row deltas, then per-row inclusive prefix
     __shared__ int32_t srow[B][T];                // V term: +-1 at the first row below a column crossing
     __shared__ uint32_t c_item[B + 1];            // prefix of the commands' segment counts
@@ -824,23 +821,45 @@ void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n, const Ta
     if (n)
         hipLaunchKernelGGL(k_draw_prep, dim3((n + 255) / 256), dim3(256), 0, s, draws, n, tg, grads, out);
 }
-void launch_fine(hipStream_t s, uint32_t tile, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
-                 const DrawPrep *dprep, const vkh_gradient *grads, const PathTile *pts, const uint32_t *segs,
-                 const Edge *edges, uint32_t n_edges, const uint32_t *tile_off, const Cmd *cmds) {
+template <int T, int W>
+static void fine_launch(hipStream_t s, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
+                        const DrawPrep *dprep, const vkh_gradient *grads, const PathTile *pts, const uint32_t *segs,
+                        const Edge *edges, uint32_t n_edges, const uint32_t *tile_off, const Cmd *cmds) {
+    constexpr int NT = T * T >= 256 ? 256 : T * T;
+    hipLaunchKernelGGL((k_fine<T, W>), dim3(n_tiles), dim3(NT), 0, s, tg, n_targets, dprep, grads, pts, segs, edges,
+                       n_edges, tile_off, cmds);
+}
+template <int T>
+static void fine_words(hipStream_t s, uint32_t words, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
+                       const DrawPrep *dprep, const vkh_gradient *grads, const PathTile *pts, const uint32_t *segs,
+                       const Edge *edges, uint32_t n_edges, const uint32_t *tile_off, const Cmd *cmds) {
+    switch (words) {
+    case 2048:
+        fine_launch<T, 2048>(s, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, tile_off, cmds);
+        break;
+    case 8192:
+        fine_launch<T, 8192>(s, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, tile_off, cmds);
+        break;
+    default:
+        fine_launch<T, 4096>(s, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, tile_off, cmds);
+        break;
+    }
+}
+void launch_fine(hipStream_t s, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,
+                 uint32_t n_targets, const DrawPrep *dprep, const vkh_gradient *grads, const PathTile *pts,
+                 const uint32_t *segs, const Edge *edges, uint32_t n_edges, const uint32_t *tile_off,
+                 const Cmd *cmds) {
     if (!n_tiles)
         return;
     switch (tile) {
     case 8:
-        hipLaunchKernelGGL(k_fine<8>, dim3(n_tiles), dim3(64), 0, s, tg, n_targets, dprep, grads, pts, segs, edges,
-                           n_edges, tile_off, cmds);
+        fine_words<8>(s, words, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, tile_off, cmds);
         break;
     case 32:
-        hipLaunchKernelGGL(k_fine<32>, dim3(n_tiles), dim3(256), 0, s, tg, n_targets, dprep, grads, pts, segs, edges,
-                           n_edges, tile_off, cmds);
+        fine_words<32>(s, words, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, tile_off, cmds);
         break;
     default:
-        hipLaunchKernelGGL(k_fine<16>, dim3(n_tiles), dim3(256), 0, s, tg, n_targets, dprep, grads, pts, segs, edges,
-                           n_edges, tile_off, cmds);
+        fine_words<16>(s, words, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, tile_off, cmds);
         break;
     }
 }
