@@ -376,7 +376,7 @@ __device__ __forceinline__ void batch_coverage(float (*acc)[VKD_T][VKD_T + 1], u
     }
 }
 
-__global__ __launch_bounds__(256) void k_dfine(uint32_t W, uint32_t H, uint32_t tilesW,
+__global__ __launch_bounds__(256) void k_dfine(uint32_t W, uint32_t H, uint32_t tilesW, uint32_t nch,
                                                const uint32_t *__restrict__ tile_off, const Cmd *__restrict__ cmds,
                                                const PathTile *__restrict__ ptl, const uint32_t *__restrict__ segs,
                                                const float4 *__restrict__ edges, const float4 *__restrict__ rgba,
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(256) void k_dfine(uint32_t W, uint32_t H, uint32_t 
     __shared__ float acc[VKD_B][VKD_T][VKD_T + 1];
     __shared__ uint32_t c_nseg[VKD_B], c_segoff[VKD_B], c_draw[VKD_B];
     const uint32_t t = blockIdx.x, tid = threadIdx.x;
-    const uint32_t c0 = tile_off[t], c1 = tile_off[t + 1];
+    const uint32_t c0 = tile_off[(size_t)t * nch], c1 = tile_off[(size_t)(t + 1) * nch];
     const int32_t X0 = (int32_t)(t % tilesW) * VKD_T, Y0 = (int32_t)(t / tilesW) * VKD_T;
     const int32_t px = X0 + (int32_t)(tid % VKD_T), py = Y0 + (int32_t)(tid / VKD_T);
     const bool valid = px < (int32_t)W && py < (int32_t)H;
@@ -441,7 +441,7 @@ __device__ __forceinline__ float wave_multi_sum32(float (&v)[32], uint32_t lane,
     return v[0] + __shfl_xor(v[0], 1);
 }
 
-__global__ __launch_bounds__(256) void k_dback(uint32_t W, uint32_t H, uint32_t tilesW,
+__global__ __launch_bounds__(256) void k_dback(uint32_t W, uint32_t H, uint32_t tilesW, uint32_t nch,
                                                const uint32_t *__restrict__ tile_off, const Cmd *__restrict__ cmds,
                                                const PathTile *__restrict__ ptl, const uint32_t *__restrict__ segs,
                                                const float4 *__restrict__ edges, const float4 *__restrict__ rgba,
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(256) void k_dback(uint32_t W, uint32_t H, uint32_t 
     __shared__ float s_red[4][32];
     __shared__ uint32_t c_nseg[VKD_B], c_segoff[VKD_B], c_draw[VKD_B];
     const uint32_t t = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
-    const uint32_t c0 = tile_off[t], c1 = tile_off[t + 1];
+    const uint32_t c0 = tile_off[(size_t)t * nch], c1 = tile_off[(size_t)(t + 1) * nch];
     if (c0 == c1)
         return;
     const int32_t X0 = (int32_t)(t % tilesW) * VKD_T, Y0 = (int32_t)(t / tilesW) * VKD_T;
@@ -661,10 +661,11 @@ __global__ void k_dg_paths(uint32_t n_paths, const float4 *__restrict__ rgba, co
     g_width[p] = path_stroke[p] ? 0.5f * g_h[p] : 0.f;
 }
 
-__global__ void k_dbatches(uint32_t n_tiles, const uint32_t *__restrict__ tile_off, uint32_t *__restrict__ nb) {
+__global__ void k_dbatches(uint32_t n_tiles, uint32_t nch, const uint32_t *__restrict__ tile_off,
+                           uint32_t *__restrict__ nb) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n_tiles)
-        nb[t] = (tile_off[t + 1] - tile_off[t] + VKD_B - 1) / VKD_B;
+        nb[t] = (tile_off[(size_t)(t + 1) * nch] - tile_off[(size_t)t * nch] + VKD_B - 1) / VKD_B;
 }
 
 __global__ void k_dzero(uint32_t *p, size_t n) {
@@ -682,7 +683,7 @@ struct Buf {
 
 struct vkvg_diff_scene_t {
     uint32_t W = 0, H = 0, n_paths = 0, n_pts = 0, n_ctrl = 0, n_edges = 0, tilesW = 0, tilesH = 0, n_tiles = 0;
-    uint32_t bin_total = 0;
+    uint32_t bin_total = 0, nch = 1;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
     std::string err;
@@ -755,6 +756,7 @@ VkvgDiffScene vkvg_diff_create(uint32_t width, uint32_t height, uint32_t n_paths
     s->tilesW = (width + VKD_T - 1) / VKD_T;
     s->tilesH = (height + VKD_T - 1) / VKD_T;
     s->n_tiles = s->tilesW * s->tilesH;
+    s->nch = (n_paths + VKH_COARSE_CHUNK - 1) / VKH_COARSE_CHUNK;
     std::vector<uint32_t> first_pt(n_paths + 1), first_ctrl(n_paths + 1), first_edge(n_paths + 1), pt_path, ctrl_path;
     std::vector<uint32_t> stroke(n_paths), nc(n_paths);
     uint64_t np = 0, nctl = 0, ne = 0;
@@ -819,7 +821,8 @@ VkvgDiffScene vkvg_diff_create(uint32_t width, uint32_t height, uint32_t n_paths
          ensure(s, s->src, sizeof(DSrc) * ne) && ensure(s, s->bbox, 16ull * n_paths) &&
          ensure(s, s->dtiles, sizeof(DrawTiles) * n_paths) && ensure(s, s->pt_count, 4ull * (n_paths + 1)) &&
          ensure(s, s->pt_off, 4ull * (n_paths + 1)) && ensure(s, s->seg_alloc, 256) && ensure(s, s->segs, 4096) &&
-         ensure(s, s->tile_cnt, 4ull * (s->n_tiles + 1)) && ensure(s, s->tile_off, 4ull * (s->n_tiles + 1)) &&
+         ensure(s, s->tile_cnt, 4ull * ((size_t)s->n_tiles * s->nch + 1)) &&
+         ensure(s, s->tile_off, 4ull * ((size_t)s->n_tiles * s->nch + 1)) &&
          ensure(s, s->nbat, 4ull * (s->n_tiles + 1)) && ensure(s, s->batch_off, 4ull * (s->n_tiles + 1)) &&
          ensure(s, s->g_edge, 16ull * ne) && ensure(s, s->g_pt, 8ull * np) && ensure(s, s->g_n, 8ull * np) &&
          ensure(s, s->g_h, 4ull * n_paths) && ensure(s, s->g_col, 16ull * n_paths);
@@ -901,21 +904,22 @@ int vkvg_diff_forward(VkvgDiffScene s, const float *ctrl, const float *rgba, con
             return -1;
         }
     }
-    zero(s, s->tile_cnt, s->n_tiles + 1);
-    vkh::launch_coarse(st, false, s->bin_total, P<TargetInfo>(s->tg), 1, P<vkh_draw>(s->draws),
+    const uint32_t n_tc = s->n_tiles * s->nch;
+    zero(s, s->tile_cnt, n_tc + 1);
+    vkh::launch_coarse(st, false, s->bin_total, s->nch, P<TargetInfo>(s->tg), 1, P<vkh_draw>(s->draws),
                        P<DrawTiles>(s->dtiles), P<PathTile>(s->ptiles), P<uint32_t>(s->tile_cnt), nullptr, nullptr);
-    if (!scan(s, P<uint32_t>(s->tile_cnt), s->n_tiles, P<uint32_t>(s->tile_off)))
+    if (!scan(s, P<uint32_t>(s->tile_cnt), n_tc, P<uint32_t>(s->tile_off)))
         return -1;
-    if (!read_u32(s, P<uint32_t>(s->tile_off) + s->n_tiles, &s->n_cmds))
+    if (!read_u32(s, P<uint32_t>(s->tile_off) + n_tc, &s->n_cmds))
         return -1;
     if (!ensure(s, s->cmds, sizeof(Cmd) * ((size_t)s->n_cmds + 1)))
         return -1;
-    vkh::launch_coarse(st, true, s->bin_total, P<TargetInfo>(s->tg), 1, P<vkh_draw>(s->draws),
+    vkh::launch_coarse(st, true, s->bin_total, s->nch, P<TargetInfo>(s->tg), 1, P<vkh_draw>(s->draws),
                        P<DrawTiles>(s->dtiles), P<PathTile>(s->ptiles), nullptr, P<uint32_t>(s->tile_off),
                        P<Cmd>(s->cmds));
     float4 *ck_ptr = nullptr;
     if (keep) {
-        hipLaunchKernelGGL(k_dbatches, dim3(blocks(s->n_tiles)), dim3(256), 0, st, s->n_tiles,
+        hipLaunchKernelGGL(k_dbatches, dim3(blocks(s->n_tiles)), dim3(256), 0, st, s->n_tiles, s->nch,
                            P<uint32_t>(s->tile_off), P<uint32_t>(s->nbat));
         if (!scan(s, P<uint32_t>(s->nbat), s->n_tiles, P<uint32_t>(s->batch_off)))
             return -1;
@@ -925,7 +929,8 @@ int vkvg_diff_forward(VkvgDiffScene s, const float *ctrl, const float *rgba, con
             return -1;
         ck_ptr = P<float4>(s->ckpt);
     }
-    hipLaunchKernelGGL(k_dfine, dim3(s->n_tiles), dim3(256), 0, st, s->W, s->H, s->tilesW, P<uint32_t>(s->tile_off),
+    hipLaunchKernelGGL(k_dfine, dim3(s->n_tiles), dim3(256), 0, st, s->W, s->H, s->tilesW, s->nch,
+                       P<uint32_t>(s->tile_off),
                        P<Cmd>(s->cmds), P<PathTile>(s->ptiles), P<uint32_t>(s->segs), P<float4>(s->edges),
                        (const float4 *)rgba, (float4 *)image, ck_ptr, P<uint32_t>(s->batch_off));
     if (!ck(s, hipGetLastError(), "forward launch") || !ck(s, hipEventRecord(s->ev[1], st), "event") ||
@@ -957,7 +962,8 @@ int vkvg_diff_backward(VkvgDiffScene s, const float *ctrl, const float *rgba, co
     zero(s, s->g_n, 2ull * s->n_pts);
     zero(s, s->g_h, s->n_paths);
     zero(s, s->g_col, 4ull * s->n_paths);
-    hipLaunchKernelGGL(k_dback, dim3(s->n_tiles), dim3(256), 0, st, s->W, s->H, s->tilesW, P<uint32_t>(s->tile_off),
+    hipLaunchKernelGGL(k_dback, dim3(s->n_tiles), dim3(256), 0, st, s->W, s->H, s->tilesW, s->nch,
+                       P<uint32_t>(s->tile_off),
                        P<Cmd>(s->cmds), P<PathTile>(s->ptiles), P<uint32_t>(s->segs), P<float4>(s->edges),
                        (const float4 *)rgba, (const float4 *)grad_image, P<float4>(s->ckpt),
                        P<uint32_t>(s->batch_off), P<float4>(s->g_edge), P<float>(s->g_col));

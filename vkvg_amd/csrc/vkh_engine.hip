@@ -2,6 +2,7 @@
 // include/vkvg_hip.h. One engine per device; a frame may hold draws for many targets.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -126,8 +127,8 @@ struct vkh_engine {
 
     DevBuf blob, units, tgts, item_cnt, item_off, pts, pflags, sp_first, sp_count;
     DevBuf fe, nv, nt, fe_off, nv_off, nt_off, verts, tris, edges, edraw;
-    DevBuf bbox, erange, dtiles, pt_count, pt_off, ptiles, seg_alloc, segs, dprep;
-    DevBuf tile_cnt, tile_off, cmds, scan_tmp, readback;
+    DevBuf bbox, erange, dtiles, pt_count, pt_off, ptiles, counters, segs, dprep; // counters: [0] segment allocator, [1] flatten slot allocator
+    DevBuf tile_cnt, tile_off, cmds, scan_tmp, readback, item_slot, slot_item, flat_scratch;
     DevBuf step_nv, step_nt, sv_off, st_off, u_toff, u_vbeg, u_vend;
     HostBuf hblob, htot;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
@@ -147,7 +148,7 @@ struct vkh_engine {
     bool resident = false;
     size_t o_items, o_sp, o_draws, o_styles, o_dashes, o_grads, o_tg, o_units;
     uint32_t n_items, n_sp, n_draws, n_styles, n_dashes, n_grads, n_targets, n_units;
-    uint32_t tile_total, bin_total, frame_tile;
+    uint32_t tile_total, bin_total, frame_tile, coarse_nch = 1, n_cubic_items = 0;
 };
 
 static bool check(vkh_engine *e, hipError_t r, const char *what) {
@@ -261,9 +262,10 @@ void vkvg_hip_engine_destroy(vkh_engine *e) {
     DevBuf *bufs[] = {&e->blob, &e->units, &e->tgts, &e->item_cnt, &e->item_off, &e->pts, &e->pflags,
                       &e->sp_first, &e->sp_count, &e->fe, &e->nv, &e->nt, &e->fe_off, &e->nv_off, &e->nt_off,
                       &e->verts, &e->tris, &e->edges, &e->edraw, &e->bbox, &e->dtiles, &e->pt_count, &e->pt_off,
-                      &e->erange, &e->ptiles, &e->seg_alloc, &e->segs, &e->dprep, &e->tile_cnt,
+                      &e->erange, &e->ptiles, &e->counters, &e->segs, &e->dprep, &e->tile_cnt,
                       &e->tile_off, &e->cmds, &e->scan_tmp, &e->readback, &e->step_nv, &e->step_nt,
-                      &e->sv_off, &e->st_off, &e->u_toff, &e->u_vbeg, &e->u_vend};
+                      &e->sv_off, &e->st_off, &e->u_toff, &e->u_vbeg, &e->u_vend, &e->item_slot,
+                      &e->slot_item, &e->flat_scratch};
     for (DevBuf *b : bufs)
         if (b->p)
             (void)hipFree(b->p);
@@ -508,6 +510,15 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
     eng->bin_total = bin_total;
     eng->frame_tile = T;
     eng->h_tg = tg;
+    eng->n_cubic_items = 0;
+    for (uint32_t i = 0; i < n_items; i++)
+        eng->n_cubic_items += (f->items[i].flags & VKH_ITEM_CUBIC) ? 1u : 0u;
+    {
+        uint32_t most = 1;
+        for (const TargetInfo &ti : tg)
+            most = std::max(most, ti.draw_end - ti.draw_begin);
+        eng->coarse_nch = (most + VKH_COARSE_CHUNK - 1) / VKH_COARSE_CHUNK;
+    }
     eng->frame_gen++;
     eng->gfail_key = 0;
     eng->resident = true;
@@ -554,9 +565,15 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     const Unit *d_units = (const Unit *)(db + eng->o_units);
 
     // ---- flatten
-    if (!need(eng, cap, eng->item_cnt, 4 * (n_items + 1)) || !need(eng, cap, eng->item_off, 4 * (n_items + 1)))
+    const uint32_t n_cubics = eng->n_cubic_items;
+    if (!need(eng, cap, eng->item_cnt, 4 * (n_items + 1)) || !need(eng, cap, eng->item_off, 4 * (n_items + 1)) ||
+        !need(eng, cap, eng->item_slot, 4 * (n_items + 1)) || !need(eng, cap, eng->slot_item, 4ull * (n_cubics + 1)) ||
+        !need(eng, cap, eng->flat_scratch, 8ull * VKH_FLAT_SLOT * (n_cubics + 1)) ||
+        !need(eng, cap, eng->counters, 256))
         return -1;
-    vkh::launch_flatten_count(s, d_items, n_items, P<uint32_t>(eng->item_cnt));
+    vkh::launch_zero_u32(s, P<uint32_t>(eng->counters) + 1, 1);
+    vkh::launch_flatten_stage(s, d_items, n_items, P<uint32_t>(eng->item_cnt), P<uint32_t>(eng->counters) + 1,
+                              P<uint32_t>(eng->slot_item), P<uint32_t>(eng->item_slot), P<v2>(eng->flat_scratch));
     if (scan_u32(eng, P<uint32_t>(eng->item_cnt), n_items, P<uint32_t>(eng->item_off), cap))
         return -1;
     uint32_t tot[8];
@@ -567,8 +584,11 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     if (!need(eng, cap, eng->pts, 8ull * n_pts + 8) || !need(eng, cap, eng->pflags, n_pts + 8) ||
         !need(eng, cap, eng->sp_first, 4 * (n_sp + 1)) || !need(eng, cap, eng->sp_count, 4 * (n_sp + 1)))
         return -1;
-    vkh::launch_flatten_emit(s, d_items, n_items, P<uint32_t>(eng->item_off), P<v2>(eng->pts),
+    vkh::launch_flatten_copy(s, d_items, n_cubics, P<uint32_t>(eng->slot_item), P<uint32_t>(eng->item_cnt),
+                             P<uint32_t>(eng->item_off), P<v2>(eng->flat_scratch), P<v2>(eng->pts),
                              P<uint8_t>(eng->pflags));
+    vkh::launch_flatten_emit(s, d_items, n_items, P<uint32_t>(eng->item_cnt), P<uint32_t>(eng->item_off),
+                             P<v2>(eng->pts), P<uint8_t>(eng->pflags));
     vkh::launch_subpath_ranges(s, d_sp, n_sp, P<uint32_t>(eng->item_off), P<uint32_t>(eng->sp_first),
                                P<uint32_t>(eng->sp_count));
     CK(hipGetLastError());
@@ -639,7 +659,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     size_t dn4 = 4ull * (n_draws + 1);
     if (!need(eng, cap, eng->bbox, sizeof(int4) * (n_draws + 1)) || !need(eng, cap, eng->dtiles, sizeof(DrawTiles) * (n_draws + 1)) ||
         !need(eng, cap, eng->erange, sizeof(uint2) * (n_draws + 1)) || !need(eng, cap, eng->pt_count, dn4) ||
-        !need(eng, cap, eng->pt_off, dn4) || !need(eng, cap, eng->seg_alloc, 256))
+        !need(eng, cap, eng->pt_off, dn4) || !need(eng, cap, eng->counters, 256))
         return -1;
     vkh::launch_draw_bbox_init(s, P<int4>(eng->bbox), P<uint2>(eng->erange), n_draws);
     vkh::launch_draw_bbox(s, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), n_edges, P<int4>(eng->bbox),
@@ -665,12 +685,12 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     uint32_t n_segs = 0;
     for (int pass = 0; pass < 2; pass++) {
         const uint32_t seg_cap = (uint32_t)std::min<size_t>(eng->segs.cap / 4, 0xFFFFFFFFull);
-        vkh::launch_zero_u32(s, P<uint32_t>(eng->seg_alloc), 1);
+        vkh::launch_zero_u32(s, P<uint32_t>(eng->counters), 1);
         vkh::launch_bin_draws(s, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), P<uint2>(eng->erange),
                               P<DrawTiles>(eng->dtiles), n_draws, T, P<PathTile>(eng->ptiles), P<uint32_t>(eng->segs),
-                              seg_cap, P<uint32_t>(eng->seg_alloc));
+                              seg_cap, P<uint32_t>(eng->counters));
         CK(hipGetLastError());
-        if (totals(eng, cap, {P<uint32_t>(eng->seg_alloc)}, tot, 6))
+        if (totals(eng, cap, {P<uint32_t>(eng->counters)}, tot, 6))
             return -1;
         n_segs = tot[0];
         if (n_segs <= seg_cap)
@@ -687,22 +707,28 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     if (!cap)
         CK(hipEventRecord(eng->ev_stage[3], s)); // binning done
     // ---- coarse: ordered command list per tile
-    if (!need(eng, cap, eng->tgts, sizeof(TargetInfo) * (n_targets + 1)) ||
-        !need(eng, cap, eng->tile_cnt, 4ull * (tile_total + 1)) || !need(eng, cap, eng->tile_off, 4ull * (tile_total + 1)))
+    const uint32_t nch = eng->coarse_nch;
+    const uint64_t n_tc64 = (uint64_t)tile_total * nch;
+    if (n_tc64 > 0x7FFFFFFFull) {
+        eng->err = "frame too large (tiles x draw chunks)";
+        return -3;
+    }
+    const uint32_t n_tc = (uint32_t)n_tc64;
+    if (!need(eng, cap, eng->tile_cnt, 4ull * (n_tc + 1)) || !need(eng, cap, eng->tile_off, 4ull * (n_tc + 1)))
         return -1;
-    vkh::launch_zero_u32(s, P<uint32_t>(eng->tile_cnt), tile_total + 1);
-    vkh::launch_coarse(s, false, bin_total, d_tg, n_targets, d_draws, P<DrawTiles>(eng->dtiles),
+    vkh::launch_zero_u32(s, P<uint32_t>(eng->tile_cnt), n_tc + 1);
+    vkh::launch_coarse(s, false, bin_total, nch, d_tg, n_targets, d_draws, P<DrawTiles>(eng->dtiles),
                        P<PathTile>(eng->ptiles), P<uint32_t>(eng->tile_cnt), nullptr, nullptr);
-    if (scan_u32(eng, P<uint32_t>(eng->tile_cnt), tile_total, P<uint32_t>(eng->tile_off), cap))
+    if (scan_u32(eng, P<uint32_t>(eng->tile_cnt), n_tc, P<uint32_t>(eng->tile_off), cap))
         return -1;
-    if (totals(eng, cap, {P<uint32_t>(eng->tile_off) + tile_total}, tot, 7))
+    if (totals(eng, cap, {P<uint32_t>(eng->tile_off) + n_tc}, tot, 7))
         return -1;
     const uint32_t n_cmds = tot[0];
     st.commands = n_cmds;
     st.tiles = tile_total;
     if (!need(eng, cap, eng->cmds, sizeof(Cmd) * (size_t)n_cmds + 16))
         return -1;
-    vkh::launch_coarse(s, true, bin_total, d_tg, n_targets, d_draws, P<DrawTiles>(eng->dtiles),
+    vkh::launch_coarse(s, true, bin_total, nch, d_tg, n_targets, d_draws, P<DrawTiles>(eng->dtiles),
                        P<PathTile>(eng->ptiles), nullptr, P<uint32_t>(eng->tile_off), P<Cmd>(eng->cmds));
     CK(hipGetLastError());
 
@@ -712,7 +738,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     if (!cap)
         CK(hipEventRecord(eng->ev0, s));
     vkh::launch_fine(s, T, eng->fine_words, tile_total, d_tg, n_targets, P<DrawPrep>(eng->dprep), d_grads, P<PathTile>(eng->ptiles),
-                     P<uint32_t>(eng->segs), P<Edge>(eng->edges), n_edges, P<uint32_t>(eng->tile_off),
+                     P<uint32_t>(eng->segs), P<Edge>(eng->edges), n_edges, nch, P<uint32_t>(eng->tile_off),
                      P<Cmd>(eng->cmds));
     CK(hipGetLastError());
     if (cap)

@@ -11,23 +11,75 @@
 #include "vkh_internal.h"
 
 // ---------------------------------------------------------------------------------------- flatten
-__global__ void k_flatten_count(const vkh_item *__restrict__ items, uint32_t n, uint32_t *__restrict__ counts) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// Flattening in one subdivision pass: every cubic item takes a scratch slot of VKH_FLAT_SLOT
+// points (one atomic per wavefront), runs the AGG traversal once, keeps its first VKH_FLAT_SLOT
+// points there and records its point count. After the scan of the counts, k_flatten_copy moves
+// the slots to their final positions and k_flatten_emit writes the line items and runs the
+// traversal again only for the rare cubics with more points than a slot holds.
+__global__ void k_flatten_stage(const vkh_item *__restrict__ items, uint32_t n, uint32_t *__restrict__ counts,
+                                uint32_t *__restrict__ slot_alloc, uint32_t *__restrict__ slot_item,
+                                uint32_t *__restrict__ item_slot, v2 *__restrict__ scratch) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    vkh_item it;
+    bool cubic = false;
+    if (i < n) {
+        it = items[i];
+        cubic = (it.flags & VKH_ITEM_CUBIC) != 0;
+    }
+    // one slot allocation per wavefront
+    const uint64_t m = __ballot(cubic);
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    if (m) {
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+        if (lane == leader)
+            base = atomicAdd(slot_alloc, (uint32_t)__popcll(m));
+        base = __shfl(base, (int)leader);
+    }
     if (i >= n)
         return;
-    vkh_item it = items[i];
-    uint32_t c = 1;
-    if (it.flags & VKH_ITEM_CUBIC) {
-        Cubic cb{it.p[0], it.p[1], it.p[2], it.p[3], it.p[4], it.p[5], it.p[6], it.p[7]};
-        uint32_t k = 0;
-        agg_flatten(cb, it.tol, [&](v2) { k++; });
-        c = k + ((it.flags & VKH_ITEM_NO_END) ? 0u : 1u);
+    if (!cubic) {
+        counts[i] = 1;
+        return;
     }
-    counts[i] = c;
+    const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    slot_item[slot] = i;
+    item_slot[i] = slot;
+    v2 *out = scratch + (size_t)slot * VKH_FLAT_SLOT;
+    Cubic cb{it.p[0], it.p[1], it.p[2], it.p[3], it.p[4], it.p[5], it.p[6], it.p[7]};
+    uint32_t k = 0;
+    agg_flatten(cb, it.tol, [&](v2 p) {
+        if (k < VKH_FLAT_SLOT)
+            out[k] = p;
+        k++;
+    });
+    if (!(it.flags & VKH_ITEM_NO_END)) {
+        if (k < VKH_FLAT_SLOT)
+            out[k] = V(it.p[6], it.p[7]);
+        k++;
+    }
+    counts[i] = k;
 }
 
-__global__ void k_flatten_emit(const vkh_item *__restrict__ items, uint32_t n, const uint32_t *__restrict__ offs,
+// one thread per (slot, point): slot points to their final positions
+__global__ void k_flatten_copy(const vkh_item *__restrict__ items, uint32_t n_slots,
+                               const uint32_t *__restrict__ slot_item, const uint32_t *__restrict__ counts,
+                               const uint32_t *__restrict__ offs, const v2 *__restrict__ scratch,
                                v2 *__restrict__ pts, uint8_t *__restrict__ pflags) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t slot = (uint32_t)(t / VKH_FLAT_SLOT), k = (uint32_t)(t % VKH_FLAT_SLOT);
+    if (slot >= n_slots)
+        return;
+    const uint32_t i = slot_item[slot];
+    if (counts[i] > VKH_FLAT_SLOT || k >= counts[i])
+        return; // overflowing cubics are written by k_flatten_emit
+    const uint32_t o = offs[i] + k;
+    pts[o] = scratch[t];
+    pflags[o] = (items[i].flags & VKH_ITEM_CURVED) ? 1 : 0;
+}
+
+__global__ void k_flatten_emit(const vkh_item *__restrict__ items, uint32_t n, const uint32_t *__restrict__ counts,
+                               const uint32_t *__restrict__ offs, v2 *__restrict__ pts, uint8_t *__restrict__ pflags) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
@@ -39,6 +91,8 @@ __global__ void k_flatten_emit(const vkh_item *__restrict__ items, uint32_t n, c
         pflags[o] = f;
         return;
     }
+    if (counts[i] <= VKH_FLAT_SLOT)
+        return; // copied from its slot
     Cubic cb{it.p[0], it.p[1], it.p[2], it.p[3], it.p[4], it.p[5], it.p[6], it.p[7]};
     uint32_t k = o;
     agg_flatten(cb, it.tol, [&](v2 p) {
@@ -738,14 +792,23 @@ __global__ __launch_bounds__(256) void k_tri_edges(const Unit *__restrict__ unit
 
 // ---------------------------------------------------------------------------------------- launchers
 namespace vkh {
-void launch_flatten_count(hipStream_t s, const vkh_item *items, uint32_t n, uint32_t *counts) {
+void launch_flatten_stage(hipStream_t s, const vkh_item *items, uint32_t n, uint32_t *counts, uint32_t *slot_alloc,
+                          uint32_t *slot_item, uint32_t *item_slot, v2 *scratch) {
     if (n)
-        hipLaunchKernelGGL(k_flatten_count, dim3((n + 255) / 256), dim3(256), 0, s, items, n, counts);
+        hipLaunchKernelGGL(k_flatten_stage, dim3((n + 255) / 256), dim3(256), 0, s, items, n, counts, slot_alloc,
+                           slot_item, item_slot, scratch);
 }
-void launch_flatten_emit(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *offs, v2 *pts,
-                         uint8_t *pf) {
+void launch_flatten_copy(hipStream_t s, const vkh_item *items, uint32_t n_slots, const uint32_t *slot_item,
+                         const uint32_t *counts, const uint32_t *offs, const v2 *scratch, v2 *pts, uint8_t *pf) {
+    const size_t threads = (size_t)n_slots * VKH_FLAT_SLOT;
+    if (threads)
+        hipLaunchKernelGGL(k_flatten_copy, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, items, n_slots,
+                           slot_item, counts, offs, scratch, pts, pf);
+}
+void launch_flatten_emit(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *counts,
+                         const uint32_t *offs, v2 *pts, uint8_t *pf) {
     if (n)
-        hipLaunchKernelGGL(k_flatten_emit, dim3((n + 255) / 256), dim3(256), 0, s, items, n, offs, pts, pf);
+        hipLaunchKernelGGL(k_flatten_emit, dim3((n + 255) / 256), dim3(256), 0, s, items, n, counts, offs, pts, pf);
 }
 void launch_subpath_ranges(hipStream_t s, const vkh_subpath *sp, uint32_t n, const uint32_t *item_off,
                            uint32_t *first, uint32_t *count) {

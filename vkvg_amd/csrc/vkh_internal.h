@@ -7,7 +7,9 @@
 #include "vkh_device.h"
 
 #define VKH_NO_DRAW 0xFFFFFFFFu
-#define VKH_BIN_TILES 16 // a coarse bin is 16 x 16 tiles
+#define VKH_BIN_TILES 16        // a coarse bin is 16 x 16 tiles
+#define VKH_COARSE_CHUNK 1024   // draws per coarse workgroup
+#define VKH_FLAT_SLOT 64        // flattened points kept per cubic by the subdivision pass
 
 // one (draw, subpath) pair of the frame; fills and strokes are expanded per unit
 struct Unit {
@@ -70,8 +72,12 @@ struct Cmd {
 namespace vkh {
 size_t scan_temp_words(uint32_t n);
 void scan_u32(hipStream_t s, const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *tmp);
-void launch_flatten_count(hipStream_t s, const vkh_item *items, uint32_t n, uint32_t *counts);
-void launch_flatten_emit(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *offs, v2 *pts, uint8_t *pf);
+void launch_flatten_stage(hipStream_t s, const vkh_item *items, uint32_t n, uint32_t *counts, uint32_t *slot_alloc,
+                          uint32_t *slot_item, uint32_t *item_slot, v2 *scratch);
+void launch_flatten_copy(hipStream_t s, const vkh_item *items, uint32_t n_slots, const uint32_t *slot_item,
+                         const uint32_t *counts, const uint32_t *offs, const v2 *scratch, v2 *pts, uint8_t *pf);
+void launch_flatten_emit(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *counts,
+                         const uint32_t *offs, v2 *pts, uint8_t *pf);
 void launch_subpath_ranges(hipStream_t s, const vkh_subpath *sp, uint32_t n, const uint32_t *item_off,
                            uint32_t *first, uint32_t *count);
 void launch_unit_count(hipStream_t s, const Unit *units, uint32_t n_units, const vkh_draw *draws,
@@ -107,12 +113,12 @@ void launch_bin_draws(hipStream_t s, const Edge *edges, const uint32_t *edraw, c
                       uint32_t seg_cap, uint32_t *seg_alloc);
 void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n_draws, const TargetInfo *tg,
                       const vkh_gradient *grads, DrawPrep *out);
-void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, const TargetInfo *tg, uint32_t n_targets,
-                   const vkh_draw *draws, const DrawTiles *dt, const PathTile *pts, uint32_t *tile_count,
-                   const uint32_t *tile_off, Cmd *cmds);
+void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, uint32_t nch, const TargetInfo *tg,
+                   uint32_t n_targets, const vkh_draw *draws, const DrawTiles *dt, const PathTile *pts,
+                   uint32_t *tile_count, const uint32_t *tile_off, Cmd *cmds);
 void launch_fine(hipStream_t s, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,
                  uint32_t n_targets, const DrawPrep *dprep, const vkh_gradient *grads, const PathTile *pts,
-                 const uint32_t *segs, const Edge *edges, uint32_t n_edges, const uint32_t *tile_off,
+                 const uint32_t *segs, const Edge *edges, uint32_t n_edges, uint32_t nch, const uint32_t *tile_off,
                  const Cmd *cmds);
 void launch_unpremultiply(hipStream_t s, const uint8_t *src, uint8_t *dst, size_t npix);
 } // namespace vkh

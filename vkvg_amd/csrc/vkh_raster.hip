@@ -322,14 +322,16 @@ __device__ __forceinline__ uint32_t find_target_by(const TargetInfo *tg, uint32_
     return lo - 1;
 }
 
-// one workgroup per bin of 16 x 16 tiles, one thread per tile; walks the target's draws in
-// order, so each tile's command list is ordered by draw without any sort.
+// One workgroup per (bin of 16 x 16 tiles, chunk of VKH_COARSE_CHUNK draws), one thread per
+// tile; walks the chunk's draws in order. Counts are laid out tile-major (tile * nch + chunk), so
+// one scan gives every tile a contiguous command list ordered by draw, without any sort.
 __global__ __launch_bounds__(256) void k_coarse(bool write, const TargetInfo *__restrict__ tg, uint32_t n_targets,
                                                 const vkh_draw *__restrict__ draws, const DrawTiles *__restrict__ dts,
-                                                const PathTile *__restrict__ pts, uint32_t *__restrict__ tile_count,
-                                                const uint32_t *__restrict__ tile_off, Cmd *__restrict__ cmds) {
+                                                const PathTile *__restrict__ pts, uint32_t nch,
+                                                uint32_t *__restrict__ tile_count, const uint32_t *__restrict__ tile_off,
+                                                Cmd *__restrict__ cmds) {
     __shared__ uint64_t masks[4];
-    uint32_t bin = blockIdx.x;
+    uint32_t bin = blockIdx.x, chunk = blockIdx.y;
     uint32_t t = find_target_by(tg, n_targets, bin, true);
     const TargetInfo &T = tg[t];
     uint32_t lb = bin - T.bin_base;
@@ -340,12 +342,16 @@ __global__ __launch_bounds__(256) void k_coarse(bool write, const TargetInfo *__
     uint32_t gtile = T.tile_base + (uint32_t)ty * T.tilesW + (uint32_t)tx;
     int32_t bx0 = bxi * VKH_BIN_TILES, by0 = byi * VKH_BIN_TILES;
     int32_t bx1 = bx0 + VKH_BIN_TILES - 1, by1 = by0 + VKH_BIN_TILES - 1;
+    const uint32_t d_begin = T.draw_begin + chunk * VKH_COARSE_CHUNK;
+    if (d_begin >= T.draw_end)
+        return; // uniform over the workgroup
+    const uint32_t d_end = min(T.draw_end, d_begin + VKH_COARSE_CHUNK);
     uint32_t count = 0;
-    uint32_t out = (write && valid) ? tile_off[gtile] : 0;
-    for (uint32_t c0 = T.draw_begin; c0 < T.draw_end; c0 += 256) {
+    uint32_t out = (write && valid) ? tile_off[(size_t)gtile * nch + chunk] : 0;
+    for (uint32_t c0 = d_begin; c0 < d_end; c0 += 256) {
         uint32_t d = c0 + tid;
         bool hit = false;
-        if (d < T.draw_end) {
+        if (d < d_end) {
             DrawTiles b = dts[d];
             hit = b.bx0 <= b.bx1 && b.bx0 <= bx1 && b.bx1 >= bx0 && b.by0 <= by1 && b.by1 >= by0;
         }
@@ -382,7 +388,7 @@ __global__ __launch_bounds__(256) void k_coarse(bool write, const TargetInfo *__
         __syncthreads();
     }
     if (!write && valid)
-        tile_count[gtile] = count;
+        tile_count[(size_t)gtile * nch + chunk] = count;
 }
 
 // ----------------------------------------------------------------------------------- fine
@@ -539,7 +545,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
                                               const DrawPrep *__restrict__ dprep,
                                               const vkh_gradient *__restrict__ grads,
                                               const PathTile *__restrict__ pts, const uint32_t *__restrict__ segs,
-                                              const Edge *__restrict__ edges, uint32_t n_edges,
+                                              const Edge *__restrict__ edges, uint32_t n_edges, uint32_t nch,
                                               const uint32_t *__restrict__ tile_off, const Cmd *__restrict__ cmds) {
     constexpr int NPIX = T * T;
     constexpr int NT = NPIX >= 256 ? 256 : NPIX;  // threads
@@ -552,7 +558,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
     __shared__ int32_t c_back[B];
 
     uint32_t gtile = blockIdx.x;
-    uint32_t c0 = tile_off[gtile], c1 = tile_off[gtile + 1];
+    uint32_t c0 = tile_off[(size_t)gtile * nch], c1 = tile_off[(size_t)(gtile + 1) * nch];
     if (c0 == c1)
         return;
     const uint32_t tid = threadIdx.x;
@@ -809,12 +815,12 @@ void launch_bin_draws(hipStream_t s, const Edge *edges, const uint32_t *edraw, c
         hipLaunchKernelGGL(k_bin_draw, dim3(n_draws), dim3(256), 0, s, edges, edraw, erange, dt, tile, pts, segs,
                            seg_cap, seg_alloc);
 }
-void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, const TargetInfo *tg, uint32_t n_targets,
-                   const vkh_draw *draws, const DrawTiles *dt, const PathTile *pts, uint32_t *tile_count,
-                   const uint32_t *tile_off, Cmd *cmds) {
-    if (n_bins)
-        hipLaunchKernelGGL(k_coarse, dim3(n_bins), dim3(256), 0, s, write, tg, n_targets, draws, dt, pts, tile_count,
-                           tile_off, cmds);
+void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, uint32_t nch, const TargetInfo *tg,
+                   uint32_t n_targets, const vkh_draw *draws, const DrawTiles *dt, const PathTile *pts,
+                   uint32_t *tile_count, const uint32_t *tile_off, Cmd *cmds) {
+    if (n_bins && nch)
+        hipLaunchKernelGGL(k_coarse, dim3(n_bins, nch), dim3(256), 0, s, write, tg, n_targets, draws, dt, pts, nch,
+                           tile_count, tile_off, cmds);
 }
 void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n, const TargetInfo *tg,
                       const vkh_gradient *grads, DrawPrep *out) {
@@ -824,42 +830,44 @@ void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n, const Ta
 template <int T, int W>
 static void fine_launch(hipStream_t s, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
                         const DrawPrep *dprep, const vkh_gradient *grads, const PathTile *pts, const uint32_t *segs,
-                        const Edge *edges, uint32_t n_edges, const uint32_t *tile_off, const Cmd *cmds) {
+                        const Edge *edges, uint32_t n_edges, uint32_t nch, const uint32_t *tile_off,
+                        const Cmd *cmds) {
     constexpr int NT = T * T >= 256 ? 256 : T * T;
     hipLaunchKernelGGL((k_fine<T, W>), dim3(n_tiles), dim3(NT), 0, s, tg, n_targets, dprep, grads, pts, segs, edges,
-                       n_edges, tile_off, cmds);
+                       n_edges, nch, tile_off, cmds);
 }
 template <int T>
 static void fine_words(hipStream_t s, uint32_t words, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
                        const DrawPrep *dprep, const vkh_gradient *grads, const PathTile *pts, const uint32_t *segs,
-                       const Edge *edges, uint32_t n_edges, const uint32_t *tile_off, const Cmd *cmds) {
+                       const Edge *edges, uint32_t n_edges, uint32_t nch, const uint32_t *tile_off,
+                       const Cmd *cmds) {
     switch (words) {
     case 2048:
-        fine_launch<T, 2048>(s, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, tile_off, cmds);
+        fine_launch<T, 2048>(s, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, nch, tile_off, cmds);
         break;
     case 8192:
-        fine_launch<T, 8192>(s, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, tile_off, cmds);
+        fine_launch<T, 8192>(s, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, nch, tile_off, cmds);
         break;
     default:
-        fine_launch<T, 4096>(s, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, tile_off, cmds);
+        fine_launch<T, 4096>(s, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, nch, tile_off, cmds);
         break;
     }
 }
 void launch_fine(hipStream_t s, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,
                  uint32_t n_targets, const DrawPrep *dprep, const vkh_gradient *grads, const PathTile *pts,
-                 const uint32_t *segs, const Edge *edges, uint32_t n_edges, const uint32_t *tile_off,
+                 const uint32_t *segs, const Edge *edges, uint32_t n_edges, uint32_t nch, const uint32_t *tile_off,
                  const Cmd *cmds) {
     if (!n_tiles)
         return;
     switch (tile) {
     case 8:
-        fine_words<8>(s, words, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, tile_off, cmds);
+        fine_words<8>(s, words, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, nch, tile_off, cmds);
         break;
     case 32:
-        fine_words<32>(s, words, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, tile_off, cmds);
+        fine_words<32>(s, words, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, nch, tile_off, cmds);
         break;
     default:
-        fine_words<16>(s, words, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, tile_off, cmds);
+        fine_words<16>(s, words, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, nch, tile_off, cmds);
         break;
     }
 }
