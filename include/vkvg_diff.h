@@ -46,6 +46,12 @@ vkvg_public void vkvg_diff_destroy(VkvgDiffScene s);
 vkvg_public int vkvg_diff_forward(VkvgDiffScene s, const float *ctrl, const float *rgba, const float *width, float *image,
                       int keep);
 
+/* The same forward evaluated in double precision from the same float parameters, into a double
+ * image (4 * width * height): the reference for finite differences, whose perturbations are far
+ * below the spacing of float coordinates. Leaves no checkpoints. */
+vkvg_public int vkvg_diff_forward_f64(VkvgDiffScene s, const float *ctrl, const float *rgba, const float *width,
+                                      double *image);
+
 /* Gradients of the loss given grad_image = dL/dimage. Must follow a vkvg_diff_forward with
  * keep != 0 on the same parameters. Outputs are overwritten. */
 vkvg_public int vkvg_diff_backward(VkvgDiffScene s, const float *ctrl, const float *rgba, const float *width,

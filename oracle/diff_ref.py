@@ -123,16 +123,23 @@ def pieces(e, W: int, H: int):
         x0, x1 = ax + s0 * (bx - ax), ax + s1 * (bx - ax)
         xl, xr = min(x0, x1), max(x0, x1)
 
-        def s_of(x):
-            return (x - ax) / (bx - ax)
+        sL, sR = (s0, s1) if x0 <= x1 else (s1, s0)
+
+        def s_of(x):  # interior split points only, clamped to the row's range
+            return min(max((x - ax) / (bx - ax), s0), s1)
+
+        dy_row = (yhi - ylo) if by > ay else (ylo - yhi)
 
         def emit(col, xa, xb):
+            whole = True
             if bx == ax:
                 ps0, ps1 = s0, s1
             else:
-                u, v = s_of(xa), s_of(xb)
+                u = sL if xa <= xl else s_of(xa)
+                v = sR if xb >= xr else s_of(xb)
                 ps0, ps1 = min(u, v), max(u, v)
-            dy = (ps1 - ps0) * (by - ay)
+                whole = xa <= xl and xb >= xr
+            dy = dy_row if whole else (ps1 - ps0) * (by - ay)
             return (r, col, dy, 0.5 * (xa + xb) - col if col >= 0 else 0.0, ps0, ps1)
 
         if xl < 0:

@@ -55,12 +55,14 @@ def test_gradients_match_fp64_reference():
 
 
 def test_gradients_match_central_differences_on_256_parameters():
-    # the full 512x512, 1024-path scene; h = 1e-3 px (and 1e-3 on colours / widths); the loss
-    # is accumulated in float64; tolerance 1e-3 relative, floored at 1e-2 of the largest
-    # gradient checked (parameters whose gradient is ~0 compare absolutely)
+    # The full 512x512, 1024-path scene. Central differences with h = 1e-3 (px for control points
+    # and widths, absolute for colours) of the same forward evaluated in double precision: near
+    # x = 500 px neighbouring float coordinates are 3e-5 px apart, 3% of h, so differences of the
+    # float forward would measure its rounding. The denominator is the actual (float) step.
+    # Tolerance 1e-3 relative, floored at 1e-2 of the largest gradient checked.
     sc = scenes.config5()
     torch, diff, R, ctrl, rgba, width, tctrl = _setup(sc)
-    target = R.forward(tctrl, rgba, width).double()
+    target = R.forward64(tctrl, rgba, width)
     img = R.forward(ctrl, rgba, width, keep=True)
     gc, gr, gw = R.backward(ctrl, rgba, width, (2.0 * (img.double() - target)).float())
     rng = np.random.default_rng(11)
@@ -77,24 +79,35 @@ def test_gradients_match_central_differences_on_256_parameters():
     h = 1e-3
 
     def loss():
-        return float(((R.forward(ctrl, rgba, width).double() - target) ** 2).sum())
+        return float(((R.forward64(ctrl, rgba, width) - target) ** 2).sum())
 
     an, fd = [], []
     for name, t, g, i in picks:
         v = t.view(-1)
         old = float(v[i])
         v[i] = old + h
+        xp = float(v[i])
         lp = loss()
         v[i] = old - h
+        xm = float(v[i])
         lm = loss()
         v[i] = old
-        fd.append((lp - lm) / (2 * h))
+        fd.append((lp - lm) / (xp - xm))
         an.append(float(g.view(-1)[i]))
     an, fd = np.array(an), np.array(fd)
     e = _rel_err(an, fd, floor_frac=1e-2)
     print("FD: max rel err", e.max(), "median", np.median(e), "largest |g|", np.abs(fd).max(), flush=True)
     assert np.isfinite(an).all()
     assert e.max() < 1e-3
+    R.close()
+
+
+def test_fp64_forward_agrees_with_fp32_forward():
+    sc = scenes.config5(n_paths=64, size=192)
+    torch, diff, R, ctrl, rgba, width, _ = _setup(sc)
+    a = R.forward(ctrl, rgba, width).double()
+    b = R.forward64(ctrl, rgba, width)
+    assert float((a - b).abs().max()) < 1e-4
     R.close()
 
 

@@ -78,7 +78,7 @@ EXPORTED = [
     "vkvg_hip_last_error",
     # engine tuning (include/vkvg_hip.h) and the differentiable renderer (include/vkvg_diff.h)
     "vkvg_hip_autotune", "vkvg_hip_retile", "vkvg_hip_set_fine_words", "vkvg_hip_set_graph",
-    "vkvg_diff_create", "vkvg_diff_destroy", "vkvg_diff_forward", "vkvg_diff_backward", "vkvg_diff_last_ms", "vkvg_diff_stats", "vkvg_diff_last_error",
+    "vkvg_diff_create", "vkvg_diff_destroy", "vkvg_diff_forward", "vkvg_diff_forward_f64", "vkvg_diff_backward", "vkvg_diff_last_ms", "vkvg_diff_stats", "vkvg_diff_last_error",
 ]
 
 

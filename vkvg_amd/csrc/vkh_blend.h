@@ -1,0 +1,65 @@
+// UNORM8 premultiplied compositing of one pixel, shared by the HIP kernels and the CPU test that
+// checks it exhaustively (tests/test_blend.py). Operators of src/vkvg_device_internal.c:200-218:
+// OVER / SOURCE d' = s + d (1 - s.a), DIFFERENCE d' = s - d (1 - s.a) clamped at 0 (blend op
+// SUBTRACT), CLEAR (logic op) 0. d (1 - s.a) per channel is the exact rounding of d*(255-sa)/255.
+#ifndef VKH_BLEND_H
+#define VKH_BLEND_H
+
+#include <stdint.h>
+
+#ifdef __HIPCC__
+#define VKH_HD __host__ __device__ __forceinline__
+#else
+#define VKH_HD static inline
+#endif
+
+// rounding of a*b/255 for a, b <= 255
+VKH_HD uint32_t mul255(uint32_t a, uint32_t b) {
+    uint32_t t = a * b + 128u;
+    return (t + (t >> 8)) >> 8;
+}
+
+// reference form: one channel at a time
+VKH_HD uint32_t blend_px_channels(uint32_t op, uint32_t d, uint32_t s) {
+    if (op == 0)
+        return 0;
+    uint32_t ia = 255u - (s >> 24), r = 0;
+    for (int i = 0; i < 4; i++) {
+        uint32_t dc = (d >> (8 * i)) & 0xFFu, sc = (s >> (8 * i)) & 0xFFu, m = mul255(dc, ia), v;
+        if (op == 3)
+            v = sc > m ? sc - m : 0u;
+        else
+            v = (sc + m) < 255u ? sc + m : 255u;
+        r |= v << (8 * i);
+    }
+    return r;
+}
+
+// two channels at once, each in the low byte of a 16-bit lane (no lane ever carries into the
+// next: x*ia + 128 <= 65153 and t + (t >> 8) <= 65407)
+VKH_HD uint32_t mul255x2(uint32_t x, uint32_t ia) {
+    uint32_t t = x * ia + 0x00800080u;
+    return ((t + ((t >> 8) & 0x00FF00FFu)) >> 8) & 0x00FF00FFu;
+}
+
+// same result as blend_px_channels for every input
+VKH_HD uint32_t blend_px(uint32_t op, uint32_t d, uint32_t s) {
+    if (op == 0)
+        return 0;
+    const uint32_t ia = 255u - (s >> 24);
+    const uint32_t m_lo = mul255x2(d & 0x00FF00FFu, ia), m_hi = mul255x2((d >> 8) & 0x00FF00FFu, ia);
+    const uint32_t s_lo = s & 0x00FF00FFu, s_hi = (s >> 8) & 0x00FF00FFu;
+    uint32_t v_lo, v_hi;
+    if (op == 3) { // 256 + s - m keeps bit 8 exactly when s >= m
+        const uint32_t x_lo = (s_lo | 0x01000100u) - m_lo, x_hi = (s_hi | 0x01000100u) - m_hi;
+        v_lo = x_lo & (((x_lo >> 8) & 0x00010001u) * 0xFFu);
+        v_hi = x_hi & (((x_hi >> 8) & 0x00010001u) * 0xFFu);
+    } else { // s + m <= 510: bit 8 marks saturation
+        const uint32_t x_lo = s_lo + m_lo, x_hi = s_hi + m_hi;
+        v_lo = (x_lo | (((x_lo >> 8) & 0x00010001u) * 0xFFu)) & 0x00FF00FFu;
+        v_hi = (x_hi | (((x_hi >> 8) & 0x00010001u) * 0xFFu)) & 0x00FF00FFu;
+    }
+    return v_lo | (v_hi << 8);
+}
+
+#endif

@@ -146,12 +146,16 @@ __device__ __forceinline__ int agg_leaf(const Cubic &c, float tol, v2 *pts) {
 }
 
 // Depth-first traversal; calls emit(v2) for every emitted point in the reference's order.
-// Path bits: bit (level-1) of `path` is 1 when the node at `level` is a right child.
+// Path bits: bit (level-1) of `path` is 1 when the node at `level` is a right child. The parent
+// and grandparent of the current node are kept, so moving to the sibling or to the parent's
+// sibling takes one split; deeper returns recompute the node from the root. Every node is thus
+// produced by the same chain of splits as in the reference's recursion.
 template <typename Emit>
 __device__ void agg_flatten(const Cubic &root, float tol, Emit emit) {
     uint64_t path_lo = 0, path_hi = 0; // 128 bits of choices, levels 1..101
     int level = 1;                      // the root (level 0) always subdivides
-    Cubic node;
+    Cubic node, par = root, gpar = root;
+    bool gpar_ok = false; // the parent is always known; the grandparent not after a two-level move
     agg_split(root, false, node);
     while (true) {
         bool leaf;
@@ -165,15 +169,18 @@ __device__ void agg_flatten(const Cubic &root, float tol, Emit emit) {
                 emit(pts[i]);
         }
         if (!leaf) {
-            // descend to the left child
+            // descend to the left child (bit level stays 0)
+            gpar = par;
+            gpar_ok = true;
+            par = node;
             Cubic ch;
             agg_split(node, false, ch);
             node = ch;
             level++;
-            // bit (level-1) stays 0 (left)
             continue;
         }
         // backtrack: pop right children until a left child is found, then go to its sibling
+        int popped = 0;
         while (level >= 1) {
             int b = level - 1;
             bool is_right = b < 64 ? ((path_lo >> b) & 1ull) : ((path_hi >> (b - 64)) & 1ull);
@@ -184,6 +191,7 @@ __device__ void agg_flatten(const Cubic &root, float tol, Emit emit) {
             else
                 path_hi &= ~(1ull << (b - 64));
             level--;
+            popped++;
         }
         if (level < 1)
             return;
@@ -194,16 +202,30 @@ __device__ void agg_flatten(const Cubic &root, float tol, Emit emit) {
             else
                 path_hi |= (1ull << (b - 64));
         }
-        // recompute the node at `level` from the root along the path
-        Cubic cur = root;
-        for (int l = 1; l <= level; l++) {
-            int b = l - 1;
-            bool r = b < 64 ? ((path_lo >> b) & 1ull) : ((path_hi >> (b - 64)) & 1ull);
-            Cubic nx;
-            agg_split(cur, r, nx);
-            cur = nx;
+        Cubic nx;
+        if (popped == 0) { // sibling of a left child: same parent
+            agg_split(par, true, nx);
+            node = nx;
+        } else if (popped == 1 && gpar_ok) { // sibling of the parent: the grandparent's right child
+            agg_split(gpar, true, nx);
+            node = nx;
+            par = gpar;
+            gpar_ok = false;
+        } else { // recompute the node at `level` from the root, keeping its two ancestors
+            Cubic cur = root, p1 = root, p2 = root;
+            for (int l = 1; l <= level; l++) {
+                int b = l - 1;
+                bool r = b < 64 ? ((path_lo >> b) & 1ull) : ((path_hi >> (b - 64)) & 1ull);
+                p2 = p1;
+                p1 = cur;
+                agg_split(cur, r, nx);
+                cur = nx;
+            }
+            node = cur;
+            par = p1;
+            gpar = p2;
+            gpar_ok = level >= 2;
         }
-        node = cur;
     }
 }
 
@@ -296,11 +318,7 @@ __device__ __forceinline__ bool point_above(const Edge &e, int32_t px, int32_t p
 }
 
 // ---------------------------------------------------------------------------------------
-// UNORM8 compositing (premultiplied OVER in 8-bit arithmetic).
-__device__ __forceinline__ uint32_t mul255(uint32_t a, uint32_t b) {
-    uint32_t t = a * b + 128u;
-    return (t + (t >> 8)) >> 8;
-}
+// float colour channel -> UNORM8 (clamped, rounded to nearest)
 __device__ __forceinline__ uint8_t f2u8(float f) {
     if (!(f > 0.f))
         return 0;

@@ -34,6 +34,7 @@
 #define VKD_NSEG 16 // polyline segments per cubic
 #define VKD_T 16    // tile size in pixels
 #define VKD_B 8     // commands per batch
+#define VKD_BIN_MARGIN 0.01f // px added around an edge when binning (rounding safety)
 
 namespace {
 
@@ -43,11 +44,21 @@ struct DSrc { // endpoint tags of an edge: point index, normal index | (sign + 1
 
 __device__ __forceinline__ float2 f2(float x, float y) { return make_float2(x, y); }
 
+// geometry in R = float (product path) or double (the FP64 evaluation used by finite differences)
+template <typename R> struct alignas(2 * sizeof(R)) RV2 {
+    R x, y;
+};
+template <typename R> struct alignas(4 * sizeof(R)) RV4 {
+    R x, y, z, w;
+};
+template <typename R> __device__ __forceinline__ RV2<R> r2(R x, R y) { return RV2<R>{x, y}; }
+
 // --------------------------------------------------------------------------------- geometry
+template <typename R>
 __global__ void k_dflatten(uint32_t n_pts, const uint32_t *__restrict__ pt_path,
                            const uint32_t *__restrict__ path_first_pt, const uint32_t *__restrict__ path_first_ctrl,
                            const uint32_t *__restrict__ path_ncubic, const float2 *__restrict__ ctrl,
-                           float2 *__restrict__ pts) {
+                           RV2<R> *__restrict__ pts) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_pts)
         return;
@@ -56,10 +67,11 @@ __global__ void k_dflatten(uint32_t n_pts, const uint32_t *__restrict__ pt_path,
     uint32_t q = j / VKD_NSEG, i = j % VKD_NSEG;
     uint32_t n3 = 3 * path_ncubic[path];
     const float2 *c = ctrl + path_first_ctrl[path];
-    float2 c0 = c[3 * q], c1 = c[3 * q + 1], c2 = c[3 * q + 2], c3 = c[(3 * q + 3) % n3];
-    float t = (float)i / (float)VKD_NSEG, u = 1.0f - t;
-    float b0 = u * u * u, b1 = 3.0f * u * u * t, b2 = 3.0f * u * t * t, b3 = t * t * t;
-    pts[p] = f2(b0 * c0.x + b1 * c1.x + b2 * c2.x + b3 * c3.x, b0 * c0.y + b1 * c1.y + b2 * c2.y + b3 * c3.y);
+    const float2 c0 = c[3 * q], c1 = c[3 * q + 1], c2 = c[3 * q + 2], c3 = c[(3 * q + 3) % n3];
+    const R t = (R)i / (R)VKD_NSEG, u = (R)1 - t;
+    const R b0 = u * u * u, b1 = (R)3 * u * u * t, b2 = (R)3 * u * t * t, b3 = t * t * t;
+    pts[p] = r2<R>(b0 * (R)c0.x + b1 * (R)c1.x + b2 * (R)c2.x + b3 * (R)c3.x,
+                   b0 * (R)c0.y + b1 * (R)c1.y + b2 * (R)c2.y + b3 * (R)c3.y);
 }
 
 __device__ __forceinline__ uint32_t next_pt(uint32_t p, uint32_t first, uint32_t end) {
@@ -69,31 +81,34 @@ __device__ __forceinline__ uint32_t prev_pt(uint32_t p, uint32_t first, uint32_t
     return p == first ? end - 1 : p - 1;
 }
 
+template <typename R>
 __global__ void k_dnormals(uint32_t n_pts, const uint32_t *__restrict__ pt_path,
-                           const uint32_t *__restrict__ path_first_pt, const float2 *__restrict__ pts,
-                           float2 *__restrict__ nrm) {
+                           const uint32_t *__restrict__ path_first_pt, const RV2<R> *__restrict__ pts,
+                           RV2<R> *__restrict__ nrm) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_pts)
         return;
     uint32_t path = pt_path[p];
     uint32_t nx = next_pt(p, path_first_pt[path], path_first_pt[path + 1]);
-    float dx = pts[nx].x - pts[p].x, dy = pts[nx].y - pts[p].y;
-    float L = sqrtf(dx * dx + dy * dy);
-    nrm[p] = L > 0.f ? f2(-dy / L, dx / L) : f2(0.f, 0.f);
+    R dx = pts[nx].x - pts[p].x, dy = pts[nx].y - pts[p].y;
+    R L = sqrt(dx * dx + dy * dy);
+    nrm[p] = L > (R)0 ? r2<R>(-dy / L, dx / L) : r2<R>((R)0, (R)0);
 }
 
 __device__ __forceinline__ uint32_t tag(uint32_t k, int s) { return k | ((uint32_t)(s + 1) << 30); }
-__device__ __forceinline__ float2 vpos(const float2 *pts, const float2 *nrm, float h, uint32_t i, uint32_t k, int s) {
+template <typename R>
+__device__ __forceinline__ RV2<R> vpos(const RV2<R> *pts, const RV2<R> *nrm, R h, uint32_t i, uint32_t k, int s) {
     if (!s)
         return pts[i];
-    float sh = (float)s * h;
-    return f2(pts[i].x + sh * nrm[k].x, pts[i].y + sh * nrm[k].y);
+    R sh = (R)s * h;
+    return r2<R>(pts[i].x + sh * nrm[k].x, pts[i].y + sh * nrm[k].y);
 }
 
+template <typename R>
 __global__ void k_dedges(uint32_t n_pts, const uint32_t *__restrict__ pt_path,
                          const uint32_t *__restrict__ path_first_pt, const uint32_t *__restrict__ path_first_edge,
                          const uint32_t *__restrict__ path_stroke, const float *__restrict__ width,
-                         const float2 *__restrict__ pts, const float2 *__restrict__ nrm, float4 *__restrict__ edges,
+                         const RV2<R> *__restrict__ pts, const RV2<R> *__restrict__ nrm, RV4<R> *__restrict__ edges,
                          DSrc *__restrict__ src) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_pts)
@@ -104,24 +119,26 @@ __global__ void k_dedges(uint32_t n_pts, const uint32_t *__restrict__ pt_path,
     uint32_t nx = next_pt(p, first, end);
     if (!path_stroke[path]) {
         uint32_t e = path_first_edge[path] + j;
-        edges[e] = make_float4(pts[p].x, pts[p].y, pts[nx].x, pts[nx].y);
-        src[e] = DSrc{p, tag(0, 0), nx, tag(0, 0)};
+        edges[e] = RV4<R>{pts[p].x, pts[p].y, pts[nx].x, pts[nx].y};
+        if (src)
+            src[e] = DSrc{p, tag(0, 0), nx, tag(0, 0)};
         return;
     }
-    const float h = 0.5f * width[path];
+    const R h = (R)0.5 * (R)width[path];
     uint32_t e = path_first_edge[path] + 10 * j;
     uint32_t vi[4] = {p, nx, nx, p}, vk[4] = {p, p, p, p};
     int vs[4] = {1, 1, -1, -1};
     for (int a = 0; a < 4; a++) {
         int b = (a + 1) & 3;
-        float2 A = vpos(pts, nrm, h, vi[a], vk[a], vs[a]), Bv = vpos(pts, nrm, h, vi[b], vk[b], vs[b]);
-        edges[e] = make_float4(A.x, A.y, Bv.x, Bv.y);
-        src[e] = DSrc{vi[a], tag(vk[a], vs[a]), vi[b], tag(vk[b], vs[b])};
+        RV2<R> A = vpos(pts, nrm, h, vi[a], vk[a], vs[a]), Bv = vpos(pts, nrm, h, vi[b], vk[b], vs[b]);
+        edges[e] = RV4<R>{A.x, A.y, Bv.x, Bv.y};
+        if (src)
+            src[e] = DSrc{vi[a], tag(vk[a], vs[a]), vi[b], tag(vk[b], vs[b])};
         e++;
     }
     uint32_t pp = prev_pt(p, first, end);
-    float2 n0 = nrm[pp], n1 = nrm[p];
-    bool swap = n0.x * n1.y - n0.y * n1.x > 0.f;
+    RV2<R> n0 = nrm[pp], n1 = nrm[p];
+    bool swap = n0.x * n1.y - n0.y * n1.x > (R)0;
     for (int s = 1; s >= -1; s -= 2) {
         uint32_t ti[3] = {p, p, p}, tk[3] = {0, pp, p};
         int ts[3] = {0, s, s};
@@ -131,28 +148,31 @@ __global__ void k_dedges(uint32_t n_pts, const uint32_t *__restrict__ pt_path,
         }
         for (int a = 0; a < 3; a++) {
             int b = a == 2 ? 0 : a + 1;
-            float2 A = vpos(pts, nrm, h, ti[a], tk[a], ts[a]), Bv = vpos(pts, nrm, h, ti[b], tk[b], ts[b]);
-            edges[e] = make_float4(A.x, A.y, Bv.x, Bv.y);
-            src[e] = DSrc{ti[a], tag(tk[a], ts[a]), ti[b], tag(tk[b], ts[b])};
+            RV2<R> A = vpos(pts, nrm, h, ti[a], tk[a], ts[a]), Bv = vpos(pts, nrm, h, ti[b], tk[b], ts[b]);
+            edges[e] = RV4<R>{A.x, A.y, Bv.x, Bv.y};
+            if (src)
+                src[e] = DSrc{ti[a], tag(tk[a], ts[a]), ti[b], tag(tk[b], ts[b])};
             e++;
         }
     }
 }
 
 // bbox of a path's edges in 24.8 fixed point (floor / ceil), one workgroup per path
+template <typename R>
 __global__ __launch_bounds__(256) void k_dbbox(const uint32_t *__restrict__ path_first_edge,
-                                               const float4 *__restrict__ edges, int4 *__restrict__ bbox) {
+                                               const RV4<R> *__restrict__ edges, int4 *__restrict__ bbox) {
     __shared__ int32_t red[4][4];
     uint32_t d = blockIdx.x, tid = threadIdx.x;
     float x0 = 3.0e38f, y0 = 3.0e38f, x1 = -3.0e38f, y1 = -3.0e38f;
     for (uint32_t e = path_first_edge[d] + tid; e < path_first_edge[d + 1]; e += 256) {
-        float4 v = edges[e];
+        const RV4<R> v = edges[e];
         if (v.y == v.w || isnan(v.x) || isnan(v.y) || isnan(v.z) || isnan(v.w))
             continue; // horizontal edges contribute nothing
-        x0 = fminf(x0, fminf(v.x, v.z));
-        x1 = fmaxf(x1, fmaxf(v.x, v.z));
-        y0 = fminf(y0, fminf(v.y, v.w));
-        y1 = fmaxf(y1, fmaxf(v.y, v.w));
+        // widened by the binning margin so that rounding to float never loses a tile
+        x0 = fminf(x0, (float)fmin(v.x, v.z) - VKD_BIN_MARGIN);
+        x1 = fmaxf(x1, (float)fmax(v.x, v.z) + VKD_BIN_MARGIN);
+        y0 = fminf(y0, (float)fmin(v.y, v.w) - VKD_BIN_MARGIN);
+        y1 = fmaxf(y1, (float)fmax(v.y, v.w) + VKD_BIN_MARGIN);
     }
     for (int o = 32; o > 0; o >>= 1) {
         x0 = fminf(x0, __shfl_xor(x0, o));
@@ -192,8 +212,10 @@ __global__ __launch_bounds__(256) void k_dbbox(const uint32_t *__restrict__ path
 __device__ __forceinline__ int32_t tile_of(float v) { // clamped before the conversion
     return (int32_t)fminf(fmaxf(floorf(v / VKD_T), -1.0e6f), 1.0e6f);
 }
-// tile rows and first tile column of an edge; empty (r0 > r1) for edges with a NaN coordinate
-__device__ __forceinline__ void dbin_range(const float4 &v, const DrawTiles &dt, int32_t ty_lo, int32_t ty_hi,
+// tile rows and first tile column of an edge, widened by VKD_BIN_MARGIN px; empty (r0 > r1) for
+// edges with a NaN coordinate
+template <typename R>
+__device__ __forceinline__ void dbin_range(const RV4<R> &v, const DrawTiles &dt, int32_t ty_lo, int32_t ty_hi,
                                            int32_t &r0, int32_t &r1, int32_t &c0) {
     if (isnan(v.x) || isnan(v.y) || isnan(v.z) || isnan(v.w)) {
         r0 = 1;
@@ -201,7 +223,8 @@ __device__ __forceinline__ void dbin_range(const float4 &v, const DrawTiles &dt,
         c0 = 0;
         return;
     }
-    float ymin = fminf(v.y, v.w), ymax = fmaxf(v.y, v.w), xmin = fminf(v.x, v.z);
+    const float ymin = (float)fmin(v.y, v.w) - VKD_BIN_MARGIN, ymax = (float)fmax(v.y, v.w) + VKD_BIN_MARGIN;
+    const float xmin = (float)fmin(v.x, v.z) - VKD_BIN_MARGIN;
     r0 = max(tile_of(ymin), ty_lo);
     r1 = min(tile_of(ymax), ty_hi);
     c0 = max(tile_of(xmin), dt.bx0);
@@ -236,8 +259,9 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t *a, uint32_t n, uin
     return total;
 }
 
+template <typename R>
 __global__ __launch_bounds__(256) void k_dbin(const uint32_t *__restrict__ path_first_edge,
-                                              const float4 *__restrict__ edges, const DrawTiles *__restrict__ dts,
+                                              const RV4<R> *__restrict__ edges, const DrawTiles *__restrict__ dts,
                                               PathTile *__restrict__ ptl, uint32_t *__restrict__ segs, uint32_t seg_cap,
                                               uint32_t *__restrict__ seg_alloc) {
     __shared__ uint32_t s_cnt[VKD_BIN_LDS];
@@ -257,7 +281,7 @@ __global__ __launch_bounds__(256) void k_dbin(const uint32_t *__restrict__ path_
             s_cnt[k] = 0;
         __syncthreads();
         for (uint32_t e = e0 + tid; e < e1; e += 256) {
-            float4 v = edges[e];
+            const RV4<R> v = edges[e];
             if (v.y == v.w)
                 continue;
             int32_t r0, r1, c0;
@@ -286,7 +310,7 @@ __global__ __launch_bounds__(256) void k_dbin(const uint32_t *__restrict__ path_
         __syncthreads();
         if (fits) {
             for (uint32_t e = e0 + tid; e < e1; e += 256) {
-                float4 v = edges[e];
+                const RV4<R> v = edges[e];
                 if (v.y == v.w)
                     continue;
                 int32_t r0, r1, c0;
@@ -304,38 +328,48 @@ __global__ __launch_bounds__(256) void k_dbin(const uint32_t *__restrict__ path_
 // Calls f(lc, dy, fm, s0, s1) for the pieces of edge v inside pixel row [yr, yr + 1): lc = -1 for
 // the part left of the tile column X0 (merged), 0..T-1 for tile cells; dy = (s1 - s0)(by - ay)
 // signed; fm the piece's mean x inside its cell; s0 < s1 its parameter range on the edge.
-template <typename F>
-__device__ __forceinline__ void row_pieces(const float4 &v, float yr, float X0, F f) {
-    const float ax = v.x, ay = v.y, bx = v.z, by = v.w;
+template <typename R, typename F>
+__device__ __forceinline__ void row_pieces(const RV4<R> &v, R yr, R X0, F f) {
+    const R ax = v.x, ay = v.y, bx = v.z, by = v.w;
     if (ay == by)
         return;
-    const float ylo = fmaxf(fminf(ay, by), yr), yhi = fminf(fmaxf(ay, by), yr + 1.0f);
+    const R ylo = fmax(fmin(ay, by), yr), yhi = fmin(fmax(ay, by), yr + (R)1);
     if (!(ylo < yhi))
         return;
-    const float dyt = by - ay, dxt = bx - ax;
-    const float sa = (ylo - ay) / dyt, sb = (yhi - ay) / dyt;
-    const float s0 = fminf(sa, sb), s1 = fmaxf(sa, sb);
-    const float x0 = ax + s0 * dxt, x1 = ax + s1 * dxt;
-    const float xl = fminf(x0, x1), xr = fmaxf(x0, x1);
+    const R dyt = by - ay, dxt = bx - ax;
+    const R sa = (ylo - ay) / dyt, sb = (yhi - ay) / dyt;
+    const R s0 = fmin(sa, sb), s1 = fmax(sa, sb);
+    const R x0 = ax + s0 * dxt, x1 = ax + s1 * dxt;
+    const R xl = fmin(x0, x1), xr = fmax(x0, x1);
     const bool vert = bx == ax;
-    auto emit = [&](int lc, float xa, float xb) {
-        float p0 = s0, p1 = s1;
+    // parameters at the row segment's ends come from the y clip; x-derived split points are used
+    // only strictly inside (clamped), which keeps nearly vertical edges well conditioned
+    const R sL = x0 <= x1 ? s0 : s1, sR = x0 <= x1 ? s1 : s0;
+    // a piece spanning its whole row segment takes dy from the y clip itself, so the carries of a
+    // closed outline cancel exactly to the right of it
+    const R dy_row = dyt > (R)0 ? yhi - ylo : ylo - yhi;
+    auto emit = [&](int lc, R xa, R xb) {
+        R p0 = s0, p1 = s1;
+        bool whole = true;
         if (!vert) {
-            float u = (xa - ax) / dxt, w = (xb - ax) / dxt;
-            p0 = fminf(u, w);
-            p1 = fmaxf(u, w);
+            const bool at_l = xa <= xl, at_r = xb >= xr;
+            R u = at_l ? sL : fmin(fmax((xa - ax) / dxt, s0), s1);
+            R w = at_r ? sR : fmin(fmax((xb - ax) / dxt, s0), s1);
+            p0 = fmin(u, w);
+            p1 = fmax(u, w);
+            whole = at_l && at_r;
         }
-        float dy = (p1 - p0) * dyt;
-        float fm = lc >= 0 ? 0.5f * (xa + xb) - (X0 + (float)lc) : 0.f;
+        R dy = whole ? dy_row : (p1 - p0) * dyt;
+        R fm = lc >= 0 ? (R)0.5 * (xa + xb) - (X0 + (R)lc) : (R)0;
         f(lc, dy, fm, p0, p1);
     };
     if (xl < X0)
-        emit(-1, xl, fminf(xr, X0));
-    const int32_t cl = (int32_t)floorf(xl) - (int32_t)X0;
-    const int32_t cr = (xr > xl ? (int32_t)floorf(xr) : (int32_t)floorf(xl)) - (int32_t)X0;
+        emit(-1, xl, fmin(xr, X0));
+    const int32_t cl = (int32_t)floor(xl) - (int32_t)X0;
+    const int32_t cr = (xr > xl ? (int32_t)floor(xr) : (int32_t)floor(xl)) - (int32_t)X0;
     for (int32_t c = max(cl, 0); c <= min(cr, VKD_T - 1); c++) {
-        const float cx = X0 + (float)c;
-        const float xa = fmaxf(xl, cx), xb = fminf(xr, cx + 1.0f);
+        const R cx = X0 + (R)c;
+        const R xa = fmax(xl, cx), xb = fmin(xr, cx + (R)1);
         if (xb < xa || (xb == xa && xr > xl))
             continue;
         emit(c, xa, xb);
@@ -346,55 +380,57 @@ __device__ __forceinline__ void row_pieces(const float4 &v, float yr, float X0, 
 __device__ __forceinline__ float4 premul(float4 c) { return make_float4(c.x * c.w, c.y * c.w, c.z * c.w, c.w); }
 
 // coverage of batch commands: thread (j, r) owns row r of command j (deterministic order)
-__device__ __forceinline__ void batch_coverage(float (*acc)[VKD_T][VKD_T + 1], uint32_t nb, const uint32_t *c_nseg,
+template <typename R>
+__device__ __forceinline__ void batch_coverage(R (*acc)[VKD_T][VKD_T + 1], uint32_t nb, const uint32_t *c_nseg,
                                                const uint32_t *c_segoff, const uint32_t *__restrict__ segs,
-                                               const float4 *__restrict__ edges, float X0, float Y0) {
+                                               const RV4<R> *__restrict__ edges, R X0, R Y0) {
     const uint32_t tid = threadIdx.x;
     const uint32_t j = tid / VKD_T, r = tid % VKD_T;
     if (j >= nb)
         return;
-    float *a = acc[j][r];
+    R *a = acc[j][r];
     for (int c = 0; c <= VKD_T; c++)
-        a[c] = 0.f;
-    const float yr = Y0 + (float)r;
+        a[c] = (R)0;
+    const R yr = Y0 + (R)r;
     const uint32_t n = c_nseg[j], off = c_segoff[j];
     for (uint32_t s = 0; s < n; s++) {
-        const float4 v = edges[segs[off + s]];
-        row_pieces(v, yr, X0, [&](int lc, float dy, float fm, float, float) {
+        const RV4<R> v = edges[segs[off + s]];
+        row_pieces<R>(v, yr, X0, [&](int lc, R dy, R fm, R, R) {
             if (lc < 0) {
                 a[0] += dy;
             } else {
-                a[lc] += dy * (1.0f - fm);
+                a[lc] += dy * ((R)1 - fm);
                 a[lc + 1] += dy * fm;
             }
         });
     }
-    float run = 0.f;
+    R run = (R)0;
     for (int c = 0; c < VKD_T; c++) {
         run += a[c];
         a[c] = run;
     }
 }
 
+template <typename R>
 __global__ __launch_bounds__(256) void k_dfine(uint32_t W, uint32_t H, uint32_t tilesW, uint32_t nch,
                                                const uint32_t *__restrict__ tile_off, const Cmd *__restrict__ cmds,
                                                const PathTile *__restrict__ ptl, const uint32_t *__restrict__ segs,
-                                               const float4 *__restrict__ edges, const float4 *__restrict__ rgba,
-                                               float4 *__restrict__ image, float4 *__restrict__ ckpt,
+                                               const RV4<R> *__restrict__ edges, const float4 *__restrict__ rgba,
+                                               RV4<R> *__restrict__ image, float4 *__restrict__ ckpt,
                                                const uint32_t *__restrict__ batch_off) {
-    __shared__ float acc[VKD_B][VKD_T][VKD_T + 1];
+    __shared__ R acc[VKD_B][VKD_T][VKD_T + 1];
     __shared__ uint32_t c_nseg[VKD_B], c_segoff[VKD_B], c_draw[VKD_B];
     const uint32_t t = blockIdx.x, tid = threadIdx.x;
     const uint32_t c0 = tile_off[(size_t)t * nch], c1 = tile_off[(size_t)(t + 1) * nch];
     const int32_t X0 = (int32_t)(t % tilesW) * VKD_T, Y0 = (int32_t)(t / tilesW) * VKD_T;
     const int32_t px = X0 + (int32_t)(tid % VKD_T), py = Y0 + (int32_t)(tid / VKD_T);
     const bool valid = px < (int32_t)W && py < (int32_t)H;
-    float4 st = make_float4(1.f, 1.f, 1.f, 1.f);
+    RV4<R> st{(R)1, (R)1, (R)1, (R)1};
     uint32_t b = 0;
     for (uint32_t base = c0; base < c1; base += VKD_B, b++) {
         const uint32_t nb = min((uint32_t)VKD_B, c1 - base);
         if (ckpt)
-            ckpt[(size_t)(batch_off[t] + b) * 256 + tid] = st;
+            ckpt[(size_t)(batch_off[t] + b) * 256 + tid] = make_float4((float)st.x, (float)st.y, (float)st.z, (float)st.w);
         if (tid < nb) {
             Cmd cm = cmds[base + tid];
             PathTile p = ptl[cm.pt];
@@ -403,14 +439,16 @@ __global__ __launch_bounds__(256) void k_dfine(uint32_t W, uint32_t H, uint32_t 
             c_segoff[tid] = p.seg_off;
         }
         __syncthreads();
-        batch_coverage(acc, nb, c_nseg, c_segoff, segs, edges, (float)X0, (float)Y0);
+        batch_coverage<R>(acc, nb, c_nseg, c_segoff, segs, edges, (R)X0, (R)Y0);
         __syncthreads();
         for (uint32_t j = 0; j < nb; j++) {
-            const float A = acc[j][tid / VKD_T][tid % VKD_T];
-            const float k = fminf(1.0f, fabsf(A));
-            const float4 c = premul(rgba[c_draw[j]]);
-            const float ia = 1.0f - c.w * k;
-            st = make_float4(c.x * k + st.x * ia, c.y * k + st.y * ia, c.z * k + st.z * ia, c.w * k + st.w * ia);
+            const R A = acc[j][tid / VKD_T][tid % VKD_T];
+            const R k = fmin((R)1, fabs(A));
+            const float4 cf = rgba[c_draw[j]];
+            const R ca = (R)cf.w;
+            const R cx = (R)cf.x * ca, cy = (R)cf.y * ca, cz = (R)cf.z * ca;
+            const R ia = (R)1 - ca * k;
+            st = RV4<R>{cx * k + st.x * ia, cy * k + st.y * ia, cz * k + st.z * ia, ca * k + st.w * ia};
         }
         __syncthreads();
     }
@@ -444,7 +482,7 @@ __device__ __forceinline__ float wave_multi_sum32(float (&v)[32], uint32_t lane,
 __global__ __launch_bounds__(256) void k_dback(uint32_t W, uint32_t H, uint32_t tilesW, uint32_t nch,
                                                const uint32_t *__restrict__ tile_off, const Cmd *__restrict__ cmds,
                                                const PathTile *__restrict__ ptl, const uint32_t *__restrict__ segs,
-                                               const float4 *__restrict__ edges, const float4 *__restrict__ rgba,
+                                               const RV4<float> *__restrict__ edges, const float4 *__restrict__ rgba,
                                                const float4 *__restrict__ grad_img, const float4 *__restrict__ ckpt,
                                                const uint32_t *__restrict__ batch_off, float4 *__restrict__ g_edge,
                                                float *__restrict__ g_col) {
@@ -472,7 +510,7 @@ __global__ __launch_bounds__(256) void k_dback(uint32_t W, uint32_t H, uint32_t 
             c_segoff[tid] = p.seg_off;
         }
         __syncthreads();
-        batch_coverage(acc, nb, c_nseg, c_segoff, segs, edges, (float)X0, (float)Y0);
+        batch_coverage<float>(acc, nb, c_nseg, c_segoff, segs, edges, (float)X0, (float)Y0);
         __syncthreads();
         // replay the batch's compositing from its checkpoint, then walk it backwards
         float4 st[VKD_B + 1];
@@ -537,9 +575,9 @@ __global__ __launch_bounds__(256) void k_dback(uint32_t W, uint32_t H, uint32_t 
                 const uint32_t n = c_nseg[j], off = c_segoff[j];
                 for (uint32_t s = 0; s < n; s++) {
                     const uint32_t ei = segs[off + s];
-                    const float4 v = edges[ei];
+                    const RV4<float> v = edges[ei];
                     float ga = 0.f, gb = 0.f; // scalar factors of (-dy, dx)
-                    row_pieces(v, yr, (float)X0, [&](int lc, float, float, float p0, float p1) {
+                    row_pieces<float>(v, yr, (float)X0, [&](int lc, float, float, float p0, float p1) {
                         if (lc < 0)
                             return;
                         const float Gv = G[j][r][lc];
@@ -693,7 +731,7 @@ struct vkvg_diff_scene_t {
     // static topology
     Buf pt_path, ctrl_path, first_pt, first_ctrl, first_edge, ncubic, stroke, draws, tg;
     // per call
-    Buf pts, nrm, edges, src, bbox, dtiles, pt_count, pt_off, ptiles, segs, seg_alloc, tile_cnt, tile_off, cmds, nbat,
+    Buf pts, nrm, edges, pts64, nrm64, edges64, src, bbox, dtiles, pt_count, pt_off, ptiles, segs, seg_alloc, tile_cnt, tile_off, cmds, nbat,
         batch_off, ckpt, scan_tmp, host_tot;
     Buf g_edge, g_pt, g_n, g_h, g_col;
 };
@@ -840,7 +878,7 @@ void vkvg_diff_destroy(VkvgDiffScene s) {
     if (s->stream)
         (void)hipStreamSynchronize(s->stream);
     Buf *bufs[] = {&s->pt_path, &s->ctrl_path, &s->first_pt, &s->first_ctrl, &s->first_edge, &s->ncubic, &s->stroke,
-                   &s->draws, &s->tg, &s->pts, &s->nrm, &s->edges, &s->src, &s->bbox, &s->dtiles, &s->pt_count,
+                   &s->draws, &s->tg, &s->pts, &s->nrm, &s->edges, &s->pts64, &s->nrm64, &s->edges64, &s->src, &s->bbox, &s->dtiles, &s->pt_count,
                    &s->pt_off, &s->ptiles, &s->segs, &s->seg_alloc, &s->tile_cnt, &s->tile_off, &s->cmds, &s->nbat,
                    &s->batch_off, &s->ckpt, &s->scan_tmp, &s->host_tot, &s->g_edge, &s->g_pt, &s->g_n, &s->g_h,
                    &s->g_col};
@@ -855,28 +893,33 @@ void vkvg_diff_destroy(VkvgDiffScene s) {
     delete s;
 }
 
-int vkvg_diff_forward(VkvgDiffScene s, const float *ctrl, const float *rgba, const float *width, float *image,
-                      int keep) {
-    if (!s)
-        return -1;
+} // extern "C"
+
+template <typename R>
+static int forward_impl(VkvgDiffScene s, const float *ctrl, const float *rgba, const float *width, RV4<R> *image,
+                        int keep, Buf &pts, Buf &nrm, Buf &edges) {
     if (!ctrl || !rgba || !width || !image) {
         s->err = "null pointer";
         return -1;
     }
     hipStream_t st = s->stream;
     s->have_ckpt = false;
+    if (!ensure(s, pts, 2 * sizeof(R) * (size_t)s->n_pts) || !ensure(s, nrm, 2 * sizeof(R) * (size_t)s->n_pts) ||
+        !ensure(s, edges, 4 * sizeof(R) * (size_t)s->n_edges))
+        return -1;
     if (!ck(s, hipEventRecord(s->ev[0], st), "event"))
         return -1;
     const uint32_t np = s->n_pts, nd = s->n_paths;
-    hipLaunchKernelGGL(k_dflatten, dim3(blocks(np)), dim3(256), 0, st, np, P<uint32_t>(s->pt_path),
+    hipLaunchKernelGGL(k_dflatten<R>, dim3(blocks(np)), dim3(256), 0, st, np, P<uint32_t>(s->pt_path),
                        P<uint32_t>(s->first_pt), P<uint32_t>(s->first_ctrl), P<uint32_t>(s->ncubic),
-                       (const float2 *)ctrl, P<float2>(s->pts));
-    hipLaunchKernelGGL(k_dnormals, dim3(blocks(np)), dim3(256), 0, st, np, P<uint32_t>(s->pt_path),
-                       P<uint32_t>(s->first_pt), P<float2>(s->pts), P<float2>(s->nrm));
-    hipLaunchKernelGGL(k_dedges, dim3(blocks(np)), dim3(256), 0, st, np, P<uint32_t>(s->pt_path),
+                       (const float2 *)ctrl, P<RV2<R>>(pts));
+    hipLaunchKernelGGL(k_dnormals<R>, dim3(blocks(np)), dim3(256), 0, st, np, P<uint32_t>(s->pt_path),
+                       P<uint32_t>(s->first_pt), P<RV2<R>>(pts), P<RV2<R>>(nrm));
+    hipLaunchKernelGGL(k_dedges<R>, dim3(blocks(np)), dim3(256), 0, st, np, P<uint32_t>(s->pt_path),
                        P<uint32_t>(s->first_pt), P<uint32_t>(s->first_edge), P<uint32_t>(s->stroke), width,
-                       P<float2>(s->pts), P<float2>(s->nrm), P<float4>(s->edges), P<DSrc>(s->src));
-    hipLaunchKernelGGL(k_dbbox, dim3(nd), dim3(256), 0, st, P<uint32_t>(s->first_edge), P<float4>(s->edges),
+                       P<RV2<R>>(pts), P<RV2<R>>(nrm), P<RV4<R>>(edges),
+                       sizeof(R) == sizeof(float) ? P<DSrc>(s->src) : (DSrc *)nullptr);
+    hipLaunchKernelGGL(k_dbbox<R>, dim3(nd), dim3(256), 0, st, P<uint32_t>(s->first_edge), P<RV4<R>>(edges),
                        P<int4>(s->bbox));
     vkh::launch_draw_tiles(st, P<vkh_draw>(s->draws), nd, P<TargetInfo>(s->tg), P<int4>(s->bbox), VKD_T,
                            P<DrawTiles>(s->dtiles), P<uint32_t>(s->pt_count));
@@ -891,7 +934,7 @@ int vkvg_diff_forward(VkvgDiffScene s, const float *ctrl, const float *rgba, con
     for (int pass = 0; pass < 2; pass++) {
         const uint32_t cap = (uint32_t)std::min<size_t>(s->segs.cap / 4, 0xFFFFFFFFull);
         zero(s, s->seg_alloc, 1);
-        hipLaunchKernelGGL(k_dbin, dim3(nd), dim3(256), 0, st, P<uint32_t>(s->first_edge), P<float4>(s->edges),
+        hipLaunchKernelGGL(k_dbin<R>, dim3(nd), dim3(256), 0, st, P<uint32_t>(s->first_edge), P<RV4<R>>(edges),
                            P<DrawTiles>(s->dtiles), P<PathTile>(s->ptiles), P<uint32_t>(s->segs), cap,
                            P<uint32_t>(s->seg_alloc));
         if (!read_u32(s, P<uint32_t>(s->seg_alloc), &s->n_segs))
@@ -918,7 +961,7 @@ int vkvg_diff_forward(VkvgDiffScene s, const float *ctrl, const float *rgba, con
                        P<DrawTiles>(s->dtiles), P<PathTile>(s->ptiles), nullptr, P<uint32_t>(s->tile_off),
                        P<Cmd>(s->cmds));
     float4 *ck_ptr = nullptr;
-    if (keep) {
+    if (keep && sizeof(R) == sizeof(float)) {
         hipLaunchKernelGGL(k_dbatches, dim3(blocks(s->n_tiles)), dim3(256), 0, st, s->n_tiles, s->nch,
                            P<uint32_t>(s->tile_off), P<uint32_t>(s->nbat));
         if (!scan(s, P<uint32_t>(s->nbat), s->n_tiles, P<uint32_t>(s->batch_off)))
@@ -929,16 +972,31 @@ int vkvg_diff_forward(VkvgDiffScene s, const float *ctrl, const float *rgba, con
             return -1;
         ck_ptr = P<float4>(s->ckpt);
     }
-    hipLaunchKernelGGL(k_dfine, dim3(s->n_tiles), dim3(256), 0, st, s->W, s->H, s->tilesW, s->nch,
-                       P<uint32_t>(s->tile_off),
-                       P<Cmd>(s->cmds), P<PathTile>(s->ptiles), P<uint32_t>(s->segs), P<float4>(s->edges),
-                       (const float4 *)rgba, (float4 *)image, ck_ptr, P<uint32_t>(s->batch_off));
+    hipLaunchKernelGGL(k_dfine<R>, dim3(s->n_tiles), dim3(256), 0, st, s->W, s->H, s->tilesW, s->nch,
+                       P<uint32_t>(s->tile_off), P<Cmd>(s->cmds), P<PathTile>(s->ptiles), P<uint32_t>(s->segs),
+                       P<RV4<R>>(edges), (const float4 *)rgba, image, ck_ptr, P<uint32_t>(s->batch_off));
     if (!ck(s, hipGetLastError(), "forward launch") || !ck(s, hipEventRecord(s->ev[1], st), "event") ||
         !ck(s, hipStreamSynchronize(st), "forward"))
         return -1;
     (void)hipEventElapsedTime(&s->ms_fwd, s->ev[0], s->ev[1]);
-    s->have_ckpt = keep != 0;
+    s->have_ckpt = ck_ptr != nullptr;
     return 0;
+}
+
+extern "C" {
+
+int vkvg_diff_forward(VkvgDiffScene s, const float *ctrl, const float *rgba, const float *width, float *image,
+                      int keep) {
+    if (!s)
+        return -1;
+    return forward_impl<float>(s, ctrl, rgba, width, (RV4<float> *)image, keep, s->pts, s->nrm, s->edges);
+}
+
+int vkvg_diff_forward_f64(VkvgDiffScene s, const float *ctrl, const float *rgba, const float *width,
+                          double *image) {
+    if (!s)
+        return -1;
+    return forward_impl<double>(s, ctrl, rgba, width, (RV4<double> *)image, 0, s->pts64, s->nrm64, s->edges64);
 }
 
 int vkvg_diff_backward(VkvgDiffScene s, const float *ctrl, const float *rgba, const float *width,
@@ -964,7 +1022,7 @@ int vkvg_diff_backward(VkvgDiffScene s, const float *ctrl, const float *rgba, co
     zero(s, s->g_col, 4ull * s->n_paths);
     hipLaunchKernelGGL(k_dback, dim3(s->n_tiles), dim3(256), 0, st, s->W, s->H, s->tilesW, s->nch,
                        P<uint32_t>(s->tile_off),
-                       P<Cmd>(s->cmds), P<PathTile>(s->ptiles), P<uint32_t>(s->segs), P<float4>(s->edges),
+                       P<Cmd>(s->cmds), P<PathTile>(s->ptiles), P<uint32_t>(s->segs), P<RV4<float>>(s->edges),
                        (const float4 *)rgba, (const float4 *)grad_image, P<float4>(s->ckpt),
                        P<uint32_t>(s->batch_off), P<float4>(s->g_edge), P<float>(s->g_col));
     hipLaunchKernelGGL(k_dg_edges, dim3(blocks(s->n_edges)), dim3(256), 0, st, s->n_edges, P<DSrc>(s->src),

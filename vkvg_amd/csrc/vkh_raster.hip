@@ -13,6 +13,7 @@
 //   H  signed crossings of row r between columns X0 and c (edges inside the tile)
 // so only the edges touching a tile are read when rasterizing it.
 #include "vkh_internal.h"
+#include "vkh_blend.h"
 
 #include <hip/hip_runtime.h>
 
@@ -508,25 +509,6 @@ __device__ void gradient_color(const vkh_gradient &g, const DrawPrep &P, float p
         for (int i = 2; i < (int)g.count; i++)
             mix4(col, g.colors[i], smoothstepf(g.stops[i - 1], g.stops[i], grad));
     }
-}
-
-__device__ __forceinline__ uint32_t blend_px(uint32_t op, uint32_t d, uint32_t s) {
-    if (op == 0) // CLEAR: logic op clear
-        return 0;
-    uint32_t ia = 255u - (s >> 24);
-    uint32_t r = 0;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        uint32_t dc = (d >> (8 * i)) & 0xFF, sc = (s >> (8 * i)) & 0xFF;
-        uint32_t m = mul255(dc, ia);
-        uint32_t v;
-        if (op == 3) // DIFFERENCE: BLEND_OP_SUBTRACT
-            v = sc > m ? sc - m : 0;
-        else
-            v = min(sc + m, 255u);
-        r |= v << (8 * i);
-    }
-    return r;
 }
 
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }

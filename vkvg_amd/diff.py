@@ -28,6 +28,8 @@ def _bind():
     L.vkvg_diff_destroy.argtypes = [vp]
     L.vkvg_diff_forward.argtypes = [vp, vp, vp, vp, vp, i32]
     L.vkvg_diff_forward.restype = i32
+    L.vkvg_diff_forward_f64.argtypes = [vp, vp, vp, vp, vp]
+    L.vkvg_diff_forward_f64.restype = i32
     L.vkvg_diff_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
     L.vkvg_diff_backward.restype = i32
     L.vkvg_diff_last_ms.argtypes = [vp, i32]
@@ -83,6 +85,16 @@ class DiffRenderer:
         c, r, w = self._args(ctrl, rgba, width)
         torch.cuda.synchronize()
         self._check(self._L.vkvg_diff_forward(self._h, c, r, w, img.data_ptr(), 1 if keep else 0), "forward")
+        return img
+
+    def forward64(self, ctrl, rgba, width):
+        """The same forward in double precision (float parameters in, float64 image out): the
+        function finite differences are taken of."""
+        import torch
+        img = torch.empty((self.height, self.width, 4), dtype=torch.float64, device=ctrl.device)
+        c, r, w = self._args(ctrl, rgba, width)
+        torch.cuda.synchronize()
+        self._check(self._L.vkvg_diff_forward_f64(self._h, c, r, w, img.data_ptr()), "forward_f64")
         return img
 
     def backward(self, ctrl, rgba, width, grad_img):
