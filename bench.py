@@ -185,6 +185,11 @@ def main() -> int:
         print(json.dumps({"error": "this backend renders on one GPU; multi-GPU sharding is design-only "
                                    "(see DESIGN.md), run with --gpus 1"}))
         return 2
+    try:  # torch (used by the differentiable part) must bring up its device before the library does
+        import torch
+        torch.cuda.init()
+    except Exception:
+        pass
     import vkvg_amd
     from vkvg_amd import scenes
 

@@ -17,9 +17,10 @@
 //   k_dback         per tile, batches in reverse: recomputes the batch's coverage, replays the
 //                   batch's compositing from its checkpoint, walks it backwards to get dL/dA per
 //                   (command, pixel) and the colour gradients, then turns dL/dA into edge-endpoint
-//                   gradients with the boundary integral over every edge piece inside a pixel
-//   k_dg_edges, k_dg_normals, k_dg_ctrl, k_dg_paths: chain rule to points, normals, half-widths,
-//                   control points, colours and widths
+//                   gradients with the boundary integral over every edge piece inside a pixel (one
+//                   thread per edge touching the tile)
+//   k_dg_normal_sums, k_dg_points, k_dg_ctrl, k_dg_paths: chain rule to normals, half-widths,
+//                   points, control points, colours and widths, all as gathers
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -259,12 +260,20 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t *a, uint32_t n, uin
     return total;
 }
 
+// Per tile the list holds the edges touching the tile's columns first (PathTile.reserved of
+// them), then the edges entirely left of it, which only carry their rows' dy.
+template <typename R>
+__device__ __forceinline__ int32_t last_touch_col(const RV4<R> &v) {
+    return tile_of((float)fmax(v.x, v.z) + VKD_BIN_MARGIN);
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_dbin(const uint32_t *__restrict__ path_first_edge,
                                               const RV4<R> *__restrict__ edges, const DrawTiles *__restrict__ dts,
                                               PathTile *__restrict__ ptl, uint32_t *__restrict__ segs, uint32_t seg_cap,
                                               uint32_t *__restrict__ seg_alloc) {
-    __shared__ uint32_t s_cnt[VKD_BIN_LDS];
+    __shared__ uint32_t s_cnt[VKD_BIN_LDS];   // counts, then offsets, then cursors of touching edges
+    __shared__ uint32_t s_touch[VKD_BIN_LDS]; // touching counts, then cursors of left-only edges
     __shared__ uint32_t s_wave[4];
     __shared__ uint32_t s_base;
     const uint32_t d = blockIdx.x, tid = threadIdx.x;
@@ -277,8 +286,10 @@ __global__ __launch_bounds__(256) void k_dbin(const uint32_t *__restrict__ path_
     for (uint32_t rb = 0; rb < bh; rb += band_rows) {
         const uint32_t nr = min(band_rows, bh - rb), nt = nr * bw;
         const int32_t ty_lo = dt.by0 + (int32_t)rb, ty_hi = ty_lo + (int32_t)nr - 1;
-        for (uint32_t k = tid; k < nt; k += 256)
+        for (uint32_t k = tid; k < nt; k += 256) {
             s_cnt[k] = 0;
+            s_touch[k] = 0;
+        }
         __syncthreads();
         for (uint32_t e = e0 + tid; e < e1; e += 256) {
             const RV4<R> v = edges[e];
@@ -286,9 +297,14 @@ __global__ __launch_bounds__(256) void k_dbin(const uint32_t *__restrict__ path_
                 continue;
             int32_t r0, r1, c0;
             dbin_range(v, dt, ty_lo, ty_hi, r0, r1, c0);
+            const int32_t ct = last_touch_col(v);
             for (int32_t r = r0; r <= r1; r++)
-                for (int32_t c = c0; c <= dt.bx1; c++)
-                    atomicAdd(&s_cnt[(uint32_t)(r - ty_lo) * bw + (uint32_t)(c - dt.bx0)], 1u);
+                for (int32_t c = c0; c <= dt.bx1; c++) {
+                    const uint32_t k = (uint32_t)(r - ty_lo) * bw + (uint32_t)(c - dt.bx0);
+                    atomicAdd(&s_cnt[k], 1u);
+                    if (c <= ct)
+                        atomicAdd(&s_touch[k], 1u);
+                }
         }
         __syncthreads();
         const uint32_t total = block_excl_scan(s_cnt, nt, s_wave);
@@ -304,9 +320,12 @@ __global__ __launch_bounds__(256) void k_dbin(const uint32_t *__restrict__ path_
             pt.backdrop = 0;
             pt.nseg = next - off;
             pt.seg_off = base + off;
-            pt.reserved = 0;
+            pt.reserved = s_touch[k]; // edges touching the tile, listed first
             out[k] = pt;
         }
+        __syncthreads();
+        for (uint32_t k = tid; k < nt; k += 256)
+            s_touch[k] += s_cnt[k]; // cursor of the left-only part
         __syncthreads();
         if (fits) {
             for (uint32_t e = e0 + tid; e < e1; e += 256) {
@@ -315,9 +334,13 @@ __global__ __launch_bounds__(256) void k_dbin(const uint32_t *__restrict__ path_
                     continue;
                 int32_t r0, r1, c0;
                 dbin_range(v, dt, ty_lo, ty_hi, r0, r1, c0);
+                const int32_t ct = last_touch_col(v);
                 for (int32_t r = r0; r <= r1; r++)
-                    for (int32_t c = c0; c <= dt.bx1; c++)
-                        segs[base + atomicAdd(&s_cnt[(uint32_t)(r - ty_lo) * bw + (uint32_t)(c - dt.bx0)], 1u)] = e;
+                    for (int32_t c = c0; c <= dt.bx1; c++) {
+                        const uint32_t k = (uint32_t)(r - ty_lo) * bw + (uint32_t)(c - dt.bx0);
+                        const uint32_t pos = c <= ct ? atomicAdd(&s_cnt[k], 1u) : atomicAdd(&s_touch[k], 1u);
+                        segs[base + pos] = e;
+                    }
             }
         }
         __syncthreads();
@@ -382,7 +405,7 @@ __device__ __forceinline__ float4 premul(float4 c) { return make_float4(c.x * c.
 // coverage of batch commands: thread (j, r) owns row r of command j (deterministic order)
 template <typename R>
 __device__ __forceinline__ void batch_coverage(R (*acc)[VKD_T][VKD_T + 1], uint32_t nb, const uint32_t *c_nseg,
-                                               const uint32_t *c_segoff, const uint32_t *__restrict__ segs,
+                                               const uint32_t *c_ntouch, const uint32_t *c_segoff, const uint32_t *__restrict__ segs,
                                                const RV4<R> *__restrict__ edges, R X0, R Y0) {
     const uint32_t tid = threadIdx.x;
     const uint32_t j = tid / VKD_T, r = tid % VKD_T;
@@ -392,8 +415,8 @@ __device__ __forceinline__ void batch_coverage(R (*acc)[VKD_T][VKD_T + 1], uint3
     for (int c = 0; c <= VKD_T; c++)
         a[c] = (R)0;
     const R yr = Y0 + (R)r;
-    const uint32_t n = c_nseg[j], off = c_segoff[j];
-    for (uint32_t s = 0; s < n; s++) {
+    const uint32_t n = c_nseg[j], nt = c_ntouch[j], off = c_segoff[j];
+    for (uint32_t s = 0; s < nt; s++) {
         const RV4<R> v = edges[segs[off + s]];
         row_pieces<R>(v, yr, X0, [&](int lc, R dy, R fm, R, R) {
             if (lc < 0) {
@@ -403,6 +426,13 @@ __device__ __forceinline__ void batch_coverage(R (*acc)[VKD_T][VKD_T + 1], uint3
                 a[lc + 1] += dy * fm;
             }
         });
+    }
+    // edges left of the tile: the row's whole y-extent, as row_pieces gives for them
+    for (uint32_t s = nt; s < n; s++) {
+        const RV4<R> v = edges[segs[off + s]];
+        const R ylo = fmax(fmin(v.y, v.w), yr), yhi = fmin(fmax(v.y, v.w), yr + (R)1);
+        if (ylo < yhi)
+            a[0] += v.w > v.y ? yhi - ylo : ylo - yhi;
     }
     R run = (R)0;
     for (int c = 0; c < VKD_T; c++) {
@@ -419,7 +449,7 @@ __global__ __launch_bounds__(256) void k_dfine(uint32_t W, uint32_t H, uint32_t 
                                                RV4<R> *__restrict__ image, float4 *__restrict__ ckpt,
                                                const uint32_t *__restrict__ batch_off) {
     __shared__ R acc[VKD_B][VKD_T][VKD_T + 1];
-    __shared__ uint32_t c_nseg[VKD_B], c_segoff[VKD_B], c_draw[VKD_B];
+    __shared__ uint32_t c_nseg[VKD_B], c_ntouch[VKD_B], c_segoff[VKD_B], c_draw[VKD_B];
     const uint32_t t = blockIdx.x, tid = threadIdx.x;
     const uint32_t c0 = tile_off[(size_t)t * nch], c1 = tile_off[(size_t)(t + 1) * nch];
     const int32_t X0 = (int32_t)(t % tilesW) * VKD_T, Y0 = (int32_t)(t / tilesW) * VKD_T;
@@ -436,10 +466,11 @@ __global__ __launch_bounds__(256) void k_dfine(uint32_t W, uint32_t H, uint32_t 
             PathTile p = ptl[cm.pt];
             c_draw[tid] = cm.draw;
             c_nseg[tid] = p.nseg;
+            c_ntouch[tid] = p.reserved;
             c_segoff[tid] = p.seg_off;
         }
         __syncthreads();
-        batch_coverage<R>(acc, nb, c_nseg, c_segoff, segs, edges, (R)X0, (R)Y0);
+        batch_coverage<R>(acc, nb, c_nseg, c_ntouch, c_segoff, segs, edges, (R)X0, (R)Y0);
         __syncthreads();
         for (uint32_t j = 0; j < nb; j++) {
             const R A = acc[j][tid / VKD_T][tid % VKD_T];
@@ -489,7 +520,8 @@ __global__ __launch_bounds__(256) void k_dback(uint32_t W, uint32_t H, uint32_t 
     __shared__ float acc[VKD_B][VKD_T][VKD_T + 1];
     __shared__ float G[VKD_B][VKD_T][VKD_T];
     __shared__ float s_red[4][32];
-    __shared__ uint32_t c_nseg[VKD_B], c_segoff[VKD_B], c_draw[VKD_B];
+    __shared__ uint32_t c_item[VKD_B + 1];
+    __shared__ uint32_t c_nseg[VKD_B], c_ntouch[VKD_B], c_segoff[VKD_B], c_draw[VKD_B];
     const uint32_t t = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
     const uint32_t c0 = tile_off[(size_t)t * nch], c1 = tile_off[(size_t)(t + 1) * nch];
     if (c0 == c1)
@@ -507,10 +539,11 @@ __global__ __launch_bounds__(256) void k_dback(uint32_t W, uint32_t H, uint32_t 
             PathTile p = ptl[cm.pt];
             c_draw[tid] = cm.draw;
             c_nseg[tid] = p.nseg;
+            c_ntouch[tid] = p.reserved;
             c_segoff[tid] = p.seg_off;
         }
         __syncthreads();
-        batch_coverage<float>(acc, nb, c_nseg, c_segoff, segs, edges, (float)X0, (float)Y0);
+        batch_coverage<float>(acc, nb, c_nseg, c_ntouch, c_segoff, segs, edges, (float)X0, (float)Y0);
         __syncthreads();
         // replay the batch's compositing from its checkpoint, then walk it backwards
         float4 st[VKD_B + 1];
@@ -567,34 +600,50 @@ __global__ __launch_bounds__(256) void k_dback(uint32_t W, uint32_t H, uint32_t 
             if (v != 0.f)
                 atomicAdd(&g_col[4 * c_draw[tid >> 2] + (tid & 3)], v);
         }
-        // edge-endpoint gradients: thread (j, r) integrates dL/dA along the pieces in its row
-        {
-            const uint32_t j = tid / VKD_T, r = tid % VKD_T;
-            if (j < nb) {
-                const float yr = (float)(Y0 + (int32_t)r);
-                const uint32_t n = c_nseg[j], off = c_segoff[j];
-                for (uint32_t s = 0; s < n; s++) {
-                    const uint32_t ei = segs[off + s];
-                    const RV4<float> v = edges[ei];
-                    float ga = 0.f, gb = 0.f; // scalar factors of (-dy, dx)
-                    row_pieces<float>(v, yr, (float)X0, [&](int lc, float, float, float p0, float p1) {
-                        if (lc < 0)
-                            return;
-                        const float Gv = G[j][r][lc];
-                        const float I1 = 0.5f * (p1 * p1 - p0 * p0);
-                        const float I0 = (p1 - p0) - I1;
-                        ga += Gv * I0;
-                        gb += Gv * I1;
-                    });
-                    if (ga != 0.f || gb != 0.f) {
-                        const float dx = v.z - v.x, dy = v.w - v.y;
-                        float *ge = (float *)(g_edge + ei);
-                        atomicAdd(ge + 0, -dy * ga);
-                        atomicAdd(ge + 1, dx * ga);
-                        atomicAdd(ge + 2, -dy * gb);
-                        atomicAdd(ge + 3, dx * gb);
-                    }
-                }
+        // edge-endpoint gradients: one thread per touching segment of the batch integrates dL/dA
+        // along its pieces in every row of the tile it spans (one set of atomics per edge and tile)
+        if (tid == 0) {
+            uint32_t acc_n = 0;
+            for (uint32_t j = 0; j < nb; j++) {
+                c_item[j] = acc_n;
+                acc_n += c_ntouch[j];
+            }
+            c_item[nb] = acc_n;
+        }
+        __syncthreads();
+        for (uint32_t it = tid; it < c_item[nb]; it += 256) {
+            uint32_t lo = 0, hi = nb; // last j with c_item[j] <= it
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (c_item[mid] <= it)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const uint32_t j = lo;
+            const uint32_t ei = segs[c_segoff[j] + (it - c_item[j])];
+            const RV4<float> v = edges[ei];
+            const int32_t r0 = max((int32_t)floorf(fminf(v.y, v.w)) - Y0, 0);
+            const int32_t r1 = min((int32_t)floorf(fmaxf(v.y, v.w)) - Y0, VKD_T - 1);
+            float ga = 0.f, gb = 0.f; // scalar factors of (-dy, dx)
+            for (int32_t r = r0; r <= r1; r++) {
+                row_pieces<float>(v, (float)(Y0 + r), (float)X0, [&](int lc, float, float, float p0, float p1) {
+                    if (lc < 0)
+                        return;
+                    const float Gv = G[j][r][lc];
+                    const float I1 = 0.5f * (p1 * p1 - p0 * p0);
+                    const float I0 = (p1 - p0) - I1;
+                    ga += Gv * I0;
+                    gb += Gv * I1;
+                });
+            }
+            if (ga != 0.f || gb != 0.f) {
+                const float dx = v.z - v.x, dy = v.w - v.y;
+                float *ge = (float *)(g_edge + ei);
+                atomicAdd(ge + 0, -dy * ga);
+                atomicAdd(ge + 1, dx * ga);
+                atomicAdd(ge + 2, -dy * gb);
+                atomicAdd(ge + 3, dx * gb);
             }
         }
         __syncthreads();
@@ -602,59 +651,111 @@ __global__ __launch_bounds__(256) void k_dback(uint32_t W, uint32_t H, uint32_t 
 }
 
 // --------------------------------------------------------------------------- chain rule
-__global__ void k_dg_edges(uint32_t n_edges, const DSrc *__restrict__ src, const float4 *__restrict__ g_edge,
-                           const uint32_t *__restrict__ pt_path, const float *__restrict__ width,
-                           const float2 *__restrict__ nrm, float2 *__restrict__ g_pt, float2 *__restrict__ g_n,
-                           float *__restrict__ g_h) {
-    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n_edges)
-        return;
-    const float4 ge = g_edge[e];
-    if (ge.x == 0.f && ge.y == 0.f && ge.z == 0.f && ge.w == 0.f)
-        return;
-    const DSrc s = src[e];
-    const uint32_t ii[2] = {s.ia, s.ib}, kk[2] = {s.ka, s.kb};
-    const float gx[2] = {ge.x, ge.z}, gy[2] = {ge.y, ge.w};
-    for (int a = 0; a < 2; a++) {
-        atomicAdd(&g_pt[ii[a]].x, gx[a]);
-        atomicAdd(&g_pt[ii[a]].y, gy[a]);
-        const int sg = (int)(kk[a] >> 30) - 1;
-        if (sg) {
-            const uint32_t k = kk[a] & 0x3FFFFFFFu;
-            const uint32_t path = pt_path[ii[a]];
-            const float h = 0.5f * width[path];
-            atomicAdd(&g_n[k].x, (float)sg * h * gx[a]);
-            atomicAdd(&g_n[k].y, (float)sg * h * gy[a]);
-            atomicAdd(&g_h[path], (float)sg * (nrm[k].x * gx[a] + nrm[k].y * gy[a]));
-        }
-    }
+// Chain rule from edge endpoints to points, normals and half-widths, as gathers (no atomics).
+// Edge layout per path: a fill has edge j = p_j -> p_j+1; a stroke has, per point j, 10 edges
+// [10j, 10j+10): the rectangle of segment j (points j, j+1, normal j), then the two bevel triangles
+// at vertex j (point j, normals j-1 and j). So the endpoints that use point p or normal p lie in the
+// edges of points p-1 and p (point p) or p and p+1 (normal p).
+__device__ __forceinline__ void endpoint(const DSrc &s, int which, uint32_t &i, uint32_t &k, int &sg) {
+    i = which ? s.ib : s.ia;
+    const uint32_t t = which ? s.kb : s.ka;
+    k = t & 0x3FFFFFFFu;
+    sg = (int)(t >> 30) - 1;
 }
 
-// normal n = (-d.y, d.x)/|d| of the segment p -> next: g_d = ((g.y, -g.x) - d (n.g)/|d|)/|d|
-__global__ void k_dg_normals(uint32_t n_pts, const uint32_t *__restrict__ pt_path,
-                             const uint32_t *__restrict__ path_first_pt, const uint32_t *__restrict__ path_stroke,
-                             const float2 *__restrict__ pts, const float2 *__restrict__ nrm,
-                             const float2 *__restrict__ g_n, float2 *__restrict__ g_pt) {
+// per stroke point p: g_n[p] and the half-width partial sum of the endpoints using normal p
+__global__ void k_dg_normal_sums(uint32_t n_pts, const uint32_t *__restrict__ pt_path,
+                                 const uint32_t *__restrict__ path_first_pt,
+                                 const uint32_t *__restrict__ path_first_edge, const uint32_t *__restrict__ path_stroke,
+                                 const float *__restrict__ width, const DSrc *__restrict__ src,
+                                 const float4 *__restrict__ g_edge, const float2 *__restrict__ nrm,
+                                 float2 *__restrict__ g_n, float *__restrict__ g_hp) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_pts)
         return;
     const uint32_t path = pt_path[p];
-    if (!path_stroke[path])
+    if (!path_stroke[path]) {
+        g_n[p] = f2(0.f, 0.f);
+        g_hp[p] = 0.f;
         return;
-    const float2 gn = g_n[p];
-    if (gn.x == 0.f && gn.y == 0.f)
-        return;
-    const uint32_t nx = next_pt(p, path_first_pt[path], path_first_pt[path + 1]);
-    const float dx = pts[nx].x - pts[p].x, dy = pts[nx].y - pts[p].y;
+    }
+    const uint32_t first = path_first_pt[path], end = path_first_pt[path + 1];
+    const float h = 0.5f * width[path];
+    float gx = 0.f, gy = 0.f, gh = 0.f;
+    const uint32_t owners[2] = {p, next_pt(p, first, end)};
+    for (int o = 0; o < 2; o++) {
+        const uint32_t e0 = path_first_edge[path] + 10 * (owners[o] - first);
+        for (uint32_t e = e0; e < e0 + 10; e++) {
+            const DSrc sr = src[e];
+            const float4 ge = g_edge[e];
+            for (int w = 0; w < 2; w++) {
+                uint32_t i, k;
+                int sg;
+                endpoint(sr, w, i, k, sg);
+                if (sg == 0 || k != p)
+                    continue;
+                const float vx = w ? ge.z : ge.x, vy = w ? ge.w : ge.y;
+                gx += (float)sg * h * vx;
+                gy += (float)sg * h * vy;
+                gh += (float)sg * (nrm[p].x * vx + nrm[p].y * vy);
+            }
+        }
+    }
+    g_n[p] = f2(gx, gy);
+    g_hp[p] = gh;
+}
+
+// per point p: endpoint gradients using point p, plus, for strokes, the normals' dependence on
+// the segments p-1 -> p and p -> p+1: n = (-d.y, d.x)/|d|, g_d = ((g.y, -g.x) - d (n.g)/|d|)/|d|
+__device__ __forceinline__ float2 seg_grad(const float2 *pts, const float2 *nrm, const float2 *g_n, uint32_t a,
+                                           uint32_t b) {
+    const float2 gn = g_n[a];
+    const float dx = pts[b].x - pts[a].x, dy = pts[b].y - pts[a].y;
     const float L = sqrtf(dx * dx + dy * dy);
-    if (!(L > 0.f))
+    if (!(L > 0.f) || (gn.x == 0.f && gn.y == 0.f))
+        return f2(0.f, 0.f);
+    const float ng = nrm[a].x * gn.x + nrm[a].y * gn.y;
+    return f2((gn.y - dx * ng / L) / L, (-gn.x - dy * ng / L) / L);
+}
+
+__global__ void k_dg_points(uint32_t n_pts, const uint32_t *__restrict__ pt_path,
+                            const uint32_t *__restrict__ path_first_pt, const uint32_t *__restrict__ path_first_edge,
+                            const uint32_t *__restrict__ path_stroke, const DSrc *__restrict__ src,
+                            const float4 *__restrict__ g_edge, const float2 *__restrict__ pts,
+                            const float2 *__restrict__ nrm, const float2 *__restrict__ g_n,
+                            float2 *__restrict__ g_pt) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pts)
         return;
-    const float ng = nrm[p].x * gn.x + nrm[p].y * gn.y;
-    const float gdx = (gn.y - dx * ng / L) / L, gdy = (-gn.x - dy * ng / L) / L;
-    atomicAdd(&g_pt[p].x, -gdx);
-    atomicAdd(&g_pt[p].y, -gdy);
-    atomicAdd(&g_pt[nx].x, gdx);
-    atomicAdd(&g_pt[nx].y, gdy);
+    const uint32_t path = pt_path[p];
+    const uint32_t first = path_first_pt[path], end = path_first_pt[path + 1];
+    const uint32_t pp = prev_pt(p, first, end), nx = next_pt(p, first, end);
+    const bool stroke = path_stroke[path] != 0;
+    const uint32_t per = stroke ? 10u : 1u;
+    float gx = 0.f, gy = 0.f;
+    const uint32_t owners[2] = {pp, p};
+    for (int o = 0; o < 2; o++) {
+        const uint32_t e0 = path_first_edge[path] + per * (owners[o] - first);
+        for (uint32_t e = e0; e < e0 + per; e++) {
+            const DSrc sr = src[e];
+            const float4 ge = g_edge[e];
+            if (sr.ia == p) {
+                gx += ge.x;
+                gy += ge.y;
+            }
+            if (sr.ib == p) {
+                gx += ge.z;
+                gy += ge.w;
+            }
+        }
+    }
+    if (stroke) {
+        const float2 din = seg_grad(pts, nrm, g_n, pp, p); // segment pp -> p: +g_d at its end p
+        const float2 dout = seg_grad(pts, nrm, g_n, p, nx); // segment p -> nx: -g_d at its start p
+        gx += din.x - dout.x;
+        gy += din.y - dout.y;
+    }
+    g_pt[p] = f2(gx, gy);
 }
 
 // one thread per control point: gathers the Bernstein-weighted point gradients of its cubic(s)
@@ -688,15 +789,20 @@ __global__ void k_dg_ctrl(uint32_t n_ctrl, const uint32_t *__restrict__ ctrl_pat
 
 // per path: colour chain c = (r a, g a, b a, a) and width = 2 h
 __global__ void k_dg_paths(uint32_t n_paths, const float4 *__restrict__ rgba, const float *__restrict__ g_col,
-                           const float *__restrict__ g_h, const uint32_t *__restrict__ path_stroke,
-                           float4 *__restrict__ g_rgba, float *__restrict__ g_width) {
+                           const float *__restrict__ g_hp, const uint32_t *__restrict__ path_first_pt,
+                           const uint32_t *__restrict__ path_stroke, float4 *__restrict__ g_rgba,
+                           float *__restrict__ g_width) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_paths)
         return;
     const float4 c = rgba[p];
     const float g0 = g_col[4 * p], g1 = g_col[4 * p + 1], g2 = g_col[4 * p + 2], g3 = g_col[4 * p + 3];
     g_rgba[p] = make_float4(c.w * g0, c.w * g1, c.w * g2, c.x * g0 + c.y * g1 + c.z * g2 + g3);
-    g_width[p] = path_stroke[p] ? 0.5f * g_h[p] : 0.f;
+    float gh = 0.f;
+    if (path_stroke[p])
+        for (uint32_t k = path_first_pt[p]; k < path_first_pt[p + 1]; k++)
+            gh += g_hp[k];
+    g_width[p] = 0.5f * gh;
 }
 
 __global__ void k_dbatches(uint32_t n_tiles, uint32_t nch, const uint32_t *__restrict__ tile_off,
@@ -863,7 +969,7 @@ VkvgDiffScene vkvg_diff_create(uint32_t width, uint32_t height, uint32_t n_paths
          ensure(s, s->tile_off, 4ull * ((size_t)s->n_tiles * s->nch + 1)) &&
          ensure(s, s->nbat, 4ull * (s->n_tiles + 1)) && ensure(s, s->batch_off, 4ull * (s->n_tiles + 1)) &&
          ensure(s, s->g_edge, 16ull * ne) && ensure(s, s->g_pt, 8ull * np) && ensure(s, s->g_n, 8ull * np) &&
-         ensure(s, s->g_h, 4ull * n_paths) && ensure(s, s->g_col, 16ull * n_paths);
+         ensure(s, s->g_h, 4ull * np) && ensure(s, s->g_col, 16ull * n_paths);
     if (!ok) {
         fprintf(stderr, "vkvg_diff_create: %s\n", s->err.c_str());
         vkvg_diff_destroy(s);
@@ -1016,26 +1122,26 @@ int vkvg_diff_backward(VkvgDiffScene s, const float *ctrl, const float *rgba, co
     if (!ck(s, hipEventRecord(s->ev[2], st), "event"))
         return -1;
     zero(s, s->g_edge, 4ull * s->n_edges);
-    zero(s, s->g_pt, 2ull * s->n_pts);
-    zero(s, s->g_n, 2ull * s->n_pts);
-    zero(s, s->g_h, s->n_paths);
     zero(s, s->g_col, 4ull * s->n_paths);
     hipLaunchKernelGGL(k_dback, dim3(s->n_tiles), dim3(256), 0, st, s->W, s->H, s->tilesW, s->nch,
                        P<uint32_t>(s->tile_off),
                        P<Cmd>(s->cmds), P<PathTile>(s->ptiles), P<uint32_t>(s->segs), P<RV4<float>>(s->edges),
                        (const float4 *)rgba, (const float4 *)grad_image, P<float4>(s->ckpt),
                        P<uint32_t>(s->batch_off), P<float4>(s->g_edge), P<float>(s->g_col));
-    hipLaunchKernelGGL(k_dg_edges, dim3(blocks(s->n_edges)), dim3(256), 0, st, s->n_edges, P<DSrc>(s->src),
-                       P<float4>(s->g_edge), P<uint32_t>(s->pt_path), width, P<float2>(s->nrm), P<float2>(s->g_pt),
+    hipLaunchKernelGGL(k_dg_normal_sums, dim3(blocks(s->n_pts)), dim3(256), 0, st, s->n_pts,
+                       P<uint32_t>(s->pt_path), P<uint32_t>(s->first_pt), P<uint32_t>(s->first_edge),
+                       P<uint32_t>(s->stroke), width, P<DSrc>(s->src), P<float4>(s->g_edge), P<float2>(s->nrm),
                        P<float2>(s->g_n), P<float>(s->g_h));
-    hipLaunchKernelGGL(k_dg_normals, dim3(blocks(s->n_pts)), dim3(256), 0, st, s->n_pts, P<uint32_t>(s->pt_path),
-                       P<uint32_t>(s->first_pt), P<uint32_t>(s->stroke), P<float2>(s->pts), P<float2>(s->nrm),
+    hipLaunchKernelGGL(k_dg_points, dim3(blocks(s->n_pts)), dim3(256), 0, st, s->n_pts, P<uint32_t>(s->pt_path),
+                       P<uint32_t>(s->first_pt), P<uint32_t>(s->first_edge), P<uint32_t>(s->stroke),
+                       P<DSrc>(s->src), P<float4>(s->g_edge), P<float2>(s->pts), P<float2>(s->nrm),
                        P<float2>(s->g_n), P<float2>(s->g_pt));
     hipLaunchKernelGGL(k_dg_ctrl, dim3(blocks(s->n_ctrl)), dim3(256), 0, st, s->n_ctrl, P<uint32_t>(s->ctrl_path),
                        P<uint32_t>(s->first_ctrl), P<uint32_t>(s->first_pt), P<uint32_t>(s->ncubic),
                        P<float2>(s->g_pt), (float2 *)grad_ctrl);
     hipLaunchKernelGGL(k_dg_paths, dim3(blocks(s->n_paths)), dim3(256), 0, st, s->n_paths, (const float4 *)rgba,
-                       P<float>(s->g_col), P<float>(s->g_h), P<uint32_t>(s->stroke), (float4 *)grad_rgba, grad_width);
+                       P<float>(s->g_col), P<float>(s->g_h), P<uint32_t>(s->first_pt), P<uint32_t>(s->stroke),
+                       (float4 *)grad_rgba, grad_width);
     if (!ck(s, hipGetLastError(), "backward launch") || !ck(s, hipEventRecord(s->ev[3], st), "event") ||
         !ck(s, hipStreamSynchronize(st), "backward"))
         return -1;
