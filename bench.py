@@ -13,6 +13,7 @@ Prints ONE JSON line.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -49,15 +50,22 @@ def _read_traffic(name: str):
 
 
 def _timed(dev, vkvg_amd, steps):
-    """Executes the resident frame `steps` times; returns wall ms per frame and per-run stats."""
-    _sync()
-    runs = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        vkvg_amd.execute(dev)
-        runs.append(dev.stats())
-    _sync()
-    return (time.perf_counter() - t0) * 1e3 / steps, runs
+    """Executes the resident frame `steps` times; returns wall ms per frame and per-run stats.
+    Python's garbage collector is run before and held off during the timed loop (the scene's
+    command stream is a large Python container, and a collection inside the loop would be timed)."""
+    gc.collect()
+    gc.disable()
+    try:
+        _sync()
+        runs = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            vkvg_amd.execute(dev)
+            runs.append(dev.stats())
+        _sync()
+        return (time.perf_counter() - t0) * 1e3 / steps, runs
+    finally:
+        gc.enable()
 
 
 def bench_c3(args, dev, vkvg_amd, scenes):
@@ -74,6 +82,7 @@ def bench_c3(args, dev, vkvg_amd, scenes):
     st = runs[-1]
     # headline: the same frame replayed as one captured graph per step
     ms_step, dev_ms = ms_eager, float(np.mean([r["ms_device"] for r in runs]))
+    step_ms = [round(r["ms_total"], 3) for r in runs]
     if not args.no_graph:
         dev.set_graph(True)
         vkvg_amd.execute(dev)  # sizing run + capture
@@ -81,10 +90,11 @@ def bench_c3(args, dev, vkvg_amd, scenes):
             vkvg_amd.execute(dev)
         ms_step, g_runs = _timed(dev, vkvg_amd, args.steps)
         assert all(r["graph"] == 1 for r in g_runs)
+        step_ms = [round(r["ms_total"], 3) for r in g_runs]
         dev_ms = float(np.mean([r["ms_device"] for r in g_runs]))
         dev.set_graph(False)
     surf.close()
-    return sc, ms_step, ms_eager, fine_ms, dev_ms, stage, st, tune
+    return sc, ms_step, ms_eager, fine_ms, dev_ms, stage, st, tune, step_ms
 
 
 def bench_c4(args, dev, vkvg_amd, scenes):
@@ -95,13 +105,18 @@ def bench_c4(args, dev, vkvg_amd, scenes):
     dev.set_tile_size(16)
     dev.set_fine_words(4096)
     plain = 0.0
-    for s, w in zip(surfs, words):
-        s.upload(w)
-        vkvg_amd.execute(dev)  # warm
-        _sync()
-        t = time.perf_counter()
-        vkvg_amd.execute(dev)
-        plain += time.perf_counter() - t
+    gc.collect()
+    gc.disable()
+    try:
+        for s, w in zip(surfs, words):
+            s.upload(w)
+            vkvg_amd.execute(dev)  # warm
+            _sync()
+            t = time.perf_counter()
+            vkvg_amd.execute(dev)
+            plain += time.perf_counter() - t
+    finally:
+        gc.enable()
     # batched: one frame over all 256 surfaces, autotuned, replayed as one graph launch
     vkvg_amd.upload_batch(dev, surfs, words)
     tune = dev.autotune(reps=1) if not args.no_tune else None
@@ -173,7 +188,7 @@ def main() -> int:
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-paths", type=int, default=5000)
+    ap.add_argument("--cpu-paths", type=int, default=15000)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (tile 16, 4096 words)")
     ap.add_argument("--no-graph", action="store_true", help="time eager executions instead of graph replays")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -195,7 +210,7 @@ def main() -> int:
 
     dev = vkvg_amd.Device()
     dev.set_tile_size(args.tile)
-    sc, ms_step, ms_eager, fine_ms, dev_ms, stage, st, tune = bench_c3(args, dev, vkvg_amd, scenes)
+    sc, ms_step, ms_eager, fine_ms, dev_ms, stage, st, tune, step_ms = bench_c3(args, dev, vkvg_amd, scenes)
     paths_per_s = sc.n_paths / (ms_step / 1e3)
     gpix = sc.width * sc.height / (ms_step / 1e3) / 1e9
     # dominant kernel: the fine coverage + composite kernel; its algorithmic bytes per launch
@@ -227,6 +242,7 @@ def main() -> int:
         "frame_hbm_gbs": sc.algorithmic_bytes() / (ms_step / 1e3) / 1e9,
         "device_ms": dev_ms,
         "eager_ms_per_step": ms_eager,
+        "step_host_ms": step_ms,
         "graph_replay": not args.no_graph,
         "tuned": tune["best"] if tune else {"tile": st["tile_size"], "fine_words": st["fine_words"]},
         "stages_ms": stage,

@@ -127,8 +127,8 @@ struct vkh_engine {
 
     DevBuf blob, units, tgts, item_cnt, item_off, pts, pflags, sp_first, sp_count;
     DevBuf fe, nv, nt, fe_off, nv_off, nt_off, verts, tris, edges, edraw;
-    DevBuf bbox, erange, dtiles, pt_count, pt_off, ptiles, counters, segs, dprep; // counters: [0] segment allocator, [1] flatten slot allocator
-    DevBuf tile_cnt, tile_off, cmds, scan_tmp, readback, item_slot, slot_item, flat_scratch;
+    DevBuf bbox, erange, dtiles, pt_count, pt_off, ptiles, counters, segs, dprep; // counters: [0] segment allocator, [1] flatten slots, [2] overflow cubics, [64..191] flatten order
+    DevBuf tile_cnt, tile_off, cmds, scan_tmp, readback, item_slot, slot_item, flat_scratch, flat_perm, flat_ovf;
     DevBuf step_nv, step_nt, sv_off, st_off, u_toff, u_vbeg, u_vend;
     HostBuf hblob, htot;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
@@ -140,7 +140,8 @@ struct vkh_engine {
     hipGraphExec_t gexec = nullptr;
     uint64_t gkey = 0, gfail_key = 0;
     uint64_t frame_gen = 0, alloc_gen = 0;
-    uint32_t sz[8] = {};  // points, fill edges, stroke steps, stroke vertices, triangles, path tiles, segments, commands
+    uint32_t sz[9] = {};  // points, overflow cubics, fill edges, stroke steps, stroke vertices, triangles, path tiles,
+                          // segments, commands
     uint32_t fine_words = 4096;
     std::vector<TargetInfo> h_tg;
 
@@ -265,7 +266,7 @@ void vkvg_hip_engine_destroy(vkh_engine *e) {
                       &e->erange, &e->ptiles, &e->counters, &e->segs, &e->dprep, &e->tile_cnt,
                       &e->tile_off, &e->cmds, &e->scan_tmp, &e->readback, &e->step_nv, &e->step_nt,
                       &e->sv_off, &e->st_off, &e->u_toff, &e->u_vbeg, &e->u_vend, &e->item_slot,
-                      &e->slot_item, &e->flat_scratch};
+                      &e->slot_item, &e->flat_scratch, &e->flat_perm, &e->flat_ovf};
     for (DevBuf *b : bufs)
         if (b->p)
             (void)hipFree(b->p);
@@ -569,17 +570,22 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     if (!need(eng, cap, eng->item_cnt, 4 * (n_items + 1)) || !need(eng, cap, eng->item_off, 4 * (n_items + 1)) ||
         !need(eng, cap, eng->item_slot, 4 * (n_items + 1)) || !need(eng, cap, eng->slot_item, 4ull * (n_cubics + 1)) ||
         !need(eng, cap, eng->flat_scratch, 8ull * VKH_FLAT_SLOT * (n_cubics + 1)) ||
-        !need(eng, cap, eng->counters, 256))
+        !need(eng, cap, eng->flat_perm, 4 * (n_items + 1)) || !need(eng, cap, eng->flat_ovf, 4ull * (n_cubics + 1)) ||
+        !need(eng, cap, eng->counters, 1024))
         return -1;
-    vkh::launch_zero_u32(s, P<uint32_t>(eng->counters) + 1, 1);
-    vkh::launch_flatten_stage(s, d_items, n_items, P<uint32_t>(eng->item_cnt), P<uint32_t>(eng->counters) + 1,
-                              P<uint32_t>(eng->slot_item), P<uint32_t>(eng->item_slot), P<v2>(eng->flat_scratch));
+    uint32_t *ctr = P<uint32_t>(eng->counters); // [0] segments, [1] flatten slots, [2] overflow, [64..] order
+    vkh::launch_zero_u32(s, ctr + 1, 2);
+    vkh::launch_zero_u32(s, ctr + 64, 64);
+    vkh::launch_flatten_order(s, d_items, n_items, ctr + 64, ctr + 128, P<uint32_t>(eng->flat_perm));
+    vkh::launch_flatten_stage(s, d_items, n_items, P<uint32_t>(eng->flat_perm), P<uint32_t>(eng->item_cnt), ctr + 1,
+                              P<uint32_t>(eng->slot_item), P<uint32_t>(eng->item_slot), P<v2>(eng->flat_scratch),
+                              ctr + 2, P<uint32_t>(eng->flat_ovf));
     if (scan_u32(eng, P<uint32_t>(eng->item_cnt), n_items, P<uint32_t>(eng->item_off), cap))
         return -1;
     uint32_t tot[8];
-    if (totals(eng, cap, {P<uint32_t>(eng->item_off) + n_items}, tot, 0))
+    if (totals(eng, cap, {P<uint32_t>(eng->item_off) + n_items, ctr + 2}, tot, 0))
         return -1;
-    const uint32_t n_pts = tot[0];
+    const uint32_t n_pts = tot[0], n_ovf = tot[1];
     st.points = n_pts;
     if (!need(eng, cap, eng->pts, 8ull * n_pts + 8) || !need(eng, cap, eng->pflags, n_pts + 8) ||
         !need(eng, cap, eng->sp_first, 4 * (n_sp + 1)) || !need(eng, cap, eng->sp_count, 4 * (n_sp + 1)))
@@ -587,8 +593,10 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     vkh::launch_flatten_copy(s, d_items, n_cubics, P<uint32_t>(eng->slot_item), P<uint32_t>(eng->item_cnt),
                              P<uint32_t>(eng->item_off), P<v2>(eng->flat_scratch), P<v2>(eng->pts),
                              P<uint8_t>(eng->pflags));
-    vkh::launch_flatten_emit(s, d_items, n_items, P<uint32_t>(eng->item_cnt), P<uint32_t>(eng->item_off),
-                             P<v2>(eng->pts), P<uint8_t>(eng->pflags));
+    vkh::launch_flatten_lines(s, d_items, n_items, P<uint32_t>(eng->item_off), P<v2>(eng->pts),
+                              P<uint8_t>(eng->pflags));
+    vkh::launch_flatten_overflow(s, d_items, P<uint32_t>(eng->flat_ovf), n_ovf, P<uint32_t>(eng->item_off),
+                                 P<v2>(eng->pts), P<uint8_t>(eng->pflags));
     vkh::launch_subpath_ranges(s, d_sp, n_sp, P<uint32_t>(eng->item_off), P<uint32_t>(eng->sp_first),
                                P<uint32_t>(eng->sp_count));
     CK(hipGetLastError());
@@ -607,7 +615,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     if (scan_u32(eng, P<uint32_t>(eng->fe), n_units, P<uint32_t>(eng->fe_off), cap) ||
         scan_u32(eng, d_steps, n_units, d_step_off, cap))
         return -1;
-    if (totals(eng, cap, {P<uint32_t>(eng->fe_off) + n_units, d_step_off + n_units}, tot, 1))
+    if (totals(eng, cap, {P<uint32_t>(eng->fe_off) + n_units, d_step_off + n_units}, tot, 2))
         return -1;
     const uint32_t n_fill_edges = tot[0], n_steps = tot[1];
     size_t sn4 = 4ull * (n_steps + 1);
@@ -620,7 +628,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     if (scan_u32(eng, P<uint32_t>(eng->step_nv), n_steps, P<uint32_t>(eng->sv_off), cap) ||
         scan_u32(eng, P<uint32_t>(eng->step_nt), n_steps, P<uint32_t>(eng->st_off), cap))
         return -1;
-    if (totals(eng, cap, {P<uint32_t>(eng->sv_off) + n_steps, P<uint32_t>(eng->st_off) + n_steps}, tot, 3))
+    if (totals(eng, cap, {P<uint32_t>(eng->sv_off) + n_steps, P<uint32_t>(eng->st_off) + n_steps}, tot, 4))
         return -1;
     const uint32_t n_verts = tot[0], n_tris = tot[1];
     const uint64_t n_edges64 = (uint64_t)n_fill_edges + 3ull * n_tris;
@@ -671,7 +679,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     vkh::launch_draw_prep(s, d_draws, n_draws, d_tg, d_grads, P<DrawPrep>(eng->dprep));
     if (scan_u32(eng, P<uint32_t>(eng->pt_count), n_draws, P<uint32_t>(eng->pt_off), cap))
         return -1;
-    if (totals(eng, cap, {P<uint32_t>(eng->pt_off) + n_draws}, tot, 5))
+    if (totals(eng, cap, {P<uint32_t>(eng->pt_off) + n_draws}, tot, 6))
         return -1;
     const uint32_t n_ptiles = tot[0];
     st.path_tiles = n_ptiles;
@@ -690,7 +698,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
                               P<DrawTiles>(eng->dtiles), n_draws, T, P<PathTile>(eng->ptiles), P<uint32_t>(eng->segs),
                               seg_cap, P<uint32_t>(eng->counters));
         CK(hipGetLastError());
-        if (totals(eng, cap, {P<uint32_t>(eng->counters)}, tot, 6))
+        if (totals(eng, cap, {P<uint32_t>(eng->counters)}, tot, 7))
             return -1;
         n_segs = tot[0];
         if (n_segs <= seg_cap)
@@ -721,7 +729,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
                        P<PathTile>(eng->ptiles), P<uint32_t>(eng->tile_cnt), nullptr, nullptr);
     if (scan_u32(eng, P<uint32_t>(eng->tile_cnt), n_tc, P<uint32_t>(eng->tile_off), cap))
         return -1;
-    if (totals(eng, cap, {P<uint32_t>(eng->tile_off) + n_tc}, tot, 7))
+    if (totals(eng, cap, {P<uint32_t>(eng->tile_off) + n_tc}, tot, 8))
         return -1;
     const uint32_t n_cmds = tot[0];
     st.commands = n_cmds;

@@ -11,18 +11,80 @@
 #include "vkh_internal.h"
 
 // ---------------------------------------------------------------------------------------- flatten
-// Flattening in one subdivision pass: every cubic item takes a scratch slot of VKH_FLAT_SLOT
-// points (one atomic per wavefront), runs the AGG traversal once, keeps its first VKH_FLAT_SLOT
-// points there and records its point count. After the scan of the counts, k_flatten_copy moves
-// the slots to their final positions and k_flatten_emit writes the line items and runs the
-// traversal again only for the rare cubics with more points than a slot holds.
-__global__ void k_flatten_stage(const vkh_item *__restrict__ items, uint32_t n, uint32_t *__restrict__ counts,
-                                uint32_t *__restrict__ slot_alloc, uint32_t *__restrict__ slot_item,
-                                uint32_t *__restrict__ item_slot, v2 *__restrict__ scratch) {
+// Flattening in one subdivision pass. Cubics are processed in descending order of an estimate
+// of their subdivision work (64 buckets of log2(1 + |second differences| / tolerance), counting
+// sort), so the lanes of a wavefront carry similar work. Every cubic item takes a scratch slot of
+// VKH_FLAT_SLOT points (one atomic per wavefront), runs the AGG traversal once, keeps its first
+// VKH_FLAT_SLOT points there and records its point count; the rare cubics with more points are
+// listed. After the scan of the counts, k_flatten_copy moves the slots to their final positions,
+// k_flatten_overflow runs the listed cubics again into theirs, and k_flatten_lines writes the
+// explicit points.
+__device__ __forceinline__ uint32_t flat_key(const vkh_item &it) {
+    if (!(it.flags & VKH_ITEM_CUBIC))
+        return 63;
+    const float ax = fabsf(it.p[0] - 2.f * it.p[2] + it.p[4]), ay = fabsf(it.p[1] - 2.f * it.p[3] + it.p[5]);
+    const float bx = fabsf(it.p[2] - 2.f * it.p[4] + it.p[6]), by = fabsf(it.p[3] - 2.f * it.p[5] + it.p[7]);
+    const float d = fmaxf(fmaxf(ax, ay), fmaxf(bx, by)) / fmaxf(it.tol, 1e-12f);
+    const float b = 4.f * log2f(1.f + fminf(d, 1e18f)); // NaN -> fminf gives 1e18
+    return 62u - (uint32_t)fminf(fmaxf(b, 0.f), 62.f);  // heavy cubics first
+}
+
+__global__ __launch_bounds__(256) void k_flat_hist(const vkh_item *__restrict__ items, uint32_t n,
+                                                   uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[64];
+    if (threadIdx.x < 64)
+        h[threadIdx.x] = 0;
+    __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        atomicAdd(&h[flat_key(items[i])], 1u);
+    __syncthreads();
+    if (threadIdx.x < 64 && h[threadIdx.x])
+        atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// exclusive scan of the 64 bucket counts into cursors (one workgroup of 64)
+__global__ void k_flat_cursors(const uint32_t *__restrict__ hist, uint32_t *__restrict__ cursor) {
+    const uint32_t lane = threadIdx.x;
+    uint32_t v = hist[lane], x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if ((int)lane >= o)
+            x += y;
+    }
+    cursor[lane] = x - v;
+}
+
+__global__ __launch_bounds__(256) void k_flat_scatter(const vkh_item *__restrict__ items, uint32_t n,
+                                                      uint32_t *__restrict__ cursor, uint32_t *__restrict__ perm) {
+    __shared__ uint32_t h[64], base[64];
+    if (threadIdx.x < 64)
+        h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t key = 0, rank = 0;
+    if (i < n) {
+        key = flat_key(items[i]);
+        rank = atomicAdd(&h[key], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64 && h[threadIdx.x])
+        base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], h[threadIdx.x]);
+    __syncthreads();
+    if (i < n)
+        perm[base[key] + rank] = i;
+}
+
+__global__ void k_flatten_stage(const vkh_item *__restrict__ items, uint32_t n, const uint32_t *__restrict__ perm,
+                                uint32_t *__restrict__ counts, uint32_t *__restrict__ slot_alloc,
+                                uint32_t *__restrict__ slot_item, uint32_t *__restrict__ item_slot,
+                                v2 *__restrict__ scratch, uint32_t *__restrict__ ovf_count,
+                                uint32_t *__restrict__ ovf_list) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = t < n ? perm[t] : 0u;
     vkh_item it;
     bool cubic = false;
-    if (i < n) {
+    if (t < n) {
         it = items[i];
         cubic = (it.flags & VKH_ITEM_CUBIC) != 0;
     }
@@ -36,7 +98,7 @@ __global__ void k_flatten_stage(const vkh_item *__restrict__ items, uint32_t n, 
             base = atomicAdd(slot_alloc, (uint32_t)__popcll(m));
         base = __shfl(base, (int)leader);
     }
-    if (i >= n)
+    if (t >= n)
         return;
     if (!cubic) {
         counts[i] = 1;
@@ -59,42 +121,57 @@ __global__ void k_flatten_stage(const vkh_item *__restrict__ items, uint32_t n, 
         k++;
     }
     counts[i] = k;
+    if (k > VKH_FLAT_SLOT)
+        ovf_list[atomicAdd(ovf_count, 1u)] = i;
 }
 
-// one thread per (slot, point): slot points to their final positions
+// one wavefront per slot: its points to their final positions
 __global__ void k_flatten_copy(const vkh_item *__restrict__ items, uint32_t n_slots,
                                const uint32_t *__restrict__ slot_item, const uint32_t *__restrict__ counts,
                                const uint32_t *__restrict__ offs, const v2 *__restrict__ scratch,
                                v2 *__restrict__ pts, uint8_t *__restrict__ pflags) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t slot = (uint32_t)(t / VKH_FLAT_SLOT), k = (uint32_t)(t % VKH_FLAT_SLOT);
+    const uint32_t slot = (uint32_t)(t >> 6), lane = (uint32_t)(t & 63);
     if (slot >= n_slots)
         return;
     const uint32_t i = slot_item[slot];
-    if (counts[i] > VKH_FLAT_SLOT || k >= counts[i])
-        return; // overflowing cubics are written by k_flatten_emit
-    const uint32_t o = offs[i] + k;
-    pts[o] = scratch[t];
-    pflags[o] = (items[i].flags & VKH_ITEM_CURVED) ? 1 : 0;
+    const uint32_t n = counts[i];
+    if (n > VKH_FLAT_SLOT)
+        return; // overflowing cubics are written by k_flatten_overflow
+    const uint32_t o = offs[i];
+    const uint8_t f = (items[i].flags & VKH_ITEM_CURVED) ? 1 : 0;
+    const v2 *src = scratch + (size_t)slot * VKH_FLAT_SLOT;
+    for (uint32_t k = lane; k < n; k += 64) {
+        pts[o + k] = src[k];
+        pflags[o + k] = f;
+    }
 }
 
-__global__ void k_flatten_emit(const vkh_item *__restrict__ items, uint32_t n, const uint32_t *__restrict__ counts,
-                               const uint32_t *__restrict__ offs, v2 *__restrict__ pts, uint8_t *__restrict__ pflags) {
+__global__ void k_flatten_lines(const vkh_item *__restrict__ items, uint32_t n, const uint32_t *__restrict__ offs,
+                                v2 *__restrict__ pts, uint8_t *__restrict__ pflags) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    vkh_item it = items[i];
-    uint32_t o = offs[i];
-    uint8_t f = (it.flags & VKH_ITEM_CURVED) ? 1 : 0;
-    if (!(it.flags & VKH_ITEM_CUBIC)) {
-        pts[o] = V(it.p[0], it.p[1]);
-        pflags[o] = f;
+    const vkh_item it = items[i];
+    if (it.flags & VKH_ITEM_CUBIC)
         return;
-    }
-    if (counts[i] <= VKH_FLAT_SLOT)
-        return; // copied from its slot
+    const uint32_t o = offs[i];
+    pts[o] = V(it.p[0], it.p[1]);
+    pflags[o] = (it.flags & VKH_ITEM_CURVED) ? 1 : 0;
+}
+
+// the listed cubics with more points than a slot holds, traversed again into their positions
+__global__ void k_flatten_overflow(const vkh_item *__restrict__ items, const uint32_t *__restrict__ list,
+                                   uint32_t n_list, const uint32_t *__restrict__ offs, v2 *__restrict__ pts,
+                                   uint8_t *__restrict__ pflags) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_list)
+        return;
+    const uint32_t i = list[t];
+    const vkh_item it = items[i];
+    const uint8_t f = (it.flags & VKH_ITEM_CURVED) ? 1 : 0;
     Cubic cb{it.p[0], it.p[1], it.p[2], it.p[3], it.p[4], it.p[5], it.p[6], it.p[7]};
-    uint32_t k = o;
+    uint32_t k = offs[i];
     agg_flatten(cb, it.tol, [&](v2 p) {
         pts[k] = p;
         pflags[k] = f;
@@ -792,23 +869,38 @@ __global__ __launch_bounds__(256) void k_tri_edges(const Unit *__restrict__ unit
 
 // ---------------------------------------------------------------------------------------- launchers
 namespace vkh {
-void launch_flatten_stage(hipStream_t s, const vkh_item *items, uint32_t n, uint32_t *counts, uint32_t *slot_alloc,
-                          uint32_t *slot_item, uint32_t *item_slot, v2 *scratch) {
+void launch_flatten_order(hipStream_t s, const vkh_item *items, uint32_t n, uint32_t *hist, uint32_t *cursor,
+                          uint32_t *perm) {
+    if (!n)
+        return;
+    hipLaunchKernelGGL(k_flat_hist, dim3((n + 255) / 256), dim3(256), 0, s, items, n, hist);
+    hipLaunchKernelGGL(k_flat_cursors, dim3(1), dim3(64), 0, s, hist, cursor);
+    hipLaunchKernelGGL(k_flat_scatter, dim3((n + 255) / 256), dim3(256), 0, s, items, n, cursor, perm);
+}
+void launch_flatten_stage(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *perm, uint32_t *counts,
+                          uint32_t *slot_alloc, uint32_t *slot_item, uint32_t *item_slot, v2 *scratch,
+                          uint32_t *ovf_count, uint32_t *ovf_list) {
     if (n)
-        hipLaunchKernelGGL(k_flatten_stage, dim3((n + 255) / 256), dim3(256), 0, s, items, n, counts, slot_alloc,
-                           slot_item, item_slot, scratch);
+        hipLaunchKernelGGL(k_flatten_stage, dim3((n + 255) / 256), dim3(256), 0, s, items, n, perm, counts, slot_alloc,
+                           slot_item, item_slot, scratch, ovf_count, ovf_list);
 }
 void launch_flatten_copy(hipStream_t s, const vkh_item *items, uint32_t n_slots, const uint32_t *slot_item,
                          const uint32_t *counts, const uint32_t *offs, const v2 *scratch, v2 *pts, uint8_t *pf) {
-    const size_t threads = (size_t)n_slots * VKH_FLAT_SLOT;
+    const size_t threads = (size_t)n_slots * 64;
     if (threads)
         hipLaunchKernelGGL(k_flatten_copy, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, items, n_slots,
                            slot_item, counts, offs, scratch, pts, pf);
 }
-void launch_flatten_emit(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *counts,
-                         const uint32_t *offs, v2 *pts, uint8_t *pf) {
+void launch_flatten_lines(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *offs, v2 *pts,
+                          uint8_t *pf) {
     if (n)
-        hipLaunchKernelGGL(k_flatten_emit, dim3((n + 255) / 256), dim3(256), 0, s, items, n, counts, offs, pts, pf);
+        hipLaunchKernelGGL(k_flatten_lines, dim3((n + 255) / 256), dim3(256), 0, s, items, n, offs, pts, pf);
+}
+void launch_flatten_overflow(hipStream_t s, const vkh_item *items, const uint32_t *list, uint32_t n_list,
+                             const uint32_t *offs, v2 *pts, uint8_t *pf) {
+    if (n_list)
+        hipLaunchKernelGGL(k_flatten_overflow, dim3((n_list + 255) / 256), dim3(256), 0, s, items, list, n_list, offs,
+                           pts, pf);
 }
 void launch_subpath_ranges(hipStream_t s, const vkh_subpath *sp, uint32_t n, const uint32_t *item_off,
                            uint32_t *first, uint32_t *count) {

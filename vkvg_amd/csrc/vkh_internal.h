@@ -9,7 +9,7 @@
 #define VKH_NO_DRAW 0xFFFFFFFFu
 #define VKH_BIN_TILES 16        // a coarse bin is 16 x 16 tiles
 #define VKH_COARSE_CHUNK 1024   // draws per coarse workgroup
-#define VKH_FLAT_SLOT 64        // flattened points kept per cubic by the subdivision pass
+#define VKH_FLAT_SLOT 128       // flattened points kept per cubic by the subdivision pass
 
 // one (draw, subpath) pair of the frame; fills and strokes are expanded per unit
 struct Unit {
@@ -72,12 +72,17 @@ struct Cmd {
 namespace vkh {
 size_t scan_temp_words(uint32_t n);
 void scan_u32(hipStream_t s, const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *tmp);
-void launch_flatten_stage(hipStream_t s, const vkh_item *items, uint32_t n, uint32_t *counts, uint32_t *slot_alloc,
-                          uint32_t *slot_item, uint32_t *item_slot, v2 *scratch);
+void launch_flatten_order(hipStream_t s, const vkh_item *items, uint32_t n, uint32_t *hist, uint32_t *cursor,
+                          uint32_t *perm);
+void launch_flatten_stage(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *perm, uint32_t *counts,
+                          uint32_t *slot_alloc, uint32_t *slot_item, uint32_t *item_slot, v2 *scratch,
+                          uint32_t *ovf_count, uint32_t *ovf_list);
 void launch_flatten_copy(hipStream_t s, const vkh_item *items, uint32_t n_slots, const uint32_t *slot_item,
                          const uint32_t *counts, const uint32_t *offs, const v2 *scratch, v2 *pts, uint8_t *pf);
-void launch_flatten_emit(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *counts,
-                         const uint32_t *offs, v2 *pts, uint8_t *pf);
+void launch_flatten_lines(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *offs, v2 *pts,
+                          uint8_t *pf);
+void launch_flatten_overflow(hipStream_t s, const vkh_item *items, const uint32_t *list, uint32_t n_list,
+                             const uint32_t *offs, v2 *pts, uint8_t *pf);
 void launch_subpath_ranges(hipStream_t s, const vkh_subpath *sp, uint32_t n, const uint32_t *item_off,
                            uint32_t *first, uint32_t *count);
 void launch_unit_count(hipStream_t s, const Unit *units, uint32_t n_units, const vkh_draw *draws,
