@@ -199,3 +199,15 @@ def bench(steps: int = 10, warmup: int = 2, variant: int = 0) -> dict:
             "device_forward_ms": float(np.mean(dev_f)), "device_fwd_bwd_ms": float(np.mean(dev_fb)),
             "device_ratio": float(np.mean(dev_fb) / np.mean(dev_f)), "paths": sc.n_paths,
             "surface": f"{sc.width}x{sc.height} FP32 RGBA", **st}
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+    try:  # torch brings up its device before the library does
+        import torch
+        torch.cuda.init()
+    except Exception:
+        pass
+    steps = int(sys.argv[1]) if len(sys.argv) > 1 else 10
+    print(json.dumps(bench(steps=steps, warmup=1)))
