@@ -1,9 +1,13 @@
-"""The packed UNORM8 blend used by the fine kernel equals the per-channel form for every
-(destination, source, operator): compiled from the same header with gcc, checked exhaustively
-over source alpha x destination channel x source channel."""
+"""The packed UNORM8 blends used by the fine kernel (two channels per 16-bit lane, in 32-bit
+words and in 16-bit vector lanes) equal the per-channel form for every (destination, source,
+operator): compiled from the same header with gcc and with the ROCm clang (the device compiler,
+which takes the clang builtins of the 16-bit form), checked exhaustively over source alpha x
+destination channel x source channel."""
 import os
 import subprocess
 import tempfile
+
+import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -20,6 +24,8 @@ int main(void) {
                     uint32_t s = sc | (((sc * 5 + 1) & 255) << 8) | (((sc ^ 0x5A) & 255) << 16) | (sa << 24);
                     bad += blend_px(op, d, s) != blend_px_channels(op, d, s);
                     bad += blend_px(op, s, d) != blend_px_channels(op, s, d);
+                    bad += blend_px16(op, d, vkh_src16_of(s)) != blend_px_channels(op, d, s);
+                    bad += blend_px16(op, s, vkh_src16_of(d)) != blend_px_channels(op, s, d);
                 }
     printf("%llu\n", bad);
     return 0;
@@ -27,12 +33,20 @@ int main(void) {
 """
 
 
-def test_packed_blend_equals_per_channel_blend():
+def _compilers():
+    out = ["gcc"]
+    if os.path.exists("/opt/rocm/llvm/bin/clang"):
+        out.append("/opt/rocm/llvm/bin/clang")
+    return out
+
+
+@pytest.mark.parametrize("cc", _compilers(), ids=os.path.basename)
+def test_packed_blend_equals_per_channel_blend(cc):
     with tempfile.TemporaryDirectory() as t:
         src = os.path.join(t, "b.c")
         exe = os.path.join(t, "b")
         open(src, "w").write(PROG)
-        subprocess.run(["gcc", "-O2", "-std=c11", "-I", os.path.join(ROOT, "vkvg_amd", "csrc"), "-o", exe, src],
+        subprocess.run([cc, "-O2", "-std=c11", "-I", os.path.join(ROOT, "vkvg_amd", "csrc"), "-o", exe, src],
                        check=True)
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.strip()
     assert out == "0"

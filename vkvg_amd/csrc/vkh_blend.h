@@ -62,4 +62,62 @@ VKH_HD uint32_t blend_px(uint32_t op, uint32_t d, uint32_t s) {
     return v_lo | (v_hi << 8);
 }
 
+// The same blend on 16-bit lanes (GCC vector extension, understood by gcc and clang): lanes hold
+// channels (0, 2) and (1, 3); on gfx9 each operation is one packed instruction (v_pk_mad_u16,
+// v_pk_lshrrev_b16, v_pk_add_u16, v_pk_min_u16, v_pk_sub_u16 with clamp). No lane overflows:
+// x*ia + 128 <= 65153, t + (t >> 8) <= 65407, s + m <= 510.
+typedef uint16_t vkh_u16x2 __attribute__((vector_size(4)));
+
+VKH_HD vkh_u16x2 vkh_u16x2_of(uint32_t v) {
+    vkh_u16x2 r;
+    __builtin_memcpy(&r, &v, 4);
+    return r;
+}
+VKH_HD uint32_t vkh_u32_of(vkh_u16x2 v) {
+    uint32_t r;
+    __builtin_memcpy(&r, &v, 4);
+    return r;
+}
+#ifdef __clang__
+#define VKH_MIN_U16X2(a, b) __builtin_elementwise_min(a, b)
+#define VKH_SUBSAT_U16X2(a, b) __builtin_elementwise_sub_sat(a, b)
+#else
+#define VKH_MIN_U16X2(a, b) ((a) ^ (((a) ^ (b)) & (vkh_u16x2)((b) < (a))))
+#define VKH_SUBSAT_U16X2(a, b) (((a) - (b)) & (vkh_u16x2)((a) > (b)))
+#endif
+
+// source-dependent operands of blend_px16, computed once for a uniform source
+typedef struct {
+    vkh_u16x2 ia, s_lo, s_hi;
+} vkh_src16;
+
+VKH_HD vkh_src16 vkh_src16_of(uint32_t s) {
+    vkh_src16 r;
+    const uint16_t ia = (uint16_t)(255u - (s >> 24));
+    r.ia = (vkh_u16x2){ia, ia};
+    r.s_lo = vkh_u16x2_of(s & 0x00FF00FFu);
+    r.s_hi = vkh_u16x2_of((s >> 8) & 0x00FF00FFu);
+    return r;
+}
+
+// same result as blend_px_channels for every input
+VKH_HD uint32_t blend_px16(uint32_t op, uint32_t d, vkh_src16 S) {
+    if (op == 0)
+        return 0;
+    const vkh_u16x2 h = {128, 128};
+    const vkh_u16x2 t_lo = vkh_u16x2_of(d & 0x00FF00FFu) * S.ia + h;
+    const vkh_u16x2 t_hi = vkh_u16x2_of((d >> 8) & 0x00FF00FFu) * S.ia + h;
+    const vkh_u16x2 m_lo = (t_lo + (t_lo >> 8)) >> 8, m_hi = (t_hi + (t_hi >> 8)) >> 8;
+    vkh_u16x2 v_lo, v_hi;
+    if (op == 3) {
+        v_lo = VKH_SUBSAT_U16X2(S.s_lo, m_lo);
+        v_hi = VKH_SUBSAT_U16X2(S.s_hi, m_hi);
+    } else {
+        const vkh_u16x2 c = {255, 255};
+        v_lo = VKH_MIN_U16X2(S.s_lo + m_lo, c);
+        v_hi = VKH_MIN_U16X2(S.s_hi + m_hi, c);
+    }
+    return vkh_u32_of(v_lo) | (vkh_u32_of(v_hi) << 8);
+}
+
 #endif

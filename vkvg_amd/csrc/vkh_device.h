@@ -320,11 +320,8 @@ __device__ __forceinline__ bool point_above(const Edge &e, int32_t px, int32_t p
 // ---------------------------------------------------------------------------------------
 // float colour channel -> UNORM8 (clamped, rounded to nearest)
 __device__ __forceinline__ uint8_t f2u8(float f) {
-    if (!(f > 0.f))
-        return 0;
-    if (f >= 1.f)
-        return 255;
-    return (uint8_t)rintf(f * 255.0f);
+    // NaN and f <= 0 give 0 (fmaxf returns the non-NaN operand), f >= 1 gives 255
+    return (uint8_t)rintf(fminf(fmaxf(f, 0.f), 1.f) * 255.0f);
 }
 __device__ __forceinline__ uint32_t draw_key(uint32_t n) {
     uint32_t x = n + 0x9E3779B9u;

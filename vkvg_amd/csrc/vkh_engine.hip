@@ -674,9 +674,14 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
                           P<uint2>(eng->erange));
     vkh::launch_draw_tiles(s, d_draws, n_draws, d_tg, P<int4>(eng->bbox), T, P<DrawTiles>(eng->dtiles),
                            P<uint32_t>(eng->pt_count));
-    if (!need(eng, cap, eng->dprep, sizeof(DrawPrep) * (n_draws + 1)))
+    // draw records, gradient records (one per draw), then one GradPrep per gradient
+    if (!need(eng, cap, eng->dprep,
+              (sizeof(DrawPrep) + sizeof(GradRec)) * (n_draws + 1) + sizeof(GradPrep) * (eng->n_grads + 1)))
         return -1;
-    vkh::launch_draw_prep(s, d_draws, n_draws, d_tg, d_grads, P<DrawPrep>(eng->dprep));
+    GradRec *d_grec = (GradRec *)(P<DrawPrep>(eng->dprep) + n_draws + 1);
+    GradPrep *d_gprep = (GradPrep *)(d_grec + n_draws + 1);
+    vkh::launch_draw_prep(s, d_draws, n_draws, d_tg, d_grads, eng->n_grads, P<DrawPrep>(eng->dprep), d_grec,
+                          d_gprep);
     if (scan_u32(eng, P<uint32_t>(eng->pt_count), n_draws, P<uint32_t>(eng->pt_off), cap))
         return -1;
     if (totals(eng, cap, {P<uint32_t>(eng->pt_off) + n_draws}, tot, 6))
@@ -692,10 +697,10 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     // was too small (the allocator then holds the exact total)
     uint32_t n_segs = 0;
     for (int pass = 0; pass < 2; pass++) {
-        const uint32_t seg_cap = (uint32_t)std::min<size_t>(eng->segs.cap / 4, 0xFFFFFFFFull);
+        const uint32_t seg_cap = (uint32_t)std::min<size_t>(eng->segs.cap / sizeof(Edge), 0xFFFFFFFFull);
         vkh::launch_zero_u32(s, P<uint32_t>(eng->counters), 1);
         vkh::launch_bin_draws(s, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), P<uint2>(eng->erange),
-                              P<DrawTiles>(eng->dtiles), n_draws, T, P<PathTile>(eng->ptiles), P<uint32_t>(eng->segs),
+                              P<DrawTiles>(eng->dtiles), n_draws, T, P<PathTile>(eng->ptiles), P<Edge>(eng->segs),
                               seg_cap, P<uint32_t>(eng->counters));
         CK(hipGetLastError());
         if (totals(eng, cap, {P<uint32_t>(eng->counters)}, tot, 7))
@@ -707,7 +712,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
             eng->err = "segment buffer overflow after resize";
             return -1;
         }
-        if (!need(eng, cap, eng->segs, 4ull * n_segs + 4))
+        if (!need(eng, cap, eng->segs, sizeof(Edge) * (size_t)n_segs + sizeof(Edge)))
             return -1;
     }
     st.segments = n_segs;
@@ -745,8 +750,9 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
         CK(hipEventRecord(eng->ev_stage[4], s)); // coarse done
     if (!cap)
         CK(hipEventRecord(eng->ev0, s));
-    vkh::launch_fine(s, T, eng->fine_words, tile_total, d_tg, n_targets, P<DrawPrep>(eng->dprep), d_grads, P<PathTile>(eng->ptiles),
-                     P<uint32_t>(eng->segs), P<Edge>(eng->edges), n_edges, nch, P<uint32_t>(eng->tile_off),
+    vkh::launch_fine(s, T, eng->fine_words, tile_total, d_tg, n_targets, P<DrawPrep>(eng->dprep), d_grec, d_grads, d_gprep,
+                     P<PathTile>(eng->ptiles),
+                     P<Edge>(eng->segs), nch, P<uint32_t>(eng->tile_off),
                      P<Cmd>(eng->cmds));
     CK(hipGetLastError());
     if (cap)

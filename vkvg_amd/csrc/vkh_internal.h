@@ -61,8 +61,30 @@ struct DrawPrep {
     float opacity;
     uint32_t gflags; // VKH_GF_*
     float g[6];      // gradient invariants: linear p0x, l, m, bb, sqrt(1+m*m); radial c0x, c0y, r0, dx, dy, cc
-    uint32_t pad[4];
+    float rg[2];     // linear: vkh_rcp_for_div of g[1] and g[4] (0: divide per pixel)
+    uint32_t pad[2];
 };
+
+// Per gradient: rspan[i] = vkh_rcp_for_div(stops[i] - stops[i-1]) for the colour step i >= 1
+struct GradPrep {
+    float rspan[16];
+};
+
+// Per gradient draw: everything the per-pixel evaluation reads, in one contiguous 160-byte record
+// that the fine kernel reads with independent scalar loads (no chain of dependent loads through
+// the draw and gradient tables). Stops beyond the first VKH_GR_INLINE come from the gradient table.
+#define VKH_GR_INLINE 4
+struct GradRec {
+    uint32_t type, count, gflags;
+    float opacity;
+    float g[6], rg[2];                   // as in DrawPrep
+    float stops[VKH_GR_INLINE];
+    float rspan[VKH_GR_INLINE];          // as in GradPrep
+    float colors[VKH_GR_INLINE][4];
+    int32_t grad;
+    uint32_t pad[3];
+};
+static_assert(sizeof(GradRec) == 160, "GradRec layout");
 
 struct Cmd {
     uint32_t draw;
@@ -114,16 +136,17 @@ void launch_draw_tiles(hipStream_t s, const vkh_draw *draws, uint32_t n_draws, c
 void launch_set_pt_base(hipStream_t s, DrawTiles *dt, const uint32_t *pt_off, uint32_t n_draws);
 void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
 void launch_bin_draws(hipStream_t s, const Edge *edges, const uint32_t *edraw, const uint2 *erange,
-                      const DrawTiles *dt, uint32_t n_draws, uint32_t tile, PathTile *pts, uint32_t *segs,
+                      const DrawTiles *dt, uint32_t n_draws, uint32_t tile, PathTile *pts, Edge *segs,
                       uint32_t seg_cap, uint32_t *seg_alloc);
 void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n_draws, const TargetInfo *tg,
-                      const vkh_gradient *grads, DrawPrep *out);
+                      const vkh_gradient *grads, uint32_t n_grads, DrawPrep *out, GradRec *rec,
+                      GradPrep *gout);
 void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, uint32_t nch, const TargetInfo *tg,
                    uint32_t n_targets, const vkh_draw *draws, const DrawTiles *dt, const PathTile *pts,
                    uint32_t *tile_count, const uint32_t *tile_off, Cmd *cmds);
 void launch_fine(hipStream_t s, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,
-                 uint32_t n_targets, const DrawPrep *dprep, const vkh_gradient *grads, const PathTile *pts,
-                 const uint32_t *segs, const Edge *edges, uint32_t n_edges, uint32_t nch, const uint32_t *tile_off,
-                 const Cmd *cmds);
+                 uint32_t n_targets, const DrawPrep *dprep, const GradRec *grec, const vkh_gradient *grads,
+                 const GradPrep *gprep, const PathTile *pts, const Edge *segs, uint32_t nch,
+                 const uint32_t *tile_off, const Cmd *cmds);
 void launch_unpremultiply(hipStream_t s, const uint8_t *src, uint8_t *dst, size_t npix);
 } // namespace vkh

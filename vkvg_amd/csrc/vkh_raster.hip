@@ -14,6 +14,7 @@
 // so only the edges touching a tile are read when rasterizing it.
 #include "vkh_internal.h"
 #include "vkh_blend.h"
+#include "vkh_fdiv.h"
 
 #include <hip/hip_runtime.h>
 
@@ -133,14 +134,15 @@ __global__ void k_set_pt_base(DrawTiles *dt, const uint32_t *pt_off, uint32_t n)
 // sample row added to the backdrop delta of the first tile whose first column lies right of the
 // crossing. The counts are scanned in LDS, the band's segment range is taken from a global
 // allocator with one atomic, the backdrop deltas are prefix-summed along each row and the path
-// tiles are written; the scatter pass then writes the edge indices through LDS cursors. A band
+// tiles are written; the scatter pass then writes the edges themselves (16 bytes, so the fine
+// kernel reads a tile's segments without a dependent load) through LDS cursors. A band
 // whose range does not fit the segment buffer writes no segments; the host sees the allocator
 // beyond the capacity, grows the buffer and bins again.
 #define VKH_BIN_LDS_TILES 4096
 
 template <bool SCATTER>
-__device__ __forceinline__ void bin_edge(const Edge &e, uint32_t ei, const DrawTiles &dt, int32_t T, int32_t ty_lo,
-                                         int32_t ty_hi, uint32_t *s_cnt, int32_t *s_back, uint32_t *segs,
+__device__ __forceinline__ void bin_edge(const Edge &e, const DrawTiles &dt, int32_t T, int32_t ty_lo,
+                                         int32_t ty_hi, uint32_t *s_cnt, int32_t *s_back, Edge *segs,
                                          uint32_t base) {
     const int32_t span = 256 * T;
     const int32_t ymin = min(e.y0, e.y1), ymax = max(e.y0, e.y1);
@@ -172,7 +174,7 @@ __device__ __forceinline__ void bin_edge(const Edge &e, uint32_t ei, const DrawT
             if (!SCATTER)
                 atomicAdd(&s_cnt[k], 1u);
             else
-                segs[base + atomicAdd(&s_cnt[k], 1u)] = ei;
+                segs[base + atomicAdd(&s_cnt[k], 1u)] = e;
         }
         if (!SCATTER && !horiz) {
             const int32_t sy = ty * span + 128; // first sample row of the tile row
@@ -224,7 +226,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t *a, uint32_t n
 __global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges, const uint32_t *__restrict__ edraw,
                                                   const uint2 *__restrict__ erange,
                                                   const DrawTiles *__restrict__ dts, uint32_t T,
-                                                  PathTile *__restrict__ pts, uint32_t *__restrict__ segs,
+                                                  PathTile *__restrict__ pts, Edge *__restrict__ segs,
                                                   uint32_t seg_cap, uint32_t *__restrict__ seg_alloc) {
     __shared__ uint32_t s_cnt[VKH_BIN_LDS_TILES];
     __shared__ int32_t s_back[VKH_BIN_LDS_TILES];
@@ -250,7 +252,7 @@ __global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges
             if (edraw[i] != d)
                 continue;
             const Edge e = edges[i];
-            bin_edge<false>(e, i, dt, (int32_t)T, ty_lo, ty_hi, s_cnt, s_back, nullptr, 0);
+            bin_edge<false>(e, dt, (int32_t)T, ty_lo, ty_hi, s_cnt, s_back, nullptr, 0);
         }
         __syncthreads();
         const uint32_t total = block_exclusive_scan(s_cnt, nt, s_wave);
@@ -283,7 +285,7 @@ __global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges
                 if (edraw[i] != d)
                     continue;
                 const Edge e = edges[i];
-                bin_edge<true>(e, i, dt, (int32_t)T, ty_lo, ty_hi, s_cnt, s_back, segs, base);
+                bin_edge<true>(e, dt, (int32_t)T, ty_lo, ty_hi, s_cnt, s_back, segs, base);
             }
         }
         __syncthreads();
@@ -394,8 +396,10 @@ __global__ __launch_bounds__(256) void k_coarse(bool write, const TargetInfo *__
 
 // ----------------------------------------------------------------------------------- fine
 __device__ __forceinline__ float clamp01(float x) { return fminf(fmaxf(x, 0.f), 1.f); }
-__device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
-    float t = clamp01((x - e0) / (e1 - e0));
+// smoothstep(e0, e1, x) with the division (x - e0) / (e1 - e0) taken through r, the prepared
+// reciprocal of e1 - e0 (same rounding as the IEEE division)
+__device__ __forceinline__ float smoothstep_r(float e0, float e1, float r, float x) {
+    float t = clamp01(vkh_div_rcp(x - e0, e1 - e0, r));
     return t * t * (3.f - 2.f * t);
 }
 __device__ __forceinline__ void mix4(float *c, const float *d, float a) {
@@ -408,7 +412,8 @@ __device__ __forceinline__ void mix4(float *c, const float *d, float a) {
 // pixel-independent terms of shaders/vkvg_main.frag:83-144 (p = gl_FragCoord.xy / size), each
 // computed with the same float operations the per-pixel evaluation used to repeat.
 __global__ void k_draw_prep(const vkh_draw *__restrict__ draws, uint32_t n, const TargetInfo *__restrict__ tg,
-                            const vkh_gradient *__restrict__ grads, DrawPrep *__restrict__ out) {
+                            const vkh_gradient *__restrict__ grads, DrawPrep *__restrict__ out,
+                            GradRec *__restrict__ rec) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
@@ -429,8 +434,8 @@ __global__ void k_draw_prep(const vkh_draw *__restrict__ draws, uint32_t n, cons
     p.gflags = 0;
     for (int k = 0; k < 6; k++)
         p.g[k] = 0.f;
-    for (int k = 0; k < 4; k++)
-        p.pad[k] = 0;
+    p.rg[0] = p.rg[1] = 0.f;
+    p.pad[0] = p.pad[1] = 0;
     if (d.grad >= 0) {
         const vkh_gradient &g = grads[d.grad];
         const float W = (float)tg[d.target].width, H = (float)tg[d.target].height;
@@ -450,7 +455,9 @@ __global__ void k_draw_prep(const vkh_draw *__restrict__ draws, uint32_t n, cons
                 p.g[3] = p0y - m * p0x;
                 p.g[4] = sqrtf(1 + m * m);
                 p.gflags = uy < 0 ? VKH_GF_UYNEG : 0u;
+                p.rg[1] = vkh_rcp_for_div(p.g[4]);
             }
+            p.rg[0] = vkh_rcp_for_div(l);
         } else {
             float c0x = g.cp[0][0] / W, c0y = g.cp[0][1] / H;
             float c1x = g.cp[1][0] / W, c1y = g.cp[1][1] / H;
@@ -465,27 +472,57 @@ __global__ void k_draw_prep(const vkh_draw *__restrict__ draws, uint32_t n, cons
         }
     }
     out[i] = p;
+    if (d.grad >= 0) {
+        const vkh_gradient &g = grads[d.grad];
+        GradRec r;
+        r.type = g.type;
+        r.count = g.count;
+        r.gflags = p.gflags;
+        r.opacity = p.opacity;
+        for (int k = 0; k < 6; k++)
+            r.g[k] = p.g[k];
+        r.rg[0] = p.rg[0];
+        r.rg[1] = p.rg[1];
+        for (int k = 0; k < VKH_GR_INLINE; k++) {
+            r.stops[k] = g.stops[k];
+            r.rspan[k] = k ? vkh_rcp_for_div(g.stops[k] - g.stops[k - 1]) : 0.f;
+            for (int c = 0; c < 4; c++)
+                r.colors[k][c] = g.colors[k][c];
+        }
+        r.grad = d.grad;
+        r.pad[0] = r.pad[1] = r.pad[2] = 0;
+        rec[i] = r;
+    }
 }
 
-// shaders/vkvg_main.frag:83-144 at p = (px, py), with the draw's invariant terms from DrawPrep
-__device__ void gradient_color(const vkh_gradient &g, const DrawPrep &P, float px, float py, float *col) {
-    if (g.type == 2) {
-        float dist;
+__global__ void k_grad_prep(const vkh_gradient *__restrict__ grads, uint32_t n, GradPrep *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    GradPrep p;
+    p.rspan[0] = 0.f;
+    for (int k = 1; k < 16; k++)
+        p.rspan[k] = vkh_rcp_for_div(grads[i].stops[k] - grads[i].stops[k - 1]);
+    out[i] = p;
+}
+
+// shaders/vkvg_main.frag:83-144 at p = (px, py), from the draw's staged record R (invariant terms,
+// first stops with their spans' reciprocals and colours); later stops come from the gradient table
+// g and its reciprocals gp. Both gradient types apply colour step 1 and then steps 2 .. count-1
+// (the linear loop of the shader indexes the same steps from 0).
+__device__ void gradient_color(const GradRec &P, const vkh_gradient *__restrict__ grads,
+                               const GradPrep *__restrict__ gprep, float px, float py, float *col) {
+    float x;
+    if (P.type == 2) {
         if (P.gflags & VKH_GF_UY0) {
+            x = vkh_div_rcp(px - P.g[0], P.g[1], P.rg[0]);
             if (P.gflags & VKH_GF_UXNEG)
-                dist = -(px - P.g[0]) / P.g[1];
-            else
-                dist = (px - P.g[0]) / P.g[1];
+                x = -x; // -(a) / b and -(a / b) round alike
         } else {
-            dist = ((py - P.g[2] * px - P.g[3]) / P.g[4]) / P.g[1];
+            x = vkh_div_rcp(vkh_div_rcp(py - P.g[2] * px - P.g[3], P.g[4], P.rg[1]), P.g[1], P.rg[0]);
             if (P.gflags & VKH_GF_UYNEG)
-                dist = -dist;
+                x = -x;
         }
-        for (int i = 0; i < 4; i++)
-            col[i] = g.colors[0][i];
-        mix4(col, g.colors[1], smoothstepf(g.stops[0], g.stops[1], dist));
-        for (int i = 1; i < (int)g.count - 1; ++i)
-            mix4(col, g.colors[i + 1], smoothstepf(g.stops[i], g.stops[i + 1], dist));
     } else {
         const float c0x = P.g[0], c0y = P.g[1], r0 = P.g[2], diffx = P.g[3], diffy = P.g[4], cc = P.g[5];
         float gradLength = 1.0f;
@@ -502,12 +539,24 @@ __device__ void gradient_color(const vkh_gradient &g, const DrawPrep &P, float p
             gradLength = sqrtf(ddx * ddx + ddy * ddy) - r0;
         }
         float qx = px - c0x, qy = py - c0y;
-        float grad = (sqrtf(qx * qx + qy * qy) - r0) / gradLength;
-        for (int i = 0; i < 4; i++)
-            col[i] = g.colors[0][i];
-        mix4(col, g.colors[1], smoothstepf(g.stops[0], g.stops[1], grad));
-        for (int i = 2; i < (int)g.count; i++)
-            mix4(col, g.colors[i], smoothstepf(g.stops[i - 1], g.stops[i], grad));
+        x = (sqrtf(qx * qx + qy * qy) - r0) / gradLength;
+    }
+    for (int i = 0; i < 4; i++)
+        col[i] = P.colors[0][i];
+    const int steps = max((int)P.count, 2);
+    const int inl = min(steps, VKH_GR_INLINE);
+#pragma unroll
+    for (int i = 1; i < VKH_GR_INLINE; i++) {
+        if (i >= inl)
+            break;
+        const float e0 = P.stops[i - 1], e1 = P.stops[i];
+        mix4(col, P.colors[i], smoothstep_r(e0, e1, P.rspan[i], x));
+    }
+    if (steps > VKH_GR_INLINE) {
+        const vkh_gradient &g = grads[P.grad];
+        const GradPrep &gp = gprep[P.grad];
+        for (int i = VKH_GR_INLINE; i < steps; i++)
+            mix4(col, g.colors[i], smoothstep_r(g.stops[i - 1], g.stops[i], gp.rspan[i], x));
     }
 }
 
@@ -525,9 +574,11 @@ __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) { return (uint32_t)_
 template <int T, int WORDS>
 __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg, uint32_t n_targets,
                                               const DrawPrep *__restrict__ dprep,
+                                              const GradRec *__restrict__ grec,
                                               const vkh_gradient *__restrict__ grads,
-                                              const PathTile *__restrict__ pts, const uint32_t *__restrict__ segs,
-                                              const Edge *__restrict__ edges, uint32_t n_edges, uint32_t nch,
+                                              const GradPrep *__restrict__ gprep,
+                                              const PathTile *__restrict__ pts, const Edge *__restrict__ segs,
+                                              uint32_t nch,
                                               const uint32_t *__restrict__ tile_off, const Cmd *__restrict__ cmds) {
     constexpr int NPIX = T * T;
     constexpr int NT = NPIX >= 256 ? 256 : NPIX;  // threads
@@ -639,10 +690,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
                     hi = mid;
             }
             const uint32_t j = lo;
-            const uint32_t ei = segs[c_segoff[j] + (it - c_item[j])];
-            if (ei >= n_edges)
-                continue;
-            const Edge e = edges[ei];
+            const Edge e = segs[c_segoff[j] + (it - c_item[j])];
             // V term: the edge separates (xs0, s0y) from (xs0, sy(r)) for every row r >= r*
             if (min(e.x0, e.x1) <= xs0 && xs0 < max(e.x0, e.x1) && point_above(e, xs0, s0y)) {
                 int32_t a = 1, b = T; // first r in [1, T) with !point_above(sy(r)), T if none
@@ -708,7 +756,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
             const int32_t bk = (int32_t)uniform_u32((uint32_t)c_back[j]);
             const uint32_t key = uniform_u32(c_key[j]);
             const uint32_t solid = uniform_u32(c_solid[j]);
-            const uint32_t dd = uniform_u32(c_draw[j]);
+            const vkh_src16 S0 = vkh_src16_of(solid);
 #pragma unroll
             for (int q = 0; q < PPT; q++) {
                 int k = 1;
@@ -718,20 +766,23 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
                 }
                 if (k <= 0 || !valid[q])
                     continue;
-                uint32_t s = solid;
+                vkh_src16 S = S0;
                 if (meta & VKH_DP_GRAD) {
-                    const DrawPrep &P = dprep[dd];
+                    const GradRec &P = grec[uniform_u32(c_draw[j])]; // scalar loads
                     const float op_ = P.opacity;
                     float col[4];
-                    gradient_color(grads[P.grad], P, gx[q], gy[q], col);
-                    s = (uint32_t)f2u8(col[0] * op_) | ((uint32_t)f2u8(col[1] * op_) << 8) |
-                        ((uint32_t)f2u8(col[2] * op_) << 16) | ((uint32_t)f2u8(col[3] * op_) << 24);
+                    gradient_color(P, grads, gprep, gx[q], gy[q], col);
+                    S = vkh_src16_of((uint32_t)f2u8(col[0] * op_) | ((uint32_t)f2u8(col[1] * op_) << 8) |
+                                     ((uint32_t)f2u8(col[2] * op_) << 16) | ((uint32_t)f2u8(col[3] * op_) << 24));
                 }
-                uint32_t v = px[q];
-                for (int jj = 0; jj < k; jj++)
-                    v = blend_px(op, v, s);
+                // a stroke pixel covered by k triangles is blended k times
+                uint32_t v = px[q], sg = sig[q];
+                do {
+                    v = blend_px16(op, v, S);
+                    sg += key;
+                } while (--k > 0);
                 px[q] = v;
-                sig[q] += (uint32_t)k * key;
+                sig[q] = sg;
             }
         }
         __syncthreads();
@@ -791,7 +842,7 @@ void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n) {
         hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
 }
 void launch_bin_draws(hipStream_t s, const Edge *edges, const uint32_t *edraw, const uint2 *erange,
-                      const DrawTiles *dt, uint32_t n_draws, uint32_t tile, PathTile *pts, uint32_t *segs,
+                      const DrawTiles *dt, uint32_t n_draws, uint32_t tile, PathTile *pts, Edge *segs,
                       uint32_t seg_cap, uint32_t *seg_alloc) {
     if (n_draws)
         hipLaunchKernelGGL(k_bin_draw, dim3(n_draws), dim3(256), 0, s, edges, edraw, erange, dt, tile, pts, segs,
@@ -805,51 +856,54 @@ void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, uint32_t nch, con
                            tile_count, tile_off, cmds);
 }
 void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n, const TargetInfo *tg,
-                      const vkh_gradient *grads, DrawPrep *out) {
+                      const vkh_gradient *grads, uint32_t n_grads, DrawPrep *out, GradRec *rec,
+                      GradPrep *gout) {
     if (n)
-        hipLaunchKernelGGL(k_draw_prep, dim3((n + 255) / 256), dim3(256), 0, s, draws, n, tg, grads, out);
+        hipLaunchKernelGGL(k_draw_prep, dim3((n + 255) / 256), dim3(256), 0, s, draws, n, tg, grads, out, rec);
+    if (n_grads)
+        hipLaunchKernelGGL(k_grad_prep, dim3((n_grads + 255) / 256), dim3(256), 0, s, grads, n_grads, gout);
 }
 template <int T, int W>
 static void fine_launch(hipStream_t s, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
-                        const DrawPrep *dprep, const vkh_gradient *grads, const PathTile *pts, const uint32_t *segs,
-                        const Edge *edges, uint32_t n_edges, uint32_t nch, const uint32_t *tile_off,
+                        const DrawPrep *dprep, const GradRec *grec, const vkh_gradient *grads, const GradPrep *gprep,
+                        const PathTile *pts, const Edge *segs, uint32_t nch, const uint32_t *tile_off,
                         const Cmd *cmds) {
     constexpr int NT = T * T >= 256 ? 256 : T * T;
-    hipLaunchKernelGGL((k_fine<T, W>), dim3(n_tiles), dim3(NT), 0, s, tg, n_targets, dprep, grads, pts, segs, edges,
-                       n_edges, nch, tile_off, cmds);
+    hipLaunchKernelGGL((k_fine<T, W>), dim3(n_tiles), dim3(NT), 0, s, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch,
+                       tile_off, cmds);
 }
 template <int T>
 static void fine_words(hipStream_t s, uint32_t words, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
-                       const DrawPrep *dprep, const vkh_gradient *grads, const PathTile *pts, const uint32_t *segs,
-                       const Edge *edges, uint32_t n_edges, uint32_t nch, const uint32_t *tile_off,
+                       const DrawPrep *dprep, const GradRec *grec, const vkh_gradient *grads, const GradPrep *gprep,
+                       const PathTile *pts, const Edge *segs, uint32_t nch, const uint32_t *tile_off,
                        const Cmd *cmds) {
     switch (words) {
     case 2048:
-        fine_launch<T, 2048>(s, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, nch, tile_off, cmds);
+        fine_launch<T, 2048>(s, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     case 8192:
-        fine_launch<T, 8192>(s, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, nch, tile_off, cmds);
+        fine_launch<T, 8192>(s, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     default:
-        fine_launch<T, 4096>(s, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, nch, tile_off, cmds);
+        fine_launch<T, 4096>(s, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     }
 }
 void launch_fine(hipStream_t s, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,
-                 uint32_t n_targets, const DrawPrep *dprep, const vkh_gradient *grads, const PathTile *pts,
-                 const uint32_t *segs, const Edge *edges, uint32_t n_edges, uint32_t nch, const uint32_t *tile_off,
-                 const Cmd *cmds) {
+                 uint32_t n_targets, const DrawPrep *dprep, const GradRec *grec, const vkh_gradient *grads,
+                 const GradPrep *gprep, const PathTile *pts, const Edge *segs, uint32_t nch,
+                 const uint32_t *tile_off, const Cmd *cmds) {
     if (!n_tiles)
         return;
     switch (tile) {
     case 8:
-        fine_words<8>(s, words, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, nch, tile_off, cmds);
+        fine_words<8>(s, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     case 32:
-        fine_words<32>(s, words, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, nch, tile_off, cmds);
+        fine_words<32>(s, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     default:
-        fine_words<16>(s, words, n_tiles, tg, n_targets, dprep, grads, pts, segs, edges, n_edges, nch, tile_off, cmds);
+        fine_words<16>(s, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     }
 }
