@@ -273,50 +273,6 @@ __device__ __forceinline__ int64_t crossing_column(int32_t xt, int32_t yt, int32
     return ceil_div_pos(N, 256 * dyb);
 }
 
-// crossing_column over successive sample rows without a division per row. At row sy:
-// q = ceil(N / D) and rem = q*D - N in [0, D), D = 256*(yb - yt) > 0. One row down adds
-// dN = 256*(xb - xt) to N; with dq = floor(dN / D) and dr = dN - dq*D in [0, D) the next row has
-// q' = q + dq + (dr > rem) and rem' = rem - dr (+ D when it carried). The quotients start from a
-// double estimate, off by at most one, and are corrected exactly in 64-bit integers.
-__device__ __forceinline__ void crossing_dda(int32_t xt, int32_t yt, int32_t xb, int32_t yb, int32_t sy, int64_t &q,
-                                             int64_t &rem, int64_t &dq, int64_t &dr) {
-    const int64_t dyb = (int64_t)yb - yt;
-    const int64_t D = 256 * dyb;
-    const int64_t N = dyb * ((int64_t)xt - 128) + ((int64_t)xb - xt) * ((int64_t)sy - yt);
-    q = (int64_t)ceil((double)N / (double)D);
-    rem = q * D - N;
-    if (rem < 0) {
-        q += 1;
-        rem += D;
-    } else if (rem >= D) {
-        q -= 1;
-        rem -= D;
-    }
-    const int64_t dN = 256 * ((int64_t)xb - xt);
-    dq = (int64_t)floor((double)dN / (double)D);
-    dr = dN - dq * D;
-    if (dr < 0) {
-        dq -= 1;
-        dr += D;
-    } else if (dr >= D) {
-        dq += 1;
-        dr -= D;
-    }
-}
-
-// Is the perturbed point (px, py) above the edge (towards -y)? Only called for edges that
-// cross the vertical line through px (min x <= px < max x).
-__device__ __forceinline__ bool point_above(const Edge &e, int32_t px, int32_t py) {
-    int32_t lx, ly, rx, ry;
-    if (e.x0 < e.x1) {
-        lx = e.x0; ly = e.y0; rx = e.x1; ry = e.y1;
-    } else {
-        lx = e.x1; ly = e.y1; rx = e.x0; ry = e.y0;
-    }
-    int64_t E0 = ((int64_t)rx - lx) * ((int64_t)py - ly) - ((int64_t)ry - ly) * ((int64_t)px - lx);
-    return E0 < 0 || (E0 == 0 && ry > ly);
-}
-
 // ---------------------------------------------------------------------------------------
 // float colour channel -> UNORM8 (clamped, rounded to nearest)
 __device__ __forceinline__ uint8_t f2u8(float f) {

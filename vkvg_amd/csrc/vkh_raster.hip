@@ -15,6 +15,7 @@
 #include "vkh_internal.h"
 #include "vkh_blend.h"
 #include "vkh_fdiv.h"
+#include "vkh_cross.h"
 
 #include <hip/hip_runtime.h>
 
@@ -692,17 +693,12 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
             const uint32_t j = lo;
             const Edge e = segs[c_segoff[j] + (it - c_item[j])];
             // V term: the edge separates (xs0, s0y) from (xs0, sy(r)) for every row r >= r*
-            if (min(e.x0, e.x1) <= xs0 && xs0 < max(e.x0, e.x1) && point_above(e, xs0, s0y)) {
-                int32_t a = 1, b = T; // first r in [1, T) with !point_above(sy(r)), T if none
-                while (a < b) {
-                    int32_t m = (a + b) >> 1;
-                    if (point_above(e, xs0, s0y + 256 * m))
-                        a = m + 1;
-                    else
-                        b = m;
-                }
-                if (a < T)
-                    atomicAdd(&srow[j][a], e.x1 > e.x0 ? -1 : 1);
+            if (min(e.x0, e.x1) <= xs0 && xs0 < max(e.x0, e.x1)) {
+                const bool lr = e.x0 < e.x1;
+                const int32_t r = cross_first_row(lr ? e.x0 : e.x1, lr ? e.y0 : e.y1, lr ? e.x1 : e.x0,
+                                                  lr ? e.y1 : e.y0, xs0, s0y, 0, T);
+                if (r >= 1 && r < T) // r* = 0: (xs0, s0y) is not above the edge
+                    atomicAdd(&srow[j][r], lr ? -1 : 1);
             }
             // H term: rows with ytop <= sy(r) < ybot
             const bool down = e.y0 < e.y1;
@@ -713,22 +709,14 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
             if (r_lo > r_hi)
                 continue;
             const int32_t sgn = down ? 1 : -1;
-            int64_t q, rem, dq, dr;
-            crossing_dda(xt, yt, xb, yb, s0y + 256 * r_lo, q, rem, dq, dr);
-            const int64_t D = 256 * ((int64_t)yb - yt);
+            vkh_dda dda = cross_dda_start(xt, yt, xb, yb, s0y + 256 * r_lo);
             for (int32_t r = r_lo;; r++) {
-                const int64_t lc = q - X0;
+                const int32_t lc = dda.q - X0;
                 if (lc >= 1 && lc < T)
                     atomicAdd(&sd[j][r][lc], sgn);
                 if (r == r_hi)
                     break;
-                q += dq;
-                if (dr > rem) {
-                    q += 1;
-                    rem += D - dr;
-                } else {
-                    rem -= dr;
-                }
+                cross_dda_step(&dda);
             }
         }
         __syncthreads();
