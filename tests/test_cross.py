@@ -1,7 +1,8 @@
 """The fine kernel's crossing arithmetic in doubles (vkvg_amd/csrc/vkh_cross.h) equals exact
 integer arithmetic: the first sample row below an edge on a sample column (against a row-by-row
 scan of the perturbed above-edge predicate in 128-bit integers) and the crossing column of every
-sample row an edge spans (against ceil(N / D) in 128-bit integers). Random edges cover tile-sized,
+sample row an edge spans, stepped row by row and computed directly (against ceil(N / D) in 128-bit
+integers). Random edges cover tile-sized,
 surface-sized and extreme (+-2^23 fixed point, the clamp of edge coordinates) extents, horizontal,
 vertical and near-vertical slopes, and samples on the edge (the tie-break)."""
 import os
@@ -59,6 +60,7 @@ static void check(int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t X0, in
                 i128 q = ceil_div(N, (i128)256 * (yb - yt));
                 n_cols++;
                 if ((i128)s.q != q) { if (bad < 5) printf("COL %d %d %d %d r=%d got %d\n", x0, y0, x1, y1, r, s.q); bad++; }
+                if ((i128)cross_column(xt, yt, xb, yb, y) != q) bad++;
                 if (r < r_hi) cross_dda_step(&s);
             }
         }

@@ -48,8 +48,8 @@ typedef struct {
     double rem, dr, D;  // exact integers below 2^33
 } vkh_dda;
 
-VKH_CR vkh_dda cross_dda_start(int32_t xt, int32_t yt, int32_t xb, int32_t yb, int32_t y) {
-    vkh_dda s;
+// q = ceil(N / D) at row y, with rem = q D - N
+VKH_CR double cross_column_rem(int32_t xt, int32_t yt, int32_t xb, int32_t yb, int32_t y, double *rem_out) {
     const double dy = (double)(yb - yt);
     const double D = 256.0 * dy;
     const double N = fma(dy, (double)(xt - 128), (double)(xb - xt) * (double)(y - yt)); // exact, < 2^50
@@ -62,6 +62,21 @@ VKH_CR vkh_dda cross_dda_start(int32_t xt, int32_t yt, int32_t xb, int32_t yb, i
         q -= 1.0;
         rem -= D;
     }
+    *rem_out = rem;
+    return q;
+}
+
+// the first column whose sample at row y lies right of the edge (yt <= y < yb)
+VKH_CR int32_t cross_column(int32_t xt, int32_t yt, int32_t xb, int32_t yb, int32_t y) {
+    double rem;
+    return (int32_t)cross_column_rem(xt, yt, xb, yb, y, &rem);
+}
+
+VKH_CR vkh_dda cross_dda_start(int32_t xt, int32_t yt, int32_t xb, int32_t yb, int32_t y) {
+    vkh_dda s;
+    const double D = 256.0 * (double)(yb - yt);
+    double rem;
+    const double q = cross_column_rem(xt, yt, xb, yb, y, &rem);
     const double dN = 256.0 * (double)(xb - xt);
     double dq = floor(dN / D);
     double dr = fma(-dq, D, dN);

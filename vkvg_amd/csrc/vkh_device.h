@@ -260,19 +260,6 @@ struct Edge {
     int32_t x0, y0, x1, y1;
 };
 
-__device__ __forceinline__ int64_t ceil_div_pos(int64_t n, int64_t d) { // d > 0, |n|,d < 2^53, |n/d| < 2^20
-    double q = (double)n / (double)d;
-    return (int64_t)ceil(q);
-}
-
-// First column whose (perturbed) sample at row sy lies right of the edge; the caller
-// guarantees the edge crosses the row (ytop <= sy < ybot).
-__device__ __forceinline__ int64_t crossing_column(int32_t xt, int32_t yt, int32_t xb, int32_t yb, int32_t sy) {
-    int64_t dyb = (int64_t)yb - yt;
-    int64_t N = dyb * ((int64_t)xt - 128) + ((int64_t)xb - xt) * ((int64_t)sy - yt);
-    return ceil_div_pos(N, 256 * dyb);
-}
-
 // ---------------------------------------------------------------------------------------
 // float colour channel -> UNORM8 (clamped, rounded to nearest)
 __device__ __forceinline__ uint8_t f2u8(float f) {

@@ -19,12 +19,8 @@
 
 #include <hip/hip_runtime.h>
 
-__device__ __forceinline__ int32_t floor_div32(int32_t a, int32_t b) { // b > 0
-    int32_t q = a / b;
-    if ((a % b != 0) && (a < 0))
-        q--;
-    return q;
-}
+// Tiles are T = 1 << lt pixels, so a tile spans 1 << (lt + 8) in 24.8 fixed point and floor
+// division by the span is an arithmetic shift.
 
 // ----------------------------------------------------------------------------------- draw bbox
 __global__ void k_bbox_init(int4 *bbox, uint2 *erange, uint32_t n) {
@@ -99,15 +95,15 @@ __global__ void k_draw_tiles(const vkh_draw *__restrict__ draws, uint32_t n, con
         return;
     }
     int4 b = bbox[i];
-    int32_t span = 256 * (int32_t)T;
+    const int32_t ssh = 8 + (int32_t)__builtin_ctz(T); // log2 of the tile span
     if (b.x > b.z) {
         o.bx0 = o.by0 = 0;
         o.bx1 = o.by1 = -1;
     } else {
-        o.bx0 = max(floor_div32(b.x, span), 0);
-        o.by0 = max(floor_div32(b.y, span), 0);
-        o.bx1 = min(floor_div32(b.z, span), (int32_t)t.tilesW - 1);
-        o.by1 = min(floor_div32(b.w, span), (int32_t)t.tilesH - 1);
+        o.bx0 = max(b.x >> ssh, 0);
+        o.by0 = max(b.y >> ssh, 0);
+        o.bx1 = min(b.z >> ssh, (int32_t)t.tilesW - 1);
+        o.by1 = min(b.w >> ssh, (int32_t)t.tilesH - 1);
     }
     if (o.bx0 > o.bx1 || o.by0 > o.by1) {
         o.bx0 = o.by0 = 0;
@@ -142,13 +138,13 @@ __global__ void k_set_pt_base(DrawTiles *dt, const uint32_t *pt_off, uint32_t n)
 #define VKH_BIN_LDS_TILES 4096
 
 template <bool SCATTER>
-__device__ __forceinline__ void bin_edge(const Edge &e, const DrawTiles &dt, int32_t T, int32_t ty_lo,
+__device__ __forceinline__ void bin_edge(const Edge &e, const DrawTiles &dt, int32_t lt, int32_t ty_lo,
                                          int32_t ty_hi, uint32_t *s_cnt, int32_t *s_back, Edge *segs,
                                          uint32_t base) {
-    const int32_t span = 256 * T;
+    const int32_t ssh = lt + 8, span = 1 << ssh;
     const int32_t ymin = min(e.y0, e.y1), ymax = max(e.y0, e.y1);
-    const int32_t ty0 = max(floor_div32(ymin, span), ty_lo);
-    const int32_t ty1 = min(floor_div32(ymax, span), ty_hi);
+    const int32_t ty0 = max(ymin >> ssh, ty_lo);
+    const int32_t ty1 = min(ymax >> ssh, ty_hi);
     const bool horiz = e.y0 == e.y1;
     const double invdy = horiz ? 0.0 : 1.0 / ((double)e.y1 - (double)e.y0);
     const int dir = e.y0 < e.y1 ? 1 : -1;
@@ -167,8 +163,8 @@ __device__ __forceinline__ void bin_edge(const Edge &e, const DrawTiles &dt, int
             xlo = (int32_t)floor(fmin(xa, xc)) - 256;
             xhi = (int32_t)ceil(fmax(xa, xc)) + 256;
         }
-        const int32_t tx0 = max(floor_div32(xlo, span), dt.bx0);
-        const int32_t tx1 = min(floor_div32(xhi, span), dt.bx1);
+        const int32_t tx0 = max(xlo >> ssh, dt.bx0);
+        const int32_t tx1 = min(xhi >> ssh, dt.bx1);
         const uint32_t rowbase = (uint32_t)(ty - ty_lo) * dt.bw;
         for (int32_t tx = tx0; tx <= tx1; tx++) {
             const uint32_t k = rowbase + (uint32_t)(tx - dt.bx0);
@@ -180,9 +176,8 @@ __device__ __forceinline__ void bin_edge(const Edge &e, const DrawTiles &dt, int
         if (!SCATTER && !horiz) {
             const int32_t sy = ty * span + 128; // first sample row of the tile row
             if (yt <= sy && sy < yb) {
-                const int64_t c = crossing_column(xt, yt, xb, yb, sy);
-                // first tile tx with tx*T >= c
-                int64_t t0 = c >= 0 ? (c + T - 1) / T : -((-c) / T);
+                const int32_t c = cross_column(xt, yt, xb, yb, sy);
+                int32_t t0 = -((-c) >> lt); // first tile tx with tx*T >= c: ceil(c / T)
                 if (t0 < dt.bx0)
                     t0 = dt.bx0;
                 if (t0 <= dt.bx1)
@@ -238,6 +233,7 @@ __global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges
     if (dt.bw == 0 || dt.bx0 > dt.bx1 || dt.by0 > dt.by1)
         return; // paint / clear, or no edges
     const uint2 er = erange[d];
+    const int32_t lt = (int32_t)__builtin_ctz(T);
     const uint32_t bw = dt.bw, bh = (uint32_t)(dt.by1 - dt.by0 + 1);
     const uint32_t band_rows = max(1u, VKH_BIN_LDS_TILES / bw); // bw <= 4096 (width <= 32768, T >= 8)
     for (uint32_t r0 = 0; r0 < bh; r0 += band_rows) {
@@ -253,7 +249,7 @@ __global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges
             if (edraw[i] != d)
                 continue;
             const Edge e = edges[i];
-            bin_edge<false>(e, dt, (int32_t)T, ty_lo, ty_hi, s_cnt, s_back, nullptr, 0);
+            bin_edge<false>(e, dt, lt, ty_lo, ty_hi, s_cnt, s_back, nullptr, 0);
         }
         __syncthreads();
         const uint32_t total = block_exclusive_scan(s_cnt, nt, s_wave);
@@ -286,7 +282,7 @@ __global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges
                 if (edraw[i] != d)
                     continue;
                 const Edge e = edges[i];
-                bin_edge<true>(e, dt, (int32_t)T, ty_lo, ty_hi, s_cnt, s_back, segs, base);
+                bin_edge<true>(e, dt, lt, ty_lo, ty_hi, s_cnt, s_back, segs, base);
             }
         }
         __syncthreads();
