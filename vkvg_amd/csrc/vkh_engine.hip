@@ -653,25 +653,25 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
                              P<uint32_t>(eng->sv_off), P<uint32_t>(eng->st_off), P<uint32_t>(eng->tris));
     vkh::launch_unit_ranges(s, n_units, d_step_off, P<uint32_t>(eng->sv_off), P<uint32_t>(eng->st_off),
                             P<uint32_t>(eng->u_toff), P<uint32_t>(eng->u_vbeg), P<uint32_t>(eng->u_vend));
-    vkh::launch_fill_edges(s, d_units, n_units, P<uint32_t>(eng->fe_off), n_fill_edges, d_draws, d_tg,
-                           P<uint32_t>(eng->sp_first), P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<Edge>(eng->edges),
-                           P<uint32_t>(eng->edraw));
-    vkh::launch_tri_edges(s, d_units, n_units, P<uint32_t>(eng->u_toff), P<uint32_t>(eng->u_vbeg),
-                          P<uint32_t>(eng->u_vend), n_tris, d_draws, d_tg, P<v2>(eng->verts), P<uint32_t>(eng->tris),
-                          P<Edge>(eng->edges) + n_fill_edges, P<uint32_t>(eng->edraw) + n_fill_edges);
-    CK(hipGetLastError());
-
-    if (!cap)
-        CK(hipEventRecord(eng->ev_stage[1], s)); // geometry done
-    // ---- tiles of every draw
+    // the edge kernels also reduce each draw's edge bbox and edge index range
     size_t dn4 = 4ull * (n_draws + 1);
     if (!need(eng, cap, eng->bbox, sizeof(int4) * (n_draws + 1)) || !need(eng, cap, eng->dtiles, sizeof(DrawTiles) * (n_draws + 1)) ||
         !need(eng, cap, eng->erange, sizeof(uint2) * (n_draws + 1)) || !need(eng, cap, eng->pt_count, dn4) ||
         !need(eng, cap, eng->pt_off, dn4) || !need(eng, cap, eng->counters, 256))
         return -1;
     vkh::launch_draw_bbox_init(s, P<int4>(eng->bbox), P<uint2>(eng->erange), n_draws);
-    vkh::launch_draw_bbox(s, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), n_edges, P<int4>(eng->bbox),
-                          P<uint2>(eng->erange));
+    vkh::launch_fill_edges(s, d_units, n_units, P<uint32_t>(eng->fe_off), n_fill_edges, d_draws, d_tg,
+                           P<uint32_t>(eng->sp_first), P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<Edge>(eng->edges),
+                           P<uint32_t>(eng->edraw), P<int4>(eng->bbox), P<uint2>(eng->erange));
+    vkh::launch_tri_edges(s, d_units, n_units, P<uint32_t>(eng->u_toff), P<uint32_t>(eng->u_vbeg),
+                          P<uint32_t>(eng->u_vend), n_tris, d_draws, d_tg, P<v2>(eng->verts), P<uint32_t>(eng->tris),
+                          P<Edge>(eng->edges) + n_fill_edges, P<uint32_t>(eng->edraw) + n_fill_edges, n_fill_edges,
+                          P<int4>(eng->bbox), P<uint2>(eng->erange));
+    CK(hipGetLastError());
+
+    if (!cap)
+        CK(hipEventRecord(eng->ev_stage[1], s)); // geometry done
+    // ---- tiles of every draw
     vkh::launch_draw_tiles(s, d_draws, n_draws, d_tg, P<int4>(eng->bbox), T, P<DrawTiles>(eng->dtiles),
                            P<uint32_t>(eng->pt_count));
     // draw records, gradient records (one per draw), then one GradPrep per gradient

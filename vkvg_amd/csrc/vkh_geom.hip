@@ -735,24 +735,28 @@ __device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *a, uint32_t 
 __global__ void k_fill_edges(const Unit *__restrict__ units, uint32_t n_units, const uint32_t *__restrict__ fill_off,
                              uint32_t total, const vkh_draw *__restrict__ draws, const TargetInfo *__restrict__ tgts,
                              const uint32_t *__restrict__ sp_first, const uint32_t *__restrict__ sp_count,
-                             const v2 *__restrict__ pts, Edge *__restrict__ edges, uint32_t *__restrict__ edraw) {
+                             const v2 *__restrict__ pts, Edge *__restrict__ edges, uint32_t *__restrict__ edraw,
+                             int4 *__restrict__ bbox, uint2 *__restrict__ erange) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total)
-        return;
-    uint32_t u = upper_bound_u32(fill_off, n_units, i) - 1;
-    Unit un = units[u];
-    const vkh_draw &d = draws[un.draw];
-    uint32_t j = i - fill_off[u];
-    uint32_t n = sp_count[un.subpath];
-    uint32_t f = sp_first[un.subpath];
-    v2 a = pts[f + j], b = pts[f + (j + 1 == n ? 0 : j + 1)];
-    float sw = (float)tgts[d.target].width, sh = (float)tgts[d.target].height;
-    Edge e;
-    snap_vertex(a.x, a.y, d.mat, sw, sh, e.x0, e.y0);
-    snap_vertex(b.x, b.y, d.mat, sw, sh, e.x1, e.y1);
-    bool bad = e.x0 == VKH_INVALID_FIX || e.x1 == VKH_INVALID_FIX || (e.x0 == e.x1 && e.y0 == e.y1);
-    edges[i] = e;
-    edraw[i] = bad ? VKH_NO_DRAW : un.draw;
+    uint32_t dr = VKH_NO_DRAW;
+    Edge e{0, 0, 0, 0};
+    if (i < total) {
+        uint32_t u = upper_bound_u32(fill_off, n_units, i) - 1;
+        Unit un = units[u];
+        const vkh_draw &d = draws[un.draw];
+        uint32_t j = i - fill_off[u];
+        uint32_t n = sp_count[un.subpath];
+        uint32_t f = sp_first[un.subpath];
+        v2 a = pts[f + j], b = pts[f + (j + 1 == n ? 0 : j + 1)];
+        float sw = (float)tgts[d.target].width, sh = (float)tgts[d.target].height;
+        snap_vertex(a.x, a.y, d.mat, sw, sh, e.x0, e.y0);
+        snap_vertex(b.x, b.y, d.mat, sw, sh, e.x1, e.y1);
+        bool bad = e.x0 == VKH_INVALID_FIX || e.x1 == VKH_INVALID_FIX || (e.x0 == e.x1 && e.y0 == e.y1);
+        dr = bad ? VKH_NO_DRAW : un.draw;
+        edges[i] = e;
+        edraw[i] = dr;
+    }
+    wave_draw_bbox(dr, min(e.x0, e.x1), min(e.y0, e.y1), max(e.x0, e.x1), max(e.y0, e.y1), i, i + 1, bbox, erange);
 }
 
 __device__ __forceinline__ bool tri_has_edge(const int32_t (&N)[3][4], int32_t x0, int32_t y0, int32_t x1,
@@ -779,7 +783,8 @@ __global__ __launch_bounds__(256) void k_tri_edges(const Unit *__restrict__ unit
                                                    const vkh_draw *__restrict__ draws,
                                                    const TargetInfo *__restrict__ tgts, const v2 *__restrict__ verts,
                                                    const uint32_t *__restrict__ tris, Edge *__restrict__ edges,
-                                                   uint32_t *__restrict__ edraw) {
+                                                   uint32_t *__restrict__ edraw, uint32_t ebase,
+                                                   int4 *__restrict__ bbox, uint2 *__restrict__ erange) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = t < total_tris;
     uint32_t draw = VKH_NO_DRAW;
@@ -851,20 +856,30 @@ __global__ __launch_bounds__(256) void k_tri_edges(const Unit *__restrict__ unit
                 same_p2[k] = same;
         }
     }
-    if (!live)
-        return;
-    Edge *out = edges + 3 * (size_t)t;
-    uint32_t *od = edraw + 3 * (size_t)t;
     const bool fwd_visible = lane >= 1 && lane <= 61; // t-1 .. t+2 in this wavefront
     const bool bwd_visible = lane >= 2 && lane <= 62; // t-2 .. t+1 in this wavefront
+    int32_t bx0 = INT32_MAX, by0 = INT32_MAX, bx1 = INT32_MIN, by1 = INT32_MIN;
+    uint32_t i0 = 0xFFFFFFFFu, i1 = 0;
+    const uint32_t eb = ebase + 3 * t; // index of the triangle's first edge in the frame's edge list
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const bool fwd = fwd_visible && rev_p1[k] && !rev_m1[k] && !same_p2[k];
         const bool bwd = bwd_visible && rev_m1[k] && !same_m2[k] && !rev_p1[k];
         const bool keep = draw != VKH_NO_DRAW && !fwd && !bwd;
-        out[k] = keep ? Edge{E[k][0], E[k][1], E[k][2], E[k][3]} : Edge{0, 0, 0, 0};
-        od[k] = keep ? draw : VKH_NO_DRAW;
+        if (live) {
+            edges[3 * (size_t)t + k] = keep ? Edge{E[k][0], E[k][1], E[k][2], E[k][3]} : Edge{0, 0, 0, 0};
+            edraw[3 * (size_t)t + k] = keep ? draw : VKH_NO_DRAW;
+        }
+        if (keep) {
+            bx0 = min(bx0, min(E[k][0], E[k][2]));
+            bx1 = max(bx1, max(E[k][0], E[k][2]));
+            by0 = min(by0, min(E[k][1], E[k][3]));
+            by1 = max(by1, max(E[k][1], E[k][3]));
+            i0 = min(i0, eb + k);
+            i1 = eb + k + 1;
+        }
     }
+    wave_draw_bbox(i1 ? draw : VKH_NO_DRAW, bx0, by0, bx1, by1, i0, i1, bbox, erange);
 }
 
 // ---------------------------------------------------------------------------------------- launchers
@@ -944,16 +959,17 @@ void launch_unit_ranges(hipStream_t s, uint32_t n_units, const uint32_t *step_of
 }
 void launch_fill_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *fill_off, uint32_t total,
                        const vkh_draw *draws, const TargetInfo *tg, const uint32_t *spf, const uint32_t *spc,
-                       const v2 *pts, Edge *edges, uint32_t *edraw) {
+                       const v2 *pts, Edge *edges, uint32_t *edraw, int4 *bbox, uint2 *erange) {
     if (total)
         hipLaunchKernelGGL(k_fill_edges, dim3((total + 255) / 256), dim3(256), 0, s, units, n_units, fill_off, total,
-                           draws, tg, spf, spc, pts, edges, edraw);
+                           draws, tg, spf, spc, pts, edges, edraw, bbox, erange);
 }
 void launch_tri_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *u_toff,
                       const uint32_t *u_vbeg, const uint32_t *u_vend, uint32_t total_tris, const vkh_draw *draws,
-                      const TargetInfo *tg, const v2 *verts, const uint32_t *tris, Edge *edges, uint32_t *edraw) {
+                      const TargetInfo *tg, const v2 *verts, const uint32_t *tris, Edge *edges, uint32_t *edraw,
+                      uint32_t ebase, int4 *bbox, uint2 *erange) {
     if (total_tris)
         hipLaunchKernelGGL(k_tri_edges, dim3((total_tris + 255) / 256), dim3(256), 0, s, units, n_units, u_toff,
-                           u_vbeg, u_vend, total_tris, draws, tg, verts, tris, edges, edraw);
+                           u_vbeg, u_vend, total_tris, draws, tg, verts, tris, edges, edraw, ebase, bbox, erange);
 }
 } // namespace vkh

@@ -4,9 +4,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#define VKH_NO_DRAW 0xFFFFFFFFu
+
 #include "vkh_device.h"
 
-#define VKH_NO_DRAW 0xFFFFFFFFu
 #define VKH_BIN_TILES 16        // a coarse bin is 16 x 16 tiles
 #define VKH_COARSE_CHUNK 1024   // draws per coarse workgroup
 #define VKH_FLAT_SLOT 128       // flattened points kept per cubic by the subdivision pass
@@ -122,15 +123,14 @@ void launch_unit_ranges(hipStream_t s, uint32_t n_units, const uint32_t *step_of
                         const uint32_t *st_off, uint32_t *u_toff, uint32_t *u_vbeg, uint32_t *u_vend);
 void launch_fill_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *fill_off, uint32_t total,
                        const vkh_draw *draws, const TargetInfo *tg, const uint32_t *spf, const uint32_t *spc,
-                       const v2 *pts, Edge *edges, uint32_t *edraw);
+                       const v2 *pts, Edge *edges, uint32_t *edraw, int4 *bbox, uint2 *erange);
 void launch_tri_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *u_toff,
                       const uint32_t *u_vbeg, const uint32_t *u_vend, uint32_t total_tris, const vkh_draw *draws,
-                      const TargetInfo *tg, const v2 *verts, const uint32_t *tris, Edge *edges, uint32_t *edraw);
+                      const TargetInfo *tg, const v2 *verts, const uint32_t *tris, Edge *edges, uint32_t *edraw,
+                      uint32_t ebase, int4 *bbox, uint2 *erange);
 
 // raster stage (vkh_raster.hip)
 void launch_draw_bbox_init(hipStream_t s, int4 *bbox, uint2 *erange, uint32_t n_draws);
-void launch_draw_bbox(hipStream_t s, const Edge *edges, const uint32_t *edraw, uint32_t n_edges, int4 *bbox,
-                      uint2 *erange);
 void launch_draw_tiles(hipStream_t s, const vkh_draw *draws, uint32_t n_draws, const TargetInfo *tg,
                        const int4 *bbox, uint32_t tile, DrawTiles *dt, uint32_t *pt_count);
 void launch_set_pt_base(hipStream_t s, DrawTiles *dt, const uint32_t *pt_off, uint32_t n_draws);

@@ -31,49 +31,6 @@ __global__ void k_bbox_init(int4 *bbox, uint2 *erange, uint32_t n) {
     }
 }
 
-// per draw: bbox of its edges and the range [first, last + 1) of its edge indices
-__global__ void k_draw_bbox(const Edge *__restrict__ edges, const uint32_t *__restrict__ edraw, uint32_t n,
-                            int4 *__restrict__ bbox, uint2 *__restrict__ erange) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t d = i < n ? edraw[i] : VKH_NO_DRAW;
-    int32_t x0 = INT32_MAX, y0 = INT32_MAX, x1 = INT32_MIN, y1 = INT32_MIN;
-    if (d != VKH_NO_DRAW) {
-        Edge e = edges[i];
-        x0 = min(e.x0, e.x1);
-        x1 = max(e.x0, e.x1);
-        y0 = min(e.y0, e.y1);
-        y1 = max(e.y0, e.y1);
-    }
-    // one set of atomics per distinct draw of the wavefront (edges of a draw are contiguous, so
-    // a wavefront usually holds one or two draws; dropped edges carry no draw)
-    const uint32_t lane = threadIdx.x & 63;
-    uint64_t pending = __ballot(d != VKH_NO_DRAW);
-    while (pending) {
-        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1;
-        const uint32_t dl = __shfl(d, (int)leader);
-        const bool mine = d == dl;
-        int32_t a = mine ? x0 : INT32_MAX, b = mine ? y0 : INT32_MAX;
-        int32_t c = mine ? x1 : INT32_MIN, e = mine ? y1 : INT32_MIN;
-        for (int o = 32; o > 0; o >>= 1) {
-            a = min(a, __shfl_xor(a, o));
-            b = min(b, __shfl_xor(b, o));
-            c = max(c, __shfl_xor(c, o));
-            e = max(e, __shfl_xor(e, o));
-        }
-        const uint64_t mm = __ballot(mine);
-        if (lane == leader) {
-            atomicMin(&bbox[dl].x, a);
-            atomicMin(&bbox[dl].y, b);
-            atomicMax(&bbox[dl].z, c);
-            atomicMax(&bbox[dl].w, e);
-            const uint32_t wbase = i - lane;
-            atomicMin(&erange[dl].x, wbase + leader);
-            atomicMax(&erange[dl].y, wbase + 64u - (uint32_t)__clzll((long long)mm));
-        }
-        pending &= ~mm;
-    }
-}
-
 __global__ void k_draw_tiles(const vkh_draw *__restrict__ draws, uint32_t n, const TargetInfo *__restrict__ tg,
                              const int4 *__restrict__ bbox, uint32_t T, DrawTiles *__restrict__ dt,
                              uint32_t *__restrict__ pt_count) {
@@ -805,11 +762,6 @@ namespace vkh {
 void launch_draw_bbox_init(hipStream_t s, int4 *bbox, uint2 *erange, uint32_t n) {
     if (n)
         hipLaunchKernelGGL(k_bbox_init, dim3((n + 255) / 256), dim3(256), 0, s, bbox, erange, n);
-}
-void launch_draw_bbox(hipStream_t s, const Edge *edges, const uint32_t *edraw, uint32_t n, int4 *bbox,
-                      uint2 *erange) {
-    if (n)
-        hipLaunchKernelGGL(k_draw_bbox, dim3((n + 255) / 256), dim3(256), 0, s, edges, edraw, n, bbox, erange);
 }
 void launch_draw_tiles(hipStream_t s, const vkh_draw *draws, uint32_t n, const TargetInfo *tg, const int4 *bbox,
                        uint32_t tile, DrawTiles *dt, uint32_t *pt_count) {
