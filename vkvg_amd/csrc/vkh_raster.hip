@@ -282,18 +282,24 @@ __device__ __forceinline__ uint32_t find_target_by(const TargetInfo *tg, uint32_
 // One workgroup per (bin of 16 x 16 tiles, chunk of VKH_COARSE_CHUNK draws), one thread per
 // tile; walks the chunk's draws in order. Counts are laid out tile-major (tile * nch + chunk), so
 // one scan gives every tile a contiguous command list ordered by draw, without any sort.
+// Per 256 draws: the draws whose tile bbox meets the bin are compacted into LDS (in draw order,
+// with their bbox and kind), then every tile walks that list, loading the path tiles of four
+// draws at a time so their latencies overlap.
 __global__ __launch_bounds__(256) void k_coarse(bool write, const TargetInfo *__restrict__ tg, uint32_t n_targets,
                                                 const vkh_draw *__restrict__ draws, const DrawTiles *__restrict__ dts,
                                                 const PathTile *__restrict__ pts, uint32_t nch,
                                                 uint32_t *__restrict__ tile_count, const uint32_t *__restrict__ tile_off,
                                                 Cmd *__restrict__ cmds) {
     __shared__ uint64_t masks[4];
+    __shared__ DrawTiles s_dt[256];
+    __shared__ uint32_t s_draw[256], s_kind[256];
     uint32_t bin = blockIdx.x, chunk = blockIdx.y;
     uint32_t t = find_target_by(tg, n_targets, bin, true);
     const TargetInfo &T = tg[t];
     uint32_t lb = bin - T.bin_base;
     int32_t bxi = (int32_t)(lb % T.binsW), byi = (int32_t)(lb / T.binsW);
     uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63, wv = tid >> 6;
     int32_t tx = bxi * VKH_BIN_TILES + (int32_t)(tid & 15), ty = byi * VKH_BIN_TILES + (int32_t)(tid >> 4);
     bool valid = tx < (int32_t)T.tilesW && ty < (int32_t)T.tilesH;
     uint32_t gtile = T.tile_base + (uint32_t)ty * T.tilesW + (uint32_t)tx;
@@ -306,38 +312,59 @@ __global__ __launch_bounds__(256) void k_coarse(bool write, const TargetInfo *__
     uint32_t count = 0;
     uint32_t out = (write && valid) ? tile_off[(size_t)gtile * nch + chunk] : 0;
     for (uint32_t c0 = d_begin; c0 < d_end; c0 += 256) {
-        uint32_t d = c0 + tid;
+        const uint32_t d = c0 + tid;
         bool hit = false;
+        DrawTiles b;
         if (d < d_end) {
-            DrawTiles b = dts[d];
+            b = dts[d];
             hit = b.bx0 <= b.bx1 && b.bx0 <= bx1 && b.bx1 >= bx0 && b.by0 <= by1 && b.by1 >= by0;
         }
-        uint64_t m = __ballot(hit);
-        if ((tid & 63) == 0)
-            masks[tid >> 6] = m;
+        const uint64_t m = __ballot(hit);
+        if (lane == 0)
+            masks[wv] = m;
         __syncthreads();
-        for (int w = 0; w < 4; w++) {
-            uint64_t mm = masks[w];
-            while (mm) {
-                int j = __ffsll((unsigned long long)mm) - 1;
-                mm &= mm - 1;
-                uint32_t dd = c0 + (uint32_t)(w * 64 + j);
-                if (!valid)
-                    continue;
-                DrawTiles b = dts[dd];
-                if (tx < b.bx0 || tx > b.bx1 || ty < b.by0 || ty > b.by1)
-                    continue;
-                uint32_t kind = draws[dd].kind;
-                uint32_t ptid = VKH_NO_DRAW;
-                bool emit = true;
-                if (kind != VKH_DRAW_PAINT && kind != VKH_DRAW_CLEAR) {
-                    ptid = b.pt_base + (uint32_t)(ty - b.by0) * b.bw + (uint32_t)(tx - b.bx0);
-                    PathTile p = pts[ptid];
-                    emit = p.nseg > 0 || coverage_count(kind, p.backdrop) != 0;
+        uint32_t before = 0, nh = 0;
+        for (uint32_t w = 0; w < 4; w++) {
+            const uint32_t c = (uint32_t)__popcll((unsigned long long)masks[w]);
+            before += w < wv ? c : 0u;
+            nh += c;
+        }
+        if (hit) {
+            const uint32_t r = before + (uint32_t)__popcll((unsigned long long)(m & ((1ull << lane) - 1ull)));
+            s_dt[r] = b;
+            s_draw[r] = d;
+            s_kind[r] = draws[d].kind;
+        }
+        __syncthreads();
+        for (uint32_t h0 = 0; h0 < nh; h0 += 4) {
+            uint32_t ptid[4];
+            PathTile p[4];
+            bool in[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                in[u] = false;
+                ptid[u] = VKH_NO_DRAW;
+                p[u] = PathTile{0, 0, 0, 0};
+                const uint32_t h = h0 + (uint32_t)u;
+                if (h < nh && valid) {
+                    const DrawTiles hb = s_dt[h];
+                    in[u] = tx >= hb.bx0 && tx <= hb.bx1 && ty >= hb.by0 && ty <= hb.by1;
+                    const uint32_t kind = s_kind[h];
+                    if (in[u] && kind != VKH_DRAW_PAINT && kind != VKH_DRAW_CLEAR) {
+                        ptid[u] = hb.pt_base + (uint32_t)(ty - hb.by0) * hb.bw + (uint32_t)(tx - hb.bx0);
+                        p[u] = pts[ptid[u]];
+                    }
                 }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (!in[u])
+                    continue;
+                const uint32_t h = h0 + (uint32_t)u;
+                const bool emit = ptid[u] == VKH_NO_DRAW || p[u].nseg > 0 || coverage_count(s_kind[h], p[u].backdrop) != 0;
                 if (emit) {
                     if (write)
-                        cmds[out + count] = Cmd{dd, ptid};
+                        cmds[out + count] = Cmd{s_draw[h], ptid[u]};
                     count++;
                 }
             }
@@ -673,14 +700,20 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
             }
         }
         __syncthreads();
-        // ---- per (command, row): inclusive prefix with the row's V term
+        // ---- per (command, row): inclusive prefix with the row's V term. The T rows of a command
+        // are T consecutive lanes (nb * T and the loop stride are multiples of T, so a group is
+        // wholly active or inactive): a segmented shuffle scan sums srow[j][1 .. r] (srow[j][0] is 0).
         for (uint32_t q = tid; q < nb * (uint32_t)T; q += NT) {
             const uint32_t j = q / T, r = q % T;
+            int32_t acc = srow[j][r];
+#pragma unroll
+            for (int o = 1; o < T; o <<= 1) {
+                const int32_t y = __shfl_up(acc, (unsigned)o, T);
+                if ((int)r >= o)
+                    acc += y;
+            }
             if (!c_nseg[j])
                 continue;
-            int32_t acc = 0;
-            for (uint32_t rr = 1; rr <= r; rr++)
-                acc += srow[j][rr];
 #pragma unroll
             for (int c = 0; c < T; c++) {
                 acc += sd[j][r][c];
