@@ -827,7 +827,7 @@ struct Buf {
 
 struct vkvg_diff_scene_t {
     uint32_t W = 0, H = 0, n_paths = 0, n_pts = 0, n_ctrl = 0, n_edges = 0, tilesW = 0, tilesH = 0, n_tiles = 0;
-    uint32_t bin_total = 0, nch = 1;
+    uint32_t bin_total = 0, nch = 1, chunk = VKH_COARSE_CHUNK;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
     std::string err;
@@ -900,7 +900,6 @@ VkvgDiffScene vkvg_diff_create(uint32_t width, uint32_t height, uint32_t n_paths
     s->tilesW = (width + VKD_T - 1) / VKD_T;
     s->tilesH = (height + VKD_T - 1) / VKD_T;
     s->n_tiles = s->tilesW * s->tilesH;
-    s->nch = (n_paths + VKH_COARSE_CHUNK - 1) / VKH_COARSE_CHUNK;
     std::vector<uint32_t> first_pt(n_paths + 1), first_ctrl(n_paths + 1), first_edge(n_paths + 1), pt_path, ctrl_path;
     std::vector<uint32_t> stroke(n_paths), nc(n_paths);
     uint64_t np = 0, nctl = 0, ne = 0;
@@ -952,6 +951,8 @@ VkvgDiffScene vkvg_diff_create(uint32_t width, uint32_t height, uint32_t n_paths
     ti.draw_begin = 0;
     ti.draw_end = n_paths;
     s->bin_total = ti.binsW * ti.binsH;
+    s->chunk = vkh::coarse_chunk(n_paths, s->bin_total);
+    s->nch = (n_paths + s->chunk - 1) / s->chunk;
     bool ok = ck(s, hipStreamCreate(&s->stream), "stream");
     for (int k = 0; ok && k < 4; k++)
         ok = ck(s, hipEventCreate(&s->ev[k]), "event");
@@ -1055,7 +1056,7 @@ static int forward_impl(VkvgDiffScene s, const float *ctrl, const float *rgba, c
     }
     const uint32_t n_tc = s->n_tiles * s->nch;
     zero(s, s->tile_cnt, n_tc + 1);
-    vkh::launch_coarse(st, false, s->bin_total, s->nch, P<TargetInfo>(s->tg), 1, P<vkh_draw>(s->draws),
+    vkh::launch_coarse(st, false, s->bin_total, s->nch, s->chunk, P<TargetInfo>(s->tg), 1, P<vkh_draw>(s->draws),
                        P<DrawTiles>(s->dtiles), P<PathTile>(s->ptiles), P<uint32_t>(s->tile_cnt), nullptr, nullptr);
     if (!scan(s, P<uint32_t>(s->tile_cnt), n_tc, P<uint32_t>(s->tile_off)))
         return -1;
@@ -1063,7 +1064,7 @@ static int forward_impl(VkvgDiffScene s, const float *ctrl, const float *rgba, c
         return -1;
     if (!ensure(s, s->cmds, sizeof(Cmd) * ((size_t)s->n_cmds + 1)))
         return -1;
-    vkh::launch_coarse(st, true, s->bin_total, s->nch, P<TargetInfo>(s->tg), 1, P<vkh_draw>(s->draws),
+    vkh::launch_coarse(st, true, s->bin_total, s->nch, s->chunk, P<TargetInfo>(s->tg), 1, P<vkh_draw>(s->draws),
                        P<DrawTiles>(s->dtiles), P<PathTile>(s->ptiles), nullptr, P<uint32_t>(s->tile_off),
                        P<Cmd>(s->cmds));
     float4 *ck_ptr = nullptr;

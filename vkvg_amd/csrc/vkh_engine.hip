@@ -149,7 +149,8 @@ struct vkh_engine {
     bool resident = false;
     size_t o_items, o_sp, o_draws, o_styles, o_dashes, o_grads, o_tg, o_units;
     uint32_t n_items, n_sp, n_draws, n_styles, n_dashes, n_grads, n_targets, n_units;
-    uint32_t tile_total, bin_total, frame_tile, coarse_nch = 1, n_cubic_items = 0;
+    uint32_t tile_total, bin_total, frame_tile, coarse_nch = 1, coarse_chunk = VKH_COARSE_CHUNK, n_cubic_items = 0;
+    uint32_t most_draws = 1; // most draws on one target
 };
 
 static bool check(vkh_engine *e, hipError_t r, const char *what) {
@@ -379,6 +380,12 @@ void vkvg_hip_get_stats(vkh_engine *e, vkh_stats *out) {
 }
 const char *vkvg_hip_last_error(vkh_engine *e) { return e ? e->err.c_str() : "null engine"; }
 
+// coarse work split for the current tiling: draws per chunk and chunks per bin
+static void set_coarse_chunks(vkh_engine *eng) {
+    eng->coarse_chunk = vkh::coarse_chunk(eng->most_draws, eng->bin_total);
+    eng->coarse_nch = (eng->most_draws + eng->coarse_chunk - 1) / eng->coarse_chunk;
+}
+
 int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
     if (!eng || !f)
         return -1;
@@ -514,12 +521,10 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
     eng->n_cubic_items = 0;
     for (uint32_t i = 0; i < n_items; i++)
         eng->n_cubic_items += (f->items[i].flags & VKH_ITEM_CUBIC) ? 1u : 0u;
-    {
-        uint32_t most = 1;
-        for (const TargetInfo &ti : tg)
-            most = std::max(most, ti.draw_end - ti.draw_begin);
-        eng->coarse_nch = (most + VKH_COARSE_CHUNK - 1) / VKH_COARSE_CHUNK;
-    }
+    eng->most_draws = 1;
+    for (const TargetInfo &ti : tg)
+        eng->most_draws = std::max(eng->most_draws, ti.draw_end - ti.draw_begin);
+    set_coarse_chunks(eng);
     eng->frame_gen++;
     eng->gfail_key = 0;
     eng->resident = true;
@@ -730,7 +735,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     if (!need(eng, cap, eng->tile_cnt, 4ull * (n_tc + 1)) || !need(eng, cap, eng->tile_off, 4ull * (n_tc + 1)))
         return -1;
     vkh::launch_zero_u32(s, P<uint32_t>(eng->tile_cnt), n_tc + 1);
-    vkh::launch_coarse(s, false, bin_total, nch, d_tg, n_targets, d_draws, P<DrawTiles>(eng->dtiles),
+    vkh::launch_coarse(s, false, bin_total, nch, eng->coarse_chunk, d_tg, n_targets, d_draws, P<DrawTiles>(eng->dtiles),
                        P<PathTile>(eng->ptiles), P<uint32_t>(eng->tile_cnt), nullptr, nullptr);
     if (scan_u32(eng, P<uint32_t>(eng->tile_cnt), n_tc, P<uint32_t>(eng->tile_off), cap))
         return -1;
@@ -741,7 +746,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     st.tiles = tile_total;
     if (!need(eng, cap, eng->cmds, sizeof(Cmd) * (size_t)n_cmds + 16))
         return -1;
-    vkh::launch_coarse(s, true, bin_total, nch, d_tg, n_targets, d_draws, P<DrawTiles>(eng->dtiles),
+    vkh::launch_coarse(s, true, bin_total, nch, eng->coarse_chunk, d_tg, n_targets, d_draws, P<DrawTiles>(eng->dtiles),
                        P<PathTile>(eng->ptiles), nullptr, P<uint32_t>(eng->tile_off), P<Cmd>(eng->cmds));
     CK(hipGetLastError());
 
@@ -886,6 +891,7 @@ int vkvg_hip_retile(vkh_engine *eng, uint32_t T) {
     eng->tile_total = tile_total;
     eng->bin_total = bin_total;
     eng->frame_tile = T;
+    set_coarse_chunks(eng);
     eng->frame_gen++;
     eng->gfail_key = 0;
     return 0;

@@ -9,7 +9,7 @@
 #include "vkh_device.h"
 
 #define VKH_BIN_TILES 16        // a coarse bin is 16 x 16 tiles
-#define VKH_COARSE_CHUNK 1024   // draws per coarse workgroup
+#define VKH_COARSE_CHUNK 1024   // most draws per coarse workgroup
 #define VKH_FLAT_SLOT 128       // flattened points kept per cubic by the subdivision pass
 
 // one (draw, subpath) pair of the frame; fills and strokes are expanded per unit
@@ -141,7 +141,9 @@ void launch_bin_draws(hipStream_t s, const Edge *edges, const uint32_t *edraw, c
 void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n_draws, const TargetInfo *tg,
                       const vkh_gradient *grads, uint32_t n_grads, DrawPrep *out, GradRec *rec,
                       GradPrep *gout);
-void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, uint32_t nch, const TargetInfo *tg,
+// draws per coarse workgroup: about 1024 workgroups over n_bins bins, 64 .. VKH_COARSE_CHUNK draws
+uint32_t coarse_chunk(uint32_t most_draws, uint32_t n_bins);
+void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, uint32_t nch, uint32_t chunk, const TargetInfo *tg,
                    uint32_t n_targets, const vkh_draw *draws, const DrawTiles *dt, const PathTile *pts,
                    uint32_t *tile_count, const uint32_t *tile_off, Cmd *cmds);
 void launch_fine(hipStream_t s, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,

@@ -19,6 +19,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 // Tiles are T = 1 << lt pixels, so a tile spans 1 << (lt + 8) in 24.8 fixed point and floor
 // division by the span is an arithmetic shift.
 
@@ -279,7 +281,7 @@ __device__ __forceinline__ uint32_t find_target_by(const TargetInfo *tg, uint32_
     return lo - 1;
 }
 
-// One workgroup per (bin of 16 x 16 tiles, chunk of VKH_COARSE_CHUNK draws), one thread per
+// One workgroup per (bin of 16 x 16 tiles, chunk of `chunk` draws), one thread per
 // tile; walks the chunk's draws in order. Counts are laid out tile-major (tile * nch + chunk), so
 // one scan gives every tile a contiguous command list ordered by draw, without any sort.
 // Per 256 draws: the draws whose tile bbox meets the bin are compacted into LDS (in draw order,
@@ -287,7 +289,7 @@ __device__ __forceinline__ uint32_t find_target_by(const TargetInfo *tg, uint32_
 // draws at a time so their latencies overlap.
 __global__ __launch_bounds__(256) void k_coarse(bool write, const TargetInfo *__restrict__ tg, uint32_t n_targets,
                                                 const vkh_draw *__restrict__ draws, const DrawTiles *__restrict__ dts,
-                                                const PathTile *__restrict__ pts, uint32_t nch,
+                                                const PathTile *__restrict__ pts, uint32_t nch, uint32_t chunk_draws,
                                                 uint32_t *__restrict__ tile_count, const uint32_t *__restrict__ tile_off,
                                                 Cmd *__restrict__ cmds) {
     __shared__ uint64_t masks[4];
@@ -305,10 +307,10 @@ __global__ __launch_bounds__(256) void k_coarse(bool write, const TargetInfo *__
     uint32_t gtile = T.tile_base + (uint32_t)ty * T.tilesW + (uint32_t)tx;
     int32_t bx0 = bxi * VKH_BIN_TILES, by0 = byi * VKH_BIN_TILES;
     int32_t bx1 = bx0 + VKH_BIN_TILES - 1, by1 = by0 + VKH_BIN_TILES - 1;
-    const uint32_t d_begin = T.draw_begin + chunk * VKH_COARSE_CHUNK;
+    const uint32_t d_begin = T.draw_begin + chunk * chunk_draws;
     if (d_begin >= T.draw_end)
         return; // uniform over the workgroup
-    const uint32_t d_end = min(T.draw_end, d_begin + VKH_COARSE_CHUNK);
+    const uint32_t d_end = min(T.draw_end, d_begin + chunk_draws);
     uint32_t count = 0;
     uint32_t out = (write && valid) ? tile_off[(size_t)gtile * nch + chunk] : 0;
     for (uint32_t c0 = d_begin; c0 < d_end; c0 += 256) {
@@ -817,12 +819,18 @@ void launch_bin_draws(hipStream_t s, const Edge *edges, const uint32_t *edraw, c
         hipLaunchKernelGGL(k_bin_draw, dim3(n_draws), dim3(256), 0, s, edges, edraw, erange, dt, tile, pts, segs,
                            seg_cap, seg_alloc);
 }
-void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, uint32_t nch, const TargetInfo *tg,
+uint32_t coarse_chunk(uint32_t most_draws, uint32_t n_bins) {
+    const uint32_t want = (1024 + std::max(n_bins, 1u) - 1) / std::max(n_bins, 1u); // chunks per bin
+    uint32_t c = (most_draws + want - 1) / want;
+    c = (c + 63) / 64 * 64;
+    return std::min<uint32_t>(std::max<uint32_t>(c, 64), VKH_COARSE_CHUNK);
+}
+void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, uint32_t nch, uint32_t chunk, const TargetInfo *tg,
                    uint32_t n_targets, const vkh_draw *draws, const DrawTiles *dt, const PathTile *pts,
                    uint32_t *tile_count, const uint32_t *tile_off, Cmd *cmds) {
     if (n_bins && nch)
         hipLaunchKernelGGL(k_coarse, dim3(n_bins, nch), dim3(256), 0, s, write, tg, n_targets, draws, dt, pts, nch,
-                           tile_count, tile_off, cmds);
+                           chunk, tile_count, tile_off, cmds);
 }
 void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n, const TargetInfo *tg,
                       const vkh_gradient *grads, uint32_t n_grads, DrawPrep *out, GradRec *rec,
