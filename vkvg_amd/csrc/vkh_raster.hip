@@ -95,6 +95,7 @@ __global__ void k_set_pt_base(DrawTiles *dt, const uint32_t *pt_off, uint32_t n)
 // whose range does not fit the segment buffer writes no segments; the host sees the allocator
 // beyond the capacity, grows the buffer and bins again.
 #define VKH_BIN_LDS_TILES 4096
+#define VKH_BIN_U 4 // edges per thread per load step
 
 template <bool SCATTER>
 __device__ __forceinline__ void bin_edge(const Edge &e, const DrawTiles &dt, int32_t lt, int32_t ty_lo,
@@ -194,6 +195,38 @@ __global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges
     const uint2 er = erange[d];
     const int32_t lt = (int32_t)__builtin_ctz(T);
     const uint32_t bw = dt.bw, bh = (uint32_t)(dt.by1 - dt.by0 + 1);
+    // The draw's edges, VKH_BIN_U per thread per step with every load issued before any is used.
+    // A draw of at most 256 * VKH_BIN_U edges (most of them) keeps its edges in registers for both
+    // passes of every band; a larger one streams them each pass.
+    const bool cached = er.y - er.x <= 256u * VKH_BIN_U;
+    uint32_t cdr[VKH_BIN_U];
+    Edge ce[VKH_BIN_U];
+    auto load = [&](uint32_t b0) {
+#pragma unroll
+        for (int u = 0; u < VKH_BIN_U; u++) {
+            const uint32_t i = b0 + (uint32_t)u * 256u + tid;
+            cdr[u] = i < er.y ? edraw[i] : VKH_NO_DRAW;
+            ce[u] = i < er.y ? edges[i] : Edge{0, 0, 0, 0};
+        }
+    };
+    auto for_edges = [&](auto &&f) {
+        if (cached) {
+#pragma unroll
+            for (int u = 0; u < VKH_BIN_U; u++)
+                if (cdr[u] == d)
+                    f(ce[u]);
+            return;
+        }
+        for (uint32_t b0 = er.x; b0 < er.y; b0 += 256u * VKH_BIN_U) {
+            load(b0);
+#pragma unroll
+            for (int u = 0; u < VKH_BIN_U; u++)
+                if (cdr[u] == d)
+                    f(ce[u]);
+        }
+    };
+    if (cached)
+        load(er.x);
     const uint32_t band_rows = max(1u, VKH_BIN_LDS_TILES / bw); // bw <= 4096 (width <= 32768, T >= 8)
     for (uint32_t r0 = 0; r0 < bh; r0 += band_rows) {
         const uint32_t nr = min(band_rows, bh - r0);
@@ -204,12 +237,7 @@ __global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges
             s_back[k] = 0;
         }
         __syncthreads();
-        for (uint32_t i = er.x + tid; i < er.y; i += 256) {
-            if (edraw[i] != d)
-                continue;
-            const Edge e = edges[i];
-            bin_edge<false>(e, dt, lt, ty_lo, ty_hi, s_cnt, s_back, nullptr, 0);
-        }
+        for_edges([&](const Edge &e) { bin_edge<false>(e, dt, lt, ty_lo, ty_hi, s_cnt, s_back, nullptr, 0); });
         __syncthreads();
         const uint32_t total = block_exclusive_scan(s_cnt, nt, s_wave);
         if (tid == 0)
@@ -236,14 +264,8 @@ __global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges
             out[k] = p;
         }
         __syncthreads();
-        if (fits) {
-            for (uint32_t i = er.x + tid; i < er.y; i += 256) {
-                if (edraw[i] != d)
-                    continue;
-                const Edge e = edges[i];
-                bin_edge<true>(e, dt, lt, ty_lo, ty_hi, s_cnt, s_back, segs, base);
-            }
-        }
+        if (fits)
+            for_edges([&](const Edge &e) { bin_edge<true>(e, dt, lt, ty_lo, ty_hi, s_cnt, s_back, segs, base); });
         __syncthreads();
     }
 }
