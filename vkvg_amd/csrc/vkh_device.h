@@ -229,6 +229,77 @@ __device__ void agg_flatten(const Cubic &root, float tol, Emit emit) {
     }
 }
 
+// The same traversal with the ancestors of the current node on a stack (levels 0 .. D-1 kept by the
+// caller, e.g. in LDS): every move costs one split. The parent of a right sibling at `level` is
+// read from the stack, or, beyond level D-1, recomputed from the level D-1 ancestor along the path
+// bits; all nodes come from the same chain of splits as in the recursion.
+template <int D, typename Emit, typename Store, typename Load>
+__device__ void agg_flatten_stack(const Cubic &root, float tol, Emit emit, Store store, Load load) {
+    uint64_t path_lo = 0, path_hi = 0; // bit (l-1): the node at level l is a right child
+    int level = 1;
+    store(0, root);
+    Cubic node;
+    agg_split(root, false, node);
+    while (true) {
+        bool leaf;
+        if (level > 100) {
+            leaf = true; // recursion limit: returns without emitting
+        } else {
+            v2 pts[2];
+            const int n = agg_leaf(node, tol, pts);
+            leaf = n > 0;
+            if (n > 0)
+                emit(pts[0]);
+            if (n > 1)
+                emit(pts[1]);
+        }
+        if (!leaf) {
+            if (level < D)
+                store(level, node);
+            Cubic ch;
+            agg_split(node, false, ch);
+            node = ch;
+            level++;
+            continue;
+        }
+        // pop right children, then step to the right sibling of the first left child
+        while (level >= 1) {
+            const int b = level - 1;
+            const bool is_right = b < 64 ? ((path_lo >> b) & 1ull) : ((path_hi >> (b - 64)) & 1ull);
+            if (!is_right)
+                break;
+            if (b < 64)
+                path_lo &= ~(1ull << b);
+            else
+                path_hi &= ~(1ull << (b - 64));
+            level--;
+        }
+        if (level < 1)
+            return;
+        {
+            const int b = level - 1;
+            if (b < 64)
+                path_lo |= (1ull << b);
+            else
+                path_hi |= (1ull << (b - 64));
+        }
+        Cubic par;
+        if (level - 1 < D) {
+            par = load(level - 1);
+        } else {
+            par = load(D - 1);
+            for (int l = D; l <= level - 1; l++) {
+                const int b = l - 1;
+                const bool r = b < 64 ? ((path_lo >> b) & 1ull) : ((path_hi >> (b - 64)) & 1ull);
+                Cubic nx;
+                agg_split(par, r, nx);
+                par = nx;
+            }
+        }
+        agg_split(par, true, node);
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Vertex transform and 1/256 snap (shaders/vkvg_main.vert:74-79 + viewport transform).
 #define VKH_INVALID_FIX ((int32_t)0x80000000)

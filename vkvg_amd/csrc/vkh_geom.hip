@@ -75,11 +75,15 @@ __global__ __launch_bounds__(256) void k_flat_scatter(const vkh_item *__restrict
         perm[base[key] + rank] = i;
 }
 
-__global__ void k_flatten_stage(const vkh_item *__restrict__ items, uint32_t n, const uint32_t *__restrict__ perm,
+#define VKH_FLAT_STACK 8   // ancestor levels kept in LDS per thread by the subdivision pass
+#define VKH_FLAT_BLOCK 128 // threads per workgroup of the subdivision pass (32 KB of stacks)
+
+__global__ __launch_bounds__(VKH_FLAT_BLOCK) void k_flatten_stage(const vkh_item *__restrict__ items, uint32_t n, const uint32_t *__restrict__ perm,
                                 uint32_t *__restrict__ counts, uint32_t *__restrict__ slot_alloc,
                                 uint32_t *__restrict__ slot_item, uint32_t *__restrict__ item_slot,
                                 v2 *__restrict__ scratch, uint32_t *__restrict__ ovf_count,
                                 uint32_t *__restrict__ ovf_list) {
+    __shared__ float4 s_stk[VKH_FLAT_STACK][2][VKH_FLAT_BLOCK]; // ancestors, two float4 per cubic
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t i = t < n ? perm[t] : 0u;
     vkh_item it;
@@ -110,11 +114,21 @@ __global__ void k_flatten_stage(const vkh_item *__restrict__ items, uint32_t n, 
     v2 *out = scratch + (size_t)slot * VKH_FLAT_SLOT;
     Cubic cb{it.p[0], it.p[1], it.p[2], it.p[3], it.p[4], it.p[5], it.p[6], it.p[7]};
     uint32_t k = 0;
-    agg_flatten(cb, it.tol, [&](v2 p) {
-        if (k < VKH_FLAT_SLOT)
-            out[k] = p;
-        k++;
-    });
+    agg_flatten_stack<VKH_FLAT_STACK>(
+        cb, it.tol,
+        [&](v2 p) {
+            if (k < VKH_FLAT_SLOT)
+                out[k] = p;
+            k++;
+        },
+        [&](int l, const Cubic &c) {
+            s_stk[l][0][threadIdx.x] = make_float4(c.x1, c.y1, c.x2, c.y2);
+            s_stk[l][1][threadIdx.x] = make_float4(c.x3, c.y3, c.x4, c.y4);
+        },
+        [&](int l) {
+            const float4 a = s_stk[l][0][threadIdx.x], b = s_stk[l][1][threadIdx.x];
+            return Cubic{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        });
     if (!(it.flags & VKH_ITEM_NO_END)) {
         if (k < VKH_FLAT_SLOT)
             out[k] = V(it.p[6], it.p[7]);
@@ -896,7 +910,8 @@ void launch_flatten_stage(hipStream_t s, const vkh_item *items, uint32_t n, cons
                           uint32_t *slot_alloc, uint32_t *slot_item, uint32_t *item_slot, v2 *scratch,
                           uint32_t *ovf_count, uint32_t *ovf_list) {
     if (n)
-        hipLaunchKernelGGL(k_flatten_stage, dim3((n + 255) / 256), dim3(256), 0, s, items, n, perm, counts, slot_alloc,
+        hipLaunchKernelGGL(k_flatten_stage, dim3((n + VKH_FLAT_BLOCK - 1) / VKH_FLAT_BLOCK), dim3(VKH_FLAT_BLOCK), 0, s,
+                           items, n, perm, counts, slot_alloc,
                            slot_item, item_slot, scratch, ovf_count, ovf_list);
 }
 void launch_flatten_copy(hipStream_t s, const vkh_item *items, uint32_t n_slots, const uint32_t *slot_item,
