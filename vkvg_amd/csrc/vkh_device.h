@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #include "vkvg_hip.h"
+#include "vkh_fatan.h"
 
 #define VKH_PIF   3.14159265358979323846f
 #define VKH_PIF_2 1.57079632679489661923f
@@ -145,6 +146,99 @@ __device__ __forceinline__ int agg_leaf(const Cubic &c, float tol, v2 *pts) {
     return 0;
 }
 
+// The same leaf test with the float arctangent of vkh_fatan.h. Each of its angles differs from the
+// double-rounded one by at most E = VKH_ATAN2_ERR + 2^-22; a difference of two angles (below 2 pi)
+// by at most 2E + 2^-21 after both roundings, the wrapped value (2/pi - d, below 8 in magnitude)
+// by 2^-21 more, and a sum of two wrapped values (below 16) by twice that plus 2^-20. A comparison
+// is taken only when its operand lies further than that from the threshold; otherwise, or when an
+// argument is outside the fast arctangent's range, the result is -1 and the caller runs agg_leaf.
+// Any other result is the one agg_leaf gives.
+#define VKH_ANG_E (VKH_ATAN2_ERR + 0x1p-22f)
+#define VKH_ANG_M1 (2.f * VKH_ANG_E + 0x1p-21f)
+#define VKH_ANG_M1W (VKH_ANG_M1 + 0x1p-21f)
+#define VKH_ANG_M2 (2.f * VKH_ANG_M1W + 0x1p-20f)
+__device__ __forceinline__ int agg_leaf_fast(const Cubic &c, float tol, v2 *pts) {
+    float x1 = c.x1, y1 = c.y1, x2 = c.x2, y2 = c.y2, x3 = c.x3, y3 = c.y3, x4 = c.x4, y4 = c.y4;
+    float dx = x4 - x1, dy = y4 - y1;
+    float d2 = fabsf(((x2 - x4) * dy - (y2 - y4) * dx));
+    float d3 = fabsf(((x3 - x4) * dy - (y3 - y4) * dx));
+    if ((double)d2 > 1.7 && (double)d3 > 1.7) {
+        if ((d2 + d3) * (d2 + d3) <= (dx * dx + dy * dy) * tol) {
+            int ok = 1;
+            const float a23 = vkh_atan2_fast(y3 - y2, x3 - x2, &ok);
+            float da1 = fabsf(a23 - vkh_atan2_fast(y2 - y1, x2 - x1, &ok));
+            float da2 = fabsf(vkh_atan2_fast(y4 - y3, x4 - x3, &ok) - a23);
+            if (!ok || fabsf(da1 - VKH_PIF) <= VKH_ANG_M1 || fabsf(da2 - VKH_PIF) <= VKH_ANG_M1)
+                return -1;
+            if (da1 >= VKH_PIF)
+                da1 = VKH_2_PIF - da1;
+            if (da2 >= VKH_PIF)
+                da2 = VKH_2_PIF - da2;
+            const float sum = da1 + da2;
+            if (fabsf(sum - (float)0.01) <= VKH_ANG_M2)
+                return -1;
+            if (sum < (float)0.01) {
+                float x12 = (x1 + x2) / 2, y12 = (y1 + y2) / 2, x23 = (x2 + x3) / 2, y23 = (y2 + y3) / 2;
+                float x34 = (x3 + x4) / 2, y34 = (y3 + y4) / 2;
+                float x123 = (x12 + x23) / 2, y123 = (y12 + y23) / 2, x234 = (x23 + x34) / 2, y234 = (y23 + y34) / 2;
+                pts[0] = V((x123 + x234) / 2, (y123 + y234) / 2);
+                return 1;
+            }
+            if (fabs((double)da1 - 0.01) <= (double)VKH_ANG_M1W)
+                return -1;
+            if ((double)da1 > 0.01) {
+                pts[0] = V(x2, y2);
+                return 1;
+            }
+            if (fabs((double)da2 - 0.01) <= (double)VKH_ANG_M1W)
+                return -1;
+            if ((double)da2 > 0.01) {
+                pts[0] = V(x3, y3);
+                return 1;
+            }
+        }
+    } else {
+        const bool first = (double)d2 > 1.7;
+        if (first || (double)d3 > 1.7) {
+            const float dd = first ? d2 : d3;
+            if (dd * dd <= tol * (dx * dx + dy * dy)) {
+                int ok = 1;
+                // first: angle(2-3) - angle(1-2); second: angle(3-4) - angle(2-3)
+                const float ua = first ? vkh_atan2_fast(y3 - y2, x3 - x2, &ok) : vkh_atan2_fast(y4 - y3, x4 - x3, &ok);
+                const float ub = first ? vkh_atan2_fast(y2 - y1, x2 - x1, &ok) : vkh_atan2_fast(y3 - y2, x3 - x2, &ok);
+                float da1 = fabsf(ua - ub);
+                if (!ok || fabsf(da1 - VKH_PIF) <= VKH_ANG_M1)
+                    return -1;
+                if (da1 >= VKH_PIF)
+                    da1 = VKH_2_PIF - da1;
+                if (fabs((double)da1 - 0.01) <= (double)VKH_ANG_M1W)
+                    return -1;
+                if ((double)da1 < 0.01) {
+                    pts[0] = V(x2, y2);
+                    pts[1] = V(x3, y3);
+                    return 2;
+                }
+                if ((double)da1 > 0.01) {
+                    pts[0] = first ? V(x2, y2) : V(x3, y3);
+                    return 1;
+                }
+            }
+        } else {
+            float x12 = (x1 + x2) / 2, y12 = (y1 + y2) / 2, x23 = (x2 + x3) / 2, y23 = (y2 + y3) / 2;
+            float x34 = (x3 + x4) / 2, y34 = (y3 + y4) / 2;
+            float x123 = (x12 + x23) / 2, y123 = (y12 + y23) / 2, x234 = (x23 + x34) / 2, y234 = (y23 + y34) / 2;
+            float x1234 = (x123 + x234) / 2, y1234 = (y123 + y234) / 2;
+            float ex = x1234 - (x1 + x4) / 2;
+            float ey = y1234 - (y1 + y4) / 2;
+            if (ex * ex + ey * ey <= tol) {
+                pts[0] = V(x1234, y1234);
+                return 1;
+            }
+        }
+    }
+    return 0;
+}
+
 // Depth-first traversal; calls emit(v2) for every emitted point in the reference's order.
 // Path bits: bit (level-1) of `path` is 1 when the node at `level` is a right child. The parent
 // and grandparent of the current node are kept, so moving to the sibling or to the parent's
@@ -246,7 +340,9 @@ __device__ void agg_flatten_stack(const Cubic &root, float tol, Emit emit, Store
             leaf = true; // recursion limit: returns without emitting
         } else {
             v2 pts[2];
-            const int n = agg_leaf(node, tol, pts);
+            int n = agg_leaf_fast(node, tol, pts);
+            if (n < 0)
+                n = agg_leaf(node, tol, pts);
             leaf = n > 0;
             if (n > 0)
                 emit(pts[0]);
