@@ -162,13 +162,22 @@ __device__ __forceinline__ int agg_leaf_fast(const Cubic &c, float tol, v2 *pts)
     float dx = x4 - x1, dy = y4 - y1;
     float d2 = fabsf(((x2 - x4) * dy - (y2 - y4) * dx));
     float d3 = fabsf(((x3 - x4) * dy - (y3 - y4) * dx));
-    if ((double)d2 > 1.7 && (double)d3 > 1.7) {
-        if ((d2 + d3) * (d2 + d3) <= (dx * dx + dy * dy) * tol) {
-            int ok = 1;
-            const float a23 = vkh_atan2_fast(y3 - y2, x3 - x2, &ok);
-            float da1 = fabsf(a23 - vkh_atan2_fast(y2 - y1, x2 - x1, &ok));
-            float da2 = fabsf(vkh_atan2_fast(y4 - y3, x4 - x3, &ok) - a23);
-            if (!ok || fabsf(da1 - VKH_PIF) <= VKH_ANG_M1 || fabsf(da2 - VKH_PIF) <= VKH_ANG_M1)
+    const bool b2 = (double)d2 > 1.7, b3 = (double)d3 > 1.7;
+    const float len2 = dx * dx + dy * dy;
+    // the reference's cases: both control points off the chord (f1), only the first (f2), only the
+    // second (f3), each when flat enough; the three leg angles are computed once for all of them
+    const bool f1 = b2 && b3 && (d2 + d3) * (d2 + d3) <= len2 * tol;
+    const bool f2 = b2 && !b3 && d2 * d2 <= tol * len2;
+    const bool f3 = !b2 && b3 && d3 * d3 <= tol * len2;
+    if (f1 || f2 || f3) {
+        int ok12 = 1, ok23 = 1, ok34 = 1;
+        const float a12 = vkh_atan2_fast(y2 - y1, x2 - x1, &ok12);
+        const float a23 = vkh_atan2_fast(y3 - y2, x3 - x2, &ok23);
+        const float a34 = vkh_atan2_fast(y4 - y3, x4 - x3, &ok34);
+        if (f1) {
+            float da1 = fabsf(a23 - a12);
+            float da2 = fabsf(a34 - a23);
+            if (!(ok12 && ok23 && ok34) || fabsf(da1 - VKH_PIF) <= VKH_ANG_M1 || fabsf(da2 - VKH_PIF) <= VKH_ANG_M1)
                 return -1;
             if (da1 >= VKH_PIF)
                 da1 = VKH_2_PIF - da1;
@@ -196,44 +205,37 @@ __device__ __forceinline__ int agg_leaf_fast(const Cubic &c, float tol, v2 *pts)
                 pts[0] = V(x3, y3);
                 return 1;
             }
+            return 0;
         }
-    } else {
-        const bool first = (double)d2 > 1.7;
-        if (first || (double)d3 > 1.7) {
-            const float dd = first ? d2 : d3;
-            if (dd * dd <= tol * (dx * dx + dy * dy)) {
-                int ok = 1;
-                // first: angle(2-3) - angle(1-2); second: angle(3-4) - angle(2-3)
-                const float ua = first ? vkh_atan2_fast(y3 - y2, x3 - x2, &ok) : vkh_atan2_fast(y4 - y3, x4 - x3, &ok);
-                const float ub = first ? vkh_atan2_fast(y2 - y1, x2 - x1, &ok) : vkh_atan2_fast(y3 - y2, x3 - x2, &ok);
-                float da1 = fabsf(ua - ub);
-                if (!ok || fabsf(da1 - VKH_PIF) <= VKH_ANG_M1)
-                    return -1;
-                if (da1 >= VKH_PIF)
-                    da1 = VKH_2_PIF - da1;
-                if (fabs((double)da1 - 0.01) <= (double)VKH_ANG_M1W)
-                    return -1;
-                if ((double)da1 < 0.01) {
-                    pts[0] = V(x2, y2);
-                    pts[1] = V(x3, y3);
-                    return 2;
-                }
-                if ((double)da1 > 0.01) {
-                    pts[0] = first ? V(x2, y2) : V(x3, y3);
-                    return 1;
-                }
-            }
-        } else {
-            float x12 = (x1 + x2) / 2, y12 = (y1 + y2) / 2, x23 = (x2 + x3) / 2, y23 = (y2 + y3) / 2;
-            float x34 = (x3 + x4) / 2, y34 = (y3 + y4) / 2;
-            float x123 = (x12 + x23) / 2, y123 = (y12 + y23) / 2, x234 = (x23 + x34) / 2, y234 = (y23 + y34) / 2;
-            float x1234 = (x123 + x234) / 2, y1234 = (y123 + y234) / 2;
-            float ex = x1234 - (x1 + x4) / 2;
-            float ey = y1234 - (y1 + y4) / 2;
-            if (ex * ex + ey * ey <= tol) {
-                pts[0] = V(x1234, y1234);
-                return 1;
-            }
+        // f2: angle(2-3) - angle(1-2); f3: angle(3-4) - angle(2-3)
+        float da1 = f2 ? fabsf(a23 - a12) : fabsf(a34 - a23);
+        if (!(ok23 && (f2 ? ok12 : ok34)) || fabsf(da1 - VKH_PIF) <= VKH_ANG_M1)
+            return -1;
+        if (da1 >= VKH_PIF)
+            da1 = VKH_2_PIF - da1;
+        if (fabs((double)da1 - 0.01) <= (double)VKH_ANG_M1W)
+            return -1;
+        if ((double)da1 < 0.01) {
+            pts[0] = V(x2, y2);
+            pts[1] = V(x3, y3);
+            return 2;
+        }
+        if ((double)da1 > 0.01) {
+            pts[0] = V(f2 ? x2 : x3, f2 ? y2 : y3);
+            return 1;
+        }
+        return 0;
+    }
+    if (!b2 && !b3) {
+        float x12 = (x1 + x2) / 2, y12 = (y1 + y2) / 2, x23 = (x2 + x3) / 2, y23 = (y2 + y3) / 2;
+        float x34 = (x3 + x4) / 2, y34 = (y3 + y4) / 2;
+        float x123 = (x12 + x23) / 2, y123 = (y12 + y23) / 2, x234 = (x23 + x34) / 2, y234 = (y23 + y34) / 2;
+        float x1234 = (x123 + x234) / 2, y1234 = (y123 + y234) / 2;
+        float ex = x1234 - (x1 + x4) / 2;
+        float ey = y1234 - (y1 + y4) / 2;
+        if (ex * ex + ey * ey <= tol) {
+            pts[0] = V(x1234, y1234);
+            return 1;
         }
     }
     return 0;
