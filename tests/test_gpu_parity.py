@@ -55,3 +55,8 @@ def test_clear(device):
 @pytest.mark.parametrize("tile", [8, 32])
 def test_feature_mix_tiles(device, tile):
     _check(device, scenes.feature_mix(3, size=300, n=48), tile=tile)
+
+
+def test_long_runs_of_one_draw_kind(device):
+    # fill-edge and stroke-triangle runs spanning more than 64 draws per wavefront
+    _check(device, scenes.runs_scene())

@@ -746,6 +746,46 @@ __device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *a, uint32_t 
     return lo;
 }
 
+// upper_bound_u32(a, n, t) - 1 for the wavefront's consecutive indices t = t0 + lane, with the
+// dependent global loads of one search shared by the wavefront: a 64-way search (each lane probes
+// one position, a ballot narrows the range 64 times) finds lane 0's entry in about log64(n)
+// round trips; each lane then finds its own among the next 64 entries by a search over lanes
+// (shuffles). A wavefront spanning more than 64 entries finishes those lanes with the plain search.
+// Every lane of the wavefront must call it.
+__device__ __forceinline__ uint32_t wave_entry_of(const uint32_t *__restrict__ a, uint32_t n, uint32_t t) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t t0 = (uint32_t)__shfl((int)t, 0);
+    uint32_t lo = 0, hi = n; // upper_bound(t0) in [lo, hi]
+    while (hi - lo > 64) {
+        const uint32_t step = (hi - lo + 63) / 64;
+        const uint32_t p = lo + (lane + 1) * step;
+        const bool le = p < hi && a[p] <= t0;
+        const uint32_t c = (uint32_t)__popcll((unsigned long long)__ballot(le)); // probes <= t0 form a prefix
+        const uint32_t nlo = c ? lo + c * step + 1 : lo;
+        const uint32_t pn = lo + (c + 1) * step;
+        hi = (c < 64 && pn < hi) ? pn : hi;
+        lo = nlo;
+    }
+    {
+        const uint32_t p = lo + lane;
+        const bool le = p < hi && a[p] <= t0;
+        lo += (uint32_t)__popcll((unsigned long long)__ballot(le));
+    }
+    const uint32_t u0 = lo - 1; // entry of t0
+    const uint32_t q = u0 + 1 + lane;
+    const uint32_t v = q < n ? a[q] : 0xFFFFFFFFu; // starts of the next 64 entries, ascending over lanes
+    uint32_t k = 0;
+#pragma unroll
+    for (int sh = 32; sh > 0; sh >>= 1) {
+        const uint32_t vv = (uint32_t)__shfl((int)v, (int)(k + (uint32_t)sh - 1));
+        if (vv <= t)
+            k += (uint32_t)sh;
+    }
+    if (k == 63 && (uint32_t)__shfl((int)v, 63) <= t) // beyond the 64 entries
+        return upper_bound_u32(a, n, t) - 1;
+    return u0 + k;
+}
+
 __global__ void k_fill_edges(const Unit *__restrict__ units, uint32_t n_units, const uint32_t *__restrict__ fill_off,
                              uint32_t total, const vkh_draw *__restrict__ draws, const TargetInfo *__restrict__ tgts,
                              const uint32_t *__restrict__ sp_first, const uint32_t *__restrict__ sp_count,
@@ -754,8 +794,8 @@ __global__ void k_fill_edges(const Unit *__restrict__ units, uint32_t n_units, c
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t dr = VKH_NO_DRAW;
     Edge e{0, 0, 0, 0};
+    const uint32_t u = wave_entry_of(fill_off, n_units, i);
     if (i < total) {
-        uint32_t u = upper_bound_u32(fill_off, n_units, i) - 1;
         Unit un = units[u];
         const vkh_draw &d = draws[un.draw];
         uint32_t j = i - fill_off[u];
@@ -803,8 +843,8 @@ __global__ __launch_bounds__(256) void k_tri_edges(const Unit *__restrict__ unit
     const bool live = t < total_tris;
     uint32_t draw = VKH_NO_DRAW;
     int32_t X[3] = {0, 0, 0}, Y[3] = {0, 0, 0};
+    const uint32_t u = wave_entry_of(u_toff, n_units, t);
     if (live) {
-        uint32_t u = upper_bound_u32(u_toff, n_units, t) - 1;
         Unit un = units[u];
         const vkh_draw &d = draws[un.draw];
         uint32_t vb = u_vbeg[u], ve = u_vend[u];

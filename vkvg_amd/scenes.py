@@ -338,6 +338,46 @@ def clear_scene(size: int = 96) -> Scene:
     return Scene("clear", size, size, s, 2)
 
 
+def runs_scene(size: int = 256, runs: int = 4, run_len: int = 150) -> Scene:
+    """Long runs of draws of one kind between single draws of the other: run_len short strokes
+    (no fill edges) then one filled triangle, and run_len small fills (no stroke triangles) then one
+    stroke, repeated. A 64-wide run of consecutive fill edges (or stroke triangles) then spans more
+    than 64 draws, the case the edge kernels' per-wavefront draw lookup hands to a plain search."""
+    rng = Rng(0x5EED0000 + 16 * 7)
+    s = Stream()
+    _white(s)
+    n = 0
+    for _ in range(runs):
+        s.set_line_width(1.5)
+        for _ in range(run_len):
+            x, y = rng.uniform(4, size - 4, 2)
+            s.set_source_rgba(*rng.u(3), 0.8)
+            s.move_to(x, y)
+            s.line_to(x + rng.uniform(-6, 6), y + rng.uniform(-6, 6))
+            s.stroke()
+            n += 1
+        x, y = rng.uniform(10, size - 10, 2)
+        s.set_source_rgba(*rng.u(3), 0.6)
+        s.move_to(x, y)
+        s.line_to(x + 9, y + 2)
+        s.line_to(x + 3, y + 8)
+        s.fill()
+        n += 1
+        for _ in range(run_len):
+            x, y = rng.uniform(4, size - 8, 2)
+            s.set_source_rgba(*rng.u(3), 0.7)
+            s.rectangle(x, y, rng.uniform(0.5, 4), rng.uniform(0.5, 4))
+            s.fill()
+            n += 1
+        x, y = rng.uniform(10, size - 10, 2)
+        s.set_source_rgba(*rng.u(3), 0.9)
+        s.move_to(x, y)
+        s.line_to(x + 12, y + 5)
+        s.stroke()
+        n += 1
+    return Scene("runs", size, size, s, n)
+
+
 @dataclass
 class DiffScene:
     """Differentiable-render scene: topology plus FP32 parameters and the target's control points."""
