@@ -139,13 +139,14 @@ __global__ __launch_bounds__(VKH_FLAT_BLOCK) void k_flatten_stage(const vkh_item
         ovf_list[atomicAdd(ovf_count, 1u)] = i;
 }
 
-// one wavefront per slot: its points to their final positions
+// VKH_FLAT_COPY_LANES lanes per slot (a cubic has about 10 points): its points to their final positions
+#define VKH_FLAT_COPY_LANES 16
 __global__ void k_flatten_copy(const vkh_item *__restrict__ items, uint32_t n_slots,
                                const uint32_t *__restrict__ slot_item, const uint32_t *__restrict__ counts,
                                const uint32_t *__restrict__ offs, const v2 *__restrict__ scratch,
                                v2 *__restrict__ pts, uint8_t *__restrict__ pflags) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t slot = (uint32_t)(t >> 6), lane = (uint32_t)(t & 63);
+    const uint32_t slot = (uint32_t)(t / VKH_FLAT_COPY_LANES), lane = (uint32_t)(t % VKH_FLAT_COPY_LANES);
     if (slot >= n_slots)
         return;
     const uint32_t i = slot_item[slot];
@@ -155,7 +156,7 @@ __global__ void k_flatten_copy(const vkh_item *__restrict__ items, uint32_t n_sl
     const uint32_t o = offs[i];
     const uint8_t f = (items[i].flags & VKH_ITEM_CURVED) ? 1 : 0;
     const v2 *src = scratch + (size_t)slot * VKH_FLAT_SLOT;
-    for (uint32_t k = lane; k < n; k += 64) {
+    for (uint32_t k = lane; k < n; k += VKH_FLAT_COPY_LANES) {
         pts[o + k] = src[k];
         pflags[o + k] = f;
     }
@@ -956,7 +957,7 @@ void launch_flatten_stage(hipStream_t s, const vkh_item *items, uint32_t n, cons
 }
 void launch_flatten_copy(hipStream_t s, const vkh_item *items, uint32_t n_slots, const uint32_t *slot_item,
                          const uint32_t *counts, const uint32_t *offs, const v2 *scratch, v2 *pts, uint8_t *pf) {
-    const size_t threads = (size_t)n_slots * 64;
+    const size_t threads = (size_t)n_slots * VKH_FLAT_COPY_LANES;
     if (threads)
         hipLaunchKernelGGL(k_flatten_copy, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, items, n_slots,
                            slot_item, counts, offs, scratch, pts, pf);
