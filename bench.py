@@ -37,16 +37,21 @@ def _sync():
         pass
 
 
-def _read_traffic(name: str):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary."""
+def _read_pmc(name: str, key: str):
+    """A per-launch figure of the dominant kernel from the committed rocprofv3 --pmc summary."""
     p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
     try:
         with open(p) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
+            return json.load(f).get(key)
     except Exception:
         return None
+
+
+def _read_traffic(name: str):
+    """HBM bytes per launch of the dominant kernel."""
+    return _read_pmc(name, "hbm_bytes_per_launch")
 
 
 def _timed(dev, vkvg_amd, steps):
@@ -219,6 +224,15 @@ def main() -> int:
     fine_bytes = 4 * sc.width * sc.height + 32 * sc.n_paths + 24 * sc.gradients + 20 * sc.gradient_stops
     achieved = fine_bytes / (fine_ms / 1e3) / 1e9
     traffic = _read_traffic("fine_c3_pmc.json")
+    # The fine kernel is limited by vector instruction issue, not by HBM: a wave64 VALU instruction
+    # occupies a 16-lane SIMD for 4 cycles, so the chip issues at most 1024 SIMDs x 2.4 GHz / 4 wave
+    # instructions per second. Its VALU instructions per launch come from the same counter summary.
+    valu = _read_pmc("fine_c3_pmc.json", "valu_insts_per_launch")
+    issue_peak = 1024 * 2.4e9 / 4
+    issue = None if valu is None else {"achieved": valu / (fine_ms / 1e3), "peak": issue_peak,
+                                       "unit": "wave64 VALU instructions/s",
+                                       "frac": valu / (fine_ms / 1e3) / issue_peak,
+                                       "valu_per_launch": valu}
     out = {
         "metric": METRIC,
         "value": paths_per_s / 1e6,
@@ -249,7 +263,8 @@ def main() -> int:
         "counts": {k: st[k] for k in ("points", "stroke_triangles", "edges", "path_tiles", "segments", "commands")},
         "roofline": {"kernel": f"k_fine<{st['tile_size']},{st['fine_words']}>", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": fine_bytes, "avg_launch_ms": fine_ms},
+                     "algorithmic_bytes_per_launch": fine_bytes, "avg_launch_ms": fine_ms,
+                     "vector_issue": issue},
     }
     if not args.no_batch:
         out["batched"] = bench_c4(args, dev, vkvg_amd, scenes)
