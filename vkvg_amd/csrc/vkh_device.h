@@ -471,6 +471,16 @@ __device__ __forceinline__ uint8_t f2u8(float f) {
     // NaN and f <= 0 give 0 (fmaxf returns the non-NaN operand), f >= 1 gives 255
     return (uint8_t)rintf(fminf(fmaxf(f, 0.f), 1.f) * 255.0f);
 }
+// Four colour channels -> packed premultiplied RGBA8, each byte f2u8 of its channel, with the
+// hardware conversion: v_cvt_pk_u8_f32(x * 255) rounds to nearest even and saturates to [0, 255]
+// (NaN -> 0), which equals f2u8(x) for every float x (checked over all 2^32 bit patterns by
+// vkh_check_u8_conversion, run by tests/test_gpu_convert.py).
+__device__ __forceinline__ uint32_t f2u8x4(float r, float g, float b, float a) {
+    uint32_t s = __builtin_amdgcn_cvt_pk_u8_f32(r * 255.0f, 0, 0u);
+    s = __builtin_amdgcn_cvt_pk_u8_f32(g * 255.0f, 1, s);
+    s = __builtin_amdgcn_cvt_pk_u8_f32(b * 255.0f, 2, s);
+    return __builtin_amdgcn_cvt_pk_u8_f32(a * 255.0f, 3, s);
+}
 __device__ __forceinline__ uint32_t draw_key(uint32_t n) {
     uint32_t x = n + 0x9E3779B9u;
     x ^= x >> 16;

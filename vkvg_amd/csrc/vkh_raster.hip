@@ -770,8 +770,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
                     const float op_ = P.opacity;
                     float col[4];
                     gradient_color(P, grads, gprep, gx[q], gy[q], col);
-                    S = vkh_src16_of((uint32_t)f2u8(col[0] * op_) | ((uint32_t)f2u8(col[1] * op_) << 8) |
-                                     ((uint32_t)f2u8(col[2] * op_) << 16) | ((uint32_t)f2u8(col[3] * op_) << 24));
+                    S = vkh_src16_of(f2u8x4(col[0] * op_, col[1] * op_, col[2] * op_, col[3] * op_));
                 }
                 // a stroke pixel covered by k triangles is blended k times
                 uint32_t v = px[q], sg = sig[q];
@@ -794,6 +793,21 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
         if (TI.sig)
             TI.sig[o] += sig[q];
     }
+}
+
+// Diagnostic: f2u8x4's byte conversions against f2u8 over all 2^32 float bit patterns, in every
+// byte position (2^24 threads x 256 values); counts mismatches.
+__global__ void k_check_u8_conversion(unsigned long long *__restrict__ bad) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long nb = 0;
+    for (uint32_t j = 0; j < 256; j++) {
+        const float x = __uint_as_float((uint32_t)(t * 256u + j));
+        const uint32_t want = f2u8(x);
+        const uint32_t got = f2u8x4(x, x, x, x);
+        nb += (got != want * 0x01010101u) ? 1u : 0u;
+    }
+    if (nb)
+        atomicAdd(bad, nb);
 }
 
 // readback with the reference's truncating unpremultiply (src/vkvg_surface.c:441-452)
@@ -905,6 +919,9 @@ void launch_fine(hipStream_t s, uint32_t tile, uint32_t words, uint32_t n_tiles,
         fine_words<16>(s, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     }
+}
+void launch_check_u8_conversion(hipStream_t s, unsigned long long *bad) {
+    hipLaunchKernelGGL(k_check_u8_conversion, dim3(1u << 16), dim3(256), 0, s, bad);
 }
 void launch_unpremultiply(hipStream_t s, const uint8_t *src, uint8_t *dst, size_t npix) {
     if (npix)
