@@ -106,7 +106,12 @@ VKH_HD uint32_t blend_px16(uint32_t op, uint32_t d, vkh_src16 S) {
         return 0;
     const vkh_u16x2 h = {128, 128};
     const vkh_u16x2 t_lo = vkh_u16x2_of(d & 0x00FF00FFu) * S.ia + h;
+#if defined(__HIP_DEVICE_COMPILE__)
+    // bytes 1 and 3 of d into the low bytes of the 16-bit lanes in one v_perm_b32
+    const vkh_u16x2 t_hi = vkh_u16x2_of(__builtin_amdgcn_perm(0u, d, 0x0c030c01u)) * S.ia + h;
+#else
     const vkh_u16x2 t_hi = vkh_u16x2_of((d >> 8) & 0x00FF00FFu) * S.ia + h;
+#endif
     const vkh_u16x2 m_lo = (t_lo + (t_lo >> 8)) >> 8, m_hi = (t_hi + (t_hi >> 8)) >> 8;
     vkh_u16x2 v_lo, v_hi;
     if (op == 3) {
