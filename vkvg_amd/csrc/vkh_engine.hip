@@ -129,7 +129,7 @@ struct vkh_engine {
     DevBuf fe, nv, nt, fe_off, nv_off, nt_off, verts, tris, edges, edraw;
     DevBuf bbox, erange, dtiles, pt_count, pt_off, ptiles, counters, segs, dprep; // counters: [0] segment allocator, [1] flatten slots, [2] overflow cubics, [64..191] flatten order
     DevBuf tile_cnt, tile_off, cmds, scan_tmp, readback, item_slot, slot_item, flat_scratch, flat_perm, flat_ovf;
-    DevBuf step_nv, step_nt, sv_off, st_off, u_toff, u_vbeg, u_vend;
+    DevBuf step_nv, step_nt, sv_off, st_off, u_toff, u_vbeg, u_vend, dash_st; // dash_st: per dashed piece, its starting dash state
     HostBuf hblob, htot;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     hipEvent_t ev_stage[6] = {};
@@ -266,7 +266,7 @@ void vkvg_hip_engine_destroy(vkh_engine *e) {
                       &e->verts, &e->tris, &e->edges, &e->edraw, &e->bbox, &e->dtiles, &e->pt_count, &e->pt_off,
                       &e->erange, &e->ptiles, &e->counters, &e->segs, &e->dprep, &e->tile_cnt,
                       &e->tile_off, &e->cmds, &e->scan_tmp, &e->readback, &e->step_nv, &e->step_nt,
-                      &e->sv_off, &e->st_off, &e->u_toff, &e->u_vbeg, &e->u_vend, &e->item_slot,
+                      &e->sv_off, &e->st_off, &e->dash_st, &e->u_toff, &e->u_vbeg, &e->u_vend, &e->item_slot,
                       &e->slot_item, &e->flat_scratch, &e->flat_perm, &e->flat_ovf};
     for (DevBuf *b : bufs)
         if (b->p)
@@ -615,8 +615,8 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
         !need(eng, cap, eng->u_vend, un4))
         return -1;
     uint32_t *d_steps = P<uint32_t>(eng->nv), *d_step_off = P<uint32_t>(eng->nv_off);
-    vkh::launch_unit_count(s, d_units, n_units, d_draws, P<uint32_t>(eng->sp_count), d_styles, P<uint32_t>(eng->fe),
-                           d_steps);
+    vkh::launch_unit_count(s, d_units, n_units, d_draws, d_sp, P<uint32_t>(eng->sp_count), d_styles,
+                           P<uint32_t>(eng->fe), d_steps);
     if (scan_u32(eng, P<uint32_t>(eng->fe), n_units, P<uint32_t>(eng->fe_off), cap) ||
         scan_u32(eng, d_steps, n_units, d_step_off, cap))
         return -1;
@@ -625,11 +625,12 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     const uint32_t n_fill_edges = tot[0], n_steps = tot[1];
     size_t sn4 = 4ull * (n_steps + 1);
     if (!need(eng, cap, eng->step_nv, sn4) || !need(eng, cap, eng->step_nt, sn4) || !need(eng, cap, eng->sv_off, sn4) ||
-        !need(eng, cap, eng->st_off, sn4))
+        !need(eng, cap, eng->st_off, sn4) || !need(eng, cap, eng->dash_st, 2 * sn4))
         return -1;
     vkh::launch_stroke_steps(s, false, d_units, n_units, d_step_off, n_steps, d_draws, d_sp, P<uint32_t>(eng->sp_first),
                              P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes,
-                             P<uint32_t>(eng->step_nv), P<uint32_t>(eng->step_nt), nullptr, nullptr, nullptr, nullptr);
+                             P<uint32_t>(eng->step_nv), P<uint32_t>(eng->step_nt), nullptr, nullptr, nullptr, nullptr,
+                             P<uint2>(eng->dash_st));
     if (scan_u32(eng, P<uint32_t>(eng->step_nv), n_steps, P<uint32_t>(eng->sv_off), cap) ||
         scan_u32(eng, P<uint32_t>(eng->step_nt), n_steps, P<uint32_t>(eng->st_off), cap))
         return -1;
@@ -652,7 +653,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     vkh::launch_stroke_steps(s, true, d_units, n_units, d_step_off, n_steps, d_draws, d_sp, P<uint32_t>(eng->sp_first),
                              P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes,
                              nullptr, nullptr, P<uint32_t>(eng->sv_off), P<uint32_t>(eng->st_off), P<v2>(eng->verts),
-                             P<uint32_t>(eng->tris));
+                             P<uint32_t>(eng->tris), P<uint2>(eng->dash_st));
     vkh::launch_stroke_patch(s, d_units, n_units, d_step_off, d_draws, d_sp, P<uint32_t>(eng->sp_first),
                              P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes,
                              P<uint32_t>(eng->sv_off), P<uint32_t>(eng->st_off), P<uint32_t>(eng->tris));

@@ -1,9 +1,10 @@
 // Geometry stage of the vkvg HIP backend: flattening, stroke expansion, edge generation.
 //
-//   flatten      one thread per path item; cubics run the AGG subdivision twice (count, then
-//                emit) with a prefix sum between the passes for output allocation
-//                (src/vkvg_context.c:541-566, src/vkvg_context_internal.c:1313-1461)
-//   stroke       one thread per (stroke draw, subpath): joins, caps and dashes emitted into
+//   flatten      one thread per path item in work-sorted order; cubics run the AGG subdivision
+//                once into per-cubic scratch slots, a prefix sum places them, a copy pass moves
+//                them (src/vkvg_context.c:541-566, src/vkvg_context_internal.c:1313-1461)
+//   stroke       one thread per step: the join or cap at one point of a solid sub-path, one
+//                segment piece of a dashed sub-path; counted, prefix-summed, then emitted into
 //                vertex / triangle buffers exactly as the reference's caches fill
 //                (src/vkvg_context.c:822-948, src/vkvg_context_internal.c:924-1282)
 //   edges        fills: closed polygon edges of every subpath with more than 2 points;
@@ -489,75 +490,51 @@ __device__ void dashed_segment(const StrokeCtx &c, S &s, DashState &dc, uint32_t
     dc.curDashOffset = fmodf(dc.curDashOffset, dc.totDashLength);
 }
 
-// one subpath of _stroke_preserve (src/vkvg_context.c:836-945); the join at point i uses the
-// curve flag of the segment holding point i, the closing join never does.
-template <typename S>
-__device__ void stroke_subpath(const StrokeCtx &c, bool closed, float dashOffset, S &s) {
-    uint32_t n = c.n;
-    if (n < 2)
-        return;
-    uint32_t last = n - 1;
-    uint32_t firstIdx = s.nv;
-    uint32_t iL, iR = 0, cp = 0;
+// A dashed sub-path of _stroke_preserve (src/vkvg_context.c:836-945) in pieces: piece k < last is
+// the dashed segment from point k (iL is the last point for k = 0, as there), a closed path adds
+// the closing segment as piece `last`, and the last piece also emits the final cap. Run one after
+// another over all pieces, dash_start / dash_piece perform the reference's operations in its order;
+// the counting pass records the dash state at each piece's start so that the pieces can be
+// emitted independently.
+__device__ __forceinline__ uint32_t dash_pieces(uint32_t n, bool closed) { return n < 2 ? 0u : (closed ? n : n - 1); }
+
+__device__ __forceinline__ DashState dash_start(const StrokeCtx &c, float dashOffset) {
     DashState dc;
-    dc.dashOn = false;
+    dc.dashOn = true;
     dc.curDash = 0;
-    dc.curDashOffset = 0;
     dc.totDashLength = 0;
+    for (uint32_t i = 0; i < c.dashCount; i++)
+        dc.totDashLength += c.dashes[i];
+    dc.curDashOffset = fmodf(dashOffset, dc.totDashLength);
     dc.normal = V(0, 0);
-    bool dashed = c.dashCount > 0;
-    if (dashed) {
-        dc.dashOn = true;
-        for (uint32_t i = 0; i < c.dashCount; i++)
-            dc.totDashLength += c.dashes[i];
-        dc.curDashOffset = fmodf(dashOffset, dc.totDashLength);
-        iL = last;
-    } else if (closed) {
-        iL = last;
-    } else {
-        v2 a = c.P[0], b = c.P[1];
-        v2 d = V(b.x - a.x, b.y - a.y);
-        float md = sqrtf(d.x * d.x + d.y * d.y);
-        d.x /= md;
-        d.y /= md;
-        stroke_cap(c, s, a, d, true);
-        iL = 0;
-        cp = 1;
-    }
-    while (cp < last) {
-        iR = cp + 1;
-        bool curved = c.F[cp] != 0;
-        if (dashed)
-            dashed_segment(c, s, dc, iL, cp, iR, curved);
-        else
-            vb_step(c, s, iL, cp, iR, curved);
-        iL = cp++;
-    }
-    if (dashed) {
-        if (closed) {
-            iR = 0;
-            dashed_segment(c, s, dc, iL, cp, iR, false);
-        }
-        if (!dc.dashOn) {
-            int32_t prevDash = (int32_t)dc.curDash - 1;
-            if (prevDash < 0)
-                dc.curDash = c.dashCount - 1;
-            float dprev = c.dashes[prevDash < 0 ? (int32_t)c.dashCount - 1 : prevDash];
-            float m = fminf(dprev - dc.curDashOffset, c.dashes[dc.curDash]);
-            v2 p = vsub(c.P[iR], vmul(dc.normal, m));
-            stroke_cap(c, s, p, dc.normal, false);
-        }
-    } else if (closed) {
+    return dc;
+}
+
+template <typename S>
+__device__ void dash_piece(const StrokeCtx &c, bool closed, uint32_t k, DashState &dc, S &s) {
+    const uint32_t last = c.n - 1;
+    uint32_t iL, cp, iR;
+    bool curved;
+    if (k < last) {
+        cp = k;
+        iR = k + 1;
+        iL = k == 0 ? last : k - 1;
+        curved = c.F[cp] != 0;
+    } else { // closing segment of a closed path
+        cp = last;
         iR = 0;
-        bool inverse = vb_step(c, s, iL, cp, iR, false);
-        s.patch(firstIdx, inverse);
-    } else {
-        v2 a = c.P[cp - 1], b = c.P[cp];
-        v2 d = V(b.x - a.x, b.y - a.y);
-        float md = sqrtf(d.x * d.x + d.y * d.y);
-        d.x /= md;
-        d.y /= md;
-        stroke_cap(c, s, b, d, false);
+        iL = last - 1;
+        curved = false;
+    }
+    dashed_segment(c, s, dc, iL, cp, iR, curved);
+    if (k + 1 == dash_pieces(c.n, closed) && !dc.dashOn) {
+        int32_t prevDash = (int32_t)dc.curDash - 1;
+        if (prevDash < 0)
+            dc.curDash = c.dashCount - 1;
+        float dprev = c.dashes[prevDash < 0 ? (int32_t)c.dashCount - 1 : prevDash];
+        float m = fminf(dprev - dc.curDashOffset, c.dashes[dc.curDash]);
+        v2 p = vsub(c.P[iR], vmul(dc.normal, m));
+        stroke_cap(c, s, p, dc.normal, false);
     }
 }
 
@@ -604,9 +581,10 @@ __device__ bool stroke_step(const StrokeCtx &c, bool closed, uint32_t i, S &s) {
 }
 
 // per unit: fill -> number of polygon edges; stroke -> number of work steps (one per point for
-// a solid stroke, one for a whole dashed sub-path)
+// a solid stroke, one per piece of a dashed sub-path)
 __global__ void k_unit_count(const Unit *__restrict__ units, uint32_t n_units, const vkh_draw *__restrict__ draws,
-                             const uint32_t *__restrict__ sp_count, const vkh_stroke_style *__restrict__ styles,
+                             const vkh_subpath *__restrict__ sps, const uint32_t *__restrict__ sp_count,
+                             const vkh_stroke_style *__restrict__ styles,
                              uint32_t *__restrict__ fill_edges, uint32_t *__restrict__ steps) {
     uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= n_units)
@@ -616,7 +594,8 @@ __global__ void k_unit_count(const Unit *__restrict__ units, uint32_t n_units, c
     uint32_t n = sp_count[un.subpath];
     if (d.kind == VKH_DRAW_STROKE) {
         fill_edges[u] = 0;
-        steps[u] = n < 2 ? 0u : (styles[d.style].dash_count ? 1u : n);
+        const bool closed = (sps[un.subpath].flags & VKH_SP_CLOSED) != 0;
+        steps[u] = n < 2 ? 0u : (styles[d.style].dash_count ? dash_pieces(n, closed) : n);
     } else {
         fill_edges[u] = n > 2 ? n : 0;
         steps[u] = 0;
@@ -644,7 +623,7 @@ __global__ void k_stroke_steps(const Unit *__restrict__ units, uint32_t n_units,
                                const float *__restrict__ dashes, uint32_t *__restrict__ step_nv,
                                uint32_t *__restrict__ step_nt, const uint32_t *__restrict__ sv_off,
                                const uint32_t *__restrict__ st_off, v2 *__restrict__ verts,
-                               uint32_t *__restrict__ tris) {
+                               uint32_t *__restrict__ tris, uint2 *__restrict__ dash_st) {
     uint32_t sidx = blockIdx.x * blockDim.x + threadIdx.x;
     if (sidx >= n_steps)
         return;
@@ -656,11 +635,24 @@ __global__ void k_stroke_steps(const Unit *__restrict__ units, uint32_t n_units,
     bool closed = (sps[un.subpath].flags & VKH_SP_CLOSED) != 0;
     uint32_t i = sidx - step_off[u];
     if (!EMIT) {
+        if (st.dash_count) {
+            // the first piece's thread walks the whole dashed sub-path in order, recording each
+            // piece's starting dash state and output counts
+            if (i != 0)
+                return;
+            DashState dc = dash_start(c, st.dash_offset);
+            const uint32_t np = dash_pieces(c.n, closed);
+            for (uint32_t k = 0; k < np; k++) {
+                dash_st[sidx + k] = make_uint2((dc.dashOn ? 1u : 0u) | (dc.curDash << 1), __float_as_uint(dc.curDashOffset));
+                CountSink ks;
+                dash_piece(c, closed, k, dc, ks);
+                step_nv[sidx + k] = ks.nv;
+                step_nt[sidx + k] = ks.ni / 3;
+            }
+            return;
+        }
         CountSink k;
-        if (st.dash_count)
-            stroke_subpath(c, closed, st.dash_offset, k);
-        else
-            stroke_step(c, closed, i, k);
+        stroke_step(c, closed, i, k);
         step_nv[sidx] = k.nv;
         step_nt[sidx] = k.ni / 3;
     } else {
@@ -669,10 +661,16 @@ __global__ void k_stroke_steps(const Unit *__restrict__ units, uint32_t n_units,
         k.nv = sv_off[sidx];
         k.idx = tris + 3 * (size_t)st_off[sidx];
         k.ni = 0;
-        if (st.dash_count)
-            stroke_subpath(c, closed, st.dash_offset, k);
-        else
+        if (st.dash_count) {
+            DashState dc = dash_start(c, st.dash_offset);
+            const uint2 ds = dash_st[sidx];
+            dc.dashOn = (ds.x & 1u) != 0;
+            dc.curDash = ds.x >> 1;
+            dc.curDashOffset = __uint_as_float(ds.y);
+            dash_piece(c, closed, i, dc, k);
+        } else {
             stroke_step(c, closed, i, k);
+        }
     }
 }
 
@@ -979,26 +977,27 @@ void launch_subpath_ranges(hipStream_t s, const vkh_subpath *sp, uint32_t n, con
         hipLaunchKernelGGL(k_subpath_ranges, dim3((n + 255) / 256), dim3(256), 0, s, sp, n, item_off, first, count);
 }
 void launch_unit_count(hipStream_t s, const Unit *units, uint32_t n_units, const vkh_draw *draws,
-                       const uint32_t *spc, const vkh_stroke_style *styles, uint32_t *fe, uint32_t *steps) {
+                       const vkh_subpath *sps, const uint32_t *spc, const vkh_stroke_style *styles, uint32_t *fe,
+                       uint32_t *steps) {
     if (n_units)
-        hipLaunchKernelGGL(k_unit_count, dim3((n_units + 255) / 256), dim3(256), 0, s, units, n_units, draws, spc,
+        hipLaunchKernelGGL(k_unit_count, dim3((n_units + 255) / 256), dim3(256), 0, s, units, n_units, draws, sps, spc,
                            styles, fe, steps);
 }
 void launch_stroke_steps(hipStream_t s, bool emit, const Unit *units, uint32_t n_units, const uint32_t *step_off,
                          uint32_t n_steps, const vkh_draw *draws, const vkh_subpath *sps, const uint32_t *spf,
                          const uint32_t *spc, const v2 *pts, const uint8_t *pf, const vkh_stroke_style *styles,
                          const float *dashes, uint32_t *step_nv, uint32_t *step_nt, const uint32_t *sv_off,
-                         const uint32_t *st_off, v2 *verts, uint32_t *tris) {
+                         const uint32_t *st_off, v2 *verts, uint32_t *tris, uint2 *dash_st) {
     if (!n_steps)
         return;
     if (emit)
         hipLaunchKernelGGL(k_stroke_steps<true>, dim3((n_steps + 255) / 256), dim3(256), 0, s, units, n_units, step_off,
                            n_steps, draws, sps, spf, spc, pts, pf, styles, dashes, step_nv, step_nt, sv_off, st_off,
-                           verts, tris);
+                           verts, tris, dash_st);
     else
         hipLaunchKernelGGL(k_stroke_steps<false>, dim3((n_steps + 255) / 256), dim3(256), 0, s, units, n_units,
                            step_off, n_steps, draws, sps, spf, spc, pts, pf, styles, dashes, step_nv, step_nt, sv_off,
-                           st_off, verts, tris);
+                           st_off, verts, tris, dash_st);
 }
 void launch_stroke_patch(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *step_off,
                          const vkh_draw *draws, const vkh_subpath *sps, const uint32_t *spf, const uint32_t *spc,

@@ -109,12 +109,13 @@ void launch_flatten_overflow(hipStream_t s, const vkh_item *items, const uint32_
 void launch_subpath_ranges(hipStream_t s, const vkh_subpath *sp, uint32_t n, const uint32_t *item_off,
                            uint32_t *first, uint32_t *count);
 void launch_unit_count(hipStream_t s, const Unit *units, uint32_t n_units, const vkh_draw *draws,
-                       const uint32_t *spc, const vkh_stroke_style *styles, uint32_t *fe, uint32_t *steps);
+                       const vkh_subpath *sps, const uint32_t *spc, const vkh_stroke_style *styles, uint32_t *fe,
+                       uint32_t *steps);
 void launch_stroke_steps(hipStream_t s, bool emit, const Unit *units, uint32_t n_units, const uint32_t *step_off,
                          uint32_t n_steps, const vkh_draw *draws, const vkh_subpath *sps, const uint32_t *spf,
                          const uint32_t *spc, const v2 *pts, const uint8_t *pf, const vkh_stroke_style *styles,
                          const float *dashes, uint32_t *step_nv, uint32_t *step_nt, const uint32_t *sv_off,
-                         const uint32_t *st_off, v2 *verts, uint32_t *tris);
+                         const uint32_t *st_off, v2 *verts, uint32_t *tris, uint2 *dash_st);
 void launch_stroke_patch(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *step_off,
                          const vkh_draw *draws, const vkh_subpath *sps, const uint32_t *spf, const uint32_t *spc,
                          const v2 *pts, const uint8_t *pf, const vkh_stroke_style *styles, const float *dashes,
