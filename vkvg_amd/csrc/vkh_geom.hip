@@ -538,6 +538,34 @@ __device__ void dash_piece(const StrokeCtx &c, bool closed, uint32_t k, DashStat
     }
 }
 
+// The dash state after piece k without emitting anything: dashed_segment's state updates only
+// (the join and caps it emits do not change the state), with the same float operations.
+__device__ __forceinline__ void dash_walk_piece(const StrokeCtx &c, uint32_t k, DashState &dc) {
+    const uint32_t last = c.n - 1;
+    const uint32_t cp = k < last ? k : last, iR = k < last ? k + 1 : 0;
+    v2 d = vsub(c.P[iR], c.P[cp]);
+    float segLen = vlen(d);
+    while (dc.curDashOffset < segLen) {
+        dc.dashOn = !dc.dashOn;
+        dc.curDashOffset += c.dashes[dc.curDash];
+        if (++dc.curDash == c.dashCount)
+            dc.curDash = 0;
+    }
+    dc.curDashOffset -= segLen;
+    dc.curDashOffset = fmodf(dc.curDashOffset, dc.totDashLength);
+}
+
+__device__ __forceinline__ uint2 dash_pack(const DashState &dc) {
+    return make_uint2((dc.dashOn ? 1u : 0u) | (dc.curDash << 1), __float_as_uint(dc.curDashOffset));
+}
+__device__ __forceinline__ DashState dash_unpack(const StrokeCtx &c, float dashOffset, uint2 ds) {
+    DashState dc = dash_start(c, dashOffset);
+    dc.dashOn = (ds.x & 1u) != 0;
+    dc.curDash = ds.x >> 1;
+    dc.curDashOffset = __uint_as_float(ds.y);
+    return dc;
+}
+
 __device__ __forceinline__ StrokeCtx make_stroke_ctx(const vkh_stroke_style &st, const float *dashes, const v2 *pts,
                                                      const uint8_t *pf, uint32_t first, uint32_t n) {
     StrokeCtx c;
@@ -614,6 +642,34 @@ __device__ __forceinline__ uint32_t unit_of(const uint32_t *off, uint32_t n, uin
     return lo - 1;
 }
 
+// per dashed sub-path (its first piece's step): the dash state at the start of every piece
+__global__ void k_dash_walk(const Unit *__restrict__ units, uint32_t n_units, const uint32_t *__restrict__ step_off,
+                            uint32_t n_steps, const vkh_draw *__restrict__ draws, const vkh_subpath *__restrict__ sps,
+                            const uint32_t *__restrict__ sp_first, const uint32_t *__restrict__ sp_count,
+                            const v2 *__restrict__ pts, const uint8_t *__restrict__ pf,
+                            const vkh_stroke_style *__restrict__ styles, const float *__restrict__ dashes,
+                            uint2 *__restrict__ dash_st) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n_units)
+        return;
+    const uint32_t s0 = step_off[u], s1 = step_off[u + 1];
+    if (s1 <= s0)
+        return;
+    Unit un = units[u];
+    const vkh_draw &d = draws[un.draw];
+    if (d.kind != VKH_DRAW_STROKE)
+        return;
+    const vkh_stroke_style &st = styles[d.style];
+    if (!st.dash_count)
+        return;
+    StrokeCtx c = make_stroke_ctx(st, dashes, pts, pf, sp_first[un.subpath], sp_count[un.subpath]);
+    DashState dc = dash_start(c, st.dash_offset);
+    for (uint32_t k = 0; k < s1 - s0; k++) {
+        dash_st[s0 + k] = dash_pack(dc);
+        dash_walk_piece(c, k, dc);
+    }
+}
+
 template <bool EMIT>
 __global__ void k_stroke_steps(const Unit *__restrict__ units, uint32_t n_units, const uint32_t *__restrict__ step_off,
                                uint32_t n_steps, const vkh_draw *__restrict__ draws,
@@ -635,24 +691,13 @@ __global__ void k_stroke_steps(const Unit *__restrict__ units, uint32_t n_units,
     bool closed = (sps[un.subpath].flags & VKH_SP_CLOSED) != 0;
     uint32_t i = sidx - step_off[u];
     if (!EMIT) {
-        if (st.dash_count) {
-            // the first piece's thread walks the whole dashed sub-path in order, recording each
-            // piece's starting dash state and output counts
-            if (i != 0)
-                return;
-            DashState dc = dash_start(c, st.dash_offset);
-            const uint32_t np = dash_pieces(c.n, closed);
-            for (uint32_t k = 0; k < np; k++) {
-                dash_st[sidx + k] = make_uint2((dc.dashOn ? 1u : 0u) | (dc.curDash << 1), __float_as_uint(dc.curDashOffset));
-                CountSink ks;
-                dash_piece(c, closed, k, dc, ks);
-                step_nv[sidx + k] = ks.nv;
-                step_nt[sidx + k] = ks.ni / 3;
-            }
-            return;
-        }
         CountSink k;
-        stroke_step(c, closed, i, k);
+        if (st.dash_count) {
+            DashState dc = dash_unpack(c, st.dash_offset, dash_st[sidx]);
+            dash_piece(c, closed, i, dc, k);
+        } else {
+            stroke_step(c, closed, i, k);
+        }
         step_nv[sidx] = k.nv;
         step_nt[sidx] = k.ni / 3;
     } else {
@@ -662,11 +707,7 @@ __global__ void k_stroke_steps(const Unit *__restrict__ units, uint32_t n_units,
         k.idx = tris + 3 * (size_t)st_off[sidx];
         k.ni = 0;
         if (st.dash_count) {
-            DashState dc = dash_start(c, st.dash_offset);
-            const uint2 ds = dash_st[sidx];
-            dc.dashOn = (ds.x & 1u) != 0;
-            dc.curDash = ds.x >> 1;
-            dc.curDashOffset = __uint_as_float(ds.y);
+            DashState dc = dash_unpack(c, st.dash_offset, dash_st[sidx]);
             dash_piece(c, closed, i, dc, k);
         } else {
             stroke_step(c, closed, i, k);
@@ -990,6 +1031,9 @@ void launch_stroke_steps(hipStream_t s, bool emit, const Unit *units, uint32_t n
                          const uint32_t *st_off, v2 *verts, uint32_t *tris, uint2 *dash_st) {
     if (!n_steps)
         return;
+    if (!emit)
+        hipLaunchKernelGGL(k_dash_walk, dim3((n_units + 255) / 256), dim3(256), 0, s, units, n_units, step_off, n_steps,
+                           draws, sps, spf, spc, pts, pf, styles, dashes, dash_st);
     if (emit)
         hipLaunchKernelGGL(k_stroke_steps<true>, dim3((n_steps + 255) / 256), dim3(256), 0, s, units, n_units, step_off,
                            n_steps, draws, sps, spf, spc, pts, pf, styles, dashes, step_nv, step_nt, sv_off, st_off,
