@@ -239,6 +239,13 @@ vkvg_public void vkvg_fill_preserve(VkvgContext ctx);   /* :1271 */
 vkvg_public void vkvg_paint(VkvgContext ctx);           /* :1280 */
 vkvg_public void vkvg_clear(VkvgContext ctx);           /* :1291 */
 
+/* ---- clip and graphic-state stack (include/vkvg.h:1299-1325, :1565-1574) */
+vkvg_public void vkvg_reset_clip(VkvgContext ctx);      /* :1299 */
+vkvg_public void vkvg_clip(VkvgContext ctx);            /* :1317 */
+vkvg_public void vkvg_clip_preserve(VkvgContext ctx);   /* :1325 */
+vkvg_public void vkvg_save(VkvgContext ctx);            /* :1565 */
+vkvg_public void vkvg_restore(VkvgContext ctx);         /* :1574 */
+
 /* ---- state (include/vkvg.h:1334-1527) */
 vkvg_public void  vkvg_set_opacity(VkvgContext ctx, float opacity);                            /* :1334 */
 vkvg_public float vkvg_get_opacity(VkvgContext ctx);                                           /* :1335 */

@@ -53,8 +53,20 @@ enum {
     VKH_DRAW_FILL_EO = 1, /* k = w & 1 (fan INVERT + cover)     src/vkvg_context_internal.c:1583-1655 */
     VKH_DRAW_STROKE = 2,  /* k = w = covering triangles         src/vkvg_context.c:822-948 */
     VKH_DRAW_PAINT = 3,   /* k = 1 on every pixel               src/vkvg_context.c:990-1003 */
-    VKH_DRAW_CLEAR = 4,   /* colour := 0 on every pixel         src/vkvg_context.c:734-753 */
+    VKH_DRAW_CLEAR = 4,   /* colour := 0, stencil := 0 on every pixel    src/vkvg_context.c:734-753 */
+    /* Stencil draws. A target with a stencil plane (vkh_target.stencil) keeps per pixel the CLIP
+     * bit (bit 1: draws pass where it is 0), six saved CLIP bits (bits 2..7) and three copies of
+     * that byte (bytes 1..3) for save levels beyond six. They follow src/vkvg_context.c:698-795
+     * and :1251-1512 with the pipeline states of src/vkvg_device_internal.c:226-246. */
+    VKH_DRAW_CLIP_NZ = 5,       /* CLIP |= !(w != 0); key folded where CLIP stays 0   :754-795 */
+    VKH_DRAW_CLIP_EO = 6,       /* CLIP |= !(w & 1) */
+    VKH_DRAW_CLIP_SAVE = 7,     /* saved bit `style` (0..5) := CLIP                     :1320-1341 */
+    VKH_DRAW_CLIP_RESTORE = 8,  /* CLIP := saved bit `style` (0..5)                     :1399-1428 */
+    VKH_DRAW_CLIP_RESET = 9,    /* stencil byte := 0                                    :706-718 */
+    VKH_DRAW_STENCIL_SPILL = 10,   /* copy `style` (0..2) := stencil byte               :1270-1317 */
+    VKH_DRAW_STENCIL_UNSPILL = 11, /* stencil byte := copy `style` (0..2)               :1432-1474 */
 };
+#define VKH_STENCIL_SPILLS 3 /* byte copies per pixel: 6 * (1 + 3) = 24 nested clip saves */
 
 typedef struct {
     uint32_t kind;           /* VKH_DRAW_* */
@@ -65,7 +77,7 @@ typedef struct {
     float    mat[6];         /* CTM (xx yx xy yy x0 y0) applied as in shaders/vkvg_main.vert:74-79 */
     uint32_t first_subpath;
     uint32_t n_subpaths;
-    int32_t  style;          /* stroke style index (STROKE only) */
+    int32_t  style;          /* stroke style index (STROKE); bit or copy index (stencil draws) */
     uint32_t target;         /* render target index */
     uint32_t key_index;      /* draw number on its target (coverage-signature key) */
 } vkh_draw;
@@ -92,6 +104,10 @@ typedef struct {
     void     *rgba;       /* device pointer, width*height*4 bytes, premultiplied RGBA8 */
     uint32_t  width, height;
     uint32_t *sig;        /* optional device pointer to width*height uint32 coverage signatures */
+    uint32_t *stencil;    /* optional device pointer to width*height uint32 stencil words; needed
+                           * by the stencil draws, and then every draw of the target is clipped */
+    uint32_t  stencil_load; /* 1: the frame starts from the plane's contents; 0: from zero */
+    uint32_t  reserved;
 } vkh_target;
 
 typedef struct {

@@ -78,13 +78,19 @@ enum {
     OVG_OP_ELLIPTIC_ARC_TO = 38, /* x y u32 large u32 sweep rx ry phi */
     OVG_OP_ROUNDED_RECT = 39,    /* x y w h r */
     OVG_OP_SET_SOURCE_COLOR = 40,/* u32 */
+    OVG_OP_CLIP = 41,
+    OVG_OP_CLIP_PRESERVE = 42,
+    OVG_OP_RESET_CLIP = 43,
+    OVG_OP_SAVE = 44,
+    OVG_OP_RESTORE = 45,
+    OVG_OP_FLUSH = 46,           /* vkvg_flush: a frame boundary, no drawing state changes */
     OVG_OP_COUNT
 };
 
 typedef struct {
     uint32_t width, height;
     uint8_t *rgba;     /* premultiplied RGBA8, row-major, as stored on the device */
-    uint8_t *stencil;  /* S8: bit0 FILL, bit1 CLIP */
+    uint8_t *stencil;  /* S8: bit0 FILL, bit1 CLIP, bits 2..7 saved CLIP bits */
     uint32_t *sig;     /* per-pixel coverage signature */
     int32_t *acc;      /* scratch for the non-zero winding pass */
     uint32_t draw_counter;

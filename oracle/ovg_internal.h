@@ -54,7 +54,10 @@ typedef struct {
     uint32_t count;
 } ovg_grad;
 
-typedef struct {
+/* vkvg_clip_state_t, src/vkvg_context_internal.h:94-99 */
+enum { OVG_CLIP_NONE = 0, OVG_CLIP_CLEAR = 1, OVG_CLIP_CLIP = 2, OVG_CLIP_SAVED = 6 };
+
+typedef struct octx_s {
     int      status;
     ovg_surface *surf;
 
@@ -74,6 +77,15 @@ typedef struct {
     mat6     mat;
     int      srcType;       /* SRC_SOLID or gradient */
     ovg_grad grad;          /* control points already transformed at set_source time */
+    uint32_t patId;         /* identity of the current gradient pattern, 0 for a solid colour */
+    ovg_grad gradDef;       /* that pattern as defined (before the CTM), for vkvg_restore */
+
+    /* clip and save/restore state (src/vkvg_context_internal.h:87-130) */
+    int       clipState;    /* OVG_CLIP_* */
+    uint32_t  curSavBit;    /* stencil save levels in use */
+    struct osave *saved;    /* vkvg_save stack */
+    uint8_t **spills;       /* whole-stencil copies taken every 6 save levels */
+    uint32_t  nSpills;
 
     /* vertex / index caches used by stroke and fill geometry */
     vtx      *vc;
@@ -81,6 +93,20 @@ typedef struct {
     uint32_t *ic;
     uint32_t  indCount, sizeI;
 } octx;
+
+/* one vkvg_save record (vkvg_context_save_t, src/vkvg_context_internal.h:101-130) */
+struct osave {
+    struct osave *next;
+    int      clipState;
+    float    dashOffset;
+    uint32_t dashCount;
+    float   *dashes;
+    float    lineWidth, miterLimit, opacity;
+    int      op, lineCap, fillRule;
+    mat6     mat;
+    uint32_t curColor, patId;
+    ovg_grad gradDef;
+};
 
 /* geometry (ovg_geom.c) */
 void octx_init(octx *c, ovg_surface *s);
@@ -119,6 +145,15 @@ void r_fill(octx *c);
 void r_stroke(octx *c);
 void r_paint(octx *c);
 void r_clear(octx *c);
+void r_clip(octx *c);
+/* clip state machine and save/restore (ovg_state.c) */
+void o_clip_preserve(octx *c);
+void o_reset_clip(octx *c);
+void o_clear(octx *c);
+void o_save(octx *c);
+void o_restore(octx *c);
+void o_set_gradient(octx *c, const ovg_grad *def);
+void o_state_free(octx *c);
 uint32_t o_rgbaf(float r, float g, float b, float a);
 
 #endif

@@ -465,6 +465,79 @@ static uint32_t list_subpaths(octx *c, uint32_t *first, uint32_t *cnt, uint32_t 
     return n;
 }
 
+/* _poly_fill, src/vkvg_context_internal.c:1583-1655 with polyFillOpState
+ * (src/vkvg_device_internal.c:226-232): one triangle fan per subpath INVERTs the FILL bit where
+ * CLIP == 0. Accumulates the CTM-transformed point bounds (starting from `bounds`) and the pixel
+ * bbox of the fans. */
+static void eo_fans(octx *c, const uint32_t *first, const uint32_t *cnt, uint32_t nsp, float *bounds, bbox_t *bb) {
+    ovg_surface *S = c->surf;
+    const float m[6] = {c->mat.xx, c->mat.yx, c->mat.xy, c->mat.yy, c->mat.x0, c->mat.y0};
+    for (uint32_t s = 0; s < nsp; s++) {
+        uint32_t n = cnt[s];
+        int32_t *X = (int32_t *)malloc(n * sizeof(int32_t)), *Y = (int32_t *)malloc(n * sizeof(int32_t));
+        for (uint32_t i = 0; i < n; i++) {
+            v2 p = c->points[first[s] + i];
+            ovg_snap_vertex(p.x, p.y, S->width, S->height, m, &X[i], &Y[i]);
+            float bx = p.x, by = p.y;
+            m_transform_point(&c->mat, &bx, &by);
+            if (bx < bounds[0]) bounds[0] = bx;
+            if (bx > bounds[2]) bounds[2] = bx;
+            if (by < bounds[1]) bounds[1] = by;
+            if (by > bounds[3]) bounds[3] = by;
+        }
+        for (uint32_t i = 1; i + 1 < n; i++) {
+            int32_t tx[3] = {X[0], X[i], X[i + 1]}, ty[3] = {Y[0], Y[i], Y[i + 1]};
+            bbox_t tb = {INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN};
+            raster_tri(tx, ty, S->width, S->height, S->acc, &tb);
+            S->tri_count++;
+            if (tb.x0 > tb.x1)
+                continue;
+            for (int32_t r = tb.y0; r <= tb.y1; r++)
+                for (int32_t x = tb.x0; x <= tb.x1; x++) {
+                    size_t k = (size_t)r * S->width + x;
+                    if (S->acc[k]) {
+                        if (!(S->stencil[k] & 2))
+                            S->stencil[k] ^= 1u;
+                        S->acc[k] = 0;
+                    }
+                }
+            if (tb.x0 < bb->x0) bb->x0 = tb.x0;
+            if (tb.x1 > bb->x1) bb->x1 = tb.x1;
+            if (tb.y0 < bb->y0) bb->y0 = tb.y0;
+            if (tb.y1 > bb->y1) bb->y1 = tb.y1;
+        }
+        free(X);
+        free(Y);
+    }
+}
+
+/* the region [winding != 0] over all contours (what glutess triangulates,
+ * src/vkvg_context_internal.c:1747-1793), added into S->acc */
+static void nz_winding(octx *c, const uint32_t *first, const uint32_t *cnt, uint32_t nsp, bbox_t *bb) {
+    ovg_surface *S = c->surf;
+    const float m[6] = {c->mat.xx, c->mat.yx, c->mat.xy, c->mat.yy, c->mat.x0, c->mat.y0};
+    edge_t *E = NULL;
+    size_t ne = 0, ecap = 0;
+    for (uint32_t s = 0; s < nsp; s++) {
+        uint32_t n = cnt[s];
+        int32_t *X = (int32_t *)malloc(n * sizeof(int32_t)), *Y = (int32_t *)malloc(n * sizeof(int32_t));
+        for (uint32_t i = 0; i < n; i++) {
+            v2 p = c->points[first[s] + i];
+            ovg_snap_vertex(p.x, p.y, S->width, S->height, m, &X[i], &Y[i]);
+        }
+        for (uint32_t i = 0; i < n; i++) {
+            uint32_t j = (i + 1) % n;
+            push_edge(&E, &ne, &ecap, X[i], Y[i], X[j], Y[j]);
+        }
+        free(X);
+        free(Y);
+    }
+    winding_edges(E, ne, S->width, S->height, S->acc, bb);
+    free(E);
+}
+
+static bool convex_fast_path(const octx *c) { return c->pathPtr == 1 && (c->pathes[0] & P_CONVEX); }
+
 void r_fill(octx *c) {
     ovg_surface *S = c->surf;
     o_finish_path(c);
@@ -473,49 +546,13 @@ void r_fill(octx *c) {
     uint32_t cap = c->pathPtr + 1;
     uint32_t *first = (uint32_t *)malloc(cap * sizeof(uint32_t)), *cnt = (uint32_t *)malloc(cap * sizeof(uint32_t));
     uint32_t nsp = list_subpaths(c, first, cnt, cap);
-    const float m[6] = {c->mat.xx, c->mat.yx, c->mat.xy, c->mat.yy, c->mat.x0, c->mat.y0};
 
     if (c->fillRule == 0) {
-        /* EVEN_ODD: fan triangles INVERT the FILL bit (where CLIP == 0) */
-        float xMin = FLT_MAX, yMin = FLT_MAX, xMax = FLT_MIN, yMax = FLT_MIN;
+        /* EVEN_ODD: fans INVERT FILL, bounds start as src/vkvg_context.c:806 has them */
+        float bounds[4] = {FLT_MAX, FLT_MAX, FLT_MIN, FLT_MIN};
         bbox_t bb = {INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN};
-        for (uint32_t s = 0; s < nsp; s++) {
-            uint32_t n = cnt[s];
-            int32_t *X = (int32_t *)malloc(n * sizeof(int32_t)), *Y = (int32_t *)malloc(n * sizeof(int32_t));
-            for (uint32_t i = 0; i < n; i++) {
-                v2 p = c->points[first[s] + i];
-                ovg_snap_vertex(p.x, p.y, S->width, S->height, m, &X[i], &Y[i]);
-                float bx = p.x, by = p.y;
-                m_transform_point(&c->mat, &bx, &by);
-                if (bx < xMin) xMin = bx;
-                if (bx > xMax) xMax = bx;
-                if (by < yMin) yMin = by;
-                if (by > yMax) yMax = by;
-            }
-            for (uint32_t i = 1; i + 1 < n; i++) {
-                int32_t tx[3] = {X[0], X[i], X[i + 1]}, ty[3] = {Y[0], Y[i], Y[i + 1]};
-                bbox_t tb = {INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN};
-                raster_tri(tx, ty, S->width, S->height, S->acc, &tb);
-                S->tri_count++;
-                if (tb.x0 > tb.x1)
-                    continue;
-                for (int32_t r = tb.y0; r <= tb.y1; r++)
-                    for (int32_t x = tb.x0; x <= tb.x1; x++) {
-                        size_t k = (size_t)r * S->width + x;
-                        if (S->acc[k]) {
-                            if (!(S->stencil[k] & 2))
-                                S->stencil[k] ^= 1u;
-                            S->acc[k] = 0;
-                        }
-                    }
-                if (tb.x0 < bb.x0) bb.x0 = tb.x0;
-                if (tb.x1 > bb.x1) bb.x1 = tb.x1;
-                if (tb.y0 < bb.y0) bb.y0 = tb.y0;
-                if (tb.y1 > bb.y1) bb.y1 = tb.y1;
-            }
-            free(X);
-            free(Y);
-        }
+        eo_fans(c, first, cnt, nsp, bounds, &bb);
+        const float xMin = bounds[0], yMin = bounds[1], xMax = bounds[2], yMax = bounds[3];
         /* cover: full-screen triangle scissored to the bounds, passes where FILL == 1 */
         int32_t sx = (int32_t)fmaxf(xMin, 0), sy = (int32_t)fmaxf(yMin, 0);
         int32_t sw = (int32_t)fmaxf(xMax - (float)(int32_t)xMin + 1, 1);
@@ -536,7 +573,7 @@ void r_fill(octx *c) {
                     S->frag_count++;
                 }
             }
-    } else if (c->pathPtr == 1 && (c->pathes[0] & P_CONVEX)) {
+    } else if (convex_fast_path(c)) {
         /* convex fast path: fan (0, i-1, i) drawn with pipe_OVER */
         uint32_t n = c->pathes[0] & P_MASK;
         c->vertCount = c->indCount = 0;
@@ -562,28 +599,75 @@ void r_fill(octx *c) {
         draw_triangles(c);
     } else {
         /* NON_ZERO: the region with winding != 0 over all contours, composited once */
-        edge_t *E = NULL;
-        size_t ne = 0, ecap = 0;
-        for (uint32_t s = 0; s < nsp; s++) {
-            uint32_t n = cnt[s];
-            int32_t *X = (int32_t *)malloc(n * sizeof(int32_t)), *Y = (int32_t *)malloc(n * sizeof(int32_t));
-            for (uint32_t i = 0; i < n; i++) {
-                v2 p = c->points[first[s] + i];
-                ovg_snap_vertex(p.x, p.y, S->width, S->height, m, &X[i], &Y[i]);
-            }
-            for (uint32_t i = 0; i < n; i++) {
-                uint32_t j = (i + 1) % n;
-                push_edge(&E, &ne, &ecap, X[i], Y[i], X[j], Y[j]);
-            }
-            free(X);
-            free(Y);
-        }
         bbox_t bb = {INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN};
-        winding_edges(E, ne, S->width, S->height, S->acc, &bb);
-        free(E);
+        nz_winding(c, first, cnt, nsp, &bb);
         composite_counts(c, c->curColor, S->acc, bb, true);
     }
     S->draw_counter++;
+    free(first);
+    free(cnt);
+}
+
+/* _clip_preserve's stencil passes, src/vkvg_context.c:754-795 with clipingOpState
+ * (src/vkvg_device_internal.c:233-239):
+ *   1. FILL := coverage of the path where CLIP == 0 (even-odd: the _poly_fill fans; non-zero:
+ *      pipelineClipping with reference FILL, compare mask CLIP, write mask FILL over the
+ *      _fill_non_zero triangles, whose fail op ZEROes FILL where CLIP == 1);
+ *   2. a full-screen quad with reference CLIP, compare mask FILL, write mask FILL|CLIP: where
+ *      FILL == 0 the pass op REPLACEs the two bits with CLIP, elsewhere the fail op ZEROes them.
+ * The clip therefore only ever shrinks: CLIP' = CLIP | !coverage. The pixels that keep CLIP == 0
+ * fold the draw's key into the signature (the 1-sample mask of the clip). */
+void r_clip(octx *c) {
+    ovg_surface *S = c->surf;
+    o_finish_path(c);
+    if (!c->pathPtr)
+        return;
+    uint32_t cap = c->pathPtr + 1;
+    uint32_t *first = (uint32_t *)malloc(cap * sizeof(uint32_t)), *cnt = (uint32_t *)malloc(cap * sizeof(uint32_t));
+    uint32_t nsp = list_subpaths(c, first, cnt, cap);
+    const size_t np = (size_t)S->width * S->height;
+    if (c->fillRule == 0) {
+        float bounds[4] = {FLT_MAX, FLT_MAX, FLT_MIN, FLT_MIN}; /* not used: no scissor */
+        bbox_t bb = {INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN};
+        eo_fans(c, first, cnt, nsp, bounds, &bb);
+    } else {
+        bbox_t bb = {INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN};
+        if (convex_fast_path(c)) {
+            const float m[6] = {c->mat.xx, c->mat.yx, c->mat.xy, c->mat.yy, c->mat.x0, c->mat.y0};
+            uint32_t n = c->pathes[0] & P_MASK;
+            int32_t *X = (int32_t *)malloc((n + 1) * sizeof(int32_t)), *Y = (int32_t *)malloc((n + 1) * sizeof(int32_t));
+            for (uint32_t i = 0; i < n; i++)
+                ovg_snap_vertex(c->points[i].x, c->points[i].y, S->width, S->height, m, &X[i], &Y[i]);
+            for (uint32_t i = 2; i < n; i++) {
+                int32_t tx[3] = {X[0], X[i - 1], X[i]}, ty[3] = {Y[0], Y[i - 1], Y[i]};
+                raster_tri(tx, ty, S->width, S->height, S->acc, &bb);
+                S->tri_count++;
+            }
+            free(X);
+            free(Y);
+        } else {
+            nz_winding(c, first, cnt, nsp, &bb);
+        }
+        if (bb.x0 <= bb.x1)
+            for (int32_t r = bb.y0; r <= bb.y1; r++)
+                for (int32_t x = bb.x0; x <= bb.x1; x++) {
+                    size_t k = (size_t)r * S->width + x;
+                    if (S->acc[k] && !(S->stencil[k] & 2))
+                        S->stencil[k] |= 1u; /* REPLACE with reference FILL, write mask FILL */
+                    S->acc[k] = 0;
+                }
+    }
+    uint32_t key = ovg_draw_key(S->draw_counter);
+    for (size_t k = 0; k < np; k++) {
+        if (S->stencil[k] & 1) {
+            S->stencil[k] &= (uint8_t)~3u;
+            S->sig[k] += key;
+        } else {
+            S->stencil[k] = (uint8_t)((S->stencil[k] & ~3u) | 2u);
+        }
+    }
+    S->draw_counter++;
+    c->clipState = OVG_CLIP_CLIP;
     free(first);
     free(cnt);
 }

@@ -27,25 +27,6 @@ static void edit_radial(ovg_grad *g, float cx0, float cy0, float radius0, float 
     g->cp[0][0] = c0x; g->cp[0][1] = c0y; g->cp[0][2] = radius0; g->cp[0][3] = 0;
     g->cp[1][0] = c1x; g->cp[1][1] = c1y; g->cp[1][2] = radius1; g->cp[1][3] = 0;
 }
-/* _update_cur_pattern for LINEAR/RADIAL, src/vkvg_context_internal.c:774-826 (no pattern matrix) */
-static void set_gradient(octx *c, const ovg_grad *src) {
-    ovg_grad g = *src;
-    if (g.count < 2) {
-        c->status = 10; /* VKVG_STATUS_PATTERN_INVALID_GRADIENT */
-        return;
-    }
-    m_transform_point(&c->mat, &g.cp[0][0], &g.cp[0][1]);
-    if (g.type == SRC_LINEAR) {
-        m_transform_point(&c->mat, &g.cp[0][2], &g.cp[0][3]);
-    } else {
-        m_transform_point(&c->mat, &g.cp[1][0], &g.cp[1][1]);
-        m_transform_distance(&c->mat, &g.cp[0][2], &g.cp[0][3]);
-        m_transform_distance(&c->mat, &g.cp[1][2], &g.cp[0][3]); /* as written in the reference */
-    }
-    c->grad = g;
-    c->srcType = g.type;
-}
-
 int ovg_render_stream(uint32_t width, uint32_t height, const uint32_t *w, size_t nw, uint8_t *out_rgba,
                       uint32_t *out_sig, uint64_t *out_stats) {
     ovg_surface S;
@@ -59,6 +40,7 @@ int ovg_render_stream(uint32_t width, uint32_t height, const uint32_t *w, size_t
     S.acc = (int32_t *)calloc(np, sizeof(int32_t));
     octx c;
     octx_init(&c, &S);
+    uint32_t n_patterns = 0;
 
     size_t i = 0;
     while (i < nw && !c.status) {
@@ -93,18 +75,26 @@ int ovg_render_stream(uint32_t width, uint32_t height, const uint32_t *w, size_t
         case OVG_OP_STROKE: r_stroke(&c); o_clear_path(&c); break;
         case OVG_OP_STROKE_PRESERVE: r_stroke(&c); break;
         case OVG_OP_PAINT: r_paint(&c); break;
-        case OVG_OP_CLEAR: r_clear(&c); break;
+        case OVG_OP_CLEAR: o_clear(&c); break;
+        case OVG_OP_CLIP: o_clip_preserve(&c); o_clear_path(&c); break;
+        case OVG_OP_CLIP_PRESERVE: o_clip_preserve(&c); break;
+        case OVG_OP_RESET_CLIP: o_reset_clip(&c); break;
+        case OVG_OP_SAVE: o_save(&c); break;
+        case OVG_OP_RESTORE: o_restore(&c); break;
+        case OVG_OP_FLUSH: break; /* frame boundary: no state change */
         case OVG_OP_SET_SOURCE_RGBA:
             c.curColor = o_rgbaf(F(a), F(a + 1), F(a + 2), F(a + 3));
             c.srcType = SRC_SOLID;
+            c.patId = 0;
             i += 4;
             break;
         case OVG_OP_SET_SOURCE_RGB:
             c.curColor = o_rgbaf(F(a), F(a + 1), F(a + 2), 1);
             c.srcType = SRC_SOLID;
+            c.patId = 0;
             i += 3;
             break;
-        case OVG_OP_SET_SOURCE_COLOR: c.curColor = a[0]; c.srcType = SRC_SOLID; i += 1; break;
+        case OVG_OP_SET_SOURCE_COLOR: c.curColor = a[0]; c.srcType = SRC_SOLID; c.patId = 0; i += 1; break;
         case OVG_OP_SET_LINE_WIDTH: c.lineWidth = F(a); i += 1; break;
         case OVG_OP_SET_LINE_JOIN: c.lineJoin = (int)a[0]; i += 1; break;
         case OVG_OP_SET_LINE_CAP: c.lineCap = (int)a[0]; i += 1; break;
@@ -149,7 +139,9 @@ int ovg_render_stream(uint32_t width, uint32_t height, const uint32_t *w, size_t
                 g.stops[g.count] = off;
                 g.count++;
             }
-            set_gradient(&c, &g);
+            c.patId = ++n_patterns; /* every gradient op sets a new pattern */
+            c.gradDef = g;
+            o_set_gradient(&c, &g);
             i += ncp + 1 + 5 * ns;
             break;
         }
@@ -198,6 +190,7 @@ int ovg_render_stream(uint32_t width, uint32_t height, const uint32_t *w, size_t
         out_stats[1] = S.tri_count;
         out_stats[2] = S.frag_count;
     }
+    o_state_free(&c);
     octx_free(&c);
     free(S.rgba);
     free(S.stencil);

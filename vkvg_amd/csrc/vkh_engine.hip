@@ -151,6 +151,7 @@ struct vkh_engine {
     uint32_t n_items, n_sp, n_draws, n_styles, n_dashes, n_grads, n_targets, n_units;
     uint32_t tile_total, bin_total, frame_tile, coarse_nch = 1, coarse_chunk = VKH_COARSE_CHUNK, n_cubic_items = 0;
     uint32_t most_draws = 1; // most draws on one target
+    bool frame_stencil = false; // some target of the frame has a stencil plane
 };
 
 static bool check(vkh_engine *e, hipError_t r, const char *what) {
@@ -408,7 +409,22 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
             eng->err = "draws must be grouped by target in ascending order";
             return -2;
         }
-        if (dr.kind == VKH_DRAW_PAINT || dr.kind == VKH_DRAW_CLEAR)
+        if (dr.kind > VKH_DRAW_STENCIL_UNSPILL) {
+            eng->err = "unknown draw kind";
+            return -2;
+        }
+        if ((dr.kind >= VKH_DRAW_CLIP_NZ) && !f->targets[dr.target].stencil) {
+            eng->err = "clip or stencil draw on a target without a stencil plane";
+            return -2;
+        }
+        if (dr.kind >= VKH_DRAW_CLIP_SAVE) {
+            const uint32_t lim = dr.kind >= VKH_DRAW_STENCIL_SPILL ? VKH_STENCIL_SPILLS : 6u;
+            if (dr.style < 0 || (uint32_t)dr.style >= lim) {
+                eng->err = "stencil draw index out of range";
+                return -2;
+            }
+        }
+        if (!kind_has_tiles(dr.kind))
             continue;
         if (dr.kind == VKH_DRAW_STROKE && (dr.style < 0 || (uint32_t)dr.style >= f->n_styles)) {
             eng->err = "stroke draw without a style";
@@ -439,6 +455,7 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
     }
     std::vector<TargetInfo> tg(f->n_targets);
     uint32_t tile_total = 0, bin_total = 0;
+    bool any_stencil = false;
     {
         uint32_t d = 0;
         for (uint32_t t = 0; t < f->n_targets; t++) {
@@ -463,6 +480,11 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
             ti.draw_end = d;
             ti.rgba = (uint8_t *)f->targets[t].rgba;
             ti.sig = f->targets[t].sig;
+            ti.stencil = f->targets[t].stencil;
+            ti.stencil_load = f->targets[t].stencil_load ? 1u : 0u;
+            ti.pad = 0;
+            if (ti.stencil)
+                any_stencil = true;
         }
     }
     const uint32_t n_units = (uint32_t)units.size();
@@ -524,6 +546,7 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
     eng->most_draws = 1;
     for (const TargetInfo &ti : tg)
         eng->most_draws = std::max(eng->most_draws, ti.draw_end - ti.draw_begin);
+    eng->frame_stencil = any_stencil;
     set_coarse_chunks(eng);
     eng->frame_gen++;
     eng->gfail_key = 0;
@@ -756,7 +779,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
         CK(hipEventRecord(eng->ev_stage[4], s)); // coarse done
     if (!cap)
         CK(hipEventRecord(eng->ev0, s));
-    vkh::launch_fine(s, T, eng->fine_words, tile_total, d_tg, n_targets, P<DrawPrep>(eng->dprep), d_grec, d_grads, d_gprep,
+    vkh::launch_fine(s, eng->frame_stencil, T, eng->fine_words, tile_total, d_tg, n_targets, P<DrawPrep>(eng->dprep), d_grec, d_grads, d_gprep,
                      P<PathTile>(eng->ptiles),
                      P<Edge>(eng->segs), nch, P<uint32_t>(eng->tile_off),
                      P<Cmd>(eng->cmds));

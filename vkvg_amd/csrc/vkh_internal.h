@@ -28,7 +28,18 @@ struct TargetInfo {
     uint32_t draw_begin, draw_end;
     uint8_t *rgba;
     uint32_t *sig;
+    uint32_t *stencil;      // stencil words (byte 0: FILL, CLIP, saved CLIP bits; bytes 1..3: copies)
+    uint32_t stencil_load;  // 1: start from the plane, 0: from zero
+    uint32_t pad;
 };
+
+// draw kinds with path tiles (edges binned into tiles); clip draws take every tile of the target
+__host__ __device__ __forceinline__ bool kind_has_tiles(uint32_t kind) {
+    return kind <= VKH_DRAW_STROKE || kind == VKH_DRAW_CLIP_NZ || kind == VKH_DRAW_CLIP_EO;
+}
+__host__ __device__ __forceinline__ bool kind_is_clip(uint32_t kind) {
+    return kind == VKH_DRAW_CLIP_NZ || kind == VKH_DRAW_CLIP_EO;
+}
 
 // per draw, computed on the device
 struct DrawTiles {
@@ -147,7 +158,8 @@ uint32_t coarse_chunk(uint32_t most_draws, uint32_t n_bins);
 void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, uint32_t nch, uint32_t chunk, const TargetInfo *tg,
                    uint32_t n_targets, const vkh_draw *draws, const DrawTiles *dt, const PathTile *pts,
                    uint32_t *tile_count, const uint32_t *tile_off, Cmd *cmds);
-void launch_fine(hipStream_t s, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,
+// clip: some target has a stencil plane (the stencil-aware variant of the kernel runs)
+void launch_fine(hipStream_t s, bool clip, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,
                  uint32_t n_targets, const DrawPrep *dprep, const GradRec *grec, const vkh_gradient *grads,
                  const GradPrep *gprep, const PathTile *pts, const Edge *segs, uint32_t nch,
                  const uint32_t *tile_off, const Cmd *cmds);

@@ -43,14 +43,16 @@ __global__ void k_draw_tiles(const vkh_draw *__restrict__ draws, uint32_t n, con
     const TargetInfo &t = tg[d.target];
     DrawTiles o;
     o.pt_base = 0;
-    if (d.kind == VKH_DRAW_PAINT || d.kind == VKH_DRAW_CLEAR) {
+    if (!kind_has_tiles(d.kind) || kind_is_clip(d.kind)) {
+        // paint, clear and the stencil draws cover the target without path tiles; a clip changes
+        // every pixel (outside its path CLIP is set), so it takes a path tile everywhere
         o.bx0 = 0;
         o.by0 = 0;
         o.bx1 = (int32_t)t.tilesW - 1;
         o.by1 = (int32_t)t.tilesH - 1;
-        o.bw = 0;
+        o.bw = kind_is_clip(d.kind) ? t.tilesW : 0;
         dt[i] = o;
-        pt_count[i] = 0;
+        pt_count[i] = o.bw * t.tilesH;
         return;
     }
     int4 b = bbox[i];
@@ -280,8 +282,10 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
 __device__ __forceinline__ int coverage_count(uint32_t kind, int32_t w) {
     switch (kind) {
     case VKH_DRAW_FILL_NZ:
+    case VKH_DRAW_CLIP_NZ:
         return w != 0 ? 1 : 0;
     case VKH_DRAW_FILL_EO:
+    case VKH_DRAW_CLIP_EO:
         return w & 1;
     case VKH_DRAW_STROKE:
         return w > 0 ? w : 0;
@@ -374,7 +378,7 @@ __global__ __launch_bounds__(256) void k_coarse(bool write, const TargetInfo *__
                     const DrawTiles hb = s_dt[h];
                     in[u] = tx >= hb.bx0 && tx <= hb.bx1 && ty >= hb.by0 && ty <= hb.by1;
                     const uint32_t kind = s_kind[h];
-                    if (in[u] && kind != VKH_DRAW_PAINT && kind != VKH_DRAW_CLEAR) {
+                    if (in[u] && kind_has_tiles(kind)) {
                         ptid[u] = hb.pt_base + (uint32_t)(ty - hb.by0) * hb.bw + (uint32_t)(tx - hb.bx0);
                         p[u] = pts[ptid[u]];
                     }
@@ -385,7 +389,9 @@ __global__ __launch_bounds__(256) void k_coarse(bool write, const TargetInfo *__
                 if (!in[u])
                     continue;
                 const uint32_t h = h0 + (uint32_t)u;
-                const bool emit = ptid[u] == VKH_NO_DRAW || p[u].nseg > 0 || coverage_count(s_kind[h], p[u].backdrop) != 0;
+                const uint32_t kind = s_kind[h];
+                const bool emit = ptid[u] == VKH_NO_DRAW || p[u].nseg > 0 || kind_is_clip(kind) ||
+                                  coverage_count(kind, p[u].backdrop) != 0;
                 if (emit) {
                     if (write)
                         cmds[out + count] = Cmd{s_draw[h], ptid[u]};
@@ -436,6 +442,8 @@ __global__ void k_draw_prep(const vkh_draw *__restrict__ draws, uint32_t n, cons
               ((uint32_t)f2u8(((float)((c >> 8) & 0xFF) / 255.0f) * o) << 8) |
               ((uint32_t)f2u8(((float)((c >> 16) & 0xFF) / 255.0f) * o) << 16) |
               ((uint32_t)f2u8(((float)((c >> 24) & 0xFF) / 255.0f) * o) << 24);
+    if (kind >= VKH_DRAW_CLIP_SAVE)
+        p.solid = (uint32_t)d.style; // the saved bit or copy index of a stencil draw
     p.gflags = 0;
     for (int k = 0; k < 6; k++)
         p.g[k] = 0.f;
@@ -567,6 +575,32 @@ __device__ void gradient_color(const GradRec &P, const vkh_gradient *__restrict_
 
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
+// One pixel's stencil word under a stencil draw without geometry: byte 0 holds FILL (bit 0, always
+// 0 between draws), CLIP (bit 1) and the CLIP bits of six saved levels (bits 2..7); bytes 1..3 are
+// the copies the reference keeps in extra stencil images beyond six levels.
+__device__ __forceinline__ uint32_t stencil_apply(uint32_t kind, uint32_t prm, uint32_t st) {
+    switch (kind) {
+    case VKH_DRAW_CLIP_SAVE: { // pass REPLACE / fail ZERO on the saved bit, where CLIP is set
+        const uint32_t b = 4u << prm;
+        return (st & ~b) | ((st & 2u) ? b : 0u);
+    }
+    case VKH_DRAW_CLIP_RESTORE: { // the same with the roles of the two bits exchanged
+        const uint32_t b = 4u << prm;
+        return (st & ~2u) | ((st & b) ? 2u : 0u);
+    }
+    case VKH_DRAW_CLIP_RESET:
+        return st & ~0xFFu;
+    case VKH_DRAW_STENCIL_SPILL: {
+        const uint32_t sh = 8u * (prm + 1u);
+        return (st & ~(0xFFu << sh)) | ((st & 0xFFu) << sh);
+    }
+    case VKH_DRAW_STENCIL_UNSPILL:
+        return (st & ~0xFFu) | ((st >> (8u * (prm + 1u))) & 0xFFu);
+    default:
+        return st;
+    }
+}
+
 
 // One workgroup per T x T tile. Commands are processed in batches of B: the batch's command
 // records (draw record + path tile) are staged in LDS with one parallel load; every segment of the
@@ -576,7 +610,11 @@ __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) { return (uint32_t)_
 // values. The dependent global loads (command -> draw / path tile -> segment -> edge) are thus
 // paid once per batch and once per segment.
 // WORDS: row-delta words of LDS per workgroup (B = WORDS / pixels commands per batch)
-template <int T, int WORDS>
+// CL: the frame has targets with a stencil plane. Each pixel then also carries its stencil word
+// (loaded from the plane or zero, stored back at the end): draws blend only where CLIP is 0, clear
+// zeroes the stencil byte, and the clip and stencil draws rewrite it (src/vkvg_context.c:698-795,
+// :1251-1512; pipeline states of src/vkvg_device_internal.c:226-246).
+template <int T, int WORDS, bool CL>
 __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg, uint32_t n_targets,
                                               const DrawPrep *__restrict__ dprep,
                                               const GradRec *__restrict__ grec,
@@ -597,7 +635,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
 
     uint32_t gtile = blockIdx.x;
     uint32_t c0 = tile_off[(size_t)gtile * nch], c1 = tile_off[(size_t)(gtile + 1) * nch];
-    if (c0 == c1)
+    if (!CL && c0 == c1)
         return;
     const uint32_t tid = threadIdx.x;
     uint32_t t = find_target_by(tg, n_targets, gtile, false);
@@ -605,6 +643,16 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
     uint32_t lt = gtile - TI.tile_base;
     int32_t tx = (int32_t)(lt % TI.tilesW), ty = (int32_t)(lt / TI.tilesW);
     int32_t X0 = tx * T, Y0 = ty * T;
+    if (CL && c0 == c1) {
+        // no command: the plane keeps its words, or is zeroed when the frame starts from zero
+        if (TI.stencil && !TI.stencil_load)
+            for (int q = 0; q < NPIX / NT; q++) {
+                const int p = (int)tid + q * NT, x = X0 + p % T, y = Y0 + p / T;
+                if (x < (int32_t)TI.width && y < (int32_t)TI.height)
+                    TI.stencil[(size_t)y * TI.width + (size_t)x] = 0u;
+            }
+        return;
+    }
     const float Wf = (float)TI.width, Hf = (float)TI.height;
     const int32_t xs0 = 256 * X0 + 128, s0y = 256 * Y0 + 128;
 
@@ -622,18 +670,29 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
         gx[q] = ((float)(X0 + lx[q]) + 0.5f) / Wf;
         gy[q] = ((float)(Y0 + ly[q]) + 0.5f) / Hf;
     }
-    // the destination need not be read when the first command overwrites every pixel
+    // the destination need not be read when the first command overwrites every pixel (under a
+    // stencil only a clear does: any other draw may be clipped)
     bool need_read = true;
     {
         const Cmd f = cmds[c0];
         if (f.pt == VKH_NO_DRAW) {
             const DrawPrep &d0 = dprep[f.draw];
             const uint32_t kind0 = d0.meta & 15, op0 = (d0.meta >> 4) & 15;
-            if (kind0 == VKH_DRAW_CLEAR || op0 == 0)
+            if (kind0 == VKH_DRAW_CLEAR)
                 need_read = false;
-            else if (!(d0.meta & VKH_DP_GRAD) && (op0 == 1 || op0 == 2) && (d0.solid >> 24) == 255u)
+            else if (!CL && kind0 == VKH_DRAW_PAINT && op0 == 0)
+                need_read = false;
+            else if (!CL && kind0 == VKH_DRAW_PAINT && !(d0.meta & VKH_DP_GRAD) && (op0 == 1 || op0 == 2) &&
+                     (d0.solid >> 24) == 255u)
                 need_read = false;
         }
+    }
+    uint32_t st[PPT]; // stencil words
+#pragma unroll
+    for (int q = 0; q < PPT; q++) {
+        st[q] = 0u;
+        if (CL && TI.stencil && TI.stencil_load && valid[q])
+            st[q] = TI.stencil[(size_t)(Y0 + ly[q]) * TI.width + (size_t)(X0 + lx[q])];
     }
     uint32_t *img = (uint32_t *)TI.rgba;
 #pragma unroll
@@ -651,7 +710,8 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
             uint32_t meta = d.meta;
             const uint32_t kind = meta & 15;
             if (cm.pt == VKH_NO_DRAW) {
-                meta = (meta & ~15u) | (kind == VKH_DRAW_CLEAR ? VKH_DRAW_CLEAR : VKH_DRAW_PAINT);
+                if (kind <= VKH_DRAW_CLEAR)
+                    meta = (meta & ~15u) | (kind == VKH_DRAW_CLEAR ? VKH_DRAW_CLEAR : VKH_DRAW_PAINT);
                 c_back[j] = 0;
                 c_nseg[j] = 0;
                 c_segoff[j] = 0;
@@ -754,6 +814,25 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
             const int32_t bk = (int32_t)uniform_u32((uint32_t)c_back[j]);
             const uint32_t key = uniform_u32(c_key[j]);
             const uint32_t solid = uniform_u32(c_solid[j]);
+            if (CL && kind >= VKH_DRAW_CLIP_NZ) { // stencil draws (uniform branch)
+                if (kind_is_clip(kind)) {
+#pragma unroll
+                    for (int q = 0; q < PPT; q++) {
+                        const int32_t w = bk + (hasseg ? sd[j][ly[q]][lx[q]] : 0);
+                        if (coverage_count(kind, w) && !(st[q] & 2u)) {
+                            if (valid[q])
+                                sig[q] += key; // the clip's 1-sample mask
+                        } else {
+                            st[q] |= 2u;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < PPT; q++)
+                        st[q] = stencil_apply(kind, solid, st[q]);
+                }
+                continue;
+            }
             const vkh_src16 S0 = vkh_src16_of(solid);
 #pragma unroll
             for (int q = 0; q < PPT; q++) {
@@ -764,6 +843,12 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
                 }
                 if (k <= 0 || !valid[q])
                     continue;
+                if (CL) {
+                    if (kind == VKH_DRAW_CLEAR)
+                        st[q] &= ~0xFFu; // vkCmdClearAttachments ignores the stencil test
+                    else if (st[q] & 2u)
+                        continue; // clipped
+                }
                 vkh_src16 S = S0;
                 if (meta & VKH_DP_GRAD) {
                     const GradRec &P = grec[uniform_u32(c_draw[j])]; // scalar loads
@@ -792,6 +877,8 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
         img[o] = px[q];
         if (TI.sig)
             TI.sig[o] += sig[q];
+        if (CL && TI.stencil)
+            TI.stencil[o] = st[q];
     }
 }
 
@@ -877,32 +964,36 @@ void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n, const Ta
         hipLaunchKernelGGL(k_grad_prep, dim3((n_grads + 255) / 256), dim3(256), 0, s, grads, n_grads, gout);
 }
 template <int T, int W>
-static void fine_launch(hipStream_t s, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
+static void fine_launch(hipStream_t s, bool clip, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
                         const DrawPrep *dprep, const GradRec *grec, const vkh_gradient *grads, const GradPrep *gprep,
                         const PathTile *pts, const Edge *segs, uint32_t nch, const uint32_t *tile_off,
                         const Cmd *cmds) {
     constexpr int NT = T * T >= 256 ? 256 : T * T;
-    hipLaunchKernelGGL((k_fine<T, W>), dim3(n_tiles), dim3(NT), 0, s, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch,
-                       tile_off, cmds);
+    if (clip)
+        hipLaunchKernelGGL((k_fine<T, W, true>), dim3(n_tiles), dim3(NT), 0, s, tg, n_targets, dprep, grec, grads, gprep,
+                           pts, segs, nch, tile_off, cmds);
+    else
+        hipLaunchKernelGGL((k_fine<T, W, false>), dim3(n_tiles), dim3(NT), 0, s, tg, n_targets, dprep, grec, grads,
+                           gprep, pts, segs, nch, tile_off, cmds);
 }
 template <int T>
-static void fine_words(hipStream_t s, uint32_t words, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
+static void fine_words(hipStream_t s, bool clip, uint32_t words, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
                        const DrawPrep *dprep, const GradRec *grec, const vkh_gradient *grads, const GradPrep *gprep,
                        const PathTile *pts, const Edge *segs, uint32_t nch, const uint32_t *tile_off,
                        const Cmd *cmds) {
     switch (words) {
     case 2048:
-        fine_launch<T, 2048>(s, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_launch<T, 2048>(s, clip, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     case 8192:
-        fine_launch<T, 8192>(s, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_launch<T, 8192>(s, clip, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     default:
-        fine_launch<T, 4096>(s, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_launch<T, 4096>(s, clip, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     }
 }
-void launch_fine(hipStream_t s, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,
+void launch_fine(hipStream_t s, bool clip, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,
                  uint32_t n_targets, const DrawPrep *dprep, const GradRec *grec, const vkh_gradient *grads,
                  const GradPrep *gprep, const PathTile *pts, const Edge *segs, uint32_t nch,
                  const uint32_t *tile_off, const Cmd *cmds) {
@@ -910,13 +1001,13 @@ void launch_fine(hipStream_t s, uint32_t tile, uint32_t words, uint32_t n_tiles,
         return;
     switch (tile) {
     case 8:
-        fine_words<8>(s, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_words<8>(s, clip, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     case 32:
-        fine_words<32>(s, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_words<32>(s, clip, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     default:
-        fine_words<16>(s, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_words<16>(s, clip, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     }
 }

@@ -74,6 +74,7 @@ struct _vkvg_surface_t {
     uint32_t      width, height;
     void         *img; /* device RGBA8 premultiplied */
     uint32_t     *sig; /* device coverage signatures, optional */
+    uint32_t     *stencil; /* device stencil words, allocated by the first clip */
     uint32_t      draw_counter;
 };
 
@@ -137,6 +138,34 @@ struct _vkvg_context_t {
     uint32_t seg_ptr;         /* 0 = not segmented */
     uint32_t seg_first_item;  /* first item of the current segment */
     uint32_t seg_count;
+
+    /* clip and save/restore (src/vkvg_context_internal.h:87-130, :218-224) */
+    int              clip_state;  /* CLIP_STATE_* */
+    uint32_t         sav_bit;     /* stencil save levels in use (curSavBit) */
+    struct ctx_save *saved;       /* vkvg_save stack */
+    int              stencil_use; /* STENCIL_*: how this context's frames use the surface's plane */
+};
+
+/* vkvg_clip_state_t, src/vkvg_context_internal.h:94-99 */
+enum { CLIP_STATE_NONE = 0, CLIP_STATE_CLEAR = 1, CLIP_STATE_CLIP = 2, CLIP_STATE_SAVED = 6 };
+/* the stencil plane starts from zero in the first frame of a context that clips (a context's first
+ * render pass clears the stencil, src/vkvg_context.c:42-49); later frames load it */
+enum { STENCIL_UNUSED = 0, STENCIL_FIRST_FRAME = 1, STENCIL_LOADED = 2 };
+
+/* vkvg_context_save_t, src/vkvg_context_internal.h:101-125 */
+struct ctx_save {
+    struct ctx_save *next;
+    int              clip_state;
+    float            dashOffset;
+    uint32_t         dashCount;
+    float           *dashes;
+    float            lineWidth, miterLimit, opacity;
+    vkvg_operator_t  op;
+    vkvg_line_cap_t  lineCap;
+    vkvg_fill_rule_t fillRule;
+    vkvg_matrix_t    mat;
+    uint32_t         curColor;
+    VkvgPattern      pattern; /* referenced */
 };
 
 static vkvg_status_t s_no_mem = VKVG_STATUS_NO_MEMORY;
@@ -342,6 +371,8 @@ void vkvg_surface_destroy(VkvgSurface surf) {
     vkvg_hip_image_free(dev->eng, surf->img);
     if (surf->sig)
         vkvg_hip_image_free(dev->eng, surf->sig);
+    if (surf->stencil)
+        vkvg_hip_image_free(dev->eng, surf->stencil);
     free(surf);
     vkvg_device_destroy(dev);
 }
@@ -710,6 +741,9 @@ static void fill_frame(VkvgContext c, vkh_frame *fr, vkh_target *t) {
     t->width = c->surf->width;
     t->height = c->surf->height;
     t->sig = c->surf->sig;
+    t->stencil = c->stencil_use ? c->surf->stencil : NULL;
+    t->stencil_load = c->stencil_use == STENCIL_LOADED;
+    t->reserved = 0;
     for (uint32_t i = 0; i < f->draws.n; i++)
         f->draws.p[i].target = 0;
     fr->items = f->items.p;
@@ -743,6 +777,8 @@ static void submit_frame_ex(VkvgContext c, bool upload_only) {
         c->status = VKVG_STATUS_DEVICE_ERROR;
         return;
     }
+    if (c->stencil_use)
+        c->stencil_use = STENCIL_LOADED;
     frame_reset_keep_path(c);
 }
 static void submit_frame(VkvgContext c) { submit_frame_ex(c, false); }
@@ -771,6 +807,14 @@ void vkvg_destroy(VkvgContext ctx) {
     if (--ctx->references > 0)
         return;
     submit_frame(ctx);
+    while (ctx->saved) {
+        struct ctx_save *sv = ctx->saved;
+        ctx->saved = sv->next;
+        if (sv->pattern)
+            vkvg_pattern_destroy(sv->pattern);
+        free(sv->dashes);
+        free(sv);
+    }
     if (ctx->pattern)
         vkvg_pattern_destroy(ctx->pattern);
     free(ctx->dashes);
@@ -1345,10 +1389,170 @@ void vkvg_paint(VkvgContext ctx) {
     }
     new_draw(ctx, VKH_DRAW_PAINT);
 }
+static void set_solid(VkvgContext ctx);
+static int prev_clip_state(VkvgContext c) { return c->saved ? c->saved->clip_state : CLIP_STATE_CLEAR; }
+
+/* vkvg_clear, src/vkvg_context.c:734-753: colour and stencil cleared */
 void vkvg_clear(VkvgContext ctx) {
     if (vkvg_status(ctx))
         return;
+    ctx->clip_state = prev_clip_state(ctx) == CLIP_STATE_CLEAR ? CLIP_STATE_NONE : CLIP_STATE_CLEAR;
     new_draw(ctx, VKH_DRAW_CLEAR);
+}
+
+/* ---- clip and save/restore */
+/* a draw that only rewrites the stencil plane: no geometry and no signature key; nothing to do
+ * while this context has not clipped (its stencil is still all zero) */
+static void stencil_draw(VkvgContext c, uint32_t kind, uint32_t param) {
+    if (!c->stencil_use)
+        return;
+    vkh_draw d;
+    memset(&d, 0, sizeof(d));
+    d.kind = kind;
+    d.op = VKVG_OPERATOR_OVER;
+    d.grad = -1;
+    d.opacity = 1.f;
+    d.mat[0] = d.mat[3] = 1.f;
+    d.style = (int32_t)param;
+    VEC_PUSH(c->f.draws, d);
+}
+/* _clip_preserve, src/vkvg_context.c:754-795 */
+static void clip_preserve_i(VkvgContext c) {
+    finish_path(c);
+    if (no_committed_subpath(c))
+        return;
+    if (!c->surf->stencil) {
+        void *p = NULL;
+        if (vkvg_hip_image_alloc(c->surf->dev->eng, c->surf->width, c->surf->height, &p)) {
+            c->status = VKVG_STATUS_NO_MEMORY;
+            return;
+        }
+        c->surf->stencil = (uint32_t *)p;
+    }
+    if (!c->stencil_use)
+        c->stencil_use = STENCIL_FIRST_FRAME;
+    vkh_draw *d = new_draw(c, c->fillRule == VKVG_FILL_RULE_EVEN_ODD ? VKH_DRAW_CLIP_EO : VKH_DRAW_CLIP_NZ);
+    d->grad = -1;
+    c->path_drawn = true;
+    c->clip_state = CLIP_STATE_CLIP;
+}
+void vkvg_clip_preserve(VkvgContext ctx) {
+    if (vkvg_status(ctx))
+        return;
+    clip_preserve_i(ctx);
+}
+void vkvg_clip(VkvgContext ctx) {
+    if (vkvg_status(ctx))
+        return;
+    clip_preserve_i(ctx);
+    clear_path(ctx);
+}
+/* vkvg_reset_clip, src/vkvg_context.c:719-733 */
+void vkvg_reset_clip(VkvgContext ctx) {
+    if (vkvg_status(ctx))
+        return;
+    if (ctx->clip_state == CLIP_STATE_CLEAR)
+        return;
+    ctx->clip_state = prev_clip_state(ctx) == CLIP_STATE_CLEAR ? CLIP_STATE_NONE : CLIP_STATE_CLEAR;
+    stencil_draw(ctx, VKH_DRAW_CLIP_RESET, 0);
+}
+/* vkvg_save, src/vkvg_context.c:1251-1375 */
+void vkvg_save(VkvgContext ctx) {
+    if (vkvg_status(ctx))
+        return;
+    struct ctx_save *sv = (struct ctx_save *)calloc(1, sizeof(struct ctx_save));
+    if (!sv) {
+        ctx->status = VKVG_STATUS_NO_MEMORY;
+        return;
+    }
+    if (ctx->clip_state == CLIP_STATE_CLIP) {
+        sv->clip_state = CLIP_STATE_SAVED;
+        if (ctx->sav_bit > 0 && ctx->sav_bit % 6 == 0) { /* the reference copies the stencil to a new image */
+            const uint32_t idx = ctx->sav_bit / 6 - 1;
+            if (idx >= VKH_STENCIL_SPILLS) { /* more nested clip saves than the plane holds */
+                free(sv);
+                ctx->status = VKVG_STATUS_NO_MEMORY;
+                return;
+            }
+            stencil_draw(ctx, VKH_DRAW_STENCIL_SPILL, idx);
+        }
+        stencil_draw(ctx, VKH_DRAW_CLIP_SAVE, ctx->sav_bit % 6);
+        ctx->sav_bit++;
+    } else if (ctx->clip_state == CLIP_STATE_NONE) {
+        sv->clip_state = prev_clip_state(ctx) & 0x03;
+    } else {
+        sv->clip_state = CLIP_STATE_CLEAR;
+    }
+    sv->dashOffset = ctx->dashOffset;
+    sv->dashCount = ctx->dashCount;
+    if (ctx->dashCount) {
+        sv->dashes = (float *)malloc(sizeof(float) * ctx->dashCount);
+        if (!sv->dashes) {
+            free(sv);
+            ctx->status = VKVG_STATUS_NO_MEMORY;
+            return;
+        }
+        memcpy(sv->dashes, ctx->dashes, sizeof(float) * ctx->dashCount);
+    }
+    sv->lineWidth = ctx->lineWidth;
+    sv->miterLimit = ctx->miterLimit;
+    sv->op = ctx->op;
+    sv->lineCap = ctx->lineCap;
+    sv->fillRule = ctx->fillRule;
+    sv->mat = ctx->mat;         /* the push constants: CTM ... */
+    sv->opacity = ctx->opacity; /* ... and opacity */
+    if (ctx->pattern)
+        sv->pattern = vkvg_pattern_reference(ctx->pattern);
+    else
+        sv->curColor = ctx->curColor;
+    sv->next = ctx->saved;
+    ctx->saved = sv;
+}
+/* vkvg_restore, src/vkvg_context.c:1376-1512 */
+void vkvg_restore(VkvgContext ctx) {
+    if (vkvg_status(ctx))
+        return;
+    struct ctx_save *sv = ctx->saved;
+    if (!sv) {
+        ctx->status = VKVG_STATUS_INVALID_RESTORE;
+        return;
+    }
+    ctx->saved = sv->next;
+    ctx->mat = sv->mat;
+    ctx->opacity = sv->opacity;
+    if (ctx->clip_state != CLIP_STATE_NONE) {
+        if (ctx->clip_state == CLIP_STATE_CLIP && sv->clip_state == CLIP_STATE_CLEAR)
+            stencil_draw(ctx, VKH_DRAW_CLIP_RESET, 0);
+        else if (ctx->sav_bit > 0)
+            stencil_draw(ctx, VKH_DRAW_CLIP_RESTORE, (ctx->sav_bit - 1) % 6);
+        /* with no level saved the reference's pass copies CLIP onto itself */
+    }
+    if (sv->clip_state == CLIP_STATE_SAVED) {
+        ctx->sav_bit--;
+        if (ctx->sav_bit > 0 && ctx->sav_bit % 6 == 0) /* the saved stencil image comes back */
+            stencil_draw(ctx, VKH_DRAW_STENCIL_UNSPILL, ctx->sav_bit / 6 - 1);
+    }
+    ctx->clip_state = CLIP_STATE_NONE;
+    ctx->dashOffset = sv->dashOffset;
+    free(ctx->dashes);
+    ctx->dashes = sv->dashes; /* ownership moves back */
+    ctx->dashCount = sv->dashCount;
+    sv->dashes = NULL;
+    ctx->lineWidth = sv->lineWidth;
+    ctx->miterLimit = sv->miterLimit;
+    ctx->op = sv->op;
+    ctx->lineCap = sv->lineCap;
+    ctx->lineJoin = VKVG_LINE_JOIN_MITER; /* read from a save field vkvg_save never writes, :1492 */
+    ctx->fillRule = sv->fillRule;
+    if (sv->pattern) {
+        if (sv->pattern != ctx->pattern) /* _update_cur_pattern under the restored CTM */
+            vkvg_set_source(ctx, sv->pattern);
+        vkvg_pattern_destroy(sv->pattern);
+    } else {
+        ctx->curColor = sv->curColor;
+        set_solid(ctx);
+    }
+    free(sv);
 }
 
 /* ---- state */
@@ -1652,6 +1856,12 @@ static int stream_step(VkvgContext c, const uint32_t *w, size_t avail, size_t *u
     case 38: NEED(7); vkvg_elliptic_arc_to(c, F(a), F(a + 1), a[2] != 0, a[3] != 0, F(a + 4), F(a + 5), F(a + 6)); break;
     case 39: NEED(5); vkvg_rounded_rectangle(c, F(a), F(a + 1), F(a + 2), F(a + 3), F(a + 4)); break;
     case 40: NEED(1); vkvg_set_source_color(c, a[0]); break;
+    case 41: vkvg_clip(c); break;
+    case 42: vkvg_clip_preserve(c); break;
+    case 43: vkvg_reset_clip(c); break;
+    case 44: vkvg_save(c); break;
+    case 45: vkvg_restore(c); break;
+    case 46: vkvg_flush(c); break;
     default: return -1;
     }
 #undef NEED
@@ -1731,7 +1941,7 @@ vkvg_status_t vkvg_amd_upload_batch(VkvgDevice dev, uint32_t n, const VkvgSurfac
         for (uint32_t k = 0; k < f->draws.n; k++) {
             vkh_draw d = f->draws.p[k];
             d.first_subpath += sb;
-            if (d.style >= 0)
+            if (d.kind == VKH_DRAW_STROKE && d.style >= 0)
                 d.style += (int32_t)stb;
             if (d.grad >= 0)
                 d.grad += (int32_t)gb;
@@ -1742,6 +1952,8 @@ vkvg_status_t vkvg_amd_upload_batch(VkvgDevice dev, uint32_t n, const VkvgSurfac
         tg[i].width = surfs[i]->width;
         tg[i].height = surfs[i]->height;
         tg[i].sig = surfs[i]->sig;
+        tg[i].stencil = ctxs[i]->stencil_use ? surfs[i]->stencil : NULL;
+        tg[i].stencil_load = 0; /* a fresh context: its stencil starts from zero */
     }
     n_draws_total = all.draws.n;
     if (!st && all.draws.n) {

@@ -378,6 +378,203 @@ def runs_scene(size: int = 256, runs: int = 4, run_len: int = 150) -> Scene:
     return Scene("runs", size, size, s, n)
 
 
+def clip_reference_scenes(width: int = 1024, height: int = 800) -> list[Scene]:
+    """The call sequences of the reference's visual clip tests (tests/clip.c:3-155: clipped paints
+    with both fill rules, reset_clip, transformed clips, a clip_preserve + CLEAR-operator fill, nested
+    clips inside save / restore) and of tests/save_restore.c:3-18 (20 nested saves with transformed
+    strokes), each on a transparent surface. The tests themselves only write images."""
+    out = []
+
+    def clipped_paint(rule, second):
+        st = Stream()
+        st.set_fill_rule(rule)
+        st.set_source_rgb(1, 0, 0)
+        st.paint()
+        st.set_source_rgb(0, 1, 0)
+        st.rectangle(100, 100, 300, 200)
+        st.clip()
+        st.paint()
+        if second:
+            st.reset_clip()
+            st.rectangle(200, 200, 300, 200)
+            st.set_source_rgb(0, 0, 1)
+            st.paint()
+        return st
+
+    for rule, tag in ((FILL_NON_ZERO, "ec"), (FILL_EVEN_ODD, "eo")):
+        out.append(Scene(f"clipped_paint_{tag}", width, height, clipped_paint(rule, False), 0))
+        out.append(Scene(f"clipped_paint2_{tag}", width, height, clipped_paint(rule, True), 1))
+        st = Stream()
+        st.set_fill_rule(rule)
+        st.set_source_rgb(1, 0, 0)
+        st.paint()
+        st.rotate(0.2)
+        st.set_source_rgb(0, 1, 0)
+        st.rectangle(100, 100, 300, 200)
+        st.clip()
+        st.paint()
+        out.append(Scene(f"clipped_transformed_{tag}", width, height, st, 0))
+    st = Stream()
+    st.set_fill_rule(FILL_NON_ZERO)
+    st.set_source_rgb(1, 0, 0)
+    st.paint()
+    st.rotate(0.1)
+    st.set_source_rgb(0, 1, 0)
+    st.rectangle(100, 100, 300, 200)
+    st.clip()
+    st.paint()
+    st.translate(200, 100)
+    st.set_source_rgb(0, 0, 1)
+    st.paint()
+    out.append(Scene("clip_transformed_ec", width, height, st, 0))
+    st = Stream()  # test_clip
+    st.clear()
+    st.set_fill_rule(FILL_EVEN_ODD)
+    st.polyline(np.array([[10, 10], [400, 150], [900, 10], [700, 450], [900, 750], [500, 650], [100, 800],
+                          [150, 400]], np.float32))
+    st.clip_preserve()
+    st.set_operator(0)
+    st.fill_preserve()
+    st.clip()
+    st.set_operator(2)
+    st.set_source_rgb(1, 0, 0)
+    st.paint()
+    out.append(Scene("test_clip", width, height, st, 1))
+    st = Stream()  # test_clip2
+    st.clear()
+    st.set_fill_rule(FILL_NON_ZERO)
+    st.rectangle(50, 50, 500, 500)
+    st.clip()
+    st.set_source_rgb(1, 0, 0)
+    st.paint()
+    st.save()
+    st.rectangle(100, 100, 350, 350)
+    st.clip()
+    st.save()
+    st.set_source_rgb(1, 1, 0)
+    st.paint()
+    st.rectangle(200, 200, 200, 200)
+    st.clip()
+    st.set_source_rgb(0, 1, 0)
+    st.paint()
+    st.restore()
+    st.rectangle(350, 350, 420, 420)
+    st.set_source_rgb(0, 0, 1)
+    st.fill()
+    st.restore()
+    out.append(Scene("test_clip2", width, height, st, 1))
+    st = Stream()  # tests/save_restore.c recurse_draw, depth 20
+    for depth in range(1, 21):
+        st.save()
+        st.translate(5, 5)
+        st.rectangle(float(depth), float(depth), 200, 200)
+        st.set_source_rgb(0, 1, 0)
+        st.stroke()
+    for _ in range(20):
+        st.restore()
+    out.append(Scene("save_restore", width, height, st, 20))
+    return out
+
+
+def clip_mix(variant: int = 0, size: int = 320, n: int = 80, flushes: bool = True) -> Scene:
+    """Random fills, strokes and paints interleaved with clip / clip_preserve (both fill rules,
+    transformed, curved), reset_clip, clear, save / restore with state changes in between (nesting
+    deep enough to pass the six stencil levels), gradients saved and restored, and flushes (frame
+    boundaries that the stencil must survive)."""
+    rng = Rng(0x5EED0000 + 16 * 11 + variant)
+    s = Stream()
+    _white(s)
+    depth = 0
+    draws = 0
+    for i in range(n):
+        r = rng.u()
+        cx, cy = rng.uniform(0, size), rng.uniform(0, size)
+        rad = rng.loguniform(8, size / 2)
+        if r < 0.16:  # clip
+            s.set_fill_rule(FILL_EVEN_ODD if rng.u() < 0.4 else FILL_NON_ZERO)
+            kind = rng.randint(0, 3)
+            if kind == 0:
+                s.rectangle(cx - rad, cy - rad * 0.7, 2 * rad, 1.4 * rad)
+            elif kind == 1:
+                s.arc(cx, cy, rad, 0, 6.2831853)
+            elif kind == 2:  # star polygon: self-intersecting, fill rules differ
+                m = 5
+                ang = np.arange(m) * (4 * math.pi / m) + rng.uniform(0, 1)
+                s.polyline(np.stack([cx + rad * np.cos(ang), cy + rad * np.sin(ang)], 1), close=True)
+            else:
+                pts = rng.uniform(-size * 0.1, size * 1.1, 2 + 6 * 2).astype(np.float32)
+                s.curves(pts[:2], pts[2:].reshape(2, 6), close=True)
+            if rng.u() < 0.3:
+                s.clip_preserve()
+                s.set_source_rgba(*rng.u(3), 0.5)
+                s.stroke()
+                draws += 1
+            else:
+                s.clip()
+            draws += 1
+        elif r < 0.30 and depth < 16:
+            s.save()
+            depth += 1
+            s.translate(float(rng.uniform(-8, 8)), float(rng.uniform(-8, 8)))
+            s.set_line_width(float(rng.loguniform(0.8, 10)))
+            s.set_opacity(float(rng.uniform(0.5, 1.0)))
+            if rng.u() < 0.3:
+                s.set_linear(0, 0, size, size, [[0, *rng.u(3), 1], [1, *rng.u(3), 0.7]])
+        elif r < 0.42 and depth > 0:
+            s.restore()
+            depth -= 1
+        elif r < 0.45:
+            s.reset_clip()
+        elif r < 0.47:
+            s.set_source_rgba(*rng.u(3), rng.uniform(0.3, 1))
+            s.paint()
+            draws += 1
+        elif r < 0.48:
+            s.clear()
+            draws += 1
+        elif r < 0.51 and flushes:
+            s.flush()
+        else:
+            s.set_source_rgba(*rng.u(3), rng.uniform(0.3, 1.0))
+            if rng.u() < 0.5:
+                s.set_fill_rule(FILL_EVEN_ODD if rng.u() < 0.3 else FILL_NON_ZERO)
+                s.rectangle(cx - rad / 2, cy - rad / 3, rad, rad * 0.6)
+                s.arc(cx + rad / 3, cy, rad / 3, 0, 6.2831853)
+                s.fill()
+            else:
+                s.set_line_join(int(rng.randint(0, 2)))
+                m = rng.randint(2, 6)
+                s.polyline(np.stack([rng.uniform(0, size, m), rng.uniform(0, size, m)], 1), close=bool(i % 2))
+                s.stroke()
+            draws += 1
+    while depth > 0:
+        s.restore()
+        depth -= 1
+        s.set_source_rgba(*rng.u(3), 0.4)
+        s.paint()
+        draws += 1
+    return Scene(f"clip_mix{variant}", size, size, s, draws)
+
+
+def deep_clip_scene(size: int = 256, depth: int = 14) -> Scene:
+    """Nested save + clip to shrinking concentric squares, one paint per level, then restores with a
+    paint after each: crosses the six-level stencil byte twice (two whole-stencil copies)."""
+    s = Stream()
+    s.set_source_rgb(1, 1, 1)
+    s.paint()
+    for d in range(1, depth + 1):
+        s.save()
+        s.rectangle(float(8 * d), float(8 * d), float(size - 16 * d), float(size - 16 * d))
+        s.clip()
+        s.set_source_rgb((d * 37 % 256) / 255.0, (d * 71 % 256) / 255.0, (d * 113 % 256) / 255.0)
+        s.paint()
+    for d in range(depth, 0, -1):
+        s.restore()
+        s.set_source_rgba((d * 53 % 256) / 255.0, (d * 29 % 256) / 255.0, (d * 97 % 256) / 255.0, 0.5)
+        s.paint()
+    return Scene("deep_clip", size, size, s, 2 * depth)
+
+
 @dataclass
 class DiffScene:
     """Differentiable-render scene: topology plus FP32 parameters and the target's control points."""

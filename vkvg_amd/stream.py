@@ -49,6 +49,12 @@ OP_SET_MATRIX = 37
 OP_ELLIPTIC_ARC_TO = 38
 OP_ROUNDED_RECT = 39
 OP_SET_SOURCE_COLOR = 40
+OP_CLIP = 41
+OP_CLIP_PRESERVE = 42
+OP_RESET_CLIP = 43
+OP_SAVE = 44
+OP_RESTORE = 45
+OP_FLUSH = 46
 
 # enum values of include/vkvg.h
 JOIN_MITER, JOIN_ROUND, JOIN_BEVEL = 0, 1, 2
@@ -133,6 +139,16 @@ class Stream:
     def stroke_preserve(self): self._op(OP_STROKE_PRESERVE); self.n_paths += 1
     def paint(self): self._op(OP_PAINT)
     def clear(self): self._op(OP_CLEAR)
+
+    # -- clip and graphic-state stack
+    def clip(self): self._op(OP_CLIP)
+    def clip_preserve(self): self._op(OP_CLIP_PRESERVE)
+    def reset_clip(self): self._op(OP_RESET_CLIP)
+    def save(self): self._op(OP_SAVE)
+    def restore(self): self._op(OP_RESTORE)
+    def flush(self):
+        """vkvg_flush: renders what is recorded so far (the next draws go into a new frame)."""
+        self._op(OP_FLUSH)
 
     # -- state
     def set_source_rgba(self, r, g, b, a): self._op(OP_SET_SOURCE_RGBA, r, g, b, a)
