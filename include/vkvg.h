@@ -252,6 +252,7 @@ vkvg_public float vkvg_get_opacity(VkvgContext ctx);                            
 vkvg_public void  vkvg_set_source_color(VkvgContext ctx, uint32_t c);                          /* :1350 */
 vkvg_public void  vkvg_set_source_rgba(VkvgContext ctx, float r, float g, float b, float a);   /* :1361 */
 vkvg_public void  vkvg_set_source_rgb(VkvgContext ctx, float r, float g, float b);             /* :1376 */
+vkvg_public void  vkvg_set_source_surface(VkvgContext ctx, VkvgSurface surf, float x, float y); /* :1386 */
 vkvg_public void  vkvg_set_source(VkvgContext ctx, VkvgPattern pat);                           /* :1394 */
 vkvg_public void  vkvg_set_line_width(VkvgContext ctx, float width);                           /* :1404 */
 vkvg_public void  vkvg_set_miter_limit(VkvgContext ctx, float limit);                          /* :1421 */
@@ -283,6 +284,7 @@ vkvg_public void vkvg_identity_matrix(VkvgContext ctx);                         
 vkvg_public vkvg_status_t vkvg_pattern_status(VkvgPattern pat);                                    /* :1763 */
 vkvg_public VkvgPattern   vkvg_pattern_reference(VkvgPattern pat);                                 /* :1771 */
 vkvg_public uint32_t      vkvg_pattern_get_reference_count(VkvgPattern pat);                       /* :1779 */
+vkvg_public VkvgPattern   vkvg_pattern_create_for_surface(VkvgSurface surf);                       /* :1790 */
 vkvg_public VkvgPattern   vkvg_pattern_create_linear(float x0, float y0, float x1, float y1);      /* :1803 */
 vkvg_public vkvg_status_t vkvg_pattern_edit_linear(VkvgPattern pat, float x0, float y0, float x1, float y1); /* :1815 */
 vkvg_public vkvg_status_t vkvg_pattern_get_linear_points(VkvgPattern pat, float *x0, float *y0, float *x1,

@@ -21,6 +21,10 @@ extern "C" {
  * vkvg_amd/stream.py for the opcode table) by calling the vkvg_* entry points in order.
  * Returns the context status afterwards. */
 vkvg_public vkvg_status_t vkvg_amd_replay(VkvgContext ctx, const uint32_t *words, size_t n_words);
+/* The same, with the surfaces that the stream's surface-source opcodes index (out of range:
+ * VKVG_STATUS_INVALID_INDEX). */
+vkvg_public vkvg_status_t vkvg_amd_replay_sources(VkvgContext ctx, const uint32_t *words, size_t n_words,
+                                                  const VkvgSurface *sources, uint32_t n_sources);
 
 /* Coverage signatures: when enabled, every draw adds k * key(n) to a per-pixel uint32, where
  * k is the number of blends the draw made at that pixel and n the draw's number on the

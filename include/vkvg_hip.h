@@ -80,6 +80,7 @@ typedef struct {
     int32_t  style;          /* stroke style index (STROKE); bit or copy index (stencil draws) */
     uint32_t target;         /* render target index */
     uint32_t key_index;      /* draw number on its target (coverage-signature key) */
+    int32_t  source;         /* surface source index, -1 for none (then grad / color apply) */
 } vkh_draw;
 
 typedef struct {
@@ -100,6 +101,19 @@ typedef struct {
     float    cp[2][4];    /* control points, transformed by the CTM at set_source time */
 } vkh_gradient;
 
+/* a surface used as source (src/vkvg_context_internal.c:705-772, shaders/vkvg_main.frag:72-82):
+ * at pixel centre f, p = f - rect.xy, uv = mat_inv(p) / rect.zw, sampled with the sampler the
+ * pattern's filter and extend select (NEAREST / LINEAR; CLAMP_TO_BORDER with transparent black,
+ * REPEAT, MIRRORED_REPEAT, CLAMP_TO_EDGE) and multiplied by the draw's opacity */
+typedef struct {
+    const void *rgba;     /* device pointer of the source surface's image (RGBA8 as stored) */
+    uint32_t width, height;
+    uint32_t extend;      /* vkvg_extend_t */
+    uint32_t filter;      /* vkvg_filter_t: BILINEAR and BEST sample linearly, others nearest */
+    float    rect[4];     /* the context's push-constant source rectangle: x, y, width, height */
+    float    mat_inv[6];  /* the context's push-constant inverse matrix (xx yx xy yy x0 y0) */
+} vkh_surface_source;
+
 typedef struct {
     void     *rgba;       /* device pointer, width*height*4 bytes, premultiplied RGBA8 */
     uint32_t  width, height;
@@ -118,6 +132,7 @@ typedef struct {
     const float            *dashes;   uint32_t n_dashes;
     const vkh_gradient     *grads;    uint32_t n_grads;
     const vkh_target       *targets;  uint32_t n_targets;
+    const vkh_surface_source *sources; uint32_t n_sources;
 } vkh_frame;
 
 /* per-frame counters (sizes of the intermediate buffers the last render used) */

@@ -84,6 +84,9 @@ enum {
     OVG_OP_SAVE = 44,
     OVG_OP_RESTORE = 45,
     OVG_OP_FLUSH = 46,           /* vkvg_flush: a frame boundary, no drawing state changes */
+    OVG_OP_SET_SOURCE_SURFACE = 47, /* u32 source index, x, y */
+    OVG_OP_SOURCE_PATTERN = 48,  /* u32 source index, u32 extend, u32 filter, u32 has_matrix, 6 floats:
+                                  * pattern for the surface, set_source, destroy */
     OVG_OP_COUNT
 };
 
@@ -98,6 +101,12 @@ typedef struct {
     uint64_t frag_count;  /* fragments blended (stats) */
 } ovg_surface;
 
+/* a source surface of a stream: premultiplied RGBA8 as stored, row-major */
+typedef struct {
+    const uint8_t *rgba;
+    uint32_t width, height;
+} ovg_source;
+
 /* key mixed into the signature for the n-th rasterized draw (a 32-bit bijective hash of n) */
 uint32_t ovg_draw_key(uint32_t n);
 
@@ -106,6 +115,10 @@ uint32_t ovg_draw_key(uint32_t n);
  * Returns the context status (0 = success). */
 int ovg_render_stream(uint32_t width, uint32_t height, const uint32_t *words, size_t nwords,
                       uint8_t *out_rgba, uint32_t *out_sig, uint64_t *out_stats);
+/* the same with source surfaces that OVG_OP_SET_SOURCE_SURFACE / OVG_OP_SOURCE_PATTERN index */
+int ovg_render_stream_src(uint32_t width, uint32_t height, const uint32_t *words, size_t nwords,
+                          const ovg_source *sources, uint32_t n_sources, uint8_t *out_rgba, uint32_t *out_sig,
+                          uint64_t *out_stats);
 
 /* Readback as vkvg_surface_write_to_memory: RGBA with truncating unpremultiply. */
 void ovg_unpremultiply(const uint8_t *premul, uint8_t *out, size_t npixels);

@@ -33,6 +33,9 @@ def _lib(variant: str = "") -> ctypes.CDLL:
     lib.ovg_render_stream.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_size_t,
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     lib.ovg_render_stream.restype = ctypes.c_int
+    lib.ovg_render_stream_src.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_size_t, ctypes.c_void_p,
+                                          ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    lib.ovg_render_stream_src.restype = ctypes.c_int
     lib.ovg_unpremultiply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
     lib.ovg_draw_key.argtypes = [ctypes.c_uint32]
     lib.ovg_draw_key.restype = ctypes.c_uint32
@@ -55,15 +58,24 @@ def _p(a: np.ndarray, t=ctypes.c_uint32):
     return a.ctypes.data_as(ctypes.POINTER(t))
 
 
-def render(width: int, height: int, words: np.ndarray, variant: str = "", with_sig: bool = True):
-    """Returns (premultiplied RGBA8 HxWx4, signature HxW uint32 or None, stats dict, status)."""
+class _Source(ctypes.Structure):
+    _fields_ = [("rgba", ctypes.c_void_p), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32)]
+
+
+def render(width: int, height: int, words: np.ndarray, variant: str = "", with_sig: bool = True,
+           sources: list | None = None):
+    """Returns (premultiplied RGBA8 HxWx4, signature HxW uint32 or None, stats dict, status).
+    `sources`: HxWx4 uint8 images the stream's surface-source opcodes index."""
     lib = _lib(variant)
     words = np.ascontiguousarray(words, np.uint32)
     rgba = np.zeros((height, width, 4), np.uint8)
     sig = np.zeros((height, width), np.uint32) if with_sig else None
     stats = np.zeros(3, np.uint64)
-    st = lib.ovg_render_stream(width, height, _p(words), words.size, rgba.ctypes.data,
-                               sig.ctypes.data if sig is not None else None, stats.ctypes.data)
+    srcs = [np.ascontiguousarray(s, np.uint8) for s in (sources or [])]
+    tab = (_Source * max(1, len(srcs)))(*[_Source(s.ctypes.data, s.shape[1], s.shape[0]) for s in srcs])
+    st = lib.ovg_render_stream_src(width, height, _p(words), words.size, ctypes.addressof(tab), len(srcs),
+                                   rgba.ctypes.data, sig.ctypes.data if sig is not None else None,
+                                   stats.ctypes.data)
     return rgba, sig, {"draws": int(stats[0]), "triangles": int(stats[1]), "fragments": int(stats[2])}, st
 
 

@@ -43,7 +43,15 @@ typedef struct { float x, y; } v2;
 typedef struct { float xx, yx, xy, yy, x0, y0; } mat6;
 typedef struct { float x, y; uint32_t color; } vtx;
 
-enum { SRC_SOLID = 0, SRC_LINEAR = 2, SRC_RADIAL = 3 };
+enum { SRC_SOLID = 0, SRC_SURFACE = 1, SRC_LINEAR = 2, SRC_RADIAL = 3 };
+
+/* a surface pattern as defined (vkvg_pattern_create_for_surface + extend / filter / matrix) */
+typedef struct {
+    uint32_t index;       /* into the stream's sources */
+    uint32_t extend, filter;
+    bool     hasMatrix;
+    mat6     matrix;
+} ovg_surf_pat;
 enum { OP_CLEAR = 0, OP_SOURCE = 1, OP_OVER = 2, OP_DIFFERENCE = 3 };
 
 typedef struct {
@@ -77,8 +85,14 @@ typedef struct octx_s {
     mat6     mat;
     int      srcType;       /* SRC_SOLID or gradient */
     ovg_grad grad;          /* control points already transformed at set_source time */
-    uint32_t patId;         /* identity of the current gradient pattern, 0 for a solid colour */
-    ovg_grad gradDef;       /* that pattern as defined (before the CTM), for vkvg_restore */
+    uint32_t patId;         /* identity of the current pattern, 0 for a solid colour */
+    ovg_grad gradDef;       /* a gradient pattern as defined (before the CTM), for vkvg_restore */
+    ovg_surf_pat surfPat;   /* a surface pattern (srcType SRC_SURFACE) */
+    /* push constants a surface source reads (src/vkvg_context_internal.h:74-81) */
+    float    srcRect[4];    /* x, y, width, height */
+    mat6     matInv;        /* inverse of the CTM (times a surface pattern's matrix) */
+    const ovg_source *sources;
+    uint32_t nSources;
 
     /* clip and save/restore state (src/vkvg_context_internal.h:87-130) */
     int       clipState;    /* OVG_CLIP_* */
@@ -103,9 +117,12 @@ struct osave {
     float   *dashes;
     float    lineWidth, miterLimit, opacity;
     int      op, lineCap, fillRule;
-    mat6     mat;
+    mat6     mat, matInv;
+    float    srcRect[4];
     uint32_t curColor, patId;
+    int      patType;
     ovg_grad gradDef;
+    ovg_surf_pat surfPat;
 };
 
 /* geometry (ovg_geom.c) */
@@ -153,6 +170,8 @@ void o_clear(octx *c);
 void o_save(octx *c);
 void o_restore(octx *c);
 void o_set_gradient(octx *c, const ovg_grad *def);
+void o_set_surface_pattern(octx *c, const ovg_surf_pat *p);
+void o_update_mat_inv(octx *c);
 void o_state_free(octx *c);
 uint32_t o_rgbaf(float r, float g, float b, float a);
 

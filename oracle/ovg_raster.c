@@ -373,9 +373,71 @@ static void grad_color(const octx *c, float fx, float fy, float *col) {
             mix4(col, g->colors[i], smoothstepf(g->stops[i - 1], g->stops[i], grad));
     }
 }
+/* ---- surface sources: texture(source, uv) of shaders/vkvg_main.frag:72-82 with the sampler that
+ * src/vkvg_context_internal.c:730-755 creates, evaluated with the Vulkan specification's formulas
+ * for unnormalized texel coordinates (u' = u * width; NEAREST: floor(u'); LINEAR: floor(u' - 0.5)
+ * and its fraction, the four texels weighted ((w00 t00 + w10 t10) + w01 t01) + w11 t11) */
+enum { EXT_NONE = 0, EXT_REPEAT = 1, EXT_REFLECT = 2, EXT_PAD = 3 };
+static int64_t floor_index(float x) { return (int64_t)fminf(fmaxf(floorf(x), -1073741824.f), 1073741824.f); }
+/* address modes: NONE = CLAMP_TO_BORDER, REPEAT, REFLECT = MIRRORED_REPEAT, PAD = CLAMP_TO_EDGE;
+ * -1 for the border */
+static int64_t wrap_index(int64_t i, int64_t n, uint32_t extend) {
+    switch (extend) {
+    case EXT_REPEAT:
+        return ((i % n) + n) % n;
+    case EXT_REFLECT: {
+        int64_t t = ((i % (2 * n)) + 2 * n) % (2 * n);
+        return t < n ? t : 2 * n - 1 - t;
+    }
+    case EXT_PAD:
+        return i < 0 ? 0 : (i >= n ? n - 1 : i);
+    default:
+        return (i < 0 || i >= n) ? -1 : i;
+    }
+}
+static void texel(const ovg_source *src, int64_t i, int64_t j, uint32_t extend, float *t) {
+    int64_t wi = wrap_index(i, src->width, extend), wj = wrap_index(j, src->height, extend);
+    if (wi < 0 || wj < 0) { /* border colour: transparent black */
+        t[0] = t[1] = t[2] = t[3] = 0.f;
+        return;
+    }
+    const uint8_t *q = src->rgba + 4 * ((size_t)wj * src->width + (size_t)wi);
+    for (int k = 0; k < 4; k++)
+        t[k] = (float)q[k] / 255.0f;
+}
+static void surface_color(const octx *c, float fx, float fy, float *col) {
+    const ovg_surf_pat *sp = &c->surfPat;
+    const ovg_source *src = &c->sources[sp->index];
+    const mat6 *m = &c->matInv;
+    const float px = fx - c->srcRect[0], py = fy - c->srcRect[1];
+    float u = m->xx * px + m->xy * py + m->x0;
+    float v = m->yx * px + m->yy * py + m->y0;
+    u = u / c->srcRect[2];
+    v = v / c->srcRect[3];
+    const float uu = u * (float)src->width, vv = v * (float)src->height;
+    if (sp->filter == 2 || sp->filter == 4) { /* VKVG_FILTER_BEST, VKVG_FILTER_BILINEAR: LINEAR */
+        const float a = uu - 0.5f, b = vv - 0.5f;
+        const float fa = floorf(a), fb = floorf(b);
+        const float al = a - fa, be = b - fb;
+        const int64_t i0 = floor_index(a), j0 = floor_index(b);
+        float t00[4], t10[4], t01[4], t11[4];
+        texel(src, i0, j0, sp->extend, t00);
+        texel(src, i0 + 1, j0, sp->extend, t10);
+        texel(src, i0, j0 + 1, sp->extend, t01);
+        texel(src, i0 + 1, j0 + 1, sp->extend, t11);
+        const float w00 = (1.f - al) * (1.f - be), w10 = al * (1.f - be), w01 = (1.f - al) * be, w11 = al * be;
+        for (int k = 0; k < 4; k++)
+            col[k] = ((w00 * t00[k] + w10 * t10[k]) + w01 * t01[k]) + w11 * t11[k];
+    } else {
+        texel(src, floor_index(uu), floor_index(vv), sp->extend, col);
+    }
+}
+
 static void src_at(const octx *c, uint32_t color, int x, int y, uint8_t *s) {
     float col[4];
-    if (c->srcType == SRC_SOLID) {
+    if (c->srcType == SRC_SURFACE) {
+        surface_color(c, (float)x + 0.5f, (float)y + 0.5f, col);
+    } else if (c->srcType == SRC_SOLID) {
         col[0] = (float)(color & 0xFF) / 255.0f;
         col[1] = (float)((color >> 8) & 0xFF) / 255.0f;
         col[2] = (float)((color >> 16) & 0xFF) / 255.0f;

@@ -29,6 +29,11 @@ static void edit_radial(ovg_grad *g, float cx0, float cy0, float radius0, float 
 }
 int ovg_render_stream(uint32_t width, uint32_t height, const uint32_t *w, size_t nw, uint8_t *out_rgba,
                       uint32_t *out_sig, uint64_t *out_stats) {
+    return ovg_render_stream_src(width, height, w, nw, NULL, 0, out_rgba, out_sig, out_stats);
+}
+
+int ovg_render_stream_src(uint32_t width, uint32_t height, const uint32_t *w, size_t nw, const ovg_source *sources,
+                          uint32_t n_sources, uint8_t *out_rgba, uint32_t *out_sig, uint64_t *out_stats) {
     ovg_surface S;
     memset(&S, 0, sizeof(S));
     S.width = width;
@@ -40,6 +45,10 @@ int ovg_render_stream(uint32_t width, uint32_t height, const uint32_t *w, size_t
     S.acc = (int32_t *)calloc(np, sizeof(int32_t));
     octx c;
     octx_init(&c, &S);
+    c.sources = sources;
+    c.nSources = n_sources;
+    c.srcRect[3] = 1.f; /* {.a = 1}, src/vkvg_context.c:29 */
+    c.matInv = c.mat;
     uint32_t n_patterns = 0;
 
     size_t i = 0;
@@ -82,6 +91,31 @@ int ovg_render_stream(uint32_t width, uint32_t height, const uint32_t *w, size_t
         case OVG_OP_SAVE: o_save(&c); break;
         case OVG_OP_RESTORE: o_restore(&c); break;
         case OVG_OP_FLUSH: break; /* frame boundary: no state change */
+        case OVG_OP_SET_SOURCE_SURFACE: /* src/vkvg_context.c:1025-1033 */
+        case OVG_OP_SOURCE_PATTERN: {
+            if (a[0] >= c.nSources) {
+                c.status = 7; /* VKVG_STATUS_INVALID_INDEX */
+                break;
+            }
+            ovg_surf_pat sp;
+            memset(&sp, 0, sizeof(sp));
+            sp.index = a[0];
+            if (op == OVG_OP_SET_SOURCE_SURFACE) {
+                c.srcRect[0] = F(a + 1);
+                c.srcRect[1] = F(a + 2);
+                i += 3;
+            } else {
+                sp.extend = a[1];
+                sp.filter = a[2];
+                sp.hasMatrix = a[3] != 0;
+                mat6 pm = {F(a + 4), F(a + 5), F(a + 6), F(a + 7), F(a + 8), F(a + 9)};
+                sp.matrix = pm;
+                i += 10;
+            }
+            c.patId = ++n_patterns;
+            o_set_surface_pattern(&c, &sp);
+            break;
+        }
         case OVG_OP_SET_SOURCE_RGBA:
             c.curColor = o_rgbaf(F(a), F(a + 1), F(a + 2), F(a + 3));
             c.srcType = SRC_SOLID;
@@ -149,12 +183,14 @@ int ovg_render_stream(uint32_t width, uint32_t height, const uint32_t *w, size_t
             mat6 t = {1, 0, 0, 1, F(a), F(a + 1)};
             m_multiply(&c.mat, &t, &c.mat);
             i += 2;
+            o_update_mat_inv(&c);
             break;
         }
         case OVG_OP_SCALE: {
             mat6 t = {F(a), 0, 0, F(a + 1), 0, 0};
             m_multiply(&c.mat, &t, &c.mat);
             i += 2;
+            o_update_mat_inv(&c);
             break;
         }
         case OVG_OP_ROTATE: {
@@ -162,17 +198,20 @@ int ovg_render_stream(uint32_t width, uint32_t height, const uint32_t *w, size_t
             mat6 t = {co, s, -s, co, 0, 0};
             m_multiply(&c.mat, &t, &c.mat);
             i += 1;
+            o_update_mat_inv(&c);
             break;
         }
         case OVG_OP_IDENTITY_MATRIX: {
             mat6 t = {1, 0, 0, 1, 0, 0};
             c.mat = t;
+            o_update_mat_inv(&c);
             break;
         }
         case OVG_OP_SET_MATRIX: {
             mat6 t = {F(a), F(a + 1), F(a + 2), F(a + 3), F(a + 4), F(a + 5)};
             c.mat = t;
             i += 6;
+            o_update_mat_inv(&c);
             break;
         }
         default:

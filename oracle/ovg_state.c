@@ -19,6 +19,10 @@ static void stencil_clear(octx *c) {
  * matrix): the control points go through the CTM current at this call */
 void o_set_gradient(octx *c, const ovg_grad *def) {
     ovg_grad g = *def;
+    /* the target's size goes to the unused source field (:782-786) */
+    c->srcRect[0] = (float)c->surf->width;
+    c->srcRect[1] = (float)c->surf->height;
+    c->srcRect[2] = c->srcRect[3] = 0.f;
     if (g.count < 2) {
         c->status = 10; /* VKVG_STATUS_PATTERN_INVALID_GRADIENT */
         return;
@@ -33,6 +37,23 @@ void o_set_gradient(octx *c, const ovg_grad *def) {
     }
     c->grad = g;
     c->srcType = g.type;
+}
+
+/* _update_cur_pattern for SURFACE, src/vkvg_context_internal.c:705-772: the source size, and the
+ * pattern's matrix (not inverted) multiplied into the inverse matrix */
+void o_set_surface_pattern(octx *c, const ovg_surf_pat *p) {
+    c->surfPat = *p;
+    c->srcType = SRC_SURFACE;
+    c->srcRect[2] = (float)c->sources[p->index].width;
+    c->srcRect[3] = (float)c->sources[p->index].height;
+    if (p->hasMatrix)
+        m_multiply(&c->matInv, &c->matInv, &p->matrix);
+}
+
+/* _set_mat_inv_and_vkCmdPush, src/vkvg_context_internal.c:672-676 */
+void o_update_mat_inv(octx *c) {
+    c->matInv = c->mat;
+    m_invert(&c->matInv);
 }
 
 /* vkvg_clip_preserve, src/vkvg_context.c:971-976 -> _clip_preserve :754-795 */
@@ -92,13 +113,18 @@ void o_save(octx *c) {
     sav->op = c->op;
     sav->lineCap = c->lineCap;
     sav->fillRule = c->fillRule;
-    sav->mat = c->mat;         /* pushConsts: the CTM ... */
-    sav->opacity = c->opacity; /* ... and the opacity */
+    sav->mat = c->mat; /* pushConsts: CTM, inverse, source rectangle, opacity */
+    sav->matInv = c->matInv;
+    memcpy(sav->srcRect, c->srcRect, sizeof(sav->srcRect));
+    sav->opacity = c->opacity;
     sav->patId = c->patId;
-    if (c->patId)
+    sav->patType = c->srcType;
+    if (c->patId) {
         sav->gradDef = c->gradDef;
-    else
+        sav->surfPat = c->surfPat;
+    } else {
         sav->curColor = c->curColor;
+    }
     sav->next = c->saved;
     c->saved = sav;
 }
@@ -114,6 +140,8 @@ void o_restore(octx *c) {
     struct osave *sav = c->saved;
     c->saved = sav->next;
     c->mat = sav->mat;
+    c->matInv = sav->matInv;
+    memcpy(c->srcRect, sav->srcRect, sizeof(c->srcRect));
     c->opacity = sav->opacity;
     if (c->clipState != OVG_CLIP_NONE) {
         if (c->clipState == OVG_CLIP_CLIP && sav->clipState == OVG_CLIP_CLEAR) {
@@ -153,10 +181,14 @@ void o_restore(octx *c) {
     c->lineJoin = 0; /* restored from a field vkvg_save never writes (calloc'd: MITER), :1492 */
     c->fillRule = sav->fillRule;
     if (sav->patId) {
-        if (sav->patId != c->patId) { /* _update_cur_pattern with the restored CTM */
+        if (sav->patId != c->patId) { /* _update_cur_pattern with the restored push constants */
             c->patId = sav->patId;
-            c->gradDef = sav->gradDef;
-            o_set_gradient(c, &sav->gradDef);
+            if (sav->patType == SRC_SURFACE) {
+                o_set_surface_pattern(c, &sav->surfPat);
+            } else {
+                c->gradDef = sav->gradDef;
+                o_set_gradient(c, &sav->gradDef);
+            }
         }
     } else {
         c->curColor = sav->curColor;

@@ -56,6 +56,7 @@ EXPORTED = [
     "vkvg_stroke_preserve", "vkvg_fill", "vkvg_fill_preserve", "vkvg_paint", "vkvg_clear", "vkvg_reset_clip",
     "vkvg_clip", "vkvg_clip_preserve", "vkvg_save", "vkvg_restore", "vkvg_set_opacity",
     "vkvg_get_opacity", "vkvg_set_source_color", "vkvg_set_source_rgba", "vkvg_set_source_rgb", "vkvg_set_source",
+    "vkvg_set_source_surface", "vkvg_pattern_create_for_surface",
     "vkvg_set_line_width", "vkvg_set_miter_limit", "vkvg_get_miter_limit", "vkvg_set_line_cap",
     "vkvg_set_line_join", "vkvg_set_operator", "vkvg_set_fill_rule", "vkvg_set_dash", "vkvg_get_dash",
     "vkvg_get_line_width", "vkvg_get_line_cap", "vkvg_get_line_join", "vkvg_get_operator", "vkvg_get_fill_rule",
@@ -67,7 +68,7 @@ EXPORTED = [
     "vkvg_pattern_add_color_stop", "vkvg_pattern_set_extend", "vkvg_pattern_set_filter", "vkvg_pattern_get_extend",
     "vkvg_pattern_get_filter", "vkvg_pattern_get_type", "vkvg_pattern_set_matrix", "vkvg_pattern_get_matrix",
     # include/vkvg_amd.h
-    "vkvg_amd_replay", "vkvg_amd_surface_enable_signature", "vkvg_amd_surface_read_signature",
+    "vkvg_amd_replay", "vkvg_amd_replay_sources", "vkvg_amd_surface_enable_signature", "vkvg_amd_surface_read_signature",
     "vkvg_amd_surface_read_premultiplied", "vkvg_amd_surface_device_ptr", "vkvg_amd_render_batch",
     "vkvg_amd_device_get_stats", "vkvg_amd_device_set_tile_size", "vkvg_amd_device_engine",
     "vkvg_amd_upload", "vkvg_amd_execute", "vkvg_amd_upload_batch",
@@ -116,6 +117,10 @@ def lib() -> ctypes.CDLL:
     L.vkvg_status.restype = ctypes.c_int
     L.vkvg_amd_replay.argtypes = [vp, ctypes.POINTER(u32), sz]
     L.vkvg_amd_replay.restype = ctypes.c_int
+    L.vkvg_amd_replay_sources.argtypes = [vp, ctypes.POINTER(u32), sz, ctypes.POINTER(vp), u32]
+    L.vkvg_amd_replay_sources.restype = ctypes.c_int
+    L.vkvg_surface_create_from_bitmap.argtypes = [vp, vp, u32, u32]
+    L.vkvg_surface_create_from_bitmap.restype = vp
     L.vkvg_amd_surface_enable_signature.argtypes = [vp]
     L.vkvg_amd_surface_enable_signature.restype = ctypes.c_int
     L.vkvg_amd_surface_read_signature.argtypes = [vp, vp]
@@ -230,15 +235,34 @@ class Surface:
         if signature and L.vkvg_amd_surface_enable_signature(self.h):
             raise VkvgError("cannot enable coverage signatures")
 
-    def replay(self, words: np.ndarray) -> None:
-        """vkvg_create + replay of the stream + vkvg_destroy (which renders the frame)."""
+    @classmethod
+    def from_bitmap(cls, dev: "Device", rgba: np.ndarray, signature: bool = False) -> "Surface":
+        """vkvg_surface_create_from_bitmap: the bytes (H x W x 4) become the surface's pixels."""
+        L = lib()
+        a = np.ascontiguousarray(rgba, np.uint8)
+        h, w = a.shape[:2]
+        self = cls.__new__(cls)
+        self.dev, self.width, self.height = dev, w, h
+        self.h = L.vkvg_surface_create_from_bitmap(dev.h, a.ctypes.data, w, h)
+        st = L.vkvg_surface_status(self.h)
+        if st:
+            raise VkvgError(f"vkvg_surface_create_from_bitmap failed (status {st})")
+        if signature and L.vkvg_amd_surface_enable_signature(self.h):
+            raise VkvgError("cannot enable coverage signatures")
+        return self
+
+    def replay(self, words: np.ndarray, sources: list | None = None) -> None:
+        """vkvg_create + replay of the stream + vkvg_destroy (which renders the frame); `sources` are
+        the surfaces the stream's surface-source opcodes index."""
         L = lib()
         w = np.ascontiguousarray(words, np.uint32)
         ctx = L.vkvg_create(self.h)
         st = L.vkvg_status(ctx)
         if st:
             raise VkvgError(f"vkvg_create failed (status {st})")
-        st = L.vkvg_amd_replay(ctx, _u32p(w), w.size)
+        srcs = sources or []
+        hs = (ctypes.c_void_p * max(1, len(srcs)))(*[s.h for s in srcs])
+        st = L.vkvg_amd_replay_sources(ctx, _u32p(w), w.size, hs, len(srcs))
         L.vkvg_destroy(ctx)
         if st:
             raise VkvgError(f"replay failed (status {st}): "

@@ -937,6 +937,7 @@ VkvgDiffScene vkvg_diff_create(uint32_t width, uint32_t height, uint32_t n_paths
         dr[i].kind = stroke[i] ? VKH_DRAW_STROKE : VKH_DRAW_FILL_NZ;
         dr[i].target = 0;
         dr[i].grad = -1;
+        dr[i].source = -1;
     }
     TargetInfo ti;
     memset(&ti, 0, sizeof(ti));

@@ -147,7 +147,7 @@ struct vkh_engine {
 
     // the uploaded (resident) frame
     bool resident = false;
-    size_t o_items, o_sp, o_draws, o_styles, o_dashes, o_grads, o_tg, o_units;
+    size_t o_items, o_sp, o_draws, o_styles, o_dashes, o_grads, o_tg, o_units, o_srcs;
     uint32_t n_items, n_sp, n_draws, n_styles, n_dashes, n_grads, n_targets, n_units;
     uint32_t tile_total, bin_total, frame_tile, coarse_nch = 1, coarse_chunk = VKH_COARSE_CHUNK, n_cubic_items = 0;
     uint32_t most_draws = 1; // most draws on one target
@@ -434,6 +434,17 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
             eng->err = "gradient index out of range";
             return -2;
         }
+        if (dr.source >= 0) {
+            if ((uint32_t)dr.source >= f->n_sources || !f->sources) {
+                eng->err = "surface source index out of range";
+                return -2;
+            }
+            const vkh_surface_source &src = f->sources[dr.source];
+            if (!src.rgba || src.width == 0 || src.height == 0 || src.width > 32768 || src.height > 32768) {
+                eng->err = "invalid surface source";
+                return -2;
+            }
+        }
         if ((uint64_t)dr.first_subpath + dr.n_subpaths > f->n_subpaths) {
             eng->err = "subpath range out of bounds";
             return -2;
@@ -499,7 +510,8 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
     size_t o_grads = align256(o_dashes + sizeof(float) * f->n_dashes);
     size_t o_tg = align256(o_grads + sizeof(vkh_gradient) * f->n_grads);
     size_t o_units = align256(o_tg + sizeof(TargetInfo) * f->n_targets);
-    size_t total = align256(o_units + sizeof(Unit) * n_units) + 256;
+    size_t o_srcs = align256(o_units + sizeof(Unit) * n_units);
+    size_t total = align256(o_srcs + sizeof(vkh_surface_source) * f->n_sources) + 256;
     if (!ensure_host(eng, eng->hblob, total) || !ensure(eng, eng->blob, total))
         return -1;
     uint8_t *hb = (uint8_t *)eng->hblob.p;
@@ -519,6 +531,8 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
         memcpy(hb + o_tg, tg.data(), sizeof(TargetInfo) * f->n_targets);
     if (n_units)
         memcpy(hb + o_units, units.data(), sizeof(Unit) * n_units);
+    if (f->n_sources)
+        memcpy(hb + o_srcs, f->sources, sizeof(vkh_surface_source) * f->n_sources);
     CK(hipMemcpyAsync(eng->blob.p, hb, total, hipMemcpyHostToDevice, s));
     eng->o_items = o_items;
     eng->o_sp = o_sp;
@@ -528,6 +542,7 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
     eng->o_grads = o_grads;
     eng->o_tg = o_tg;
     eng->o_units = o_units;
+    eng->o_srcs = o_srcs;
     eng->n_items = n_items;
     eng->n_sp = n_sp;
     eng->n_draws = n_draws;
@@ -709,7 +724,8 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
         return -1;
     GradRec *d_grec = (GradRec *)(P<DrawPrep>(eng->dprep) + n_draws + 1);
     GradPrep *d_gprep = (GradPrep *)(d_grec + n_draws + 1);
-    vkh::launch_draw_prep(s, d_draws, n_draws, d_tg, d_grads, eng->n_grads, P<DrawPrep>(eng->dprep), d_grec,
+    vkh::launch_draw_prep(s, d_draws, n_draws, d_tg, d_grads, eng->n_grads,
+                          (const vkh_surface_source *)(db + eng->o_srcs), P<DrawPrep>(eng->dprep), d_grec,
                           d_gprep);
     if (scan_u32(eng, P<uint32_t>(eng->pt_count), n_draws, P<uint32_t>(eng->pt_off), cap))
         return -1;

@@ -60,6 +60,7 @@ struct PathTile {
 // the source evaluated once (same float operations as per pixel, so the values are identical)
 enum : uint32_t {
     VKH_DP_GRAD = 1u << 9,     // meta: gradient source
+    VKH_DP_SURF = 1u << 10,    // meta: surface source (the draw's GradRec slot holds a SurfRec)
     VKH_DP_HASSEG = 1u << 8,   // meta (set per command in the fine kernel): the path tile has segments
     VKH_GF_UY0 = 1u,           // linear: direction has no y component
     VKH_GF_UXNEG = 2u,         // linear: ... and points to -x
@@ -97,6 +98,20 @@ struct GradRec {
     uint32_t pad[3];
 };
 static_assert(sizeof(GradRec) == 160, "GradRec layout");
+
+// Per surface-source draw, in the draw's GradRec slot: the source image and the sampler, the
+// push constants the fragment shader reads (source rectangle, inverse matrix) and the opacity
+struct SurfRec {
+    const uint32_t *img;
+    uint32_t width, height;
+    uint32_t extend;      // vkvg_extend_t
+    uint32_t linear;      // 1: LINEAR filtering, 0: NEAREST
+    float rect[4];
+    float m[6];
+    float opacity;
+    uint32_t pad[23];
+};
+static_assert(sizeof(SurfRec) == sizeof(GradRec), "SurfRec shares the GradRec slot");
 
 struct Cmd {
     uint32_t draw;
@@ -151,8 +166,8 @@ void launch_bin_draws(hipStream_t s, const Edge *edges, const uint32_t *edraw, c
                       const DrawTiles *dt, uint32_t n_draws, uint32_t tile, PathTile *pts, Edge *segs,
                       uint32_t seg_cap, uint32_t *seg_alloc);
 void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n_draws, const TargetInfo *tg,
-                      const vkh_gradient *grads, uint32_t n_grads, DrawPrep *out, GradRec *rec,
-                      GradPrep *gout);
+                      const vkh_gradient *grads, uint32_t n_grads, const vkh_surface_source *srcs, DrawPrep *out,
+                      GradRec *rec, GradPrep *gout);
 // draws per coarse workgroup: about 1024 workgroups over n_bins bins, 64 .. VKH_COARSE_CHUNK draws
 uint32_t coarse_chunk(uint32_t most_draws, uint32_t n_bins);
 void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, uint32_t nch, uint32_t chunk, const TargetInfo *tg,

@@ -423,8 +423,8 @@ __device__ __forceinline__ void mix4(float *c, const float *d, float a) {
 // pixel-independent terms of shaders/vkvg_main.frag:83-144 (p = gl_FragCoord.xy / size), each
 // computed with the same float operations the per-pixel evaluation used to repeat.
 __global__ void k_draw_prep(const vkh_draw *__restrict__ draws, uint32_t n, const TargetInfo *__restrict__ tg,
-                            const vkh_gradient *__restrict__ grads, DrawPrep *__restrict__ out,
-                            GradRec *__restrict__ rec) {
+                            const vkh_gradient *__restrict__ grads, const vkh_surface_source *__restrict__ srcs,
+                            DrawPrep *__restrict__ out, GradRec *__restrict__ rec) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
@@ -432,7 +432,8 @@ __global__ void k_draw_prep(const vkh_draw *__restrict__ draws, uint32_t n, cons
     DrawPrep p;
     const uint32_t kind = d.kind;
     const uint32_t op = kind == VKH_DRAW_CLEAR ? 0u : d.op;
-    p.meta = kind | (op << 4) | (d.grad >= 0 ? VKH_DP_GRAD : 0u);
+    const bool surf = d.source >= 0 && kind <= VKH_DRAW_PAINT; // fills, strokes, paint
+    p.meta = kind | (op << 4) | (surf ? VKH_DP_SURF : (d.grad >= 0 ? VKH_DP_GRAD : 0u));
     p.key = draw_key(d.key_index);
     p.grad = d.grad;
     p.opacity = d.opacity;
@@ -506,6 +507,23 @@ __global__ void k_draw_prep(const vkh_draw *__restrict__ draws, uint32_t n, cons
         r.pad[0] = r.pad[1] = r.pad[2] = 0;
         rec[i] = r;
     }
+    if (surf) {
+        const vkh_surface_source &src = srcs[d.source];
+        SurfRec r;
+        r.img = (const uint32_t *)src.rgba;
+        r.width = src.width;
+        r.height = src.height;
+        r.extend = src.extend;
+        r.linear = (src.filter == 2u || src.filter == 4u) ? 1u : 0u; // VKVG_FILTER_BEST, _BILINEAR
+        for (int k = 0; k < 4; k++)
+            r.rect[k] = src.rect[k];
+        for (int k = 0; k < 6; k++)
+            r.m[k] = src.mat_inv[k];
+        r.opacity = d.opacity;
+        for (int k = 0; k < 23; k++)
+            r.pad[k] = 0;
+        *reinterpret_cast<SurfRec *>(rec + i) = r;
+    }
 }
 
 __global__ void k_grad_prep(const vkh_gradient *__restrict__ grads, uint32_t n, GradPrep *__restrict__ out) {
@@ -574,6 +592,65 @@ __device__ void gradient_color(const GradRec &P, const vkh_gradient *__restrict_
 }
 
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+// Surface source sampling: texture(source, uv) of shaders/vkvg_main.frag:72-82 under the sampler of
+// src/vkvg_context_internal.c:730-755, with the Vulkan specification's unnormalized-coordinate
+// formulas (u' = u * width; NEAREST: floor(u'); LINEAR: floor(u' - 0.5) and its fraction, texels
+// summed ((w00 t00 + w10 t10) + w01 t01) + w11 t11). Address modes: extend NONE = CLAMP_TO_BORDER
+// (transparent black), REPEAT, REFLECT = MIRRORED_REPEAT, PAD = CLAMP_TO_EDGE.
+__device__ __forceinline__ int32_t surf_floor_index(float x) {
+    return (int32_t)fminf(fmaxf(floorf(x), -1073741824.f), 1073741824.f);
+}
+__device__ __forceinline__ int32_t surf_wrap(int32_t i, int32_t n, uint32_t extend) {
+    switch (extend) {
+    case 1: { // REPEAT
+        const int32_t r = i % n;
+        return r < 0 ? r + n : r;
+    }
+    case 2: { // MIRRORED_REPEAT
+        int32_t t = i % (2 * n);
+        t = t < 0 ? t + 2 * n : t;
+        return t < n ? t : 2 * n - 1 - t;
+    }
+    case 3: // CLAMP_TO_EDGE
+        return min(max(i, 0), n - 1);
+    default: // CLAMP_TO_BORDER
+        return (i < 0 || i >= n) ? -1 : i;
+    }
+}
+__device__ __forceinline__ void surf_texel(const SurfRec &R, int32_t i, int32_t j, float *t) {
+    const int32_t wi = surf_wrap(i, (int32_t)R.width, R.extend), wj = surf_wrap(j, (int32_t)R.height, R.extend);
+    uint32_t v = 0u;
+    if (wi >= 0 && wj >= 0)
+        v = R.img[(size_t)wj * R.width + (size_t)wi];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        t[k] = (float)((v >> (8 * k)) & 0xFFu) / 255.0f;
+}
+__device__ __forceinline__ void surface_color(const SurfRec &R, float fx, float fy, float *col) {
+    const float px = fx - R.rect[0], py = fy - R.rect[1];
+    float u = R.m[0] * px + R.m[2] * py + R.m[4];
+    float v = R.m[1] * px + R.m[3] * py + R.m[5];
+    u = u / R.rect[2];
+    v = v / R.rect[3];
+    const float uu = u * (float)R.width, vv = v * (float)R.height;
+    if (R.linear) {
+        const float a = uu - 0.5f, b = vv - 0.5f;
+        const float al = a - floorf(a), be = b - floorf(b);
+        const int32_t i0 = surf_floor_index(a), j0 = surf_floor_index(b);
+        float t00[4], t10[4], t01[4], t11[4];
+        surf_texel(R, i0, j0, t00);
+        surf_texel(R, i0 + 1, j0, t10);
+        surf_texel(R, i0, j0 + 1, t01);
+        surf_texel(R, i0 + 1, j0 + 1, t11);
+        const float w00 = (1.f - al) * (1.f - be), w10 = al * (1.f - be), w01 = (1.f - al) * be, w11 = al * be;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            col[k] = ((w00 * t00[k] + w10 * t10[k]) + w01 * t01[k]) + w11 * t11[k];
+    } else {
+        surf_texel(R, surf_floor_index(uu), surf_floor_index(vv), col);
+    }
+}
 
 // One pixel's stencil word under a stencil draw without geometry: byte 0 holds FILL (bit 0, always
 // 0 between draws), CLIP (bit 1) and the CLIP bits of six saved levels (bits 2..7); bytes 1..3 are
@@ -850,11 +927,17 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
                         continue; // clipped
                 }
                 vkh_src16 S = S0;
-                if (meta & VKH_DP_GRAD) {
-                    const GradRec &P = grec[uniform_u32(c_draw[j])]; // scalar loads
-                    const float op_ = P.opacity;
-                    float col[4];
-                    gradient_color(P, grads, gprep, gx[q], gy[q], col);
+                if (meta & (VKH_DP_GRAD | VKH_DP_SURF)) {
+                    float col[4], op_;
+                    if (meta & VKH_DP_GRAD) {
+                        const GradRec &P = grec[uniform_u32(c_draw[j])]; // scalar loads
+                        op_ = P.opacity;
+                        gradient_color(P, grads, gprep, gx[q], gy[q], col);
+                    } else {
+                        const SurfRec &R = *reinterpret_cast<const SurfRec *>(&grec[uniform_u32(c_draw[j])]);
+                        op_ = R.opacity;
+                        surface_color(R, (float)(X0 + lx[q]) + 0.5f, (float)(Y0 + ly[q]) + 0.5f, col);
+                    }
                     S = vkh_src16_of(f2u8x4(col[0] * op_, col[1] * op_, col[2] * op_, col[3] * op_));
                 }
                 // a stroke pixel covered by k triangles is blended k times
@@ -956,10 +1039,10 @@ void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, uint32_t nch, uin
                            chunk, tile_count, tile_off, cmds);
 }
 void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n, const TargetInfo *tg,
-                      const vkh_gradient *grads, uint32_t n_grads, DrawPrep *out, GradRec *rec,
-                      GradPrep *gout) {
+                      const vkh_gradient *grads, uint32_t n_grads, const vkh_surface_source *srcs, DrawPrep *out,
+                      GradRec *rec, GradPrep *gout) {
     if (n)
-        hipLaunchKernelGGL(k_draw_prep, dim3((n + 255) / 256), dim3(256), 0, s, draws, n, tg, grads, out, rec);
+        hipLaunchKernelGGL(k_draw_prep, dim3((n + 255) / 256), dim3(256), 0, s, draws, n, tg, grads, srcs, out, rec);
     if (n_grads)
         hipLaunchKernelGGL(k_grad_prep, dim3((n_grads + 255) / 256), dim3(256), 0, s, grads, n_grads, gout);
 }

@@ -94,6 +94,7 @@ struct _vkvg_pattern_t {
     vkvg_matrix_t       matrix;
     bool                hasMatrix;
     grad_data           g;
+    VkvgSurface         surf; /* SURFACE patterns: the referenced source surface */
 };
 
 typedef struct {
@@ -103,6 +104,7 @@ typedef struct {
     VEC(vkh_stroke_style) styles;
     VEC(float) dashes;
     VEC(vkh_gradient) grads;
+    VEC(vkh_surface_source) srcs;
 } frame_t;
 
 struct _vkvg_context_t {
@@ -123,6 +125,10 @@ struct _vkvg_context_t {
     vkvg_matrix_t    mat;
     VkvgPattern      pattern;  /* referenced source pattern or NULL */
     int32_t          cur_grad; /* index in frame.grads, -1 for solid */
+    /* the push constants a surface source reads (src/vkvg_context_internal.h:74-81): the source
+     * rectangle and the inverse matrix, updated as the reference updates them */
+    float            src_rect[4];
+    vkvg_matrix_t    mat_inv;
 
     /* path recording */
     frame_t  f;
@@ -163,7 +169,8 @@ struct ctx_save {
     vkvg_operator_t  op;
     vkvg_line_cap_t  lineCap;
     vkvg_fill_rule_t fillRule;
-    vkvg_matrix_t    mat;
+    vkvg_matrix_t    mat, mat_inv;
+    float            src_rect[4];
     uint32_t         curColor;
     VkvgPattern      pattern; /* referenced */
 };
@@ -338,19 +345,10 @@ VkvgSurface vkvg_surface_create_from_bitmap(VkvgDevice dev, unsigned char *img, 
     VkvgSurface s = vkvg_surface_create(dev, width, height);
     if (vkvg_surface_status(s) || !img)
         return s;
-    /* RGBA8 straight alpha in, premultiplied on upload as the reference's blit+shader does */
-    size_t n = (size_t)width * height;
-    unsigned char *tmp = (unsigned char *)malloc(n * 4);
-    if (!tmp)
-        return s;
-    for (size_t i = 0; i < n; i++) {
-        unsigned a = img[4 * i + 3];
-        for (int k = 0; k < 3; k++)
-            tmp[4 * i + k] = (unsigned char)((img[4 * i + k] * a + 127) / 255);
-        tmp[4 * i + 3] = (unsigned char)a;
-    }
-    vkvg_hip_image_upload(dev->eng, s->img, tmp, n * 4);
-    free(tmp);
+    /* the reference blits the bytes into the surface and paints them through a surface source onto
+     * the cleared image (src/vkvg_surface.c:82-179): nearest samples at pixel centres, opacity 1,
+     * OVER onto zero, so the surface holds the bytes as given (used as premultiplied from then on) */
+    vkvg_hip_image_upload(dev->eng, s->img, img, (size_t)width * height * 4);
     return s;
 }
 vkvg_status_t vkvg_surface_status(VkvgSurface surf) { return !surf ? VKVG_STATUS_NULL_POINTER : surf->status; }
@@ -645,7 +643,19 @@ void vkvg_pattern_destroy(VkvgPattern pat) {
         return;
     if (--pat->references > 0)
         return;
+    if (pat->surf)
+        vkvg_surface_destroy(pat->surf);
     free(pat);
+}
+/* vkvg_pattern_create_for_surface, src/vkvg_pattern.c:28-49: extend NONE, filter FAST */
+VkvgPattern vkvg_pattern_create_for_surface(VkvgSurface surf) {
+    if (vkvg_surface_status(surf))
+        return (VkvgPattern)&s_null;
+    VkvgPattern p = pattern_new(VKVG_PATTERN_TYPE_SURFACE);
+    if (vkvg_pattern_status(p))
+        return p;
+    p->surf = vkvg_surface_reference(surf);
+    return p;
 }
 
 /* ---------------------------------------------------------------------------- context */
@@ -656,6 +666,7 @@ static void frame_free(frame_t *f) {
     free(f->styles.p);
     free(f->dashes.p);
     free(f->grads.p);
+    free(f->srcs.p);
     memset(f, 0, sizeof(*f));
 }
 
@@ -691,6 +702,8 @@ VkvgContext vkvg_create(VkvgSurface surf) {
     c->lineJoin = VKVG_LINE_JOIN_MITER;
     c->curColor = 0xff000000u;
     vkvg_matrix_init_identity(&c->mat);
+    vkvg_matrix_init_identity(&c->mat_inv);
+    c->src_rect[3] = 1.f; /* {.a = 1}, src/vkvg_context.c:29 */
     c->cur_grad = -1;
     reset_open_subpath(c);
     return c;
@@ -725,6 +738,7 @@ static void frame_reset_keep_path(VkvgContext c) {
     f->draws.n = 0;
     f->styles.n = 0;
     f->dashes.n = 0;
+    f->srcs.n = 0;
     /* keep the current gradient source */
     if (c->cur_grad >= 0) {
         vkh_gradient g = f->grads.p[c->cur_grad];
@@ -758,6 +772,8 @@ static void fill_frame(VkvgContext c, vkh_frame *fr, vkh_target *t) {
     fr->n_dashes = f->dashes.n;
     fr->grads = f->grads.p;
     fr->n_grads = f->grads.n;
+    fr->sources = f->srcs.p;
+    fr->n_sources = f->srcs.n;
     fr->targets = t;
     fr->n_targets = 1;
 }
@@ -1310,6 +1326,30 @@ static vkh_draw *new_draw(VkvgContext c, uint32_t kind) {
     d.style = -1;
     d.target = 0;
     d.key_index = c->surf->draw_counter++;
+    d.source = -1;
+    if (c->pattern && c->pattern->type == VKVG_PATTERN_TYPE_SURFACE) {
+        /* the sampler state of the pattern and the push constants current at this draw */
+        vkh_surface_source src;
+        memset(&src, 0, sizeof(src));
+        const VkvgSurface ss = c->pattern->surf;
+        src.rgba = ss->img;
+        src.width = ss->width;
+        src.height = ss->height;
+        src.extend = (uint32_t)c->pattern->extend;
+        src.filter = (uint32_t)c->pattern->filter;
+        memcpy(src.rect, c->src_rect, sizeof(src.rect));
+        const vkvg_matrix_t *m = &c->mat_inv;
+        src.mat_inv[0] = m->xx;
+        src.mat_inv[1] = m->yx;
+        src.mat_inv[2] = m->xy;
+        src.mat_inv[3] = m->yy;
+        src.mat_inv[4] = m->x0;
+        src.mat_inv[5] = m->y0;
+        if (!c->f.srcs.n || memcmp(&c->f.srcs.p[c->f.srcs.n - 1], &src, sizeof(src)))
+            VEC_PUSH(c->f.srcs, src);
+        d.source = (int32_t)(c->f.srcs.n - 1);
+        d.grad = -1;
+    }
     VEC_PUSH(c->f.draws, d);
     return &c->f.draws.p[c->f.draws.n - 1];
 }
@@ -1414,6 +1454,7 @@ static void stencil_draw(VkvgContext c, uint32_t kind, uint32_t param) {
     d.opacity = 1.f;
     d.mat[0] = d.mat[3] = 1.f;
     d.style = (int32_t)param;
+    d.source = -1;
     VEC_PUSH(c->f.draws, d);
 }
 /* _clip_preserve, src/vkvg_context.c:754-795 */
@@ -1433,6 +1474,7 @@ static void clip_preserve_i(VkvgContext c) {
         c->stencil_use = STENCIL_FIRST_FRAME;
     vkh_draw *d = new_draw(c, c->fillRule == VKVG_FILL_RULE_EVEN_ODD ? VKH_DRAW_CLIP_EO : VKH_DRAW_CLIP_NZ);
     d->grad = -1;
+    d->source = -1;
     c->path_drawn = true;
     c->clip_state = CLIP_STATE_CLIP;
 }
@@ -1499,8 +1541,10 @@ void vkvg_save(VkvgContext ctx) {
     sv->op = ctx->op;
     sv->lineCap = ctx->lineCap;
     sv->fillRule = ctx->fillRule;
-    sv->mat = ctx->mat;         /* the push constants: CTM ... */
-    sv->opacity = ctx->opacity; /* ... and opacity */
+    sv->mat = ctx->mat; /* the push constants: CTM, inverse, source rectangle, opacity */
+    sv->mat_inv = ctx->mat_inv;
+    memcpy(sv->src_rect, ctx->src_rect, sizeof(sv->src_rect));
+    sv->opacity = ctx->opacity;
     if (ctx->pattern)
         sv->pattern = vkvg_pattern_reference(ctx->pattern);
     else
@@ -1519,6 +1563,8 @@ void vkvg_restore(VkvgContext ctx) {
     }
     ctx->saved = sv->next;
     ctx->mat = sv->mat;
+    ctx->mat_inv = sv->mat_inv;
+    memcpy(ctx->src_rect, sv->src_rect, sizeof(ctx->src_rect));
     ctx->opacity = sv->opacity;
     if (ctx->clip_state != CLIP_STATE_NONE) {
         if (ctx->clip_state == CLIP_STATE_CLIP && sv->clip_state == CLIP_STATE_CLEAR)
@@ -1596,14 +1642,29 @@ void vkvg_set_source_rgb(VkvgContext ctx, float r, float g, float b) {
 void vkvg_set_source(VkvgContext ctx, VkvgPattern pat) {
     if (vkvg_status(ctx) || vkvg_pattern_status(pat))
         return;
-    if (pat->type != VKVG_PATTERN_TYPE_LINEAR && pat->type != VKVG_PATTERN_TYPE_RADIAL) {
-        ctx->status = VKVG_STATUS_PATTERN_TYPE_MISMATCH; /* surface sources are not supported */
+    if (pat->type != VKVG_PATTERN_TYPE_LINEAR && pat->type != VKVG_PATTERN_TYPE_RADIAL &&
+        pat->type != VKVG_PATTERN_TYPE_SURFACE) {
+        ctx->status = VKVG_STATUS_PATTERN_TYPE_MISMATCH;
         return;
     }
     vkvg_pattern_reference(pat);
     if (ctx->pattern)
         vkvg_pattern_destroy(ctx->pattern);
     ctx->pattern = pat;
+    if (pat->type == VKVG_PATTERN_TYPE_SURFACE) {
+        /* src/vkvg_context_internal.c:705-772: the source size, and the pattern matrix (not
+         * inverted) multiplied into the inverse matrix */
+        ctx->cur_grad = -1;
+        ctx->src_rect[2] = (float)pat->surf->width;
+        ctx->src_rect[3] = (float)pat->surf->height;
+        if (pat->hasMatrix)
+            vkvg_matrix_multiply(&ctx->mat_inv, &ctx->mat_inv, &pat->matrix);
+        return;
+    }
+    /* the target's size stored in the unused source field (:782-786) */
+    ctx->src_rect[0] = (float)ctx->surf->width;
+    ctx->src_rect[1] = (float)ctx->surf->height;
+    ctx->src_rect[2] = ctx->src_rect[3] = 0.f;
     if (pat->g.count < 2) {
         ctx->status = VKVG_STATUS_PATTERN_INVALID_GRADIENT;
         return;
@@ -1642,6 +1703,20 @@ void vkvg_set_source(VkvgContext ctx, VkvgPattern pat) {
     }
     VEC_PUSH(ctx->f.grads, g);
     ctx->cur_grad = (int32_t)(ctx->f.grads.n - 1);
+}
+/* vkvg_set_source_surface, src/vkvg_context.c:1025-1033 */
+void vkvg_set_source_surface(VkvgContext ctx, VkvgSurface surf, float x, float y) {
+    if (vkvg_status(ctx) || vkvg_surface_status(surf))
+        return;
+    ctx->src_rect[0] = x;
+    ctx->src_rect[1] = y;
+    VkvgPattern p = vkvg_pattern_create_for_surface(surf);
+    if (vkvg_pattern_status(p)) {
+        ctx->status = VKVG_STATUS_NO_MEMORY;
+        return;
+    }
+    vkvg_set_source(ctx, p);
+    vkvg_pattern_destroy(p); /* the context keeps the only reference */
 }
 VkvgPattern vkvg_get_source(VkvgContext ctx) {
     if (vkvg_status(ctx))
@@ -1710,17 +1785,29 @@ void vkvg_get_dash(VkvgContext ctx, const float *dashes, uint32_t *num_dashes, f
 }
 
 /* ---- transforms, src/vkvg_context.c:1514-1577 */
+/* _set_mat_inv_and_vkCmdPush, src/vkvg_context_internal.c:672-676: the inverse of a singular
+ * matrix stays the matrix itself (vkvg_matrix_invert leaves it untouched) */
+static void update_mat_inv(VkvgContext ctx) {
+    ctx->mat_inv = ctx->mat;
+    vkvg_matrix_invert(&ctx->mat_inv);
+}
 void vkvg_translate(VkvgContext ctx, float dx, float dy) {
-    if (!vkvg_status(ctx))
-        vkvg_matrix_translate(&ctx->mat, dx, dy);
+    if (vkvg_status(ctx))
+        return;
+    vkvg_matrix_translate(&ctx->mat, dx, dy);
+    update_mat_inv(ctx);
 }
 void vkvg_scale(VkvgContext ctx, float sx, float sy) {
-    if (!vkvg_status(ctx))
-        vkvg_matrix_scale(&ctx->mat, sx, sy);
+    if (vkvg_status(ctx))
+        return;
+    vkvg_matrix_scale(&ctx->mat, sx, sy);
+    update_mat_inv(ctx);
 }
 void vkvg_rotate(VkvgContext ctx, float radians) {
-    if (!vkvg_status(ctx))
-        vkvg_matrix_rotate(&ctx->mat, radians);
+    if (vkvg_status(ctx))
+        return;
+    vkvg_matrix_rotate(&ctx->mat, radians);
+    update_mat_inv(ctx);
 }
 void vkvg_transform(VkvgContext ctx, const vkvg_matrix_t *matrix) {
     if (vkvg_status(ctx) || !matrix)
@@ -1728,14 +1815,19 @@ void vkvg_transform(VkvgContext ctx, const vkvg_matrix_t *matrix) {
     vkvg_matrix_t res;
     vkvg_matrix_multiply(&res, &ctx->mat, matrix);
     ctx->mat = res;
+    update_mat_inv(ctx);
 }
 void vkvg_identity_matrix(VkvgContext ctx) {
-    if (!vkvg_status(ctx))
-        vkvg_matrix_init_identity(&ctx->mat);
+    if (vkvg_status(ctx))
+        return;
+    vkvg_matrix_init_identity(&ctx->mat);
+    update_mat_inv(ctx);
 }
 void vkvg_set_matrix(VkvgContext ctx, const vkvg_matrix_t *matrix) {
-    if (!vkvg_status(ctx) && matrix)
-        ctx->mat = *matrix;
+    if (vkvg_status(ctx) || !matrix)
+        return;
+    ctx->mat = *matrix;
+    update_mat_inv(ctx);
 }
 void vkvg_get_matrix(VkvgContext ctx, vkvg_matrix_t *const matrix) {
     if (!vkvg_status(ctx) && matrix)
@@ -1775,7 +1867,8 @@ static inline float F(const uint32_t *w) {
     return f;
 }
 /* operand counts per opcode (fixed part); SET_DASH and the gradients carry a count */
-static int stream_step(VkvgContext c, const uint32_t *w, size_t avail, size_t *used) {
+static int stream_step(VkvgContext c, const uint32_t *w, size_t avail, size_t *used, const VkvgSurface *srcs,
+                       uint32_t n_srcs) {
     uint32_t op = w[0];
     const uint32_t *a = w + 1;
     size_t need = 1;
@@ -1862,19 +1955,45 @@ static int stream_step(VkvgContext c, const uint32_t *w, size_t avail, size_t *u
     case 44: vkvg_save(c); break;
     case 45: vkvg_restore(c); break;
     case 46: vkvg_flush(c); break;
+    case 47:
+        NEED(3);
+        if (a[0] >= n_srcs) {
+            c->status = VKVG_STATUS_INVALID_INDEX;
+            break;
+        }
+        vkvg_set_source_surface(c, srcs[a[0]], F(a + 1), F(a + 2));
+        break;
+    case 48: {
+        NEED(10);
+        if (a[0] >= n_srcs) {
+            c->status = VKVG_STATUS_INVALID_INDEX;
+            break;
+        }
+        VkvgPattern p = vkvg_pattern_create_for_surface(srcs[a[0]]);
+        vkvg_pattern_set_extend(p, (vkvg_extend_t)a[1]);
+        vkvg_pattern_set_filter(p, (vkvg_filter_t)a[2]);
+        if (a[3]) {
+            vkvg_matrix_t m = {F(a + 4), F(a + 5), F(a + 6), F(a + 7), F(a + 8), F(a + 9)};
+            vkvg_pattern_set_matrix(p, &m);
+        }
+        vkvg_set_source(c, p);
+        vkvg_pattern_destroy(p);
+        break;
+    }
     default: return -1;
     }
 #undef NEED
     *used = need;
     return 0;
 }
-vkvg_status_t vkvg_amd_replay(VkvgContext ctx, const uint32_t *words, size_t n) {
+vkvg_status_t vkvg_amd_replay_sources(VkvgContext ctx, const uint32_t *words, size_t n, const VkvgSurface *sources,
+                                      uint32_t n_sources) {
     if (vkvg_status(ctx))
         return vkvg_status(ctx);
     size_t i = 0;
     while (i < n && !ctx->status) {
         size_t used = 0;
-        int r = stream_step(ctx, words + i, n - i, &used);
+        int r = stream_step(ctx, words + i, n - i, &used, sources, sources ? n_sources : 0);
         if (r < 0) {
             ctx->status = VKVG_STATUS_INVALID_STATUS;
             break;
@@ -1884,6 +2003,10 @@ vkvg_status_t vkvg_amd_replay(VkvgContext ctx, const uint32_t *words, size_t n) 
         i += used;
     }
     return ctx->status;
+}
+
+vkvg_status_t vkvg_amd_replay(VkvgContext ctx, const uint32_t *words, size_t n) {
+    return vkvg_amd_replay_sources(ctx, words, n, NULL, 0);
 }
 
 /* ---------------------------------------------------------------------------- batch */
@@ -1920,7 +2043,8 @@ vkvg_status_t vkvg_amd_upload_batch(VkvgDevice dev, uint32_t n, const VkvgSurfac
             break;
         /* append this context's frame, rebasing indices */
         frame_t *f = &ctxs[i]->f;
-        uint32_t ib = all.items.n, sb = all.sps.n, stb = all.styles.n, db = all.dashes.n, gb = all.grads.n;
+        uint32_t ib = all.items.n, sb = all.sps.n, stb = all.styles.n, db = all.dashes.n, gb = all.grads.n,
+                 srb = all.srcs.n;
         VEC_RESERVE(all.items, all.items.n + f->items.n);
         memcpy(all.items.p + all.items.n, f->items.p, sizeof(vkh_item) * f->items.n);
         all.items.n += f->items.n;
@@ -1938,6 +2062,8 @@ vkvg_status_t vkvg_amd_upload_batch(VkvgDevice dev, uint32_t n, const VkvgSurfac
             VEC_PUSH(all.dashes, f->dashes.p[k]);
         for (uint32_t k = 0; k < f->grads.n; k++)
             VEC_PUSH(all.grads, f->grads.p[k]);
+        for (uint32_t k = 0; k < f->srcs.n; k++)
+            VEC_PUSH(all.srcs, f->srcs.p[k]);
         for (uint32_t k = 0; k < f->draws.n; k++) {
             vkh_draw d = f->draws.p[k];
             d.first_subpath += sb;
@@ -1945,6 +2071,8 @@ vkvg_status_t vkvg_amd_upload_batch(VkvgDevice dev, uint32_t n, const VkvgSurfac
                 d.style += (int32_t)stb;
             if (d.grad >= 0)
                 d.grad += (int32_t)gb;
+            if (d.source >= 0)
+                d.source += (int32_t)srb;
             d.target = i;
             VEC_PUSH(all.draws, d);
         }
@@ -1970,6 +2098,8 @@ vkvg_status_t vkvg_amd_upload_batch(VkvgDevice dev, uint32_t n, const VkvgSurfac
         fr.n_dashes = all.dashes.n;
         fr.grads = all.grads.p;
         fr.n_grads = all.grads.n;
+        fr.sources = all.srcs.p;
+        fr.n_sources = all.srcs.n;
         fr.targets = tg;
         fr.n_targets = n;
         if (vkvg_hip_upload(dev->eng, &fr)) {

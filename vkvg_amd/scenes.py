@@ -575,6 +575,71 @@ def deep_clip_scene(size: int = 256, depth: int = 14) -> Scene:
     return Scene("deep_clip", size, size, s, 2 * depth)
 
 
+def surface_mix(variant: int = 0, size: int = 256, n: int = 40, n_sources: int = 3):
+    """Surface sources: random bitmaps of uneven sizes painted and used to fill and stroke, placed
+    with set_source_surface offsets or as patterns with every extend mode, nearest and bilinear
+    filters, pattern matrices, rotated / scaled CTMs, opacity, clips and save / restore. Returns the
+    scene and the source bitmaps (H x W x 4 uint8) that its surface-source opcodes index."""
+    rng = Rng(0x5EED0000 + 16 * 12 + variant)
+    sources = []
+    for k in range(n_sources):
+        w, h = int(rng.randint(5, 70)), int(rng.randint(5, 70))
+        px = (rng.u(w * h * 4) * 256).astype(np.int64).clip(0, 255).astype(np.uint8).reshape(h, w, 4)
+        if k % 2 == 0:  # premultiplied-valid texels for half of the sources
+            a = px[..., 3:4].astype(np.uint16)
+            px[..., :3] = (px[..., :3].astype(np.uint16) * a // 255).astype(np.uint8)
+        sources.append(px)
+    s = Stream()
+    _white(s)
+    depth = 0
+    for i in range(n):
+        r = rng.u()
+        k = int(rng.randint(0, n_sources - 1))
+        if r < 0.35:
+            s.set_source_surface(k, float(rng.uniform(-20, size)), float(rng.uniform(-20, size)))
+        elif r < 0.7:
+            mat = None
+            if rng.u() < 0.5:
+                a = float(rng.uniform(0, 6.3))
+                sc = float(rng.loguniform(0.3, 3))
+                mat = (sc * math.cos(a), sc * math.sin(a), -sc * math.sin(a), sc * math.cos(a),
+                       float(rng.uniform(-30, 30)), float(rng.uniform(-30, 30)))
+            s.set_source_pattern(k, extend=int(rng.randint(0, 3)), filt=int(rng.randint(0, 5)), matrix=mat)
+        elif r < 0.78:
+            s.rotate(float(rng.uniform(-0.5, 0.5)))
+        elif r < 0.84:
+            s.scale(float(rng.uniform(0.6, 1.6)), float(rng.uniform(0.6, 1.6)))
+        elif r < 0.88:
+            s.identity_matrix()
+        elif r < 0.92 and depth < 8:
+            s.save()
+            depth += 1
+            cx, cy = rng.uniform(0, size, 2)
+            s.arc(float(cx), float(cy), float(rng.uniform(20, size / 2)), 0, 6.2831853)
+            s.clip()
+        elif depth > 0 and r < 0.96:
+            s.restore()
+            depth -= 1
+        s.set_opacity(float(rng.uniform(0.4, 1.0)) if rng.u() < 0.3 else 1.0)
+        what = rng.u()
+        if what < 0.3:
+            s.paint()
+        elif what < 0.7:
+            cx, cy = rng.uniform(0, size, 2)
+            s.rectangle(float(cx - 40), float(cy - 30), float(rng.uniform(10, 120)), float(rng.uniform(10, 90)))
+            s.arc(float(cx), float(cy), float(rng.uniform(5, 50)), 0, 6.2831853)
+            s.fill()
+        else:
+            s.set_line_width(float(rng.uniform(2, 14)))
+            m = int(rng.randint(2, 5))
+            s.polyline(np.stack([rng.uniform(0, size, m), rng.uniform(0, size, m)], 1))
+            s.stroke()
+    while depth > 0:
+        s.restore()
+        depth -= 1
+    return Scene(f"surface_mix{variant}", size, size, s, n), sources
+
+
 @dataclass
 class DiffScene:
     """Differentiable-render scene: topology plus FP32 parameters and the target's control points."""

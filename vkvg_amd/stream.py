@@ -55,12 +55,16 @@ OP_RESET_CLIP = 43
 OP_SAVE = 44
 OP_RESTORE = 45
 OP_FLUSH = 46
+OP_SET_SOURCE_SURFACE = 47
+OP_SOURCE_PATTERN = 48
 
 # enum values of include/vkvg.h
 JOIN_MITER, JOIN_ROUND, JOIN_BEVEL = 0, 1, 2
 CAP_BUTT, CAP_ROUND, CAP_SQUARE = 0, 1, 2
 FILL_EVEN_ODD, FILL_NON_ZERO = 0, 1
 OPERATOR_CLEAR, OPERATOR_SOURCE, OPERATOR_OVER, OPERATOR_DIFFERENCE = 0, 1, 2, 3
+EXTEND_NONE, EXTEND_REPEAT, EXTEND_REFLECT, EXTEND_PAD = 0, 1, 2, 3
+FILTER_FAST, FILTER_GOOD, FILTER_BEST, FILTER_NEAREST, FILTER_BILINEAR, FILTER_GAUSSIAN = 0, 1, 2, 3, 4, 5
 
 
 def _f(*vals) -> np.ndarray:
@@ -146,6 +150,16 @@ class Stream:
     def reset_clip(self): self._op(OP_RESET_CLIP)
     def save(self): self._op(OP_SAVE)
     def restore(self): self._op(OP_RESTORE)
+    # -- surface sources (index into the surfaces passed with the stream)
+    def set_source_surface(self, index, x, y):
+        self._chunks.append(np.concatenate([np.array([OP_SET_SOURCE_SURFACE, index], np.uint32), _f(x, y)]))
+
+    def set_source_pattern(self, index, extend=EXTEND_NONE, filt=FILTER_FAST, matrix=None):
+        """vkvg_pattern_create_for_surface + set_extend / set_filter (/ set_matrix), set_source, destroy."""
+        m = (1, 0, 0, 1, 0, 0) if matrix is None else tuple(matrix)
+        self._chunks.append(np.concatenate([np.array([OP_SOURCE_PATTERN, index, extend, filt, matrix is not None],
+                                                     np.uint32), _f(*m)]))
+
     def flush(self):
         """vkvg_flush: renders what is recorded so far (the next draws go into a new frame)."""
         self._op(OP_FLUSH)
