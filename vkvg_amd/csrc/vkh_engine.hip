@@ -151,7 +151,7 @@ struct vkh_engine {
     uint32_t n_items, n_sp, n_draws, n_styles, n_dashes, n_grads, n_targets, n_units;
     uint32_t tile_total, bin_total, frame_tile, coarse_nch = 1, coarse_chunk = VKH_COARSE_CHUNK, n_cubic_items = 0;
     uint32_t most_draws = 1; // most draws on one target
-    bool frame_stencil = false; // some target of the frame has a stencil plane
+    bool frame_ext = false; // the frame has surface sources or a target with a stencil plane
 };
 
 static bool check(vkh_engine *e, hipError_t r, const char *what) {
@@ -397,6 +397,7 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
     CK(hipStreamSynchronize(s)); // the staging blob may still feed an earlier copy
 
     // ---- host-side frame tables: work units and targets
+    bool any_stencil = false, any_source = false;
     std::vector<Unit> units;
     units.reserve(f->n_subpaths + 16);
     for (uint32_t d = 0; d < f->n_draws; d++) {
@@ -440,6 +441,7 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
                 return -2;
             }
             const vkh_surface_source &src = f->sources[dr.source];
+            any_source = true;
             if (!src.rgba || src.width == 0 || src.height == 0 || src.width > 32768 || src.height > 32768) {
                 eng->err = "invalid surface source";
                 return -2;
@@ -466,7 +468,6 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
     }
     std::vector<TargetInfo> tg(f->n_targets);
     uint32_t tile_total = 0, bin_total = 0;
-    bool any_stencil = false;
     {
         uint32_t d = 0;
         for (uint32_t t = 0; t < f->n_targets; t++) {
@@ -561,7 +562,7 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
     eng->most_draws = 1;
     for (const TargetInfo &ti : tg)
         eng->most_draws = std::max(eng->most_draws, ti.draw_end - ti.draw_begin);
-    eng->frame_stencil = any_stencil;
+    eng->frame_ext = any_stencil || any_source;
     set_coarse_chunks(eng);
     eng->frame_gen++;
     eng->gfail_key = 0;
@@ -795,7 +796,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
         CK(hipEventRecord(eng->ev_stage[4], s)); // coarse done
     if (!cap)
         CK(hipEventRecord(eng->ev0, s));
-    vkh::launch_fine(s, eng->frame_stencil, T, eng->fine_words, tile_total, d_tg, n_targets, P<DrawPrep>(eng->dprep), d_grec, d_grads, d_gprep,
+    vkh::launch_fine(s, eng->frame_ext, T, eng->fine_words, tile_total, d_tg, n_targets, P<DrawPrep>(eng->dprep), d_grec, d_grads, d_gprep,
                      P<PathTile>(eng->ptiles),
                      P<Edge>(eng->segs), nch, P<uint32_t>(eng->tile_off),
                      P<Cmd>(eng->cmds));

@@ -173,8 +173,8 @@ uint32_t coarse_chunk(uint32_t most_draws, uint32_t n_bins);
 void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, uint32_t nch, uint32_t chunk, const TargetInfo *tg,
                    uint32_t n_targets, const vkh_draw *draws, const DrawTiles *dt, const PathTile *pts,
                    uint32_t *tile_count, const uint32_t *tile_off, Cmd *cmds);
-// clip: some target has a stencil plane (the stencil-aware variant of the kernel runs)
-void launch_fine(hipStream_t s, bool clip, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,
+// ext: the frame has surface sources or a target with a stencil plane (the extended variant runs)
+void launch_fine(hipStream_t s, bool ext, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,
                  uint32_t n_targets, const DrawPrep *dprep, const GradRec *grec, const vkh_gradient *grads,
                  const GradPrep *gprep, const PathTile *pts, const Edge *segs, uint32_t nch,
                  const uint32_t *tile_off, const Cmd *cmds);

@@ -282,10 +282,8 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
 __device__ __forceinline__ int coverage_count(uint32_t kind, int32_t w) {
     switch (kind) {
     case VKH_DRAW_FILL_NZ:
-    case VKH_DRAW_CLIP_NZ:
         return w != 0 ? 1 : 0;
     case VKH_DRAW_FILL_EO:
-    case VKH_DRAW_CLIP_EO:
         return w & 1;
     case VKH_DRAW_STROKE:
         return w > 0 ? w : 0;
@@ -687,11 +685,12 @@ __device__ __forceinline__ uint32_t stencil_apply(uint32_t kind, uint32_t prm, u
 // values. The dependent global loads (command -> draw / path tile -> segment -> edge) are thus
 // paid once per batch and once per segment.
 // WORDS: row-delta words of LDS per workgroup (B = WORDS / pixels commands per batch)
-// CL: the frame has targets with a stencil plane. Each pixel then also carries its stencil word
+// EXT: the frame has surface sources or targets with a stencil plane (the plain variant compiles
+// neither). With a stencil plane each pixel also carries its stencil word
 // (loaded from the plane or zero, stored back at the end): draws blend only where CLIP is 0, clear
 // zeroes the stencil byte, and the clip and stencil draws rewrite it (src/vkvg_context.c:698-795,
 // :1251-1512; pipeline states of src/vkvg_device_internal.c:226-246).
-template <int T, int WORDS, bool CL>
+template <int T, int WORDS, bool EXT>
 __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg, uint32_t n_targets,
                                               const DrawPrep *__restrict__ dprep,
                                               const GradRec *__restrict__ grec,
@@ -712,7 +711,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
 
     uint32_t gtile = blockIdx.x;
     uint32_t c0 = tile_off[(size_t)gtile * nch], c1 = tile_off[(size_t)(gtile + 1) * nch];
-    if (!CL && c0 == c1)
+    if (!EXT && c0 == c1)
         return;
     const uint32_t tid = threadIdx.x;
     uint32_t t = find_target_by(tg, n_targets, gtile, false);
@@ -720,7 +719,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
     uint32_t lt = gtile - TI.tile_base;
     int32_t tx = (int32_t)(lt % TI.tilesW), ty = (int32_t)(lt / TI.tilesW);
     int32_t X0 = tx * T, Y0 = ty * T;
-    if (CL && c0 == c1) {
+    if (EXT && c0 == c1) {
         // no command: the plane keeps its words, or is zeroed when the frame starts from zero
         if (TI.stencil && !TI.stencil_load)
             for (int q = 0; q < NPIX / NT; q++) {
@@ -757,9 +756,9 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
             const uint32_t kind0 = d0.meta & 15, op0 = (d0.meta >> 4) & 15;
             if (kind0 == VKH_DRAW_CLEAR)
                 need_read = false;
-            else if (!CL && kind0 == VKH_DRAW_PAINT && op0 == 0)
+            else if (!EXT && kind0 == VKH_DRAW_PAINT && op0 == 0)
                 need_read = false;
-            else if (!CL && kind0 == VKH_DRAW_PAINT && !(d0.meta & VKH_DP_GRAD) && (op0 == 1 || op0 == 2) &&
+            else if (!EXT && kind0 == VKH_DRAW_PAINT && !(d0.meta & VKH_DP_GRAD) && (op0 == 1 || op0 == 2) &&
                      (d0.solid >> 24) == 255u)
                 need_read = false;
         }
@@ -768,7 +767,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
 #pragma unroll
     for (int q = 0; q < PPT; q++) {
         st[q] = 0u;
-        if (CL && TI.stencil && TI.stencil_load && valid[q])
+        if (EXT && TI.stencil && TI.stencil_load && valid[q])
             st[q] = TI.stencil[(size_t)(Y0 + ly[q]) * TI.width + (size_t)(X0 + lx[q])];
     }
     uint32_t *img = (uint32_t *)TI.rgba;
@@ -891,12 +890,13 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
             const int32_t bk = (int32_t)uniform_u32((uint32_t)c_back[j]);
             const uint32_t key = uniform_u32(c_key[j]);
             const uint32_t solid = uniform_u32(c_solid[j]);
-            if (CL && kind >= VKH_DRAW_CLIP_NZ) { // stencil draws (uniform branch)
+            if (EXT && kind >= VKH_DRAW_CLIP_NZ) { // stencil draws (uniform branch)
                 if (kind_is_clip(kind)) {
 #pragma unroll
                     for (int q = 0; q < PPT; q++) {
                         const int32_t w = bk + (hasseg ? sd[j][ly[q]][lx[q]] : 0);
-                        if (coverage_count(kind, w) && !(st[q] & 2u)) {
+                        const bool cov = kind == VKH_DRAW_CLIP_NZ ? w != 0 : (w & 1) != 0;
+                        if (cov && !(st[q] & 2u)) {
                             if (valid[q])
                                 sig[q] += key; // the clip's 1-sample mask
                         } else {
@@ -920,16 +920,16 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
                 }
                 if (k <= 0 || !valid[q])
                     continue;
-                if (CL) {
+                if (EXT) {
                     if (kind == VKH_DRAW_CLEAR)
                         st[q] &= ~0xFFu; // vkCmdClearAttachments ignores the stencil test
                     else if (st[q] & 2u)
                         continue; // clipped
                 }
                 vkh_src16 S = S0;
-                if (meta & (VKH_DP_GRAD | VKH_DP_SURF)) {
+                if (meta & (EXT ? (VKH_DP_GRAD | VKH_DP_SURF) : VKH_DP_GRAD)) {
                     float col[4], op_;
-                    if (meta & VKH_DP_GRAD) {
+                    if (!EXT || (meta & VKH_DP_GRAD)) {
                         const GradRec &P = grec[uniform_u32(c_draw[j])]; // scalar loads
                         op_ = P.opacity;
                         gradient_color(P, grads, gprep, gx[q], gy[q], col);
@@ -960,7 +960,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
         img[o] = px[q];
         if (TI.sig)
             TI.sig[o] += sig[q];
-        if (CL && TI.stencil)
+        if (EXT && TI.stencil)
             TI.stencil[o] = st[q];
     }
 }
@@ -1047,12 +1047,12 @@ void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n, const Ta
         hipLaunchKernelGGL(k_grad_prep, dim3((n_grads + 255) / 256), dim3(256), 0, s, grads, n_grads, gout);
 }
 template <int T, int W>
-static void fine_launch(hipStream_t s, bool clip, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
+static void fine_launch(hipStream_t s, bool ext, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
                         const DrawPrep *dprep, const GradRec *grec, const vkh_gradient *grads, const GradPrep *gprep,
                         const PathTile *pts, const Edge *segs, uint32_t nch, const uint32_t *tile_off,
                         const Cmd *cmds) {
     constexpr int NT = T * T >= 256 ? 256 : T * T;
-    if (clip)
+    if (ext)
         hipLaunchKernelGGL((k_fine<T, W, true>), dim3(n_tiles), dim3(NT), 0, s, tg, n_targets, dprep, grec, grads, gprep,
                            pts, segs, nch, tile_off, cmds);
     else
@@ -1060,23 +1060,23 @@ static void fine_launch(hipStream_t s, bool clip, uint32_t n_tiles, const Target
                            gprep, pts, segs, nch, tile_off, cmds);
 }
 template <int T>
-static void fine_words(hipStream_t s, bool clip, uint32_t words, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
+static void fine_words(hipStream_t s, bool ext, uint32_t words, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
                        const DrawPrep *dprep, const GradRec *grec, const vkh_gradient *grads, const GradPrep *gprep,
                        const PathTile *pts, const Edge *segs, uint32_t nch, const uint32_t *tile_off,
                        const Cmd *cmds) {
     switch (words) {
     case 2048:
-        fine_launch<T, 2048>(s, clip, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_launch<T, 2048>(s, ext, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     case 8192:
-        fine_launch<T, 8192>(s, clip, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_launch<T, 8192>(s, ext, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     default:
-        fine_launch<T, 4096>(s, clip, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_launch<T, 4096>(s, ext, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     }
 }
-void launch_fine(hipStream_t s, bool clip, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,
+void launch_fine(hipStream_t s, bool ext, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,
                  uint32_t n_targets, const DrawPrep *dprep, const GradRec *grec, const vkh_gradient *grads,
                  const GradPrep *gprep, const PathTile *pts, const Edge *segs, uint32_t nch,
                  const uint32_t *tile_off, const Cmd *cmds) {
@@ -1084,13 +1084,13 @@ void launch_fine(hipStream_t s, bool clip, uint32_t tile, uint32_t words, uint32
         return;
     switch (tile) {
     case 8:
-        fine_words<8>(s, clip, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_words<8>(s, ext, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     case 32:
-        fine_words<32>(s, clip, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_words<32>(s, ext, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     default:
-        fine_words<16>(s, clip, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_words<16>(s, ext, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
         break;
     }
 }
