@@ -41,6 +41,7 @@ typedef struct VkInstance_T *VkInstance;
 typedef struct VkPhysicalDevice_T *VkPhysicalDevice;
 typedef struct VkDevice_T *VkDevice;
 typedef struct VkImage_T *VkImage;
+typedef struct VkPhysicalDeviceFeatures VkPhysicalDeviceFeatures;
 typedef enum VkFormat {
     VK_FORMAT_UNDEFINED = 0,
     VK_FORMAT_R8G8B8A8_UNORM = 37,
@@ -178,6 +179,12 @@ vkvg_public VkvgDevice    vkvg_device_reference(VkvgDevice dev);               /
 vkvg_public uint32_t      vkvg_device_get_reference_count(VkvgDevice dev);     /* :664 */
 vkvg_public void          vkvg_device_set_dpy(VkvgDevice dev, int hdpy, int vdpy);   /* :673 */
 vkvg_public void          vkvg_device_get_dpy(VkvgDevice dev, int *hdpy, int *vdpy); /* :683 */
+/* Vulkan set-up helpers (include/vkvg.h:692-710): this backend needs no Vulkan instance or device,
+ * so it requires no extension and no feature chain */
+vkvg_public void          vkvg_get_required_instance_extensions(const char **pExtensions, uint32_t *pExtCount); /* :692 */
+vkvg_public vkvg_status_t vkvg_get_required_device_extensions(VkPhysicalDevice phy, const char **pExtensions,
+                                                              uint32_t *pExtCount); /* :701 */
+vkvg_public const void   *vkvg_get_device_requirements(VkPhysicalDeviceFeatures *pEnabledFeatures); /* :710 */
 
 /* ---- surface (include/vkvg.h:725-845) */
 vkvg_public VkvgSurface   vkvg_surface_create(VkvgDevice dev, uint32_t width, uint32_t height); /* :725 */
@@ -223,6 +230,8 @@ vkvg_public void vkvg_rel_quadratic_to(VkvgContext ctx, float x1, float y1, floa
 vkvg_public vkvg_status_t vkvg_rectangle(VkvgContext ctx, float x, float y, float w, float h);  /* :1157 */
 vkvg_public vkvg_status_t vkvg_rounded_rectangle(VkvgContext ctx, float x, float y, float w, float h,
                                                  float radius); /* :1170 */
+vkvg_public void vkvg_rounded_rectangle2(VkvgContext ctx, float x, float y, float w, float h, float rx,
+                                         float ry); /* :1184 */
 vkvg_public void vkvg_ellipse(VkvgContext ctx, float radiusX, float radiusY, float x, float y,
                               float rotationAngle); /* :1198 */
 vkvg_public void vkvg_elliptic_arc_to(VkvgContext ctx, float x, float y, bool large_arc_flag, bool sweep_flag,

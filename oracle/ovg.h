@@ -87,6 +87,7 @@ enum {
     OVG_OP_SET_SOURCE_SURFACE = 47, /* u32 source index, x, y */
     OVG_OP_SOURCE_PATTERN = 48,  /* u32 source index, u32 extend, u32 filter, u32 has_matrix, 6 floats:
                                   * pattern for the surface, set_source, destroy */
+    OVG_OP_ROUNDED_RECT2 = 49,   /* x y w h rx ry */
     OVG_OP_COUNT
 };
 

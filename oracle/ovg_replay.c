@@ -91,6 +91,21 @@ int ovg_render_stream_src(uint32_t width, uint32_t height, const uint32_t *w, si
         case OVG_OP_SAVE: o_save(&c); break;
         case OVG_OP_RESTORE: o_restore(&c); break;
         case OVG_OP_FLUSH: break; /* frame boundary: no state change */
+        case OVG_OP_ROUNDED_RECT2: { /* vkvg_rounded_rectangle2, src/vkvg_context.c:665-684 */
+            const float x = F(a), y = F(a + 1), w2 = F(a + 2), h2 = F(a + 3), rx = F(a + 4), ry = F(a + 5);
+            o_move_to(&c, x + rx, y);
+            o_line_to(&c, x + w2 - rx, y);
+            o_elliptic_arc_to(&c, x + w2, y + ry, false, true, rx, ry, 0);
+            o_line_to(&c, x + w2, y + h2 - ry);
+            o_elliptic_arc_to(&c, x + w2 - rx, y + h2, false, true, rx, ry, 0);
+            o_line_to(&c, x + rx, y + h2);
+            o_elliptic_arc_to(&c, x, y + h2 - ry, false, true, rx, ry, 0);
+            o_line_to(&c, x, y + ry);
+            o_elliptic_arc_to(&c, x + rx, y, false, true, rx, ry, 0);
+            o_close_path(&c);
+            i += 6;
+            break;
+        }
         case OVG_OP_SET_SOURCE_SURFACE: /* src/vkvg_context.c:1025-1033 */
         case OVG_OP_SOURCE_PATTERN: {
             if (a[0] >= c.nSources) {

@@ -68,3 +68,31 @@ def test_unbalanced_restore_reports_invalid_restore(device):
             s.replay(st.words())
     finally:
         s.close()
+
+
+def test_clipped_batch_replayed_as_graph(device):
+    # clip frames start from a zero stencil at every execution: eager, graph sizing run and replay
+    # each give the oracle's image, and the signatures accumulate once per execution
+    import vkvg_amd
+    from parity_util import render_oracle
+    scs = [scenes.clip_mix(v, size=100 + 30 * v, n=60, flushes=False) for v in range(3)]
+    surfs = [vkvg_amd.Surface(device, sc.width, sc.height, signature=True) for sc in scs]
+    oracle = [render_oracle(sc) for sc in scs]
+    try:
+        print("case clipped batch, graph replay", flush=True)
+        device.set_tile_size(16)
+        device.set_graph(False)
+        vkvg_amd.upload_batch(device, surfs, [sc.words() for sc in scs])
+        vkvg_amd.execute(device)
+        device.set_graph(True)
+        vkvg_amd.execute(device)
+        vkvg_amd.execute(device)
+        assert device.stats()["graph"] == 1
+        for sc, s, (o_rgba, o_sig, _) in zip(scs, surfs, oracle):
+            assert np.array_equal(s.premultiplied(), o_rgba), sc.name
+            want = (3 * o_sig.astype(np.uint64)) & 0xFFFFFFFF
+            assert np.array_equal(s.signature().astype(np.uint64), want), sc.name
+    finally:
+        device.set_graph(False)
+        for s in surfs:
+            s.close()

@@ -52,7 +52,8 @@ EXPORTED = [
     "vkvg_close_path", "vkvg_new_sub_path", "vkvg_get_current_point", "vkvg_line_to", "vkvg_rel_line_to",
     "vkvg_move_to", "vkvg_rel_move_to", "vkvg_arc", "vkvg_arc_negative", "vkvg_curve_to", "vkvg_rel_curve_to",
     "vkvg_quadratic_to", "vkvg_rel_quadratic_to", "vkvg_rectangle", "vkvg_rounded_rectangle", "vkvg_ellipse",
-    "vkvg_elliptic_arc_to", "vkvg_rel_elliptic_arc_to", "vkvg_has_current_point", "vkvg_stroke",
+    "vkvg_elliptic_arc_to", "vkvg_rel_elliptic_arc_to", "vkvg_rounded_rectangle2",
+    "vkvg_get_required_instance_extensions", "vkvg_get_required_device_extensions", "vkvg_get_device_requirements", "vkvg_has_current_point", "vkvg_stroke",
     "vkvg_stroke_preserve", "vkvg_fill", "vkvg_fill_preserve", "vkvg_paint", "vkvg_clear", "vkvg_reset_clip",
     "vkvg_clip", "vkvg_clip_preserve", "vkvg_save", "vkvg_restore", "vkvg_set_opacity",
     "vkvg_get_opacity", "vkvg_set_source_color", "vkvg_set_source_rgba", "vkvg_set_source_rgb", "vkvg_set_source",

@@ -285,6 +285,22 @@ VkvgDevice vkvg_device_create(vkvg_device_create_info_t *info) {
     dev->hdpy = dev->vdpy = 96;
     return dev;
 }
+void vkvg_get_required_instance_extensions(const char **pExtensions, uint32_t *pExtCount) {
+    (void)pExtensions;
+    if (pExtCount)
+        *pExtCount = 0;
+}
+vkvg_status_t vkvg_get_required_device_extensions(VkPhysicalDevice phy, const char **pExtensions, uint32_t *pExtCount) {
+    (void)phy;
+    (void)pExtensions;
+    if (pExtCount)
+        *pExtCount = 0;
+    return VKVG_STATUS_SUCCESS;
+}
+const void *vkvg_get_device_requirements(VkPhysicalDeviceFeatures *pEnabledFeatures) {
+    (void)pEnabledFeatures;
+    return NULL;
+}
 vkvg_status_t vkvg_device_status(VkvgDevice dev) { return !dev ? VKVG_STATUS_NULL_POINTER : dev->status; }
 VkvgDevice vkvg_device_reference(VkvgDevice dev) {
     if (!vkvg_device_status(dev))
@@ -1180,6 +1196,21 @@ vkvg_status_t vkvg_rounded_rectangle(VkvgContext ctx, float x, float y, float w,
     return VKVG_STATUS_SUCCESS;
 }
 /* vkvg_ellipse, src/vkvg_context.c:1605-1639 */
+/* vkvg_rounded_rectangle2, src/vkvg_context.c:665-684: corners as elliptic arcs */
+void vkvg_rounded_rectangle2(VkvgContext ctx, float x, float y, float w, float h, float rx, float ry) {
+    if (vkvg_status(ctx))
+        return;
+    vkvg_move_to(ctx, x + rx, y);
+    vkvg_line_to(ctx, x + w - rx, y);
+    vkvg_elliptic_arc_to(ctx, x + w, y + ry, false, true, rx, ry, 0);
+    vkvg_line_to(ctx, x + w, y + h - ry);
+    vkvg_elliptic_arc_to(ctx, x + w - rx, y + h, false, true, rx, ry, 0);
+    vkvg_line_to(ctx, x + rx, y + h);
+    vkvg_elliptic_arc_to(ctx, x, y + h - ry, false, true, rx, ry, 0);
+    vkvg_line_to(ctx, x, y + ry);
+    vkvg_elliptic_arc_to(ctx, x + rx, y, false, true, rx, ry, 0);
+    vkvg_close_path(ctx);
+}
 void vkvg_ellipse(VkvgContext ctx, float radiusX, float radiusY, float x, float y, float rot) {
     if (vkvg_status(ctx))
         return;
@@ -1963,6 +1994,7 @@ static int stream_step(VkvgContext c, const uint32_t *w, size_t avail, size_t *u
         }
         vkvg_set_source_surface(c, srcs[a[0]], F(a + 1), F(a + 2));
         break;
+    case 49: NEED(6); vkvg_rounded_rectangle2(c, F(a), F(a + 1), F(a + 2), F(a + 3), F(a + 4), F(a + 5)); break;
     case 48: {
         NEED(10);
         if (a[0] >= n_srcs) {

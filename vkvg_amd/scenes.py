@@ -272,6 +272,8 @@ def feature_mix(variant: int = 0, size: int = 384, n: int = 60) -> Scene:
             s.rounded_rectangle(cx - r, cy - r / 2, 2 * r, r, r * 0.3)
             s.rectangle(cx - r / 2, cy - r / 4, r, r / 2)
             s.fill()
+            s.rounded_rectangle2(cx - r / 3, cy - r / 3, r, r * 0.8, r * 0.2, r * 0.1)
+            s.stroke()
             s.set_fill_rule(FILL_NON_ZERO)
         elif kind == 5:  # quadratic + relative curves, open stroke
             s.move_to(cx, cy)

@@ -57,6 +57,7 @@ OP_RESTORE = 45
 OP_FLUSH = 46
 OP_SET_SOURCE_SURFACE = 47
 OP_SOURCE_PATTERN = 48
+OP_ROUNDED_RECT2 = 49
 
 # enum values of include/vkvg.h
 JOIN_MITER, JOIN_ROUND, JOIN_BEVEL = 0, 1, 2
@@ -107,6 +108,7 @@ class Stream:
     def rectangle(self, x, y, w, h): self._op(OP_RECTANGLE, x, y, w, h)
     def rounded_rectangle(self, x, y, w, h, r): self._op(OP_ROUNDED_RECT, x, y, w, h, r)
     def ellipse(self, rx, ry, x, y, rot): self._op(OP_ELLIPSE, rx, ry, x, y, rot)
+    def rounded_rectangle2(self, x, y, w, h, rx, ry): self._op(OP_ROUNDED_RECT2, x, y, w, h, rx, ry)
 
     def elliptic_arc_to(self, x, y, large, sweep, rx, ry, phi):
         self._chunks.append(np.concatenate([np.array([OP_ELLIPTIC_ARC_TO], np.uint32), _f(x, y),
