@@ -233,6 +233,18 @@ def main() -> int:
                                        "unit": "wave64 VALU instructions/s",
                                        "frac": valu / (fine_ms / 1e3) / issue_peak,
                                        "valu_per_launch": valu}
+    # LDS from the same counter summary: SQ_LDS_IDX_ACTIVE over (GRBM_GUI_ACTIVE per XCD x 256 CUs)
+    # is the fraction of CU-cycles the LDS served indexed accesses (peak 1); LDS atomics per second
+    # against one wave64 LDS instruction per CU per cycle
+    lds_act = _read_pmc("fine_c3_pmc.json", "lds_idx_active_cycles")
+    lds_bank = _read_pmc("fine_c3_pmc.json", "lds_bank_conflict_cycles")
+    gui = _read_pmc("fine_c3_pmc.json", "grbm_gui_active_cycles")
+    lds_atom = _read_pmc("fine_c3_pmc.json", "lds_atomic_insts_per_launch")
+    lds = None
+    if None not in (lds_act, lds_bank, gui, lds_atom):
+        lds = {"util": lds_act / (gui / 8 * 256), "bank_conflict_share": lds_bank / lds_act,
+               "atomic_insts_per_s": lds_atom / (fine_ms / 1e3), "atomic_peak": 256 * 2.4e9,
+               "atomic_frac": lds_atom / (fine_ms / 1e3) / (256 * 2.4e9)}
     out = {
         "metric": METRIC,
         "value": paths_per_s / 1e6,
@@ -261,10 +273,11 @@ def main() -> int:
         "tuned": tune["best"] if tune else {"tile": st["tile_size"], "fine_words": st["fine_words"]},
         "stages_ms": stage,
         "counts": {k: st[k] for k in ("points", "stroke_triangles", "edges", "path_tiles", "segments", "commands")},
-        "roofline": {"kernel": f"k_fine<{st['tile_size']},{st['fine_words']}>", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"kernel": f"k_fine<{st['tile_size']},{st['fine_words']},false>", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": fine_bytes, "avg_launch_ms": fine_ms,
-                     "vector_issue": issue},
+                     "vector_issue": issue, "lds": lds},
     }
     if not args.no_batch:
         out["batched"] = bench_c4(args, dev, vkvg_amd, scenes)
