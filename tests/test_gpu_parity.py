@@ -60,3 +60,35 @@ def test_feature_mix_tiles(device, tile):
 def test_long_runs_of_one_draw_kind(device):
     # fill-edge and stroke-triangle runs spanning more than 64 draws per wavefront
     _check(device, scenes.runs_scene())
+
+
+def test_c2_full_scene(device):
+    # the whole 1920x1080 configuration: 5000 polylines (25 % dashed) and 2000 even-odd stars
+    _check(device, scenes.config2())
+
+
+def test_c3_full_scene(device):
+    # the benchmark frame itself: 20000 paths on 4096x4096, every pixel and every draw's mask
+    _check(device, scenes.config3())
+
+
+def test_c4_full_batch_sampled_surfaces(device):
+    # the benchmark's batched frame (all 256 surfaces in one pipeline); every 16th surface, at its
+    # full size and path count, compared with its own oracle render
+    import vkvg_amd
+    subs = scenes.config4()
+    picked = set(range(0, len(subs), 16))
+    surfs = [vkvg_amd.Surface(device, s.width, s.height, signature=(i in picked)) for i, s in enumerate(subs)]
+    try:
+        device.set_tile_size(16)
+        print("case c4 full batch", flush=True)
+        vkvg_amd.render_batch(device, surfs, [s.words() for s in subs])
+        for i in sorted(picked):
+            sc, s = subs[i], surfs[i]
+            print(f"case c4 surface {i} {sc.width}x{sc.height} {sc.n_paths} paths", flush=True)
+            o_rgba, o_sig, _ = render_oracle(sc)
+            r = compare(s.premultiplied(), s.signature(), o_rgba, o_sig)
+            assert r["mask_mismatch_px"] == 0 and r["rgba_max_err"] == 0, (i, r)
+    finally:
+        for s in surfs:
+            s.close()
