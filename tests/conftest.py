@@ -14,6 +14,12 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def device():
+    # torch (its own HIP runtime) creates its GPU context first, whatever order the tests run in:
+    # some GPU tests use torch tensors beside the library
+    import torch
+    if torch.cuda.is_available():
+        torch.zeros(1, device="cuda")
+        torch.cuda.synchronize()
     import vkvg_amd
     dev = vkvg_amd.Device()
     yield dev
