@@ -77,3 +77,42 @@ def test_source_rendered_on_the_gpu(device):
     finally:
         t.close()
         sa.close()
+
+
+def test_source_released_before_the_frame_renders(device):
+    # the application drops its last reference to the source surface (and the context's pattern)
+    # before the context renders: the frame still samples the source's pixels
+    import ctypes
+    import vkvg_amd
+    from oracle import ovg
+    from vkvg_amd.stream import Stream
+    L = vkvg_amd.lib()
+    b = (np.arange(40 * 30 * 4) * 13 % 256).astype(np.uint8).reshape(30, 40, 4)
+    b[..., 3] = 255
+    src = vkvg_amd.Surface.from_bitmap(device, b)
+    t = vkvg_amd.Surface(device, 64, 48, signature=True)
+    st = Stream()
+    st.set_source_surface(0, 5, 3)
+    st.paint()
+    st.set_source_rgb(0, 0, 1)  # the context lets go of the surface pattern
+    w = st.words()
+    try:
+        print("case source released before render", flush=True)
+        ctx = L.vkvg_create(t.h)
+        hs = (ctypes.c_void_p * 1)(src.h)
+        assert L.vkvg_amd_replay_sources(ctx, w.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), w.size, hs, 1) == 0
+        src.close()  # the last reference the application holds
+        other = vkvg_amd.Surface(device, 40, 30)  # a new image of the same size, painted red
+        red = Stream()
+        red.set_source_rgb(1, 0, 0)
+        red.paint()
+        other.replay(red.words())
+        L.vkvg_destroy(ctx)  # renders the frame
+        other.close()
+        o_rgba, o_sig, _, status = ovg.render(64, 48, w, sources=[b])
+        assert status == 0
+        r = compare(t.premultiplied(), t.signature(), o_rgba, o_sig)
+        assert r["mask_mismatch_px"] == 0 and r["rgba_max_err"] == 0, r
+    finally:
+        t.close()
+        src.close()

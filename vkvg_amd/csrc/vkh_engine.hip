@@ -425,12 +425,7 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
                 return -2;
             }
         }
-        if (!kind_has_tiles(dr.kind))
-            continue;
-        if (dr.kind == VKH_DRAW_STROKE && (dr.style < 0 || (uint32_t)dr.style >= f->n_styles)) {
-            eng->err = "stroke draw without a style";
-            return -2;
-        }
+        // sources are checked for every draw (paints take them too)
         if (dr.grad >= 0 && (uint32_t)dr.grad >= f->n_grads) {
             eng->err = "gradient index out of range";
             return -2;
@@ -446,6 +441,12 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
                 eng->err = "invalid surface source";
                 return -2;
             }
+        }
+        if (!kind_has_tiles(dr.kind))
+            continue;
+        if (dr.kind == VKH_DRAW_STROKE && (dr.style < 0 || (uint32_t)dr.style >= f->n_styles)) {
+            eng->err = "stroke draw without a style";
+            return -2;
         }
         if ((uint64_t)dr.first_subpath + dr.n_subpaths > f->n_subpaths) {
             eng->err = "subpath range out of bounds";

@@ -105,6 +105,7 @@ typedef struct {
     VEC(float) dashes;
     VEC(vkh_gradient) grads;
     VEC(vkh_surface_source) srcs;
+    VEC(VkvgSurface) src_refs; /* source surfaces referenced until the frame has rendered */
 } frame_t;
 
 struct _vkvg_context_t {
@@ -675,7 +676,14 @@ VkvgPattern vkvg_pattern_create_for_surface(VkvgSurface surf) {
 }
 
 /* ---------------------------------------------------------------------------- context */
+static void frame_release_sources(frame_t *f) {
+    for (uint32_t i = 0; i < f->src_refs.n; i++)
+        vkvg_surface_destroy(f->src_refs.p[i]);
+    f->src_refs.n = 0;
+}
 static void frame_free(frame_t *f) {
+    frame_release_sources(f);
+    free(f->src_refs.p);
     free(f->items.p);
     free(f->sps.p);
     free(f->draws.p);
@@ -755,6 +763,7 @@ static void frame_reset_keep_path(VkvgContext c) {
     f->styles.n = 0;
     f->dashes.n = 0;
     f->srcs.n = 0;
+    frame_release_sources(f);
     /* keep the current gradient source */
     if (c->cur_grad >= 0) {
         vkh_gradient g = f->grads.p[c->cur_grad];
@@ -1376,8 +1385,11 @@ static vkh_draw *new_draw(VkvgContext c, uint32_t kind) {
         src.mat_inv[3] = m->yy;
         src.mat_inv[4] = m->x0;
         src.mat_inv[5] = m->y0;
-        if (!c->f.srcs.n || memcmp(&c->f.srcs.p[c->f.srcs.n - 1], &src, sizeof(src)))
+        if (!c->f.srcs.n || memcmp(&c->f.srcs.p[c->f.srcs.n - 1], &src, sizeof(src))) {
             VEC_PUSH(c->f.srcs, src);
+            /* the frame reads the image when it renders: keep the surface alive until then */
+            VEC_PUSH(c->f.src_refs, vkvg_surface_reference(ss));
+        }
         d.source = (int32_t)(c->f.srcs.n - 1);
         d.grad = -1;
     }
