@@ -202,8 +202,8 @@ def main() -> int:
     if args.copy_probe:
         return copy_probe(args.copy_probe)
     if args.gpus != 1:
-        print(json.dumps({"error": "this backend renders on one GPU; multi-GPU sharding is design-only "
-                                   "(see DESIGN.md), run with --gpus 1"}))
+        print(json.dumps({"error": "this backend renders on one GPU (how frames would shard over several is "
+                                   "a design, not built): run with --gpus 1"}))
         return 2
     try:  # torch (used by the differentiable part) must bring up its device before the library does
         import torch
