@@ -116,3 +116,18 @@ def test_source_released_before_the_frame_renders(device):
     finally:
         t.close()
         src.close()
+
+
+@pytest.mark.parametrize("variant", range(8))
+def test_grand_mix_matches_oracle(device, variant):
+    # every feature interleaved at random (sources, operators, transforms, clips, save / restore,
+    # dashes, clears, flushes), three tile sizes
+    from oracle import ovg
+    sc, srcs = scenes.grand_mix(variant)
+    tile = (8, 16, 32)[variant % 3]
+    print(f"case {sc.name} tile {tile}", flush=True)
+    g_rgba, g_sig = _gpu(device, sc, srcs, tile)
+    o_rgba, o_sig, _, st = ovg.render(sc.width, sc.height, sc.words(), sources=srcs)
+    assert st == 0
+    r = compare(g_rgba, g_sig, o_rgba, o_sig)
+    assert r["mask_mismatch_px"] == 0 and r["rgba_max_err"] == 0, r

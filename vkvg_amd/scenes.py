@@ -642,6 +642,100 @@ def surface_mix(variant: int = 0, size: int = 256, n: int = 40, n_sources: int =
     return Scene(f"surface_mix{variant}", size, size, s, n), sources
 
 
+def grand_mix(variant: int = 0, size: int = 288, n: int = 120):
+    """Everything interleaved at random: fills (both rules) and strokes (joins, caps, dashes) of lines,
+    curves and arcs, solid / gradient / surface sources, opacity, operators, transforms, clips,
+    reset_clip, save / restore, clear, paint and flushes. Returns the scene and its source bitmaps."""
+    rng = Rng(0x5EED0000 + 16 * 13 + variant)
+    sources = []
+    for _ in range(2):
+        w, h = int(rng.randint(8, 48)), int(rng.randint(8, 48))
+        px = (rng.u(w * h * 4) * 256).astype(np.int64).clip(0, 255).astype(np.uint8).reshape(h, w, 4)
+        a = px[..., 3:4].astype(np.uint16)
+        px[..., :3] = (px[..., :3].astype(np.uint16) * a // 255).astype(np.uint8)
+        sources.append(px)
+    s = Stream()
+    _white(s)
+    depth = 0
+    for i in range(n):
+        r = rng.u()
+        cx, cy = rng.uniform(0, size), rng.uniform(0, size)
+        rad = float(rng.loguniform(6, size / 2.5))
+        if r < 0.08:
+            s.set_source_rgba(*rng.u(3), float(rng.uniform(0.2, 1)))
+        elif r < 0.13:
+            s.set_linear(cx - rad, cy, cx + rad, cy + rad / 2, [[0, *rng.u(3), 1], [0.5, *rng.u(3), 0.7],
+                                                               [1, *rng.u(3), 0.9]])
+        elif r < 0.17:
+            s.set_radial(cx, cy, rad * 0.1, cx + 3, cy, rad, [[0, *rng.u(3), 0.8], [1, *rng.u(3), 1]])
+        elif r < 0.21:
+            s.set_source_surface(int(rng.randint(0, 1)), float(cx - 20), float(cy - 20))
+        elif r < 0.24:
+            s.set_source_pattern(int(rng.randint(0, 1)), extend=int(rng.randint(0, 3)),
+                                 filt=int(rng.randint(0, 5)), matrix=(0.8, 0.2, -0.2, 0.8, 3, 1))
+        elif r < 0.28:
+            s.set_opacity(float(rng.uniform(0.3, 1.0)))
+        elif r < 0.30:
+            s.set_operator(int(rng.randint(0, 3)))
+        elif r < 0.33:
+            s.rotate(float(rng.uniform(-0.3, 0.3)))
+        elif r < 0.35:
+            s.identity_matrix()
+        elif r < 0.41:
+            s.set_fill_rule(FILL_EVEN_ODD if rng.u() < 0.5 else FILL_NON_ZERO)
+            if rng.u() < 0.5:
+                s.arc(cx, cy, rad, 0, 6.2831853)
+            else:
+                m = 5
+                ang = np.arange(m) * (4 * math.pi / m)
+                s.polyline(np.stack([cx + rad * np.cos(ang), cy + rad * np.sin(ang)], 1), close=True)
+            if rng.u() < 0.3:
+                s.clip_preserve()
+                s.stroke()
+            else:
+                s.clip()
+        elif r < 0.44:
+            s.reset_clip()
+        elif r < 0.50 and depth < 14:
+            s.save()
+            depth += 1
+            s.set_line_width(float(rng.uniform(1, 9)))
+        elif r < 0.56 and depth > 0:
+            s.restore()
+            depth -= 1
+        elif r < 0.57:
+            s.clear()
+        elif r < 0.59:
+            s.paint()
+        elif r < 0.61:
+            s.flush()
+        elif r < 0.80:
+            s.set_fill_rule(FILL_EVEN_ODD if rng.u() < 0.3 else FILL_NON_ZERO)
+            pts = rng.uniform(-size * 0.1, size * 1.1, 2 + 6 * 3).astype(np.float32)
+            s.curves(pts[:2], pts[2:].reshape(3, 6), close=True)
+            s.rectangle(float(cx), float(cy), float(rad), float(rad * 0.5))
+            s.fill()
+        else:
+            s.set_line_width(float(rng.uniform(0.8, 12)))
+            s.set_line_join(int(rng.randint(0, 2)))
+            s.set_line_cap(int(rng.randint(0, 2)))
+            if rng.u() < 0.3:
+                s.set_dash([float(rng.uniform(2, 10)), float(rng.uniform(1, 6))], float(rng.uniform(0, 5)))
+            m = int(rng.randint(2, 6))
+            s.polyline(np.stack([rng.uniform(0, size, m), rng.uniform(0, size, m)], 1), close=bool(i % 3 == 0))
+            if rng.u() < 0.3:
+                s.arc(cx, cy, rad / 2, 0, float(rng.uniform(1, 6)))
+            s.stroke()
+            s.set_dash([], 0.0)
+    while depth > 0:
+        s.restore()
+        depth -= 1
+    s.set_operator(2)
+    s.set_source_rgba(0.1, 0.2, 0.3, 0.5)
+    s.paint()
+    return Scene(f"grand_mix{variant}", size, size, s, n), sources
+
+
 @dataclass
 class DiffScene:
     """Differentiable-render scene: topology plus FP32 parameters and the target's control points."""
