@@ -577,6 +577,563 @@ def deep_clip_scene(size: int = 256, depth: int = 14) -> Scene:
     return Scene("deep_clip", size, size, s, 2 * depth)
 
 
+def _f32(v: float) -> float:
+    return float(np.float32(v))
+
+
+PIF = _f32(math.pi)  # M_PIF of the reference (tests/common/test.h:15)
+
+
+def reference_test_scenes(width: int = 512, height: int = 512, random_size: int = 60) -> list[Scene]:
+    """The call sequences of the reference's visual test programs (tests/*.c), one scene per test
+    function, on a transparent surface of the tests' default size (512 x 512,
+    tests/common/test.c:38-39). `_initCtx` (tests/common/test.c:686-697) sets width 2, MITER, BUTT,
+    no dash, NON_ZERO and clears. Float constants are rounded as the C code rounds them (M_PIF, the
+    float products, (float)-M_PI_2). Tests that draw random shapes use this module's generator and
+    `random_size` items (the reference draws `test_size` = 500 with its own generator) (*). The
+    interactive tests/inverse_colinear.c is swept over its key toggles (join, cap, closed, arcs at
+    either end, dash on / off and the dash's first entry, point count). The tests only write
+    images: these scenes are parity cases, with no expected image of their own."""
+    out: list[Scene] = []
+
+    def add(name, st, n):
+        out.append(Scene(name, width, height, st, n))
+
+    def init_ctx(st):  # tests/common/test.c:686-697 with the default globals (:677-684)
+        st.set_line_width(2)
+        st.set_line_join(JOIN_MITER)
+        st.set_line_cap(CAP_BUTT)
+        st.set_dash([], 0)
+        st.set_fill_rule(FILL_NON_ZERO)
+        st.clear()
+
+    # tests/line_caps.c:3-52
+    st = Stream()
+    st.set_source_rgba(0.9, 0.9, 0.9, 1)
+    st.paint()
+    x, y, dx, dy = 20.0, 20.0, 40.0, 60.0
+    st.set_line_width(30)
+    st.set_source_rgba(0, 0, 0, 1)
+    st.move_to(x, y)
+    st.rel_line_to(0, dy)
+    st.stroke()
+    st.set_line_cap(CAP_SQUARE)
+    st.move_to(x + dx, y)
+    st.rel_line_to(0, dy)
+    st.stroke()
+    st.set_line_cap(CAP_ROUND)
+    st.move_to(x + 2 * dx, y)
+    for mx, my, lx, ly in ((None, None, 0, dy), (dx, -dy, dx, dy), (dx, -dy / 2, dx, 0), (dx, dy / 2, dx, -dy),
+                           (dx, dy, 0, -dy), (dx * 2, dy, -dx, -dy), (dx * 3, dy / 2, -dx, 0)):
+        if mx is not None:
+            st.rel_move_to(mx, my)
+        st.rel_line_to(lx, ly)
+    st.stroke()
+    st.set_line_cap(CAP_BUTT)
+    st.set_line_width(1)
+    st.set_source_rgba(1, 0, 0, 1)
+    st.move_to(x, y)
+    st.rel_line_to(0, dy)
+    for _ in range(2):
+        st.rel_move_to(dx, -dy)
+        st.rel_line_to(0, dy)
+    st.stroke()
+    add("line_caps", st, 4)
+
+    # tests/colinear.c:3-19
+    st = Stream()
+    st.set_source_rgba(0.7, 0.7, 0.7, 1)
+    st.paint()
+    st.set_source_rgba(0, 1, 0, 1)
+    st.set_line_width(10)
+    st.polyline(np.array([[100, 100], [100, 200], [100, 300]], np.float32))
+    st.stroke()
+    add("colinear", st, 1)
+
+    # tests/fill_non_zero.c:3-24
+    st = Stream()
+    st.save()
+    st.set_line_width(30)
+    st.set_fill_rule(FILL_NON_ZERO)
+    st.set_source_rgba(0.1, 0.9, 0.1, 1.0)
+    st.move_to(100, 100)
+    for lx, ly in ((200, 0), (0, 150), (-200, 0)):
+        st.rel_line_to(lx, ly)
+    st.close_path()
+    st.move_to(150, 150)
+    for lx, ly in ((0, 50), (100, 0), (0, -50)):
+        st.rel_line_to(lx, ly)
+    st.close_path()
+    st.fill()
+    add("fill_non_zero", st, 1)
+
+    # tests/radial_gradient.c:3-15
+    st = Stream()
+    st.set_radial(250, 250, 0, 300, 300, 260, [[0, 1, 0, 0, 1], [0.5, 0, 1, 0, 1], [1, 0, 0, 0, 1]])
+    st.paint()
+    add("radial_gradient", st, 0)
+
+    # tests/line_join.c:3-72 (test)
+    st = Stream()
+    x, y = 250.0, 150.0
+    st.set_line_width(100)
+    st.set_source_rgba(0, 1, 0, 1)
+    st.set_line_join(JOIN_ROUND)
+    st.move_to(x, y)
+    for lx, ly in ((-50, 30), (0, 60), (50, 30)):
+        st.rel_line_to(lx, ly)
+    st.close_path()
+    st.stroke()
+    st.set_source_rgba(0, 0, 1, 1)
+    st.move_to(x + 250, y)
+    for lx, ly in ((50, -30), (50, 0), (50, 30), (0, 60), (-50, 70), (-50, 0), (-50, -70)):
+        st.rel_line_to(lx, ly)
+    st.close_path()
+    st.stroke()
+    add("line_join", st, 2)
+
+    # tests/line_join.c:74-122 (test2) and :124-172 (test3)
+    for name, legs, starts, shift in (
+            ("line_join2", ((70, 50), (-70, 50)), (((100, 100), 1, 1), ((200, 600), 1, -1), ((400, 100), -1, 1),
+                                                  ((500, 600), -1, -1)), 500),
+            ("line_join3", ((50, 70), (50, -70)), (((100, 100), 1, 1), ((500, 200), -1, 1), ((100, 400), 1, -1),
+                                                  ((500, 500), -1, -1)), 450)):
+        st = Stream()
+        st.set_fill_rule(FILL_EVEN_ODD)
+        st.set_line_width(30)
+        st.set_line_join(JOIN_ROUND)
+        st.translate(-50, -50)
+        st.set_source_rgb(0.5, 0, 0)
+        reps = 5 if name == "line_join2" else 4
+        for _ in range(2):
+            for (px, py), sx, sy in starts:
+                st.move_to(px, py)
+                for _ in range(reps):
+                    for lx, ly in legs:
+                        st.rel_line_to(sx * lx, sy * ly)
+                st.stroke()
+            st.set_line_join(JOIN_BEVEL)
+            st.translate(shift, 0)
+        add(name, st, 8)
+
+    # tests/dashes.c:3-57, the dash offset after 1, 5 and 50 runs (dash_offset += 0.2f per run)
+    off = np.float32(0)
+    for run in range(1, 51):
+        off = np.float32(off + np.float32(0.2))
+        if run not in (1, 5, 50):
+            continue
+        st = Stream()
+        st.clear()
+        st.set_source_rgb(1, 1, 1)
+        st.paint()
+        st.set_line_cap(CAP_ROUND)
+        st.set_line_join(JOIN_ROUND)
+        st.set_dash([100, 20, 20, 20], float(off))
+        st.set_line_width(4)
+        st.set_source_rgb(0, 0, 1)
+        st.move_to(50, 50)
+        for lx, ly in ((500, 0), (0, 200), (200, 0), (0, 500), (-700, 0)):
+            st.rel_line_to(lx, ly)
+        st.close_path()
+        st.stroke()
+        st.set_dash([0, 20], float(off))
+        st.set_source_rgb(0, 1, 0)
+        st.move_to(100, 100)
+        for lx, ly in ((400, 0), (0, 200), (200, 0), (0, 400), (-600, 0)):
+            st.rel_line_to(lx, ly)
+        st.close_path()
+        st.stroke()
+        st.set_line_width(10)
+        st.set_source_rgb(1, 0, 0)
+        st.set_dash([80, 20], float(off))
+        st.rectangle(200, 300, 200, 200)
+        st.stroke()
+        add(f"dashes_run{run}", st, 3)
+
+    # tests/dashes.c:59-93 (_long_curve, _long_path) with main's globals (:99-103): dash {0, 10},
+    # width 4, ROUND caps
+    w, h = float(width), float(height)
+    for name in ("dashes_path", "dashes_curve"):
+        rng = Rng(0x5EED0100 + (name == "dashes_curve"))
+        st = Stream()
+        st.set_line_width(4)
+        st.set_line_join(JOIN_MITER)
+        st.set_line_cap(CAP_ROUND)
+        st.set_dash([0, 10], 0)
+        st.set_fill_rule(FILL_NON_ZERO)
+        st.clear()
+        st.set_source_rgba(*[_f32(v) for v in rng.u(4)])
+        if name == "dashes_path":
+            ww, hh = w - 10, h - 10
+            pts = np.stack([ww * rng.u(random_size + 1), hh * rng.u(random_size + 1)], 1)
+            st.polyline(pts.astype(np.float32))
+        else:
+            st.move_to(_f32(w * rng.u()), _f32(h * rng.u()))
+            for _ in range(random_size):
+                cx1, cy1, cx2, cy2, x2, y2 = (w * rng.u(), h * rng.u(), w * rng.u(), h * rng.u(), w * rng.u(),
+                                              h * rng.u())
+                st.curve_to(cx1, cy1, cx2, cy2, x2, y2)
+        st.stroke()
+        add(name, st, 1)
+
+    # tests/arcs.c:15-26 (scaled_up), :28-38 (sizes), :40-86 (test)
+    def growing_arcs(st, y, count, end):  # tests/arcs.c:3-13, tests/circles.c:3-10
+        x = np.float32(2)
+        for i in range(1, count):
+            x = np.float32(x + np.float32(0.5) * np.float32(i))
+            st.arc(float(np.float32(x + 2)), y, float(np.float32(0.5) * np.float32(i)), 0, end)
+            x = np.float32(x + np.float32(np.float32(0.5) * np.float32(i) + np.float32(5)))
+
+    st = Stream()
+    st.set_source_rgb(1, 1, 1)
+    st.paint()
+    st.set_source_rgb(0, 0, 0)
+    st.scale(10, 10)
+    st.arc(20, 20, 2.0, 0, _f32(np.float32(PIF) / np.float32(2)))
+    st.stroke()
+    add("arcs_scaled_up", st, 1)
+    st = Stream()
+    st.set_source_rgb(1, 1, 1)
+    st.paint()
+    st.set_source_rgb(0, 0, 0)
+    growing_arcs(st, 100, 40, _f32(np.float32(PIF) * np.float32(1.5)))
+    st.stroke()
+    add("arcs_sizes", st, 1)
+    st = Stream()
+    st.set_source_rgb(1, 1, 1)
+    st.paint()
+    st.set_source_rgb(0, 0, 0)
+    half, full = _f32(np.float32(PIF) / np.float32(2)), _f32(np.float32(PIF) * np.float32(2))
+    st.set_source_rgb(1, 0, 1)
+    st.set_line_width(5.0)
+    st.arc(100, 100, 20, 0, half)
+    st.stroke()
+    st.set_source_rgb(0, 1, 1)
+    st.arc_negative(100, 100, 20, 0, half)
+    st.stroke()
+    st.set_source_rgb(1, 0, 1)
+    st.arc(100, 200, 20, half, 0)
+    st.stroke()
+    st.set_source_rgb(0, 1, 1)
+    st.arc_negative(100, 200, 20, half, 0)
+    st.stroke()
+    st.set_source_rgb(0, 0, 1)
+    st.set_line_width(10.0)
+    st.arc(350, 100, 40, 0, full)
+    st.stroke()
+    st.set_source_rgb(0, 1, 0)
+    st.set_line_width(1.0)
+    for rgb in ((0, 1, 0), (1, 0, 0)):
+        if rgb == (1, 0, 0):
+            st.set_source_rgb(*rgb)
+            st.scale(3, 3)
+        st.arc(150, 100, 3.5, 0, full)
+        st.stroke()
+        st.arc(200, 200, 10, 0, full)
+        st.fill()
+    add("arcs_test", st, 10)
+
+    # tests/stroke.c:3-52
+    st = Stream()
+    st.set_line_width(1)
+    st.set_source_rgba(1, 0, 0, 1)
+    st.polyline(np.array([[200.5, 200.5], [400.5, 200.5], [400.5, 400.5], [200.5, 400.5]], np.float32), close=True)
+    st.stroke()
+    for rgb, o in (((0, 1, 0), 300.5), ((0.5, 1, 0), 320.5)):
+        st.set_source_rgba(*rgb, 1)
+        st.polyline(np.array([[o, o], [o + 200, o], [o + 200, o + 200], [o, o + 200]], np.float32))
+        st.stroke()
+    st.set_line_width(40)
+    st.set_source_rgba(0.5, 0.6, 1, 1.0)
+    st.move_to(700, 475)
+    st.line_to(400, 475)
+    st.stroke()
+    st.set_source_rgba(0, 0.5, 0.5, 0.5)
+    st.move_to(300, 200)
+    st.arc(200, 200, 100, 0, PIF)
+    st.stroke()
+    st.set_line_width(20)
+    st.set_source_rgba(0.1, 0.1, 0.1, 0.5)
+    st.move_to(100, 60)
+    st.line_to(400, 600)
+    st.stroke()
+    st.set_source_rgba(1, 1, 1, 1)
+    st.set_line_width(1)
+    st.rectangle(600.5, 200.5, 100, 60)
+    st.stroke()
+    add("stroke", st, 7)
+
+    # tests/curve.c:4-14 (test3)
+    st = Stream()
+    st.clear()
+    st.set_line_width(20)
+    st.set_source_rgb(1, 0, 0)
+    st.move_to(80, 170)
+    st.curve_to(100, 170, 160, 171, 180, 170)
+    st.close_path()
+    st.stroke()
+    add("curve_test3", st, 1)
+    # tests/curve.c:16-69 (test)
+    st = Stream()
+    st.set_line_width(20)
+    st.scale(2, 2)
+    st.set_line_join(JOIN_BEVEL)
+    st.set_source_rgb(0.5, 0, 0)
+    st.translate(200, 30)
+    st.arc(200, 200, 20, 0, full)
+    st.set_source_rgba(0.5, 0.0, 1.0, 0.5)
+    st.move_to(100, 100)
+    st.line_to(200, 100)
+    st.curve_to(250, 100, 300, 150, 300, 200)
+    st.line_to(300, 300)
+    st.curve_to(300, 350, 250, 400, 200, 400)
+    st.line_to(100, 400)
+    st.curve_to(50, 400, 10, 350, 10, 300)
+    st.line_to(10, 200)
+    st.curve_to(10, 150, 50, 100, 100, 100)
+    st.fill_preserve()
+    st.set_source_rgba(0.1, 0.3, 0.7, 0.5)
+    st.stroke()
+    add("curve_test", st, 2)
+    # tests/curve.c:71-100 (curved_rect)
+    st = Stream()
+    x, y, cw, ch, r = 50.0, 50.0, 150.0, 140.0, 30.0
+    mpi2 = _f32(math.pi / 2)
+    st.scale(2, 2)
+    st.set_line_width(15)
+    st.set_source_rgba(0, 0.5, 0.4, 1)
+    st.move_to(x, y + r)
+    st.arc(x + r, y + r, r, PIF, -mpi2)
+    st.line_to(x + cw - r, y)
+    st.arc(x + cw - r, y + r, r, -mpi2, 0)
+    st.line_to(x + cw, y + ch - r)
+    st.arc(x + cw - r, y + ch - r, r, 0, mpi2)
+    st.line_to(x + r, y + ch)
+    st.arc(x + r, y + ch - r, r, mpi2, PIF)
+    st.line_to(x, y + r)
+    st.close_path()
+    st.fill_preserve()
+    st.set_source_rgba(0.5, 0, 0, 0.5)
+    st.stroke()
+    add("curve_curved_rect", st, 2)
+    # tests/curve.c:102-121 (test2)
+    st = Stream()
+    st.move_to(100, 400)
+    st.curve_to(100, 100, 600, 700, 600, 400)
+    st.curve_to(1000, 100, 100, 800, 1000, 800)
+    st.curve_to(1000, 500, 700, 500, 700, 100)
+    st.close_path()
+    st.set_source_rgba(1, 0, 0, 1)
+    st.set_line_width(40)
+    st.stroke()
+    add("curve_test2", st, 1)
+    # tests/curve.c:123-140 (random_curves_stroke), :142-185 (_long_curv and its fill-rule variants)
+    rng = Rng(0x5EED0102)
+    st = Stream()
+    init_ctx(st)
+    for _ in range(random_size):
+        st.set_source_rgba(*[_f32(v) for v in rng.u(4)])
+        st.move_to(_f32(rng.u() * w), _f32(rng.u() * h))
+        st.curve_to(*[(w if k % 2 == 0 else h) * rng.u() for k in range(6)])
+        st.stroke()
+    add("curve_random_curves_stroke", st, random_size)
+    for k, (rule, stroke_too) in enumerate(((FILL_NON_ZERO, False), (FILL_EVEN_ODD, False),
+                                            (FILL_NON_ZERO, True), (FILL_EVEN_ODD, True))):
+        rng = Rng(0x5EED0110 + k)
+        st = Stream()
+        init_ctx(st)
+        st.set_fill_rule(rule)
+        st.set_source_rgba(*[_f32(v) for v in rng.u(4)])
+        st.move_to(_f32(rng.u() * w), _f32(rng.u() * h))
+        for _ in range(random_size):
+            st.curve_to(*[(w if j % 2 == 0 else h) * rng.u() for j in range(6)])
+        if stroke_too:
+            st.fill_preserve()
+            st.set_source_rgba(*[_f32(v) for v in rng.u(4)])
+            st.stroke()
+        else:
+            st.fill()
+        add(f"curve_long_curv_{'fill_stroke' if stroke_too else 'fill'}_{'nz' if rule else 'eo'}", st, 1 + stroke_too)
+
+    # tests/gradient.c:3-9 (create_grad), :11-65 (paint), :67-76 (paint_repeat: the extend of a gradient
+    # is not read by the reference, src/vkvg_context_internal.c:705-772, so it paints the PAD gradient)
+    def grad3(x0):
+        return (x0, 0, 300, 0), [[0, 1, 0, 0, 1], [0.5, 0, 1, 0, 1], [1, 0, 0, 1, 1]]
+
+    st = Stream()
+    init_ctx(st)
+    st.rectangle(0, 0, 400, 460)
+    st.set_linear(*grad3(0)[0], grad3(0)[1])
+    st.fill()
+    gx = 100.0
+    st.rectangle(gx, 200, 50, 50)
+    st.set_linear(*grad3(gx)[0], grad3(gx)[1])
+    st.fill()
+    gx += 100
+    st.rectangle(10, 10, 300, 50)
+    st.set_linear(10, 0, 300, 0, [[0, 0, 0, 1, 0], [1, 1, 0, 0, 1]])
+    st.fill()
+    for step in (200, 100):
+        gx += step
+        st.rectangle(gx, 200, 50, 50)
+        st.set_linear(*grad3(gx)[0], grad3(gx)[1])
+        st.fill()
+    add("gradient_paint", st, 5)
+    st = Stream()
+    init_ctx(st)
+    st.set_linear(*grad3(0)[0], grad3(0)[1])
+    st.paint()
+    add("gradient_paint_repeat", st, 0)
+    # tests/gradient.c:78-89 (test), :91-108 (test2), :110-132 (gradient_transform), :134-148 (gradient_alpha)
+    st = Stream()
+    init_ctx(st)
+    st.set_linear(*grad3(0)[0], grad3(0)[1])
+    st.rectangle(100, 100, 200, 200)
+    st.set_line_width(20)
+    st.stroke()
+    add("gradient_test", st, 1)
+    st = Stream()
+    init_ctx(st)
+    st.set_source_rgb(1, 0, 0)
+    st.paint()
+    st.set_line_width(20)
+    st.set_linear(100, 0, 300, 0, [[0, 1, 1, 1, 1], [1, 1, 1, 0, 0]])
+    st.rectangle(100, 100, 200, 200)
+    st.fill()
+    add("gradient_test2", st, 1)
+    st = Stream()
+    init_ctx(st)
+    st.translate(200, 100)
+    st.rotate(0.5)
+    st.set_line_width(20)
+    st.set_linear(0, 0, 400, 0, [[0, 1, 0, 0, 1], [0.5, 0, 1, 0, 1], [1, 0, 0, 1, 1]])
+    st.rectangle(0, 0, 400, 200)
+    st.stroke()
+    add("gradient_transform", st, 1)
+    st = Stream()
+    init_ctx(st)
+    st.set_source_rgb(0, 1, 0)
+    st.paint()
+    st.set_linear(10, 0, 300, 0, [[1, 1, 0, 0, 0.5], [1, 1, 0, 0, 0.5]])
+    st.rectangle(10, 10, 300, 200)
+    st.fill()
+    add("gradient_alpha", st, 1)
+
+    # tests/fill_and_stroke.c:3-21
+    st = Stream()
+    for px in (100, 300):
+        st.move_to(px, 100)
+        st.rel_line_to(50, -80)
+        st.rel_line_to(50, 80)
+        st.close_path()
+    st.set_line_width(10.0)
+    st.set_source_rgb(0, 0, 1)
+    st.fill_preserve()
+    st.set_source_rgb(1, 0, 0)
+    st.stroke()
+    add("fill_and_stroke", st, 2)
+
+    # tests/simple_paint.c:3-45: paint, with a rotation or a scale, with and without a path
+    for name, xf, rect, alpha in (("paint", None, False, 0.5), ("paint_with_rotation", "rot", False, 1),
+                                  ("paint_with_scale", "scale", False, 1), ("paint_rect", None, True, 1),
+                                  ("paint_rect_with_rotation", "rot", True, 1),
+                                  ("paint_rect_with_scale", "scale", True, 1)):
+        st = Stream()
+        init_ctx(st)
+        if xf == "rot":
+            st.rotate(45)
+        elif xf == "scale":
+            st.scale(0.2, 0.2)
+        st.set_source_rgba(1, 0, 0, alpha)
+        if rect:
+            st.rectangle(100, 100, 300, 200)
+        st.paint()
+        add(f"simple_{name}", st, 0)
+
+    # tests/compositing.c:3-15 (compositing), :17-30 (opacity)
+    for name in ("compositing", "opacity"):
+        st = Stream()
+        st.clear()
+        st.set_source_rgba(1, 0, 0, 0.5 if name == "compositing" else 1.0)
+        st.rectangle(100, 100, 200, 200)
+        st.fill()
+        if name == "opacity":
+            st.set_opacity(0.5)
+        st.set_source_rgba(0, 0, 1, 0.5 if name == "compositing" else 1.0)
+        st.rectangle(200, 200, 200, 200)
+        st.fill()
+        add(name, st, 2)
+
+    # tests/circles.c:12-22 (scaled_up), :24-34 (fill_and_stroke), :36-52 (sizes)
+    st = Stream()
+    st.clear()
+    st.set_source_rgb(1, 1, 1)
+    st.scale(100, 100)
+    st.arc(2, 2, 0.5, 0, full)
+    st.stroke()
+    add("circles_scaled_up", st, 1)
+    st = Stream()
+    st.clear()
+    st.set_source_rgba(0, 0.1, 0.8, 0.5)
+    st.set_line_width(10)
+    st.arc(300, 300, 150.0, 0, full)
+    st.fill_preserve()
+    st.stroke()
+    add("circles_fill_and_stroke", st, 2)
+    st = Stream()
+    st.clear()
+    st.set_source_rgb(1, 1, 1)
+    growing_arcs(st, 100, 40, full)
+    st.fill()
+    growing_arcs(st, 200, 40, full)
+    st.stroke()
+    st.set_source_rgba(0, 0, 1, 0.4)
+    growing_arcs(st, 300, 40, full)
+    st.fill_preserve()
+    st.set_line_width(5)
+    st.stroke()
+    add("circles_sizes", st, 4)
+
+    # tests/inverse_colinear.c:31-72 (draw), swept over the toggles of its key handler (:74-128)
+    ic_pts = np.array([[150, 150], [200, 300], [250, 150], [280, 350], [300, 100]], np.float32)
+    a15 = _f32(np.float32(PIF) * np.float32(1.5))
+    k = 0
+    for join in (JOIN_MITER, JOIN_ROUND, JOIN_BEVEL):
+        for cap in (CAP_BUTT, CAP_ROUND, CAP_SQUARE):
+            for closed in (False, True):
+                for dash in (None, (0.0, 60.0), (80.0, 60.0)):
+                    start_arc, end_arc = bool(k & 1), bool(k & 2)
+                    n_pts = 2 + k % 4
+                    limit = (10.0, 2.5, 40.0)[k % 3]
+                    k += 1
+                    st = Stream()
+                    st.clear()
+                    if dash is not None:
+                        st.set_dash(list(dash), 0)
+                    st.set_source_rgba(1, 0, 0, 0.3)
+                    st.set_line_width(50.0)
+                    st.set_line_join(join)
+                    st.set_line_cap(cap)
+                    st.set_miter_limit(limit)
+                    if start_arc:
+                        st.arc_negative(float(ic_pts[0, 0]), float(ic_pts[0, 1]), 200, a15, PIF)
+                    else:
+                        st.move_to(float(ic_pts[0, 0]), float(ic_pts[0, 1]))
+                    for i in range(1, n_pts - 1):
+                        st.line_to(float(ic_pts[i, 0]), float(ic_pts[i, 1]))
+                    last = ic_pts[n_pts - 1]
+                    if end_arc:
+                        st.arc_negative(float(last[0]), float(last[1]), 200, a15, PIF)
+                    else:
+                        st.line_to(float(last[0]), float(last[1]))
+                    if closed:
+                        st.close_path()
+                    st.stroke()
+                    add(f"inverse_colinear_j{join}c{cap}{'z' if closed else ''}"
+                        f"{'' if dash is None else 'd%d' % int(dash[0])}_{k - 1}", st, 1)
+    return out
+
+
 def surface_mix(variant: int = 0, size: int = 256, n: int = 40, n_sources: int = 3):
     """Surface sources: random bitmaps of uneven sizes painted and used to fill and stroke, placed
     with set_source_surface offsets or as patterns with every extend mode, nearest and bilinear
