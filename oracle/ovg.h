@@ -88,6 +88,8 @@ enum {
     OVG_OP_SOURCE_PATTERN = 48,  /* u32 source index, u32 extend, u32 filter, u32 has_matrix, 6 floats:
                                   * pattern for the surface, set_source, destroy */
     OVG_OP_ROUNDED_RECT2 = 49,   /* x y w h rx ry */
+    OVG_OP_LINEAR_MATRIX = 50,   /* 6 floats (pattern matrix), then the operands of OVG_OP_LINEAR */
+    OVG_OP_RADIAL_MATRIX = 51,   /* 6 floats (pattern matrix), then the operands of OVG_OP_RADIAL */
     OVG_OP_COUNT
 };
 

@@ -60,6 +60,8 @@ typedef struct {
     float    stops[16];
     float    cp[2][4];
     uint32_t count;
+    bool     hasMatrix;     /* vkvg_pattern_set_matrix */
+    mat6     matrix;
 } ovg_grad;
 
 /* vkvg_clip_state_t, src/vkvg_context_internal.h:94-99 */

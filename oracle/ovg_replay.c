@@ -163,9 +163,18 @@ int ovg_render_stream_src(uint32_t width, uint32_t height, const uint32_t *w, si
         case OVG_OP_SET_OPACITY: c.opacity = F(a); i += 1; break;
         case OVG_OP_SET_OPERATOR: c.op = (int)a[0]; i += 1; break;
         case OVG_OP_LINEAR:
-        case OVG_OP_RADIAL: {
+        case OVG_OP_RADIAL:
+        case OVG_OP_LINEAR_MATRIX:
+        case OVG_OP_RADIAL_MATRIX: {
             ovg_grad g;
             memset(&g, 0, sizeof(g));
+            if (op == OVG_OP_LINEAR_MATRIX || op == OVG_OP_RADIAL_MATRIX) { /* vkvg_pattern_set_matrix */
+                g.hasMatrix = true;
+                g.matrix = (mat6){F(a), F(a + 1), F(a + 2), F(a + 3), F(a + 4), F(a + 5)};
+                a += 6;
+                i += 6;
+                op = op == OVG_OP_LINEAR_MATRIX ? OVG_OP_LINEAR : OVG_OP_RADIAL;
+            }
             uint32_t ncp = (op == OVG_OP_LINEAR) ? 4 : 6;
             if (op == OVG_OP_LINEAR) {
                 g.type = SRC_LINEAR;

@@ -15,8 +15,9 @@ static void stencil_clear(octx *c) {
     memset(c->surf->stencil, 0, (size_t)c->surf->width * c->surf->height);
 }
 
-/* _update_cur_pattern for LINEAR/RADIAL, src/vkvg_context_internal.c:774-826 (no pattern
- * matrix): the control points go through the CTM current at this call */
+/* _update_cur_pattern for LINEAR/RADIAL, src/vkvg_context_internal.c:774-826: the control points
+ * go through the inverse of the pattern's matrix (identity when it has none or it is singular,
+ * :797-803), then through the CTM current at this call */
 void o_set_gradient(octx *c, const ovg_grad *def) {
     ovg_grad g = *def;
     /* the target's size goes to the unused source field (:782-786) */
@@ -27,11 +28,26 @@ void o_set_gradient(octx *c, const ovg_grad *def) {
         c->status = 10; /* VKVG_STATUS_PATTERN_INVALID_GRADIENT */
         return;
     }
+    mat6 pm = g.matrix;
+    if (g.hasMatrix) {
+        if (m_invert(&pm) != 0)
+            pm = (mat6){1, 0, 0, 1, 0, 0};
+        m_transform_point(&pm, &g.cp[0][0], &g.cp[0][1]);
+    }
     m_transform_point(&c->mat, &g.cp[0][0], &g.cp[0][1]);
     if (g.type == SRC_LINEAR) {
+        if (g.hasMatrix)
+            m_transform_point(&pm, &g.cp[0][2], &g.cp[0][3]);
         m_transform_point(&c->mat, &g.cp[0][2], &g.cp[0][3]);
     } else {
+        if (g.hasMatrix) {
+            m_transform_point(&pm, &g.cp[1][0], &g.cp[1][1]);
+        }
         m_transform_point(&c->mat, &g.cp[1][0], &g.cp[1][1]);
+        if (g.hasMatrix) { /* radii, :816-819 */
+            m_transform_distance(&pm, &g.cp[0][2], &g.cp[0][3]);
+            m_transform_distance(&pm, &g.cp[1][2], &g.cp[0][3]);
+        }
         m_transform_distance(&c->mat, &g.cp[0][2], &g.cp[0][3]);
         m_transform_distance(&c->mat, &g.cp[1][2], &g.cp[0][3]); /* as written in the reference */
     }

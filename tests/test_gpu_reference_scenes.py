@@ -47,3 +47,32 @@ def test_reference_test_scenes_in_one_batch(device):
     finally:
         for s in surfs:
             s.close()
+
+
+def test_reference_source_scenes(device):
+    # sources drawn by the library (as the reference's tests draw theirs with vkvg), the oracle's
+    # sources drawn by the oracle; pattern matrices on surface patterns and on gradients
+    import numpy as np
+    import vkvg_amd
+    from oracle import ovg
+    device.set_tile_size(16)
+    for sc, specs in scenes.reference_source_scenes():
+        print("case", sc.name, flush=True)
+        srcs, o_srcs = [], []
+        t = vkvg_amd.Surface(device, sc.width, sc.height, signature=True)
+        try:
+            for w, h, st in specs:
+                s = vkvg_amd.Surface(device, w, h)
+                srcs.append(s)
+                s.replay(st.words())
+                o = ovg.render(w, h, st.words())[0]
+                assert np.array_equal(s.premultiplied(), o), sc.name
+                o_srcs.append(o)
+            t.replay(sc.words(), sources=srcs or None)
+            o_rgba, o_sig, _, status = ovg.render(sc.width, sc.height, sc.words(), sources=o_srcs or None)
+            assert status == 0
+            _exact(sc.name, compare(t.premultiplied(), t.signature(), o_rgba, o_sig))
+        finally:
+            t.close()
+            for s in srcs:
+                s.close()

@@ -70,3 +70,65 @@ def test_gradient_extend_is_not_read():
     # constant red left of x = 0 (no pixel), blue right of x = 300
     got = _rgba(_by_name()["gradient_paint_repeat"])
     assert (got[:, 301:] == [0, 0, 255, 255]).all()
+
+
+def test_gradient_pattern_matrix_is_inverted_onto_the_control_points():
+    # src/vkvg_context_internal.c:797-822: the control points go through the INVERSE of the pattern
+    # matrix, then the CTM. Scale 0.5 inverts exactly to 2, so the gradient with the matrix equals
+    # the gradient with doubled control points (a matrix applied uninverted would halve them).
+    from vkvg_amd.stream import Stream
+    stops = [[0, 1, 0, 0, 1], [0.5, 0, 1, 0, 1], [1.0, 0, 0, 1, 1]]
+
+    def lin(geom, m):
+        st = Stream()
+        st.translate(30, 20)
+        st.set_linear(*geom, stops, matrix=m)
+        st.rectangle(0, 0, 400, 140)
+        st.fill()
+        return _rgba(scenes.Scene("l", 480, 200, st, 1))
+
+    def rad(geom, m):
+        st = Stream()
+        st.set_radial(*geom, stops, matrix=m)
+        st.paint()
+        return _rgba(scenes.Scene("r", 400, 300, st, 0))
+
+    half = (0.5, 0, 0, 0.5, 0, 0)
+    assert np.array_equal(lin((0, 0, 200, 0), half), lin((0, 0, 400, 0), None))
+    assert not np.array_equal(lin((0, 0, 200, 0), half), lin((0, 0, 100, 0), None))
+    # radii are distances: doubled too (the centres are chosen so vkvg_pattern_edit_radial moves nothing)
+    assert np.array_equal(rad((50, 50, 5, 60, 55, 40), half), rad((100, 100, 10, 120, 110, 80), None))
+    # a translation moves the gradient the other way: cp - t
+    assert np.array_equal(lin((0, 0, 200, 0), (1, 0, 0, 1, 50, 0)), lin((-50, 0, 150, 0), None))
+    # a singular matrix is replaced by the identity
+    assert np.array_equal(lin((0, 0, 200, 0), (0, 0, 0, 0, 0, 0)), lin((0, 0, 200, 0), None))
+
+
+def test_every_reference_source_scene_renders():
+    for sc, srcs in scenes.reference_source_scenes():
+        imgs = []
+        for w, h, st in srcs:
+            rgba, _, _, status = ovg.render(w, h, st.words())
+            assert status == 0
+            imgs.append(rgba)
+        rgba, _, stats, status = ovg.render(sc.width, sc.height, sc.words(), sources=imgs)
+        assert status == 0 and stats["draws"] >= 1, sc.name
+        assert rgba.any(), sc.name
+
+
+def test_paint_surf_offsets_place_the_drawn_source():
+    # tests/paint_surf.c paint_with_offset: the 256 x 256 source at (100, 100) over half green:
+    # outside it the green stays, inside it the source composited over the green
+    sc, srcs = [x for x in scenes.reference_source_scenes() if x[0].name == "paint_surf_paint_with_offset"][0]
+    w, h, st = srcs[0]
+    src = ovg.render(w, h, st.words())[0]
+    got = ovg.render(sc.width, sc.height, sc.words(), sources=[src])[0]
+    outside = np.ones(got.shape[:2], bool)
+    outside[100:356, 100:356] = False
+    assert (got[outside] == got[0, 0]).all() and got[0, 0, 1] > 0 and got[0, 0, 0] == 0
+    inside = got[100:356, 100:356].astype(np.int32)
+    # OVER: s + round(d (255 - sa) / 255) per channel, with d the uniform green
+    d = got[0, 0].astype(np.int32)
+    s = src.astype(np.int32)
+    exp = np.minimum(s + np.floor((d[None, None, :] * (255 - s[..., 3:4]) + 127) / 255).astype(np.int32), 255)
+    assert np.abs(inside - exp).max() <= 1

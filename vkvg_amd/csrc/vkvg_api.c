@@ -1956,17 +1956,27 @@ static int stream_step(VkvgContext c, const uint32_t *w, size_t avail, size_t *u
     case 21: NEED(1); vkvg_set_opacity(c, F(a)); break;
     case 22: NEED(1); vkvg_set_operator(c, (vkvg_operator_t)a[0]); break;
     case 23:
-    case 24: {
-        size_t ncp = op == 23 ? 4 : 6;
-        NEED(ncp + 1);
-        uint32_t ns = a[ncp];
-        NEED(ncp + 1 + 5 * (size_t)ns);
-        VkvgPattern p = op == 23 ? vkvg_pattern_create_linear(F(a), F(a + 1), F(a + 2), F(a + 3))
-                                 : vkvg_pattern_create_radial(F(a), F(a + 1), F(a + 2), F(a + 3), F(a + 4), F(a + 5));
-        const uint32_t *st = a + ncp + 1;
+    case 24:
+    case 50:
+    case 51: {
+        /* 50 / 51: a pattern matrix (6 floats) first, then the operands of 23 / 24 */
+        const size_t pre = op >= 50 ? 6 : 0;
+        const bool linear = op == 23 || op == 50;
+        const size_t ncp = linear ? 4 : 6;
+        NEED(pre + ncp + 1);
+        const uint32_t *g = a + pre;
+        uint32_t ns = g[ncp];
+        NEED(pre + ncp + 1 + 5 * (size_t)ns);
+        VkvgPattern p = linear ? vkvg_pattern_create_linear(F(g), F(g + 1), F(g + 2), F(g + 3))
+                               : vkvg_pattern_create_radial(F(g), F(g + 1), F(g + 2), F(g + 3), F(g + 4), F(g + 5));
+        const uint32_t *st = g + ncp + 1;
         for (uint32_t k = 0; k < ns; k++)
             vkvg_pattern_add_color_stop(p, F(st + 5 * k), F(st + 5 * k + 1), F(st + 5 * k + 2), F(st + 5 * k + 3),
                                         F(st + 5 * k + 4));
+        if (pre) {
+            vkvg_matrix_t m = {F(a), F(a + 1), F(a + 2), F(a + 3), F(a + 4), F(a + 5)};
+            vkvg_pattern_set_matrix(p, &m);
+        }
         vkvg_set_source(c, p);
         vkvg_pattern_destroy(p);
         break;

@@ -1134,6 +1134,109 @@ def reference_test_scenes(width: int = 512, height: int = 512, random_size: int 
     return out
 
 
+def reference_source_scenes(width: int = 512, height: int = 512):
+    """The reference's visual tests that take a source surface or a pattern matrix, as a list of
+    (scene, sources): each source is (width, height, stream), an image that vkvg itself draws and
+    that the scene's stream indexes. tests/paint_surf.c:3-88 draws its 256 x 256 source with
+    createSurf (:3-18: paint red at 0.5, stroke a blue rectangle at 0.5, width 10) and paints it
+    with offsets, repeats, a rotation, a scale and a path; tests/pattern_transforms.c:34-105 sets
+    pattern matrices on a surface pattern (img_scale, icon_sheet: the images of data/ are not
+    decoded here; createSurf's image of 300 x 200 and 64 x 48 stands in) and on linear gradients
+    (pat_scale; pat_rotate after 1, 100 and 1000 runs, angle += 0.001f per run). Rotation
+    matrices are rounded to float here; the reference computes them with sinf / cosf."""
+    out = []
+
+    def create_surf(w, h):  # tests/paint_surf.c:3-18
+        st = Stream()
+        st.set_line_width(10.0)
+        st.set_source_rgba(1, 0, 0, 0.5)
+        st.paint()
+        st.set_source_rgba(0, 0, 1, 0.5)
+        st.rectangle(5.0, 5.0, float(w) - 10.0, float(h) - 10.0)
+        st.stroke()
+        return (w, h, st)
+
+    def init_ctx(st):  # tests/common/test.c:686-697
+        st.set_line_width(2)
+        st.set_line_join(JOIN_MITER)
+        st.set_line_cap(CAP_BUTT)
+        st.set_dash([], 0)
+        st.set_fill_rule(FILL_NON_ZERO)
+        st.clear()
+
+    src = create_surf(256, 256)
+    for name in ("paint", "paint_with_offset", "paint_multiple", "paint_with_rotation", "paint_with_scale",
+                 "paint_rect"):
+        st = Stream()
+        init_ctx(st)
+        if name == "paint_with_offset":
+            st.set_source_rgba(0, 1, 0, 0.5)
+            st.paint()
+            st.set_source_surface(0, 100, 100)
+            st.paint()
+        elif name == "paint_multiple":
+            for i in range(10):
+                st.set_source_surface(0, i * 20, i * 20)
+                st.paint()
+        else:
+            st.set_source_surface(0, 0, 0)
+            if name == "paint_with_rotation":
+                st.rotate(45)
+            elif name == "paint_with_scale":
+                st.scale(0.2, 0.2)
+            elif name == "paint_rect":
+                st.rectangle(100, 100, 300, 200)
+            st.paint()
+        out.append((Scene(f"paint_surf_{name}", width, height, st, 0), [src]))
+
+    # tests/pattern_transforms.c:34-49 (img_scale), :50-67 (icon_sheet)
+    st = Stream()
+    init_ctx(st)
+    st.translate(100, 100)
+    st.set_source_pattern(0, matrix=(2.0, 0, 0, 2.0, 0, 0))
+    st.rectangle(0, 0, 200, 140)
+    st.fill()
+    out.append((Scene("pattern_img_scale", width, height, st, 1), [create_surf(300, 200)]))
+    scale = np.float32(0.5)
+    c, s_ = np.float32(math.cos(0.2)), np.float32(math.sin(0.2))
+    # init_translate(22, 27), then scale and rotate: each premultiplies the current matrix
+    m = np.array([[1, 0], [0, 1]], np.float32), np.array([22, 27], np.float32)
+    lin = np.array([[scale, 0], [0, scale]], np.float32) @ m[0]
+    rot = np.array([[c, s_], [-s_, c]], np.float32)
+    lin = (rot @ lin).astype(np.float32)
+    st = Stream()
+    init_ctx(st)
+    st.translate(100, 100)
+    st.set_source_pattern(0, matrix=(float(lin[0, 0]), float(lin[0, 1]), float(lin[1, 0]), float(lin[1, 1]),
+                                     float(m[1][0]), float(m[1][1])))
+    st.rectangle(0, 0, float(np.float32(24) / scale), float(np.float32(24) / scale))
+    st.fill()
+    out.append((Scene("pattern_icon_sheet", width, height, st, 1), [create_surf(64, 48)]))
+
+    # tests/pattern_transforms.c:6-12 (create_grad), :69-84 (pat_scale), :86-105 (pat_rotate)
+    stops = [[0, 1, 0, 0, 1], [0.5, 0, 1, 0, 1], [1.0, 0, 0, 1, 1]]
+    st = Stream()
+    init_ctx(st)
+    st.translate(100, 100)
+    st.set_linear(0, 0, 200, 0, stops, matrix=(0.5, 0, 0, 0.5, 0, 0))
+    st.rectangle(0, 0, 200, 140)
+    st.fill()
+    out.append((Scene("pattern_pat_scale", width, height, st, 1), []))
+    angle = np.float32(0)
+    for run in range(1, 1001):
+        angle = np.float32(angle + np.float32(0.001))
+        if run not in (1, 100, 1000):
+            continue
+        ca, sa = float(np.float32(math.cos(float(angle)))), float(np.float32(math.sin(float(angle))))
+        st = Stream()
+        init_ctx(st)
+        st.set_linear(0, 0, 200, 0, stops, matrix=(ca, sa, -sa, ca, 100, 70))
+        st.rectangle(0, 0, 200, 140)
+        st.fill()
+        out.append((Scene(f"pattern_pat_rotate_run{run}", width, height, st, 1), []))
+    return out
+
+
 def surface_mix(variant: int = 0, size: int = 256, n: int = 40, n_sources: int = 3):
     """Surface sources: random bitmaps of uneven sizes painted and used to fill and stroke, placed
     with set_source_surface offsets or as patterns with every extend mode, nearest and bilinear

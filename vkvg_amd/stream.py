@@ -58,6 +58,8 @@ OP_FLUSH = 46
 OP_SET_SOURCE_SURFACE = 47
 OP_SOURCE_PATTERN = 48
 OP_ROUNDED_RECT2 = 49
+OP_LINEAR_MATRIX = 50  # pattern matrix (6 floats), then the operands of OP_LINEAR
+OP_RADIAL_MATRIX = 51  # pattern matrix (6 floats), then the operands of OP_RADIAL
 
 # enum values of include/vkvg.h
 JOIN_MITER, JOIN_ROUND, JOIN_BEVEL = 0, 1, 2
@@ -183,17 +185,20 @@ class Stream:
         self._chunks.append(np.concatenate([np.array([OP_SET_DASH, len(d)], np.uint32), _f(offset),
                                             _f(*d) if d else np.zeros(0, np.uint32)]))
 
-    def _gradient(self, op, geom, stops):
+    def _gradient(self, op, geom, stops, matrix=None):
         st = np.asarray(stops, np.float32).reshape(-1, 5)
-        self._chunks.append(np.concatenate([np.array([op], np.uint32), _f(*geom),
-                                            np.array([st.shape[0]], np.uint32), st.reshape(-1).view(np.uint32)]))
+        head = [np.array([op], np.uint32)] if matrix is None else \
+            [np.array([op + (OP_LINEAR_MATRIX - OP_LINEAR)], np.uint32), _f(*matrix)]
+        self._chunks.append(np.concatenate(head + [_f(*geom), np.array([st.shape[0]], np.uint32),
+                                                   st.reshape(-1).view(np.uint32)]))
 
-    def set_linear(self, x0, y0, x1, y1, stops):
-        """create a linear pattern, add (offset, r, g, b, a) stops, set it as source, destroy it."""
-        self._gradient(OP_LINEAR, (x0, y0, x1, y1), stops)
+    def set_linear(self, x0, y0, x1, y1, stops, matrix=None):
+        """create a linear pattern, add (offset, r, g, b, a) stops, set its matrix (xx, yx, xy, yy,
+        x0, y0) when one is given, set it as source, destroy it."""
+        self._gradient(OP_LINEAR, (x0, y0, x1, y1), stops, matrix)
 
-    def set_radial(self, cx0, cy0, r0, cx1, cy1, r1, stops):
-        self._gradient(OP_RADIAL, (cx0, cy0, r0, cx1, cy1, r1), stops)
+    def set_radial(self, cx0, cy0, r0, cx1, cy1, r1, stops, matrix=None):
+        self._gradient(OP_RADIAL, (cx0, cy0, r0, cx1, cy1, r1), stops, matrix)
 
     def translate(self, x, y): self._op(OP_TRANSLATE, x, y)
     def scale(self, x, y): self._op(OP_SCALE, x, y)
