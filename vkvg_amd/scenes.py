@@ -1094,6 +1094,169 @@ def reference_test_scenes(width: int = 512, height: int = 512, random_size: int 
     st.stroke()
     add("circles_sizes", st, 4)
 
+    # tests/lines.c: horizontal, vertical, horzAndVert, multilines, multi_segments (random lines)
+    for k, name in enumerate(("horizontal", "vertical", "horz_and_vert", "multilines", "multi_segments")):
+        rng = Rng(0x5EED0120 + k)
+        st = Stream()
+        init_ctx(st)
+        n_draws = 1
+        if name in ("horizontal", "vertical", "horz_and_vert"):
+            n_draws = random_size
+            for _ in range(random_size):
+                st.set_source_rgba(*[_f32(v) for v in rng.u(4)])
+                x1, y1 = _f32(w * rng.u()), _f32(h * rng.u())
+                st.move_to(x1, y1)
+                if name == "horz_and_vert":
+                    st.line_to(_f32(w * rng.u()) + 1, _f32(h * rng.u()) + 1)
+                else:
+                    v = _f32(500.0 * rng.u())
+                    st.line_to(x1 + v, y1) if name == "horizontal" else st.line_to(x1, y1 + v)
+                st.stroke()
+        elif name == "multilines":
+            st.set_source_rgba(*[_f32(v) for v in rng.u(4)])
+            for _ in range(random_size):
+                st.move_to(_f32(w * rng.u()), _f32(h * rng.u()))
+                st.line_to(_f32(w * rng.u()) + 1, _f32(h * rng.u()) + 1)
+            st.stroke()
+        else:
+            st.set_source_rgba(*[_f32(v) for v in rng.u(4)])
+            pts = np.stack([(w - 10) * rng.u(random_size + 1), (h - 10) * rng.u(random_size + 1)], 1)
+            st.polyline(pts.astype(np.float32))
+            st.stroke()
+        add(f"lines_{name}", st, n_draws)
+
+    # tests/transform.c cairo_tests (rotation += 0.002f per run; runs 1 and 300) with
+    # _test_rounded_rect (the branch of a 200 x 200 rectangle with radius 102.4: four corner curves)
+    rot = np.float32(0)
+    for run in range(1, 301):
+        rot = np.float32(rot + np.float32(0.002))
+        if run not in (1, 300):
+            continue
+        st = Stream()
+        st.set_fill_rule(FILL_EVEN_ODD)
+        st.set_source_rgba(1.0, 1.0, 1.0, 1)
+        st.paint()
+        st.set_line_width(10)
+        st.set_source_rgb(0, 0, 1)
+        st.translate(float(width // 2), float(height // 2))
+        st.rotate(float(rot))
+        st.arc(0, 0, 150, 0, _f32(math.pi * 1.5))
+        st.stroke()
+        x0, y0, x1, y1 = -100.0, -100.0, 100.0, 100.0
+        st.move_to(x0, (y0 + y1) / 2)
+        st.curve_to(x0, y0, x0, y0, (x0 + x1) / 2, y0)
+        st.curve_to(x1, y0, x1, y0, x1, (y0 + y1) / 2)
+        st.curve_to(x1, y1, x1, y1, (x1 + x0) / 2, y1)
+        st.curve_to(x0, y1, x0, y1, x0, (y0 + y1) / 2)
+        st.close_path()
+        st.set_source_rgb(0.5, 0.5, 1)
+        st.fill_preserve()
+        st.set_source_rgba(0.5, 0, 0, 0.5)
+        st.set_line_width(10.0)
+        st.stroke()
+        add(f"transform_cairo_tests_run{run}", st, 3)
+
+    # tests/rect_fill.c test / test_evenodd, tests/fill.c test
+    for rule, tag in ((FILL_NON_ZERO, "nz"), (FILL_EVEN_ODD, "eo")):
+        st = Stream()
+        st.clear()
+        st.set_fill_rule(rule)
+        st.set_source_rgba(0, 0, 1, 0.5)
+        st.rectangle(100, 100, 200, 200)
+        st.fill()
+        st.rectangle(200, 200, 200, 200)
+        st.set_source_rgba(1, 0, 0, 0.5)
+        st.fill()
+        add(f"rect_fill_{tag}", st, 2)
+    st = Stream()
+    init_ctx(st)
+    st.set_source_rgba(0.1, 0.9, 0.1, 1.0)
+    st.move_to(100, 100)
+    for lx, ly in ((50, 200), (150, -100), (100, 200), (-100, 100), (-10, -100), (-190, -50)):
+        st.rel_line_to(lx, ly)
+    st.close_path()
+    st.fill()
+    add("fill", st, 1)
+
+    # tests/gradient2.c draw (interactive): linear and radial, 1 to 4 handle points, a hovered one
+    g2 = np.array([[150, 150], [250, 150], [125, 125], [145, 125]], np.float32)
+
+    def vlen(a, b):
+        d = (b - a).astype(np.float32)
+        return float(np.sqrt(np.float32(d[0] * d[0] + d[1] * d[1]), dtype=np.float32))
+
+    for radial in (False, True):
+        for n_pts in range(1, 5):
+            hover = (n_pts + radial) % 3 - 1  # -1, 0 or 1
+            st = Stream()
+            st.clear()
+            g2stops = [[0.0, 1, 0, 0, 0.5], [0.3, 0, 1, 0, 0.5], [0.6, 0, 0, 1, 0.5], [1.0, 0, 0, 0, 0.5]]
+            if radial:
+                st.set_radial(float(g2[2, 0]), float(g2[2, 1]), vlen(g2[2], g2[3]), float(g2[0, 0]), float(g2[0, 1]),
+                              vlen(g2[0], g2[1]), g2stops)
+            else:
+                st.set_linear(float(g2[0, 0]), float(g2[0, 1]), float(g2[1, 0]), float(g2[1, 1]), g2stops)
+            st.paint()
+            st.set_dash([], 0)
+            st.set_line_width(1)
+            for i in range(n_pts):
+                st.set_source_rgba(*((0.5, 0.5, 1.0, 0.7) if hover == i else (1.0, 0.5, 0.5, 0.9)))
+                st.arc(float(g2[i, 0]), float(g2[i, 1]), 7.0, 0, full)
+                st.fill_preserve()
+                st.stroke()
+            add(f"gradient2_{'radial' if radial else 'linear'}_{n_pts}pts", st, 2 * n_pts)
+
+    # tests/bezier.c draw (interactive): one cubic, swept over closed / filled / join / cap and the
+    # hovered handle (-1: none); the handles are drawn after a vkvg_flush
+    bz = np.array([[150, 150], [500, 340], [470, 150], [160, 350]], np.float32)
+    k = 0
+    for closed in (False, True):
+        for filled in (False, True):
+            for join in (JOIN_MITER, JOIN_ROUND, JOIN_BEVEL):
+                for cap in (CAP_BUTT, CAP_ROUND, CAP_SQUARE):
+                    hover = k % 5 - 1
+                    k += 1
+                    st = Stream()
+                    st.clear()
+                    st.set_line_width(10.0)
+                    st.set_line_join(join)
+                    st.set_line_cap(cap)
+                    st.move_to(float(bz[0, 0]), float(bz[0, 1]))
+                    st.curve_to(*[float(v) for v in bz[1:].reshape(-1)])
+                    if closed:
+                        st.close_path()
+                    if filled:
+                        st.set_source_rgba(0.4, 0.6, 0.4, 1)
+                        st.fill_preserve()
+                    st.set_source_rgba(1, 0, 0, 0.3)
+                    st.stroke()
+                    st.flush()
+                    st.set_line_width(1)
+                    st.set_source_rgba(0.5, 0.5, 0.5, 0.6)
+                    for i in range(4):
+                        if i == hover:
+                            st.arc(float(bz[i, 0]), float(bz[i, 1]), 7.0, 0, full)
+                            st.set_source_rgba(0.5, 0.5, 1.0, 0.6)
+                            st.fill_preserve()
+                            st.set_source_rgba(0.5, 0.5, 0.5, 0.6)
+                        else:
+                            st.arc(float(bz[i, 0]), float(bz[i, 1]), 5.0, 0, full)
+                            st.fill_preserve()
+                        st.stroke()
+                    if hover >= 0:
+                        st.stroke_preserve()
+                        st.set_dash([], 0)
+                        st.set_line_width(2)
+                        st.set_source_rgba(0, 0, 0, 1)
+                        st.stroke()
+                        st.set_source_rgba(0.5, 0.5, 1.0, 0.7)
+                        st.arc(float(bz[hover, 0]), float(bz[hover, 1]), 5.0, 0, full)
+                        st.fill_preserve()
+                        st.stroke()
+                    else:
+                        st.stroke()
+                    add(f"bezier_{'z' if closed else ''}{'f' if filled else ''}j{join}c{cap}_h{hover}", st, 12)
+
     # tests/inverse_colinear.c:31-72 (draw), swept over the toggles of its key handler (:74-128)
     ic_pts = np.array([[150, 150], [200, 300], [250, 150], [280, 350], [300, 100]], np.float32)
     a15 = _f32(np.float32(PIF) * np.float32(1.5))
