@@ -217,6 +217,8 @@ vkvg_public void vkvg_new_path(VkvgContext ctx);                                
 vkvg_public void vkvg_close_path(VkvgContext ctx);                                               /* :970 */
 vkvg_public void vkvg_new_sub_path(VkvgContext ctx);                                             /* :980 */
 vkvg_public void vkvg_get_current_point(VkvgContext ctx, float *x, float *y);                    /* :998 */
+vkvg_public void vkvg_path_extents(VkvgContext ctx, float *const x1, float *const y1, float *const x2,
+                                   float *const y2);                                             /* :989 */
 vkvg_public void vkvg_line_to(VkvgContext ctx, float x, float y);                                /* :1009 */
 vkvg_public void vkvg_rel_line_to(VkvgContext ctx, float dx, float dy);                          /* :1020 */
 vkvg_public void vkvg_move_to(VkvgContext ctx, float x, float y);                                /* :1033 */

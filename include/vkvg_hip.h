@@ -201,6 +201,13 @@ vkvg_public int  vkvg_hip_execute(vkh_engine *eng);
 vkvg_public void vkvg_hip_get_stats(vkh_engine *eng, vkh_stats *out);
 vkvg_public const char *vkvg_hip_last_error(vkh_engine *eng);
 
+/* Host side, no device needed: widens ext = {x_min, y_min, x_max, y_max} by the points that a cubic
+ * item flattens to (its subdivision points and, unless VKH_ITEM_NO_END is set, its end point), with
+ * the comparisons of _vkvg_path_extents (src/vkvg_context_internal.c:1879-1917). The subdivision is
+ * the flattening kernels' own code compiled for the host. Used by vkvg_path_extents
+ * (src/vkvg_context.c:684-696). */
+vkvg_public void vkvg_hip_item_extents(const vkh_item *item, float ext[4]);
+
 #ifdef __cplusplus
 }
 #endif

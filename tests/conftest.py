@@ -10,6 +10,13 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+    # torch (its own HIP runtime) creates its GPU context before any test loads libvkvg.so: loading
+    # the library registers its kernels with the HIP runtime it links, and a later torch context
+    # then leaves the library's device creation failing. Counting devices does not initialise one.
+    import torch
+    if torch.cuda.device_count() > 0:
+        torch.zeros(1, device="cuda")
+        torch.cuda.synchronize()
 
 
 @pytest.fixture(scope="session")

@@ -49,7 +49,7 @@ EXPORTED = [
     "vkvg_surface_get_vk_format", "vkvg_surface_get_width", "vkvg_surface_get_height", "vkvg_surface_write_to_png",
     "vkvg_surface_write_to_memory", "vkvg_surface_resolve", "vkvg_create", "vkvg_destroy", "vkvg_status",
     "vkvg_status_to_string", "vkvg_reference", "vkvg_get_reference_count", "vkvg_flush", "vkvg_new_path",
-    "vkvg_close_path", "vkvg_new_sub_path", "vkvg_get_current_point", "vkvg_line_to", "vkvg_rel_line_to",
+    "vkvg_close_path", "vkvg_new_sub_path", "vkvg_get_current_point", "vkvg_path_extents", "vkvg_line_to", "vkvg_rel_line_to",
     "vkvg_move_to", "vkvg_rel_move_to", "vkvg_arc", "vkvg_arc_negative", "vkvg_curve_to", "vkvg_rel_curve_to",
     "vkvg_quadratic_to", "vkvg_rel_quadratic_to", "vkvg_rectangle", "vkvg_rounded_rectangle", "vkvg_ellipse",
     "vkvg_elliptic_arc_to", "vkvg_rel_elliptic_arc_to", "vkvg_rounded_rectangle2",
@@ -78,7 +78,7 @@ EXPORTED = [
     "vkvg_hip_image_alloc", "vkvg_hip_image_free", "vkvg_hip_image_clear", "vkvg_hip_image_download",
     "vkvg_hip_image_upload", "vkvg_hip_image_read_unpremultiplied", "vkvg_hip_render", "vkvg_hip_upload",
     "vkvg_hip_execute", "vkvg_hip_get_stats",
-    "vkvg_hip_last_error",
+    "vkvg_hip_last_error", "vkvg_hip_item_extents",
     # engine tuning (include/vkvg_hip.h) and the differentiable renderer (include/vkvg_diff.h)
     "vkvg_hip_autotune", "vkvg_hip_retile", "vkvg_hip_set_fine_words", "vkvg_hip_set_graph",
     "vkvg_diff_create", "vkvg_diff_destroy", "vkvg_diff_forward", "vkvg_diff_forward_f64", "vkvg_diff_backward", "vkvg_diff_last_ms", "vkvg_diff_stats", "vkvg_diff_last_error",

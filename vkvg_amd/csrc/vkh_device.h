@@ -20,28 +20,31 @@
 #define VKH_2_PIF 0.63661977236758134308f /* 2/pi where 2*pi was meant (reproduced) */
 #define VKH_FLT_EPSILON 1.19209290e-7f
 
+// the vector helpers and the flattening below also compile for the host (vkvg_hip_item_extents)
+#define VKH_HD __host__ __device__ __forceinline__
+
 struct v2 {
     float x, y;
 };
 
-__device__ __forceinline__ v2 V(float x, float y) { return v2{x, y}; }
-__device__ __forceinline__ v2 vadd(v2 a, v2 b) { return V(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ v2 vsub(v2 a, v2 b) { return V(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ v2 vmul(v2 a, float m) { return V(a.x * m, a.y * m); }
-__device__ __forceinline__ v2 vdiv(v2 a, float m) { return V(a.x / m, a.y / m); }
-__device__ __forceinline__ v2 vperp(v2 a) { return V(a.y, -a.x); }
-__device__ __forceinline__ float vlen(v2 v) { return sqrtf(v.x * v.x + v.y * v.y); }
-__device__ __forceinline__ v2 vnorm(v2 a) {
+VKH_HD v2 V(float x, float y) { return v2{x, y}; }
+VKH_HD v2 vadd(v2 a, v2 b) { return V(a.x + b.x, a.y + b.y); }
+VKH_HD v2 vsub(v2 a, v2 b) { return V(a.x - b.x, a.y - b.y); }
+VKH_HD v2 vmul(v2 a, float m) { return V(a.x * m, a.y * m); }
+VKH_HD v2 vdiv(v2 a, float m) { return V(a.x / m, a.y / m); }
+VKH_HD v2 vperp(v2 a) { return V(a.y, -a.x); }
+VKH_HD float vlen(v2 v) { return sqrtf(v.x * v.x + v.y * v.y); }
+VKH_HD v2 vnorm(v2 a) {
     float m = sqrtf(a.x * a.x + a.y * a.y);
     return V(a.x / m, a.y / m);
 }
-__device__ __forceinline__ float vdot(v2 a, v2 b) { return (a.x * b.x) + (a.y * b.y); }
-__device__ __forceinline__ bool equf(float a, float b) { return fabsf(a - b) <= VKH_FLT_EPSILON; }
+VKH_HD float vdot(v2 a, v2 b) { return (a.x * b.x) + (a.y * b.y); }
+VKH_HD bool equf(float a, float b) { return fabsf(a - b) <= VKH_FLT_EPSILON; }
 
-__device__ __forceinline__ float d_atan2f(float y, float x) { return (float)atan2((double)y, (double)x); }
-__device__ __forceinline__ float d_cosf(float a) { return (float)cos((double)a); }
-__device__ __forceinline__ float d_sinf(float a) { return (float)sin((double)a); }
-__device__ __forceinline__ float d_acosf(float a) { return (float)acos((double)a); }
+VKH_HD float d_atan2f(float y, float x) { return (float)atan2((double)y, (double)x); }
+VKH_HD float d_cosf(float a) { return (float)cos((double)a); }
+VKH_HD float d_sinf(float a) { return (float)sin((double)a); }
+VKH_HD float d_acosf(float a) { return (float)acos((double)a); }
 
 // ---------------------------------------------------------------------------------------
 // AGG adaptive subdivision. The reference recurses depth-first (left child first) and emits
@@ -52,7 +55,7 @@ struct Cubic {
     float x1, y1, x2, y2, x3, y3, x4, y4;
 };
 
-__device__ __forceinline__ void agg_split(const Cubic &c, bool right, Cubic &o) {
+VKH_HD void agg_split(const Cubic &c, bool right, Cubic &o) {
     float x12 = (c.x1 + c.x2) / 2, y12 = (c.y1 + c.y2) / 2;
     float x23 = (c.x2 + c.x3) / 2, y23 = (c.y2 + c.y3) / 2;
     float x34 = (c.x3 + c.x4) / 2, y34 = (c.y3 + c.y4) / 2;
@@ -68,7 +71,7 @@ __device__ __forceinline__ void agg_split(const Cubic &c, bool right, Cubic &o) 
 // Leaf test of one node at `level` (>= 1). Returns the number of points emitted (0 means
 // "subdivide further"), writing them to pts. The comparisons against the reference's double
 // constants (1.7, 0.01) promote the float operand exactly as C does.
-__device__ __forceinline__ int agg_leaf(const Cubic &c, float tol, v2 *pts) {
+VKH_HD int agg_leaf(const Cubic &c, float tol, v2 *pts) {
     float x1 = c.x1, y1 = c.y1, x2 = c.x2, y2 = c.y2, x3 = c.x3, y3 = c.y3, x4 = c.x4, y4 = c.y4;
     float dx = x4 - x1, dy = y4 - y1;
     float d2 = fabsf(((x2 - x4) * dy - (y2 - y4) * dx));
@@ -157,7 +160,7 @@ __device__ __forceinline__ int agg_leaf(const Cubic &c, float tol, v2 *pts) {
 #define VKH_ANG_M1 (2.f * VKH_ANG_E + 0x1p-21f)
 #define VKH_ANG_M1W (VKH_ANG_M1 + 0x1p-21f)
 #define VKH_ANG_M2 (2.f * VKH_ANG_M1W + 0x1p-20f)
-__device__ __forceinline__ int agg_leaf_fast(const Cubic &c, float tol, v2 *pts) {
+VKH_HD int agg_leaf_fast(const Cubic &c, float tol, v2 *pts) {
     float x1 = c.x1, y1 = c.y1, x2 = c.x2, y2 = c.y2, x3 = c.x3, y3 = c.y3, x4 = c.x4, y4 = c.y4;
     float dx = x4 - x1, dy = y4 - y1;
     float d2 = fabsf(((x2 - x4) * dy - (y2 - y4) * dx));
@@ -247,7 +250,7 @@ __device__ __forceinline__ int agg_leaf_fast(const Cubic &c, float tol, v2 *pts)
 // sibling takes one split; deeper returns recompute the node from the root. Every node is thus
 // produced by the same chain of splits as in the reference's recursion.
 template <typename Emit>
-__device__ void agg_flatten(const Cubic &root, float tol, Emit emit) {
+__host__ __device__ void agg_flatten(const Cubic &root, float tol, Emit emit) {
     uint64_t path_lo = 0, path_hi = 0; // 128 bits of choices, levels 1..101
     int level = 1;                      // the root (level 0) always subdivides
     Cubic node, par = root, gpar = root;
@@ -330,7 +333,7 @@ __device__ void agg_flatten(const Cubic &root, float tol, Emit emit) {
 // read from the stack, or, beyond level D-1, recomputed from the level D-1 ancestor along the path
 // bits; all nodes come from the same chain of splits as in the recursion.
 template <int D, typename Emit, typename Store, typename Load>
-__device__ void agg_flatten_stack(const Cubic &root, float tol, Emit emit, Store store, Load load) {
+__host__ __device__ void agg_flatten_stack(const Cubic &root, float tol, Emit emit, Store store, Load load) {
     uint64_t path_lo = 0, path_hi = 0; // bit (l-1): the node at level l is a right child
     int level = 1;
     store(0, root);

@@ -199,6 +199,27 @@ __global__ void k_flatten_overflow(const vkh_item *__restrict__ items, const uin
     }
 }
 
+extern "C" void vkvg_hip_item_extents(const vkh_item *it, float ext[4]) {
+    auto widen = [&](v2 p) {
+        if (p.x < ext[0])
+            ext[0] = p.x;
+        if (p.x > ext[2])
+            ext[2] = p.x;
+        if (p.y < ext[1])
+            ext[1] = p.y;
+        if (p.y > ext[3])
+            ext[3] = p.y;
+    };
+    if (!(it->flags & VKH_ITEM_CUBIC)) {
+        widen(V(it->p[0], it->p[1]));
+        return;
+    }
+    const Cubic cb{it->p[0], it->p[1], it->p[2], it->p[3], it->p[4], it->p[5], it->p[6], it->p[7]};
+    agg_flatten(cb, it->tol, widen);
+    if (!(it->flags & VKH_ITEM_NO_END))
+        widen(V(it->p[6], it->p[7]));
+}
+
 __global__ void k_subpath_ranges(const vkh_subpath *__restrict__ sp, uint32_t n, const uint32_t *__restrict__ item_off,
                                  uint32_t *__restrict__ sp_first, uint32_t *__restrict__ sp_count) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -1068,6 +1068,29 @@ void vkvg_get_current_point(VkvgContext ctx, float *x, float *y) {
     *x = ctx->cur_last.x;
     *y = ctx->cur_last.y;
 }
+/* vkvg_path_extents, src/vkvg_context.c:684-696 -> _vkvg_path_extents (untransformed),
+ * src/vkvg_context_internal.c:1879-1917: the bounds of every point of the current path, curves as
+ * flattened, in user space. The maxima start at FLT_MIN as in the reference, so a path left of or
+ * above the origin reports FLT_MIN as its x2 or y2. */
+void vkvg_path_extents(VkvgContext ctx, float *const x1, float *const y1, float *const x2, float *const y2) {
+    if (vkvg_status(ctx) || !x1 || !y1 || !x2 || !y2)
+        return;
+    finish_path(ctx);
+    if (no_committed_subpath(ctx)) {
+        *x1 = *x2 = *y1 = *y2 = 0;
+        return;
+    }
+    float ext[4] = {FLT_MAX, FLT_MAX, FLT_MIN, FLT_MIN};
+    for (uint32_t s = ctx->path_sp_begin; s < ctx->f.sps.n; s++) {
+        const vkh_subpath *sp = &ctx->f.sps.p[s];
+        for (uint32_t k = 0; k < sp->n_items; k++)
+            vkvg_hip_item_extents(&ctx->f.items.p[sp->first_item + k], ext);
+    }
+    *x1 = ext[0];
+    *y1 = ext[1];
+    *x2 = ext[2];
+    *y2 = ext[3];
+}
 /* vkvg_arc / vkvg_arc_negative, src/vkvg_context.c:394-503: arc points are generated here
  * with the reference's sequential angle accumulation */
 static void arc_impl(VkvgContext c, float xc, float yc, float radius, float a1, float a2, bool neg) {
