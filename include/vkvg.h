@@ -188,6 +188,7 @@ vkvg_public const void   *vkvg_get_device_requirements(VkPhysicalDeviceFeatures 
 
 /* ---- surface (include/vkvg.h:725-845) */
 vkvg_public VkvgSurface   vkvg_surface_create(VkvgDevice dev, uint32_t width, uint32_t height); /* :725 */
+vkvg_public VkvgSurface   vkvg_surface_create_from_image(VkvgDevice dev, const char *filePath); /* :735 */
 vkvg_public VkvgSurface   vkvg_surface_create_from_bitmap(VkvgDevice dev, unsigned char *img, uint32_t width,
                                                           uint32_t height); /* :752 */
 vkvg_public vkvg_status_t vkvg_surface_status(VkvgSurface surf);                 /* :761 */

@@ -52,6 +52,12 @@ vkvg_public vkvg_status_t vkvg_amd_upload(VkvgContext ctx);
 /* Runs the pipeline on the device's resident frame; returns 0 on success. */
 vkvg_public int vkvg_amd_execute(VkvgDevice dev);
 
+/* PNG decoding behind vkvg_surface_create_from_image (no counterpart in the reference, which links
+ * stb_image): 8-bit RGBA, not premultiplied, top row first, as stbi_load returns with 4 components.
+ * Host only. Returns NULL on a malformed or unsupported file; free the result with vkvg_amd_png_free. */
+vkvg_public uint8_t *vkvg_amd_png_decode(const uint8_t *buf, size_t n, uint32_t *width, uint32_t *height);
+vkvg_public void     vkvg_amd_png_free(uint8_t *rgba);
+
 /* Backend controls */
 vkvg_public void       vkvg_amd_device_get_stats(VkvgDevice dev, vkh_stats *out);
 vkvg_public int        vkvg_amd_device_set_tile_size(VkvgDevice dev, uint32_t tile);

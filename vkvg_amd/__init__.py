@@ -44,7 +44,7 @@ EXPORTED = [
     "vkvg_matrix_multiply", "vkvg_matrix_transform_distance", "vkvg_matrix_transform_point", "vkvg_matrix_invert",
     "vkvg_matrix_get_scale", "vkvg_device_create", "vkvg_device_destroy", "vkvg_device_status",
     "vkvg_device_reference", "vkvg_device_get_reference_count", "vkvg_device_set_dpy", "vkvg_device_get_dpy",
-    "vkvg_surface_create", "vkvg_surface_create_from_bitmap", "vkvg_surface_status", "vkvg_surface_reference",
+    "vkvg_surface_create", "vkvg_surface_create_from_image", "vkvg_surface_create_from_bitmap", "vkvg_surface_status", "vkvg_surface_reference",
     "vkvg_surface_get_reference_count", "vkvg_surface_destroy", "vkvg_surface_clear", "vkvg_surface_get_vk_image",
     "vkvg_surface_get_vk_format", "vkvg_surface_get_width", "vkvg_surface_get_height", "vkvg_surface_write_to_png",
     "vkvg_surface_write_to_memory", "vkvg_surface_resolve", "vkvg_create", "vkvg_destroy", "vkvg_status",
@@ -72,7 +72,7 @@ EXPORTED = [
     "vkvg_amd_replay", "vkvg_amd_replay_sources", "vkvg_amd_surface_enable_signature", "vkvg_amd_surface_read_signature",
     "vkvg_amd_surface_read_premultiplied", "vkvg_amd_surface_device_ptr", "vkvg_amd_render_batch",
     "vkvg_amd_device_get_stats", "vkvg_amd_device_set_tile_size", "vkvg_amd_device_engine",
-    "vkvg_amd_upload", "vkvg_amd_execute", "vkvg_amd_upload_batch",
+    "vkvg_amd_upload", "vkvg_amd_execute", "vkvg_amd_upload_batch", "vkvg_amd_png_decode", "vkvg_amd_png_free",
     # include/vkvg_hip.h
     "vkvg_hip_engine_create", "vkvg_hip_engine_destroy", "vkvg_hip_set_tile_size", "vkvg_hip_set_stream",
     "vkvg_hip_image_alloc", "vkvg_hip_image_free", "vkvg_hip_image_clear", "vkvg_hip_image_download",
@@ -250,6 +250,24 @@ class Surface:
             raise VkvgError(f"vkvg_surface_create_from_bitmap failed (status {st})")
         if signature and L.vkvg_amd_surface_enable_signature(self.h):
             raise VkvgError("cannot enable coverage signatures")
+        return self
+
+    @classmethod
+    def from_image(cls, dev: "Device", path: str) -> "Surface":
+        """vkvg_surface_create_from_image: a PNG file's pixels (8-bit RGBA as decoded)."""
+        L = lib()
+        L.vkvg_surface_create_from_image.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        L.vkvg_surface_create_from_image.restype = ctypes.c_void_p
+        for f in (L.vkvg_surface_get_width, L.vkvg_surface_get_height):
+            f.argtypes = [ctypes.c_void_p]
+            f.restype = ctypes.c_uint32
+        self = cls.__new__(cls)
+        self.dev = dev
+        self.h = L.vkvg_surface_create_from_image(dev.h, os.fsencode(path))
+        st = L.vkvg_surface_status(self.h)
+        if st:
+            raise VkvgError(f"vkvg_surface_create_from_image failed (status {st})")
+        self.width, self.height = L.vkvg_surface_get_width(self.h), L.vkvg_surface_get_height(self.h)
         return self
 
     def replay(self, words: np.ndarray, sources: list | None = None) -> None:

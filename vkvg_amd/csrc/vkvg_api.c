@@ -368,6 +368,40 @@ VkvgSurface vkvg_surface_create_from_bitmap(VkvgDevice dev, unsigned char *img, 
     vkvg_hip_image_upload(dev->eng, s->img, img, (size_t)width * height * 4);
     return s;
 }
+/* vkvg_surface_create_from_image, src/vkvg_surface.c:181-194: the decoded file (PNG here) through
+ * vkvg_surface_create_from_bitmap; a file that cannot be read or decoded gives a NULL_POINTER surface */
+VkvgSurface vkvg_surface_create_from_image(VkvgDevice dev, const char *filePath) {
+    if (vkvg_device_status(dev))
+        return (VkvgSurface)&s_dev_err;
+    FILE *f = filePath ? fopen(filePath, "rb") : NULL;
+    if (!f)
+        return (VkvgSurface)&s_null;
+    uint8_t *buf = NULL;
+    size_t n = 0, cap = 0, got;
+    do {
+        if (n == cap) {
+            cap = cap ? cap * 2 : 1u << 16;
+            uint8_t *p = (uint8_t *)realloc(buf, cap);
+            if (!p) {
+                free(buf);
+                fclose(f);
+                return (VkvgSurface)&s_no_mem;
+            }
+            buf = p;
+        }
+        got = fread(buf + n, 1, cap - n, f);
+        n += got;
+    } while (got > 0);
+    fclose(f);
+    uint32_t w = 0, h = 0;
+    uint8_t *img = vkvg_amd_png_decode(buf, n, &w, &h);
+    free(buf);
+    if (!img)
+        return (VkvgSurface)&s_null;
+    VkvgSurface s = vkvg_surface_create_from_bitmap(dev, img, w, h);
+    vkvg_amd_png_free(img);
+    return s;
+}
 vkvg_status_t vkvg_surface_status(VkvgSurface surf) { return !surf ? VKVG_STATUS_NULL_POINTER : surf->status; }
 VkvgSurface vkvg_surface_reference(VkvgSurface surf) {
     if (!vkvg_surface_status(surf))
