@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void k_flat_scatter(const vkh_item *__restrict
 }
 
 #define VKH_FLAT_STACK 8   // ancestor levels kept in LDS per thread by the subdivision pass
-#define VKH_FLAT_BLOCK 128 // threads per workgroup of the subdivision pass (32 KB of stacks)
+#define VKH_FLAT_BLOCK 64 // threads per workgroup of the subdivision pass: one wavefront, 16 KB of stacks
 
 __global__ __launch_bounds__(VKH_FLAT_BLOCK) void k_flatten_stage(const vkh_item *__restrict__ items, uint32_t n, const uint32_t *__restrict__ perm,
                                 uint32_t *__restrict__ counts, uint32_t *__restrict__ slot_alloc,
