@@ -144,6 +144,23 @@ void ovg_winding_fan(const int32_t *px, const int32_t *py, uint32_t n, uint32_t 
 uint32_t ovg_flatten_cubic(float x0, float y0, float x1, float y1, float x2, float y2, float x3, float y3,
                            float tolerance, float *out_xy, uint32_t cap);
 
+/* The same subdivision started at a given recursion level: level >= 1 applies the flatness
+ * tests to the given control points themselves (level 0 always subdivides once). */
+uint32_t ovg_flatten_cubic_level(float x0, float y0, float x1, float y1, float x2, float y2, float x3, float y3,
+                                 float tolerance, unsigned level, float *out_xy, uint32_t cap);
+
+/* Geometry of the first fill or stroke of a stream, before rasterization: the finished path
+ * (points and the path-record array with its CLOSED / CURVES / CONVEX bits) and, for a stroke,
+ * the vertex positions and triangle indices the stroke emits. Counts are always the full
+ * counts; at most `cap_*` entries are written. */
+typedef struct {
+    float    *points;   uint32_t cap_points, n_points;   /* x, y pairs */
+    uint32_t *pathes;   uint32_t cap_pathes, n_pathes;
+    float    *verts;    uint32_t cap_verts,  n_verts;    /* x, y pairs */
+    uint32_t *inds;     uint32_t cap_inds,   n_inds;
+} ovg_geometry;
+int ovg_geometry_dump(uint32_t width, uint32_t height, const uint32_t *words, size_t nwords, ovg_geometry *out);
+
 /* H-blend: one premultiplied OVER step in UNORM8. */
 void ovg_blend_over(uint8_t *d, const uint8_t *s);
 

@@ -13,6 +13,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIBS: dict[str, ctypes.CDLL] = {}
+_OVERRIDE: list[str] = []  # a library path that replaces libovg.so (mutation tests)
 
 
 def build(quiet: bool = True) -> None:
@@ -20,11 +21,30 @@ def build(quiet: bool = True) -> None:
                    stdout=subprocess.DEVNULL if quiet else None)
 
 
+class using:
+    """Context manager: every call inside it goes to the oracle library at `path` (a build of
+    the oracle's sources with a deliberate change, for the mutation tests)."""
+
+    def __init__(self, path: str) -> None:
+        self.path = path
+
+    def __enter__(self):
+        _OVERRIDE.append(self.path)
+        return self
+
+    def __exit__(self, *exc):
+        _OVERRIDE.pop()
+        return False
+
+
 def _lib(variant: str = "") -> ctypes.CDLL:
-    name = "libovg_glibc.so" if variant == "glibc" else "libovg.so"
+    if _OVERRIDE:
+        name = path = _OVERRIDE[-1]
+    else:
+        name = "libovg_glibc.so" if variant == "glibc" else "libovg.so"
+        path = os.path.join(_HERE, name)
     if name in _LIBS:
         return _LIBS[name]
-    path = os.path.join(_HERE, name)
     if not os.path.exists(path):
         build()
     lib = ctypes.CDLL(path)
@@ -49,9 +69,22 @@ def _lib(variant: str = "") -> ctypes.CDLL:
         getattr(lib, fn).argtypes = [i32p, i32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, i32p]
     lib.ovg_flatten_cubic.argtypes = [ctypes.c_float] * 9 + [ctypes.POINTER(ctypes.c_float), ctypes.c_uint32]
     lib.ovg_flatten_cubic.restype = ctypes.c_uint32
+    lib.ovg_flatten_cubic_level.argtypes = [ctypes.c_float] * 9 + [ctypes.c_uint32, ctypes.POINTER(ctypes.c_float),
+                                                                    ctypes.c_uint32]
+    lib.ovg_flatten_cubic_level.restype = ctypes.c_uint32
+    lib.ovg_geometry_dump.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_size_t,
+                                      ctypes.POINTER(_Geometry)]
+    lib.ovg_geometry_dump.restype = ctypes.c_int
     lib.ovg_blend_over.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     _LIBS[name] = lib
     return lib
+
+
+class _Geometry(ctypes.Structure):
+    _fields_ = [("points", ctypes.c_void_p), ("cap_points", ctypes.c_uint32), ("n_points", ctypes.c_uint32),
+                ("pathes", ctypes.c_void_p), ("cap_pathes", ctypes.c_uint32), ("n_pathes", ctypes.c_uint32),
+                ("verts", ctypes.c_void_p), ("cap_verts", ctypes.c_uint32), ("n_verts", ctypes.c_uint32),
+                ("inds", ctypes.c_void_p), ("cap_inds", ctypes.c_uint32), ("n_inds", ctypes.c_uint32)]
 
 
 def _p(a: np.ndarray, t=ctypes.c_uint32):
@@ -130,6 +163,32 @@ def flatten_cubic(p, tol: float = 0.25, cap: int = 1 << 16, variant: str = "") -
     out = np.zeros(2 * cap, np.float32)
     n = lib.ovg_flatten_cubic(*[float(v) for v in p], tol, _p(out, ctypes.c_float), cap)
     return out[:2 * min(n, cap)].reshape(-1, 2)
+
+
+def flatten_cubic_level(p, level: int, tol: float = 0.25, cap: int = 1 << 16) -> np.ndarray:
+    """AGG subdivision started at `level` (>= 1: the flatness tests apply to `p` itself)."""
+    out = np.zeros(2 * cap, np.float32)
+    n = _lib().ovg_flatten_cubic_level(*[float(v) for v in p], tol, level, _p(out, ctypes.c_float), cap)
+    return out[:2 * min(n, cap)].reshape(-1, 2)
+
+
+P_CLOSED, P_CURVES, P_CONVEX, P_MASK = 0x80000000, 0x40000000, 0x20000000, 0x1FFFFFFF
+
+
+def geometry(words: np.ndarray, width: int = 256, height: int = 256, cap: int = 1 << 16):
+    """Geometry of the stream's first fill / stroke before rasterization: dict with the path's
+    points (n x 2 float32), path records (uint32, CLOSED / CURVES / CONVEX bits), and for a
+    stroke the emitted vertices (n x 2 float32) and triangles (n x 3 uint32)."""
+    words = np.ascontiguousarray(words, np.uint32)
+    pts = np.zeros(2 * cap, np.float32)
+    pth = np.zeros(cap, np.uint32)
+    vts = np.zeros(2 * cap, np.float32)
+    ind = np.zeros(3 * cap, np.uint32)
+    g = _Geometry(pts.ctypes.data, cap, 0, pth.ctypes.data, cap, 0, vts.ctypes.data, cap, 0, ind.ctypes.data, 3 * cap, 0)
+    st = _lib().ovg_geometry_dump(width, height, _p(words), words.size, ctypes.byref(g))
+    assert g.n_points <= cap and g.n_verts <= cap and g.n_inds <= 3 * cap
+    return {"status": st, "points": pts[:2 * g.n_points].reshape(-1, 2), "pathes": pth[:g.n_pathes].copy(),
+            "verts": vts[:2 * g.n_verts].reshape(-1, 2), "tris": ind[:g.n_inds].reshape(-1, 3)}
 
 
 def blend_over(d, s) -> np.ndarray:

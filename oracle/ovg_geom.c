@@ -349,12 +349,16 @@ static void emit_buf(void *u, float x, float y) {
     }
     b->n++;
 }
-uint32_t ovg_flatten_cubic(float x0, float y0, float x1, float y1, float x2, float y2, float x3, float y3,
-                           float tolerance, float *out_xy, uint32_t cap) {
+uint32_t ovg_flatten_cubic_level(float x0, float y0, float x1, float y1, float x2, float y2, float x3, float y3,
+                                 float tolerance, unsigned level, float *out_xy, uint32_t cap) {
     flat_buf b = {out_xy, 0, cap};
-    rbez(emit_buf, &b, tolerance, x0, y0, x1, y1, x2, y2, x3, y3, 0);
+    rbez(emit_buf, &b, tolerance, x0, y0, x1, y1, x2, y2, x3, y3, level);
     emit_buf(&b, x3, y3);
     return b.n;
+}
+uint32_t ovg_flatten_cubic(float x0, float y0, float x1, float y1, float x2, float y2, float x3, float y3,
+                           float tolerance, float *out_xy, uint32_t cap) {
+    return ovg_flatten_cubic_level(x0, y0, x1, y1, x2, y2, x3, y3, tolerance, 0, out_xy, cap);
 }
 
 /* _curve_to, src/vkvg_context.c:541-566 */

@@ -32,8 +32,35 @@ int ovg_render_stream(uint32_t width, uint32_t height, const uint32_t *w, size_t
     return ovg_render_stream_src(width, height, w, nw, NULL, 0, out_rgba, out_sig, out_stats);
 }
 
-int ovg_render_stream_src(uint32_t width, uint32_t height, const uint32_t *w, size_t nw, const ovg_source *sources,
-                          uint32_t n_sources, uint8_t *out_rgba, uint32_t *out_sig, uint64_t *out_stats) {
+/* Copies the finished path (and, for a stroke, the stroke's vertex and index caches) of the
+ * first fill / stroke of a stream into `d`, without rasterizing it. */
+static void dump_geometry(octx *c, ovg_geometry *d, bool stroke) {
+    o_finish_path(c);
+    d->n_points = c->pointCount;
+    for (uint32_t k = 0; k < c->pointCount && k < d->cap_points; k++) {
+        d->points[2 * k] = c->points[k].x;
+        d->points[2 * k + 1] = c->points[k].y;
+    }
+    d->n_pathes = c->pathPtr;
+    for (uint32_t k = 0; k < c->pathPtr && k < d->cap_pathes; k++)
+        d->pathes[k] = c->pathes[k];
+    d->n_verts = d->n_inds = 0;
+    if (!stroke)
+        return;
+    o_stroke_geometry(c);
+    d->n_verts = c->vertCount;
+    for (uint32_t k = 0; k < c->vertCount && k < d->cap_verts; k++) {
+        d->verts[2 * k] = c->vc[k].x;
+        d->verts[2 * k + 1] = c->vc[k].y;
+    }
+    d->n_inds = c->indCount;
+    for (uint32_t k = 0; k < c->indCount && k < d->cap_inds; k++)
+        d->inds[k] = c->ic[k];
+}
+
+static int run_stream(uint32_t width, uint32_t height, const uint32_t *w, size_t nw, const ovg_source *sources,
+                      uint32_t n_sources, uint8_t *out_rgba, uint32_t *out_sig, uint64_t *out_stats,
+                      ovg_geometry *dump) {
     ovg_surface S;
     memset(&S, 0, sizeof(S));
     S.width = width;
@@ -79,10 +106,22 @@ int ovg_render_stream_src(uint32_t width, uint32_t height, const uint32_t *w, si
         case OVG_OP_REL_LINE_TO: o_rel_line_to(&c, F(a), F(a + 1)); i += 2; break;
         case OVG_OP_REL_MOVE_TO: o_rel_move_to(&c, F(a), F(a + 1)); i += 2; break;
         case OVG_OP_QUADRATIC_TO: o_quadratic_to(&c, F(a), F(a + 1), F(a + 2), F(a + 3)); i += 4; break;
-        case OVG_OP_FILL: r_fill(&c); o_clear_path(&c); break;
-        case OVG_OP_FILL_PRESERVE: r_fill(&c); break;
-        case OVG_OP_STROKE: r_stroke(&c); o_clear_path(&c); break;
-        case OVG_OP_STROKE_PRESERVE: r_stroke(&c); break;
+        case OVG_OP_FILL:
+        case OVG_OP_FILL_PRESERVE:
+        case OVG_OP_STROKE:
+        case OVG_OP_STROKE_PRESERVE:
+            if (dump) {
+                dump_geometry(&c, dump, op == OVG_OP_STROKE || op == OVG_OP_STROKE_PRESERVE);
+                i = nw;
+                break;
+            }
+            if (op == OVG_OP_FILL || op == OVG_OP_FILL_PRESERVE)
+                r_fill(&c);
+            else
+                r_stroke(&c);
+            if (op == OVG_OP_FILL || op == OVG_OP_STROKE)
+                o_clear_path(&c);
+            break;
         case OVG_OP_PAINT: r_paint(&c); break;
         case OVG_OP_CLEAR: o_clear(&c); break;
         case OVG_OP_CLIP: o_clip_preserve(&c); o_clear_path(&c); break;
@@ -260,4 +299,13 @@ int ovg_render_stream_src(uint32_t width, uint32_t height, const uint32_t *w, si
     free(S.sig);
     free(S.acc);
     return status;
+}
+
+int ovg_render_stream_src(uint32_t width, uint32_t height, const uint32_t *w, size_t nw, const ovg_source *sources,
+                          uint32_t n_sources, uint8_t *out_rgba, uint32_t *out_sig, uint64_t *out_stats) {
+    return run_stream(width, height, w, nw, sources, n_sources, out_rgba, out_sig, out_stats, NULL);
+}
+
+int ovg_geometry_dump(uint32_t width, uint32_t height, const uint32_t *w, size_t nw, ovg_geometry *out) {
+    return run_stream(width, height, w, nw, NULL, 0, NULL, NULL, NULL, out);
 }
