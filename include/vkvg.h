@@ -123,6 +123,20 @@ typedef struct {
     float r, g, b, a;
 } vkvg_color_t; /* :227-232 */
 
+/* text metrics and runs (:240-294): fonts are outside this backend's scope; the text entry points
+ * below exist so that clients link, and draw nothing */
+typedef struct {
+    float ascent, descent, height, max_x_advance, max_y_advance;
+} vkvg_font_extents_t; /* :240-248 */
+typedef struct {
+    float x_bearing, y_bearing, width, height, x_advance, y_advance;
+} vkvg_text_extents_t; /* :256-265 */
+typedef struct _glyph_info_t {
+    int32_t  x_advance, y_advance, x_offset, y_offset;
+    uint32_t codepoint;
+} vkvg_glyph_info_t; /* :272-280 */
+typedef struct _vkvg_text_run_t *VkvgText; /* :294 */
+
 typedef struct _vkvg_context_t *VkvgContext; /* :302 */
 typedef struct _vkvg_surface_t *VkvgSurface; /* :314 */
 typedef struct _vkvg_device_t *VkvgDevice;   /* :321 */
@@ -179,6 +193,9 @@ vkvg_public VkvgDevice    vkvg_device_reference(VkvgDevice dev);               /
 vkvg_public uint32_t      vkvg_device_get_reference_count(VkvgDevice dev);     /* :664 */
 vkvg_public void          vkvg_device_set_dpy(VkvgDevice dev, int hdpy, int vdpy);   /* :673 */
 vkvg_public void          vkvg_device_get_dpy(VkvgDevice dev, int *hdpy, int *vdpy); /* :683 */
+/* :577: contexts are not cached by this backend (creating one allocates no device memory); the
+ * count is kept and read back by nothing */
+vkvg_public void          vkvg_device_set_context_cache_size(VkvgDevice dev, uint32_t maxCount);
 /* Vulkan set-up helpers (include/vkvg.h:692-710): this backend needs no Vulkan instance or device,
  * so it requires no extension and no feature chain */
 vkvg_public void          vkvg_get_required_instance_extensions(const char **pExtensions, uint32_t *pExtCount); /* :692 */
@@ -318,6 +335,42 @@ vkvg_public vkvg_filter_t vkvg_pattern_get_filter(VkvgPattern pat);             
 vkvg_public vkvg_pattern_type_t vkvg_pattern_get_type(VkvgPattern pat);                    /* :1951 */
 vkvg_public void vkvg_pattern_set_matrix(VkvgPattern pat, const vkvg_matrix_t *matrix);    /* :1952 */
 vkvg_public void vkvg_pattern_get_matrix(VkvgPattern pat, vkvg_matrix_t *matrix);          /* :1953 */
+
+/* ---- recording (include/vkvg.h:1960-1970; the reference builds it with VKVG_RECORDING, which this
+ * header defines). Between start and stop a context records its calls instead of executing them;
+ * vkvg_replay calls them again on any context. */
+#ifndef VKVG_RECORDING
+#define VKVG_RECORDING 1
+#endif
+typedef struct _vkvg_recording_t *VkvgRecording;
+vkvg_public void          vkvg_start_recording(VkvgContext ctx);
+vkvg_public VkvgRecording vkvg_stop_recording(VkvgContext ctx);
+vkvg_public void          vkvg_replay(VkvgContext ctx, VkvgRecording rec);
+vkvg_public void          vkvg_replay_command(VkvgContext ctx, VkvgRecording rec, uint32_t cmdIndex);
+vkvg_public void          vkvg_recording_get_command(VkvgRecording rec, uint32_t cmdIndex, uint32_t *cmd,
+                                                     void **dataOffset);
+vkvg_public uint32_t      vkvg_recording_get_count(VkvgRecording rec);
+vkvg_public void         *vkvg_recording_get_data(VkvgRecording rec);
+vkvg_public void          vkvg_recording_destroy(VkvgRecording rec);
+
+/* ---- text (include/vkvg.h:1645-1745): no font support; these draw nothing, leave the context's
+ * status unchanged, report zero extents and zero glyphs, and vkvg_text_run_create returns NULL */
+vkvg_public void     vkvg_select_font_face(VkvgContext ctx, const char *name);                    /* :1645 */
+vkvg_public void     vkvg_load_font_from_path(VkvgContext ctx, const char *path, const char *name); /* :1653 */
+vkvg_public void     vkvg_load_font_from_memory(VkvgContext ctx, unsigned char *fontBuffer, long fontBufferByteSize,
+                                                const char *name);                                /* :1662 */
+vkvg_public void     vkvg_set_font_size(VkvgContext ctx, uint32_t size);                          /* :1670 */
+vkvg_public void     vkvg_show_text(VkvgContext ctx, const char *utf8);                           /* :1680 */
+vkvg_public void     vkvg_text_extents(VkvgContext ctx, const char *utf8, vkvg_text_extents_t *extents); /* :1688 */
+vkvg_public void     vkvg_font_extents(VkvgContext ctx, vkvg_font_extents_t *extents);            /* :1695 */
+vkvg_public VkvgText vkvg_text_run_create(VkvgContext ctx, const char *text);                     /* :1705 */
+vkvg_public VkvgText vkvg_text_run_create_with_length(VkvgContext ctx, const char *text, uint32_t length); /* :1714 */
+vkvg_public void     vkvg_text_run_destroy(VkvgText textRun);                                     /* :1720 */
+vkvg_public void     vkvg_show_text_run(VkvgContext ctx, VkvgText textRun);                       /* :1727 */
+vkvg_public void     vkvg_text_run_get_extents(VkvgText textRun, vkvg_text_extents_t *extents);   /* :1734 */
+vkvg_public uint32_t vkvg_text_run_get_glyph_count(VkvgText textRun);                             /* :1740 */
+vkvg_public void     vkvg_text_run_get_glyph_position(VkvgText textRun, uint32_t index,
+                                                      vkvg_glyph_info_t *pGlyphInfo);             /* :1745 */
 
 #ifdef __cplusplus
 }

@@ -184,6 +184,9 @@ vkvg_public int  vkvg_hip_image_alloc(vkh_engine *eng, uint32_t width, uint32_t 
 vkvg_public void vkvg_hip_image_free(vkh_engine *eng, void *dev_ptr);
 vkvg_public int  vkvg_hip_image_clear(vkh_engine *eng, void *dev_ptr, size_t bytes);
 vkvg_public int  vkvg_hip_image_download(vkh_engine *eng, const void *dev_ptr, void *host, size_t bytes);
+/* clears the stencil byte of a surface's stencil plane (one 32-bit word per pixel; the bytes
+ * holding saved clip levels are kept) */
+vkvg_public int  vkvg_hip_stencil_clear(vkh_engine *eng, void *dev_ptr, size_t npixels);
 vkvg_public int  vkvg_hip_image_upload(vkh_engine *eng, void *dev_ptr, const void *host, size_t bytes);
 /* fused BGRA/RGBA swizzle-free readback with vkvg's truncating unpremultiply
  * (src/vkvg_surface.c:441-452) on the device, then copy to host */

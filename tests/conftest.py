@@ -10,10 +10,13 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
-    # torch (its own HIP runtime) creates its GPU context before any test loads libvkvg.so: loading
-    # the library registers its kernels with the HIP runtime it links, and a later torch context
-    # then leaves the library's device creation failing. Counting devices does not initialise one.
-    import torch
+    # torch is imported before any test loads libvkvg.so (vkvg_amd.lib() does the same), so the
+    # library resolves its HIP runtime to torch's copy and the process holds one runtime; torch's
+    # GPU context is created up front. Counting devices does not initialise one.
+    try:
+        import torch
+    except ImportError:
+        return
     if torch.cuda.device_count() > 0:
         torch.zeros(1, device="cuda")
         torch.cuda.synchronize()
@@ -21,12 +24,15 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def device():
-    # torch (its own HIP runtime) creates its GPU context first, whatever order the tests run in:
-    # some GPU tests use torch tensors beside the library
-    import torch
-    if torch.cuda.is_available():
-        torch.zeros(1, device="cuda")
-        torch.cuda.synchronize()
+    # torch creates its GPU context first, whatever order the tests run in: some GPU tests use
+    # torch tensors beside the library
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.zeros(1, device="cuda")
+            torch.cuda.synchronize()
+    except ImportError:
+        pass
     import vkvg_amd
     dev = vkvg_amd.Device()
     yield dev

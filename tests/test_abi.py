@@ -1,4 +1,5 @@
 """The C-ABI library loads without a GPU and exports every entry point include/*.h declares."""
+import ctypes
 import os
 import re
 import subprocess
@@ -42,12 +43,14 @@ def test_no_device_means_a_status_not_a_crash():
         return
     L = vkvg_amd.lib()
     d = L.vkvg_device_create(None)
+    assert d and L.vkvg_device_status(d) == 17  # VKVG_STATUS_INVALID_DEVICE_CREATE_INFO
+    info = (ctypes.c_uint8 * 48)()
+    d = L.vkvg_device_create(info)
     assert d and L.vkvg_device_status(d) == 16  # VKVG_STATUS_DEVICE_ERROR
     s = L.vkvg_surface_create(d, 16, 16)
     assert L.vkvg_surface_status(s) == 16
     c = L.vkvg_create(s)
     assert L.vkvg_status(c) != 0
-    import ctypes
     L.vkvg_move_to.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_float]
     L.vkvg_move_to(c, 1.0, 2.0)  # no-op on an object in error
     L.vkvg_destroy(c)

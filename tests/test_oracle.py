@@ -259,16 +259,37 @@ def test_dash_on_lengths_follow_the_pattern():
     assert np.all(on[:-1] == 8) and np.all(off == 4)
 
 
-def test_stroke_coverage_stays_within_miter_bound():
+def test_stroke_coverage_stays_within_the_join_bounds():
+    # every covered pixel centre lies within hw of the centreline (+ half a pixel diagonal), or,
+    # for miter joins, within min(lh, lhMax) of a join vertex, lh = hw / cos(alpha / 2) and
+    # lhMax = miterLimit * lineWidth; round and bevel joins never leave the hw band
     rng = np.random.default_rng(2)
-    for join in (0, 1, 2):
-        pts = rng.uniform(10, 90, (6, 2)).astype(np.float32)
-        w = 5.0
-        rgba, _, _, _ = ovg.render(100, 100, _stroke_scene(0, join, w, pts))
-        ys, xs = np.nonzero(rgba[..., 3])
-        q = np.stack([xs + 0.5, ys + 0.5], 1)
-        d = _dist_to_polyline(q, pts.astype(np.float64))
-        assert d.max() <= 10.0 * w / 2 * 2 + 1  # lhMax = miterLimit * lineWidth
+    slack = 0.5 * math.sqrt(2) + 1e-3
+    for trial in range(12):
+        pts = rng.uniform(15, 85, (6, 2)).astype(np.float32)
+        w = float(rng.uniform(2, 9))
+        hw = w / 2
+        for join, limit in ((0, 10.0), (0, 1.5), (1, 10.0), (2, 10.0)):
+            s = Stream()
+            s.set_source_rgba(0, 0, 0, 1)
+            s.set_line_width(w)
+            s.set_line_join(join)
+            s.set_miter_limit(limit)
+            s.polyline(pts)
+            s.stroke()
+            rgba, _, _, _ = ovg.render(100, 100, s.words())
+            ys, xs = np.nonzero(rgba[..., 3])
+            q = np.stack([xs + 0.5, ys + 0.5], 1)
+            ok = _dist_to_polyline(q, pts.astype(np.float64)) <= hw + slack
+            if join == 0:
+                p = pts.astype(np.float64)
+                for i in range(1, len(p) - 1):
+                    v0 = (p[i] - p[i - 1]) / np.linalg.norm(p[i] - p[i - 1])
+                    v1 = (p[i + 1] - p[i]) / np.linalg.norm(p[i + 1] - p[i])
+                    alpha = math.acos(max(-1.0, min(1.0, float(v0 @ v1))))
+                    reach = min(hw / math.cos(alpha / 2) if alpha < math.pi - 1e-6 else np.inf, limit * w)
+                    ok |= np.hypot(q[:, 0] - p[i, 0], q[:, 1] - p[i, 1]) <= reach + slack
+            assert ok.all(), (trial, join, limit, q[~ok][:4].tolist())
 
 
 def test_self_crossing_stroke_composites_per_triangle():

@@ -182,6 +182,34 @@ def test_malformed_files_are_rejected():
     # an unknown critical chunk
     crit = good[:33] + _chunk(b"ABCD", b"x") + good[33:]
     assert _decode(L, crit) is None
+    # tRNS as stb_image rejects it (src/stb_image.h:5006-5039): before PLTE, longer than the
+    # palette, after IDAT, on an image with alpha, or of the wrong length for a colour key
+    idx = np.zeros((2, 2, 1), np.int64)
+    ok_pal = _encode(idx, 2, 2, 8, 3, 1, plte=bytes(6), trns=b"\x10\x20")
+    assert _decode(L, ok_pal) is not None
+    hdr_end = 8 + 25
+    plte_chunk = _chunk(b"PLTE", bytes(6))
+    assert ok_pal[hdr_end:hdr_end + len(plte_chunk)] == plte_chunk
+    trns_chunk = _chunk(b"tRNS", b"\x10\x20")
+    before_plte = ok_pal[:hdr_end] + trns_chunk + plte_chunk + ok_pal[hdr_end + len(plte_chunk) + len(trns_chunk):]
+    assert _decode(L, before_plte) is None
+    assert _decode(L, _encode(idx, 2, 2, 8, 3, 1, plte=bytes(6), trns=b"\x01\x02\x03")) is None
+    iend = _chunk(b"IEND", b"")
+    assert _decode(L, good[:-len(iend)] + _chunk(b"tRNS", bytes(6)) + iend) is None
+    rgba = _encode(np.zeros((2, 2, 4), np.int64), 2, 2, 8, 6, 4, trns=bytes(6))
+    assert _decode(L, rgba) is None
+    assert _decode(L, _encode(np.zeros((2, 2, 3), np.int64), 2, 2, 8, 2, 3, trns=bytes(4))) is None
+
+
+def test_colour_key_of_an_8_bit_image_keeps_its_low_byte():
+    # stb_image compares 8-bit samples with (key & 255): a key of 0x0105 makes samples of 5 transparent
+    L = _lib()
+    s = np.array([[[5], [6]], [[5], [0x05]]], np.int64)
+    got = _decode(L, _encode(s, 2, 2, 8, 0, 1, trns=b"\x01\x05"))
+    assert got[..., 3].tolist() == [[0, 255], [0, 0]]
+    rgb = np.array([[[1, 2, 3], [1, 2, 4]]], np.int64)
+    got = _decode(L, _encode(rgb, 2, 1, 8, 2, 3, trns=b"\xff\x01\x00\x02\x01\x03"))
+    assert got[..., 3].tolist() == [[0, 255]]
 
 
 @pytest.mark.gpu

@@ -68,6 +68,12 @@ EXPORTED = [
     "vkvg_pattern_get_color_stop_count", "vkvg_pattern_get_color_stop_rgba", "vkvg_pattern_destroy",
     "vkvg_pattern_add_color_stop", "vkvg_pattern_set_extend", "vkvg_pattern_set_filter", "vkvg_pattern_get_extend",
     "vkvg_pattern_get_filter", "vkvg_pattern_get_type", "vkvg_pattern_set_matrix", "vkvg_pattern_get_matrix",
+    "vkvg_device_set_context_cache_size", "vkvg_select_font_face", "vkvg_load_font_from_path",
+    "vkvg_load_font_from_memory", "vkvg_set_font_size", "vkvg_show_text", "vkvg_text_extents", "vkvg_font_extents",
+    "vkvg_text_run_create", "vkvg_text_run_create_with_length", "vkvg_text_run_destroy", "vkvg_show_text_run",
+    "vkvg_text_run_get_extents", "vkvg_text_run_get_glyph_count", "vkvg_text_run_get_glyph_position",
+    "vkvg_start_recording", "vkvg_stop_recording", "vkvg_replay", "vkvg_replay_command", "vkvg_recording_get_command",
+    "vkvg_recording_get_count", "vkvg_recording_get_data", "vkvg_recording_destroy",
     # include/vkvg_amd.h
     "vkvg_amd_replay", "vkvg_amd_replay_sources", "vkvg_amd_surface_enable_signature", "vkvg_amd_surface_read_signature",
     "vkvg_amd_surface_read_premultiplied", "vkvg_amd_surface_device_ptr", "vkvg_amd_render_batch",
@@ -75,7 +81,7 @@ EXPORTED = [
     "vkvg_amd_upload", "vkvg_amd_execute", "vkvg_amd_upload_batch", "vkvg_amd_png_decode", "vkvg_amd_png_free",
     # include/vkvg_hip.h
     "vkvg_hip_engine_create", "vkvg_hip_engine_destroy", "vkvg_hip_set_tile_size", "vkvg_hip_set_stream",
-    "vkvg_hip_image_alloc", "vkvg_hip_image_free", "vkvg_hip_image_clear", "vkvg_hip_image_download",
+    "vkvg_hip_image_alloc", "vkvg_hip_image_free", "vkvg_hip_image_clear", "vkvg_hip_stencil_clear", "vkvg_hip_image_download",
     "vkvg_hip_image_upload", "vkvg_hip_image_read_unpremultiplied", "vkvg_hip_render", "vkvg_hip_upload",
     "vkvg_hip_execute", "vkvg_hip_get_stats",
     "vkvg_hip_last_error", "vkvg_hip_item_extents",
@@ -86,13 +92,24 @@ EXPORTED = [
 
 
 def lib() -> ctypes.CDLL:
-    """Loads libvkvg.so; raises when it has not been built."""
+    """Loads libvkvg.so; raises when it has not been built.
+
+    torch ships its own copy of the HIP and HSA runtimes (torch/lib/libamdhip64.so, SONAME
+    libamdhip64.so.7). When torch is loaded first, libvkvg.so's libamdhip64.so.7 dependency
+    resolves to that copy and the process holds one HIP runtime. When libvkvg.so is loaded first,
+    it maps ROCm's libamdhip64.so.7 and a later torch import maps its own copy as well: two HIP and
+    two HSA runtimes in one process, after which device creation can fail. So torch, when it is
+    installed, is imported before the library is loaded."""
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise VkvgError(f"{LIB_PATH} is missing: build it with `python -m vkvg_amd.build` "
                         "(the HIP backend has no CPU fallback)")
+    try:
+        import torch  # noqa: F401  (one HIP runtime per process: torch's, see above)
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, u32, f32, sz = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float, ctypes.c_size_t
     L.vkvg_device_create.argtypes = [vp]
@@ -158,7 +175,8 @@ def _u32p(a: np.ndarray):
 class Device:
     def __init__(self) -> None:
         L = lib()
-        self.h = L.vkvg_device_create(None)
+        self._info = (ctypes.c_uint8 * 48)()  # vkvg_device_create_info_t, zeroed: 1 sample, no Vulkan handles
+        self.h = L.vkvg_device_create(self._info)
         st = L.vkvg_device_status(self.h)
         if st:
             raise VkvgError(f"vkvg_device_create failed (status {st})")

@@ -29,7 +29,7 @@ C_FLAGS = ["-O2", "-fPIC", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-
            "-Wno-unused-function", f"-I{INC}"]
 
 HIP_SOURCES = ["vkh_geom.hip", "vkh_raster.hip", "vkh_engine.hip", "vkh_diff.hip"]
-C_SOURCES = ["vkvg_api.c", "vkvg_png.c"]
+C_SOURCES = ["vkvg_api.c", "vkvg_png.c", "vkvg_record.c"]
 
 
 def _run(cmd: list[str]) -> None:

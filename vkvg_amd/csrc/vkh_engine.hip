@@ -348,6 +348,14 @@ int vkvg_hip_image_clear(vkh_engine *eng, void *p, size_t bytes) {
     CK(hipStreamSynchronize(eng->stream));
     return 0;
 }
+int vkvg_hip_stencil_clear(vkh_engine *eng, void *p, size_t npix) {
+    if (!eng || !p)
+        return -1;
+    vkh::launch_stencil_clear(eng->stream, (uint32_t *)p, npix);
+    CK(hipGetLastError());
+    CK(hipStreamSynchronize(eng->stream));
+    return 0;
+}
 int vkvg_hip_image_download(vkh_engine *eng, const void *p, void *host, size_t bytes) {
     if (!eng || !p || !host)
         return -1;

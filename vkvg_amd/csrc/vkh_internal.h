@@ -179,5 +179,6 @@ void launch_fine(hipStream_t s, bool ext, uint32_t tile, uint32_t words, uint32_
                  const GradPrep *gprep, const PathTile *pts, const Edge *segs, uint32_t nch,
                  const uint32_t *tile_off, const Cmd *cmds);
 void launch_unpremultiply(hipStream_t s, const uint8_t *src, uint8_t *dst, size_t npix);
+void launch_stencil_clear(hipStream_t s, uint32_t *words, size_t npix);
 void launch_check_u8_conversion(hipStream_t s, unsigned long long *bad);
 } // namespace vkh

@@ -999,6 +999,14 @@ __global__ void k_unpremultiply(const uint8_t *__restrict__ src, uint8_t *__rest
     ((uchar4 *)dst)[i] = q;
 }
 
+// clears the stencil byte (byte 0) of each pixel's stencil word; bytes 1..3 hold the saved copies a
+// context spilled, which live in images of their own in the reference and survive a surface clear
+__global__ void k_stencil_clear(uint32_t *__restrict__ words, size_t npix) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < npix)
+        words[i] &= 0xFFFFFF00u;
+}
+
 namespace vkh {
 void launch_draw_bbox_init(hipStream_t s, int4 *bbox, uint2 *erange, uint32_t n) {
     if (n)
@@ -1096,6 +1104,10 @@ void launch_fine(hipStream_t s, bool ext, uint32_t tile, uint32_t words, uint32_
 }
 void launch_check_u8_conversion(hipStream_t s, unsigned long long *bad) {
     hipLaunchKernelGGL(k_check_u8_conversion, dim3(1u << 16), dim3(256), 0, s, bad);
+}
+void launch_stencil_clear(hipStream_t s, uint32_t *words, size_t npix) {
+    if (npix)
+        hipLaunchKernelGGL(k_stencil_clear, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, words, npix);
 }
 void launch_unpremultiply(hipStream_t s, const uint8_t *src, uint8_t *dst, size_t npix) {
     if (npix)

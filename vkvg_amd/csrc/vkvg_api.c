@@ -26,6 +26,7 @@
 #include "vkvg.h"
 #include "vkvg_amd.h"
 #include "vkvg_hip.h"
+#include "vkvg_record.h"
 
 #define PIF   3.14159265358979323846f
 #define PIF_2 1.57079632679489661923f
@@ -65,6 +66,7 @@ struct _vkvg_device_t {
     uint32_t      references;
     vkh_engine   *eng;
     int           hdpy, vdpy;
+    uint32_t      ctx_cache_size; /* vkvg_device_set_context_cache_size: kept, contexts are not cached */
 };
 
 struct _vkvg_surface_t {
@@ -112,6 +114,7 @@ struct _vkvg_context_t {
     vkvg_status_t status;
     uint32_t      references;
     VkvgSurface   surf;
+    vkvg_rec     *recording; /* non-NULL between vkvg_start_recording and vkvg_stop_recording */
     bool          deferred; /* batch mode: frames are collected, not rendered */
 
     /* graphic state */
@@ -180,6 +183,7 @@ static vkvg_status_t s_no_mem = VKVG_STATUS_NO_MEMORY;
 static vkvg_status_t s_null = VKVG_STATUS_NULL_POINTER;
 static vkvg_status_t s_dev_err = VKVG_STATUS_DEVICE_ERROR;
 static vkvg_status_t s_inv_surf = VKVG_STATUS_INVALID_SURFACE;
+static vkvg_status_t s_inv_dev_ci = VKVG_STATUS_INVALID_DEVICE_CREATE_INFO;
 
 /* ---------------------------------------------------------------------------- matrices
  * src/vkvg_matrix.c:107-229 (cairo's formulas) */
@@ -272,8 +276,11 @@ void vkvg_matrix_get_scale(const vkvg_matrix_t *m, float *sx, float *sy) {
 }
 
 /* ---------------------------------------------------------------------------- device */
+/* src/vkvg_device.c:286-291: a NULL create-info is an error; its samples and Vulkan handles have no
+ * meaning here (1 sample, HIP device 0) */
 VkvgDevice vkvg_device_create(vkvg_device_create_info_t *info) {
-    (void)info; /* samples / Vulkan handles have no meaning here: 1 sample, HIP device 0 */
+    if (!info)
+        return (VkvgDevice)&s_inv_dev_ci;
     VkvgDevice dev = (VkvgDevice)calloc(1, sizeof(struct _vkvg_device_t));
     if (!dev)
         return (VkvgDevice)&s_no_mem;
@@ -303,6 +310,10 @@ const void *vkvg_get_device_requirements(VkPhysicalDeviceFeatures *pEnabledFeatu
     return NULL;
 }
 vkvg_status_t vkvg_device_status(VkvgDevice dev) { return !dev ? VKVG_STATUS_NULL_POINTER : dev->status; }
+void vkvg_device_set_context_cache_size(VkvgDevice dev, uint32_t maxCount) {
+    if (!vkvg_device_status(dev))
+        dev->ctx_cache_size = maxCount;
+}
 VkvgDevice vkvg_device_reference(VkvgDevice dev) {
     if (!vkvg_device_status(dev))
         dev->references++;
@@ -425,10 +436,15 @@ void vkvg_surface_destroy(VkvgSurface surf) {
     free(surf);
     vkvg_device_destroy(dev);
 }
+/* vkvg_surface_clear, src/vkvg_surface.c:32-36: colour and stencil cleared. The stencil plane's
+ * word per pixel keeps the saved clip levels a context spilled in its upper bytes (context-owned
+ * images in the reference), so only the stencil byte is cleared. */
 void vkvg_surface_clear(VkvgSurface surf) {
     if (vkvg_surface_status(surf))
         return;
     vkvg_hip_image_clear(surf->dev->eng, surf->img, (size_t)surf->width * surf->height * 4);
+    if (surf->stencil)
+        vkvg_hip_stencil_clear(surf->dev->eng, surf->stencil, (size_t)surf->width * surf->height);
 }
 VkImage vkvg_surface_get_vk_image(VkvgSurface surf) {
     return vkvg_surface_status(surf) ? NULL : (VkImage)surf->img;
@@ -767,6 +783,7 @@ VkvgContext vkvg_create(VkvgSurface surf) {
     return c;
 }
 vkvg_status_t vkvg_status(VkvgContext ctx) { return !ctx ? VKVG_STATUS_NULL_POINTER : ctx->status; }
+vkvg_rec **vkvg_ctx_recording(VkvgContext ctx) { return &ctx->recording; }
 VkvgContext vkvg_reference(VkvgContext ctx) {
     if (!vkvg_status(ctx))
         ctx->references++;
@@ -892,6 +909,7 @@ void vkvg_destroy(VkvgContext ctx) {
     }
     if (ctx->pattern)
         vkvg_pattern_destroy(ctx->pattern);
+    vkvg_rec_free(ctx->recording); /* a recording never stopped (src/vkvg_context.c:261-264) */
     free(ctx->dashes);
     frame_free(&ctx->f);
     vkvg_surface_destroy(ctx->surf);
@@ -1037,16 +1055,28 @@ static void curve_to_i(VkvgContext c, float x1, float y1, float x2, float y2, fl
 void vkvg_new_sub_path(VkvgContext ctx) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_NEW_SUB_PATH, NULL, 0);
+        return;
+    }
     finish_path(ctx);
 }
 void vkvg_new_path(VkvgContext ctx) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_NEW_PATH, NULL, 0);
+        return;
+    }
     clear_path(ctx);
 }
 void vkvg_close_path(VkvgContext ctx) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_CLOSE_PATH, NULL, 0);
+        return;
+    }
     if (ctx->cur_closed)
         return;
     if (ctx->cur_count < 3)
@@ -1062,6 +1092,10 @@ void vkvg_close_path(VkvgContext ctx) {
 void vkvg_rel_line_to(VkvgContext ctx, float dx, float dy) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_REL_LINE_TO, 2, dx, dy);
+        return;
+    }
     if (path_empty(ctx))
         add_point(ctx, 0, 0);
     v2 cp = ctx->cur_last;
@@ -1070,17 +1104,29 @@ void vkvg_rel_line_to(VkvgContext ctx, float dx, float dy) {
 void vkvg_line_to(VkvgContext ctx, float x, float y) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_LINE_TO, 2, x, y);
+        return;
+    }
     line_to_i(ctx, x, y);
 }
 void vkvg_move_to(VkvgContext ctx, float x, float y) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_MOVE_TO, 2, x, y);
+        return;
+    }
     finish_path(ctx);
     add_point(ctx, x, y);
 }
 void vkvg_rel_move_to(VkvgContext ctx, float x, float y) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_REL_MOVE_TO, 2, x, y);
+        return;
+    }
     if (path_empty(ctx))
         add_point(ctx, 0, 0);
     v2 cp = ctx->cur_last;
@@ -1180,21 +1226,37 @@ static void arc_impl(VkvgContext c, float xc, float yc, float radius, float a1, 
 void vkvg_arc(VkvgContext ctx, float xc, float yc, float radius, float a1, float a2) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_ARC, 5, xc, yc, radius, a1, a2);
+        return;
+    }
     arc_impl(ctx, xc, yc, radius, a1, a2, false);
 }
 void vkvg_arc_negative(VkvgContext ctx, float xc, float yc, float radius, float a1, float a2) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_ARC_NEG, 5, xc, yc, radius, a1, a2);
+        return;
+    }
     arc_impl(ctx, xc, yc, radius, a1, a2, true);
 }
 void vkvg_curve_to(VkvgContext ctx, float x1, float y1, float x2, float y2, float x3, float y3) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_CURVE_TO, 6, x1, y1, x2, y2, x3, y3);
+        return;
+    }
     curve_to_i(ctx, x1, y1, x2, y2, x3, y3);
 }
 void vkvg_rel_curve_to(VkvgContext ctx, float x1, float y1, float x2, float y2, float x3, float y3) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_REL_CURVE_TO, 6, x1, y1, x2, y2, x3, y3);
+        return;
+    }
     if (path_empty(ctx)) {
         ctx->status = VKVG_STATUS_NO_CURRENT_POINT;
         return;
@@ -1218,17 +1280,29 @@ static void quadratic_to_i(VkvgContext c, float x1, float y1, float x2, float y2
 void vkvg_quadratic_to(VkvgContext ctx, float x1, float y1, float x2, float y2) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_QUADRATIC_TO, 4, x1, y1, x2, y2);
+        return;
+    }
     quadratic_to_i(ctx, x1, y1, x2, y2);
 }
 void vkvg_rel_quadratic_to(VkvgContext ctx, float x1, float y1, float x2, float y2) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_REL_QUADRATIC_TO, 4, x1, y1, x2, y2);
+        return;
+    }
     v2 cp = ctx->cur_last;
     quadratic_to_i(ctx, cp.x + x1, cp.y + y1, cp.x + x2, cp.y + y2);
 }
 vkvg_status_t vkvg_rectangle(VkvgContext ctx, float x, float y, float w, float h) {
     if (vkvg_status(ctx))
         return ctx ? ctx->status : VKVG_STATUS_NULL_POINTER;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_RECTANGLE, 4, x, y, w, h);
+        return VKVG_STATUS_SUCCESS;
+    }
     finish_path(ctx);
     if (w <= 0 || h <= 0)
         return VKVG_STATUS_INVALID_RECT;
@@ -1280,6 +1354,10 @@ void vkvg_rounded_rectangle2(VkvgContext ctx, float x, float y, float w, float h
 void vkvg_ellipse(VkvgContext ctx, float radiusX, float radiusY, float x, float y, float rot) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_ELLIPSE, 5, radiusX, radiusY, x, y, rot);
+        return;
+    }
     float wtt = radiusX * 4 / 3;
     float dx1 = D_SINF(rot) * radiusY, dy1 = D_COSF(rot) * radiusY;
     float dx2 = D_COSF(rot) * wtt, dy2 = D_SINF(rot) * wtt;
@@ -1390,6 +1468,10 @@ void vkvg_elliptic_arc_to(VkvgContext ctx, float x2, float y2, bool large, bool 
                           float phi) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_elliptic(ctx->recording, VKVG_CMD_ELLIPTICAL_ARC_TO, x2, y2, rx, ry, phi, large, sweep);
+        return;
+    }
     float x1 = 0, y1 = 0;
     vkvg_get_current_point(ctx, &x1, &y1);
     elliptic_arc(ctx, x1, y1, x2, y2, large, sweep, rx, ry, phi);
@@ -1398,6 +1480,10 @@ void vkvg_rel_elliptic_arc_to(VkvgContext ctx, float x2, float y2, bool large, b
                               float phi) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_elliptic(ctx->recording, VKVG_CMD_REL_ELLIPTICAL_ARC_TO, x2, y2, rx, ry, phi, large, sweep);
+        return;
+    }
     float x1 = 0, y1 = 0;
     vkvg_get_current_point(ctx, &x1, &y1);
     elliptic_arc(ctx, x1, y1, x2 + x1, y2 + y1, large, sweep, rx, ry, phi);
@@ -1500,28 +1586,48 @@ static void stroke_preserve_i(VkvgContext c) {
 void vkvg_fill_preserve(VkvgContext ctx) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_FILL_PRESERVE, NULL, 0);
+        return;
+    }
     fill_preserve_i(ctx);
 }
 void vkvg_fill(VkvgContext ctx) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_FILL, NULL, 0);
+        return;
+    }
     fill_preserve_i(ctx);
     clear_path(ctx);
 }
 void vkvg_stroke_preserve(VkvgContext ctx) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_STROKE_PRESERVE, NULL, 0);
+        return;
+    }
     stroke_preserve_i(ctx);
 }
 void vkvg_stroke(VkvgContext ctx) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_STROKE, NULL, 0);
+        return;
+    }
     stroke_preserve_i(ctx);
     clear_path(ctx);
 }
 void vkvg_paint(VkvgContext ctx) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_PAINT, NULL, 0);
+        return;
+    }
     finish_path(ctx);
     if (!no_committed_subpath(ctx)) {
         vkvg_fill(ctx);
@@ -1536,6 +1642,10 @@ static int prev_clip_state(VkvgContext c) { return c->saved ? c->saved->clip_sta
 void vkvg_clear(VkvgContext ctx) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_CLEAR, NULL, 0);
+        return;
+    }
     ctx->clip_state = prev_clip_state(ctx) == CLIP_STATE_CLEAR ? CLIP_STATE_NONE : CLIP_STATE_CLEAR;
     new_draw(ctx, VKH_DRAW_CLEAR);
 }
@@ -1581,11 +1691,19 @@ static void clip_preserve_i(VkvgContext c) {
 void vkvg_clip_preserve(VkvgContext ctx) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_CLIP_PRESERVE, NULL, 0);
+        return;
+    }
     clip_preserve_i(ctx);
 }
 void vkvg_clip(VkvgContext ctx) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_CLIP, NULL, 0);
+        return;
+    }
     clip_preserve_i(ctx);
     clear_path(ctx);
 }
@@ -1593,6 +1711,10 @@ void vkvg_clip(VkvgContext ctx) {
 void vkvg_reset_clip(VkvgContext ctx) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_RESET_CLIP, NULL, 0);
+        return;
+    }
     if (ctx->clip_state == CLIP_STATE_CLEAR)
         return;
     ctx->clip_state = prev_clip_state(ctx) == CLIP_STATE_CLEAR ? CLIP_STATE_NONE : CLIP_STATE_CLEAR;
@@ -1602,6 +1724,10 @@ void vkvg_reset_clip(VkvgContext ctx) {
 void vkvg_save(VkvgContext ctx) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_SAVE, NULL, 0);
+        return;
+    }
     struct ctx_save *sv = (struct ctx_save *)calloc(1, sizeof(struct ctx_save));
     if (!sv) {
         ctx->status = VKVG_STATUS_NO_MEMORY;
@@ -1656,6 +1782,10 @@ void vkvg_save(VkvgContext ctx) {
 void vkvg_restore(VkvgContext ctx) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_RESTORE, NULL, 0);
+        return;
+    }
     struct ctx_save *sv = ctx->saved;
     if (!sv) {
         ctx->status = VKVG_STATUS_INVALID_RESTORE;
@@ -1705,6 +1835,10 @@ void vkvg_restore(VkvgContext ctx) {
 void vkvg_set_opacity(VkvgContext ctx, float opacity) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_SET_OPACITY, 1, opacity);
+        return;
+    }
     ctx->opacity = opacity;
 }
 float vkvg_get_opacity(VkvgContext ctx) { return vkvg_status(ctx) ? 0 : ctx->opacity; }
@@ -1723,18 +1857,30 @@ static uint32_t rgbaf(float r, float g, float b, float a) {
 void vkvg_set_source_color(VkvgContext ctx, uint32_t c) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_u32(ctx->recording, VKVG_CMD_SET_SOURCE_COLOR, c);
+        return;
+    }
     ctx->curColor = c;
     set_solid(ctx);
 }
 void vkvg_set_source_rgba(VkvgContext ctx, float r, float g, float b, float a) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_SET_SOURCE_RGBA, 4, r, g, b, a);
+        return;
+    }
     ctx->curColor = rgbaf(r, g, b, a);
     set_solid(ctx);
 }
 void vkvg_set_source_rgb(VkvgContext ctx, float r, float g, float b) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_SET_SOURCE_RGB, 3, r, g, b);
+        return;
+    }
     ctx->curColor = rgbaf(r, g, b, 1);
     set_solid(ctx);
 }
@@ -1742,6 +1888,10 @@ void vkvg_set_source_rgb(VkvgContext ctx, float r, float g, float b) {
 void vkvg_set_source(VkvgContext ctx, VkvgPattern pat) {
     if (vkvg_status(ctx) || vkvg_pattern_status(pat))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_pattern(ctx->recording, pat);
+        return;
+    }
     if (pat->type != VKVG_PATTERN_TYPE_LINEAR && pat->type != VKVG_PATTERN_TYPE_RADIAL &&
         pat->type != VKVG_PATTERN_TYPE_SURFACE) {
         ctx->status = VKVG_STATUS_PATTERN_TYPE_MISMATCH;
@@ -1808,6 +1958,10 @@ void vkvg_set_source(VkvgContext ctx, VkvgPattern pat) {
 void vkvg_set_source_surface(VkvgContext ctx, VkvgSurface surf, float x, float y) {
     if (vkvg_status(ctx) || vkvg_surface_status(surf))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_surface(ctx->recording, surf, x, y);
+        return;
+    }
     ctx->src_rect[0] = x;
     ctx->src_rect[1] = y;
     VkvgPattern p = vkvg_pattern_create_for_surface(surf);
@@ -1825,32 +1979,56 @@ VkvgPattern vkvg_get_source(VkvgContext ctx) {
     return ctx->pattern;
 }
 void vkvg_set_line_width(VkvgContext ctx, float width) {
-    if (!vkvg_status(ctx))
+    if (vkvg_status(ctx))
+        return;
+    if (ctx->recording) /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_SET_LINE_WIDTH, 1, width);
+    else
         ctx->lineWidth = width;
 }
 float vkvg_get_line_width(VkvgContext ctx) { return vkvg_status(ctx) ? 0 : ctx->lineWidth; }
 void vkvg_set_miter_limit(VkvgContext ctx, float limit) {
-    if (!vkvg_status(ctx))
+    if (vkvg_status(ctx))
+        return;
+    if (ctx->recording) /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_SET_MITER_LIMIT, 1, limit);
+    else
         ctx->miterLimit = limit;
 }
 float vkvg_get_miter_limit(VkvgContext ctx) { return vkvg_status(ctx) ? 0 : ctx->miterLimit; }
 void vkvg_set_line_cap(VkvgContext ctx, vkvg_line_cap_t cap) {
-    if (!vkvg_status(ctx))
+    if (vkvg_status(ctx))
+        return;
+    if (ctx->recording) /* recorded, not executed */
+        vkvg_rec_u32(ctx->recording, VKVG_CMD_SET_LINE_CAP, (uint32_t)cap);
+    else
         ctx->lineCap = cap;
 }
 vkvg_line_cap_t vkvg_get_line_cap(VkvgContext ctx) { return vkvg_status(ctx) ? 0 : ctx->lineCap; }
 void vkvg_set_line_join(VkvgContext ctx, vkvg_line_join_t join) {
-    if (!vkvg_status(ctx))
+    if (vkvg_status(ctx))
+        return;
+    if (ctx->recording) /* recorded, not executed */
+        vkvg_rec_u32(ctx->recording, VKVG_CMD_SET_LINE_JOIN, (uint32_t)join);
+    else
         ctx->lineJoin = join;
 }
 vkvg_line_join_t vkvg_get_line_join(VkvgContext ctx) { return vkvg_status(ctx) ? 0 : ctx->lineJoin; }
 void vkvg_set_operator(VkvgContext ctx, vkvg_operator_t op) {
-    if (!vkvg_status(ctx))
+    if (vkvg_status(ctx))
+        return;
+    if (ctx->recording) /* recorded, not executed */
+        vkvg_rec_u32(ctx->recording, VKVG_CMD_SET_OPERATOR, (uint32_t)op);
+    else
         ctx->op = op;
 }
 vkvg_operator_t vkvg_get_operator(VkvgContext ctx) { return vkvg_status(ctx) ? VKVG_OPERATOR_OVER : ctx->op; }
 void vkvg_set_fill_rule(VkvgContext ctx, vkvg_fill_rule_t fr) {
-    if (!vkvg_status(ctx))
+    if (vkvg_status(ctx))
+        return;
+    if (ctx->recording) /* recorded, not executed */
+        vkvg_rec_u32(ctx->recording, VKVG_CMD_SET_FILL_RULE, (uint32_t)fr);
+    else
         ctx->fillRule = fr;
 }
 vkvg_fill_rule_t vkvg_get_fill_rule(VkvgContext ctx) {
@@ -1859,6 +2037,10 @@ vkvg_fill_rule_t vkvg_get_fill_rule(VkvgContext ctx) {
 void vkvg_set_dash(VkvgContext ctx, const float *dashes, uint32_t num_dashes, float offset) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_dash(ctx->recording, dashes, num_dashes, offset);
+        return;
+    }
     free(ctx->dashes);
     ctx->dashes = NULL;
     ctx->dashCount = num_dashes;
@@ -1894,24 +2076,40 @@ static void update_mat_inv(VkvgContext ctx) {
 void vkvg_translate(VkvgContext ctx, float dx, float dy) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_TRANSLATE, 2, dx, dy);
+        return;
+    }
     vkvg_matrix_translate(&ctx->mat, dx, dy);
     update_mat_inv(ctx);
 }
 void vkvg_scale(VkvgContext ctx, float sx, float sy) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_SCALE, 2, sx, sy);
+        return;
+    }
     vkvg_matrix_scale(&ctx->mat, sx, sy);
     update_mat_inv(ctx);
 }
 void vkvg_rotate(VkvgContext ctx, float radians) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_floats(ctx->recording, VKVG_CMD_ROTATE, 1, radians);
+        return;
+    }
     vkvg_matrix_rotate(&ctx->mat, radians);
     update_mat_inv(ctx);
 }
 void vkvg_transform(VkvgContext ctx, const vkvg_matrix_t *matrix) {
     if (vkvg_status(ctx) || !matrix)
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_TRANSFORM, matrix, sizeof(*matrix));
+        return;
+    }
     vkvg_matrix_t res;
     vkvg_matrix_multiply(&res, &ctx->mat, matrix);
     ctx->mat = res;
@@ -1920,12 +2118,20 @@ void vkvg_transform(VkvgContext ctx, const vkvg_matrix_t *matrix) {
 void vkvg_identity_matrix(VkvgContext ctx) {
     if (vkvg_status(ctx))
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_IDENTITY_MATRIX, NULL, 0);
+        return;
+    }
     vkvg_matrix_init_identity(&ctx->mat);
     update_mat_inv(ctx);
 }
 void vkvg_set_matrix(VkvgContext ctx, const vkvg_matrix_t *matrix) {
     if (vkvg_status(ctx) || !matrix)
         return;
+    if (ctx->recording) { /* recorded, not executed */
+        vkvg_rec_push(ctx->recording, VKVG_CMD_SET_MATRIX, matrix, sizeof(*matrix));
+        return;
+    }
     ctx->mat = *matrix;
     update_mat_inv(ctx);
 }
@@ -2245,4 +2451,56 @@ vkvg_status_t vkvg_amd_render_batch(VkvgDevice dev, uint32_t n, const VkvgSurfac
         return VKVG_STATUS_DEVICE_ERROR;
     }
     return VKVG_STATUS_SUCCESS;
+}
+
+/* ---------------------------------------------------------------------------- text
+ * include/vkvg.h:1645-1745. Fonts (FreeType / HarfBuzz in the reference) are outside this backend:
+ * the entry points exist so that clients link; they draw nothing and leave the status unchanged. */
+void vkvg_select_font_face(VkvgContext ctx, const char *name) {
+    if (!vkvg_status(ctx) && ctx->recording)
+        vkvg_rec_text(ctx->recording, VKVG_CMD_SET_FONT_FACE, name);
+}
+void vkvg_load_font_from_path(VkvgContext ctx, const char *path, const char *name) {
+    (void)name;
+    if (!vkvg_status(ctx) && ctx->recording)
+        vkvg_rec_text(ctx->recording, VKVG_CMD_SET_FONT_PATH, path);
+}
+void vkvg_load_font_from_memory(VkvgContext ctx, unsigned char *buf, long size, const char *name) {
+    (void)ctx; (void)buf; (void)size; (void)name;
+}
+void vkvg_set_font_size(VkvgContext ctx, uint32_t size) {
+    if (!vkvg_status(ctx) && ctx->recording)
+        vkvg_rec_u32(ctx->recording, VKVG_CMD_SET_FONT_SIZE, size);
+}
+void vkvg_show_text(VkvgContext ctx, const char *utf8) {
+    if (!vkvg_status(ctx) && ctx->recording)
+        vkvg_rec_text(ctx->recording, VKVG_CMD_SHOW_TEXT, utf8);
+}
+void vkvg_text_extents(VkvgContext ctx, const char *utf8, vkvg_text_extents_t *extents) {
+    (void)ctx; (void)utf8;
+    if (extents)
+        memset(extents, 0, sizeof(*extents));
+}
+void vkvg_font_extents(VkvgContext ctx, vkvg_font_extents_t *extents) {
+    (void)ctx;
+    if (extents)
+        memset(extents, 0, sizeof(*extents));
+}
+VkvgText vkvg_text_run_create(VkvgContext ctx, const char *text) { (void)ctx; (void)text; return NULL; }
+VkvgText vkvg_text_run_create_with_length(VkvgContext ctx, const char *text, uint32_t length) {
+    (void)ctx; (void)text; (void)length;
+    return NULL;
+}
+void vkvg_text_run_destroy(VkvgText run) { (void)run; }
+void vkvg_show_text_run(VkvgContext ctx, VkvgText run) { (void)ctx; (void)run; }
+void vkvg_text_run_get_extents(VkvgText run, vkvg_text_extents_t *extents) {
+    (void)run;
+    if (extents)
+        memset(extents, 0, sizeof(*extents));
+}
+uint32_t vkvg_text_run_get_glyph_count(VkvgText run) { (void)run; return 0; }
+void vkvg_text_run_get_glyph_position(VkvgText run, uint32_t index, vkvg_glyph_info_t *info) {
+    (void)run; (void)index;
+    if (info)
+        memset(info, 0, sizeof(*info));
 }

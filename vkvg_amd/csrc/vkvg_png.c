@@ -59,7 +59,8 @@ typedef struct {
     uint8_t  pal[256][4];
     uint32_t n_pal;
     int      has_key;
-    uint16_t key[3]; /* tRNS colour key of a grey (key[0]) or RGB image, raw sample values */
+    uint16_t key[3]; /* tRNS colour key of a grey (key[0]) or RGB image: raw 16-bit samples for
+                      * 16-bit images, else the key's low byte scaled to 8 bits as stb_image does */
 } png_info;
 
 static uint32_t sample(const uint8_t *row, uint32_t depth, size_t i) {
@@ -87,14 +88,17 @@ static void expand_row(const png_info *in, const uint8_t *row, uint32_t n, uint8
         case 0: { /* grey */
             const uint8_t g = (uint8_t)(in->depth < 8 ? v[0] * grey_scale[in->depth] : v[0] >> sh);
             o[0] = o[1] = o[2] = g;
-            o[3] = (in->has_key && v[0] == in->key[0]) ? 0 : 255;
+            o[3] = (in->has_key && (in->depth == 16 ? v[0] : g) == in->key[0]) ? 0 : 255;
             break;
         }
         case 2: /* RGB */
             o[0] = (uint8_t)(v[0] >> sh);
             o[1] = (uint8_t)(v[1] >> sh);
             o[2] = (uint8_t)(v[2] >> sh);
-            o[3] = (in->has_key && v[0] == in->key[0] && v[1] == in->key[1] && v[2] == in->key[2]) ? 0 : 255;
+            if (in->depth == 16)
+                o[3] = (in->has_key && v[0] == in->key[0] && v[1] == in->key[1] && v[2] == in->key[2]) ? 0 : 255;
+            else
+                o[3] = (in->has_key && o[0] == in->key[0] && o[1] == in->key[1] && o[2] == in->key[2]) ? 0 : 255;
             break;
         case 3: /* palette (an index beyond the palette was rejected before) */
             memcpy(o, in->pal[v[0]], 4);
@@ -169,20 +173,23 @@ uint8_t *vkvg_amd_png_decode(const uint8_t *buf, size_t n, uint32_t *width, uint
                 in.pal[k][2] = data[3 * k + 2];
             }
         } else if (!memcmp(type, "tRNS", 4)) {
-            if (!have_hdr)
+            /* stb_image (src/stb_image.h:5006-5039): rejected after IDAT, before PLTE, longer than
+             * the palette, or on an image with alpha; a key of an image below 16 bits keeps its
+             * low byte, scaled like the samples (x255 / x85 / x17 for 1 / 2 / 4-bit grey) */
+            static const uint32_t key_scale[9] = {0, 255, 85, 0, 17, 0, 0, 0, 1};
+            if (!have_hdr || idat_n)
                 goto fail;
             if (in.ctype == 3) {
-                if (len > 256)
+                if (!in.n_pal || len > in.n_pal)
                     goto fail;
                 for (uint32_t k = 0; k < len; k++)
                     in.pal[k][3] = data[k];
-            } else if (in.ctype == 0 && len == 2) {
+            } else if ((in.ctype == 0 && len == 2) || (in.ctype == 2 && len == 6)) {
                 in.has_key = 1;
-                in.key[0] = (uint16_t)(data[0] << 8 | data[1]);
-            } else if (in.ctype == 2 && len == 6) {
-                in.has_key = 1;
-                for (int c = 0; c < 3; c++)
-                    in.key[c] = (uint16_t)(data[2 * c] << 8 | data[2 * c + 1]);
+                for (uint32_t c = 0; c < len / 2; c++) {
+                    const uint32_t raw = (uint32_t)(data[2 * c] << 8 | data[2 * c + 1]);
+                    in.key[c] = (uint16_t)(in.depth == 16 ? raw : (uint8_t)((raw & 255) * key_scale[in.depth]));
+                }
             } else {
                 goto fail;
             }

@@ -76,12 +76,13 @@ def _white(s: Stream) -> None:
     s.paint()
 
 
-def config1(variant: int = 0, n_paths: int = 200, size: int = 512) -> Scene:
+def config1(variant: int = 0, n_paths: int = 200, size: int = 512, white: bool = True) -> Scene:
     """512x512: closed paths of n~U{4..12} cubics, control points ~U[0,size)^2, non-zero fill,
-    rgb~U[0,1], alpha~U[0.5,1]."""
+    rgb~U[0,1], alpha~U[0.5,1]. `white=False` leaves the surface transparent underneath."""
     rng = Rng(0x5EED0000 + 16 * 1 + variant)
     s = Stream()
-    _white(s)
+    if white:
+        _white(s)
     cps = 0
     for _ in range(n_paths):
         n = rng.randint(4, 12)
@@ -150,7 +151,7 @@ def config2(variant: int = 0, n_lines: int = 5000, n_stars: int = 2000, width: i
 
 
 def config3(variant: int = 0, n_paths: int = 20000, size: int = 4096, blobs: int = 16,
-            sigma: float = 300.0, scale_lo: float = 8.0, scale_hi: float = 256.0) -> Scene:
+            sigma: float = 300.0, scale_lo: float = 8.0, scale_hi: float = 256.0, white: bool = True) -> Scene:
     """4096x4096 dense scene: paths of 1..40 cubics around centres from a 16-blob Gaussian mixture
     (means ~U, sigma 300, clamped to the surface (*)); control points = centre + N(0, s^2),
     s log-U[8,256] (*). 60% non-zero fill (closed), 40% open stroke with width U[0.5,8] and
@@ -160,7 +161,8 @@ def config3(variant: int = 0, n_paths: int = 20000, size: int = 4096, blobs: int
     k = size / 4096.0
     means = rng.uniform(0, size, 2 * blobs).reshape(blobs, 2)
     s = Stream()
-    _white(s)
+    if white:
+        _white(s)
     s.set_miter_limit(10.0)
     cps = grads = stops_total = 0
     for _ in range(n_paths):
