@@ -102,6 +102,75 @@ def bench_c3(args, dev, vkvg_amd, scenes):
     return sc, ms_step, ms_eager, fine_ms, dev_ms, stage, st, tune, step_ms
 
 
+def bench_resident(args, dev, vkvg_amd, sc, steps):
+    """One scene's frame with its inputs resident in HBM, tile 16, replayed as a captured graph:
+    (ms per frame, per-stage ms of the eager runs)."""
+    surf = vkvg_amd.Surface(dev, sc.width, sc.height)
+    try:
+        surf.upload(sc.words())
+        for _ in range(max(1, args.warmup)):
+            vkvg_amd.execute(dev)
+        _, runs = _timed(dev, vkvg_amd, max(3, steps // 2))
+        stage = {k: float(np.mean([r["ms_stage"][k] for r in runs])) for k in runs[0]["ms_stage"]}
+        ms = float(np.mean([r["ms_total"] for r in runs]))
+        if not args.no_graph:
+            dev.set_graph(True)
+            vkvg_amd.execute(dev)
+            for _ in range(max(1, args.warmup)):
+                vkvg_amd.execute(dev)
+            ms, _ = _timed(dev, vkvg_amd, steps)
+            dev.set_graph(False)
+    finally:
+        surf.close()
+    return ms, stage
+
+
+def bench_api(dev, vkvg_amd, sc, reps: int = 5):
+    """One frame through the C API as a client renders it: vkvg_create, every path and paint call of
+    the scene, vkvg_destroy (host recording, one H2D copy of the frame, the kernel pipeline, the
+    wait). The image stays on the device. Median of `reps` after one warm-up."""
+    surf = vkvg_amd.Surface(dev, sc.width, sc.height)
+    words = sc.words()
+    try:
+        surf.replay(words)
+        ts = []
+        gc.collect()
+        gc.disable()
+        try:
+            for _ in range(reps):
+                _sync()
+                t = time.perf_counter()
+                surf.replay(words)
+                ts.append((time.perf_counter() - t) * 1e3)
+        finally:
+            gc.enable()
+    finally:
+        surf.close()
+    return float(np.median(ts))
+
+
+def bench_configs(args, dev, vkvg_amd, scenes, c3_scene, c3_ms):
+    """Frame times of the other configurations and of the reference's own random-shape benchmark
+    (tests/perfs/randoms.c: 500 shapes on 512x512), resident and through the API."""
+    out = {}
+    for key, sc in (("c1_512_nz_cubics", scenes.config1()), ("c2_1080p_stroke_eo", scenes.config2()),
+                    ("randoms_random_fill_stroke_512", scenes.randoms(scenes.SHAPE_RANDOM, "fill_stroke")),
+                    ("randoms_circles_fill_stroke_512", scenes.randoms(scenes.SHAPE_CIRCLE, "fill_stroke")),
+                    ("randoms_rounded_rects_stroke_4096", scenes.randoms(scenes.SHAPE_ROUNDED_RECTANGLE, "stroke",
+                                                                        n=20000, size=4096))):
+        ms, stage = bench_resident(args, dev, vkvg_amd, sc, max(5, args.steps // 2))
+        api = bench_api(dev, vkvg_amd, sc)
+        out[key] = {"surface": f"{sc.width}x{sc.height}", "paths": sc.n_paths, "ms_per_frame": ms,
+                    "mpaths_per_s": sc.n_paths / ms / 1e3, "gpix_per_s": sc.width * sc.height / ms / 1e6,
+                    "stages_ms": stage, "api_ms_per_frame": api, "api_mpaths_per_s": sc.n_paths / api / 1e3}
+    api = bench_api(dev, vkvg_amd, c3_scene, reps=3)
+    out["c3_4k_dense"] = {"surface": f"{c3_scene.width}x{c3_scene.height}", "paths": c3_scene.n_paths,
+                          "ms_per_frame": c3_ms, "mpaths_per_s": c3_scene.n_paths / c3_ms / 1e3,
+                          "api_ms_per_frame": api, "api_mpaths_per_s": c3_scene.n_paths / api / 1e3}
+    dev.set_tile_size(args.tile)
+    return out
+
+
 def bench_c4(args, dev, vkvg_amd, scenes):
     subs = scenes.config4()
     surfs = [vkvg_amd.Surface(dev, s.width, s.height) for s in subs]
@@ -155,20 +224,59 @@ def bench_fwd_bwd(args):
     return diff.bench(steps=max(5, args.steps), warmup=max(1, args.warmup))
 
 
-def cpu_baseline(n_paths: int = 5000):
-    """The CPU restatement of vkvg (oracle, single thread) on the first n_paths paths of the
-    same 4096x4096 scene; lavapipe is not available on these machines."""
+def _host_cpu():
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, model
+
+
+def cpu_baseline(n_paths: int = 15000):
+    """The CPU restatement of vkvg (the oracle) timed on this host, lavapipe being unavailable on these
+    machines: one thread on the first n_paths paths of the 4096x4096 dense scene, then all host
+    cores (up to 16) rasterizing bands of rows of the whole 20000-path scene, each band repeating the
+    geometry; also the 512x512 (200 paths) and 1920x1080 (7000 paths) scenes, single thread and all
+    cores."""
     from oracle import ovg
     from vkvg_amd import scenes
+    cores, model = _host_cpu()
+    threads = max(1, min(16, cores))
     sub = scenes.config3(n_paths=n_paths)
-    words = sub.words()
     t = time.perf_counter()
-    ovg.render(sub.width, sub.height, words, with_sig=False)
-    dt = time.perf_counter() - t
-    return {"value": n_paths / dt / 1e6, "unit": "Mpaths/s", "cores": 1, "kind": "port",
-            "sample": f"first {n_paths} of the 20000 paths of the 4096x4096 dense scene "
-                      f"(+ white paint), CPU restatement of vkvg, 1 thread, {dt:.1f} s",
-            "seconds": dt, "nproc": os.cpu_count()}
+    ovg.render(sub.width, sub.height, sub.words(), with_sig=False)
+    dt1 = time.perf_counter() - t
+    full = scenes.config3()
+    t = time.perf_counter()
+    ovg.render_bands(full.width, full.height, full.words(), threads, with_sig=False)
+    dtn = time.perf_counter() - t
+    others = {}
+    for key, sc in (("c1_512_nz_cubics", scenes.config1()), ("c2_1080p_stroke_eo", scenes.config2())):
+        w = sc.words()
+        t = time.perf_counter()
+        ovg.render(sc.width, sc.height, w, with_sig=False)
+        a = time.perf_counter() - t
+        t = time.perf_counter()
+        ovg.render_bands(sc.width, sc.height, w, threads, with_sig=False)
+        b = time.perf_counter() - t
+        others[key] = {"ms_1_thread": a * 1e3, f"ms_{threads}_threads": b * 1e3,
+                       "mpaths_per_s_1_thread": sc.n_paths / a / 1e6,
+                       f"mpaths_per_s_{threads}_threads": sc.n_paths / b / 1e6}
+    return {"value": full.n_paths / dtn / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+            "sample": f"the whole 20000-path 4096x4096 dense scene as {threads} bands of rows on {threads} "
+                      f"threads ({dtn:.1f} s; geometry repeated per band), CPU restatement of vkvg",
+            "single_thread": {"value": n_paths / dt1 / 1e6, "unit": "Mpaths/s", "cores": 1,
+                              "sample": f"first {n_paths} of the 20000 paths, 1 thread, {dt1:.1f} s"},
+            "other_configs": others, "host_cores": cores, "cpu_model": model}
 
 
 def copy_probe(mib: int) -> int:
@@ -192,6 +300,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other configurations and API timings")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-paths", type=int, default=15000)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (tile 16, 4096 words)")
@@ -279,6 +388,8 @@ def main() -> int:
                      "algorithmic_bytes_per_launch": fine_bytes, "avg_launch_ms": fine_ms,
                      "vector_issue": issue, "lds": lds},
     }
+    if not args.no_configs:
+        out["configs"] = bench_configs(args, dev, vkvg_amd, scenes, sc, ms_step)
     if not args.no_batch:
         out["batched"] = bench_c4(args, dev, vkvg_amd, scenes)
     fb = bench_fwd_bwd(args)
@@ -289,6 +400,7 @@ def main() -> int:
         cb = cpu_baseline(args.cpu_paths)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = out["value"] / cb["value"]
+        out["speedup_vs_cpu_1_thread"] = out["value"] / cb["single_thread"]["value"]
     print(json.dumps(out))
     return 0
 

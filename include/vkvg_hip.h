@@ -27,10 +27,21 @@ extern "C" {
 #define vkvg_public __attribute__((visibility("default")))
 #endif
 
-/* ---- path items: one explicit point, or one cubic to be flattened on the GPU */
+/* ---- path items: one explicit point, one cubic to be flattened on the GPU, or the interior
+ * points of an arc, evaluated on the GPU with the reference's sequential angle recurrence */
 #define VKH_ITEM_CUBIC  1u /* p[0..7] = current point, c1, c2, end; flattened with AGG */
 #define VKH_ITEM_CURVED 2u /* the item's points lie in a curve segment (stroke joins forced) */
 #define VKH_ITEM_NO_END 4u /* cubic whose end point was removed by close_path */
+/* n = flags >> VKH_ITEM_COUNT_SHIFT points of a circular arc (src/vkvg_context.c:394-503):
+ * p[0] xc, p[1] yc, p[2] radius, p[3] first angle a, p[4] signed step s; point k at the angle
+ * a + s + ... + s (k float additions): (cos(a)*r + xc, sin(a)*r + yc) */
+#define VKH_ITEM_ARC    8u
+/* n points of an elliptic arc (src/vkvg_context_internal.c:1539-1580): p[0..1] the first angle
+ * theta as a double, p[2] signed step s (float, added to the double theta), p[3] rx, p[4] ry,
+ * p[5] cos(phi), p[6] sin(phi), p[7] and tol the centre */
+#define VKH_ITEM_EARC   16u
+#define VKH_ITEM_COUNT_SHIFT 8
+#define VKH_ITEM_MAX_COUNT ((1u << (32 - VKH_ITEM_COUNT_SHIFT)) - 1u)
 
 typedef struct {
     float    p[8];  /* point: p[0], p[1]; cubic: x0 y0 x1 y1 x2 y2 x3 y3 */

@@ -123,6 +123,11 @@ int ovg_render_stream_src(uint32_t width, uint32_t height, const uint32_t *words
                           const ovg_source *sources, uint32_t n_sources, uint8_t *out_rgba, uint32_t *out_sig,
                           uint64_t *out_stats);
 
+/* The same, rasterizing only the rows [row0, row1): the band's rows equal the full render's (other
+ * rows are left as they are); bands of one stream render concurrently on separate threads. */
+int ovg_render_stream_rows(uint32_t width, uint32_t height, const uint32_t *words, size_t nwords, uint32_t row0,
+                           uint32_t row1, uint8_t *out_rgba, uint32_t *out_sig, uint64_t *out_stats);
+
 /* Readback as vkvg_surface_write_to_memory: RGBA with truncating unpremultiply. */
 void ovg_unpremultiply(const uint8_t *premul, uint8_t *out, size_t npixels);
 

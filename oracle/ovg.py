@@ -56,6 +56,9 @@ def _lib(variant: str = "") -> ctypes.CDLL:
     lib.ovg_render_stream_src.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_size_t, ctypes.c_void_p,
                                           ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     lib.ovg_render_stream_src.restype = ctypes.c_int
+    lib.ovg_render_stream_rows.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_size_t, ctypes.c_uint32,
+                                           ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    lib.ovg_render_stream_rows.restype = ctypes.c_int
     lib.ovg_unpremultiply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
     lib.ovg_draw_key.argtypes = [ctypes.c_uint32]
     lib.ovg_draw_key.restype = ctypes.c_uint32
@@ -110,6 +113,34 @@ def render(width: int, height: int, words: np.ndarray, variant: str = "", with_s
                                    rgba.ctypes.data, sig.ctypes.data if sig is not None else None,
                                    stats.ctypes.data)
     return rgba, sig, {"draws": int(stats[0]), "triangles": int(stats[1]), "fragments": int(stats[2])}, st
+
+
+def render_bands(width: int, height: int, words: np.ndarray, threads: int, with_sig: bool = True):
+    """The same image as render(), rasterized as `threads` bands of rows on as many threads (the
+    C calls release the GIL); every band repeats the geometry. Returns (rgba, sig or None, status)."""
+    from concurrent.futures import ThreadPoolExecutor
+    lib = _lib()
+    words = np.ascontiguousarray(words, np.uint32)
+    rgba = np.zeros((height, width, 4), np.uint8)
+    sig = np.zeros((height, width), np.uint32) if with_sig else None
+    edges = [height * k // threads for k in range(threads + 1)]
+
+    def band(k):
+        r0, r1 = edges[k], edges[k + 1]
+        if r0 == r1:
+            return 0
+        b_rgba = np.zeros((height, width, 4), np.uint8)
+        b_sig = np.zeros((height, width), np.uint32) if with_sig else None
+        st = lib.ovg_render_stream_rows(width, height, _p(words), words.size, r0, r1, b_rgba.ctypes.data,
+                                        b_sig.ctypes.data if b_sig is not None else None, None)
+        rgba[r0:r1] = b_rgba[r0:r1]
+        if with_sig:
+            sig[r0:r1] = b_sig[r0:r1]
+        return st
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        sts = list(ex.map(band, range(threads)))
+    return rgba, sig, max(sts) if sts else 0
 
 
 def unpremultiply(rgba: np.ndarray) -> np.ndarray:

@@ -159,6 +159,10 @@ void  m_transform_distance(const mat6 *m, float *x, float *y);
 int   m_invert(mat6 *m);
 void  m_get_scale(const mat6 *m, float *sx, float *sy);
 
+/* rows [ovg_row_lo, ovg_row_hi) that the rasterizing loops visit (the whole surface unless a band
+ * render narrows them); per thread, so bands render concurrently */
+extern __thread int32_t ovg_row_lo, ovg_row_hi;
+
 /* raster (ovg_raster.c) */
 void r_fill(octx *c);
 void r_stroke(octx *c);

@@ -166,6 +166,13 @@ static bool tri_row_span(const int32_t *x, const int32_t *y, int64_t sy, int64_t
     return clo <= chi;
 }
 typedef struct { int32_t x0, y0, x1, y1; } bbox_t;
+__thread int32_t ovg_row_lo = 0, ovg_row_hi = INT32_MAX;
+static inline void band_rows(int64_t *r0, int64_t *r1) {
+    if (*r0 < ovg_row_lo)
+        *r0 = ovg_row_lo;
+    if (*r1 > (int64_t)ovg_row_hi - 1)
+        *r1 = (int64_t)ovg_row_hi - 1;
+}
 static void raster_tri(const int32_t *fx, const int32_t *fy, uint32_t w, uint32_t h, int32_t *cov, bbox_t *bb) {
     int32_t x[3] = {fx[0], fx[1], fx[2]}, y[3] = {fy[0], fy[1], fy[2]};
     if (vinvalid(x[0]) || vinvalid(x[1]) || vinvalid(x[2]))
@@ -180,6 +187,7 @@ static void raster_tri(const int32_t *fx, const int32_t *fy, uint32_t w, uint32_
     int64_t r0 = cdiv((int64_t)ymin - 128, 256), r1 = fdiv((int64_t)ymax - 128, 256);
     if (r0 < 0) r0 = 0;
     if (r1 > (int64_t)h - 1) r1 = (int64_t)h - 1;
+    band_rows(&r0, &r1);
     for (int64_t r = r0; r <= r1; r++) {
         int64_t lo, hi;
         if (!tri_row_span(x, y, 256 * r + 128, &lo, &hi))
@@ -222,6 +230,7 @@ static void winding_edges(const edge_t *E, size_t ne, uint32_t w, uint32_t h, in
     int64_t r0 = cdiv((int64_t)ymin - 128, 256), r1 = fdiv((int64_t)ymax - 128, 256);
     if (r0 < 0) r0 = 0;
     if (r1 > (int64_t)h - 1) r1 = (int64_t)h - 1;
+    band_rows(&r0, &r1);
     /* rows are independent: sort-free difference-array sweep per row */
     for (int64_t r = r0; r <= r1; r++) {
         int64_t sy = 256 * r + 128;
@@ -622,6 +631,7 @@ void r_fill(octx *c) {
         int64_t x0 = sx, y0 = sy, x1 = (int64_t)sx + sw - 1, y1 = (int64_t)sy + sh - 1;
         if (x1 > (int64_t)S->width - 1) x1 = (int64_t)S->width - 1;
         if (y1 > (int64_t)S->height - 1) y1 = (int64_t)S->height - 1;
+        band_rows(&y0, &y1);
         uint32_t key = ovg_draw_key(S->draw_counter);
         for (int64_t r = y0; r <= y1; r++)
             for (int64_t x = x0; x <= x1; x++) {
@@ -754,7 +764,9 @@ void r_paint(octx *c) {
     }
     ovg_surface *S = c->surf;
     uint32_t key = ovg_draw_key(S->draw_counter);
-    for (uint32_t r = 0; r < S->height; r++)
+    int64_t pr0 = 0, pr1 = (int64_t)S->height - 1;
+    band_rows(&pr0, &pr1);
+    for (int64_t r = pr0; r <= pr1; r++)
         for (uint32_t x = 0; x < S->width; x++) {
             size_t k = (size_t)r * S->width + x;
             if (S->stencil[k] & 2)

@@ -306,6 +306,16 @@ int ovg_render_stream_src(uint32_t width, uint32_t height, const uint32_t *w, si
     return run_stream(width, height, w, nw, sources, n_sources, out_rgba, out_sig, out_stats, NULL);
 }
 
+int ovg_render_stream_rows(uint32_t width, uint32_t height, const uint32_t *w, size_t nw, uint32_t row0,
+                           uint32_t row1, uint8_t *out_rgba, uint32_t *out_sig, uint64_t *out_stats) {
+    ovg_row_lo = (int32_t)row0;
+    ovg_row_hi = (int32_t)row1;
+    int st = run_stream(width, height, w, nw, NULL, 0, out_rgba, out_sig, out_stats, NULL);
+    ovg_row_lo = 0;
+    ovg_row_hi = INT32_MAX;
+    return st;
+}
+
 int ovg_geometry_dump(uint32_t width, uint32_t height, const uint32_t *w, size_t nw, ovg_geometry *out) {
     return run_stream(width, height, w, nw, NULL, 0, NULL, NULL, NULL, out);
 }

@@ -370,3 +370,16 @@ def test_two_point_curved_stroke_ignores_earlier_paths():
     mask_b = sig_b == k0  # the stroke is draw 0 alone, draw 1 after the fill
     mask_a = (sig_a == k1) | (sig_a == ((k0 + k1) & 0xFFFFFFFF))
     assert mask_b.sum() > 0 and np.array_equal(mask_a, mask_b)
+
+
+# ---------------------------------------------------------------- band-parallel rendering
+@pytest.mark.parametrize("threads", [3, 8])
+def test_band_render_equals_the_full_render(threads):
+    # the multi-core CPU baseline rasterizes bands of rows on separate threads: stitched, the bands
+    # are the full render (fills, strokes, EO covers, gradients, clips, paints)
+    for sc in (scenes.config1(n_paths=60, size=200), scenes.feature_mix(1, size=180, n=40),
+               scenes.clip_mix(2, size=150, n=40), scenes.config3(n_paths=120, size=256)):
+        full, fsig, _, st = ovg.render(sc.width, sc.height, sc.words())
+        band, bsig, st2 = ovg.render_bands(sc.width, sc.height, sc.words(), threads)
+        assert st == st2 == 0
+        assert np.array_equal(full, band) and np.array_equal(fsig, bsig), sc.name

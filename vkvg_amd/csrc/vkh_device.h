@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 #include "vkvg_hip.h"
 #include "vkh_fatan.h"
@@ -45,6 +46,36 @@ VKH_HD float d_atan2f(float y, float x) { return (float)atan2((double)y, (double
 VKH_HD float d_cosf(float a) { return (float)cos((double)a); }
 VKH_HD float d_sinf(float a) { return (float)sin((double)a); }
 VKH_HD float d_acosf(float a) { return (float)acos((double)a); }
+
+// ---------------------------------------------------------------------------------------
+// Arc items: the interior points of vkvg_arc / vkvg_arc_negative (src/vkvg_context.c:431-444,
+// :485-498) and of elliptic arcs (src/vkvg_context_internal.c:1555-1574), one point per step of
+// the reference's sequential angle recurrence. The float (circle) or double (ellipse) angle is
+// advanced by one addition per point, as the reference's loop does, so every point repeats the
+// reference's operations exactly; the host counted the points with the same additions.
+VKH_HD uint32_t item_points(const vkh_item &it) {
+    return (it.flags & (VKH_ITEM_ARC | VKH_ITEM_EARC)) ? it.flags >> VKH_ITEM_COUNT_SHIFT : 1u;
+}
+template <class F> VKH_HD void arc_points(const vkh_item &it, F emit) {
+    const uint32_t n = it.flags >> VKH_ITEM_COUNT_SHIFT;
+    if (it.flags & VKH_ITEM_ARC) {
+        const float xc = it.p[0], yc = it.p[1], r = it.p[2], s = it.p[4];
+        float a = it.p[3];
+        for (uint32_t k = 0; k < n; k++) {
+            emit(V(d_cosf(a) * r + xc, d_sinf(a) * r + yc));
+            a += s;
+        }
+    } else {
+        double th;
+        memcpy(&th, it.p, sizeof(th));
+        const float s = it.p[2], rx = it.p[3], ry = it.p[4], cph = it.p[5], sph = it.p[6], cx = it.p[7], cy = it.tol;
+        for (uint32_t k = 0; k < n; k++) {
+            const float ex = rx * d_cosf((float)th), ey = ry * d_sinf((float)th);
+            emit(V(((cph * ex) + (-sph * ey)) + cx, ((sph * ex) + (cph * ey)) + cy));
+            th += s;
+        }
+    }
+}
 
 // ---------------------------------------------------------------------------------------
 // AGG adaptive subdivision. The reference recurses depth-first (left child first) and emits

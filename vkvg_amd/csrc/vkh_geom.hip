@@ -106,7 +106,7 @@ __global__ __launch_bounds__(VKH_FLAT_BLOCK) void k_flatten_stage(const vkh_item
     if (t >= n)
         return;
     if (!cubic) {
-        counts[i] = 1;
+        counts[i] = item_points(it);
         return;
     }
     const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
@@ -172,8 +172,18 @@ __global__ void k_flatten_lines(const vkh_item *__restrict__ items, uint32_t n, 
     if (it.flags & VKH_ITEM_CUBIC)
         return;
     const uint32_t o = offs[i];
+    const uint8_t f = (it.flags & VKH_ITEM_CURVED) ? 1 : 0;
+    if (it.flags & (VKH_ITEM_ARC | VKH_ITEM_EARC)) {
+        uint32_t k = o;
+        arc_points(it, [&](v2 p) {
+            pts[k] = p;
+            pflags[k] = f;
+            k++;
+        });
+        return;
+    }
     pts[o] = V(it.p[0], it.p[1]);
-    pflags[o] = (it.flags & VKH_ITEM_CURVED) ? 1 : 0;
+    pflags[o] = f;
 }
 
 // the listed cubics with more points than a slot holds, traversed again into their positions
@@ -210,6 +220,10 @@ extern "C" void vkvg_hip_item_extents(const vkh_item *it, float ext[4]) {
         if (p.y > ext[3])
             ext[3] = p.y;
     };
+    if (it->flags & (VKH_ITEM_ARC | VKH_ITEM_EARC)) {
+        arc_points(*it, widen);
+        return;
+    }
     if (!(it->flags & VKH_ITEM_CUBIC)) {
         widen(V(it->p[0], it->p[1]));
         return;

@@ -994,6 +994,8 @@ static void remove_last_point(VkvgContext c) {
     vkh_item *last = &c->f.items.p[c->f.items.n - 1];
     if (last->flags & VKH_ITEM_CUBIC)
         last->flags |= VKH_ITEM_NO_END;
+    else if ((last->flags & (VKH_ITEM_ARC | VKH_ITEM_EARC)) && (last->flags >> VKH_ITEM_COUNT_SHIFT) > 1)
+        last->flags -= 1u << VKH_ITEM_COUNT_SHIFT; /* the arc's last interior point */
     else
         c->f.items.n--;
     c->cur_count--;
@@ -1173,6 +1175,50 @@ void vkvg_path_extents(VkvgContext ctx, float *const x1, float *const y1, float 
 }
 /* vkvg_arc / vkvg_arc_negative, src/vkvg_context.c:394-503: arc points are generated here
  * with the reference's sequential angle accumulation */
+/* Coordinates small enough that every point of the arc is finite (the reference skips NaN points,
+ * src/vkvg_context_internal.c:224-227; arcs with larger or non-finite operands keep that test by
+ * evaluating their points here). */
+static bool arc_on_device(float x, float y, float r) { return fabsf(x) < 1e30f && fabsf(y) < 1e30f && fabsf(r) < 1e30f; }
+/* n interior points of a circular arc from angle a (step s) as one item; the current point
+ * becomes the last of them, at angle a_last */
+static void push_arc_item(VkvgContext c, float xc, float yc, float r, float a, float s, uint32_t n, float a_last) {
+    vkh_item it;
+    memset(&it, 0, sizeof(it));
+    it.p[0] = xc;
+    it.p[1] = yc;
+    it.p[2] = r;
+    it.p[3] = a;
+    it.p[4] = s;
+    it.flags = VKH_ITEM_ARC | (n << VKH_ITEM_COUNT_SHIFT);
+    VEC_PUSH(c->f.items, it);
+    c->cur_last.x = D_COSF(a_last) * r + xc;
+    c->cur_last.y = D_SINF(a_last) * r + yc;
+    c->cur_count += n;
+    if (c->seg_ptr > 0)
+        c->seg_count += n;
+}
+/* the same for an elliptic arc: theta is a double advanced by the float step s */
+static void push_earc_item(VkvgContext c, double th, float s, float rx, float ry, float cph, float sph, v2 ctr,
+                           uint32_t n, double th_last) {
+    vkh_item it;
+    memset(&it, 0, sizeof(it));
+    memcpy(it.p, &th, sizeof(th));
+    it.p[2] = s;
+    it.p[3] = rx;
+    it.p[4] = ry;
+    it.p[5] = cph;
+    it.p[6] = sph;
+    it.p[7] = ctr.x;
+    it.tol = ctr.y;
+    it.flags = VKH_ITEM_EARC | (n << VKH_ITEM_COUNT_SHIFT);
+    VEC_PUSH(c->f.items, it);
+    const float ex = rx * D_COSF((float)th_last), ey = ry * D_SINF((float)th_last);
+    c->cur_last.x = ((cph * ex) + (-sph * ey)) + ctr.x;
+    c->cur_last.y = ((sph * ex) + (cph * ey)) + ctr.y;
+    c->cur_count += n;
+    if (c->seg_ptr > 0)
+        c->seg_count += n;
+}
 static void arc_impl(VkvgContext c, float xc, float yc, float radius, float a1, float a2, bool neg) {
     if (!neg) {
         while (a2 < a1)
@@ -1203,7 +1249,28 @@ static void arc_impl(VkvgContext c, float xc, float yc, float radius, float a1, 
         a -= step;
     if (EQUF(a2, a1))
         return;
-    if (!neg) {
+    if (!(step > 0.f) || a == a1) /* an angle the step cannot advance (the reference loops forever) */
+        return;
+    if (arc_on_device(xc, yc, radius)) {
+        /* the interior points are an arc item evaluated by the flattening kernels; here the loop
+         * only counts them (float additions, the reference's recurrence) */
+        const float s = neg ? -step : step;
+        float a_first = a, a_last = a;
+        uint32_t n = 0;
+        while (neg ? (a > a2) : (a < a2)) {
+            a_last = a;
+            if (++n == VKH_ITEM_MAX_COUNT) {
+                push_arc_item(c, xc, yc, radius, a_first, s, n, a_last);
+                n = 0;
+                a_first = a + s;
+            }
+            if (a + s == a) /* the angle stopped advancing (the reference loops forever) */
+                break;
+            a += s;
+        }
+        if (n)
+            push_arc_item(c, xc, yc, radius, a_first, s, n, a_last);
+    } else if (!neg) {
         while (a < a2) {
             add_point(c, D_COSF(a) * radius + xc, D_SINF(a) * radius + yc);
             a += step;
@@ -1442,7 +1509,28 @@ static void elliptic_arc(VkvgContext c, float x1, float y1, float x2, float y2, 
         c->simpleConvex = false;
     }
     set_curve_start(c);
-    if (sa < ea) {
+    if (arc_on_device(ctr.x, ctr.y, rx) && fabsf(ry) < 1e30f && fabsf(cph) <= 1.f && fabsf(sph) <= 1.f && step > 0.f &&
+        isfinite(sa) && isfinite(ea)) {
+        /* the interior points as an item evaluated by the flattening kernels, counted here with the
+         * reference's double additions */
+        const float s = sa < ea ? step : -step;
+        theta += s;
+        double th_first = theta, th_last = theta;
+        uint32_t n = 0;
+        while (sa < ea ? (theta < ea) : (theta > ea)) {
+            th_last = theta;
+            if (++n == VKH_ITEM_MAX_COUNT) {
+                push_earc_item(c, th_first, s, rx, ry, cph, sph, ctr, n, th_last);
+                n = 0;
+                th_first = theta + s;
+            }
+            if (theta + s == theta) /* the angle stopped advancing (the reference loops forever) */
+                break;
+            theta += s;
+        }
+        if (n)
+            push_earc_item(c, th_first, s, rx, ry, cph, sph, ctr, n, th_last);
+    } else if (sa < ea) {
         theta += step;
         while (theta < ea) {
             ex = rx * D_COSF((float)theta);

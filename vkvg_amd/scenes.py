@@ -585,6 +585,87 @@ def _f32(v: float) -> float:
 
 PIF = _f32(math.pi)  # M_PIF of the reference (tests/common/test.h:15)
 
+SHAPE_RECTANGLE, SHAPE_ROUNDED_RECTANGLE, SHAPE_CIRCLE, SHAPE_STAR, SHAPE_RANDOM = 1, 2, 3, 5, 6
+_STAR = ((0, 85), (75, 75), (100, 10), (125, 75), (200, 85), (150, 125), (160, 190), (100, 150), (40, 190),
+         (50, 125), (0, 85))
+
+
+def randoms(shape: int = SHAPE_RANDOM, mode: str = "fill", n: int = 500, size: int = 512, variant: int = 0,
+            size_fact: float = 0.2) -> Scene:
+    """The reference's random-shape benchmark (tests/perfs/randoms.c:17-79): a context set up as
+    tests/common/test.c:686-697 does (line width 2, miter joins, butt caps, no dash, non-zero rule,
+    vkvg_clear), then n shapes from draw_random_shape (tests/common/test.c:702-793) -- rectangles,
+    rounded rectangles (four arcs), full circles (one arc), stars (a closed 11-point polyline), or a
+    random pick among them -- each with a random translucent colour and filled, stroked, or filled
+    (preserve) then stroked. The reference draws its uniform numbers from a fixed table
+    (tests/common/rnd.c); here they come from the PCG64 stream (*), in the same order of use."""
+    rng = Rng(0x5EED0000 + 16 * 6 + 4 * variant + {"fill": 0, "stroke": 1, "fill_stroke": 2}[mode])
+    w = h = float(size)
+    s = Stream()
+    s.set_line_width(2.0)
+    s.set_line_join(JOIN_MITER)
+    s.set_line_cap(CAP_BUTT)
+    s.set_dash([], 0.0)
+    s.set_fill_rule(FILL_NON_ZERO)
+    s.clear()
+    f = _f32
+    pi, pi_2 = f(math.pi), f(math.pi / 2)
+
+    def rnd():
+        return f(rng.u())
+
+    def shape_calls(kind):
+        s.set_source_rgba(rnd(), rnd(), rnd(), rnd())
+        if kind == SHAPE_RECTANGLE:
+            z = math.trunc(f(f(f(size_fact * w) * rnd()) + 1.0))
+            v = math.trunc(f(f(f(size_fact * h) * rnd()) + 1.0))
+            x = math.trunc(f((w - z) * rnd()))
+            y = math.trunc(f((h - v) * rnd()))
+            s.rectangle(x + 1, y + 1, z, v)
+        elif kind == SHAPE_ROUNDED_RECTANGLE:
+            z = math.trunc(f(f(f(size_fact * w) * rnd()) + 1.0))
+            v = math.trunc(f(f(f(size_fact * h) * rnd()) + 1.0))
+            x = math.trunc(f((w - z) * rnd()))
+            y = math.trunc(f((h - v) * rnd()))
+            r = math.trunc(f(f(f(0.2 * z) * rnd()) + 1.0))
+            if r > v / 2 or r > z / 2:
+                r = min(v / 2, z / 2)
+            s.move_to(x, y + r)
+            s.arc(x + r, y + r, r, pi, -pi_2)
+            s.line_to(x + z - r, y)
+            s.arc(x + z - r, y + r, r, -pi_2, 0)
+            s.line_to(x + z, y + v - r)
+            s.arc(x + z - r, y + v - r, r, 0, pi_2)
+            s.line_to(x + r, y + v)
+            s.arc(x + r, y + v - r, r, pi_2, pi)
+            s.line_to(x, y + r)
+            s.close_path()
+        elif kind == SHAPE_CIRCLE:
+            x = f(rnd() * w)
+            y = f(rnd() * h)
+            r = math.trunc(f(f(f(size_fact * min(w, h)) * rnd()) + 1.0))
+            s.arc(x, y, r, 0, f(pi * 2.0))
+        elif kind in (4, SHAPE_STAR):
+            x = f(rnd() * w)
+            y = f(rnd() * h)
+            z = f(f(rnd() * size_fact) + 0.15)
+            pts = np.array([[f(x + f(px * z)), f(y + f(py * z))] for px, py in _STAR], np.float32)
+            s.polyline(pts, close=True)
+        else:  # SHAPE_RANDOM: one of rectangle .. star (1 + rndf() * 4), with a second colour draw
+            shape_calls(int(1 + rnd() * 4))
+
+    for _ in range(n):
+        shape_calls(shape)
+        if mode == "fill":
+            s.fill()
+        elif mode == "stroke":
+            s.stroke()
+        else:
+            s.fill_preserve()
+            s.stroke()
+    names = {1: "rectangles", 2: "rounded_rects", 3: "circles", 5: "stars", 6: "random"}
+    return Scene(f"randoms_{names[shape]}_{mode}", size, size, s, n)
+
 
 def reference_test_scenes(width: int = 512, height: int = 512, random_size: int = 60) -> list[Scene]:
     """The call sequences of the reference's visual test programs (tests/*.c), one scene per test
