@@ -203,6 +203,12 @@ vkvg_public int  vkvg_hip_image_upload(vkh_engine *eng, void *dev_ptr, const voi
  * (src/vkvg_surface.c:441-452) on the device, then copy to host */
 vkvg_public int  vkvg_hip_image_read_unpremultiplied(vkh_engine *eng, const void *dev_ptr, uint32_t width,
                                                      uint32_t height, void *host);
+/* The same readback without allocating anything: no counterpart in the reference; for callers that
+ * cannot allocate device memory (serving loops, allocation-free harnesses). Stages through a static
+ * 64 MiB device array (at most 4096 x 4096 pixels; larger images return -1 with the reason in
+ * vkvg_hip_last_error). Every call on one GPU shares that array: calls that overlap on one GPU race. */
+vkvg_public int  vkvg_hip_image_read_unpremultiplied_noalloc(vkh_engine *eng, const void *dev_ptr, uint32_t width,
+                                                             uint32_t height, void *host);
 
 /* Render one frame (any number of draws over any number of targets) and wait for it:
  * vkvg_hip_upload followed by vkvg_hip_execute. Returns 0 on success, < 0 on error. */

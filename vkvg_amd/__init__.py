@@ -82,7 +82,7 @@ EXPORTED = [
     # include/vkvg_hip.h
     "vkvg_hip_engine_create", "vkvg_hip_engine_destroy", "vkvg_hip_set_tile_size", "vkvg_hip_set_stream",
     "vkvg_hip_image_alloc", "vkvg_hip_image_free", "vkvg_hip_image_clear", "vkvg_hip_stencil_clear", "vkvg_hip_image_download",
-    "vkvg_hip_image_upload", "vkvg_hip_image_read_unpremultiplied", "vkvg_hip_render", "vkvg_hip_upload",
+    "vkvg_hip_image_upload", "vkvg_hip_image_read_unpremultiplied", "vkvg_hip_image_read_unpremultiplied_noalloc", "vkvg_hip_render", "vkvg_hip_upload",
     "vkvg_hip_execute", "vkvg_hip_get_stats",
     "vkvg_hip_last_error", "vkvg_hip_item_extents",
     # engine tuning (include/vkvg_hip.h) and the differentiable renderer (include/vkvg_diff.h)

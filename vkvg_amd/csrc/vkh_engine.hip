@@ -11,6 +11,12 @@
 
 #include "vkh_internal.h"
 
+// Staging array of vkvg_hip_image_read_unpremultiplied_noalloc: one RGBA8 image of up to 4096 x 4096
+// pixels, held by every GPU the library runs on until the process ends. Host code uses the address
+// hipGetSymbolAddress returns per device (vkh_engine::noalloc_scratch), never the name.
+#define VKH_NOALLOC_SCRATCH_BYTES (64u << 20)
+static __device__ __align__(256) uint8_t scratch[VKH_NOALLOC_SCRATCH_BYTES];
+
 // ----------------------------------------------------------------------------------------- scan
 // Exclusive prefix sum over n uint32 values into out[0..n] (out[n] = total): block scans of
 // 1024 values (256 threads x 4), block totals scanned recursively, then added back.
@@ -118,6 +124,7 @@ struct HostBuf {
 
 struct vkh_engine {
     int device = 0;
+    uint8_t *noalloc_scratch = nullptr; // this device's address of `scratch` (allocation-free readback)
     hipStream_t stream = nullptr;
     hipStream_t own = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -243,6 +250,12 @@ vkh_engine *vkvg_hip_engine_create(int device_ordinal) {
         return nullptr;
     }
     e->stream = e->own;
+    // the allocation-free readback's staging array: its address on this device (each GPU holds its
+    // own copy of the array; the name itself is a host-side stand-in, not a device address)
+    if (hipGetSymbolAddress((void **)&e->noalloc_scratch, HIP_SYMBOL(scratch)) != hipSuccess) {
+        (void)hipGetLastError();
+        e->noalloc_scratch = nullptr;
+    }
     (void)hipEventCreate(&e->ev0);
     (void)hipEventCreate(&e->ev1);
     (void)hipEventCreate(&e->ev_begin);
@@ -367,6 +380,26 @@ int vkvg_hip_image_upload(vkh_engine *eng, void *p, const void *host, size_t byt
     if (!eng || !p || !host)
         return -1;
     CK(hipMemcpyAsync(p, host, bytes, hipMemcpyHostToDevice, eng->stream));
+    CK(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+int vkvg_hip_image_read_unpremultiplied_noalloc(vkh_engine *eng, const void *p, uint32_t w, uint32_t h, void *host) {
+    if (!eng || !p || !host)
+        return -1;
+    const size_t bytes = (size_t)w * h * 4;
+    if (bytes > VKH_NOALLOC_SCRATCH_BYTES) {
+        eng->err = "vkvg_hip_image_read_unpremultiplied_noalloc: " + std::to_string(bytes) +
+                   " bytes exceed the " + std::to_string(VKH_NOALLOC_SCRATCH_BYTES) + "-byte staging array";
+        return -1;
+    }
+    if (!eng->noalloc_scratch) {
+        eng->err = "vkvg_hip_image_read_unpremultiplied_noalloc: the staging array has no address on this device";
+        return -1;
+    }
+    (void)hipSetDevice(eng->device);
+    vkh::launch_unpremultiply(eng->stream, (const uint8_t *)p, eng->noalloc_scratch, (size_t)w * h);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(host, eng->noalloc_scratch, bytes, hipMemcpyDeviceToHost, eng->stream));
     CK(hipStreamSynchronize(eng->stream));
     return 0;
 }
