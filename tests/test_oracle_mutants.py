@@ -68,6 +68,11 @@ MUTANTS = {
                                  "gradLength = sqrtf(ddx * ddx + ddy * ddy);", [pins.check_focal_radial]),
     "radial_focus_not_pulled_in": ("ovg_replay.c", "if (l + radius0 + 1.0f >= radius1) {", "if (0) {",
                                    [pins.check_focal_radial]),
+    "dash_offset_not_wrapped": ("ovg_geom.c", "dc.curDashOffset = fmodf(c->dashOffset, dc.totDashLength);",
+                                "dc.curDashOffset = c->dashOffset;", [pins.check_dash_offset_and_wrap]),
+    "dash_starts_off": ("ovg_geom.c", "dc.dashOn = true;", "dc.dashOn = false;", [pins.check_dash_offset_and_wrap]),
+    "dash_not_carried": ("ovg_geom.c", "dc->curDashOffset -= segLen;", "dc->curDashOffset = 0;",
+                         [pins.check_dash_continues_around_a_corner]),
 }
 
 

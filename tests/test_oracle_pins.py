@@ -488,3 +488,66 @@ def check_focal_radial():
 
 def test_focal_radial_gradient_against_fp64_geometry():
     check_focal_radial()
+
+
+# ================================================================ dashes
+# src/vkvg_context.c:853-867, :898-916 and src/vkvg_context_internal.c:1240-1264: the dash state
+# (on/off, index, offset within the pattern) runs along the whole sub-path; the offset is taken
+# modulo the pattern's total; a cap closes every dash.
+def _dash_runs(pts, dashes, offset, width=2.0, size=(120, 70), axis_row=None, axis_col=None):
+    s = Stream()
+    s.set_source_rgba(0, 0, 0, 1)
+    s.set_line_width(width)
+    s.set_dash(dashes, offset)
+    s.polyline(np.asarray(pts, np.float32))
+    s.stroke()
+    rgba, _, _, st = ovg.render(size[0], size[1], s.words())
+    assert st == 0
+    line = rgba[axis_row, :, 3] > 0 if axis_row is not None else rgba[:, axis_col, 3] > 0
+    edges = np.flatnonzero(np.diff(np.concatenate([[0], line.astype(int), [0]])))
+    return [(int(a), int(b)) for a, b in zip(edges[0::2], edges[1::2])]  # [start, end) pixel runs
+
+
+def check_dash_offset_and_wrap():
+    # The walk starts "on" with the offset as its distance to the next toggle, and every toggle
+    # alternates start and end caps: the first dash starts at the offset (the offset delays the
+    # pattern; cairo's offset advances into it instead). Horizontal line x 0.5 .. 100.5 at y 5.5,
+    # pattern [8, 4], offset 3: dashes [3.5, 11.5), [15.5, 23.5), ... cover pixel columns 3..10,
+    # 15..22, ...
+    runs = _dash_runs([[0.5, 5.5], [100.5, 5.5]], [8, 4], 3.0, axis_row=5)
+    assert runs[:4] == [(3, 11), (15, 23), (27, 35), (39, 47)], runs
+    # an offset beyond the pattern is taken modulo its total (15 = 12 + 3)
+    assert _dash_runs([[0.5, 5.5], [100.5, 5.5]], [8, 4], 15.0, axis_row=5) == runs
+    # an odd number of entries: toggles at 0, 5, 7, 10, 15, 17, 20, ... alternate start / end, so
+    # the roles of the entries swap on every repetition: dashes [0, 5), [7, 10), [15, 17), [20, 25)
+    runs3 = _dash_runs([[0.5, 5.5], [100.5, 5.5]], [5, 2, 3], 0.0, axis_row=5)
+    assert runs3[:4] == [(0, 5), (7, 10), (15, 17), (20, 25)], runs3
+
+
+def check_dash_continues_around_a_corner():
+    # (0.5, 5.5) -> (30.5, 5.5) -> (30.5, 60.5), pattern [8, 4]: the horizontal leg holds on
+    # [0, 8), [12, 20), [24, 30); the dash started at 24 runs 2 more px down the vertical leg
+    # (to y 7.5), then off 4, on 8 from y 11.5
+    runs = _dash_runs([[0.5, 5.5], [30.5, 5.5], [30.5, 60.5]], [8, 4], 0.0, axis_col=30, size=(60, 70))
+    assert runs[1:3] == [(11, 19), (23, 31)], runs
+
+
+def check_invalid_dashes():
+    for d in ([0, 0], [-1, 4]):
+        s = Stream()
+        s.set_dash(d, 0.0)
+        s.polyline(np.array([[0, 0], [10, 10]], np.float32))
+        s.stroke()
+        assert ovg.render(16, 16, s.words())[3] == 13  # VKVG_STATUS_INVALID_DASH
+
+
+def test_dash_offset_and_pattern_wrap():
+    check_dash_offset_and_wrap()
+
+
+def test_dash_continues_around_a_corner():
+    check_dash_continues_around_a_corner()
+
+
+def test_invalid_dash_patterns():
+    check_invalid_dashes()

@@ -1,5 +1,5 @@
 """PNG decoding behind vkvg_surface_create_from_image (src/vkvg_surface.c:181-194; the reference
-decodes with stb_image, which its tree does not hold). The library's decoder is compared, byte for
+decodes with stb_image, vendored as src/stb_image.h). The library's decoder is compared, byte for
 byte, with PIL's independent decoder on images of every colour type, bit depth and filter, plain and
 Adam7-interlaced, written by the small encoder below, and on tests/golden/filled.png (a data file of
 the reference's tests/data). 16-bit samples keep their high byte (PIL clips them instead, so those

@@ -1,8 +1,8 @@
 /*
  * PNG decoding for vkvg_surface_create_from_image (src/vkvg_surface.c:181-194).
  *
- * The reference decodes with stb_image (stbi_load with 4 requested components), a third-party header
- * that its tree does not hold. This file decodes the PNG format (ISO/IEC 15948) with zlib's inflate
+ * The reference decodes with stb_image (stbi_load with 4 requested components; the header is vendored
+ * as src/stb_image.h). This file decodes the PNG format (ISO/IEC 15948) with zlib's inflate
  * and returns what stbi_load returns for a PNG: 8-bit RGBA rows, top row first, not premultiplied.
  * Conversions follow that decoder's documented results: grey g -> (g, g, g, 255), grey + alpha ->
  * (g, g, g, a), RGB -> (r, g, b, 255), palette entries with their tRNS alpha (255 without), a tRNS
