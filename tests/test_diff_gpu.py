@@ -59,7 +59,7 @@ def test_gradients_match_central_differences_on_256_parameters():
     # and widths, absolute for colours) of the same forward evaluated in double precision: near
     # x = 500 px neighbouring float coordinates are 3e-5 px apart, 3% of h, so differences of the
     # float forward would measure its rounding. The denominator is the actual (float) step.
-    # Tolerance 1e-3 relative, floored at 1e-2 of the largest gradient checked.
+    # Tolerance 1e-3 relative, floored at 1e-3 of the largest gradient checked.
     sc = scenes.config5()
     torch, diff, R, ctrl, rgba, width, tctrl = _setup(sc)
     target = R.forward64(tctrl, rgba, width)
@@ -95,7 +95,7 @@ def test_gradients_match_central_differences_on_256_parameters():
         fd.append((lp - lm) / (xp - xm))
         an.append(float(g.view(-1)[i]))
     an, fd = np.array(an), np.array(fd)
-    e = _rel_err(an, fd, floor_frac=1e-2)
+    e = _rel_err(an, fd, floor_frac=1e-3)
     print("FD: max rel err", e.max(), "median", np.median(e), "largest |g|", np.abs(fd).max(), flush=True)
     assert np.isfinite(an).all()
     assert e.max() < 1e-3

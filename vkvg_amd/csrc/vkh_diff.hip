@@ -34,7 +34,9 @@
 
 #define VKD_NSEG 16 // polyline segments per cubic
 #define VKD_T 16    // tile size in pixels
-#define VKD_B 8     // commands per batch
+#define VKD_B 8     // commands per batch of the backward pass (and per compositing checkpoint)
+#define VKD_FB 16   // commands per coverage pass of the forward: 16 commands x 16 rows = 256 threads
+#define VKD_CHUNK 256 // edges of a batch staged in LDS at a time
 #define VKD_BIN_MARGIN 0.01f // px added around an edge when binning (rounding safety)
 
 namespace {
@@ -402,43 +404,94 @@ __device__ __forceinline__ void row_pieces(const RV4<R> &v, R yr, R X0, F f) {
 // ---------------------------------------------------------------------------------- forward
 __device__ __forceinline__ float4 premul(float4 c) { return make_float4(c.x * c.w, c.y * c.w, c.z * c.w, c.w); }
 
-// coverage of batch commands: thread (j, r) owns row r of command j (deterministic order)
+// coverage of batch commands: thread (j, r) owns row r of command j and adds its pieces in the
+// order of the command's segment list (deterministic). The batch's edges are staged in LDS, VKD_CHUNK
+// at a time, by all threads together (segment index then edge: the dependent loads run in parallel
+// across the workgroup), and every row thread of a command reads them from there.
 template <typename R>
 __device__ __forceinline__ void batch_coverage(R (*acc)[VKD_T][VKD_T + 1], uint32_t nb, const uint32_t *c_nseg,
-                                               const uint32_t *c_ntouch, const uint32_t *c_segoff, const uint32_t *__restrict__ segs,
-                                               const RV4<R> *__restrict__ edges, R X0, R Y0) {
+                                               const uint32_t *c_ntouch, const uint32_t *c_segoff,
+                                               const uint32_t *c_item, RV4<R> *s_edge,
+                                               const uint32_t *__restrict__ segs, const RV4<R> *__restrict__ edges,
+                                               R X0, R Y0) {
     const uint32_t tid = threadIdx.x;
     const uint32_t j = tid / VKD_T, r = tid % VKD_T;
-    if (j >= nb)
-        return;
-    R *a = acc[j][r];
-    for (int c = 0; c <= VKD_T; c++)
-        a[c] = (R)0;
+    const bool active = j < nb;
+    R *a = acc[active ? j : 0][r];
+    if (active)
+        for (int c = 0; c <= VKD_T; c++)
+            a[c] = (R)0;
     const R yr = Y0 + (R)r;
-    const uint32_t n = c_nseg[j], nt = c_ntouch[j], off = c_segoff[j];
-    for (uint32_t s = 0; s < nt; s++) {
-        const RV4<R> v = edges[segs[off + s]];
-        row_pieces<R>(v, yr, X0, [&](int lc, R dy, R fm, R, R) {
-            if (lc < 0) {
-                a[0] += dy;
-            } else {
-                a[lc] += dy * ((R)1 - fm);
-                a[lc + 1] += dy * fm;
+    const uint32_t total = c_item[nb];
+    for (uint32_t cb = 0; cb < total; cb += VKD_CHUNK) {
+        const uint32_t ce = min(total, cb + (uint32_t)VKD_CHUNK);
+        for (uint32_t i = cb + tid; i < ce; i += 256) {
+            uint32_t lo = 0, hi = nb; // last command with c_item <= i
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (c_item[mid] <= i)
+                    lo = mid;
+                else
+                    hi = mid;
             }
-        });
+            s_edge[i - cb] = edges[segs[c_segoff[lo] + (i - c_item[lo])]];
+        }
+        __syncthreads();
+        if (active) {
+            const uint32_t i0 = max(c_item[j], cb), i1 = min(c_item[j + 1], ce), nt = c_ntouch[j];
+            for (uint32_t i = i0; i < i1; i++) {
+                const RV4<R> v = s_edge[i - cb];
+                if (i - c_item[j] < nt) {
+                    row_pieces<R>(v, yr, X0, [&](int lc, R dy, R fm, R, R) {
+                        if (lc < 0) {
+                            a[0] += dy;
+                        } else {
+                            a[lc] += dy * ((R)1 - fm);
+                            a[lc + 1] += dy * fm;
+                        }
+                    });
+                } else { // edges left of the tile: the row's whole y-extent, as row_pieces gives for them
+                    const R ylo = fmax(fmin(v.y, v.w), yr), yhi = fmin(fmax(v.y, v.w), yr + (R)1);
+                    if (ylo < yhi)
+                        a[0] += v.w > v.y ? yhi - ylo : ylo - yhi;
+                }
+            }
+        }
+        __syncthreads();
     }
-    // edges left of the tile: the row's whole y-extent, as row_pieces gives for them
-    for (uint32_t s = nt; s < n; s++) {
-        const RV4<R> v = edges[segs[off + s]];
-        const R ylo = fmax(fmin(v.y, v.w), yr), yhi = fmin(fmax(v.y, v.w), yr + (R)1);
-        if (ylo < yhi)
-            a[0] += v.w > v.y ? yhi - ylo : ylo - yhi;
-    }
+    (void)c_nseg;
+    if (!active)
+        return;
     R run = (R)0;
     for (int c = 0; c < VKD_T; c++) {
         run += a[c];
         a[c] = run;
     }
+}
+
+// stages a batch's command records and the prefix of their segment counts
+__device__ __forceinline__ void stage_batch(uint32_t base, uint32_t nb, const Cmd *__restrict__ cmds,
+                                            const PathTile *__restrict__ ptl, uint32_t *c_draw, uint32_t *c_nseg,
+                                            uint32_t *c_ntouch, uint32_t *c_segoff, uint32_t *c_item) {
+    const uint32_t tid = threadIdx.x;
+    if (tid < nb) {
+        const Cmd cm = cmds[base + tid];
+        const PathTile p = ptl[cm.pt];
+        c_draw[tid] = cm.draw;
+        c_nseg[tid] = p.nseg;
+        c_ntouch[tid] = p.reserved;
+        c_segoff[tid] = p.seg_off;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t acc_n = 0;
+        for (uint32_t j = 0; j < nb; j++) {
+            c_item[j] = acc_n;
+            acc_n += c_nseg[j];
+        }
+        c_item[nb] = acc_n;
+    }
+    __syncthreads();
 }
 
 template <typename R>
@@ -448,31 +501,26 @@ __global__ __launch_bounds__(256) void k_dfine(uint32_t W, uint32_t H, uint32_t 
                                                const RV4<R> *__restrict__ edges, const float4 *__restrict__ rgba,
                                                RV4<R> *__restrict__ image, float4 *__restrict__ ckpt,
                                                const uint32_t *__restrict__ batch_off) {
-    __shared__ R acc[VKD_B][VKD_T][VKD_T + 1];
-    __shared__ uint32_t c_nseg[VKD_B], c_ntouch[VKD_B], c_segoff[VKD_B], c_draw[VKD_B];
+    __shared__ R acc[VKD_FB][VKD_T][VKD_T + 1];
+    __shared__ RV4<R> s_edge[VKD_CHUNK];
+    __shared__ uint32_t c_nseg[VKD_FB], c_ntouch[VKD_FB], c_segoff[VKD_FB], c_draw[VKD_FB], c_item[VKD_FB + 1];
+    static_assert(VKD_FB % VKD_B == 0, "checkpoints fall on coverage-pass boundaries or inside");
     const uint32_t t = blockIdx.x, tid = threadIdx.x;
     const uint32_t c0 = tile_off[(size_t)t * nch], c1 = tile_off[(size_t)(t + 1) * nch];
     const int32_t X0 = (int32_t)(t % tilesW) * VKD_T, Y0 = (int32_t)(t / tilesW) * VKD_T;
     const int32_t px = X0 + (int32_t)(tid % VKD_T), py = Y0 + (int32_t)(tid / VKD_T);
     const bool valid = px < (int32_t)W && py < (int32_t)H;
     RV4<R> st{(R)1, (R)1, (R)1, (R)1};
-    uint32_t b = 0;
-    for (uint32_t base = c0; base < c1; base += VKD_B, b++) {
-        const uint32_t nb = min((uint32_t)VKD_B, c1 - base);
-        if (ckpt)
-            ckpt[(size_t)(batch_off[t] + b) * 256 + tid] = make_float4((float)st.x, (float)st.y, (float)st.z, (float)st.w);
-        if (tid < nb) {
-            Cmd cm = cmds[base + tid];
-            PathTile p = ptl[cm.pt];
-            c_draw[tid] = cm.draw;
-            c_nseg[tid] = p.nseg;
-            c_ntouch[tid] = p.reserved;
-            c_segoff[tid] = p.seg_off;
-        }
-        __syncthreads();
-        batch_coverage<R>(acc, nb, c_nseg, c_ntouch, c_segoff, segs, edges, (R)X0, (R)Y0);
+    for (uint32_t base = c0; base < c1; base += VKD_FB) {
+        const uint32_t nb = min((uint32_t)VKD_FB, c1 - base);
+        stage_batch(base, nb, cmds, ptl, c_draw, c_nseg, c_ntouch, c_segoff, c_item);
+        batch_coverage<R>(acc, nb, c_nseg, c_ntouch, c_segoff, c_item, s_edge, segs, edges, (R)X0, (R)Y0);
         __syncthreads();
         for (uint32_t j = 0; j < nb; j++) {
+            // the pixel state before every backward batch of VKD_B commands
+            if (ckpt && ((base - c0 + j) % VKD_B) == 0)
+                ckpt[(size_t)(batch_off[t] + (base - c0 + j) / VKD_B) * 256 + tid] =
+                    make_float4((float)st.x, (float)st.y, (float)st.z, (float)st.w);
             const R A = acc[j][tid / VKD_T][tid % VKD_T];
             const R k = fmin((R)1, fabs(A));
             const float4 cf = rgba[c_draw[j]];
@@ -520,7 +568,8 @@ __global__ __launch_bounds__(256) void k_dback(uint32_t W, uint32_t H, uint32_t 
     __shared__ float acc[VKD_B][VKD_T][VKD_T + 1];
     __shared__ float G[VKD_B][VKD_T][VKD_T];
     __shared__ float s_red[4][32];
-    __shared__ uint32_t c_item[VKD_B + 1];
+    __shared__ RV4<float> s_edge[VKD_CHUNK];
+    __shared__ uint32_t c_item[VKD_B + 1], c_sitem[VKD_B + 1];
     __shared__ uint32_t c_nseg[VKD_B], c_ntouch[VKD_B], c_segoff[VKD_B], c_draw[VKD_B];
     const uint32_t t = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
     const uint32_t c0 = tile_off[(size_t)t * nch], c1 = tile_off[(size_t)(t + 1) * nch];
@@ -534,16 +583,9 @@ __global__ __launch_bounds__(256) void k_dback(uint32_t W, uint32_t H, uint32_t 
     for (uint32_t bb = nbatch; bb-- > 0;) {
         const uint32_t base = c0 + bb * VKD_B;
         const uint32_t nb = min((uint32_t)VKD_B, c1 - base);
-        if (tid < nb) {
-            Cmd cm = cmds[base + tid];
-            PathTile p = ptl[cm.pt];
-            c_draw[tid] = cm.draw;
-            c_nseg[tid] = p.nseg;
-            c_ntouch[tid] = p.reserved;
-            c_segoff[tid] = p.seg_off;
-        }
-        __syncthreads();
-        batch_coverage<float>(acc, nb, c_nseg, c_ntouch, c_segoff, segs, edges, (float)X0, (float)Y0);
+        stage_batch(base, nb, cmds, ptl, c_draw, c_nseg, c_ntouch, c_segoff, c_sitem);
+        batch_coverage<float>(acc, nb, c_nseg, c_ntouch, c_segoff, c_sitem, s_edge, segs, edges, (float)X0,
+                              (float)Y0);
         __syncthreads();
         // replay the batch's compositing from its checkpoint, then walk it backwards
         float4 st[VKD_B + 1];
