@@ -95,3 +95,37 @@ def test_batched_surfaces_match_oracle_eager_and_replayed(device):
         device.set_graph(False)
         for s in surfs:
             s.close()
+
+
+def test_full_batch_autotuned_and_replayed_sampled_surfaces(device):
+    # the benchmark's batched frame exactly as it is timed: all 256 surfaces resident in one frame,
+    # autotuned, then replayed as captured graphs; every 16th surface compared with its own oracle
+    # render (RGBA8 bit-exact; masks through the signature, which grows by one oracle signature per
+    # execution)
+    import vkvg_amd
+    subs = scenes.config4()
+    picked = list(range(0, len(subs), 16))
+    surfs = [vkvg_amd.Surface(device, s.width, s.height, signature=(i in picked)) for i, s in enumerate(subs)]
+    try:
+        device.set_graph(False)
+        vkvg_amd.upload_batch(device, surfs, [s.words() for s in subs])
+        tune = device.autotune(reps=1)
+        print("case c4 tuned", tune["best"], flush=True)
+        sig0 = {i: surfs[i].signature().astype(np.uint64) for i in picked}
+        device.set_graph(True)
+        for _ in range(3):  # sizing run + capture, then two replays
+            vkvg_amd.execute(device)
+        assert device.stats()["graph"] == 1
+        for i in picked:
+            sc, s = subs[i], surfs[i]
+            print(f"case c4 replayed surface {i} {sc.width}x{sc.height} {sc.n_paths} paths", flush=True)
+            o_rgba, o_sig, _ = render_oracle(sc)
+            assert np.array_equal(s.premultiplied(), o_rgba), i
+            want = (sig0[i] + 3 * o_sig.astype(np.uint64)) & 0xFFFFFFFF
+            assert np.array_equal(s.signature().astype(np.uint64), want), i
+    finally:
+        device.set_graph(False)
+        device.set_tile_size(16)
+        device.set_fine_words(4096)
+        for s in surfs:
+            s.close()

@@ -9,12 +9,18 @@
 //   k_dbbox         one workgroup per path: bbox of its edges (deterministic reduction)
 //   draw tiles, scan, k_dbin (per path, LDS counts / scan / cursors; every tile from the edge's
 //   left end to the path's right bbox end, so a tile's list holds all edges left of it), coarse
-//   k_dfine         one workgroup per 16x16 tile: per batch of 8 commands, one thread per
-//                   (command, pixel row) accumulates the row's area deltas in a fixed order and
-//                   prefix-sums them; then one thread per pixel composites in draw order. With
-//                   checkpoints on, the pixel state before every batch is kept.
+//   k_dcmd_tile     the tile of every command
+//   k_dunit_count, scan, k_dunit_cmd: every command's segment list cut into work units of at
+//                   most 64 segments
+//   k_dcov          16 units per workgroup pass, whatever their commands and tiles: one thread per
+//                   (unit, pixel row) accumulates the row's area deltas in a fixed order; a
+//                   one-unit command is prefix-summed into the coverage buffer (A per command and
+//                   pixel) at once, the units of longer ones are added in unit order by k_dcov_reduce
+//   k_dcomp         one workgroup per 16x16 tile, one thread per pixel: composites the tile's
+//                   commands in draw order from the coverage buffer. With checkpoints on, the pixel
+//                   state before every batch of 8 commands is kept, and so is the coverage buffer.
 // Backward:
-//   k_dback         per tile, batches in reverse: recomputes the batch's coverage, replays the
+//   k_dback         per tile, batches in reverse: reads the batch's coverage, replays the
 //                   batch's compositing from its checkpoint, walks it backwards to get dL/dA per
 //                   (command, pixel) and the colour gradients, then turns dL/dA into edge-endpoint
 //                   gradients with the boundary integral over every edge piece inside a pixel (one
@@ -35,8 +41,9 @@
 #define VKD_NSEG 16 // polyline segments per cubic
 #define VKD_T 16    // tile size in pixels
 #define VKD_B 8     // commands per batch of the backward pass (and per compositing checkpoint)
-#define VKD_FB 16   // commands per coverage pass of the forward: 16 commands x 16 rows = 256 threads
-#define VKD_CHUNK 256 // edges of a batch staged in LDS at a time
+#define VKD_FB 16   // work units per coverage pass of the forward: 16 units x 16 rows = 256 threads
+#define VKD_U 64      // segments per work unit of the coverage pass
+#define VKD_UC 16     // segments of each unit staged in LDS at a time
 #define VKD_BIN_MARGIN 0.01f // px added around an edge when binning (rounding safety)
 
 namespace {
@@ -404,71 +411,6 @@ __device__ __forceinline__ void row_pieces(const RV4<R> &v, R yr, R X0, F f) {
 // ---------------------------------------------------------------------------------- forward
 __device__ __forceinline__ float4 premul(float4 c) { return make_float4(c.x * c.w, c.y * c.w, c.z * c.w, c.w); }
 
-// coverage of batch commands: thread (j, r) owns row r of command j and adds its pieces in the
-// order of the command's segment list (deterministic). The batch's edges are staged in LDS, VKD_CHUNK
-// at a time, by all threads together (segment index then edge: the dependent loads run in parallel
-// across the workgroup), and every row thread of a command reads them from there.
-template <typename R>
-__device__ __forceinline__ void batch_coverage(R (*acc)[VKD_T][VKD_T + 1], uint32_t nb, const uint32_t *c_nseg,
-                                               const uint32_t *c_ntouch, const uint32_t *c_segoff,
-                                               const uint32_t *c_item, RV4<R> *s_edge,
-                                               const uint32_t *__restrict__ segs, const RV4<R> *__restrict__ edges,
-                                               R X0, R Y0) {
-    const uint32_t tid = threadIdx.x;
-    const uint32_t j = tid / VKD_T, r = tid % VKD_T;
-    const bool active = j < nb;
-    R *a = acc[active ? j : 0][r];
-    if (active)
-        for (int c = 0; c <= VKD_T; c++)
-            a[c] = (R)0;
-    const R yr = Y0 + (R)r;
-    const uint32_t total = c_item[nb];
-    for (uint32_t cb = 0; cb < total; cb += VKD_CHUNK) {
-        const uint32_t ce = min(total, cb + (uint32_t)VKD_CHUNK);
-        for (uint32_t i = cb + tid; i < ce; i += 256) {
-            uint32_t lo = 0, hi = nb; // last command with c_item <= i
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (c_item[mid] <= i)
-                    lo = mid;
-                else
-                    hi = mid;
-            }
-            s_edge[i - cb] = edges[segs[c_segoff[lo] + (i - c_item[lo])]];
-        }
-        __syncthreads();
-        if (active) {
-            const uint32_t i0 = max(c_item[j], cb), i1 = min(c_item[j + 1], ce), nt = c_ntouch[j];
-            for (uint32_t i = i0; i < i1; i++) {
-                const RV4<R> v = s_edge[i - cb];
-                if (i - c_item[j] < nt) {
-                    row_pieces<R>(v, yr, X0, [&](int lc, R dy, R fm, R, R) {
-                        if (lc < 0) {
-                            a[0] += dy;
-                        } else {
-                            a[lc] += dy * ((R)1 - fm);
-                            a[lc + 1] += dy * fm;
-                        }
-                    });
-                } else { // edges left of the tile: the row's whole y-extent, as row_pieces gives for them
-                    const R ylo = fmax(fmin(v.y, v.w), yr), yhi = fmin(fmax(v.y, v.w), yr + (R)1);
-                    if (ylo < yhi)
-                        a[0] += v.w > v.y ? yhi - ylo : ylo - yhi;
-                }
-            }
-        }
-        __syncthreads();
-    }
-    (void)c_nseg;
-    if (!active)
-        return;
-    R run = (R)0;
-    for (int c = 0; c < VKD_T; c++) {
-        run += a[c];
-        a[c] = run;
-    }
-}
-
 // stages a batch's command records and the prefix of their segment counts
 __device__ __forceinline__ void stage_batch(uint32_t base, uint32_t nb, const Cmd *__restrict__ cmds,
                                             const PathTile *__restrict__ ptl, uint32_t *c_draw, uint32_t *c_nseg,
@@ -494,44 +436,185 @@ __device__ __forceinline__ void stage_batch(uint32_t base, uint32_t nb, const Cm
     __syncthreads();
 }
 
+// the tile of every command (commands are stored tile by tile)
+__global__ void k_dcmd_tile(uint32_t n_tiles, uint32_t nch, const uint32_t *__restrict__ tile_off,
+                            uint32_t *__restrict__ cmd_tile) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tiles)
+        return;
+    for (uint32_t c = tile_off[(size_t)t * nch]; c < tile_off[(size_t)(t + 1) * nch]; c++)
+        cmd_tile[c] = t;
+}
+
+// Work units of the coverage pass: every command's segment list cut into pieces of at most VKD_U
+// segments (at least one unit per command), so a command with thousands of segments is spread
+// over many row groups instead of holding one. unit_cnt[c] = that count.
+__global__ void k_dunit_count(uint32_t n_cmds, const Cmd *__restrict__ cmds, const PathTile *__restrict__ ptl,
+                              uint32_t *__restrict__ unit_cnt) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cmds)
+        return;
+    const uint32_t n = ptl[cmds[c].pt].nseg;
+    unit_cnt[c] = n ? (n + VKD_U - 1) / VKD_U : 1u;
+}
+
+// the command of every work unit
+__global__ void k_dunit_cmd(uint32_t n_cmds, uint32_t max_units, const uint32_t *__restrict__ unit_off,
+                            uint32_t *__restrict__ unit_cmd) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cmds)
+        return;
+    for (uint32_t u = unit_off[c]; u < min(unit_off[c + 1], max_units); u++)
+        unit_cmd[u] = c;
+}
+
+// Coverage of every command, independent of compositing order: 16 work units per workgroup pass
+// (whatever their commands and tiles), one thread per (unit, pixel row). The units' edges are
+// staged in LDS VKD_UC per unit at a time, so all 16 row groups advance together; each row thread
+// adds the pieces of its unit's segments in list order. A command held by one unit gets its rows
+// prefix-summed into cov[c * 256 + pixel] here; the rows of a command cut into several units go to
+// part[u * 256 + row * 16 + column] and k_dcov_reduce adds them in unit order, so the result does
+// not depend on scheduling. Grid-stride over units: the unit count is read on the device.
 template <typename R>
-__global__ __launch_bounds__(256) void k_dfine(uint32_t W, uint32_t H, uint32_t tilesW, uint32_t nch,
-                                               const uint32_t *__restrict__ tile_off, const Cmd *__restrict__ cmds,
-                                               const PathTile *__restrict__ ptl, const uint32_t *__restrict__ segs,
-                                               const RV4<R> *__restrict__ edges, const float4 *__restrict__ rgba,
-                                               RV4<R> *__restrict__ image, float4 *__restrict__ ckpt,
-                                               const uint32_t *__restrict__ batch_off) {
+__global__ __launch_bounds__(256) void k_dcov(uint32_t n_cmds, uint32_t max_units, uint32_t tilesW,
+                                              const uint32_t *__restrict__ unit_off,
+                                              const uint32_t *__restrict__ unit_cmd,
+                                              const uint32_t *__restrict__ cmd_tile, const Cmd *__restrict__ cmds,
+                                              const PathTile *__restrict__ ptl, const uint32_t *__restrict__ segs,
+                                              const RV4<R> *__restrict__ edges, R *__restrict__ cov,
+                                              R *__restrict__ part) {
     __shared__ R acc[VKD_FB][VKD_T][VKD_T + 1];
-    __shared__ RV4<R> s_edge[VKD_CHUNK];
-    __shared__ uint32_t c_nseg[VKD_FB], c_ntouch[VKD_FB], c_segoff[VKD_FB], c_draw[VKD_FB], c_item[VKD_FB + 1];
-    static_assert(VKD_FB % VKD_B == 0, "checkpoints fall on coverage-pass boundaries or inside");
-    const uint32_t t = blockIdx.x, tid = threadIdx.x;
-    const uint32_t c0 = tile_off[(size_t)t * nch], c1 = tile_off[(size_t)(t + 1) * nch];
-    const int32_t X0 = (int32_t)(t % tilesW) * VKD_T, Y0 = (int32_t)(t / tilesW) * VKD_T;
-    const int32_t px = X0 + (int32_t)(tid % VKD_T), py = Y0 + (int32_t)(tid / VKD_T);
-    const bool valid = px < (int32_t)W && py < (int32_t)H;
-    RV4<R> st{(R)1, (R)1, (R)1, (R)1};
-    for (uint32_t base = c0; base < c1; base += VKD_FB) {
-        const uint32_t nb = min((uint32_t)VKD_FB, c1 - base);
-        stage_batch(base, nb, cmds, ptl, c_draw, c_nseg, c_ntouch, c_segoff, c_item);
-        batch_coverage<R>(acc, nb, c_nseg, c_ntouch, c_segoff, c_item, s_edge, segs, edges, (R)X0, (R)Y0);
+    __shared__ RV4<R> s_edge[VKD_FB][VKD_UC];
+    __shared__ uint32_t u_cmd[VKD_FB], u_s0[VKD_FB], u_n[VKD_FB], u_nt[VKD_FB], u_off[VKD_FB], u_tile[VKD_FB];
+    __shared__ uint32_t u_single[VKD_FB];
+    const uint32_t tid = threadIdx.x, j = tid / VKD_T, r = tid % VKD_T;
+    const uint32_t total = min(unit_off[n_cmds], max_units); // never below unit_off[n_cmds]: the host's bound
+    R *a = acc[j][r];
+    for (uint32_t base = blockIdx.x * VKD_FB; base < total; base += gridDim.x * VKD_FB) {
+        const uint32_t nb = min((uint32_t)VKD_FB, total - base);
+        if (tid < nb) {
+            const uint32_t u = base + tid, c = unit_cmd[u];
+            const PathTile p = ptl[cmds[c].pt];
+            const uint32_t s0 = (u - unit_off[c]) * VKD_U;
+            u_cmd[tid] = c;
+            u_s0[tid] = s0;
+            u_n[tid] = min(p.nseg - min(p.nseg, s0), (uint32_t)VKD_U);
+            u_nt[tid] = p.reserved;
+            u_off[tid] = p.seg_off;
+            u_tile[tid] = cmd_tile[c];
+            u_single[tid] = unit_off[c + 1] - unit_off[c] == 1u;
+        }
+        for (int c = 0; c <= VKD_T; c++)
+            a[c] = (R)0;
         __syncthreads();
-        for (uint32_t j = 0; j < nb; j++) {
-            // the pixel state before every backward batch of VKD_B commands
-            if (ckpt && ((base - c0 + j) % VKD_B) == 0)
-                ckpt[(size_t)(batch_off[t] + (base - c0 + j) / VKD_B) * 256 + tid] =
-                    make_float4((float)st.x, (float)st.y, (float)st.z, (float)st.w);
-            const R A = acc[j][tid / VKD_T][tid % VKD_T];
-            const R k = fmin((R)1, fabs(A));
-            const float4 cf = rgba[c_draw[j]];
-            const R ca = (R)cf.w;
-            const R cx = (R)cf.x * ca, cy = (R)cf.y * ca, cz = (R)cf.z * ca;
-            const R ia = (R)1 - ca * k;
-            st = RV4<R>{cx * k + st.x * ia, cy * k + st.y * ia, cz * k + st.z * ia, ca * k + st.w * ia};
+        const bool active = j < nb;
+        const uint32_t n = active ? u_n[j] : 0u, s0 = active ? u_s0[j] : 0u, nt = active ? u_nt[j] : 0u;
+        const uint32_t t = active ? u_tile[j] : 0u;
+        const R X0 = (R)((t % tilesW) * VKD_T), yr = (R)((t / tilesW) * VKD_T + r);
+        uint32_t nmax = 0;
+        for (uint32_t q = 0; q < nb; q++)
+            nmax = max(nmax, u_n[q]);
+        for (uint32_t k0 = 0; k0 < nmax; k0 += VKD_UC) {
+            if (k0 + r < n) // thread (j, r) stages item k0 + r of unit j
+                s_edge[j][r] = edges[segs[u_off[j] + s0 + k0 + r]];
+            __syncthreads();
+            const uint32_t m = n > k0 ? min((uint32_t)VKD_UC, n - k0) : 0u;
+            for (uint32_t q = 0; q < m; q++) {
+                const RV4<R> v = s_edge[j][q];
+                if (s0 + k0 + q < nt) {
+                    row_pieces<R>(v, yr, X0, [&](int lc, R dy, R fm, R, R) {
+                        if (lc < 0) {
+                            a[0] += dy;
+                        } else {
+                            a[lc] += dy * ((R)1 - fm);
+                            a[lc + 1] += dy * fm;
+                        }
+                    });
+                } else { // edges left of the tile: the row's whole y-extent, as row_pieces gives for them
+                    const R ylo = fmax(fmin(v.y, v.w), yr), yhi = fmin(fmax(v.y, v.w), yr + (R)1);
+                    if (ylo < yhi)
+                        a[0] += v.w > v.y ? yhi - ylo : ylo - yhi;
+                }
+            }
+            __syncthreads();
+        }
+        if (active) {
+            if (u_single[j]) {
+                R *out = cov + ((size_t)u_cmd[j] * VKD_T + r) * VKD_T;
+                R run = (R)0;
+#pragma unroll
+                for (int c = 0; c < VKD_T; c++) {
+                    run += a[c];
+                    out[c] = run;
+                }
+            } else {
+                R *out = part + ((size_t)(base + j) * VKD_T + r) * VKD_T;
+#pragma unroll
+                for (int c = 0; c < VKD_T; c++)
+                    out[c] = a[c];
+            }
         }
         __syncthreads();
     }
-    if (valid)
+}
+
+// rows of the commands cut into several units: partial rows added in unit order, then prefix-summed
+template <typename R>
+__global__ void k_dcov_reduce(uint32_t n_cmds, const uint32_t *__restrict__ unit_off, const R *__restrict__ part,
+                              R *__restrict__ cov) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = g / VKD_T, r = g % VKD_T;
+    if (c >= n_cmds)
+        return;
+    const uint32_t u0 = unit_off[c], u1 = unit_off[c + 1];
+    if (u1 - u0 < 2)
+        return;
+    R sum[VKD_T];
+    const R *p = part + ((size_t)u0 * VKD_T + r) * VKD_T;
+#pragma unroll
+    for (int k = 0; k < VKD_T; k++)
+        sum[k] = p[k];
+    for (uint32_t u = u0 + 1; u < u1; u++) {
+        p = part + ((size_t)u * VKD_T + r) * VKD_T;
+#pragma unroll
+        for (int k = 0; k < VKD_T; k++)
+            sum[k] += p[k];
+    }
+    R *out = cov + ((size_t)c * VKD_T + r) * VKD_T;
+    R run = (R)0;
+#pragma unroll
+    for (int k = 0; k < VKD_T; k++) {
+        run += sum[k];
+        out[k] = run;
+    }
+}
+
+// Compositing in draw order: one workgroup per 16x16 tile, one thread per pixel; coverage comes
+// from k_dcov. With checkpoints on, the pixel state before every batch of VKD_B commands is kept
+// for the backward pass.
+template <typename R>
+__global__ __launch_bounds__(256) void k_dcomp(uint32_t W, uint32_t H, uint32_t tilesW, uint32_t nch,
+                                               const uint32_t *__restrict__ tile_off, const Cmd *__restrict__ cmds,
+                                               const R *__restrict__ cov, const float4 *__restrict__ rgba,
+                                               RV4<R> *__restrict__ image, float4 *__restrict__ ckpt,
+                                               const uint32_t *__restrict__ batch_off) {
+    const uint32_t t = blockIdx.x, tid = threadIdx.x;
+    const uint32_t c0 = tile_off[(size_t)t * nch], c1 = tile_off[(size_t)(t + 1) * nch];
+    const int32_t px = (int32_t)((t % tilesW) * VKD_T + tid % VKD_T), py = (int32_t)((t / tilesW) * VKD_T + tid / VKD_T);
+    RV4<R> st{(R)1, (R)1, (R)1, (R)1};
+    for (uint32_t c = c0; c < c1; c++) {
+        if (ckpt && ((c - c0) % VKD_B) == 0)
+            ckpt[(size_t)(batch_off[t] + (c - c0) / VKD_B) * 256 + tid] =
+                make_float4((float)st.x, (float)st.y, (float)st.z, (float)st.w);
+        const R A = cov[(size_t)c * 256 + tid];
+        const R k = fmin((R)1, fabs(A));
+        const float4 cf = rgba[cmds[c].draw];
+        const R ca = (R)cf.w;
+        const R cx = (R)cf.x * ca, cy = (R)cf.y * ca, cz = (R)cf.z * ca;
+        const R ia = (R)1 - ca * k;
+        st = RV4<R>{cx * k + st.x * ia, cy * k + st.y * ia, cz * k + st.z * ia, ca * k + st.w * ia};
+    }
+    if (px < (int32_t)W && py < (int32_t)H)
         image[(size_t)py * W + px] = st;
 }
 
@@ -563,12 +646,10 @@ __global__ __launch_bounds__(256) void k_dback(uint32_t W, uint32_t H, uint32_t 
                                                const PathTile *__restrict__ ptl, const uint32_t *__restrict__ segs,
                                                const RV4<float> *__restrict__ edges, const float4 *__restrict__ rgba,
                                                const float4 *__restrict__ grad_img, const float4 *__restrict__ ckpt,
-                                               const uint32_t *__restrict__ batch_off, float4 *__restrict__ g_edge,
-                                               float *__restrict__ g_col) {
-    __shared__ float acc[VKD_B][VKD_T][VKD_T + 1];
+                                               const uint32_t *__restrict__ batch_off, const float *__restrict__ cov,
+                                               float4 *__restrict__ g_edge, float *__restrict__ g_col) {
     __shared__ float G[VKD_B][VKD_T][VKD_T];
     __shared__ float s_red[4][32];
-    __shared__ RV4<float> s_edge[VKD_CHUNK];
     __shared__ uint32_t c_item[VKD_B + 1], c_sitem[VKD_B + 1];
     __shared__ uint32_t c_nseg[VKD_B], c_ntouch[VKD_B], c_segoff[VKD_B], c_draw[VKD_B];
     const uint32_t t = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
@@ -584,16 +665,13 @@ __global__ __launch_bounds__(256) void k_dback(uint32_t W, uint32_t H, uint32_t 
         const uint32_t base = c0 + bb * VKD_B;
         const uint32_t nb = min((uint32_t)VKD_B, c1 - base);
         stage_batch(base, nb, cmds, ptl, c_draw, c_nseg, c_ntouch, c_segoff, c_sitem);
-        batch_coverage<float>(acc, nb, c_nseg, c_ntouch, c_segoff, c_sitem, s_edge, segs, edges, (float)X0,
-                              (float)Y0);
-        __syncthreads();
         // replay the batch's compositing from its checkpoint, then walk it backwards
         float4 st[VKD_B + 1];
         st[0] = ckpt[(size_t)(batch_off[t] + bb) * 256 + tid];
         float Av[VKD_B];
 #pragma unroll
         for (uint32_t j = 0; j < VKD_B; j++) {
-            Av[j] = j < nb ? acc[j][tid / VKD_T][tid % VKD_T] : 0.f;
+            Av[j] = j < nb ? cov[(size_t)(base + j) * 256 + tid] : 0.f;
             if (j < nb) {
                 const float k = fminf(1.0f, fabsf(Av[j]));
                 const float4 c = premul(rgba[c_draw[j]]);
@@ -880,7 +958,7 @@ struct vkvg_diff_scene_t {
     Buf pt_path, ctrl_path, first_pt, first_ctrl, first_edge, ncubic, stroke, draws, tg;
     // per call
     Buf pts, nrm, edges, pts64, nrm64, edges64, src, bbox, dtiles, pt_count, pt_off, ptiles, segs, seg_alloc, tile_cnt, tile_off, cmds, nbat,
-        batch_off, ckpt, scan_tmp, host_tot;
+        batch_off, ckpt, cmd_tile, unit_cnt, unit_off, unit_cmd, cov, part, scan_tmp, host_tot;
     Buf g_edge, g_pt, g_n, g_h, g_col;
 };
 
@@ -1030,7 +1108,7 @@ void vkvg_diff_destroy(VkvgDiffScene s) {
     Buf *bufs[] = {&s->pt_path, &s->ctrl_path, &s->first_pt, &s->first_ctrl, &s->first_edge, &s->ncubic, &s->stroke,
                    &s->draws, &s->tg, &s->pts, &s->nrm, &s->edges, &s->pts64, &s->nrm64, &s->edges64, &s->src, &s->bbox, &s->dtiles, &s->pt_count,
                    &s->pt_off, &s->ptiles, &s->segs, &s->seg_alloc, &s->tile_cnt, &s->tile_off, &s->cmds, &s->nbat,
-                   &s->batch_off, &s->ckpt, &s->scan_tmp, &s->host_tot, &s->g_edge, &s->g_pt, &s->g_n, &s->g_h,
+                   &s->batch_off, &s->ckpt, &s->cmd_tile, &s->unit_cnt, &s->unit_off, &s->unit_cmd, &s->cov, &s->part, &s->scan_tmp, &s->host_tot, &s->g_edge, &s->g_pt, &s->g_n, &s->g_h,
                    &s->g_col};
     for (Buf *b : bufs)
         if (b->p)
@@ -1122,9 +1200,31 @@ static int forward_impl(VkvgDiffScene s, const float *ctrl, const float *rgba, c
             return -1;
         ck_ptr = P<float4>(s->ckpt);
     }
-    hipLaunchKernelGGL(k_dfine<R>, dim3(s->n_tiles), dim3(256), 0, st, s->W, s->H, s->tilesW, s->nch,
-                       P<uint32_t>(s->tile_off), P<Cmd>(s->cmds), P<PathTile>(s->ptiles), P<uint32_t>(s->segs),
-                       P<RV4<R>>(edges), (const float4 *)rgba, image, ck_ptr, P<uint32_t>(s->batch_off));
+    // work units: at most n_cmds + n_segs / VKD_U of them (one per command plus one per full VKD_U)
+    const size_t max_units = (size_t)s->n_cmds + s->n_segs / VKD_U + 1;
+    if (!ensure(s, s->cmd_tile, 4ull * ((size_t)s->n_cmds + 1)) || !ensure(s, s->unit_cnt, 4ull * ((size_t)s->n_cmds + 1)) ||
+        !ensure(s, s->unit_off, 4ull * ((size_t)s->n_cmds + 1)) || !ensure(s, s->unit_cmd, 4ull * max_units) ||
+        !ensure(s, s->cov, sizeof(R) * 256 * ((size_t)s->n_cmds + 1)) || !ensure(s, s->part, sizeof(R) * 256 * max_units))
+        return -1;
+    hipLaunchKernelGGL(k_dcmd_tile, dim3(blocks(s->n_tiles)), dim3(256), 0, st, s->n_tiles, s->nch,
+                       P<uint32_t>(s->tile_off), P<uint32_t>(s->cmd_tile));
+    if (s->n_cmds) {
+        hipLaunchKernelGGL(k_dunit_count, dim3(blocks(s->n_cmds)), dim3(256), 0, st, s->n_cmds, P<Cmd>(s->cmds),
+                           P<PathTile>(s->ptiles), P<uint32_t>(s->unit_cnt));
+        if (!scan(s, P<uint32_t>(s->unit_cnt), s->n_cmds, P<uint32_t>(s->unit_off)))
+            return -1;
+        hipLaunchKernelGGL(k_dunit_cmd, dim3(blocks(s->n_cmds)), dim3(256), 0, st, s->n_cmds, (uint32_t)max_units,
+                           P<uint32_t>(s->unit_off), P<uint32_t>(s->unit_cmd));
+        const unsigned grid = (unsigned)std::min<size_t>((max_units + VKD_FB - 1) / VKD_FB, 8192);
+        hipLaunchKernelGGL(k_dcov<R>, dim3(grid), dim3(256), 0, st, s->n_cmds, (uint32_t)max_units, s->tilesW, P<uint32_t>(s->unit_off),
+                           P<uint32_t>(s->unit_cmd), P<uint32_t>(s->cmd_tile), P<Cmd>(s->cmds), P<PathTile>(s->ptiles),
+                           P<uint32_t>(s->segs), P<RV4<R>>(edges), P<R>(s->cov), P<R>(s->part));
+        hipLaunchKernelGGL(k_dcov_reduce<R>, dim3(blocks((size_t)s->n_cmds * VKD_T)), dim3(256), 0, st, s->n_cmds,
+                           P<uint32_t>(s->unit_off), P<R>(s->part), P<R>(s->cov));
+    }
+    hipLaunchKernelGGL(k_dcomp<R>, dim3(s->n_tiles), dim3(256), 0, st, s->W, s->H, s->tilesW, s->nch,
+                       P<uint32_t>(s->tile_off), P<Cmd>(s->cmds), P<R>(s->cov), (const float4 *)rgba, image, ck_ptr,
+                       P<uint32_t>(s->batch_off));
     if (!ck(s, hipGetLastError(), "forward launch") || !ck(s, hipEventRecord(s->ev[1], st), "event") ||
         !ck(s, hipStreamSynchronize(st), "forward"))
         return -1;
@@ -1171,7 +1271,7 @@ int vkvg_diff_backward(VkvgDiffScene s, const float *ctrl, const float *rgba, co
                        P<uint32_t>(s->tile_off),
                        P<Cmd>(s->cmds), P<PathTile>(s->ptiles), P<uint32_t>(s->segs), P<RV4<float>>(s->edges),
                        (const float4 *)rgba, (const float4 *)grad_image, P<float4>(s->ckpt),
-                       P<uint32_t>(s->batch_off), P<float4>(s->g_edge), P<float>(s->g_col));
+                       P<uint32_t>(s->batch_off), P<float>(s->cov), P<float4>(s->g_edge), P<float>(s->g_col));
     hipLaunchKernelGGL(k_dg_normal_sums, dim3(blocks(s->n_pts)), dim3(256), 0, st, s->n_pts,
                        P<uint32_t>(s->pt_path), P<uint32_t>(s->first_pt), P<uint32_t>(s->first_edge),
                        P<uint32_t>(s->stroke), width, P<DSrc>(s->src), P<float4>(s->g_edge), P<float2>(s->nrm),
