@@ -135,7 +135,7 @@ struct vkh_engine {
     DevBuf blob, units, tgts, item_cnt, item_off, pts, pflags, sp_first, sp_count;
     DevBuf fe, nv, nt, fe_off, nv_off, nt_off, verts, tris, edges, edraw;
     DevBuf bbox, erange, dtiles, pt_count, pt_off, ptiles, counters, segs, dprep; // counters: [0] segment allocator, [1] flatten slots, [2] overflow cubics, [64..191] flatten order
-    DevBuf tile_cnt, tile_off, cmds, scan_tmp, readback, item_slot, slot_item, flat_scratch, flat_perm, flat_ovf;
+    DevBuf tile_cnt, tile_off, tile_order, draw_order, cmds, scan_tmp, readback, item_slot, slot_item, flat_scratch, flat_perm, flat_ovf;
     DevBuf step_nv, step_nt, sv_off, st_off, u_toff, u_vbeg, u_vend, dash_st; // dash_st: per dashed piece, its starting dash state
     HostBuf hblob, htot;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
@@ -279,7 +279,7 @@ void vkvg_hip_engine_destroy(vkh_engine *e) {
                       &e->sp_first, &e->sp_count, &e->fe, &e->nv, &e->nt, &e->fe_off, &e->nv_off, &e->nt_off,
                       &e->verts, &e->tris, &e->edges, &e->edraw, &e->bbox, &e->dtiles, &e->pt_count, &e->pt_off,
                       &e->erange, &e->ptiles, &e->counters, &e->segs, &e->dprep, &e->tile_cnt,
-                      &e->tile_off, &e->cmds, &e->scan_tmp, &e->readback, &e->step_nv, &e->step_nt,
+                      &e->tile_off, &e->tile_order, &e->draw_order, &e->cmds, &e->scan_tmp, &e->readback, &e->step_nv, &e->step_nt,
                       &e->sv_off, &e->st_off, &e->dash_st, &e->u_toff, &e->u_vbeg, &e->u_vend, &e->item_slot,
                       &e->slot_item, &e->flat_scratch, &e->flat_perm, &e->flat_ovf};
     for (DevBuf *b : bufs)
@@ -784,12 +784,15 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     // ---- binning: per-draw segment lists and backdrops; bins again once if the segment buffer
     // was too small (the allocator then holds the exact total)
     uint32_t n_segs = 0;
+    if (!need(eng, cap, eng->draw_order, 4ull * vkh::order_words(n_draws)))
+        return -1;
+    vkh::launch_draw_order(s, n_draws, P<uint2>(eng->erange), P<uint32_t>(eng->draw_order));
     for (int pass = 0; pass < 2; pass++) {
         const uint32_t seg_cap = (uint32_t)std::min<size_t>(eng->segs.cap / sizeof(Edge), 0xFFFFFFFFull);
         vkh::launch_zero_u32(s, P<uint32_t>(eng->counters), 1);
         vkh::launch_bin_draws(s, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), P<uint2>(eng->erange),
                               P<DrawTiles>(eng->dtiles), n_draws, T, P<PathTile>(eng->ptiles), P<Edge>(eng->segs),
-                              seg_cap, P<uint32_t>(eng->counters));
+                              seg_cap, P<uint32_t>(eng->counters), P<uint32_t>(eng->draw_order));
         CK(hipGetLastError());
         if (totals(eng, cap, {P<uint32_t>(eng->counters)}, tot, 7))
             return -1;
@@ -831,6 +834,9 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
         return -1;
     vkh::launch_coarse(s, true, bin_total, nch, eng->coarse_chunk, d_tg, n_targets, d_draws, P<DrawTiles>(eng->dtiles),
                        P<PathTile>(eng->ptiles), nullptr, P<uint32_t>(eng->tile_off), P<Cmd>(eng->cmds));
+    if (!need(eng, cap, eng->tile_order, 4ull * vkh::order_words(tile_total)))
+        return -1;
+    vkh::launch_tile_order(s, tile_total, nch, P<uint32_t>(eng->tile_off), P<uint32_t>(eng->tile_order));
     CK(hipGetLastError());
 
     // ---- fine: coverage + compositing
@@ -841,7 +847,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     vkh::launch_fine(s, eng->frame_ext, T, eng->fine_words, tile_total, d_tg, n_targets, P<DrawPrep>(eng->dprep), d_grec, d_grads, d_gprep,
                      P<PathTile>(eng->ptiles),
                      P<Edge>(eng->segs), nch, P<uint32_t>(eng->tile_off),
-                     P<Cmd>(eng->cmds));
+                     P<Cmd>(eng->cmds), P<uint32_t>(eng->tile_order));
     CK(hipGetLastError());
     if (cap)
         return 0;

@@ -164,7 +164,7 @@ void launch_set_pt_base(hipStream_t s, DrawTiles *dt, const uint32_t *pt_off, ui
 void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
 void launch_bin_draws(hipStream_t s, const Edge *edges, const uint32_t *edraw, const uint2 *erange,
                       const DrawTiles *dt, uint32_t n_draws, uint32_t tile, PathTile *pts, Edge *segs,
-                      uint32_t seg_cap, uint32_t *seg_alloc);
+                      uint32_t seg_cap, uint32_t *seg_alloc, const uint32_t *order);
 void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n_draws, const TargetInfo *tg,
                       const vkh_gradient *grads, uint32_t n_grads, const vkh_surface_source *srcs, DrawPrep *out,
                       GradRec *rec, GradPrep *gout);
@@ -177,7 +177,12 @@ void launch_coarse(hipStream_t s, bool write, uint32_t n_bins, uint32_t nch, uin
 void launch_fine(hipStream_t s, bool ext, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,
                  uint32_t n_targets, const DrawPrep *dprep, const GradRec *grec, const vkh_gradient *grads,
                  const GradPrep *gprep, const PathTile *pts, const Edge *segs, uint32_t nch,
-                 const uint32_t *tile_off, const Cmd *cmds);
+                 const uint32_t *tile_off, const Cmd *cmds, const uint32_t *order);
+// heaviest first: tiles by command count (fine kernel), draws by edge count (binning kernel); work
+// holds order_words(n) words, the order in its first n
+size_t order_words(uint32_t n);
+void launch_tile_order(hipStream_t s, uint32_t n_tiles, uint32_t nch, const uint32_t *tile_off, uint32_t *work);
+void launch_draw_order(hipStream_t s, uint32_t n_draws, const uint2 *erange, uint32_t *work);
 void launch_unpremultiply(hipStream_t s, const uint8_t *src, uint8_t *dst, size_t npix);
 void launch_stencil_clear(hipStream_t s, uint32_t *words, size_t npix);
 void launch_check_u8_conversion(hipStream_t s, unsigned long long *bad);

@@ -185,12 +185,13 @@ __global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges
                                                   const uint2 *__restrict__ erange,
                                                   const DrawTiles *__restrict__ dts, uint32_t T,
                                                   PathTile *__restrict__ pts, Edge *__restrict__ segs,
-                                                  uint32_t seg_cap, uint32_t *__restrict__ seg_alloc) {
+                                                  uint32_t seg_cap, uint32_t *__restrict__ seg_alloc,
+                                                  const uint32_t *__restrict__ draw_order) {
     __shared__ uint32_t s_cnt[VKH_BIN_LDS_TILES];
     __shared__ int32_t s_back[VKH_BIN_LDS_TILES];
     __shared__ uint32_t s_wave[4];
     __shared__ uint32_t s_base;
-    const uint32_t d = blockIdx.x, tid = threadIdx.x;
+    const uint32_t d = draw_order ? draw_order[blockIdx.x] : blockIdx.x, tid = threadIdx.x;
     const DrawTiles dt = dts[d];
     if (dt.bw == 0 || dt.bx0 > dt.bx1 || dt.by0 > dt.by1)
         return; // paint / clear, or no edges
@@ -401,6 +402,84 @@ __global__ __launch_bounds__(256) void k_coarse(bool write, const TargetInfo *__
     }
     if (!write && valid)
         tile_count[(size_t)gtile * nch + chunk] = count;
+}
+
+// ----------------------------------------------------------------------------------- work order
+// Kernels that give one workgroup to each item of very uneven cost (the fine kernel's tiles, the
+// binning kernel's draws) take the items heaviest first (longest-processing-time order): an item
+// with many times the mean cost that started late would hold its CU long after the others
+// finished. Weights are bucketed exactly below 128 and in 16 steps per octave above (to 2^15),
+// buckets reversed so that bucket 0 is the heaviest; items in one bucket keep no particular order
+// (each item writes its own outputs, so no result depends on it).
+#define VKH_ORDER_BUCKETS 256
+__device__ __forceinline__ uint32_t order_bucket(uint32_t w) {
+    uint32_t b = w;
+    if (w >= 128u) {
+        const uint32_t e = 31u - (uint32_t)__clz((int)w); // >= 7
+        b = min(128u + (e - 7u) * 16u + ((w >> (e - 4u)) & 15u), 255u);
+    }
+    return (VKH_ORDER_BUCKETS - 1) - b;
+}
+// commands of a tile
+struct TileWeight {
+    uint32_t nch;
+    const uint32_t *tile_off;
+    __device__ uint32_t operator()(uint32_t t) const {
+        return tile_off[(size_t)(t + 1) * nch] - tile_off[(size_t)t * nch];
+    }
+};
+// edges of a draw
+struct DrawWeight {
+    const uint2 *erange;
+    __device__ uint32_t operator()(uint32_t d) const {
+        const uint2 e = erange[d];
+        return e.y > e.x ? e.y - e.x : 0u;
+    }
+};
+
+// pass 0: bucket sizes (LDS histogram per workgroup, one global add per non-empty bucket);
+// pass 1: each workgroup reserves its range in every bucket and writes its items there
+template <bool SCATTER, class W>
+__global__ __launch_bounds__(256) void k_work_order(uint32_t n, W weight, uint32_t *__restrict__ bucket,
+                                                    uint32_t *__restrict__ order) {
+    __shared__ uint32_t h[VKH_ORDER_BUCKETS];
+    const uint32_t tid = threadIdx.x, i = blockIdx.x * 256u + tid;
+    h[tid] = 0u;
+    __syncthreads();
+    const uint32_t b = i < n ? order_bucket(weight(i)) : 0u;
+    const uint32_t rank = i < n ? atomicAdd(&h[b], 1u) : 0u;
+    __syncthreads();
+    if (!SCATTER) {
+        if (h[tid])
+            atomicAdd(&bucket[tid], h[tid]);
+        return;
+    }
+    if (h[tid])
+        h[tid] = atomicAdd(&bucket[tid], h[tid]); // the workgroup's first slot in bucket tid
+    __syncthreads();
+    if (i < n)
+        order[h[b] + rank] = i;
+}
+
+// exclusive scan of the bucket sizes into bucket starts (one workgroup)
+__global__ __launch_bounds__(256) void k_work_order_scan(uint32_t *__restrict__ bucket) {
+    __shared__ uint32_t s_wave[4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t v = bucket[tid];
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if ((int)lane >= o)
+            x += y;
+    }
+    if (lane == 63)
+        s_wave[w] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t k = 0; k < w; k++)
+        before += s_wave[k];
+    bucket[tid] = before + x - v;
 }
 
 // ----------------------------------------------------------------------------------- fine
@@ -698,7 +777,8 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
                                               const GradPrep *__restrict__ gprep,
                                               const PathTile *__restrict__ pts, const Edge *__restrict__ segs,
                                               uint32_t nch,
-                                              const uint32_t *__restrict__ tile_off, const Cmd *__restrict__ cmds) {
+                                              const uint32_t *__restrict__ tile_off, const Cmd *__restrict__ cmds,
+                                              const uint32_t *__restrict__ tile_order) {
     constexpr int NPIX = T * T;
     constexpr int NT = NPIX >= 256 ? 256 : NPIX;  // threads
     constexpr int PPT = NPIX / NT;                // pixels per thread
@@ -709,7 +789,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
     __shared__ uint32_t c_nseg[B], c_segoff[B], c_meta[B], c_solid[B], c_key[B], c_draw[B];
     __shared__ int32_t c_back[B];
 
-    uint32_t gtile = blockIdx.x;
+    const uint32_t gtile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
     uint32_t c0 = tile_off[(size_t)gtile * nch], c1 = tile_off[(size_t)(gtile + 1) * nch];
     if (!EXT && c0 == c1)
         return;
@@ -874,6 +954,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
             }
             if (!c_nseg[j])
                 continue;
+            acc += c_back[j];
 #pragma unroll
             for (int c = 0; c < T; c++) {
                 acc += sd[j][r][c];
@@ -894,7 +975,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
                 if (kind_is_clip(kind)) {
 #pragma unroll
                     for (int q = 0; q < PPT; q++) {
-                        const int32_t w = bk + (hasseg ? sd[j][ly[q]][lx[q]] : 0);
+                        const int32_t w = hasseg ? sd[j][ly[q]][lx[q]] : bk;
                         const bool cov = kind == VKH_DRAW_CLIP_NZ ? w != 0 : (w & 1) != 0;
                         if (cov && !(st[q] & 2u)) {
                             if (valid[q])
@@ -915,7 +996,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
             for (int q = 0; q < PPT; q++) {
                 int k = 1;
                 if (!full) {
-                    const int32_t w = bk + (hasseg ? sd[j][ly[q]][lx[q]] : 0);
+                    const int32_t w = hasseg ? sd[j][ly[q]][lx[q]] : bk;
                     k = coverage_count(kind, w);
                 }
                 if (k <= 0 || !valid[q])
@@ -1028,10 +1109,10 @@ void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n) {
 }
 void launch_bin_draws(hipStream_t s, const Edge *edges, const uint32_t *edraw, const uint2 *erange,
                       const DrawTiles *dt, uint32_t n_draws, uint32_t tile, PathTile *pts, Edge *segs,
-                      uint32_t seg_cap, uint32_t *seg_alloc) {
+                      uint32_t seg_cap, uint32_t *seg_alloc, const uint32_t *order) {
     if (n_draws)
         hipLaunchKernelGGL(k_bin_draw, dim3(n_draws), dim3(256), 0, s, edges, edraw, erange, dt, tile, pts, segs,
-                           seg_cap, seg_alloc);
+                           seg_cap, seg_alloc, order);
 }
 uint32_t coarse_chunk(uint32_t most_draws, uint32_t n_bins) {
     const uint32_t want = (1024 + std::max(n_bins, 1u) - 1) / std::max(n_bins, 1u); // chunks per bin
@@ -1058,49 +1139,67 @@ template <int T, int W>
 static void fine_launch(hipStream_t s, bool ext, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
                         const DrawPrep *dprep, const GradRec *grec, const vkh_gradient *grads, const GradPrep *gprep,
                         const PathTile *pts, const Edge *segs, uint32_t nch, const uint32_t *tile_off,
-                        const Cmd *cmds) {
+                        const Cmd *cmds, const uint32_t *order) {
     constexpr int NT = T * T >= 256 ? 256 : T * T;
     if (ext)
         hipLaunchKernelGGL((k_fine<T, W, true>), dim3(n_tiles), dim3(NT), 0, s, tg, n_targets, dprep, grec, grads, gprep,
-                           pts, segs, nch, tile_off, cmds);
+                           pts, segs, nch, tile_off, cmds, order);
     else
         hipLaunchKernelGGL((k_fine<T, W, false>), dim3(n_tiles), dim3(NT), 0, s, tg, n_targets, dprep, grec, grads,
-                           gprep, pts, segs, nch, tile_off, cmds);
+                           gprep, pts, segs, nch, tile_off, cmds, order);
 }
 template <int T>
 static void fine_words(hipStream_t s, bool ext, uint32_t words, uint32_t n_tiles, const TargetInfo *tg, uint32_t n_targets,
                        const DrawPrep *dprep, const GradRec *grec, const vkh_gradient *grads, const GradPrep *gprep,
                        const PathTile *pts, const Edge *segs, uint32_t nch, const uint32_t *tile_off,
-                       const Cmd *cmds) {
+                       const Cmd *cmds, const uint32_t *order) {
     switch (words) {
     case 2048:
-        fine_launch<T, 2048>(s, ext, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_launch<T, 2048>(s, ext, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds, order);
         break;
     case 8192:
-        fine_launch<T, 8192>(s, ext, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_launch<T, 8192>(s, ext, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds, order);
         break;
     default:
-        fine_launch<T, 4096>(s, ext, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_launch<T, 4096>(s, ext, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds, order);
         break;
     }
 }
 void launch_fine(hipStream_t s, bool ext, uint32_t tile, uint32_t words, uint32_t n_tiles, const TargetInfo *tg,
                  uint32_t n_targets, const DrawPrep *dprep, const GradRec *grec, const vkh_gradient *grads,
                  const GradPrep *gprep, const PathTile *pts, const Edge *segs, uint32_t nch,
-                 const uint32_t *tile_off, const Cmd *cmds) {
+                 const uint32_t *tile_off, const Cmd *cmds, const uint32_t *order) {
     if (!n_tiles)
         return;
     switch (tile) {
     case 8:
-        fine_words<8>(s, ext, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_words<8>(s, ext, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds, order);
         break;
     case 32:
-        fine_words<32>(s, ext, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_words<32>(s, ext, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds, order);
         break;
     default:
-        fine_words<16>(s, ext, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds);
+        fine_words<16>(s, ext, words, n_tiles, tg, n_targets, dprep, grec, grads, gprep, pts, segs, nch, tile_off, cmds, order);
         break;
     }
+}
+size_t order_words(uint32_t n) { return (size_t)n + VKH_ORDER_BUCKETS; }
+template <class W>
+static void launch_order(hipStream_t s, uint32_t n, W weight, uint32_t *work) {
+    if (!n)
+        return;
+    uint32_t *bucket = work + n, *order = work;
+    const unsigned g = (n + 255u) / 256u;
+    launch_zero_u32(s, bucket, VKH_ORDER_BUCKETS);
+    hipLaunchKernelGGL((k_work_order<false, W>), dim3(g), dim3(256), 0, s, n, weight, bucket, order);
+    hipLaunchKernelGGL(k_work_order_scan, dim3(1), dim3(256), 0, s, bucket);
+    hipLaunchKernelGGL((k_work_order<true, W>), dim3(g), dim3(256), 0, s, n, weight, bucket, order);
+}
+void launch_tile_order(hipStream_t s, uint32_t n_tiles, uint32_t nch, const uint32_t *tile_off, uint32_t *work) {
+    launch_order(s, n_tiles, TileWeight{nch, tile_off}, work);
+}
+void launch_draw_order(hipStream_t s, uint32_t n_draws, const uint2 *erange, uint32_t *work) {
+    launch_order(s, n_draws, DrawWeight{erange}, work);
 }
 void launch_check_u8_conversion(hipStream_t s, unsigned long long *bad) {
     hipLaunchKernelGGL(k_check_u8_conversion, dim3(1u << 16), dim3(256), 0, s, bad);
