@@ -16,6 +16,7 @@
 //                   (unit, pixel row) accumulates the row's area deltas in a fixed order; a
 //                   one-unit command is prefix-summed into the coverage buffer (A per command and
 //                   pixel) at once, the units of longer ones are added in unit order by k_dcov_reduce
+//   k_work_order    the tiles heaviest first (commands per tile), for k_dcomp and k_dback
 //   k_dcomp         one workgroup per 16x16 tile, one thread per pixel: composites the tile's
 //                   commands in draw order from the coverage buffer. With checkpoints on, the pixel
 //                   state before every batch of 8 commands is kept, and so is the coverage buffer.
@@ -597,8 +598,9 @@ __global__ __launch_bounds__(256) void k_dcomp(uint32_t W, uint32_t H, uint32_t 
                                                const uint32_t *__restrict__ tile_off, const Cmd *__restrict__ cmds,
                                                const R *__restrict__ cov, const float4 *__restrict__ rgba,
                                                RV4<R> *__restrict__ image, float4 *__restrict__ ckpt,
-                                               const uint32_t *__restrict__ batch_off) {
-    const uint32_t t = blockIdx.x, tid = threadIdx.x;
+                                               const uint32_t *__restrict__ batch_off,
+                                               const uint32_t *__restrict__ tile_order) {
+    const uint32_t t = tile_order[blockIdx.x], tid = threadIdx.x;
     const uint32_t c0 = tile_off[(size_t)t * nch], c1 = tile_off[(size_t)(t + 1) * nch];
     const int32_t px = (int32_t)((t % tilesW) * VKD_T + tid % VKD_T), py = (int32_t)((t / tilesW) * VKD_T + tid / VKD_T);
     RV4<R> st{(R)1, (R)1, (R)1, (R)1};
@@ -647,12 +649,13 @@ __global__ __launch_bounds__(256) void k_dback(uint32_t W, uint32_t H, uint32_t 
                                                const RV4<float> *__restrict__ edges, const float4 *__restrict__ rgba,
                                                const float4 *__restrict__ grad_img, const float4 *__restrict__ ckpt,
                                                const uint32_t *__restrict__ batch_off, const float *__restrict__ cov,
-                                               float4 *__restrict__ g_edge, float *__restrict__ g_col) {
+                                               float4 *__restrict__ g_edge, float *__restrict__ g_col,
+                                               const uint32_t *__restrict__ tile_order) {
     __shared__ float G[VKD_B][VKD_T][VKD_T];
     __shared__ float s_red[4][32];
     __shared__ uint32_t c_item[VKD_B + 1], c_sitem[VKD_B + 1];
     __shared__ uint32_t c_nseg[VKD_B], c_ntouch[VKD_B], c_segoff[VKD_B], c_draw[VKD_B];
-    const uint32_t t = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const uint32_t t = tile_order[blockIdx.x], tid = threadIdx.x, lane = tid & 63;
     const uint32_t c0 = tile_off[(size_t)t * nch], c1 = tile_off[(size_t)(t + 1) * nch];
     if (c0 == c1)
         return;
@@ -958,7 +961,7 @@ struct vkvg_diff_scene_t {
     Buf pt_path, ctrl_path, first_pt, first_ctrl, first_edge, ncubic, stroke, draws, tg;
     // per call
     Buf pts, nrm, edges, pts64, nrm64, edges64, src, bbox, dtiles, pt_count, pt_off, ptiles, segs, seg_alloc, tile_cnt, tile_off, cmds, nbat,
-        batch_off, ckpt, cmd_tile, unit_cnt, unit_off, unit_cmd, cov, part, scan_tmp, host_tot;
+        batch_off, ckpt, cmd_tile, unit_cnt, unit_off, unit_cmd, cov, part, tile_order, scan_tmp, host_tot;
     Buf g_edge, g_pt, g_n, g_h, g_col;
 };
 
@@ -1108,7 +1111,7 @@ void vkvg_diff_destroy(VkvgDiffScene s) {
     Buf *bufs[] = {&s->pt_path, &s->ctrl_path, &s->first_pt, &s->first_ctrl, &s->first_edge, &s->ncubic, &s->stroke,
                    &s->draws, &s->tg, &s->pts, &s->nrm, &s->edges, &s->pts64, &s->nrm64, &s->edges64, &s->src, &s->bbox, &s->dtiles, &s->pt_count,
                    &s->pt_off, &s->ptiles, &s->segs, &s->seg_alloc, &s->tile_cnt, &s->tile_off, &s->cmds, &s->nbat,
-                   &s->batch_off, &s->ckpt, &s->cmd_tile, &s->unit_cnt, &s->unit_off, &s->unit_cmd, &s->cov, &s->part, &s->scan_tmp, &s->host_tot, &s->g_edge, &s->g_pt, &s->g_n, &s->g_h,
+                   &s->batch_off, &s->ckpt, &s->cmd_tile, &s->unit_cnt, &s->unit_off, &s->unit_cmd, &s->cov, &s->part, &s->tile_order, &s->scan_tmp, &s->host_tot, &s->g_edge, &s->g_pt, &s->g_n, &s->g_h,
                    &s->g_col};
     for (Buf *b : bufs)
         if (b->p)
@@ -1222,9 +1225,13 @@ static int forward_impl(VkvgDiffScene s, const float *ctrl, const float *rgba, c
         hipLaunchKernelGGL(k_dcov_reduce<R>, dim3(blocks((size_t)s->n_cmds * VKD_T)), dim3(256), 0, st, s->n_cmds,
                            P<uint32_t>(s->unit_off), P<R>(s->part), P<R>(s->cov));
     }
+    // tiles heaviest first (commands per tile vary several-fold), for compositing and the backward
+    if (!ensure(s, s->tile_order, 4ull * vkh::order_words(s->n_tiles)))
+        return -1;
+    vkh::launch_tile_order(st, s->n_tiles, s->nch, P<uint32_t>(s->tile_off), P<uint32_t>(s->tile_order));
     hipLaunchKernelGGL(k_dcomp<R>, dim3(s->n_tiles), dim3(256), 0, st, s->W, s->H, s->tilesW, s->nch,
                        P<uint32_t>(s->tile_off), P<Cmd>(s->cmds), P<R>(s->cov), (const float4 *)rgba, image, ck_ptr,
-                       P<uint32_t>(s->batch_off));
+                       P<uint32_t>(s->batch_off), P<uint32_t>(s->tile_order));
     if (!ck(s, hipGetLastError(), "forward launch") || !ck(s, hipEventRecord(s->ev[1], st), "event") ||
         !ck(s, hipStreamSynchronize(st), "forward"))
         return -1;
@@ -1271,7 +1278,8 @@ int vkvg_diff_backward(VkvgDiffScene s, const float *ctrl, const float *rgba, co
                        P<uint32_t>(s->tile_off),
                        P<Cmd>(s->cmds), P<PathTile>(s->ptiles), P<uint32_t>(s->segs), P<RV4<float>>(s->edges),
                        (const float4 *)rgba, (const float4 *)grad_image, P<float4>(s->ckpt),
-                       P<uint32_t>(s->batch_off), P<float>(s->cov), P<float4>(s->g_edge), P<float>(s->g_col));
+                       P<uint32_t>(s->batch_off), P<float>(s->cov), P<float4>(s->g_edge), P<float>(s->g_col),
+                       P<uint32_t>(s->tile_order));
     hipLaunchKernelGGL(k_dg_normal_sums, dim3(blocks(s->n_pts)), dim3(256), 0, st, s->n_pts,
                        P<uint32_t>(s->pt_path), P<uint32_t>(s->first_pt), P<uint32_t>(s->first_edge),
                        P<uint32_t>(s->stroke), width, P<DSrc>(s->src), P<float4>(s->g_edge), P<float2>(s->nrm),
