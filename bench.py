@@ -341,7 +341,10 @@ def main() -> int:
     issue = None if valu is None else {"achieved": valu / (fine_ms / 1e3), "peak": issue_peak,
                                        "unit": "wave64 VALU instructions/s",
                                        "frac": valu / (fine_ms / 1e3) / issue_peak,
-                                       "valu_per_launch": valu}
+                                       "valu_per_launch": valu,
+                                       "model": "4 cycles per counted wave64 VALU instruction on a 16-lane "
+                                                "SIMD; packed 16-bit instructions issue faster, so frac "
+                                                "near or above 1 means the kernel is at the issue limit"}
     # LDS from the same counter summary: SQ_LDS_IDX_ACTIVE over (GRBM_GUI_ACTIVE per XCD x 256 CUs)
     # is the fraction of CU-cycles the LDS served indexed accesses (peak 1); LDS atomics per second
     # against one wave64 LDS instruction per CU per cycle
