@@ -19,7 +19,8 @@ static __device__ __align__(256) uint8_t scratch[VKH_NOALLOC_SCRATCH_BYTES];
 
 // ----------------------------------------------------------------------------------------- scan
 // Exclusive prefix sum over n uint32 values into out[0..n] (out[n] = total): block scans of
-// 1024 values (256 threads x 4), block totals scanned recursively, then added back.
+// 1024 values (256 threads x 4), then the block totals added back (summed per block when there
+// are at most 1024 of them, scanned recursively otherwise).
 __global__ __launch_bounds__(256) void k_scan_block(const uint32_t *__restrict__ in, uint32_t n,
                                                     uint32_t *__restrict__ out, uint32_t *__restrict__ sums) {
     __shared__ uint32_t wsum[4];
@@ -53,6 +54,32 @@ __global__ __launch_bounds__(256) void k_scan_block(const uint32_t *__restrict__
     if (threadIdx.x == 255 && sums)
         sums[blockIdx.x] = woff + incl;
 }
+// block b adds the total of blocks 0 .. b-1 (at most 1024 block sums, reduced by the workgroup)
+__global__ __launch_bounds__(256) void k_scan_add_sums(uint32_t *__restrict__ out, uint32_t n,
+                                                       const uint32_t *__restrict__ sums) {
+    __shared__ uint32_t wsum[4];
+    const uint32_t b = blockIdx.x, tid = threadIdx.x;
+    uint32_t x = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t j = tid * 4u + (uint32_t)k;
+        x += j < b ? sums[j] : 0u;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        x += __shfl_xor(x, o);
+    if ((tid & 63) == 0)
+        wsum[tid >> 6] = x;
+    __syncthreads();
+    const uint32_t o = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    const uint32_t i = b * 1024u + tid;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t j = i + 256u * k;
+        if (j <= n)
+            out[j] += o;
+    }
+}
 __global__ void k_scan_add(uint32_t *__restrict__ out, uint32_t n, const uint32_t *__restrict__ offs) {
     uint32_t i = blockIdx.x * 1024u + threadIdx.x;
     uint32_t o = offs[blockIdx.x];
@@ -67,44 +94,31 @@ __global__ void k_scan_add(uint32_t *__restrict__ out, uint32_t n, const uint32_
 namespace vkh {
 // temp words needed by scan_u32 for n values
 size_t scan_temp_words(uint32_t n) {
-    size_t need = 0;
-    for (uint32_t m = n;;) {
-        uint32_t nb = (m + 1 + 1023) / 1024;
-        if (nb <= 1)
-            break;
-        need += 2ull * nb + 1; // sums + their scan
-        m = nb;
-    }
-    return need + 1;
+    const uint32_t nb = (n + 1 + 1023) / 1024;
+    if (nb <= 1)
+        return 1;
+    if (nb <= 1024)
+        return nb + 1;
+    return 2ull * nb + 1 + scan_temp_words(nb);
 }
 // exclusive scan of n values; out must hold n+1 entries (out[n] = total); tmp holds
-// scan_temp_words(n) words
+// scan_temp_words(n) words. Up to 1024 blocks of 1024 values: block scans, then each block adds
+// the sum of the blocks before it (two launches); more: the block sums are scanned recursively.
 void scan_u32(hipStream_t s, const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *tmp) {
-    struct Level {
-        uint32_t n;
-        uint32_t *out;
-        uint32_t nb;
-    };
-    Level lv[8];
-    int nl = 0;
-    const uint32_t *cin = in;
-    uint32_t cn = n;
-    uint32_t *cout = out;
-    while (true) {
-        uint32_t nb = (cn + 1 + 1023) / 1024;
-        uint32_t *sums = nb > 1 ? tmp : nullptr;
-        hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, s, cin, cn, cout, sums);
-        lv[nl++] = Level{cn, cout, nb};
-        if (nb <= 1)
-            break;
-        uint32_t *sscan = tmp + nb;
-        tmp += 2ull * nb + 1;
-        cin = sums;
-        cn = nb;
-        cout = sscan;
+    const uint32_t nb = (n + 1 + 1023) / 1024;
+    if (nb <= 1) {
+        hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(256), 0, s, in, n, out, nullptr);
+        return;
     }
-    for (int k = nl; k-- > 1;)
-        hipLaunchKernelGGL(k_scan_add, dim3(lv[k - 1].nb), dim3(256), 0, s, lv[k - 1].out, lv[k - 1].n, lv[k].out);
+    uint32_t *sums = tmp;
+    hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, s, in, n, out, sums);
+    if (nb <= 1024) {
+        hipLaunchKernelGGL(k_scan_add_sums, dim3(nb), dim3(256), 0, s, out, n, sums);
+        return;
+    }
+    uint32_t *sscan = tmp + nb; // nb + 1 entries
+    scan_u32(s, sums, nb, sscan, tmp + 2ull * nb + 1);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, s, out, n, sscan);
 }
 } // namespace vkh
 
