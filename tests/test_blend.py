@@ -24,8 +24,8 @@ int main(void) {
                     uint32_t s = sc | (((sc * 5 + 1) & 255) << 8) | (((sc ^ 0x5A) & 255) << 16) | (sa << 24);
                     bad += blend_px(op, d, s) != blend_px_channels(op, d, s);
                     bad += blend_px(op, s, d) != blend_px_channels(op, s, d);
-                    bad += blend_px16(op, d, vkh_src16_of(s)) != blend_px_channels(op, d, s);
-                    bad += blend_px16(op, s, vkh_src16_of(d)) != blend_px_channels(op, s, d);
+                    bad += blend_px16(d, vkh_src16_of(op, s)) != blend_px_channels(op, d, s);
+                    bad += blend_px16(s, vkh_src16_of(op, d)) != blend_px_channels(op, s, d);
                 }
     printf("%llu\n", bad);
     return 0;

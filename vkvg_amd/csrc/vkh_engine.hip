@@ -1052,21 +1052,28 @@ int vkvg_hip_render(vkh_engine *eng, const vkh_frame *f) {
     return vkvg_hip_execute(eng);
 }
 
-// Diagnostic, not part of the public headers: mismatches of the hardware UNORM8 conversion used
-// by the fine kernel against the reference rounding, over every float (0 expected); -1 on error.
-__attribute__((visibility("default"))) long long vkh_check_u8_conversion(void) {
+// Diagnostics, not part of the public headers (0 expected, -1 on error): mismatches of the hardware
+// UNORM8 conversion used by the fine kernel against the reference rounding, over every float; and
+// of the device form of the packed blend against the per-channel blend, over every input.
+static long long run_check(void (*launch)(hipStream_t, unsigned long long *)) {
     unsigned long long *d = nullptr, h = 0;
     if (hipMalloc(&d, sizeof(*d)) != hipSuccess)
         return -1;
     long long r = -1;
     if (hipMemset(d, 0, sizeof(*d)) == hipSuccess) {
-        vkh::launch_check_u8_conversion(nullptr, d);
+        launch(nullptr, d);
         if (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
             hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost) == hipSuccess)
             r = (long long)h;
     }
     (void)hipFree(d);
     return r;
+}
+__attribute__((visibility("default"))) long long vkh_check_u8_conversion(void) {
+    return run_check(vkh::launch_check_u8_conversion);
+}
+__attribute__((visibility("default"))) long long vkh_check_blend(void) {
+    return run_check(vkh::launch_check_blend);
 }
 
 } // extern "C"

@@ -186,4 +186,5 @@ void launch_draw_order(hipStream_t s, uint32_t n_draws, const uint2 *erange, uin
 void launch_unpremultiply(hipStream_t s, const uint8_t *src, uint8_t *dst, size_t npix);
 void launch_stencil_clear(hipStream_t s, uint32_t *words, size_t npix);
 void launch_check_u8_conversion(hipStream_t s, unsigned long long *bad);
+void launch_check_blend(hipStream_t s, unsigned long long *bad);
 } // namespace vkh

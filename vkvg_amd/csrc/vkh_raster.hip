@@ -312,7 +312,7 @@ __device__ __forceinline__ uint32_t find_target_by(const TargetInfo *tg, uint32_
 // Per 256 draws: the draws whose tile bbox meets the bin are compacted into LDS (in draw order,
 // with their bbox and kind), then every tile walks that list, loading the path tiles of four
 // draws at a time so their latencies overlap.
-__global__ __launch_bounds__(256) void k_coarse(bool write, const TargetInfo *__restrict__ tg, uint32_t n_targets,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_coarse(bool write, const TargetInfo *__restrict__ tg, uint32_t n_targets,
                                                 const vkh_draw *__restrict__ draws, const DrawTiles *__restrict__ dts,
                                                 const PathTile *__restrict__ pts, uint32_t nch, uint32_t chunk_draws,
                                                 uint32_t *__restrict__ tile_count, const uint32_t *__restrict__ tile_off,
@@ -756,6 +756,19 @@ __device__ __forceinline__ uint32_t stencil_apply(uint32_t kind, uint32_t prm, u
 }
 
 
+// Blends one source into a covered pixel: once, and for strokes once more per further covering
+// triangle (w = the winding count, >= 1 here); the signature adds the draw's key per blend.
+__device__ __forceinline__ void blend_count(const vkh_src16 &S, bool stroke, int32_t w, uint32_t key, uint32_t &v,
+                                            uint32_t &sg) {
+    v = blend_px16(v, S);
+    sg += key;
+    if (stroke)
+        for (int32_t n = 1; n < w; n++) {
+            v = blend_px16(v, S);
+            sg += key;
+        }
+}
+
 // One workgroup per T x T tile. Commands are processed in batches of B: the batch's command
 // records (draw record + path tile) are staged in LDS with one parallel load; every segment of the
 // batch (one thread each) adds its edge's crossings of the tile's sample rows into per-command
@@ -770,7 +783,7 @@ __device__ __forceinline__ uint32_t stencil_apply(uint32_t kind, uint32_t prm, u
 // zeroes the stencil byte, and the clip and stencil draws rewrite it (src/vkvg_context.c:698-795,
 // :1251-1512; pipeline states of src/vkvg_device_internal.c:226-246).
 template <int T, int WORDS, bool EXT>
-__global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg, uint32_t n_targets,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fine(const TargetInfo *__restrict__ tg, uint32_t n_targets,
                                               const DrawPrep *__restrict__ dprep,
                                               const GradRec *__restrict__ grec,
                                               const vkh_gradient *__restrict__ grads,
@@ -786,8 +799,8 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
     __shared__ int32_t sd[B][T][T];               // row deltas, then per-row inclusive prefix
     __shared__ int32_t srow[B][T];                // V term: +-1 at the first row below a column crossing
     __shared__ uint32_t c_item[B + 1];            // prefix of the commands' segment counts
-    __shared__ uint32_t c_nseg[B], c_segoff[B], c_meta[B], c_solid[B], c_key[B], c_draw[B];
-    __shared__ int32_t c_back[B];
+    __shared__ uint32_t c_nseg[B], c_segoff[B], c_draw[B];
+    __shared__ uint4 c_rec[B];                    // meta, solid, key, backdrop: one 16-byte read per command
 
     const uint32_t gtile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
     uint32_t c0 = tile_off[(size_t)gtile * nch], c1 = tile_off[(size_t)(gtile + 1) * nch];
@@ -868,20 +881,20 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
             if (cm.pt == VKH_NO_DRAW) {
                 if (kind <= VKH_DRAW_CLEAR)
                     meta = (meta & ~15u) | (kind == VKH_DRAW_CLEAR ? VKH_DRAW_CLEAR : VKH_DRAW_PAINT);
-                c_back[j] = 0;
+                c_rec[j].w = 0u;
                 c_nseg[j] = 0;
                 c_segoff[j] = 0;
             } else {
                 const PathTile p = pts[cm.pt];
-                c_back[j] = p.backdrop;
+                c_rec[j].w = (uint32_t)p.backdrop;
                 c_nseg[j] = p.nseg;
                 c_segoff[j] = p.seg_off;
                 if (p.nseg)
                     meta |= VKH_DP_HASSEG;
             }
-            c_meta[j] = meta;
-            c_solid[j] = d.solid;
-            c_key[j] = d.key;
+            c_rec[j].x = meta;
+            c_rec[j].y = d.solid;
+            c_rec[j].z = d.key;
             c_draw[j] = cm.draw;
         }
         for (uint32_t k = tid; k < nb * (uint32_t)NPIX; k += NT)
@@ -954,7 +967,7 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
             }
             if (!c_nseg[j])
                 continue;
-            acc += c_back[j];
+            acc += (int32_t)c_rec[j].w;
 #pragma unroll
             for (int c = 0; c < T; c++) {
                 acc += sd[j][r][c];
@@ -962,15 +975,16 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
             }
         }
         __syncthreads();
-        // ---- ordered compositing
+        // ---- ordered compositing. A command's coverage count k is 0 or 1 for fills (no loop) and
+        // the winding number for strokes (a pixel covered by k triangles is blended k times); solid
+        // sources keep their blend operands in scalar registers, computed once per command.
         for (uint32_t j = 0; j < nb; j++) {
-            const uint32_t meta = uniform_u32(c_meta[j]);
+            const uint4 rec = c_rec[j];
+            const uint32_t meta = uniform_u32(rec.x);
             const uint32_t kind = meta & 15, op = (meta >> 4) & 15;
-            const bool full = kind == VKH_DRAW_PAINT || kind == VKH_DRAW_CLEAR;
             const bool hasseg = (meta & VKH_DP_HASSEG) != 0;
-            const int32_t bk = (int32_t)uniform_u32((uint32_t)c_back[j]);
-            const uint32_t key = uniform_u32(c_key[j]);
-            const uint32_t solid = uniform_u32(c_solid[j]);
+            const int32_t bk = (int32_t)rec.w;
+            const uint32_t key = rec.z;
             if (EXT && kind >= VKH_DRAW_CLIP_NZ) { // stencil draws (uniform branch)
                 if (kind_is_clip(kind)) {
 #pragma unroll
@@ -985,30 +999,43 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
                         }
                     }
                 } else {
+                    const uint32_t prm = uniform_u32(rec.y);
 #pragma unroll
                     for (int q = 0; q < PPT; q++)
-                        st[q] = stencil_apply(kind, solid, st[q]);
+                        st[q] = stencil_apply(kind, prm, st[q]);
                 }
                 continue;
             }
-            const vkh_src16 S0 = vkh_src16_of(solid);
+            const bool stroke = kind == VKH_DRAW_STROKE;
+            int32_t wq[PPT];
+            bool on[PPT];
 #pragma unroll
             for (int q = 0; q < PPT; q++) {
-                int k = 1;
-                if (!full) {
-                    const int32_t w = hasseg ? sd[j][ly[q]][lx[q]] : bk;
-                    k = coverage_count(kind, w);
-                }
-                if (k <= 0 || !valid[q])
-                    continue;
-                if (EXT) {
+                const int32_t w = hasseg ? sd[j][ly[q]][lx[q]] : bk;
+                wq[q] = w;
+                on[q] = kind == VKH_DRAW_FILL_NZ ? w != 0
+                        : kind == VKH_DRAW_FILL_EO ? (w & 1) != 0
+                        : stroke ? w > 0 : true; // paint, clear
+                on[q] = on[q] && valid[q];
+                if (EXT && on[q]) {
                     if (kind == VKH_DRAW_CLEAR)
                         st[q] &= ~0xFFu; // vkCmdClearAttachments ignores the stencil test
                     else if (st[q] & 2u)
-                        continue; // clipped
+                        on[q] = false; // clipped
                 }
-                vkh_src16 S = S0;
-                if (meta & (EXT ? (VKH_DP_GRAD | VKH_DP_SURF) : VKH_DP_GRAD)) {
+            }
+            // blend operands: once per command for a solid source, per pixel for a gradient or surface
+            vkh_src16 S[PPT];
+            if (!(meta & (EXT ? (VKH_DP_GRAD | VKH_DP_SURF) : VKH_DP_GRAD))) {
+                const vkh_src16 S0 = vkh_src16_of(op, uniform_u32(rec.y));
+#pragma unroll
+                for (int q = 0; q < PPT; q++)
+                    S[q] = S0;
+            } else {
+#pragma unroll
+                for (int q = 0; q < PPT; q++) {
+                    if (!on[q])
+                        continue;
                     float col[4], op_;
                     if (!EXT || (meta & VKH_DP_GRAD)) {
                         const GradRec &P = grec[uniform_u32(c_draw[j])]; // scalar loads
@@ -1019,17 +1046,13 @@ __global__ __launch_bounds__(256) void k_fine(const TargetInfo *__restrict__ tg,
                         op_ = R.opacity;
                         surface_color(R, (float)(X0 + lx[q]) + 0.5f, (float)(Y0 + ly[q]) + 0.5f, col);
                     }
-                    S = vkh_src16_of(f2u8x4(col[0] * op_, col[1] * op_, col[2] * op_, col[3] * op_));
+                    S[q] = vkh_src16_of(op, f2u8x4(col[0] * op_, col[1] * op_, col[2] * op_, col[3] * op_));
                 }
-                // a stroke pixel covered by k triangles is blended k times
-                uint32_t v = px[q], sg = sig[q];
-                do {
-                    v = blend_px16(op, v, S);
-                    sg += key;
-                } while (--k > 0);
-                px[q] = v;
-                sig[q] = sg;
             }
+#pragma unroll
+            for (int q = 0; q < PPT; q++)
+                if (on[q])
+                    blend_count(S[q], stroke, wq[q], key, px[q], sig[q]);
         }
         __syncthreads();
     }
@@ -1056,6 +1079,23 @@ __global__ void k_check_u8_conversion(unsigned long long *__restrict__ bad) {
         const uint32_t want = f2u8(x);
         const uint32_t got = f2u8x4(x, x, x, x);
         nb += (got != want * 0x01010101u) ? 1u : 0u;
+    }
+    if (nb)
+        atomicAdd(bad, nb);
+}
+
+// Diagnostic: the device form of blend_px16 (byte permutes) against the per-channel blend over every
+// operator x source alpha x destination channel x source channel, with the other three channels of
+// both pixels derived from the varied ones (2^18 threads x 256 source channels); counts mismatches.
+__global__ void k_check_blend(unsigned long long *__restrict__ bad) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; // op << 16 | sa << 8 | dc
+    const uint32_t op = t >> 16, sa = (t >> 8) & 255u, dc = t & 255u;
+    unsigned long long nb = 0;
+    const uint32_t d = dc | (((dc * 7u + 3u) & 255u) << 8) | ((255u - dc) << 16) | (((dc * 13u) & 255u) << 24);
+    for (uint32_t sc = 0; sc < 256; sc++) {
+        const uint32_t s = sc | (((sc * 5u + 1u) & 255u) << 8) | (((sc ^ 0x5Au) & 255u) << 16) | (sa << 24);
+        nb += blend_px16(d, vkh_src16_of(op, s)) != blend_px_channels(op, d, s) ? 1u : 0u;
+        nb += blend_px16(s, vkh_src16_of(op, d)) != blend_px_channels(op, s, d) ? 1u : 0u;
     }
     if (nb)
         atomicAdd(bad, nb);
@@ -1203,6 +1243,9 @@ void launch_draw_order(hipStream_t s, uint32_t n_draws, const uint2 *erange, uin
 }
 void launch_check_u8_conversion(hipStream_t s, unsigned long long *bad) {
     hipLaunchKernelGGL(k_check_u8_conversion, dim3(1u << 16), dim3(256), 0, s, bad);
+}
+void launch_check_blend(hipStream_t s, unsigned long long *bad) {
+    hipLaunchKernelGGL(k_check_blend, dim3(1u << 10), dim3(256), 0, s, bad); // 2^18 = 4 ops x 256 x 256
 }
 void launch_stencil_clear(hipStream_t s, uint32_t *words, size_t npix) {
     if (npix)
