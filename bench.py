@@ -197,7 +197,9 @@ def bench_c4(args, dev, vkvg_amd, scenes):
     reps = max(3, args.steps // 4)
     for _ in range(max(1, args.warmup)):
         vkvg_amd.execute(dev)
-    eager, _ = _timed(dev, vkvg_amd, reps)
+    eager, runs = _timed(dev, vkvg_amd, reps)
+    stage = {k: float(np.mean([r["ms_stage"][k] for r in runs])) for k in runs[0]["ms_stage"]}
+    st = runs[-1]
     batched = eager
     if not args.no_graph:
         dev.set_graph(True)
@@ -211,7 +213,8 @@ def bench_c4(args, dev, vkvg_amd, scenes):
     return {"plain_ms": plain * 1e3, "batched_ms": batched, "batched_eager_ms": eager,
             "ratio": plain * 1e3 / batched, "surfaces": len(subs), "paths": int(sum(s.n_paths for s in subs)),
             "pixels": int(sum(s.width * s.height for s in subs)),
-            "tuned": tune["best"] if tune else None}
+            "tuned": tune["best"] if tune else None, "stages_ms": stage,
+            "counts": {k: st[k] for k in ("points", "stroke_triangles", "edges", "path_tiles", "segments", "commands")}}
 
 
 def bench_fwd_bwd(args):

@@ -362,7 +362,10 @@ __host__ __device__ void agg_flatten(const Cubic &root, float tol, Emit emit) {
 // The same traversal with the ancestors of the current node on a stack (levels 0 .. D-1 kept by the
 // caller, e.g. in LDS): every move costs one split. The parent of a right sibling at `level` is
 // read from the stack, or, beyond level D-1, recomputed from the level D-1 ancestor along the path
-// bits; all nodes come from the same chain of splits as in the recursion.
+// bits; all nodes come from the same chain of splits as in the recursion. Every iteration tests one
+// node and makes exactly one split (the left child of an inner node, or the right sibling of the
+// first left child above a leaf), so lanes of a wavefront that take different moves still share it;
+// the right children above a leaf are popped in one count of the path's leading ones.
 template <int D, typename Emit, typename Store, typename Load>
 __host__ __device__ void agg_flatten_stack(const Cubic &root, float tol, Emit emit, Store store, Load load) {
     uint64_t path_lo = 0, path_hi = 0; // bit (l-1): the node at level l is a right child
@@ -385,50 +388,59 @@ __host__ __device__ void agg_flatten_stack(const Cubic &root, float tol, Emit em
             if (n > 1)
                 emit(pts[1]);
         }
-        if (!leaf) {
+        Cubic src;
+        bool right;
+        if (!leaf) { // descend to the left child
             if (level < D)
                 store(level, node);
-            Cubic ch;
-            agg_split(node, false, ch);
-            node = ch;
+            src = node;
+            right = false;
             level++;
-            continue;
-        }
-        // pop right children, then step to the right sibling of the first left child
-        while (level >= 1) {
-            const int b = level - 1;
-            const bool is_right = b < 64 ? ((path_lo >> b) & 1ull) : ((path_hi >> (b - 64)) & 1ull);
-            if (!is_right)
-                break;
-            if (b < 64)
-                path_lo &= ~(1ull << b);
-            else
-                path_hi &= ~(1ull << (b - 64));
-            level--;
-        }
-        if (level < 1)
-            return;
-        {
-            const int b = level - 1;
-            if (b < 64)
-                path_lo |= (1ull << b);
-            else
-                path_hi |= (1ull << (b - 64));
-        }
-        Cubic par;
-        if (level - 1 < D) {
-            par = load(level - 1);
         } else {
-            par = load(D - 1);
-            for (int l = D; l <= level - 1; l++) {
-                const int b = l - 1;
-                const bool r = b < 64 ? ((path_lo >> b) & 1ull) : ((path_hi >> (b - 64)) & 1ull);
-                Cubic nx;
-                agg_split(par, r, nx);
-                par = nx;
+            // pop the right children above the leaf: the run of ones from bit level-1 downwards
+            if (level <= 64) {
+                const uint64_t up = path_lo << (64 - level); // bit level-1 at the top
+                const int ones = up == ~0ull ? level : __builtin_clzll(~up);
+                level -= ones;
+                path_lo &= level >= 64 ? ~0ull : ((1ull << level) - 1ull);
+            } else {
+                while (level >= 1) {
+                    const int b = level - 1;
+                    const bool is_right = b < 64 ? ((path_lo >> b) & 1ull) : ((path_hi >> (b - 64)) & 1ull);
+                    if (!is_right)
+                        break;
+                    if (b < 64)
+                        path_lo &= ~(1ull << b);
+                    else
+                        path_hi &= ~(1ull << (b - 64));
+                    level--;
+                }
             }
+            if (level < 1)
+                return;
+            // the first left child becomes its right sibling
+            {
+                const int b = level - 1;
+                if (b < 64)
+                    path_lo |= (1ull << b);
+                else
+                    path_hi |= (1ull << (b - 64));
+            }
+            if (level - 1 < D) {
+                src = load(level - 1);
+            } else {
+                src = load(D - 1);
+                for (int l = D; l <= level - 1; l++) {
+                    const int b = l - 1;
+                    const bool r = b < 64 ? ((path_lo >> b) & 1ull) : ((path_hi >> (b - 64)) & 1ull);
+                    Cubic nx;
+                    agg_split(src, r, nx);
+                    src = nx;
+                }
+            }
+            right = true;
         }
-        agg_split(par, true, node);
+        agg_split(src, right, node);
     }
 }
 

@@ -209,6 +209,30 @@ __global__ void k_flatten_overflow(const vkh_item *__restrict__ items, const uin
     }
 }
 
+// Diagnostic, not part of the public headers: the subdivision points of one cubic item (without its
+// end point), computed on the host by the flattening kernel's stack traversal (stack != 0, the
+// ancestors in a local array as the kernel keeps them in LDS) or by the stackless traversal of the
+// overflow kernel; returns the number of points, of which the first cap are written to xy.
+extern "C" uint32_t vkvg_hip_flatten_points(const vkh_item *it, int stack, float *xy, uint32_t cap) {
+    const Cubic cb{it->p[0], it->p[1], it->p[2], it->p[3], it->p[4], it->p[5], it->p[6], it->p[7]};
+    uint32_t k = 0;
+    auto emit = [&](v2 p) {
+        if (k < cap) {
+            xy[2 * k] = p.x;
+            xy[2 * k + 1] = p.y;
+        }
+        k++;
+    };
+    if (stack) {
+        Cubic stk[VKH_FLAT_STACK];
+        agg_flatten_stack<VKH_FLAT_STACK>(
+            cb, it->tol, emit, [&](int l, const Cubic &c) { stk[l] = c; }, [&](int l) { return stk[l]; });
+    } else {
+        agg_flatten(cb, it->tol, emit);
+    }
+    return k;
+}
+
 extern "C" void vkvg_hip_item_extents(const vkh_item *it, float ext[4]) {
     auto widen = [&](v2 p) {
         if (p.x < ext[0])
