@@ -30,6 +30,25 @@ __device__ __forceinline__ uint32_t flat_key(const vkh_item &it) {
     return 62u - (uint32_t)fminf(fmaxf(b, 0.f), 62.f);  // heavy cubics first
 }
 
+// rank of an item within its workgroup's bucket: items of the line bucket (every item of a frame
+// without curves) take their rank from one atomic per wavefront; cubics one atomic each
+__device__ __forceinline__ uint32_t flat_rank(uint32_t *h, bool valid, uint32_t key) {
+    const bool line = valid && key == 63;
+    const uint64_t m = __ballot(line);
+    uint32_t rank = 0;
+    if (m) {
+        const uint32_t lane = threadIdx.x & 63, leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == leader)
+            base = atomicAdd(&h[63], (uint32_t)__popcll(m));
+        base = __shfl(base, (int)leader);
+        rank = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    }
+    if (valid && !line)
+        rank = atomicAdd(&h[key], 1u);
+    return rank;
+}
+
 __global__ __launch_bounds__(256) void k_flat_hist(const vkh_item *__restrict__ items, uint32_t n,
                                                    uint32_t *__restrict__ hist) {
     __shared__ uint32_t h[64];
@@ -37,8 +56,7 @@ __global__ __launch_bounds__(256) void k_flat_hist(const vkh_item *__restrict__ 
         h[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        atomicAdd(&h[flat_key(items[i])], 1u);
+    (void)flat_rank(h, i < n, i < n ? flat_key(items[i]) : 0u);
     __syncthreads();
     if (threadIdx.x < 64 && h[threadIdx.x])
         atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
@@ -63,11 +81,8 @@ __global__ __launch_bounds__(256) void k_flat_scatter(const vkh_item *__restrict
         h[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t key = 0, rank = 0;
-    if (i < n) {
-        key = flat_key(items[i]);
-        rank = atomicAdd(&h[key], 1u);
-    }
+    const uint32_t key = i < n ? flat_key(items[i]) : 0u;
+    const uint32_t rank = flat_rank(h, i < n, key);
     __syncthreads();
     if (threadIdx.x < 64 && h[threadIdx.x])
         base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], h[threadIdx.x]);

@@ -870,6 +870,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
         px[q] = (need_read && valid[q]) ? img[o] : 0u;
     }
 
+    // the row deltas start at zero; afterwards every batch leaves them at zero (the row prefix
+    // clears srow as it reads it, the compositing clears each pixel's word of sd after its last read)
+    for (uint32_t k = tid; k < (uint32_t)(B * NPIX); k += NT)
+        (&sd[0][0][0])[k] = 0;
+    for (uint32_t k = tid; k < (uint32_t)(B * T); k += NT)
+        (&srow[0][0])[k] = 0;
     for (uint32_t base = c0; base < c1; base += B) {
         const uint32_t nb = min((uint32_t)B, c1 - base);
         // ---- stage the batch's records
@@ -897,10 +903,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
             c_rec[j].z = d.key;
             c_draw[j] = cm.draw;
         }
-        for (uint32_t k = tid; k < nb * (uint32_t)NPIX; k += NT)
-            (&sd[0][0][0])[k] = 0;
-        for (uint32_t k = tid; k < nb * (uint32_t)T; k += NT)
-            (&srow[0][0])[k] = 0;
         __syncthreads();
         if (tid == 0) {
             uint32_t acc = 0;
@@ -959,6 +961,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
         for (uint32_t q = tid; q < nb * (uint32_t)T; q += NT) {
             const uint32_t j = q / T, r = q % T;
             int32_t acc = srow[j][r];
+            srow[j][r] = 0;
 #pragma unroll
             for (int o = 1; o < T; o <<= 1) {
                 const int32_t y = __shfl_up(acc, (unsigned)o, T);
@@ -990,6 +993,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
 #pragma unroll
                     for (int q = 0; q < PPT; q++) {
                         const int32_t w = hasseg ? sd[j][ly[q]][lx[q]] : bk;
+                        if (hasseg)
+                            sd[j][ly[q]][lx[q]] = 0;
                         const bool cov = kind == VKH_DRAW_CLIP_NZ ? w != 0 : (w & 1) != 0;
                         if (cov && !(st[q] & 2u)) {
                             if (valid[q])
@@ -1012,6 +1017,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
 #pragma unroll
             for (int q = 0; q < PPT; q++) {
                 const int32_t w = hasseg ? sd[j][ly[q]][lx[q]] : bk;
+                if (hasseg)
+                    sd[j][ly[q]][lx[q]] = 0;
                 wq[q] = w;
                 on[q] = kind == VKH_DRAW_FILL_NZ ? w != 0
                         : kind == VKH_DRAW_FILL_EO ? (w & 1) != 0
