@@ -149,10 +149,13 @@ __device__ __forceinline__ void bin_edge(const Edge &e, const DrawTiles &dt, int
     }
 }
 
-// exclusive scan in place of a[0 .. n), n <= 16 * 256, by a 256-thread workgroup; returns the total
+// exclusive scan in place of a[0 .. n), n <= VKH_BIN_LDS_TILES, by a 256-thread workgroup (each
+// thread sums a run of VKH_BIN_LDS_TILES / 256 entries); returns the total
+static_assert(VKH_BIN_LDS_TILES % 256 == 0, "binning table: whole runs per thread");
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t *a, uint32_t n, uint32_t *s_wave) {
+    constexpr uint32_t RUN = VKH_BIN_LDS_TILES / 256;
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t beg = min(tid * 16u, n), end = min(beg + 16u, n);
+    const uint32_t beg = min(tid * RUN, n), end = min(beg + RUN, n);
     uint32_t sum = 0;
     for (uint32_t k = beg; k < end; k++)
         sum += a[k];
