@@ -803,7 +803,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
     __shared__ int32_t srow[B][T];                // V term: +-1 at the first row below a column crossing
     __shared__ uint32_t c_item[B + 1];            // prefix of the commands' segment counts
     __shared__ uint32_t c_nseg[B], c_segoff[B], c_draw[B];
-    __shared__ uint4 c_rec[B];                    // meta, solid, key, backdrop: one 16-byte read per command
+    __shared__ int32_t c_back[B];                 // path tile backdrop (the row prefix adds it)
+    __shared__ uint4 c_rec[B];  // meta, key, winding added in compositing (the backdrop without segments), solid
+    __shared__ uint4 c_src[B];  // blend operands of a solid source (vkh_src16)
 
     const uint32_t gtile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
     uint32_t c0 = tile_off[(size_t)gtile * nch], c1 = tile_off[(size_t)(gtile + 1) * nch];
@@ -890,20 +892,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
             if (cm.pt == VKH_NO_DRAW) {
                 if (kind <= VKH_DRAW_CLEAR)
                     meta = (meta & ~15u) | (kind == VKH_DRAW_CLEAR ? VKH_DRAW_CLEAR : VKH_DRAW_PAINT);
-                c_rec[j].w = 0u;
+                c_back[j] = 0;
+                c_rec[j].z = 0u;
                 c_nseg[j] = 0;
                 c_segoff[j] = 0;
             } else {
                 const PathTile p = pts[cm.pt];
-                c_rec[j].w = (uint32_t)p.backdrop;
+                c_back[j] = p.backdrop;
+                c_rec[j].z = p.nseg ? 0u : (uint32_t)p.backdrop;
                 c_nseg[j] = p.nseg;
                 c_segoff[j] = p.seg_off;
                 if (p.nseg)
                     meta |= VKH_DP_HASSEG;
             }
             c_rec[j].x = meta;
-            c_rec[j].y = d.solid;
-            c_rec[j].z = d.key;
+            c_rec[j].y = d.key;
+            c_rec[j].w = d.solid;
+            const vkh_src16 S = vkh_src16_of((meta >> 4) & 15, d.solid);
+            c_src[j] = make_uint4(vkh_u32_of(S.ia), vkh_u32_of(S.b_lo), vkh_u32_of(S.b_hi), S.x);
             c_draw[j] = cm.draw;
         }
         __syncthreads();
@@ -973,7 +979,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
             }
             if (!c_nseg[j])
                 continue;
-            acc += (int32_t)c_rec[j].w;
+            acc += c_back[j];
 #pragma unroll
             for (int c = 0; c < T; c++) {
                 acc += sd[j][r][c];
@@ -982,22 +988,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
         }
         __syncthreads();
         // ---- ordered compositing. A command's coverage count k is 0 or 1 for fills (no loop) and
-        // the winding number for strokes (a pixel covered by k triangles is blended k times); solid
-        // sources keep their blend operands in scalar registers, computed once per command.
+        // the winding number for strokes (a pixel covered by k triangles is blended k times); the
+        // blend operands of a solid source were computed once per command when the batch was staged.
         for (uint32_t j = 0; j < nb; j++) {
             const uint4 rec = c_rec[j];
             const uint32_t meta = uniform_u32(rec.x);
             const uint32_t kind = meta & 15, op = (meta >> 4) & 15;
-            const bool hasseg = (meta & VKH_DP_HASSEG) != 0;
-            const int32_t bk = (int32_t)rec.w;
-            const uint32_t key = rec.z;
+            const int32_t bk = (int32_t)rec.z; // 0 when the winding comes from the row deltas
+            const uint32_t key = rec.y;
             if (EXT && kind >= VKH_DRAW_CLIP_NZ) { // stencil draws (uniform branch)
                 if (kind_is_clip(kind)) {
 #pragma unroll
                     for (int q = 0; q < PPT; q++) {
-                        const int32_t w = hasseg ? sd[j][ly[q]][lx[q]] : bk;
-                        if (hasseg)
-                            sd[j][ly[q]][lx[q]] = 0;
+                        // the row deltas of a command without segments are zero
+                        const int32_t w = sd[j][ly[q]][lx[q]] + bk;
+                        sd[j][ly[q]][lx[q]] = 0;
                         const bool cov = kind == VKH_DRAW_CLIP_NZ ? w != 0 : (w & 1) != 0;
                         if (cov && !(st[q] & 2u)) {
                             if (valid[q])
@@ -1007,7 +1012,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
                         }
                     }
                 } else {
-                    const uint32_t prm = uniform_u32(rec.y);
+                    const uint32_t prm = uniform_u32(rec.w);
 #pragma unroll
                     for (int q = 0; q < PPT; q++)
                         st[q] = stencil_apply(kind, prm, st[q]);
@@ -1019,9 +1024,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
             bool on[PPT];
 #pragma unroll
             for (int q = 0; q < PPT; q++) {
-                const int32_t w = hasseg ? sd[j][ly[q]][lx[q]] : bk;
-                if (hasseg)
-                    sd[j][ly[q]][lx[q]] = 0;
+                // the row deltas of a command without segments are zero
+                const int32_t w = sd[j][ly[q]][lx[q]] + bk;
+                sd[j][ly[q]][lx[q]] = 0;
                 wq[q] = w;
                 on[q] = kind == VKH_DRAW_FILL_NZ ? w != 0
                         : kind == VKH_DRAW_FILL_EO ? (w & 1) != 0
@@ -1037,7 +1042,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
             // blend operands: once per command for a solid source, per pixel for a gradient or surface
             vkh_src16 S[PPT];
             if (!(meta & (EXT ? (VKH_DP_GRAD | VKH_DP_SURF) : VKH_DP_GRAD))) {
-                const vkh_src16 S0 = vkh_src16_of(op, uniform_u32(rec.y));
+                const uint4 sv = c_src[j];
+                const vkh_src16 S0 = {vkh_u16x2_of(sv.x), vkh_u16x2_of(sv.y), vkh_u16x2_of(sv.z), sv.w};
 #pragma unroll
                 for (int q = 0; q < PPT; q++)
                     S[q] = S0;
