@@ -148,8 +148,9 @@ struct vkh_engine {
 
     DevBuf blob, units, tgts, item_cnt, item_off, pts, pflags, sp_first, sp_count;
     DevBuf fe, nv, nt, fe_off, nv_off, nt_off, verts, tris, edges, edraw;
-    DevBuf bbox, erange, dtiles, pt_count, pt_off, ptiles, counters, segs, dprep; // counters: [0] segment allocator, [1] flatten slots, [2] overflow cubics, [64..191] flatten order
-    DevBuf tile_cnt, tile_off, tile_order, draw_order, cmds, scan_tmp, readback, item_slot, slot_item, flat_scratch, flat_perm, flat_ovf;
+    DevBuf bbox, erange, dtiles, pt_count, pt_off, ptiles, counters, segs, dprep; // counters: [0] segment allocator, [1] flatten slots, [2] overflow cubics
+    DevBuf tile_cnt, tile_off, tile_order, draw_order, cmds, scan_tmp, readback, item_slot, slot_item, flat_scratch, flat_perm, flat_ovf,
+        flat_wh;
     DevBuf step_nv, step_nt, sv_off, st_off, u_toff, u_vbeg, u_vend, dash_st; // dash_st: per dashed piece, its starting dash state
     HostBuf hblob, htot;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
@@ -295,7 +296,7 @@ void vkvg_hip_engine_destroy(vkh_engine *e) {
                       &e->erange, &e->ptiles, &e->counters, &e->segs, &e->dprep, &e->tile_cnt,
                       &e->tile_off, &e->tile_order, &e->draw_order, &e->cmds, &e->scan_tmp, &e->readback, &e->step_nv, &e->step_nt,
                       &e->sv_off, &e->st_off, &e->dash_st, &e->u_toff, &e->u_vbeg, &e->u_vend, &e->item_slot,
-                      &e->slot_item, &e->flat_scratch, &e->flat_perm, &e->flat_ovf};
+                      &e->slot_item, &e->flat_scratch, &e->flat_perm, &e->flat_ovf, &e->flat_wh};
     for (DevBuf *b : bufs)
         if (b->p)
             (void)hipFree(b->p);
@@ -673,10 +674,19 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
         !need(eng, cap, eng->flat_perm, 4 * (n_items + 1)) || !need(eng, cap, eng->flat_ovf, 4ull * (n_cubics + 1)) ||
         !need(eng, cap, eng->counters, 1024))
         return -1;
-    uint32_t *ctr = P<uint32_t>(eng->counters); // [0] segments, [1] flatten slots, [2] overflow, [64..] order
+    // work order of the flattening: per-workgroup bucket counts, their scan, the scatter
+    const uint32_t n_wh = 64u * vkh::flat_order_groups(n_items);
+    if (!need(eng, cap, eng->flat_wh, 4ull * (2ull * n_wh + 2)))
+        return -1;
+    uint32_t *ctr = P<uint32_t>(eng->counters); // [0] segments, [1] flatten slots, [2] overflow
     vkh::launch_zero_u32(s, ctr + 1, 2);
-    vkh::launch_zero_u32(s, ctr + 64, 64);
-    vkh::launch_flatten_order(s, d_items, n_items, ctr + 64, ctr + 128, P<uint32_t>(eng->flat_perm));
+    if (n_items) {
+        uint32_t *whist = P<uint32_t>(eng->flat_wh), *wcur = whist + n_wh + 1;
+        vkh::launch_flat_hist(s, d_items, n_items, whist);
+        if (scan_u32(eng, whist, n_wh, wcur, cap))
+            return -1;
+        vkh::launch_flat_scatter(s, d_items, n_items, wcur, P<uint32_t>(eng->flat_perm));
+    }
     vkh::launch_flatten_stage(s, d_items, n_items, P<uint32_t>(eng->flat_perm), P<uint32_t>(eng->item_cnt), ctr + 1,
                               P<uint32_t>(eng->slot_item), P<uint32_t>(eng->item_slot), P<v2>(eng->flat_scratch),
                               ctr + 2, P<uint32_t>(eng->flat_ovf));

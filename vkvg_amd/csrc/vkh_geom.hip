@@ -49,8 +49,11 @@ __device__ __forceinline__ uint32_t flat_rank(uint32_t *h, bool valid, uint32_t 
     return rank;
 }
 
+// Per workgroup of 256 items, the number of items in each bucket, stored bucket-major
+// (whist[key * n_wg + workgroup]): one exclusive scan of that array gives every workgroup its
+// first position in each bucket, without global atomics.
 __global__ __launch_bounds__(256) void k_flat_hist(const vkh_item *__restrict__ items, uint32_t n,
-                                                   uint32_t *__restrict__ hist) {
+                                                   uint32_t *__restrict__ whist, uint32_t n_wg) {
     __shared__ uint32_t h[64];
     if (threadIdx.x < 64)
         h[threadIdx.x] = 0;
@@ -58,37 +61,23 @@ __global__ __launch_bounds__(256) void k_flat_hist(const vkh_item *__restrict__ 
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     (void)flat_rank(h, i < n, i < n ? flat_key(items[i]) : 0u);
     __syncthreads();
-    if (threadIdx.x < 64 && h[threadIdx.x])
-        atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+    if (threadIdx.x < 64)
+        whist[(size_t)threadIdx.x * n_wg + blockIdx.x] = h[threadIdx.x];
 }
 
-// exclusive scan of the 64 bucket counts into cursors (one workgroup of 64)
-__global__ void k_flat_cursors(const uint32_t *__restrict__ hist, uint32_t *__restrict__ cursor) {
-    const uint32_t lane = threadIdx.x;
-    uint32_t v = hist[lane], x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o);
-        if ((int)lane >= o)
-            x += y;
-    }
-    cursor[lane] = x - v;
-}
-
+// every item to its position: the workgroup's first position in its bucket + its rank there
 __global__ __launch_bounds__(256) void k_flat_scatter(const vkh_item *__restrict__ items, uint32_t n,
-                                                      uint32_t *__restrict__ cursor, uint32_t *__restrict__ perm) {
-    __shared__ uint32_t h[64], base[64];
+                                                      const uint32_t *__restrict__ wcur, uint32_t n_wg,
+                                                      uint32_t *__restrict__ perm) {
+    __shared__ uint32_t h[64];
     if (threadIdx.x < 64)
         h[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t key = i < n ? flat_key(items[i]) : 0u;
     const uint32_t rank = flat_rank(h, i < n, key);
-    __syncthreads();
-    if (threadIdx.x < 64 && h[threadIdx.x])
-        base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], h[threadIdx.x]);
-    __syncthreads();
     if (i < n)
-        perm[base[key] + rank] = i;
+        perm[wcur[(size_t)key * n_wg + blockIdx.x] + rank] = i;
 }
 
 #define VKH_FLAT_STACK 8   // ancestor levels kept in LDS per thread by the subdivision pass
@@ -1052,13 +1041,16 @@ __global__ __launch_bounds__(256) void k_tri_edges(const Unit *__restrict__ unit
 
 // ---------------------------------------------------------------------------------------- launchers
 namespace vkh {
-void launch_flatten_order(hipStream_t s, const vkh_item *items, uint32_t n, uint32_t *hist, uint32_t *cursor,
-                          uint32_t *perm) {
-    if (!n)
-        return;
-    hipLaunchKernelGGL(k_flat_hist, dim3((n + 255) / 256), dim3(256), 0, s, items, n, hist);
-    hipLaunchKernelGGL(k_flat_cursors, dim3(1), dim3(64), 0, s, hist, cursor);
-    hipLaunchKernelGGL(k_flat_scatter, dim3((n + 255) / 256), dim3(256), 0, s, items, n, cursor, perm);
+uint32_t flat_order_groups(uint32_t n) { return (n + 255) / 256; }
+void launch_flat_hist(hipStream_t s, const vkh_item *items, uint32_t n, uint32_t *whist) {
+    if (n)
+        hipLaunchKernelGGL(k_flat_hist, dim3(flat_order_groups(n)), dim3(256), 0, s, items, n, whist,
+                           flat_order_groups(n));
+}
+void launch_flat_scatter(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *wcur, uint32_t *perm) {
+    if (n)
+        hipLaunchKernelGGL(k_flat_scatter, dim3(flat_order_groups(n)), dim3(256), 0, s, items, n, wcur,
+                           flat_order_groups(n), perm);
 }
 void launch_flatten_stage(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *perm, uint32_t *counts,
                           uint32_t *slot_alloc, uint32_t *slot_item, uint32_t *item_slot, v2 *scratch,

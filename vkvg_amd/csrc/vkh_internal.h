@@ -121,8 +121,9 @@ struct Cmd {
 namespace vkh {
 size_t scan_temp_words(uint32_t n);
 void scan_u32(hipStream_t s, const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *tmp);
-void launch_flatten_order(hipStream_t s, const vkh_item *items, uint32_t n, uint32_t *hist, uint32_t *cursor,
-                          uint32_t *perm);
+uint32_t flat_order_groups(uint32_t n); // workgroups of the flattening order's histogram and scatter
+void launch_flat_hist(hipStream_t s, const vkh_item *items, uint32_t n, uint32_t *whist);
+void launch_flat_scatter(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *wcur, uint32_t *perm);
 void launch_flatten_stage(hipStream_t s, const vkh_item *items, uint32_t n, const uint32_t *perm, uint32_t *counts,
                           uint32_t *slot_alloc, uint32_t *slot_item, uint32_t *item_slot, v2 *scratch,
                           uint32_t *ovf_count, uint32_t *ovf_list);
