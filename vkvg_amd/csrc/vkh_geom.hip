@@ -645,6 +645,21 @@ __device__ __forceinline__ StrokeCtx make_stroke_ctx(const vkh_stroke_style &st,
     return c;
 }
 
+// The dash array of a thread's stroke copied to LDS: the dash walks read one entry per dash
+// boundary and the next boundary depends on it, so a global load there is a full memory latency
+// per dash. Each thread owns a row of VKH_DASH_LDS + 1 words (an odd stride puts the lanes of a
+// wavefront in distinct banks); longer arrays stay in global memory.
+#define VKH_DASH_LDS 16
+#define VKH_DASH_ROW (VKH_DASH_LDS + 1)
+__device__ __forceinline__ void stage_dashes(StrokeCtx &c, float *s_dash) {
+    if (c.dashCount == 0 || c.dashCount > VKH_DASH_LDS)
+        return;
+    float *mine = s_dash + threadIdx.x * VKH_DASH_ROW;
+    for (uint32_t i = 0; i < c.dashCount; i++)
+        mine[i] = c.dashes[i];
+    c.dashes = mine;
+}
+
 // One step of a non-dashed sub-path: step 0 and step n-1 are the caps of an open path, the
 // others a join at point i; a closed path has a join at every point, the last one closing
 // (src/vkvg_context.c:869-935). A step's output depends only on its three points, so steps
@@ -712,6 +727,7 @@ __global__ void k_dash_walk(const Unit *__restrict__ units, uint32_t n_units, co
                             const v2 *__restrict__ pts, const uint8_t *__restrict__ pf,
                             const vkh_stroke_style *__restrict__ styles, const float *__restrict__ dashes,
                             uint2 *__restrict__ dash_st) {
+    __shared__ float s_dash[256 * VKH_DASH_ROW];
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= n_units)
         return;
@@ -726,6 +742,7 @@ __global__ void k_dash_walk(const Unit *__restrict__ units, uint32_t n_units, co
     if (!st.dash_count)
         return;
     StrokeCtx c = make_stroke_ctx(st, dashes, pts, pf, sp_first[un.subpath], sp_count[un.subpath]);
+    stage_dashes(c, s_dash);
     DashState dc = dash_start(c, st.dash_offset);
     for (uint32_t k = 0; k < s1 - s0; k++) {
         dash_st[s0 + k] = dash_pack(dc);
@@ -743,6 +760,7 @@ __global__ void k_stroke_steps(const Unit *__restrict__ units, uint32_t n_units,
                                uint32_t *__restrict__ step_nt, const uint32_t *__restrict__ sv_off,
                                const uint32_t *__restrict__ st_off, v2 *__restrict__ verts,
                                uint32_t *__restrict__ tris, uint2 *__restrict__ dash_st) {
+    __shared__ float s_dash[256 * VKH_DASH_ROW];
     uint32_t sidx = blockIdx.x * blockDim.x + threadIdx.x;
     if (sidx >= n_steps)
         return;
@@ -751,6 +769,7 @@ __global__ void k_stroke_steps(const Unit *__restrict__ units, uint32_t n_units,
     const vkh_draw &d = draws[un.draw];
     const vkh_stroke_style &st = styles[d.style];
     StrokeCtx c = make_stroke_ctx(st, dashes, pts, pf, sp_first[un.subpath], sp_count[un.subpath]);
+    stage_dashes(c, s_dash);
     bool closed = (sps[un.subpath].flags & VKH_SP_CLOSED) != 0;
     uint32_t i = sidx - step_off[u];
     if (!EMIT) {
