@@ -148,9 +148,9 @@ struct vkh_engine {
 
     DevBuf blob, units, tgts, item_cnt, item_off, pts, pflags, sp_first, sp_count;
     DevBuf fe, nv, nt, fe_off, nv_off, nt_off, verts, tris, edges, edraw;
-    DevBuf bbox, erange, dtiles, pt_count, pt_off, ptiles, counters, segs, dprep; // counters: [0] segment allocator, [1] flatten slots, [2] overflow cubics
+    DevBuf bbox, erange, dtiles, pt_count, pt_off, ptiles, counters, segs, dprep; // counters: [0] segment allocator, [1] flatten slots, [2] overflow cubics, [3] dashed pieces
     DevBuf tile_cnt, tile_off, tile_order, draw_order, cmds, scan_tmp, readback, item_slot, slot_item, flat_scratch, flat_perm, flat_ovf,
-        flat_wh;
+        flat_wh, dash_list;
     DevBuf step_nv, step_nt, sv_off, st_off, u_toff, u_vbeg, u_vend, dash_st; // dash_st: per dashed piece, its starting dash state
     HostBuf hblob, htot;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
@@ -296,7 +296,7 @@ void vkvg_hip_engine_destroy(vkh_engine *e) {
                       &e->erange, &e->ptiles, &e->counters, &e->segs, &e->dprep, &e->tile_cnt,
                       &e->tile_off, &e->tile_order, &e->draw_order, &e->cmds, &e->scan_tmp, &e->readback, &e->step_nv, &e->step_nt,
                       &e->sv_off, &e->st_off, &e->dash_st, &e->u_toff, &e->u_vbeg, &e->u_vend, &e->item_slot,
-                      &e->slot_item, &e->flat_scratch, &e->flat_perm, &e->flat_ovf, &e->flat_wh};
+                      &e->slot_item, &e->flat_scratch, &e->flat_perm, &e->flat_ovf, &e->flat_wh, &e->dash_list};
     for (DevBuf *b : bufs)
         if (b->p)
             (void)hipFree(b->p);
@@ -678,8 +678,8 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     const uint32_t n_wh = 64u * vkh::flat_order_groups(n_items);
     if (!need(eng, cap, eng->flat_wh, 4ull * (2ull * n_wh + 2)))
         return -1;
-    uint32_t *ctr = P<uint32_t>(eng->counters); // [0] segments, [1] flatten slots, [2] overflow
-    vkh::launch_zero_u32(s, ctr + 1, 2);
+    uint32_t *ctr = P<uint32_t>(eng->counters); // [0] segments, [1] flatten slots, [2] overflow, [3] dashed pieces
+    vkh::launch_zero_u32(s, ctr + 1, 3);
     if (n_items) {
         uint32_t *whist = P<uint32_t>(eng->flat_wh), *wcur = whist + n_wh + 1;
         vkh::launch_flat_hist(s, d_items, n_items, whist);
@@ -730,12 +730,13 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     const uint32_t n_fill_edges = tot[0], n_steps = tot[1];
     size_t sn4 = 4ull * (n_steps + 1);
     if (!need(eng, cap, eng->step_nv, sn4) || !need(eng, cap, eng->step_nt, sn4) || !need(eng, cap, eng->sv_off, sn4) ||
-        !need(eng, cap, eng->st_off, sn4) || !need(eng, cap, eng->dash_st, 2 * sn4))
+        !need(eng, cap, eng->st_off, sn4) || !need(eng, cap, eng->dash_st, 2 * sn4) ||
+        !need(eng, cap, eng->dash_list, 4 * sn4))
         return -1;
     vkh::launch_stroke_steps(s, false, d_units, n_units, d_step_off, n_steps, d_draws, d_sp, P<uint32_t>(eng->sp_first),
                              P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes,
                              P<uint32_t>(eng->step_nv), P<uint32_t>(eng->step_nt), nullptr, nullptr, nullptr, nullptr,
-                             P<uint2>(eng->dash_st));
+                             P<uint2>(eng->dash_st), P<uint4>(eng->dash_list), ctr + 3);
     if (scan_u32(eng, P<uint32_t>(eng->step_nv), n_steps, P<uint32_t>(eng->sv_off), cap) ||
         scan_u32(eng, P<uint32_t>(eng->step_nt), n_steps, P<uint32_t>(eng->st_off), cap))
         return -1;
@@ -758,7 +759,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     vkh::launch_stroke_steps(s, true, d_units, n_units, d_step_off, n_steps, d_draws, d_sp, P<uint32_t>(eng->sp_first),
                              P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes,
                              nullptr, nullptr, P<uint32_t>(eng->sv_off), P<uint32_t>(eng->st_off), P<v2>(eng->verts),
-                             P<uint32_t>(eng->tris), P<uint2>(eng->dash_st));
+                             P<uint32_t>(eng->tris), P<uint2>(eng->dash_st), P<uint4>(eng->dash_list), ctr + 3);
     vkh::launch_stroke_patch(s, d_units, n_units, d_step_off, d_draws, d_sp, P<uint32_t>(eng->sp_first),
                              P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes,
                              P<uint32_t>(eng->sv_off), P<uint32_t>(eng->st_off), P<uint32_t>(eng->tris));

@@ -557,8 +557,9 @@ __device__ void dashed_segment(const StrokeCtx &c, S &s, DashState &dc, uint32_t
 // the dashed segment from point k (iL is the last point for k = 0, as there), a closed path adds
 // the closing segment as piece `last`, and the last piece also emits the final cap. Run one after
 // another over all pieces, dash_start / dash_piece perform the reference's operations in its order;
-// the counting pass records the dash state at each piece's start so that the pieces can be
-// emitted independently.
+// k_dash_walk records the dash state at each piece's start so that the pieces can be counted and
+// emitted independently: by one thread each (k_stroke_steps), or, for a piece of many caps, by a
+// wavefront (k_dash_pieces).
 __device__ __forceinline__ uint32_t dash_pieces(uint32_t n, bool closed) { return n < 2 ? 0u : (closed ? n : n - 1); }
 
 __device__ __forceinline__ DashState dash_start(const StrokeCtx &c, float dashOffset) {
@@ -573,11 +574,10 @@ __device__ __forceinline__ DashState dash_start(const StrokeCtx &c, float dashOf
     return dc;
 }
 
-template <typename S>
-__device__ void dash_piece(const StrokeCtx &c, bool closed, uint32_t k, DashState &dc, S &s) {
+// the segment of piece k: its previous, start and end points and whether it comes from a curve
+__device__ __forceinline__ void dash_piece_points(const StrokeCtx &c, uint32_t k, uint32_t &iL, uint32_t &cp,
+                                                  uint32_t &iR, bool &curved) {
     const uint32_t last = c.n - 1;
-    uint32_t iL, cp, iR;
-    bool curved;
     if (k < last) {
         cp = k;
         iR = k + 1;
@@ -589,42 +589,65 @@ __device__ void dash_piece(const StrokeCtx &c, bool closed, uint32_t k, DashStat
         iL = last - 1;
         curved = false;
     }
+}
+
+// the final cap of a dashed sub-path ending inside a gap (src/vkvg_context.c:936-944), after its
+// last piece with the state that piece leaves
+template <typename S>
+__device__ void dash_final_cap(const StrokeCtx &c, S &s, DashState &dc, uint32_t iR) {
+    int32_t prevDash = (int32_t)dc.curDash - 1;
+    if (prevDash < 0)
+        dc.curDash = c.dashCount - 1;
+    float dprev = c.dashes[prevDash < 0 ? (int32_t)c.dashCount - 1 : prevDash];
+    float m = fminf(dprev - dc.curDashOffset, c.dashes[dc.curDash]);
+    v2 p = vsub(c.P[iR], vmul(dc.normal, m));
+    stroke_cap(c, s, p, dc.normal, false);
+}
+
+// piece k in one thread, in the reference's order of operations
+template <typename S>
+__device__ void dash_piece(const StrokeCtx &c, bool closed, uint32_t k, DashState &dc, S &s) {
+    uint32_t iL, cp, iR;
+    bool curved;
+    dash_piece_points(c, k, iL, cp, iR, curved);
     dashed_segment(c, s, dc, iL, cp, iR, curved);
-    if (k + 1 == dash_pieces(c.n, closed) && !dc.dashOn) {
-        int32_t prevDash = (int32_t)dc.curDash - 1;
-        if (prevDash < 0)
-            dc.curDash = c.dashCount - 1;
-        float dprev = c.dashes[prevDash < 0 ? (int32_t)c.dashCount - 1 : prevDash];
-        float m = fminf(dprev - dc.curDashOffset, c.dashes[dc.curDash]);
-        v2 p = vsub(c.P[iR], vmul(dc.normal, m));
-        stroke_cap(c, s, p, dc.normal, false);
-    }
+    if (k + 1 == dash_pieces(c.n, closed) && !dc.dashOn)
+        dash_final_cap(c, s, dc, iR);
 }
 
 // The dash state after piece k without emitting anything: dashed_segment's state updates only
 // (the join and caps it emits do not change the state), with the same float operations.
-__device__ __forceinline__ void dash_walk_piece(const StrokeCtx &c, uint32_t k, DashState &dc) {
+// Returns the number of caps the piece emits (the loop's iterations).
+__device__ __forceinline__ uint32_t dash_walk_piece(const StrokeCtx &c, uint32_t k, DashState &dc) {
     const uint32_t last = c.n - 1;
     const uint32_t cp = k < last ? k : last, iR = k < last ? k + 1 : 0;
     v2 d = vsub(c.P[iR], c.P[cp]);
     float segLen = vlen(d);
+    uint32_t caps = 0;
     while (dc.curDashOffset < segLen) {
         dc.dashOn = !dc.dashOn;
         dc.curDashOffset += c.dashes[dc.curDash];
         if (++dc.curDash == c.dashCount)
             dc.curDash = 0;
+        caps++;
     }
     dc.curDashOffset -= segLen;
     dc.curDashOffset = fmodf(dc.curDashOffset, dc.totDashLength);
+    return caps;
 }
 
-__device__ __forceinline__ uint2 dash_pack(const DashState &dc) {
-    return make_uint2((dc.dashOn ? 1u : 0u) | (dc.curDash << 1), __float_as_uint(dc.curDashOffset));
+// A piece's start state: dashOn in bit 0, the dash index above it, and in bit 31 whether the piece
+// has enough caps to be emitted by a whole wavefront (k_dash_pieces) rather than by one thread.
+#define VKH_DASH_WAVE_BIT 0x80000000u
+#define VKH_DASH_WAVE_MIN 64 // caps from which a piece is given a wavefront
+__device__ __forceinline__ uint2 dash_pack(const DashState &dc, bool wave) {
+    return make_uint2((dc.dashOn ? 1u : 0u) | (dc.curDash << 1) | (wave ? VKH_DASH_WAVE_BIT : 0u),
+                      __float_as_uint(dc.curDashOffset));
 }
 __device__ __forceinline__ DashState dash_unpack(const StrokeCtx &c, float dashOffset, uint2 ds) {
     DashState dc = dash_start(c, dashOffset);
     dc.dashOn = (ds.x & 1u) != 0;
-    dc.curDash = ds.x >> 1;
+    dc.curDash = (ds.x & ~VKH_DASH_WAVE_BIT) >> 1;
     dc.curDashOffset = __uint_as_float(ds.y);
     return dc;
 }
@@ -726,7 +749,7 @@ __global__ void k_dash_walk(const Unit *__restrict__ units, uint32_t n_units, co
                             const uint32_t *__restrict__ sp_first, const uint32_t *__restrict__ sp_count,
                             const v2 *__restrict__ pts, const uint8_t *__restrict__ pf,
                             const vkh_stroke_style *__restrict__ styles, const float *__restrict__ dashes,
-                            uint2 *__restrict__ dash_st) {
+                            uint2 *__restrict__ dash_st, uint4 *__restrict__ dlist, uint32_t *__restrict__ dcount) {
     __shared__ float s_dash[256 * VKH_DASH_ROW];
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= n_units)
@@ -743,10 +766,164 @@ __global__ void k_dash_walk(const Unit *__restrict__ units, uint32_t n_units, co
         return;
     StrokeCtx c = make_stroke_ctx(st, dashes, pts, pf, sp_first[un.subpath], sp_count[un.subpath]);
     stage_dashes(c, s_dash);
+    // pieces of many caps join the list that k_dash_pieces takes a wavefront per entry from
     DashState dc = dash_start(c, st.dash_offset);
     for (uint32_t k = 0; k < s1 - s0; k++) {
-        dash_st[s0 + k] = dash_pack(dc);
-        dash_walk_piece(c, k, dc);
+        const DashState d0 = dc;
+        const uint32_t caps = dash_walk_piece(c, k, dc);
+        const bool wave = caps >= VKH_DASH_WAVE_MIN;
+        dash_st[s0 + k] = dash_pack(d0, wave);
+        if (wave)
+            dlist[atomicAdd(dcount, 1u)] = make_uint4(s0 + k, caps, u, 0u);
+    }
+}
+
+// The dash updates of dashed_segment's loop without the cap: the state at the next boundary
+__device__ __forceinline__ void dash_advance(const StrokeCtx &c, float dreg, DashState &dc) {
+    dc.dashOn = !dc.dashOn;
+    dc.curDashOffset += c.dashCount <= 64 ? __shfl(dreg, (int)dc.curDash) : c.dashes[dc.curDash];
+    if (++dc.curDash == c.dashCount)
+        dc.curDash = 0;
+}
+
+// wave-wide exclusive prefix sum of v (all 64 lanes active); total in *tot
+__device__ __forceinline__ uint32_t wave_exclusive(uint32_t v, uint32_t lane, uint32_t *tot) {
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, (unsigned)o);
+        if ((int)lane >= o)
+            x += y;
+    }
+    *tot = __shfl(x, 63);
+    return x - v;
+}
+
+// One wavefront per dashed piece (entries of the list written by k_dash_walk): the piece's
+// operations of dashed_segment (and the final cap after the sub-path's last piece), with its caps
+// spread over the lanes. Lane l takes the caps l, l + 64,
+// ...; it reaches cap j's dash state by applying the loop's updates one by one from the piece's
+// start state (the same float additions in the same order as the sequential loop), and the cap
+// exists while the offset stays below the segment length. A cap's vertex and index counts come
+// from running it against a counting sink; their wave-wide prefix places it, so the caps land where
+// the sequential emission puts them and their relative vertex references (a start cap refers to
+// the next cap's first two vertices) resolve to the same vertices. The join before the first cap
+// and the final cap are emitted by lane 0. EMIT = false records the piece's totals.
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_dash_pieces(const Unit *__restrict__ units, uint32_t n_units,
+                                                     const uint32_t *__restrict__ step_off,
+                                                     const vkh_draw *__restrict__ draws,
+                                                     const vkh_subpath *__restrict__ sps,
+                                                     const uint32_t *__restrict__ sp_first,
+                                                     const uint32_t *__restrict__ sp_count, const v2 *__restrict__ pts,
+                                                     const uint8_t *__restrict__ pf,
+                                                     const vkh_stroke_style *__restrict__ styles,
+                                                     const float *__restrict__ dashes, uint32_t *__restrict__ step_nv,
+                                                     uint32_t *__restrict__ step_nt, const uint32_t *__restrict__ sv_off,
+                                                     const uint32_t *__restrict__ st_off, v2 *__restrict__ verts,
+                                                     uint32_t *__restrict__ tris, const uint2 *__restrict__ dash_st,
+                                                     const uint4 *__restrict__ dlist,
+                                                     const uint32_t *__restrict__ dcount) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t n_list = *dcount;
+    const uint32_t stride = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t li = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); li < n_list; li += stride) {
+        const uint4 ent = dlist[li]; // step, caps, unit
+        const uint32_t sidx = ent.x, u = ent.z;
+        const Unit un = units[u];
+        const vkh_draw &d = draws[un.draw];
+        const vkh_stroke_style &st = styles[d.style];
+        const StrokeCtx c = make_stroke_ctx(st, dashes, pts, pf, sp_first[un.subpath], sp_count[un.subpath]);
+        const bool closed = (sps[un.subpath].flags & VKH_SP_CLOSED) != 0;
+        const uint32_t k = sidx - step_off[u];
+        const float dreg = lane < c.dashCount ? c.dashes[lane] : 0.f; // lane i holds dash i
+        uint32_t iL, cp, iR;
+        bool curved;
+        dash_piece_points(c, k, iL, cp, iR, curved);
+        DashState dc0 = dash_unpack(c, st.dash_offset, dash_st[sidx]);
+        uint32_t nv = EMIT ? sv_off[sidx] : 0u, ni = EMIT ? 3u * st_off[sidx] : 0u;
+        const v2 p = c.P[cp], pR = c.P[iR];
+        if (!dc0.dashOn) { // the join at the piece's start point
+            CountSink cs;
+            vb_step(c, cs, iL, cp, iR, curved);
+            if (EMIT && lane == 0) {
+                EmitSink es;
+                es.verts = verts;
+                es.nv = nv;
+                es.idx = tris + ni;
+                es.ni = 0;
+                vb_step(c, es, iL, cp, iR, curved);
+            }
+            nv += cs.nv;
+            ni += cs.ni;
+        }
+        const v2 dv = vsub(pR, p);
+        dc0.normal = vnorm(dv);
+        const float segLen = vlen(dv);
+        // lane l: the state at cap l
+        DashState my = dc0;
+        const uint32_t setup = min(ent.y, 63u); // lane min(caps, 63) is the last one that matters
+        for (uint32_t t = 0; t < setup; t++) { // every lane takes part in the shuffles of the update
+            DashState nx = my;
+            dash_advance(c, dreg, nx);
+            if (t < lane)
+                my = nx;
+        }
+        DashState fin;
+        while (true) {
+            const bool act = my.curDashOffset < segLen;
+            const uint64_t am = __ballot(act);
+            CountSink cs;
+            const v2 p0 = vadd(p, vmul(my.normal, my.curDashOffset));
+            if (act)
+                stroke_cap(c, cs, p0, my.normal, my.dashOn);
+            uint32_t tv, ti;
+            const uint32_t ov = wave_exclusive(cs.nv, lane, &tv), oi = wave_exclusive(cs.ni, lane, &ti);
+            if (EMIT && act) {
+                EmitSink es;
+                es.verts = verts;
+                es.nv = nv + ov;
+                es.idx = tris + ni + oi;
+                es.ni = 0;
+                stroke_cap(c, es, p0, my.normal, my.dashOn);
+            }
+            nv += tv;
+            ni += ti;
+            if (am != ~0ull) {
+                // the first lane without a cap holds the state after the last cap's update
+                const int l0 = __ffsll((unsigned long long)~am) - 1;
+                fin.dashOn = __shfl((int)my.dashOn, l0) != 0;
+                fin.curDash = (uint32_t)__shfl((int)my.curDash, l0);
+                fin.curDashOffset = __shfl(my.curDashOffset, l0);
+                fin.totDashLength = my.totDashLength;
+                fin.normal = my.normal;
+                break;
+            }
+            for (uint32_t t = 0; t < 64; t++)
+                dash_advance(c, dreg, my);
+        }
+        fin.curDashOffset -= segLen;
+        fin.curDashOffset = fmodf(fin.curDashOffset, fin.totDashLength);
+        if (k + 1 == dash_pieces(c.n, closed) && !fin.dashOn) {
+            CountSink cs;
+            DashState f1 = fin;
+            dash_final_cap(c, cs, f1, iR);
+            if (EMIT && lane == 0) {
+                EmitSink es;
+                es.verts = verts;
+                es.nv = nv;
+                es.idx = tris + ni;
+                es.ni = 0;
+                DashState f2 = fin;
+                dash_final_cap(c, es, f2, iR);
+            }
+            nv += cs.nv;
+            ni += cs.ni;
+        }
+        if (!EMIT && lane == 0) {
+            step_nv[sidx] = nv;
+            step_nt[sidx] = ni / 3;
+        }
     }
 }
 
@@ -768,6 +945,12 @@ __global__ void k_stroke_steps(const Unit *__restrict__ units, uint32_t n_units,
     Unit un = units[u];
     const vkh_draw &d = draws[un.draw];
     const vkh_stroke_style &st = styles[d.style];
+    uint2 ds = make_uint2(0u, 0u);
+    if (st.dash_count) {
+        ds = dash_st[sidx];
+        if (ds.x & VKH_DASH_WAVE_BIT)
+            return; // a piece of many caps: k_dash_pieces
+    }
     StrokeCtx c = make_stroke_ctx(st, dashes, pts, pf, sp_first[un.subpath], sp_count[un.subpath]);
     stage_dashes(c, s_dash);
     bool closed = (sps[un.subpath].flags & VKH_SP_CLOSED) != 0;
@@ -775,7 +958,7 @@ __global__ void k_stroke_steps(const Unit *__restrict__ units, uint32_t n_units,
     if (!EMIT) {
         CountSink k;
         if (st.dash_count) {
-            DashState dc = dash_unpack(c, st.dash_offset, dash_st[sidx]);
+            DashState dc = dash_unpack(c, st.dash_offset, ds);
             dash_piece(c, closed, i, dc, k);
         } else {
             stroke_step(c, closed, i, k);
@@ -789,7 +972,7 @@ __global__ void k_stroke_steps(const Unit *__restrict__ units, uint32_t n_units,
         k.idx = tris + 3 * (size_t)st_off[sidx];
         k.ni = 0;
         if (st.dash_count) {
-            DashState dc = dash_unpack(c, st.dash_offset, dash_st[sidx]);
+            DashState dc = dash_unpack(c, st.dash_offset, ds);
             dash_piece(c, closed, i, dc, k);
         } else {
             stroke_step(c, closed, i, k);
@@ -1113,20 +1296,30 @@ void launch_stroke_steps(hipStream_t s, bool emit, const Unit *units, uint32_t n
                          uint32_t n_steps, const vkh_draw *draws, const vkh_subpath *sps, const uint32_t *spf,
                          const uint32_t *spc, const v2 *pts, const uint8_t *pf, const vkh_stroke_style *styles,
                          const float *dashes, uint32_t *step_nv, uint32_t *step_nt, const uint32_t *sv_off,
-                         const uint32_t *st_off, v2 *verts, uint32_t *tris, uint2 *dash_st) {
+                         const uint32_t *st_off, v2 *verts, uint32_t *tris, uint2 *dash_st, uint4 *dlist,
+                         uint32_t *dcount) {
     if (!n_steps)
         return;
     if (!emit)
         hipLaunchKernelGGL(k_dash_walk, dim3((n_units + 255) / 256), dim3(256), 0, s, units, n_units, step_off, n_steps,
-                           draws, sps, spf, spc, pts, pf, styles, dashes, dash_st);
-    if (emit)
+                           draws, sps, spf, spc, pts, pf, styles, dashes, dash_st, dlist, dcount);
+    // a wavefront per dashed piece, at most 4096 wavefronts walking the list
+    const uint32_t dblocks = (uint32_t)std::min<uint64_t>(((uint64_t)n_steps + 3) / 4, 1024);
+    if (emit) {
         hipLaunchKernelGGL(k_stroke_steps<true>, dim3((n_steps + 255) / 256), dim3(256), 0, s, units, n_units, step_off,
                            n_steps, draws, sps, spf, spc, pts, pf, styles, dashes, step_nv, step_nt, sv_off, st_off,
                            verts, tris, dash_st);
-    else
+        hipLaunchKernelGGL(k_dash_pieces<true>, dim3(dblocks), dim3(256), 0, s, units, n_units, step_off, draws, sps,
+                           spf, spc, pts, pf, styles, dashes, step_nv, step_nt, sv_off, st_off, verts, tris, dash_st,
+                           dlist, dcount);
+    } else {
         hipLaunchKernelGGL(k_stroke_steps<false>, dim3((n_steps + 255) / 256), dim3(256), 0, s, units, n_units,
                            step_off, n_steps, draws, sps, spf, spc, pts, pf, styles, dashes, step_nv, step_nt, sv_off,
                            st_off, verts, tris, dash_st);
+        hipLaunchKernelGGL(k_dash_pieces<false>, dim3(dblocks), dim3(256), 0, s, units, n_units, step_off, draws, sps,
+                           spf, spc, pts, pf, styles, dashes, step_nv, step_nt, sv_off, st_off, verts, tris, dash_st,
+                           dlist, dcount);
+    }
 }
 void launch_stroke_patch(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *step_off,
                          const vkh_draw *draws, const vkh_subpath *sps, const uint32_t *spf, const uint32_t *spc,

@@ -142,7 +142,8 @@ void launch_stroke_steps(hipStream_t s, bool emit, const Unit *units, uint32_t n
                          uint32_t n_steps, const vkh_draw *draws, const vkh_subpath *sps, const uint32_t *spf,
                          const uint32_t *spc, const v2 *pts, const uint8_t *pf, const vkh_stroke_style *styles,
                          const float *dashes, uint32_t *step_nv, uint32_t *step_nt, const uint32_t *sv_off,
-                         const uint32_t *st_off, v2 *verts, uint32_t *tris, uint2 *dash_st);
+                         const uint32_t *st_off, v2 *verts, uint32_t *tris, uint2 *dash_st, uint4 *dlist,
+                         uint32_t *dcount);
 void launch_stroke_patch(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *step_off,
                          const vkh_draw *draws, const vkh_subpath *sps, const uint32_t *spf, const uint32_t *spc,
                          const v2 *pts, const uint8_t *pf, const vkh_stroke_style *styles, const float *dashes,
