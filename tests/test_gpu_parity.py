@@ -62,6 +62,13 @@ def test_long_runs_of_one_draw_kind(device):
     _check(device, scenes.runs_scene())
 
 
+def test_long_dashed_pieces(device):
+    # pieces of a few to several hundred dash boundaries: emitted by one thread below 64 caps, by a
+    # wavefront in one, two or more rounds of 64 caps above (odd and zero-entry patterns, offsets,
+    # every cap and join, closed paths ending in a gap)
+    _check(device, scenes.long_dash_scene())
+
+
 def test_c2_full_scene(device):
     # the whole 1920x1080 configuration: 5000 polylines (25 % dashed) and 2000 even-odd stars
     _check(device, scenes.config2())

@@ -342,6 +342,41 @@ def clear_scene(size: int = 96) -> Scene:
     return Scene("clear", size, size, s, 2)
 
 
+def long_dash_scene(size: int = 640, n: int = 48) -> Scene:
+    """Dashed strokes whose pieces (segments between two points) carry from a few to several hundred
+    dash boundaries, around the counts where their emission moves from one thread to a wavefront
+    and where a wavefront takes a second and third round of 64 caps: per stroke a dash pattern of
+    1, 2, 3 or 5 entries (one of them possibly 0), a dash offset, a cap and a join; segment lengths
+    chosen as the pattern's mean boundary spacing times U{40..260}; open and closed polylines of
+    2-4 points (a closed one ends its last piece in a gap half the time)."""
+    rng = Rng(0x5EED0000 + 16 * 12)
+    s = Stream()
+    _white(s)
+    s.set_miter_limit(10.0)
+    patterns = ([3.0], [2.0, 1.0], [1.5, 0.0, 2.5], [2.0, 1.0, 0.5, 1.5, 1.0])
+    for i in range(n):
+        pat = patterns[i % len(patterns)]
+        spacing = sum(pat) / len(pat)
+        s.set_dash(pat, float(rng.uniform(0.0, 2.0 * sum(pat))))
+        s.set_line_width(float(rng.uniform(0.5, 5.0)))
+        s.set_line_cap([CAP_BUTT, CAP_ROUND, CAP_SQUARE][i % 3])
+        s.set_line_join([JOIN_MITER, JOIN_ROUND, JOIN_BEVEL][(i // 3) % 3])
+        s.set_source_rgba(*rng.u(3), float(rng.uniform(0.5, 1.0)))
+        m = 2 + i % 3
+        x, y = rng.uniform(20, size - 20, 2)
+        pts = [(x, y)]
+        for _ in range(m - 1):
+            length = spacing * rng.randint(40, 260)
+            ang = rng.uniform(0.0, 2.0 * math.pi)
+            x = min(max(x + length * math.cos(ang), 5.0), size - 5.0)
+            y = min(max(y + length * math.sin(ang), 5.0), size - 5.0)
+            pts.append((x, y))
+        s.polyline(np.array(pts, np.float32), close=(i % 4 == 3))
+        s.stroke()
+    s.set_dash([], 0.0)
+    return Scene("long_dash", size, size, s, n)
+
+
 def runs_scene(size: int = 256, runs: int = 4, run_len: int = 150) -> Scene:
     """Long runs of draws of one kind between single draws of the other: run_len short strokes
     (no fill edges) then one filled triangle, and run_len small fills (no stroke triangles) then one
