@@ -150,7 +150,7 @@ struct vkh_engine {
     DevBuf fe, nv, nt, fe_off, nv_off, nt_off, verts, tris, edges, edraw;
     DevBuf bbox, erange, dtiles, pt_count, pt_off, ptiles, counters, segs, dprep; // counters: [0] segment allocator, [1] flatten slots, [2] overflow cubics, [3] dashed pieces
     DevBuf tile_cnt, tile_off, tile_order, draw_order, cmds, scan_tmp, readback, item_slot, slot_item, flat_scratch, flat_perm, flat_ovf,
-        flat_wh, dash_list;
+        flat_wh, dash_list, tl_table;
     DevBuf step_nv, step_nt, sv_off, st_off, u_toff, u_vbeg, u_vend, dash_st; // dash_st: per dashed piece, its starting dash state
     HostBuf hblob, htot;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
@@ -166,6 +166,10 @@ struct vkh_engine {
                           // segments, commands
     uint32_t fine_words = 4096;
     std::vector<TargetInfo> h_tg;
+    // draws without path tiles (target, index within the target's draws), and per (target, coarse
+    // chunk) how many of them there are: every tile of the target holds their commands
+    std::vector<uint2> h_tileless;
+    std::vector<uint32_t> h_tl_table;
 
     // the uploaded (resident) frame
     bool resident = false;
@@ -296,7 +300,7 @@ void vkvg_hip_engine_destroy(vkh_engine *e) {
                       &e->erange, &e->ptiles, &e->counters, &e->segs, &e->dprep, &e->tile_cnt,
                       &e->tile_off, &e->tile_order, &e->draw_order, &e->cmds, &e->scan_tmp, &e->readback, &e->step_nv, &e->step_nt,
                       &e->sv_off, &e->st_off, &e->dash_st, &e->u_toff, &e->u_vbeg, &e->u_vend, &e->item_slot,
-                      &e->slot_item, &e->flat_scratch, &e->flat_perm, &e->flat_ovf, &e->flat_wh, &e->dash_list};
+                      &e->slot_item, &e->flat_scratch, &e->flat_perm, &e->flat_ovf, &e->flat_wh, &e->dash_list, &e->tl_table};
     for (DevBuf *b : bufs)
         if (b->p)
             (void)hipFree(b->p);
@@ -438,9 +442,18 @@ void vkvg_hip_get_stats(vkh_engine *e, vkh_stats *out) {
 const char *vkvg_hip_last_error(vkh_engine *e) { return e ? e->err.c_str() : "null engine"; }
 
 // coarse work split for the current tiling: draws per chunk and chunks per bin
-static void set_coarse_chunks(vkh_engine *eng) {
+static int set_coarse_chunks(vkh_engine *eng) {
     eng->coarse_chunk = vkh::coarse_chunk(eng->most_draws, eng->bin_total);
     eng->coarse_nch = (eng->most_draws + eng->coarse_chunk - 1) / eng->coarse_chunk;
+    const size_t n = (size_t)std::max<uint32_t>(eng->n_targets, 1u) * eng->coarse_nch;
+    eng->h_tl_table.assign(n, 0u);
+    for (const uint2 &tl : eng->h_tileless)
+        eng->h_tl_table[(size_t)tl.x * eng->coarse_nch + tl.y / eng->coarse_chunk]++;
+    if (!ensure(eng, eng->tl_table, 4 * n))
+        return -1;
+    CK(hipMemcpyAsync(eng->tl_table.p, eng->h_tl_table.data(), 4 * n, hipMemcpyHostToDevice, eng->stream));
+    CK(hipStreamSynchronize(eng->stream));
+    return 0;
 }
 
 int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
@@ -620,7 +633,12 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
     for (const TargetInfo &ti : tg)
         eng->most_draws = std::max(eng->most_draws, ti.draw_end - ti.draw_begin);
     eng->frame_ext = any_stencil || any_source;
-    set_coarse_chunks(eng);
+    eng->h_tileless.clear();
+    for (uint32_t i = 0; i < n_draws; i++)
+        if (!kind_has_tiles(f->draws[i].kind))
+            eng->h_tileless.push_back(make_uint2(f->draws[i].target, i - tg[f->draws[i].target].draw_begin));
+    if (set_coarse_chunks(eng))
+        return -1;
     eng->frame_gen++;
     eng->gfail_key = 0;
     eng->resident = true;
@@ -812,12 +830,25 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     if (!need(eng, cap, eng->draw_order, 4ull * vkh::order_words(n_draws)))
         return -1;
     vkh::launch_draw_order(s, n_draws, P<uint2>(eng->erange), P<uint32_t>(eng->draw_order));
+    // the coarse pass's command counts per (tile, chunk): seeded with the draws without path tiles,
+    // incremented by binning for every path tile that holds a command
+    const uint32_t nch = eng->coarse_nch;
+    const uint64_t n_tc64 = (uint64_t)tile_total * nch;
+    if (n_tc64 > 0x7FFFFFFFull) {
+        eng->err = "frame too large (tiles x draw chunks)";
+        return -3;
+    }
+    const uint32_t n_tc = (uint32_t)n_tc64;
+    if (!need(eng, cap, eng->tile_cnt, 4ull * (n_tc + 1)) || !need(eng, cap, eng->tile_off, 4ull * (n_tc + 1)))
+        return -1;
     for (int pass = 0; pass < 2; pass++) {
         const uint32_t seg_cap = (uint32_t)std::min<size_t>(eng->segs.cap / sizeof(Edge), 0xFFFFFFFFull);
         vkh::launch_zero_u32(s, P<uint32_t>(eng->counters), 1);
+        vkh::launch_tile_cnt_init(s, d_tg, n_targets, nch, P<uint32_t>(eng->tl_table), P<uint32_t>(eng->tile_cnt), n_tc);
         vkh::launch_bin_draws(s, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), P<uint2>(eng->erange),
                               P<DrawTiles>(eng->dtiles), n_draws, T, P<PathTile>(eng->ptiles), P<Edge>(eng->segs),
-                              seg_cap, P<uint32_t>(eng->counters), P<uint32_t>(eng->draw_order));
+                              seg_cap, P<uint32_t>(eng->counters), P<uint32_t>(eng->draw_order), d_draws, d_tg, nch,
+                              eng->coarse_chunk, P<uint32_t>(eng->tile_cnt));
         CK(hipGetLastError());
         if (totals(eng, cap, {P<uint32_t>(eng->counters)}, tot, 7))
             return -1;
@@ -835,19 +866,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
 
     if (!cap)
         CK(hipEventRecord(eng->ev_stage[3], s)); // binning done
-    // ---- coarse: ordered command list per tile
-    const uint32_t nch = eng->coarse_nch;
-    const uint64_t n_tc64 = (uint64_t)tile_total * nch;
-    if (n_tc64 > 0x7FFFFFFFull) {
-        eng->err = "frame too large (tiles x draw chunks)";
-        return -3;
-    }
-    const uint32_t n_tc = (uint32_t)n_tc64;
-    if (!need(eng, cap, eng->tile_cnt, 4ull * (n_tc + 1)) || !need(eng, cap, eng->tile_off, 4ull * (n_tc + 1)))
-        return -1;
-    vkh::launch_zero_u32(s, P<uint32_t>(eng->tile_cnt), n_tc + 1);
-    vkh::launch_coarse(s, false, bin_total, nch, eng->coarse_chunk, d_tg, n_targets, d_draws, P<DrawTiles>(eng->dtiles),
-                       P<PathTile>(eng->ptiles), P<uint32_t>(eng->tile_cnt), nullptr, nullptr);
+    // ---- coarse: ordered command list per tile (the counts came from binning)
     if (scan_u32(eng, P<uint32_t>(eng->tile_cnt), n_tc, P<uint32_t>(eng->tile_off), cap))
         return -1;
     if (totals(eng, cap, {P<uint32_t>(eng->tile_off) + n_tc}, tot, 8))
@@ -1005,7 +1024,8 @@ int vkvg_hip_retile(vkh_engine *eng, uint32_t T) {
     eng->tile_total = tile_total;
     eng->bin_total = bin_total;
     eng->frame_tile = T;
-    set_coarse_chunks(eng);
+    if (set_coarse_chunks(eng))
+        return -1;
     eng->frame_gen++;
     eng->gfail_key = 0;
     return 0;

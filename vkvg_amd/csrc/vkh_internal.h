@@ -166,7 +166,10 @@ void launch_set_pt_base(hipStream_t s, DrawTiles *dt, const uint32_t *pt_off, ui
 void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
 void launch_bin_draws(hipStream_t s, const Edge *edges, const uint32_t *edraw, const uint2 *erange,
                       const DrawTiles *dt, uint32_t n_draws, uint32_t tile, PathTile *pts, Edge *segs,
-                      uint32_t seg_cap, uint32_t *seg_alloc, const uint32_t *order);
+                      uint32_t seg_cap, uint32_t *seg_alloc, const uint32_t *order, const vkh_draw *draws,
+                      const TargetInfo *tg, uint32_t nch, uint32_t chunk_draws, uint32_t *tile_cnt);
+void launch_tile_cnt_init(hipStream_t s, const TargetInfo *tg, uint32_t n_targets, uint32_t nch,
+                          const uint32_t *table, uint32_t *tile_cnt, uint32_t n_tc);
 void launch_draw_prep(hipStream_t s, const vkh_draw *draws, uint32_t n_draws, const TargetInfo *tg,
                       const vkh_gradient *grads, uint32_t n_grads, const vkh_surface_source *srcs, DrawPrep *out,
                       GradRec *rec, GradPrep *gout);

@@ -99,6 +99,20 @@ __global__ void k_set_pt_base(DrawTiles *dt, const uint32_t *pt_off, uint32_t n)
 #define VKH_BIN_LDS_TILES 4096
 #define VKH_BIN_U 4 // edges per thread per load step
 
+// blends a draw of this kind makes at a pixel of winding w (1 for paint and clear)
+__device__ __forceinline__ int coverage_count(uint32_t kind, int32_t w) {
+    switch (kind) {
+    case VKH_DRAW_FILL_NZ:
+        return w != 0 ? 1 : 0;
+    case VKH_DRAW_FILL_EO:
+        return w & 1;
+    case VKH_DRAW_STROKE:
+        return w > 0 ? w : 0;
+    default:
+        return 1;
+    }
+}
+
 template <bool SCATTER>
 __device__ __forceinline__ void bin_edge(const Edge &e, const DrawTiles &dt, int32_t lt, int32_t ty_lo,
                                          int32_t ty_hi, uint32_t *s_cnt, int32_t *s_back, Edge *segs,
@@ -184,12 +198,18 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t *a, uint32_t n
     return total;
 }
 
+// Besides the path tiles and segments, every path tile that will hold a command (segments, a
+// non-zero blend count from its backdrop, or a clip) adds one to its tile's command count in the
+// draw's chunk (tile_cnt[tile * nch + chunk], the coarse pass's layout), so the coarse pass needs
+// no counting pass of its own.
 __global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges, const uint32_t *__restrict__ edraw,
                                                   const uint2 *__restrict__ erange,
                                                   const DrawTiles *__restrict__ dts, uint32_t T,
                                                   PathTile *__restrict__ pts, Edge *__restrict__ segs,
                                                   uint32_t seg_cap, uint32_t *__restrict__ seg_alloc,
-                                                  const uint32_t *__restrict__ draw_order) {
+                                                  const uint32_t *__restrict__ draw_order,
+                                                  const vkh_draw *__restrict__ draws, const TargetInfo *__restrict__ tg,
+                                                  uint32_t nch, uint32_t chunk_draws, uint32_t *__restrict__ tile_cnt) {
     __shared__ uint32_t s_cnt[VKH_BIN_LDS_TILES];
     __shared__ int32_t s_back[VKH_BIN_LDS_TILES];
     __shared__ uint32_t s_wave[4];
@@ -200,6 +220,11 @@ __global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges
         return; // paint / clear, or no edges
     const uint2 er = erange[d];
     const int32_t lt = (int32_t)__builtin_ctz(T);
+    const vkh_draw &dr = draws[d];
+    const uint32_t kind = dr.kind;
+    const bool clip = kind_is_clip(kind);
+    const TargetInfo &TI = tg[dr.target];
+    const uint32_t chunk = (d - TI.draw_begin) / chunk_draws;
     const uint32_t bw = dt.bw, bh = (uint32_t)(dt.by1 - dt.by0 + 1);
     // The draw's edges, VKH_BIN_U per thread per step with every load issued before any is used.
     // A draw of at most 256 * VKH_BIN_U edges (most of them) keeps its edges in registers for both
@@ -268,31 +293,16 @@ __global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges
             p.seg_off = base + off;
             p.reserved = 0;
             out[k] = p;
+            if (p.nseg || clip || coverage_count(kind, p.backdrop) != 0) {
+                const uint32_t r = k / bw, c = k - r * bw;
+                const uint32_t gtile = TI.tile_base + (uint32_t)(ty_lo + (int32_t)r) * TI.tilesW + (uint32_t)dt.bx0 + c;
+                atomicAdd(&tile_cnt[(size_t)gtile * nch + chunk], 1u);
+            }
         }
         __syncthreads();
         if (fits)
             for_edges([&](const Edge &e) { bin_edge<true>(e, dt, lt, ty_lo, ty_hi, s_cnt, s_back, segs, base); });
         __syncthreads();
-    }
-}
-
-__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        p[i] = 0;
-}
-
-// ----------------------------------------------------------------------------------- coarse
-__device__ __forceinline__ int coverage_count(uint32_t kind, int32_t w) {
-    switch (kind) {
-    case VKH_DRAW_FILL_NZ:
-        return w != 0 ? 1 : 0;
-    case VKH_DRAW_FILL_EO:
-        return w & 1;
-    case VKH_DRAW_STROKE:
-        return w > 0 ? w : 0;
-    default:
-        return 1;
     }
 }
 
@@ -309,9 +319,35 @@ __device__ __forceinline__ uint32_t find_target_by(const TargetInfo *tg, uint32_
     return lo - 1;
 }
 
+// The coarse pass's per (tile, chunk) command counts before binning adds the path tiles': the draws
+// without path tiles (paint, clear, stencil draws) hold a command in every tile of their target,
+// table[target * nch + chunk] of them in the chunk
+__global__ void k_tile_cnt_init(const TargetInfo *__restrict__ tg, uint32_t n_targets, uint32_t nch,
+                                const uint32_t *__restrict__ table, uint32_t *__restrict__ tile_cnt, uint32_t n_tc) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n_tc)
+        return;
+    if (i == n_tc) {
+        tile_cnt[i] = 0;
+        return;
+    }
+    const uint32_t gtile = i / nch, chunk = i - gtile * nch;
+    const uint32_t t = find_target_by(tg, n_targets, gtile, false);
+    tile_cnt[i] = table[(size_t)t * nch + chunk];
+}
+
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0;
+}
+
+// ----------------------------------------------------------------------------------- coarse
 // One workgroup per (bin of 16 x 16 tiles, chunk of `chunk` draws), one thread per
 // tile; walks the chunk's draws in order. Counts are laid out tile-major (tile * nch + chunk), so
-// one scan gives every tile a contiguous command list ordered by draw, without any sort.
+// one scan gives every tile a contiguous command list ordered by draw, without any sort. The
+// rendering pipeline takes the counts from binning (k_tile_cnt_init, k_bin_draw) and runs only the
+// writing pass; the differentiable renderer runs both.
 // Per 256 draws: the draws whose tile bbox meets the bin are compacted into LDS (in draw order,
 // with their bbox and kind), then every tile walks that list, loading the path tiles of four
 // draws at a time so their latencies overlap.
@@ -341,6 +377,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_co
     const uint32_t d_end = min(T.draw_end, d_begin + chunk_draws);
     uint32_t count = 0;
     uint32_t out = (write && valid) ? tile_off[(size_t)gtile * nch + chunk] : 0;
+    // the commands counted for this (tile, chunk): a write never leaves that range
+    const uint32_t room = (write && valid) ? tile_off[(size_t)gtile * nch + chunk + 1] - out : 0u;
     for (uint32_t c0 = d_begin; c0 < d_end; c0 += 256) {
         const uint32_t d = c0 + tid;
         bool hit = false;
@@ -395,7 +433,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_co
                 const bool emit = ptid[u] == VKH_NO_DRAW || p[u].nseg > 0 || kind_is_clip(kind) ||
                                   coverage_count(kind, p[u].backdrop) != 0;
                 if (emit) {
-                    if (write)
+                    if (write && count < room)
                         cmds[out + count] = Cmd{s_draw[h], ptid[u]};
                     count++;
                 }
@@ -1165,10 +1203,16 @@ void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n) {
 }
 void launch_bin_draws(hipStream_t s, const Edge *edges, const uint32_t *edraw, const uint2 *erange,
                       const DrawTiles *dt, uint32_t n_draws, uint32_t tile, PathTile *pts, Edge *segs,
-                      uint32_t seg_cap, uint32_t *seg_alloc, const uint32_t *order) {
+                      uint32_t seg_cap, uint32_t *seg_alloc, const uint32_t *order, const vkh_draw *draws,
+                      const TargetInfo *tg, uint32_t nch, uint32_t chunk_draws, uint32_t *tile_cnt) {
     if (n_draws)
         hipLaunchKernelGGL(k_bin_draw, dim3(n_draws), dim3(256), 0, s, edges, edraw, erange, dt, tile, pts, segs,
-                           seg_cap, seg_alloc, order);
+                           seg_cap, seg_alloc, order, draws, tg, nch, chunk_draws, tile_cnt);
+}
+void launch_tile_cnt_init(hipStream_t s, const TargetInfo *tg, uint32_t n_targets, uint32_t nch,
+                          const uint32_t *table, uint32_t *tile_cnt, uint32_t n_tc) {
+    hipLaunchKernelGGL(k_tile_cnt_init, dim3((n_tc + 256) / 256), dim3(256), 0, s, tg, n_targets, nch, table, tile_cnt,
+                       n_tc);
 }
 uint32_t coarse_chunk(uint32_t most_draws, uint32_t n_bins) {
     const uint32_t want = (1024 + std::max(n_bins, 1u) - 1) / std::max(n_bins, 1u); // chunks per bin
