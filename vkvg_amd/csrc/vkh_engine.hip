@@ -781,22 +781,23 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     vkh::launch_stroke_patch(s, d_units, n_units, d_step_off, d_draws, d_sp, P<uint32_t>(eng->sp_first),
                              P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes,
                              P<uint32_t>(eng->sv_off), P<uint32_t>(eng->st_off), P<uint32_t>(eng->tris));
-    vkh::launch_unit_ranges(s, n_units, d_step_off, P<uint32_t>(eng->sv_off), P<uint32_t>(eng->st_off),
-                            P<uint32_t>(eng->u_toff), P<uint32_t>(eng->u_vbeg), P<uint32_t>(eng->u_vend));
-    // the edge kernels also reduce each draw's edge bbox and edge index range
+    // the edge kernels also reduce each draw's edge bbox and edge index range (a stroke draw's range
+    // is opened by the unit ranges and appended to by the triangle edges)
     size_t dn4 = 4ull * (n_draws + 1);
     if (!need(eng, cap, eng->bbox, sizeof(int4) * (n_draws + 1)) || !need(eng, cap, eng->dtiles, sizeof(DrawTiles) * (n_draws + 1)) ||
         !need(eng, cap, eng->erange, sizeof(uint2) * (n_draws + 1)) || !need(eng, cap, eng->pt_count, dn4) ||
         !need(eng, cap, eng->pt_off, dn4) || !need(eng, cap, eng->counters, 256))
         return -1;
     vkh::launch_draw_bbox_init(s, P<int4>(eng->bbox), P<uint2>(eng->erange), n_draws);
+    vkh::launch_unit_ranges(s, d_units, n_units, d_draws, d_step_off, P<uint32_t>(eng->sv_off),
+                            P<uint32_t>(eng->st_off), P<uint32_t>(eng->u_toff), P<uint32_t>(eng->u_vbeg),
+                            P<uint32_t>(eng->u_vend), n_fill_edges, P<uint2>(eng->erange));
     vkh::launch_fill_edges(s, d_units, n_units, P<uint32_t>(eng->fe_off), n_fill_edges, d_draws, d_tg,
                            P<uint32_t>(eng->sp_first), P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<Edge>(eng->edges),
                            P<uint32_t>(eng->edraw), P<int4>(eng->bbox), P<uint2>(eng->erange));
     vkh::launch_tri_edges(s, d_units, n_units, P<uint32_t>(eng->u_toff), P<uint32_t>(eng->u_vbeg),
                           P<uint32_t>(eng->u_vend), n_tris, d_draws, d_tg, P<v2>(eng->verts), P<uint32_t>(eng->tris),
-                          P<Edge>(eng->edges) + n_fill_edges, P<uint32_t>(eng->edraw) + n_fill_edges, n_fill_edges,
-                          P<int4>(eng->bbox), P<uint2>(eng->erange));
+                          P<Edge>(eng->edges), P<uint32_t>(eng->edraw), P<int4>(eng->bbox), P<uint2>(eng->erange));
     CK(hipGetLastError());
 
     if (!cap)

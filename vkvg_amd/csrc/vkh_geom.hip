@@ -1022,18 +1022,26 @@ __global__ void k_stroke_patch(const Unit *__restrict__ units, uint32_t n_units,
 }
 
 // per unit: first triangle and vertex range of its stroke steps
-__global__ void k_unit_ranges(uint32_t n_units, const uint32_t *__restrict__ step_off,
-                              const uint32_t *__restrict__ sv_off, const uint32_t *__restrict__ st_off,
-                              uint32_t *__restrict__ u_toff, uint32_t *__restrict__ u_vbeg,
-                              uint32_t *__restrict__ u_vend) {
+// Per unit: its first triangle and its vertex range. The first unit of each stroke draw also opens
+// the draw's edge range at the slot of its first triangle's first edge: k_tri_edges appends the
+// draw's kept edges there (erange[d].y is the append cursor), so the range holds no dropped edges.
+__global__ void k_unit_ranges(const Unit *__restrict__ units, uint32_t n_units, const vkh_draw *__restrict__ draws,
+                              const uint32_t *__restrict__ step_off, const uint32_t *__restrict__ sv_off,
+                              const uint32_t *__restrict__ st_off, uint32_t *__restrict__ u_toff,
+                              uint32_t *__restrict__ u_vbeg, uint32_t *__restrict__ u_vend, uint32_t ebase,
+                              uint2 *__restrict__ erange) {
     uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u > n_units)
         return;
     uint32_t s0 = step_off[u];
-    u_toff[u] = st_off[s0];
+    const uint32_t t0 = st_off[s0];
+    u_toff[u] = t0;
     if (u < n_units) {
         u_vbeg[u] = sv_off[s0];
         u_vend[u] = sv_off[step_off[u + 1]];
+        const uint32_t d = units[u].draw;
+        if (draws[d].kind == VKH_DRAW_STROKE && (u == 0 || units[u - 1].draw != d))
+            erange[d] = make_uint2(ebase + 3u * t0, ebase + 3u * t0);
     }
 }
 
@@ -1127,6 +1135,52 @@ __device__ __forceinline__ bool tri_has_edge(const int32_t (&N)[3][4], int32_t x
     return h;
 }
 
+// Appends each lane's n edges of draw d to the draw's edge range (erange[d].y is its cursor) and
+// reduces the draw's bbox: one set of atomics per distinct draw among the lanes, the lanes of a
+// draw placed by a wave-wide prefix of their counts. Returns the lane's first slot. Lanes without
+// edges pass VKH_NO_DRAW; every lane of the wavefront must call it.
+__device__ __forceinline__ uint32_t wave_draw_append(uint32_t d, uint32_t n, int32_t x0, int32_t y0, int32_t x1,
+                                                     int32_t y1, int4 *__restrict__ bbox, uint2 *__restrict__ erange) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t at = 0;
+    uint64_t pending = __ballot(d != VKH_NO_DRAW);
+    while (pending) {
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1;
+        const uint32_t dl = __shfl(d, (int)leader);
+        const bool mine = d == dl;
+        int32_t a = mine ? x0 : INT32_MAX, b = mine ? y0 : INT32_MAX;
+        int32_t c = mine ? x1 : INT32_MIN, e = mine ? y1 : INT32_MIN;
+        uint32_t x = mine ? n : 0u; // inclusive prefix of the counts of this draw's lanes
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            a = min(a, __shfl_xor(a, o));
+            b = min(b, __shfl_xor(b, o));
+            c = max(c, __shfl_xor(c, o));
+            e = max(e, __shfl_xor(e, o));
+        }
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if ((int)lane >= o)
+                x += y;
+        }
+        const uint32_t total = __shfl(x, 63);
+        uint32_t b0 = 0;
+        if (lane == leader) {
+            atomicMin(&bbox[dl].x, a);
+            atomicMin(&bbox[dl].y, b);
+            atomicMax(&bbox[dl].z, c);
+            atomicMax(&bbox[dl].w, e);
+            b0 = atomicAdd(&erange[dl].y, total);
+        }
+        b0 = (uint32_t)__shfl((int)b0, (int)leader);
+        if (mine)
+            at = b0 + x - n;
+        pending &= ~__ballot(mine);
+    }
+    return at;
+}
+
 // Stroke triangles -> 3 edges each, oriented so every triangle winds +1 (ray to -x, +y edges
 // count +1; area > 0 is traversed 0 -> 2 -> 1); indices outside the sub-path's own vertices
 // (never drawn by a valid stream) or a degenerate triangle produce no edge. A stroke's coverage count is the sum of its triangles'
@@ -1134,7 +1188,10 @@ __device__ __forceinline__ bool tri_has_edge(const int32_t (&N)[3][4], int32_t x
 // term, so such pairs between neighbouring triangles of the same draw (the diagonal of each
 // segment quad, the spokes of round joins and caps) are dropped. A pair (t, t+1) is dropped only
 // when t-1 does not hold b->a and t+2 does not hold a->b, and only when t-1 .. t+2 lie in the
-// same wavefront, so the two triangles always take the same decision.
+// same wavefront, so the two triangles always take the same decision. The kept edges are appended
+// to their draw's range (k_unit_ranges opened it at the draw's first triangle slot), so the range
+// is dense; their order inside it depends on which wavefront appends first, and nothing downstream
+// depends on it (every per-tile quantity is an integer sum over the draw's edges).
 __global__ __launch_bounds__(256) void k_tri_edges(const Unit *__restrict__ units, uint32_t n_units,
                                                    const uint32_t *__restrict__ u_toff,
                                                    const uint32_t *__restrict__ u_vbeg,
@@ -1142,8 +1199,8 @@ __global__ __launch_bounds__(256) void k_tri_edges(const Unit *__restrict__ unit
                                                    const vkh_draw *__restrict__ draws,
                                                    const TargetInfo *__restrict__ tgts, const v2 *__restrict__ verts,
                                                    const uint32_t *__restrict__ tris, Edge *__restrict__ edges,
-                                                   uint32_t *__restrict__ edraw, uint32_t ebase,
-                                                   int4 *__restrict__ bbox, uint2 *__restrict__ erange) {
+                                                   uint32_t *__restrict__ edraw, int4 *__restrict__ bbox,
+                                                   uint2 *__restrict__ erange) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = t < total_tris;
     uint32_t draw = VKH_NO_DRAW;
@@ -1218,27 +1275,28 @@ __global__ __launch_bounds__(256) void k_tri_edges(const Unit *__restrict__ unit
     const bool fwd_visible = lane >= 1 && lane <= 61; // t-1 .. t+2 in this wavefront
     const bool bwd_visible = lane >= 2 && lane <= 62; // t-2 .. t+1 in this wavefront
     int32_t bx0 = INT32_MAX, by0 = INT32_MAX, bx1 = INT32_MIN, by1 = INT32_MIN;
-    uint32_t i0 = 0xFFFFFFFFu, i1 = 0;
-    const uint32_t eb = ebase + 3 * t; // index of the triangle's first edge in the frame's edge list
+    uint32_t kmask = 0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const bool fwd = fwd_visible && rev_p1[k] && !rev_m1[k] && !same_p2[k];
         const bool bwd = bwd_visible && rev_m1[k] && !same_m2[k] && !rev_p1[k];
-        const bool keep = draw != VKH_NO_DRAW && !fwd && !bwd;
-        if (live) {
-            edges[3 * (size_t)t + k] = keep ? Edge{E[k][0], E[k][1], E[k][2], E[k][3]} : Edge{0, 0, 0, 0};
-            edraw[3 * (size_t)t + k] = keep ? draw : VKH_NO_DRAW;
-        }
-        if (keep) {
+        if (draw != VKH_NO_DRAW && !fwd && !bwd) {
+            kmask |= 1u << k;
             bx0 = min(bx0, min(E[k][0], E[k][2]));
             bx1 = max(bx1, max(E[k][0], E[k][2]));
             by0 = min(by0, min(E[k][1], E[k][3]));
             by1 = max(by1, max(E[k][1], E[k][3]));
-            i0 = min(i0, eb + k);
-            i1 = eb + k + 1;
         }
     }
-    wave_draw_bbox(i1 ? draw : VKH_NO_DRAW, bx0, by0, bx1, by1, i0, i1, bbox, erange);
+    const uint32_t nk = (uint32_t)__builtin_popcount(kmask);
+    uint32_t at = wave_draw_append(nk ? draw : VKH_NO_DRAW, nk, bx0, by0, bx1, by1, bbox, erange);
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        if (kmask & (1u << k)) {
+            edges[at] = Edge{E[k][0], E[k][1], E[k][2], E[k][3]};
+            edraw[at] = draw;
+            at++;
+        }
 }
 
 // ---------------------------------------------------------------------------------------- launchers
@@ -1329,10 +1387,11 @@ void launch_stroke_patch(hipStream_t s, const Unit *units, uint32_t n_units, con
         hipLaunchKernelGGL(k_stroke_patch, dim3((n_units + 255) / 256), dim3(256), 0, s, units, n_units, step_off,
                            draws, sps, spf, spc, pts, pf, styles, dashes, sv_off, st_off, tris);
 }
-void launch_unit_ranges(hipStream_t s, uint32_t n_units, const uint32_t *step_off, const uint32_t *sv_off,
-                        const uint32_t *st_off, uint32_t *u_toff, uint32_t *u_vbeg, uint32_t *u_vend) {
-    hipLaunchKernelGGL(k_unit_ranges, dim3((n_units + 1 + 255) / 256), dim3(256), 0, s, n_units, step_off, sv_off,
-                       st_off, u_toff, u_vbeg, u_vend);
+void launch_unit_ranges(hipStream_t s, const Unit *units, uint32_t n_units, const vkh_draw *draws,
+                        const uint32_t *step_off, const uint32_t *sv_off, const uint32_t *st_off, uint32_t *u_toff,
+                        uint32_t *u_vbeg, uint32_t *u_vend, uint32_t ebase, uint2 *erange) {
+    hipLaunchKernelGGL(k_unit_ranges, dim3((n_units + 1 + 255) / 256), dim3(256), 0, s, units, n_units, draws,
+                       step_off, sv_off, st_off, u_toff, u_vbeg, u_vend, ebase, erange);
 }
 void launch_fill_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *fill_off, uint32_t total,
                        const vkh_draw *draws, const TargetInfo *tg, const uint32_t *spf, const uint32_t *spc,
@@ -1344,9 +1403,9 @@ void launch_fill_edges(hipStream_t s, const Unit *units, uint32_t n_units, const
 void launch_tri_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *u_toff,
                       const uint32_t *u_vbeg, const uint32_t *u_vend, uint32_t total_tris, const vkh_draw *draws,
                       const TargetInfo *tg, const v2 *verts, const uint32_t *tris, Edge *edges, uint32_t *edraw,
-                      uint32_t ebase, int4 *bbox, uint2 *erange) {
+                      int4 *bbox, uint2 *erange) {
     if (total_tris)
         hipLaunchKernelGGL(k_tri_edges, dim3((total_tris + 255) / 256), dim3(256), 0, s, units, n_units, u_toff,
-                           u_vbeg, u_vend, total_tris, draws, tg, verts, tris, edges, edraw, ebase, bbox, erange);
+                           u_vbeg, u_vend, total_tris, draws, tg, verts, tris, edges, edraw, bbox, erange);
 }
 } // namespace vkh

@@ -148,15 +148,16 @@ void launch_stroke_patch(hipStream_t s, const Unit *units, uint32_t n_units, con
                          const vkh_draw *draws, const vkh_subpath *sps, const uint32_t *spf, const uint32_t *spc,
                          const v2 *pts, const uint8_t *pf, const vkh_stroke_style *styles, const float *dashes,
                          const uint32_t *sv_off, const uint32_t *st_off, uint32_t *tris);
-void launch_unit_ranges(hipStream_t s, uint32_t n_units, const uint32_t *step_off, const uint32_t *sv_off,
-                        const uint32_t *st_off, uint32_t *u_toff, uint32_t *u_vbeg, uint32_t *u_vend);
+void launch_unit_ranges(hipStream_t s, const Unit *units, uint32_t n_units, const vkh_draw *draws,
+                        const uint32_t *step_off, const uint32_t *sv_off, const uint32_t *st_off, uint32_t *u_toff,
+                        uint32_t *u_vbeg, uint32_t *u_vend, uint32_t ebase, uint2 *erange);
 void launch_fill_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *fill_off, uint32_t total,
                        const vkh_draw *draws, const TargetInfo *tg, const uint32_t *spf, const uint32_t *spc,
                        const v2 *pts, Edge *edges, uint32_t *edraw, int4 *bbox, uint2 *erange);
 void launch_tri_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *u_toff,
                       const uint32_t *u_vbeg, const uint32_t *u_vend, uint32_t total_tris, const vkh_draw *draws,
                       const TargetInfo *tg, const v2 *verts, const uint32_t *tris, Edge *edges, uint32_t *edraw,
-                      uint32_t ebase, int4 *bbox, uint2 *erange);
+                      int4 *bbox, uint2 *erange);
 
 // raster stage (vkh_raster.hip)
 void launch_draw_bbox_init(hipStream_t s, int4 *bbox, uint2 *erange, uint32_t n_draws);
