@@ -445,6 +445,98 @@ __host__ __device__ void agg_flatten_stack(const Cubic &root, float tol, Emit em
 }
 
 // ---------------------------------------------------------------------------------------
+// The dash loop of _draw_dashed_segment (src/vkvg_context_internal.c:1240-1264) without its caps:
+// while the offset is below the segment length, flip dashOn, add the current dash to the offset in
+// float and move to the next dash. Each addition rounds, so the offsets follow from the additions
+// in order; but while every partial sum is a multiple of 2^g (g the exponent of the lowest set bit
+// of the offset and of the dashes) below 2^(g+24), every one of them is a float and the additions
+// are exact. A run of whole periods (a period, from any dash, holds every dash once) then adds
+// exactly their number times the period's sum, flips dashOn once per dash and leaves the dash index
+// where it was, so such a run is taken in one step with the same result as the loop.
+
+// exponent of the lowest set bit of a finite v > 0
+VKH_HD int f32_lowbit(float v) {
+    uint32_t b;
+    memcpy(&b, &v, 4);
+    int e = (int)((b >> 23) & 0xFFu);
+    uint32_t m = b & 0x7FFFFFu;
+    if (e) {
+        m |= 0x800000u;
+        e -= 150;
+    } else {
+        e = -149;
+    }
+    return e + __builtin_ctz(m);
+}
+
+struct DashPeriod {
+    double sum; // exact sum of the dashes
+    int lowbit; // lowest set bit over the non-zero dashes
+    bool ok;    // dashes finite, >= 0, with a non-zero exact sum
+};
+
+VKH_HD DashPeriod dash_period(const float *dashes, uint32_t n) {
+    DashPeriod p{0.0, 1 << 20, n > 0};
+    for (uint32_t i = 0; i < n; i++) {
+        const float v = dashes[i];
+        if (!(v >= 0.f) || !(v <= 3.0e38f)) {
+            p.ok = false;
+        } else if (v > 0.f) {
+            p.sum += (double)v;
+            const int lb = f32_lowbit(v);
+            p.lowbit = lb < p.lowbit ? lb : p.lowbit;
+        }
+    }
+    // every partial sum is a multiple of 2^lowbit; below 2^(lowbit + 52) the double additions were exact
+    if (!(p.sum > 0.0) || !(p.sum < ldexp(1.0, p.lowbit + 52)))
+        p.ok = false;
+    return p;
+}
+
+// Whole periods the loop takes from offset off < segLen with every float addition exact and every
+// offset before an addition below segLen: the largest k with off + k * sum < min(2^(g+24), segLen).
+// (A k of at least 1 needs sum < 2^(g+24), so off + k * sum is a multiple of 2^g below 2^(g+25) and
+// exact in double; the quotient's rounding is corrected by the exact test.)
+VKH_HD uint32_t dash_whole_periods(const DashPeriod &p, float off, float segLen) {
+    if (!p.ok || !(off >= 0.f))
+        return 0;
+    int g = p.lowbit;
+    if (off > 0.f) {
+        const int lo = f32_lowbit(off);
+        g = lo < g ? lo : g;
+    }
+    const double lim = fmin(ldexp(1.0, g + 24), (double)segLen);
+    const double room = lim - (double)off;
+    if (!(room > p.sum))
+        return 0;
+    const double kd = floor(room / p.sum);
+    uint32_t k = kd < 16777216.0 ? (uint32_t)kd : 16777216u;
+    if ((double)off + (double)k * p.sum >= lim)
+        k--;
+    return k;
+}
+
+// The loop over one segment: advances (on, cur, off) and returns the number of caps it makes.
+VKH_HD uint32_t dash_walk_segment(const float *dashes, uint32_t n, const DashPeriod &p, float segLen, bool &on,
+                                  uint32_t &cur, float &off) {
+    uint32_t caps = 0;
+    while (off < segLen) {
+        const uint32_t w = dash_whole_periods(p, off, segLen);
+        if (w) {
+            off = (float)((double)off + (double)w * p.sum);
+            caps += w * n;
+            on = on != (((w * n) & 1u) != 0);
+        }
+        on = !on;
+        off += dashes[cur];
+        if (++cur == n)
+            cur = 0;
+        caps++;
+    }
+    return caps;
+}
+
+// ---------------------------------------------------------------------------------------
 // Vertex transform and 1/256 snap (shaders/vkvg_main.vert:74-79 + viewport transform).
 #define VKH_INVALID_FIX ((int32_t)0x80000000)
 

@@ -237,6 +237,19 @@ extern "C" uint32_t vkvg_hip_flatten_points(const vkh_item *it, int stack, float
     return k;
 }
 
+// The dash walk of one segment as the dash-walk kernel runs it (whole periods taken at once where the
+// float additions are exact), computed on the host: state (on, cur, off) in and out; returns the caps.
+extern "C" uint32_t vkvg_hip_dash_walk(const float *dashes, uint32_t n, float seg_len, int *on, uint32_t *cur,
+                                       float *off) {
+    if (!n)
+        return 0;
+    const DashPeriod p = dash_period(dashes, n);
+    bool b = *on != 0;
+    const uint32_t caps = dash_walk_segment(dashes, n, p, seg_len, b, *cur, *off);
+    *on = b ? 1 : 0;
+    return caps;
+}
+
 extern "C" void vkvg_hip_item_extents(const vkh_item *it, float ext[4]) {
     auto widen = [&](v2 p) {
         if (p.x < ext[0])
@@ -618,19 +631,14 @@ __device__ void dash_piece(const StrokeCtx &c, bool closed, uint32_t k, DashStat
 // The dash state after piece k without emitting anything: dashed_segment's state updates only
 // (the join and caps it emits do not change the state), with the same float operations.
 // Returns the number of caps the piece emits (the loop's iterations).
-__device__ __forceinline__ uint32_t dash_walk_piece(const StrokeCtx &c, uint32_t k, DashState &dc) {
+__device__ __forceinline__ uint32_t dash_walk_piece(const StrokeCtx &c, const DashPeriod &per, uint32_t k,
+                                                   DashState &dc) {
     const uint32_t last = c.n - 1;
     const uint32_t cp = k < last ? k : last, iR = k < last ? k + 1 : 0;
     v2 d = vsub(c.P[iR], c.P[cp]);
     float segLen = vlen(d);
-    uint32_t caps = 0;
-    while (dc.curDashOffset < segLen) {
-        dc.dashOn = !dc.dashOn;
-        dc.curDashOffset += c.dashes[dc.curDash];
-        if (++dc.curDash == c.dashCount)
-            dc.curDash = 0;
-        caps++;
-    }
+    const uint32_t caps =
+        dash_walk_segment(c.dashes, c.dashCount, per, segLen, dc.dashOn, dc.curDash, dc.curDashOffset);
     dc.curDashOffset -= segLen;
     dc.curDashOffset = fmodf(dc.curDashOffset, dc.totDashLength);
     return caps;
@@ -766,11 +774,12 @@ __global__ void k_dash_walk(const Unit *__restrict__ units, uint32_t n_units, co
         return;
     StrokeCtx c = make_stroke_ctx(st, dashes, pts, pf, sp_first[un.subpath], sp_count[un.subpath]);
     stage_dashes(c, s_dash);
+    const DashPeriod per = dash_period(c.dashes, c.dashCount);
     // pieces of many caps join the list that k_dash_pieces takes a wavefront per entry from
     DashState dc = dash_start(c, st.dash_offset);
     for (uint32_t k = 0; k < s1 - s0; k++) {
         const DashState d0 = dc;
-        const uint32_t caps = dash_walk_piece(c, k, dc);
+        const uint32_t caps = dash_walk_piece(c, per, k, dc);
         const bool wave = caps >= VKH_DASH_WAVE_MIN;
         dash_st[s0 + k] = dash_pack(d0, wave);
         if (wave)
