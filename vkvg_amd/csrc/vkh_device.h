@@ -567,42 +567,6 @@ struct Edge {
     int32_t x0, y0, x1, y1;
 };
 
-// Per-draw bounding box of the edges and range [first, last + 1) of their indices, reduced across
-// a wavefront by the kernels that produce the edges: one set of atomics per distinct draw among
-// the lanes (edges of a draw are contiguous, so a wavefront usually holds one or two draws). A
-// lane passes the bbox and index range of its own edges and its draw, or VKH_NO_DRAW when it has
-// none. Every lane of the wavefront must call it.
-__device__ __forceinline__ void wave_draw_bbox(uint32_t d, int32_t x0, int32_t y0, int32_t x1, int32_t y1, uint32_t i0,
-                                               uint32_t i1, int4 *__restrict__ bbox, uint2 *__restrict__ erange) {
-    const uint32_t lane = threadIdx.x & 63;
-    uint64_t pending = __ballot(d != VKH_NO_DRAW);
-    while (pending) {
-        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1;
-        const uint32_t dl = __shfl(d, (int)leader);
-        const bool mine = d == dl;
-        int32_t a = mine ? x0 : INT32_MAX, b = mine ? y0 : INT32_MAX;
-        int32_t c = mine ? x1 : INT32_MIN, e = mine ? y1 : INT32_MIN;
-        uint32_t lo = mine ? i0 : 0xFFFFFFFFu, hi = mine ? i1 : 0u;
-        for (int o = 32; o > 0; o >>= 1) {
-            a = min(a, __shfl_xor(a, o));
-            b = min(b, __shfl_xor(b, o));
-            c = max(c, __shfl_xor(c, o));
-            e = max(e, __shfl_xor(e, o));
-            lo = min(lo, (uint32_t)__shfl_xor((int)lo, o));
-            hi = max(hi, (uint32_t)__shfl_xor((int)hi, o));
-        }
-        if (lane == leader) {
-            atomicMin(&bbox[dl].x, a);
-            atomicMin(&bbox[dl].y, b);
-            atomicMax(&bbox[dl].z, c);
-            atomicMax(&bbox[dl].w, e);
-            atomicMin(&erange[dl].x, lo);
-            atomicMax(&erange[dl].y, hi);
-        }
-        pending &= ~__ballot(mine);
-    }
-}
-
 // ---------------------------------------------------------------------------------------
 // float colour channel -> UNORM8 (clamped, rounded to nearest)
 __device__ __forceinline__ uint8_t f2u8(float f) {

@@ -147,7 +147,7 @@ struct vkh_engine {
     vkh_stats stats{};
 
     DevBuf blob, units, tgts, item_cnt, item_off, pts, pflags, sp_first, sp_count;
-    DevBuf fe, nv, nt, fe_off, nv_off, nt_off, verts, tris, edges, edraw;
+    DevBuf fe, nv, nt, fe_off, nv_off, nt_off, verts, tris, edges;
     DevBuf bbox, erange, dtiles, pt_count, pt_off, ptiles, counters, segs, dprep; // counters: [0] segment allocator, [1] flatten slots, [2] overflow cubics, [3] dashed pieces
     DevBuf tile_cnt, tile_off, tile_order, draw_order, cmds, scan_tmp, readback, item_slot, slot_item, flat_scratch, flat_perm, flat_ovf,
         flat_wh, dash_list, tl_table;
@@ -296,7 +296,7 @@ void vkvg_hip_engine_destroy(vkh_engine *e) {
         (void)hipGraphDestroy(e->graph);
     DevBuf *bufs[] = {&e->blob, &e->units, &e->tgts, &e->item_cnt, &e->item_off, &e->pts, &e->pflags,
                       &e->sp_first, &e->sp_count, &e->fe, &e->nv, &e->nt, &e->fe_off, &e->nv_off, &e->nt_off,
-                      &e->verts, &e->tris, &e->edges, &e->edraw, &e->bbox, &e->dtiles, &e->pt_count, &e->pt_off,
+                      &e->verts, &e->tris, &e->edges, &e->bbox, &e->dtiles, &e->pt_count, &e->pt_off,
                       &e->erange, &e->ptiles, &e->counters, &e->segs, &e->dprep, &e->tile_cnt,
                       &e->tile_off, &e->tile_order, &e->draw_order, &e->cmds, &e->scan_tmp, &e->readback, &e->step_nv, &e->step_nt,
                       &e->sv_off, &e->st_off, &e->dash_st, &e->u_toff, &e->u_vbeg, &e->u_vend, &e->item_slot,
@@ -771,8 +771,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     st.stroke_triangles = n_tris;
     st.edges = n_edges;
     if (!need(eng, cap, eng->verts, 8ull * n_verts + 8) || !need(eng, cap, eng->tris, 12ull * n_tris + 12) ||
-        !need(eng, cap, eng->edges, sizeof(Edge) * (size_t)n_edges + 16) ||
-        !need(eng, cap, eng->edraw, 4ull * n_edges + 4))
+        !need(eng, cap, eng->edges, sizeof(Edge) * (size_t)n_edges + 16))
         return -1;
     vkh::launch_stroke_steps(s, true, d_units, n_units, d_step_off, n_steps, d_draws, d_sp, P<uint32_t>(eng->sp_first),
                              P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<uint8_t>(eng->pflags), d_styles, d_dashes,
@@ -789,15 +788,15 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
         !need(eng, cap, eng->pt_off, dn4) || !need(eng, cap, eng->counters, 256))
         return -1;
     vkh::launch_draw_bbox_init(s, P<int4>(eng->bbox), P<uint2>(eng->erange), n_draws);
-    vkh::launch_unit_ranges(s, d_units, n_units, d_draws, d_step_off, P<uint32_t>(eng->sv_off),
+    vkh::launch_unit_ranges(s, d_units, n_units, d_draws, P<uint32_t>(eng->fe_off), d_step_off, P<uint32_t>(eng->sv_off),
                             P<uint32_t>(eng->st_off), P<uint32_t>(eng->u_toff), P<uint32_t>(eng->u_vbeg),
                             P<uint32_t>(eng->u_vend), n_fill_edges, P<uint2>(eng->erange));
     vkh::launch_fill_edges(s, d_units, n_units, P<uint32_t>(eng->fe_off), n_fill_edges, d_draws, d_tg,
                            P<uint32_t>(eng->sp_first), P<uint32_t>(eng->sp_count), P<v2>(eng->pts), P<Edge>(eng->edges),
-                           P<uint32_t>(eng->edraw), P<int4>(eng->bbox), P<uint2>(eng->erange));
+                           P<int4>(eng->bbox), P<uint2>(eng->erange));
     vkh::launch_tri_edges(s, d_units, n_units, P<uint32_t>(eng->u_toff), P<uint32_t>(eng->u_vbeg),
                           P<uint32_t>(eng->u_vend), n_tris, d_draws, d_tg, P<v2>(eng->verts), P<uint32_t>(eng->tris),
-                          P<Edge>(eng->edges), P<uint32_t>(eng->edraw), P<int4>(eng->bbox), P<uint2>(eng->erange));
+                          P<Edge>(eng->edges), P<int4>(eng->bbox), P<uint2>(eng->erange));
     CK(hipGetLastError());
 
     if (!cap)
@@ -846,7 +845,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
         const uint32_t seg_cap = (uint32_t)std::min<size_t>(eng->segs.cap / sizeof(Edge), 0xFFFFFFFFull);
         vkh::launch_zero_u32(s, P<uint32_t>(eng->counters), 1);
         vkh::launch_tile_cnt_init(s, d_tg, n_targets, nch, P<uint32_t>(eng->tl_table), P<uint32_t>(eng->tile_cnt), n_tc);
-        vkh::launch_bin_draws(s, P<Edge>(eng->edges), P<uint32_t>(eng->edraw), P<uint2>(eng->erange),
+        vkh::launch_bin_draws(s, P<Edge>(eng->edges), P<uint2>(eng->erange),
                               P<DrawTiles>(eng->dtiles), n_draws, T, P<PathTile>(eng->ptiles), P<Edge>(eng->segs),
                               seg_cap, P<uint32_t>(eng->counters), P<uint32_t>(eng->draw_order), d_draws, d_tg, nch,
                               eng->coarse_chunk, P<uint32_t>(eng->tile_cnt));

@@ -1031,14 +1031,15 @@ __global__ void k_stroke_patch(const Unit *__restrict__ units, uint32_t n_units,
 }
 
 // per unit: first triangle and vertex range of its stroke steps
-// Per unit: its first triangle and its vertex range. The first unit of each stroke draw also opens
-// the draw's edge range at the slot of its first triangle's first edge: k_tri_edges appends the
-// draw's kept edges there (erange[d].y is the append cursor), so the range holds no dropped edges.
+// Per unit: its first triangle and its vertex range. The first unit of each draw also opens the
+// draw's edge range at its first edge slot (its first fill edge, or its first triangle's first
+// edge after the ebase fill edges): the edge kernels append the draw's kept edges there
+// (erange[d].y is the append cursor), so the range holds no dropped edges.
 __global__ void k_unit_ranges(const Unit *__restrict__ units, uint32_t n_units, const vkh_draw *__restrict__ draws,
-                              const uint32_t *__restrict__ step_off, const uint32_t *__restrict__ sv_off,
-                              const uint32_t *__restrict__ st_off, uint32_t *__restrict__ u_toff,
-                              uint32_t *__restrict__ u_vbeg, uint32_t *__restrict__ u_vend, uint32_t ebase,
-                              uint2 *__restrict__ erange) {
+                              const uint32_t *__restrict__ fill_off, const uint32_t *__restrict__ step_off,
+                              const uint32_t *__restrict__ sv_off, const uint32_t *__restrict__ st_off,
+                              uint32_t *__restrict__ u_toff, uint32_t *__restrict__ u_vbeg,
+                              uint32_t *__restrict__ u_vend, uint32_t ebase, uint2 *__restrict__ erange) {
     uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u > n_units)
         return;
@@ -1049,8 +1050,10 @@ __global__ void k_unit_ranges(const Unit *__restrict__ units, uint32_t n_units, 
         u_vbeg[u] = sv_off[s0];
         u_vend[u] = sv_off[step_off[u + 1]];
         const uint32_t d = units[u].draw;
-        if (draws[d].kind == VKH_DRAW_STROKE && (u == 0 || units[u - 1].draw != d))
-            erange[d] = make_uint2(ebase + 3u * t0, ebase + 3u * t0);
+        if (u == 0 || units[u - 1].draw != d) {
+            const uint32_t b = draws[d].kind == VKH_DRAW_STROKE ? ebase + 3u * t0 : fill_off[u];
+            erange[d] = make_uint2(b, b);
+        }
     }
 }
 
@@ -1108,42 +1111,6 @@ __device__ __forceinline__ uint32_t wave_entry_of(const uint32_t *__restrict__ a
     return u0 + k;
 }
 
-__global__ void k_fill_edges(const Unit *__restrict__ units, uint32_t n_units, const uint32_t *__restrict__ fill_off,
-                             uint32_t total, const vkh_draw *__restrict__ draws, const TargetInfo *__restrict__ tgts,
-                             const uint32_t *__restrict__ sp_first, const uint32_t *__restrict__ sp_count,
-                             const v2 *__restrict__ pts, Edge *__restrict__ edges, uint32_t *__restrict__ edraw,
-                             int4 *__restrict__ bbox, uint2 *__restrict__ erange) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t dr = VKH_NO_DRAW;
-    Edge e{0, 0, 0, 0};
-    const uint32_t u = wave_entry_of(fill_off, n_units, i);
-    if (i < total) {
-        Unit un = units[u];
-        const vkh_draw &d = draws[un.draw];
-        uint32_t j = i - fill_off[u];
-        uint32_t n = sp_count[un.subpath];
-        uint32_t f = sp_first[un.subpath];
-        v2 a = pts[f + j], b = pts[f + (j + 1 == n ? 0 : j + 1)];
-        float sw = (float)tgts[d.target].width, sh = (float)tgts[d.target].height;
-        snap_vertex(a.x, a.y, d.mat, sw, sh, e.x0, e.y0);
-        snap_vertex(b.x, b.y, d.mat, sw, sh, e.x1, e.y1);
-        bool bad = e.x0 == VKH_INVALID_FIX || e.x1 == VKH_INVALID_FIX || (e.x0 == e.x1 && e.y0 == e.y1);
-        dr = bad ? VKH_NO_DRAW : un.draw;
-        edges[i] = e;
-        edraw[i] = dr;
-    }
-    wave_draw_bbox(dr, min(e.x0, e.x1), min(e.y0, e.y1), max(e.x0, e.x1), max(e.y0, e.y1), i, i + 1, bbox, erange);
-}
-
-__device__ __forceinline__ bool tri_has_edge(const int32_t (&N)[3][4], int32_t x0, int32_t y0, int32_t x1,
-                                              int32_t y1) {
-    bool h = false;
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-        h |= N[k][0] == x0 && N[k][1] == y0 && N[k][2] == x1 && N[k][3] == y1;
-    return h;
-}
-
 // Appends each lane's n edges of draw d to the draw's edge range (erange[d].y is its cursor) and
 // reduces the draw's bbox: one set of atomics per distinct draw among the lanes, the lanes of a
 // draw placed by a wave-wide prefix of their counts. Returns the lane's first slot. Lanes without
@@ -1190,6 +1157,46 @@ __device__ __forceinline__ uint32_t wave_draw_append(uint32_t d, uint32_t n, int
     return at;
 }
 
+// Fill edges, one per point of each sub-path (the point to its successor, closing); a degenerate
+// edge or one with a vertex that does not snap is dropped. The kept edges are appended to their
+// draw's range (opened by k_unit_ranges at the draw's first edge slot).
+__global__ void k_fill_edges(const Unit *__restrict__ units, uint32_t n_units, const uint32_t *__restrict__ fill_off,
+                             uint32_t total, const vkh_draw *__restrict__ draws, const TargetInfo *__restrict__ tgts,
+                             const uint32_t *__restrict__ sp_first, const uint32_t *__restrict__ sp_count,
+                             const v2 *__restrict__ pts, Edge *__restrict__ edges, int4 *__restrict__ bbox,
+                             uint2 *__restrict__ erange) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t dr = VKH_NO_DRAW;
+    Edge e{0, 0, 0, 0};
+    const uint32_t u = wave_entry_of(fill_off, n_units, i);
+    if (i < total) {
+        Unit un = units[u];
+        const vkh_draw &d = draws[un.draw];
+        uint32_t j = i - fill_off[u];
+        uint32_t n = sp_count[un.subpath];
+        uint32_t f = sp_first[un.subpath];
+        v2 a = pts[f + j], b = pts[f + (j + 1 == n ? 0 : j + 1)];
+        float sw = (float)tgts[d.target].width, sh = (float)tgts[d.target].height;
+        snap_vertex(a.x, a.y, d.mat, sw, sh, e.x0, e.y0);
+        snap_vertex(b.x, b.y, d.mat, sw, sh, e.x1, e.y1);
+        bool bad = e.x0 == VKH_INVALID_FIX || e.x1 == VKH_INVALID_FIX || (e.x0 == e.x1 && e.y0 == e.y1);
+        dr = bad ? VKH_NO_DRAW : un.draw;
+    }
+    const uint32_t at = wave_draw_append(dr, dr != VKH_NO_DRAW ? 1u : 0u, min(e.x0, e.x1), min(e.y0, e.y1),
+                                         max(e.x0, e.x1), max(e.y0, e.y1), bbox, erange);
+    if (dr != VKH_NO_DRAW)
+        edges[at] = e;
+}
+
+__device__ __forceinline__ bool tri_has_edge(const int32_t (&N)[3][4], int32_t x0, int32_t y0, int32_t x1,
+                                              int32_t y1) {
+    bool h = false;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        h |= N[k][0] == x0 && N[k][1] == y0 && N[k][2] == x1 && N[k][3] == y1;
+    return h;
+}
+
 // Stroke triangles -> 3 edges each, oriented so every triangle winds +1 (ray to -x, +y edges
 // count +1; area > 0 is traversed 0 -> 2 -> 1); indices outside the sub-path's own vertices
 // (never drawn by a valid stream) or a degenerate triangle produce no edge. A stroke's coverage count is the sum of its triangles'
@@ -1208,8 +1215,7 @@ __global__ __launch_bounds__(256) void k_tri_edges(const Unit *__restrict__ unit
                                                    const vkh_draw *__restrict__ draws,
                                                    const TargetInfo *__restrict__ tgts, const v2 *__restrict__ verts,
                                                    const uint32_t *__restrict__ tris, Edge *__restrict__ edges,
-                                                   uint32_t *__restrict__ edraw, int4 *__restrict__ bbox,
-                                                   uint2 *__restrict__ erange) {
+                                                   int4 *__restrict__ bbox, uint2 *__restrict__ erange) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = t < total_tris;
     uint32_t draw = VKH_NO_DRAW;
@@ -1303,7 +1309,6 @@ __global__ __launch_bounds__(256) void k_tri_edges(const Unit *__restrict__ unit
     for (int k = 0; k < 3; k++)
         if (kmask & (1u << k)) {
             edges[at] = Edge{E[k][0], E[k][1], E[k][2], E[k][3]};
-            edraw[at] = draw;
             at++;
         }
 }
@@ -1397,24 +1402,25 @@ void launch_stroke_patch(hipStream_t s, const Unit *units, uint32_t n_units, con
                            draws, sps, spf, spc, pts, pf, styles, dashes, sv_off, st_off, tris);
 }
 void launch_unit_ranges(hipStream_t s, const Unit *units, uint32_t n_units, const vkh_draw *draws,
-                        const uint32_t *step_off, const uint32_t *sv_off, const uint32_t *st_off, uint32_t *u_toff,
-                        uint32_t *u_vbeg, uint32_t *u_vend, uint32_t ebase, uint2 *erange) {
+                        const uint32_t *fill_off, const uint32_t *step_off, const uint32_t *sv_off,
+                        const uint32_t *st_off, uint32_t *u_toff, uint32_t *u_vbeg, uint32_t *u_vend, uint32_t ebase,
+                        uint2 *erange) {
     hipLaunchKernelGGL(k_unit_ranges, dim3((n_units + 1 + 255) / 256), dim3(256), 0, s, units, n_units, draws,
-                       step_off, sv_off, st_off, u_toff, u_vbeg, u_vend, ebase, erange);
+                       fill_off, step_off, sv_off, st_off, u_toff, u_vbeg, u_vend, ebase, erange);
 }
 void launch_fill_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *fill_off, uint32_t total,
                        const vkh_draw *draws, const TargetInfo *tg, const uint32_t *spf, const uint32_t *spc,
-                       const v2 *pts, Edge *edges, uint32_t *edraw, int4 *bbox, uint2 *erange) {
+                       const v2 *pts, Edge *edges, int4 *bbox, uint2 *erange) {
     if (total)
         hipLaunchKernelGGL(k_fill_edges, dim3((total + 255) / 256), dim3(256), 0, s, units, n_units, fill_off, total,
-                           draws, tg, spf, spc, pts, edges, edraw, bbox, erange);
+                           draws, tg, spf, spc, pts, edges, bbox, erange);
 }
 void launch_tri_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *u_toff,
                       const uint32_t *u_vbeg, const uint32_t *u_vend, uint32_t total_tris, const vkh_draw *draws,
-                      const TargetInfo *tg, const v2 *verts, const uint32_t *tris, Edge *edges, uint32_t *edraw,
-                      int4 *bbox, uint2 *erange) {
+                      const TargetInfo *tg, const v2 *verts, const uint32_t *tris, Edge *edges, int4 *bbox,
+                      uint2 *erange) {
     if (total_tris)
         hipLaunchKernelGGL(k_tri_edges, dim3((total_tris + 255) / 256), dim3(256), 0, s, units, n_units, u_toff,
-                           u_vbeg, u_vend, total_tris, draws, tg, verts, tris, edges, edraw, bbox, erange);
+                           u_vbeg, u_vend, total_tris, draws, tg, verts, tris, edges, bbox, erange);
 }
 } // namespace vkh

@@ -149,15 +149,16 @@ void launch_stroke_patch(hipStream_t s, const Unit *units, uint32_t n_units, con
                          const v2 *pts, const uint8_t *pf, const vkh_stroke_style *styles, const float *dashes,
                          const uint32_t *sv_off, const uint32_t *st_off, uint32_t *tris);
 void launch_unit_ranges(hipStream_t s, const Unit *units, uint32_t n_units, const vkh_draw *draws,
-                        const uint32_t *step_off, const uint32_t *sv_off, const uint32_t *st_off, uint32_t *u_toff,
-                        uint32_t *u_vbeg, uint32_t *u_vend, uint32_t ebase, uint2 *erange);
+                        const uint32_t *fill_off, const uint32_t *step_off, const uint32_t *sv_off,
+                        const uint32_t *st_off, uint32_t *u_toff, uint32_t *u_vbeg, uint32_t *u_vend, uint32_t ebase,
+                        uint2 *erange);
 void launch_fill_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *fill_off, uint32_t total,
                        const vkh_draw *draws, const TargetInfo *tg, const uint32_t *spf, const uint32_t *spc,
-                       const v2 *pts, Edge *edges, uint32_t *edraw, int4 *bbox, uint2 *erange);
+                       const v2 *pts, Edge *edges, int4 *bbox, uint2 *erange);
 void launch_tri_edges(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *u_toff,
                       const uint32_t *u_vbeg, const uint32_t *u_vend, uint32_t total_tris, const vkh_draw *draws,
-                      const TargetInfo *tg, const v2 *verts, const uint32_t *tris, Edge *edges, uint32_t *edraw,
-                      int4 *bbox, uint2 *erange);
+                      const TargetInfo *tg, const v2 *verts, const uint32_t *tris, Edge *edges, int4 *bbox,
+                      uint2 *erange);
 
 // raster stage (vkh_raster.hip)
 void launch_draw_bbox_init(hipStream_t s, int4 *bbox, uint2 *erange, uint32_t n_draws);
@@ -165,7 +166,7 @@ void launch_draw_tiles(hipStream_t s, const vkh_draw *draws, uint32_t n_draws, c
                        const int4 *bbox, uint32_t tile, DrawTiles *dt, uint32_t *pt_count);
 void launch_set_pt_base(hipStream_t s, DrawTiles *dt, const uint32_t *pt_off, uint32_t n_draws);
 void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
-void launch_bin_draws(hipStream_t s, const Edge *edges, const uint32_t *edraw, const uint2 *erange,
+void launch_bin_draws(hipStream_t s, const Edge *edges, const uint2 *erange,
                       const DrawTiles *dt, uint32_t n_draws, uint32_t tile, PathTile *pts, Edge *segs,
                       uint32_t seg_cap, uint32_t *seg_alloc, const uint32_t *order, const vkh_draw *draws,
                       const TargetInfo *tg, uint32_t nch, uint32_t chunk_draws, uint32_t *tile_cnt);

@@ -121,6 +121,8 @@ __device__ __forceinline__ void bin_edge(const Edge &e, const DrawTiles &dt, int
     const int32_t ymin = min(e.y0, e.y1), ymax = max(e.y0, e.y1);
     const int32_t ty0 = max(ymin >> ssh, ty_lo);
     const int32_t ty1 = min(ymax >> ssh, ty_hi);
+    if (ty0 > ty1) // outside the band of tile rows
+        return;
     const bool horiz = e.y0 == e.y1;
     const double invdy = horiz ? 0.0 : 1.0 / ((double)e.y1 - (double)e.y0);
     const int dir = e.y0 < e.y1 ? 1 : -1;
@@ -202,8 +204,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t *a, uint32_t n
 // non-zero blend count from its backdrop, or a clip) adds one to its tile's command count in the
 // draw's chunk (tile_cnt[tile * nch + chunk], the coarse pass's layout), so the coarse pass needs
 // no counting pass of its own.
-__global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges, const uint32_t *__restrict__ edraw,
-                                                  const uint2 *__restrict__ erange,
+__global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges, const uint2 *__restrict__ erange,
                                                   const DrawTiles *__restrict__ dts, uint32_t T,
                                                   PathTile *__restrict__ pts, Edge *__restrict__ segs,
                                                   uint32_t seg_cap, uint32_t *__restrict__ seg_alloc,
@@ -226,17 +227,16 @@ __global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges
     const TargetInfo &TI = tg[dr.target];
     const uint32_t chunk = (d - TI.draw_begin) / chunk_draws;
     const uint32_t bw = dt.bw, bh = (uint32_t)(dt.by1 - dt.by0 + 1);
-    // The draw's edges, VKH_BIN_U per thread per step with every load issued before any is used.
-    // A draw of at most 256 * VKH_BIN_U edges (most of them) keeps its edges in registers for both
-    // passes of every band; a larger one streams them each pass.
+    // The draw's edges (its range holds only its own, kept edges), VKH_BIN_U per thread per step
+    // with every load issued before any is used. A draw of at most 256 * VKH_BIN_U edges (most of
+    // them) keeps its edges in registers for both passes of every band; a larger one streams them
+    // each pass.
     const bool cached = er.y - er.x <= 256u * VKH_BIN_U;
-    uint32_t cdr[VKH_BIN_U];
     Edge ce[VKH_BIN_U];
     auto load = [&](uint32_t b0) {
 #pragma unroll
         for (int u = 0; u < VKH_BIN_U; u++) {
             const uint32_t i = b0 + (uint32_t)u * 256u + tid;
-            cdr[u] = i < er.y ? edraw[i] : VKH_NO_DRAW;
             ce[u] = i < er.y ? edges[i] : Edge{0, 0, 0, 0};
         }
     };
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges
         if (cached) {
 #pragma unroll
             for (int u = 0; u < VKH_BIN_U; u++)
-                if (cdr[u] == d)
+                if (er.x + (uint32_t)u * 256u + tid < er.y)
                     f(ce[u]);
             return;
         }
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(256) void k_bin_draw(const Edge *__restrict__ edges
             load(b0);
 #pragma unroll
             for (int u = 0; u < VKH_BIN_U; u++)
-                if (cdr[u] == d)
+                if (b0 + (uint32_t)u * 256u + tid < er.y)
                     f(ce[u]);
         }
     };
@@ -1201,12 +1201,12 @@ void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n) {
     if (n)
         hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
 }
-void launch_bin_draws(hipStream_t s, const Edge *edges, const uint32_t *edraw, const uint2 *erange,
+void launch_bin_draws(hipStream_t s, const Edge *edges, const uint2 *erange,
                       const DrawTiles *dt, uint32_t n_draws, uint32_t tile, PathTile *pts, Edge *segs,
                       uint32_t seg_cap, uint32_t *seg_alloc, const uint32_t *order, const vkh_draw *draws,
                       const TargetInfo *tg, uint32_t nch, uint32_t chunk_draws, uint32_t *tile_cnt) {
     if (n_draws)
-        hipLaunchKernelGGL(k_bin_draw, dim3(n_draws), dim3(256), 0, s, edges, edraw, erange, dt, tile, pts, segs,
+        hipLaunchKernelGGL(k_bin_draw, dim3(n_draws), dim3(256), 0, s, edges, erange, dt, tile, pts, segs,
                            seg_cap, seg_alloc, order, draws, tg, nch, chunk_draws, tile_cnt);
 }
 void launch_tile_cnt_init(hipStream_t s, const TargetInfo *tg, uint32_t n_targets, uint32_t nch,
