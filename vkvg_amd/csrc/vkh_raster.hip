@@ -842,7 +842,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
     __shared__ uint32_t c_item[B + 1];            // prefix of the commands' segment counts
     __shared__ uint32_t c_nseg[B], c_segoff[B], c_draw[B];
     __shared__ int32_t c_back[B];                 // path tile backdrop (the row prefix adds it)
-    __shared__ uint4 c_rec[B];  // meta, key, winding added in compositing (the backdrop without segments), solid
+    __shared__ uint4 c_rec[B];  // meta, key, winding added in compositing (the backdrop without segments),
+                                // covering winding bits (the parameter of a stencil draw)
     __shared__ uint4 c_src[B];  // blend operands of a solid source (vkh_src16)
 
     const uint32_t gtile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
@@ -931,7 +932,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
                 if (kind <= VKH_DRAW_CLEAR)
                     meta = (meta & ~15u) | (kind == VKH_DRAW_CLEAR ? VKH_DRAW_CLEAR : VKH_DRAW_PAINT);
                 c_back[j] = 0;
-                c_rec[j].z = 0u;
+                c_rec[j].z = kind <= VKH_DRAW_CLEAR ? 1u : 0u; // paint and clear: winding 1 everywhere
                 c_nseg[j] = 0;
                 c_segoff[j] = 0;
             } else {
@@ -945,7 +946,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
             }
             c_rec[j].x = meta;
             c_rec[j].y = d.key;
-            c_rec[j].w = d.solid;
+            // the winding bits that make a sample covered: odd for even-odd fills, any for non-zero
+            // fills, strokes (whose winding, a count of covering triangles, is never negative), paint
+            // and clear; the stencil draws read their parameter instead
+            c_rec[j].w = kind >= VKH_DRAW_CLIP_NZ ? d.solid : kind == VKH_DRAW_FILL_EO ? 1u : 0xFFFFFFFFu;
             const vkh_src16 S = vkh_src16_of((meta >> 4) & 15, d.solid);
             c_src[j] = make_uint4(vkh_u32_of(S.ia), vkh_u32_of(S.b_lo), vkh_u32_of(S.b_hi), S.x);
             c_draw[j] = cm.draw;
@@ -1058,6 +1062,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
                 continue;
             }
             const bool stroke = kind == VKH_DRAW_STROKE;
+            const uint32_t rule = rec.w; // covering winding bits
             int32_t wq[PPT];
             bool on[PPT];
 #pragma unroll
@@ -1066,10 +1071,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void k_fi
                 const int32_t w = sd[j][ly[q]][lx[q]] + bk;
                 sd[j][ly[q]][lx[q]] = 0;
                 wq[q] = w;
-                on[q] = kind == VKH_DRAW_FILL_NZ ? w != 0
-                        : kind == VKH_DRAW_FILL_EO ? (w & 1) != 0
-                        : stroke ? w > 0 : true; // paint, clear
-                on[q] = on[q] && valid[q];
+                on[q] = ((uint32_t)w & rule) != 0u && valid[q];
                 if (EXT && on[q]) {
                     if (kind == VKH_DRAW_CLEAR)
                         st[q] &= ~0xFFu; // vkCmdClearAttachments ignores the stencil test
