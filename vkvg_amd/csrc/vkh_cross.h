@@ -16,6 +16,20 @@
 #define VKH_CR static inline
 #endif
 
+// n / d for the quotients below, each of which is followed by an exact correction of one step in
+// either direction, so it only has to be within one of the exact quotient: a single-precision
+// reciprocal (relative error below 2^-21, with d rounded to float) refined by one Newton step
+// (below 2^-40), times n. The quotients stay below 2^25 in magnitude, so the error stays below
+// 2^-15. Cheaper than the correctly rounded division, which the corrections make unnecessary.
+VKH_CR double cross_quot(double n, double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = (double)__builtin_amdgcn_rcpf((float)d);
+#else
+    double r = (double)(1.0f / (float)d);
+#endif
+    r = fma(fma(-d, r, 1.0), r, r);
+    return n * r;
+}
 // Vertical crossing. For an edge crossing the sample column xs (lx <= xs < rx, (lx, ly) its left
 // end and (rx, ry) its right end), the sample (xs, y) lies above the edge when
 // E(y) = (rx - lx)(y - ly) - (ry - ly)(xs - lx) < 0, or E(y) = 0 and ry > ly (the tie-break that
@@ -25,11 +39,14 @@ VKH_CR int32_t cross_first_row(int32_t lx, int32_t ly, int32_t rx, int32_t ry, i
                                int32_t hi) {
     const double A = (double)(rx - lx);                       // in (0, 2^24]
     const double C = (double)(ry - ly) * (double)(xs - lx);   // |C| < 2^48
-    double f = floor(C / A); // |C / A| < |ry - ly| <= 2^24 keeps this exact; corrected anyway
+    double f = floor(cross_quot(C, A)); // |C / A| < |ry - ly| <= 2^24; within one, corrected
     double rem = fma(-f, A, C);
     if (rem < 0.0) {
         f -= 1.0;
         rem += A;
+    } else if (rem >= A) {
+        f += 1.0;
+        rem -= A;
     }
     // least u = y - ly that is not above: A u > C, or A u == C when ry <= ly
     const double t = (ry <= ly && rem == 0.0) ? f : f + 1.0;
@@ -53,7 +70,7 @@ VKH_CR double cross_column_rem(int32_t xt, int32_t yt, int32_t xb, int32_t yb, i
     const double dy = (double)(yb - yt);
     const double D = 256.0 * dy;
     const double N = fma(dy, (double)(xt - 128), (double)(xb - xt) * (double)(y - yt)); // exact, < 2^50
-    double q = ceil(N / D); // off by at most one
+    double q = ceil(cross_quot(N, D)); // off by at most one
     double rem = fma(q, D, -N);
     if (rem < 0.0) {
         q += 1.0;
@@ -78,7 +95,7 @@ VKH_CR vkh_dda cross_dda_start(int32_t xt, int32_t yt, int32_t xb, int32_t yb, i
     double rem;
     const double q = cross_column_rem(xt, yt, xb, yb, y, &rem);
     const double dN = 256.0 * (double)(xb - xt);
-    double dq = floor(dN / D);
+    double dq = floor(cross_quot(dN, D));
     double dr = fma(-dq, D, dN);
     if (dr < 0.0) {
         dq -= 1.0;
