@@ -570,9 +570,9 @@ __device__ void dashed_segment(const StrokeCtx &c, S &s, DashState &dc, uint32_t
 // the dashed segment from point k (iL is the last point for k = 0, as there), a closed path adds
 // the closing segment as piece `last`, and the last piece also emits the final cap. Run one after
 // another over all pieces, dash_start / dash_piece perform the reference's operations in its order;
-// k_dash_walk records the dash state at each piece's start so that the pieces can be counted and
-// emitted independently: by one thread each (k_stroke_steps), or, for a piece of many caps, by a
-// wavefront (k_dash_pieces).
+// k_dash_walk records the dash state at each piece's start so that the pieces can be counted (one
+// thread each, dash_piece_count) and emitted independently: by one thread each (k_stroke_steps),
+// or, for a piece of many caps, by a wavefront (k_dash_pieces).
 __device__ __forceinline__ uint32_t dash_pieces(uint32_t n, bool closed) { return n < 2 ? 0u : (closed ? n : n - 1); }
 
 __device__ __forceinline__ DashState dash_start(const StrokeCtx &c, float dashOffset) {
@@ -642,6 +642,36 @@ __device__ __forceinline__ uint32_t dash_walk_piece(const StrokeCtx &c, const Da
     dc.curDashOffset -= segLen;
     dc.curDashOffset = fmodf(dc.curDashOffset, dc.totDashLength);
     return caps;
+}
+
+// The vertex and index counts of piece k (dash_piece's) from its start state, without running its
+// caps one by one: a cap's counts depend only on whether it starts or ends a dash and on the
+// piece's direction (a round cap's arc steps run from the direction's angle; the cap's position
+// does not enter), the caps alternate between the two from the start state, and the dash walk
+// gives their number and the state the piece ends in.
+__device__ void dash_piece_count(const StrokeCtx &c, const DashPeriod &per, bool closed, uint32_t k, DashState dc,
+                                 CountSink &s) {
+    uint32_t iL, cp, iR;
+    bool curved;
+    dash_piece_points(c, k, iL, cp, iR, curved);
+    if (!dc.dashOn)
+        vb_step(c, s, iL, cp, iR, curved);
+    const v2 d = vsub(c.P[iR], c.P[cp]);
+    const v2 n = vnorm(d);
+    const float segLen = vlen(d);
+    const bool on0 = dc.dashOn;
+    const uint32_t caps =
+        dash_walk_segment(c.dashes, c.dashCount, per, segLen, dc.dashOn, dc.curDash, dc.curDashOffset);
+    CountSink cs, ce;
+    stroke_cap(c, cs, V(0.f, 0.f), n, true);
+    stroke_cap(c, ce, V(0.f, 0.f), n, false);
+    const uint32_t starts = on0 ? (caps + 1) / 2 : caps / 2, ends = caps - starts;
+    s.nv += starts * cs.nv + ends * ce.nv;
+    s.ni += starts * cs.ni + ends * ce.ni;
+    if (k + 1 == dash_pieces(c.n, closed) && !dc.dashOn) { // the final cap ends a dash
+        s.nv += ce.nv;
+        s.ni += ce.ni;
+    }
 }
 
 // A piece's start state: dashOn in bit 0, the dash index above it, and in bit 31 whether the piece
@@ -817,8 +847,7 @@ __device__ __forceinline__ uint32_t wave_exclusive(uint32_t v, uint32_t lane, ui
 // from running it against a counting sink; their wave-wide prefix places it, so the caps land where
 // the sequential emission puts them and their relative vertex references (a start cap refers to
 // the next cap's first two vertices) resolve to the same vertices. The join before the first cap
-// and the final cap are emitted by lane 0. EMIT = false records the piece's totals.
-template <bool EMIT>
+// and the final cap are emitted by lane 0. (The piece's totals come from dash_piece_count.)
 __global__ __launch_bounds__(256) void k_dash_pieces(const Unit *__restrict__ units, uint32_t n_units,
                                                      const uint32_t *__restrict__ step_off,
                                                      const vkh_draw *__restrict__ draws,
@@ -850,12 +879,12 @@ __global__ __launch_bounds__(256) void k_dash_pieces(const Unit *__restrict__ un
         bool curved;
         dash_piece_points(c, k, iL, cp, iR, curved);
         DashState dc0 = dash_unpack(c, st.dash_offset, dash_st[sidx]);
-        uint32_t nv = EMIT ? sv_off[sidx] : 0u, ni = EMIT ? 3u * st_off[sidx] : 0u;
+        uint32_t nv = sv_off[sidx], ni = 3u * st_off[sidx];
         const v2 p = c.P[cp], pR = c.P[iR];
         if (!dc0.dashOn) { // the join at the piece's start point
             CountSink cs;
             vb_step(c, cs, iL, cp, iR, curved);
-            if (EMIT && lane == 0) {
+            if (lane == 0) {
                 EmitSink es;
                 es.verts = verts;
                 es.nv = nv;
@@ -888,7 +917,7 @@ __global__ __launch_bounds__(256) void k_dash_pieces(const Unit *__restrict__ un
                 stroke_cap(c, cs, p0, my.normal, my.dashOn);
             uint32_t tv, ti;
             const uint32_t ov = wave_exclusive(cs.nv, lane, &tv), oi = wave_exclusive(cs.ni, lane, &ti);
-            if (EMIT && act) {
+            if (act) {
                 EmitSink es;
                 es.verts = verts;
                 es.nv = nv + ov;
@@ -917,7 +946,7 @@ __global__ __launch_bounds__(256) void k_dash_pieces(const Unit *__restrict__ un
             CountSink cs;
             DashState f1 = fin;
             dash_final_cap(c, cs, f1, iR);
-            if (EMIT && lane == 0) {
+            if (lane == 0) {
                 EmitSink es;
                 es.verts = verts;
                 es.nv = nv;
@@ -928,10 +957,6 @@ __global__ __launch_bounds__(256) void k_dash_pieces(const Unit *__restrict__ un
             }
             nv += cs.nv;
             ni += cs.ni;
-        }
-        if (!EMIT && lane == 0) {
-            step_nv[sidx] = nv;
-            step_nt[sidx] = ni / 3;
         }
     }
 }
@@ -957,8 +982,8 @@ __global__ void k_stroke_steps(const Unit *__restrict__ units, uint32_t n_units,
     uint2 ds = make_uint2(0u, 0u);
     if (st.dash_count) {
         ds = dash_st[sidx];
-        if (ds.x & VKH_DASH_WAVE_BIT)
-            return; // a piece of many caps: k_dash_pieces
+        if (EMIT && (ds.x & VKH_DASH_WAVE_BIT))
+            return; // a piece of many caps: emitted by k_dash_pieces
     }
     StrokeCtx c = make_stroke_ctx(st, dashes, pts, pf, sp_first[un.subpath], sp_count[un.subpath]);
     stage_dashes(c, s_dash);
@@ -966,12 +991,11 @@ __global__ void k_stroke_steps(const Unit *__restrict__ units, uint32_t n_units,
     uint32_t i = sidx - step_off[u];
     if (!EMIT) {
         CountSink k;
-        if (st.dash_count) {
-            DashState dc = dash_unpack(c, st.dash_offset, ds);
-            dash_piece(c, closed, i, dc, k);
-        } else {
+        if (st.dash_count)
+            dash_piece_count(c, dash_period(c.dashes, c.dashCount), closed, i,
+                             dash_unpack(c, st.dash_offset, ds), k);
+        else
             stroke_step(c, closed, i, k);
-        }
         step_nv[sidx] = k.nv;
         step_nt[sidx] = k.ni / 3;
     } else {
@@ -1381,16 +1405,13 @@ void launch_stroke_steps(hipStream_t s, bool emit, const Unit *units, uint32_t n
         hipLaunchKernelGGL(k_stroke_steps<true>, dim3((n_steps + 255) / 256), dim3(256), 0, s, units, n_units, step_off,
                            n_steps, draws, sps, spf, spc, pts, pf, styles, dashes, step_nv, step_nt, sv_off, st_off,
                            verts, tris, dash_st);
-        hipLaunchKernelGGL(k_dash_pieces<true>, dim3(dblocks), dim3(256), 0, s, units, n_units, step_off, draws, sps,
-                           spf, spc, pts, pf, styles, dashes, step_nv, step_nt, sv_off, st_off, verts, tris, dash_st,
-                           dlist, dcount);
+        hipLaunchKernelGGL(k_dash_pieces, dim3(dblocks), dim3(256), 0, s, units, n_units, step_off, draws, sps, spf,
+                           spc, pts, pf, styles, dashes, step_nv, step_nt, sv_off, st_off, verts, tris, dash_st, dlist,
+                           dcount);
     } else {
         hipLaunchKernelGGL(k_stroke_steps<false>, dim3((n_steps + 255) / 256), dim3(256), 0, s, units, n_units,
                            step_off, n_steps, draws, sps, spf, spc, pts, pf, styles, dashes, step_nv, step_nt, sv_off,
                            st_off, verts, tris, dash_st);
-        hipLaunchKernelGGL(k_dash_pieces<false>, dim3(dblocks), dim3(256), 0, s, units, n_units, step_off, draws, sps,
-                           spf, spc, pts, pf, styles, dashes, step_nv, step_nt, sv_off, st_off, verts, tris, dash_st,
-                           dlist, dcount);
     }
 }
 void launch_stroke_patch(hipStream_t s, const Unit *units, uint32_t n_units, const uint32_t *step_off,
