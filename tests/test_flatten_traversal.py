@@ -40,7 +40,7 @@ def _cases():
     rng = np.random.default_rng(11)
     out = []
     for k in range(600):
-        scale = float(10.0 ** rng.uniform(-1, 4.5))  # up to ~30 000 px: subdivision past 8 levels
+        scale = float(10.0 ** rng.uniform(-1, 4.5))  # up to ~30 000 px: subdivision past the stacked levels
         p = (rng.uniform(-1, 1, 8) * scale + rng.uniform(-500, 500, 2).repeat(4).reshape(2, 4).T.reshape(-1)) \
             .astype(np.float32)
         if k % 10 == 0:
@@ -66,6 +66,6 @@ def test_stack_and_stackless_traversals_emit_the_oracle_points(L):
         assert np.array_equal(got_stack.view(np.uint32), want.view(np.uint32)), (k, p, tol)
         assert np.array_equal(got_plain.view(np.uint32), want.view(np.uint32)), (k, p, tol)
         deepest = max(deepest, len(want))
-    # some cubic had more points than a tree of 8 levels has leaves: its nodes below the stacked
-    # levels were recomputed from the deepest stacked ancestor
+    # some cubic had more points than a tree of 8 levels (more than the kernel stacks) has leaves:
+    # its nodes below the stacked levels were recomputed from the deepest stacked ancestor
     assert deepest > 256

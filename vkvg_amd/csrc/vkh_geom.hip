@@ -80,8 +80,8 @@ __global__ __launch_bounds__(256) void k_flat_scatter(const vkh_item *__restrict
         perm[wcur[(size_t)key * n_wg + blockIdx.x] + rank] = i;
 }
 
-#define VKH_FLAT_STACK 8   // ancestor levels kept in LDS per thread by the subdivision pass
-#define VKH_FLAT_BLOCK 64 // threads per workgroup of the subdivision pass: one wavefront, 16 KB of stacks
+#define VKH_FLAT_STACK 6   // ancestor levels kept in LDS per thread by the subdivision pass
+#define VKH_FLAT_BLOCK 64 // threads per workgroup of the subdivision pass: one wavefront, 12 KB of stacks
 
 __global__ __launch_bounds__(VKH_FLAT_BLOCK) void k_flatten_stage(const vkh_item *__restrict__ items, uint32_t n, const uint32_t *__restrict__ perm,
                                 uint32_t *__restrict__ counts, uint32_t *__restrict__ slot_alloc,
