@@ -16,7 +16,7 @@ def L():
     lib = vkvg_amd.lib()
     f = lib.vkvg_hip_dash_walk
     f.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_uint32, ctypes.c_float, ctypes.POINTER(ctypes.c_int),
-                  ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_float)]
+                  ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int)]
     f.restype = ctypes.c_uint32
     return f
 
@@ -86,9 +86,26 @@ def test_dash_walk_matches_the_loop(L):
         o = ctypes.c_float(off[i])
         c = ctypes.c_uint32(int(cur[i]))
         b = ctypes.c_int(int(on[i]))
-        caps = L(d, len(p), float(seg[i]), ctypes.byref(b), ctypes.byref(c), ctypes.byref(o))
+        over = ctypes.c_int(7)
+        caps = L(d, len(p), float(seg[i]), ctypes.byref(b), ctypes.byref(c), ctypes.byref(o), ctypes.byref(over))
+        assert over.value == 0
         got = np.array([o.value], np.float32).view(np.uint32)[0]
         want = w_off[i:i + 1].view(np.uint32)[0]
         assert (caps, c.value, bool(b.value), got) == (w_caps[i], w_cur[i], bool(w_on[i]), want), (i, p, off[i], seg[i])
         jumped += caps > 4 * len(p)
     assert jumped > 500  # most segments span several periods, so the whole-period step is exercised
+
+
+def test_dash_walk_stops_on_a_segment_its_dashes_cannot_cross(L):
+    """Past 2^24 dash lengths the offset's float additions round the dashes away and the reference's
+    loop never ends: the walk stops at its cap bound and flags the segment (the frame is refused)."""
+    for p, seg in (([8.0, 4.0], 1e9), ([8.0, 4.0], float("inf")), ([0.5], 3e7), ([3.0, 0.0, 1.0], 1e12)):
+        d = (ctypes.c_float * len(p))(*p)
+        o, c, b, over = ctypes.c_float(0.0), ctypes.c_uint32(0), ctypes.c_int(1), ctypes.c_int(0)
+        caps = L(d, len(p), seg, ctypes.byref(b), ctypes.byref(c), ctypes.byref(o), ctypes.byref(over))
+        assert over.value == 1 and caps >= 1 << 24 and c.value < len(p), (p, seg, caps)
+    # a long segment the dashes still cross ends normally
+    d = (ctypes.c_float * 2)(8.0, 4.0)
+    o, c, b, over = ctypes.c_float(0.0), ctypes.c_uint32(0), ctypes.c_int(1), ctypes.c_int(0)
+    caps = L(d, 2, 6.0e6, ctypes.byref(b), ctypes.byref(c), ctypes.byref(o), ctypes.byref(over))
+    assert over.value == 0 and caps == 1000000

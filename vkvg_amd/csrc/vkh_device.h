@@ -516,12 +516,22 @@ VKH_HD uint32_t dash_whole_periods(const DashPeriod &p, float off, float segLen)
     return k;
 }
 
-// The loop over one segment: advances (on, cur, off) and returns the number of caps it makes.
+// The loop over one segment: advances (on, cur, off) and returns the number of caps it makes. On a
+// segment some 2^24 dash lengths long or longer the offset's float additions stop advancing it
+// (each dash rounds away), and the reference's loop never ends; here the walk stops once it has
+// made VKH_DASH_MAX_CAPS caps and sets `over`, and the frame is refused.
+#define VKH_DASH_MAX_CAPS (1u << 24)
 VKH_HD uint32_t dash_walk_segment(const float *dashes, uint32_t n, const DashPeriod &p, float segLen, bool &on,
-                                  uint32_t &cur, float &off) {
+                                  uint32_t &cur, float &off, bool &over) {
     uint32_t caps = 0;
     while (off < segLen) {
-        const uint32_t w = dash_whole_periods(p, off, segLen);
+        if (caps >= VKH_DASH_MAX_CAPS) {
+            over = true;
+            break;
+        }
+        uint32_t w = dash_whole_periods(p, off, segLen);
+        if (w > VKH_DASH_MAX_CAPS / n) // fewer periods are as exact; keeps caps within 32 bits
+            w = VKH_DASH_MAX_CAPS / n;
         if (w) {
             off = (float)((double)off + (double)w * p.sum);
             caps += w * n;

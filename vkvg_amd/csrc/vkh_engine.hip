@@ -162,8 +162,8 @@ struct vkh_engine {
     hipGraphExec_t gexec = nullptr;
     uint64_t gkey = 0, gfail_key = 0;
     uint64_t frame_gen = 0, alloc_gen = 0;
-    uint32_t sz[9] = {};  // points, overflow cubics, fill edges, stroke steps, stroke vertices, triangles, path tiles,
-                          // segments, commands
+    uint32_t sz[10] = {}; // points, overflow cubics, fill edges, stroke steps, stroke vertices, triangles, dash
+                          // walk stopped, path tiles, segments, commands
     uint32_t fine_words = 4096;
     std::vector<TargetInfo> h_tg;
     // draws without path tiles (target, index within the target's draws), and per (target, coarse
@@ -696,8 +696,10 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     const uint32_t n_wh = 64u * vkh::flat_order_groups(n_items);
     if (!need(eng, cap, eng->flat_wh, 4ull * (2ull * n_wh + 2)))
         return -1;
-    uint32_t *ctr = P<uint32_t>(eng->counters); // [0] segments, [1] flatten slots, [2] overflow, [3] dashed pieces
-    vkh::launch_zero_u32(s, ctr + 1, 3);
+    // [0] segments, [1] flatten slots, [2] overflow cubics, [3] long dashed pieces, [4] a dash walk
+    // stopped at its cap bound
+    uint32_t *ctr = P<uint32_t>(eng->counters);
+    vkh::launch_zero_u32(s, ctr + 1, 4);
     if (n_items) {
         uint32_t *whist = P<uint32_t>(eng->flat_wh), *wcur = whist + n_wh + 1;
         vkh::launch_flat_hist(s, d_items, n_items, whist);
@@ -758,8 +760,12 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     if (scan_u32(eng, P<uint32_t>(eng->step_nv), n_steps, P<uint32_t>(eng->sv_off), cap) ||
         scan_u32(eng, P<uint32_t>(eng->step_nt), n_steps, P<uint32_t>(eng->st_off), cap))
         return -1;
-    if (totals(eng, cap, {P<uint32_t>(eng->sv_off) + n_steps, P<uint32_t>(eng->st_off) + n_steps}, tot, 4))
+    if (totals(eng, cap, {P<uint32_t>(eng->sv_off) + n_steps, P<uint32_t>(eng->st_off) + n_steps, ctr + 4}, tot, 4))
         return -1;
+    if (tot[2]) {
+        eng->err = "dashed segment too long for its dash pattern (the dash loop would not end)";
+        return -3;
+    }
     const uint32_t n_verts = tot[0], n_tris = tot[1];
     const uint64_t n_edges64 = (uint64_t)n_fill_edges + 3ull * n_tris;
     if (n_edges64 > 0x7FFFFFFFull) {
@@ -815,7 +821,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
                           d_gprep);
     if (scan_u32(eng, P<uint32_t>(eng->pt_count), n_draws, P<uint32_t>(eng->pt_off), cap))
         return -1;
-    if (totals(eng, cap, {P<uint32_t>(eng->pt_off) + n_draws}, tot, 6))
+    if (totals(eng, cap, {P<uint32_t>(eng->pt_off) + n_draws}, tot, 7))
         return -1;
     const uint32_t n_ptiles = tot[0];
     st.path_tiles = n_ptiles;
@@ -850,7 +856,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
                               seg_cap, P<uint32_t>(eng->counters), P<uint32_t>(eng->draw_order), d_draws, d_tg, nch,
                               eng->coarse_chunk, P<uint32_t>(eng->tile_cnt));
         CK(hipGetLastError());
-        if (totals(eng, cap, {P<uint32_t>(eng->counters)}, tot, 7))
+        if (totals(eng, cap, {P<uint32_t>(eng->counters)}, tot, 8))
             return -1;
         n_segs = tot[0];
         if (n_segs <= seg_cap)
@@ -869,7 +875,7 @@ static int run_pipeline(vkh_engine *eng, bool cap) {
     // ---- coarse: ordered command list per tile (the counts came from binning)
     if (scan_u32(eng, P<uint32_t>(eng->tile_cnt), n_tc, P<uint32_t>(eng->tile_off), cap))
         return -1;
-    if (totals(eng, cap, {P<uint32_t>(eng->tile_off) + n_tc}, tot, 8))
+    if (totals(eng, cap, {P<uint32_t>(eng->tile_off) + n_tc}, tot, 9))
         return -1;
     const uint32_t n_cmds = tot[0];
     st.commands = n_cmds;

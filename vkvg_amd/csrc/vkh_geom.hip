@@ -240,13 +240,15 @@ extern "C" uint32_t vkvg_hip_flatten_points(const vkh_item *it, int stack, float
 // The dash walk of one segment as the dash-walk kernel runs it (whole periods taken at once where the
 // float additions are exact), computed on the host: state (on, cur, off) in and out; returns the caps.
 extern "C" uint32_t vkvg_hip_dash_walk(const float *dashes, uint32_t n, float seg_len, int *on, uint32_t *cur,
-                                       float *off) {
+                                       float *off, int *over) {
+    *over = 0;
     if (!n)
         return 0;
     const DashPeriod p = dash_period(dashes, n);
-    bool b = *on != 0;
-    const uint32_t caps = dash_walk_segment(dashes, n, p, seg_len, b, *cur, *off);
+    bool b = *on != 0, o = false;
+    const uint32_t caps = dash_walk_segment(dashes, n, p, seg_len, b, *cur, *off, o);
     *on = b ? 1 : 0;
+    *over = o ? 1 : 0;
     return caps;
 }
 
@@ -632,13 +634,13 @@ __device__ void dash_piece(const StrokeCtx &c, bool closed, uint32_t k, DashStat
 // (the join and caps it emits do not change the state), with the same float operations.
 // Returns the number of caps the piece emits (the loop's iterations).
 __device__ __forceinline__ uint32_t dash_walk_piece(const StrokeCtx &c, const DashPeriod &per, uint32_t k,
-                                                   DashState &dc) {
+                                                   DashState &dc, bool &over) {
     const uint32_t last = c.n - 1;
     const uint32_t cp = k < last ? k : last, iR = k < last ? k + 1 : 0;
     v2 d = vsub(c.P[iR], c.P[cp]);
     float segLen = vlen(d);
     const uint32_t caps =
-        dash_walk_segment(c.dashes, c.dashCount, per, segLen, dc.dashOn, dc.curDash, dc.curDashOffset);
+        dash_walk_segment(c.dashes, c.dashCount, per, segLen, dc.dashOn, dc.curDash, dc.curDashOffset, over);
     dc.curDashOffset -= segLen;
     dc.curDashOffset = fmodf(dc.curDashOffset, dc.totDashLength);
     return caps;
@@ -660,8 +662,9 @@ __device__ void dash_piece_count(const StrokeCtx &c, const DashPeriod &per, bool
     const v2 n = vnorm(d);
     const float segLen = vlen(d);
     const bool on0 = dc.dashOn;
+    bool over = false; // the dash walk of the same piece has flagged the frame
     const uint32_t caps =
-        dash_walk_segment(c.dashes, c.dashCount, per, segLen, dc.dashOn, dc.curDash, dc.curDashOffset);
+        dash_walk_segment(c.dashes, c.dashCount, per, segLen, dc.dashOn, dc.curDash, dc.curDashOffset, over);
     CountSink cs, ce;
     stroke_cap(c, cs, V(0.f, 0.f), n, true);
     stroke_cap(c, ce, V(0.f, 0.f), n, false);
@@ -807,14 +810,19 @@ __global__ void k_dash_walk(const Unit *__restrict__ units, uint32_t n_units, co
     const DashPeriod per = dash_period(c.dashes, c.dashCount);
     // pieces of many caps join the list that k_dash_pieces takes a wavefront per entry from
     DashState dc = dash_start(c, st.dash_offset);
+    // a segment too long for its dash pattern stops the walk and refuses the frame (dcount[1]); the
+    // later pieces keep the last state (a valid dash index), so their counting ends at once
+    bool over = false;
     for (uint32_t k = 0; k < s1 - s0; k++) {
         const DashState d0 = dc;
-        const uint32_t caps = dash_walk_piece(c, per, k, dc);
-        const bool wave = caps >= VKH_DASH_WAVE_MIN;
+        const uint32_t caps = over ? 0u : dash_walk_piece(c, per, k, dc, over);
+        const bool wave = !over && caps >= VKH_DASH_WAVE_MIN;
         dash_st[s0 + k] = dash_pack(d0, wave);
         if (wave)
             dlist[atomicAdd(dcount, 1u)] = make_uint4(s0 + k, caps, u, 0u);
     }
+    if (over)
+        atomicOr(dcount + 1, 1u);
 }
 
 // The dash updates of dashed_segment's loop without the cap: the state at the next boundary
