@@ -94,6 +94,12 @@ typedef struct {
     int32_t  source;         /* surface source index, -1 for none (then grad / color apply) */
 } vkh_draw;
 
+/* The smallest arc step a stroke style may carry: round joins and caps step their angle by it in
+ * float, so a step below this (a radius beyond some 160 000 px, or a negative width) would take
+ * millions of steps per arc, or never end when the steps round away. The recorder raises smaller
+ * steps to it; the backend refuses a style below it. */
+#define VKH_MIN_ARC_STEP 4.79368996e-5f /* pi / 2^16 */
+
 typedef struct {
     float    hw;          /* line width / 2 */
     float    lhMax;       /* miter limit * line width */

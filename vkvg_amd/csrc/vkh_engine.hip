@@ -535,6 +535,10 @@ int vkvg_hip_upload(vkh_engine *eng, const vkh_frame *f) {
             eng->err = "dash range out of bounds";
             return -2;
         }
+        if (!(f->styles[i].arcStep >= VKH_MIN_ARC_STEP)) {
+            eng->err = "stroke style arc step below VKH_MIN_ARC_STEP";
+            return -2;
+        }
     }
     std::vector<TargetInfo> tg(f->n_targets);
     uint32_t tile_total = 0, bin_total = 0;

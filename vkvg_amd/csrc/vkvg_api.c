@@ -1644,7 +1644,9 @@ static void stroke_preserve_i(VkvgContext c) {
     memset(&st, 0, sizeof(st));
     st.hw = c->lineWidth * 0.5f;
     st.lhMax = c->miterLimit * c->lineWidth;
-    st.arcStep = arc_step(c, st.hw);
+    /* raised to VKH_MIN_ARC_STEP where the reference's arc loops would not end (huge or
+     * negative widths); every width that renders in finite time keeps the reference's step */
+    st.arcStep = fmaxf(arc_step(c, st.hw), VKH_MIN_ARC_STEP);
     st.join = (uint32_t)c->lineJoin;
     st.cap = (uint32_t)c->lineCap;
     st.dash_first = c->f.dashes.n;
